@@ -1,0 +1,158 @@
+/*
+ * spf.h -- C ABI of the MI355X (gfx950) SPFresh batch-search library
+ * (libspf_hip.so).  Plain pointers and sizes only.
+ *
+ * The library replaces the search side of the reference crate:
+ *   SPFresh::search_batch     src/spfresh.rs:897-908   -> spf_search_batch
+ *   SPFresh::search_with_dists src/spfresh.rs:886-888  -> spf_search_batch (nq = 1)
+ *   SPFresh::search           src/spfresh.rs:890-895   -> spf_search_batch (ids only)
+ *   SPFresh::delete           src/spfresh.rs:867-874   -> spf_delete (device tombstones)
+ * and is fed the state those calls read under the RwLock read guard
+ * (Inner, src/spfresh.rs:165-195): live postings and centroids, (id, code)
+ * entries, the raw vector store, tombstones, quantizer state and
+ * SPFreshConfig::{probe_ratio, rerank_size}.
+ *
+ * Ownership: the library owns all device memory behind the opaque handle; the
+ * caller owns every host buffer.  No pointer escapes a call.  Searches on one
+ * handle may run concurrently from several host threads (each call takes its
+ * own workspace and stream); spf_delete and spf_index_destroy must not overlap
+ * a search on the same handle (the reference takes the write lock there).
+ */
+#ifndef SPF_H
+#define SPF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define SPF_OK 0
+#define SPF_EINVAL (-1)        /* bad argument (message in spf_last_error) */
+#define SPF_EDIM (-2)          /* query dim != index dim: the reference panics with
+                                  "Query dim {} != index dim {}" (src/spfresh.rs:672-678) */
+#define SPF_EUNSUPPORTED (-3)  /* no device path (PQ postings): caller keeps its CPU search */
+#define SPF_EDEVICE (-4)       /* HIP runtime error */
+#define SPF_ENOMEM (-5)        /* device allocation failed */
+
+/* metrics: the Distance impls of src/metric.rs:16-39 */
+#define SPF_METRIC_L2 0        /* DistL2    sqrt(sum (a-b)^2) */
+#define SPF_METRIC_L2SQ 1      /* DistL2Sq  sum (a-b)^2   (Python "l2") */
+#define SPF_METRIC_COSINE 2    /* DistCosine (angular)     (Python "cosine") */
+#define SPF_METRIC_DOT 3       /* DistDot   1 - a.b        (Python "dot") */
+
+/* posting codecs: QuantizerState (src/quantized.rs:150-155); PQ is not served */
+#define SPF_QUANT_NONE 0       /* f32 codes = raw vector */
+#define SPF_QUANT_F16 1        /* src/sq.rs:36-108 */
+#define SPF_QUANT_INT8 2       /* src/sq.rs:121-234, global scale/offset, u8 codes */
+#define SPF_QUANT_RABITQ 3     /* src/rabitq.rs, residual codes per posting */
+#define SPF_QUANT_PQ 4         /* -> SPF_EUNSUPPORTED */
+
+/* search precision */
+#define SPF_PREC_EXACT 0       /* reference semantics, canonical f32 order: ids match
+                                  the restated reference (ties by id) */
+#define SPF_PREC_MIXED 1       /* reduced-precision probe and scan on matrix cores,
+                                  exact f32 rerank of a candidate pool */
+
+typedef struct spf_index spf_index;
+
+typedef struct spf_index_desc {
+    uint32_t dim;
+    uint32_t metric;                /* SPF_METRIC_* */
+    uint32_t quantizer;             /* SPF_QUANT_* */
+    uint32_t n_postings;            /* live postings (Inner::cid_block.len()) */
+    const uint32_t *block_ids;      /* [n_postings] reference block index per posting;
+                                       distinct; orders equal centroid distances */
+    const float *centroids;         /* [n_postings * dim]  block_centroid of each */
+    const uint64_t *posting_offsets;/* [n_postings + 1] CSR offsets into the entries */
+    const uint64_t *entry_ids;      /* [n_entries] ids, each id at most once */
+    const uint8_t *codes;           /* [n_entries * code_bytes] posting codes */
+    uint32_t code_bytes;            /* code_size (src/spfresh.rs:99-107) */
+    const float *raw;               /* [n_raw * dim] raw store by id (Inner::raw);
+                                       required for rerank / RaBitQ, else may be NULL */
+    uint64_t n_raw;                 /* next_id */
+    const uint64_t *deleted_ids;    /* tombstones (may be NULL) */
+    uint64_t n_deleted;
+    float int8_scale, int8_offset;  /* Int8Quantizer {scale, offset} */
+    uint32_t rabitq_padded;         /* RaBitQ::padded */
+    const float *rabitq_signs;      /* [3 * rabitq_padded] RaBitQ::signs */
+    float probe_ratio;              /* SPFreshConfig::probe_ratio (inf = off) */
+    uint32_t rerank_size;           /* SPFreshConfig::rerank_size */
+    int32_t device;                 /* HIP device ordinal */
+} spf_index_desc;
+
+#define SPF_FLAG_PROFILE 1u     /* time each stage with HIP events on the call's stream;
+                                   the call then waits for its work (spf_last_profile) */
+
+typedef struct spf_search_opts {
+    uint32_t precision;             /* SPF_PREC_* (default EXACT) */
+    uint32_t rerank_size;           /* pool for MIXED (0 = index rerank_size, min k) */
+    uint32_t flags;                 /* SPF_FLAG_* */
+    uint32_t reserved[5];
+} spf_search_opts;
+
+/* stages of one batch search, in order */
+#define SPF_STAGE_PREP 0
+#define SPF_STAGE_PROBE 1
+#define SPF_STAGE_PROBE_SELECT 2
+#define SPF_STAGE_PLAN 3
+#define SPF_STAGE_SCAN 4
+#define SPF_STAGE_SELECT 5
+#define SPF_STAGE_RERANK 6
+#define SPF_STAGE_FINALIZE 7
+#define SPF_N_STAGES 8
+
+typedef struct spf_profile {
+    float stage_ms[SPF_N_STAGES];   /* HIP-event time of each stage */
+    uint64_t candidates;            /* distances computed by the posting scan */
+    uint64_t scan_code_bytes;       /* candidates * code bytes: the scan's algorithmic reads */
+    uint64_t probe_pairs;           /* (query, centroid) distances of the probe */
+    uint32_t scan_items;            /* posting-scan work items */
+    uint32_t reserved[3];
+} spf_profile;
+
+typedef struct spf_index_info {
+    uint32_t dim, metric, quantizer, n_postings;
+    uint64_t n_entries, n_raw, n_deleted;
+    uint64_t device_bytes;          /* HBM held by the index */
+    uint32_t max_posting;           /* longest posting (SPFreshStats::max_posting) */
+    uint32_t reserved[3];
+} spf_index_info;
+
+/* Build the device mirror of a search-ready index (all host pointers are
+ * copied; none is kept). */
+int spf_index_create(const spf_index_desc *desc, spf_index **out);
+void spf_index_destroy(spf_index *ix);
+int spf_index_get_info(const spf_index *ix, spf_index_info *out);
+
+/* Tombstone ids (additive, ids >= n_raw ignored), like SPFresh::delete. */
+int spf_delete(spf_index *ix, const uint64_t *ids, uint64_t n);
+
+/* Batch search from host buffers.  queries: nq x dim f32 row-major.
+ * Outputs are nq rows of k: ids (UINT64_MAX pad), dists (+inf pad), counts[nq].
+ * n_probe 0 is clamped to 1 (src/spfresh.rs:679).  k must be >= 1. */
+int spf_search_batch(spf_index *ix, const float *queries, uint32_t nq, uint32_t dim,
+                     uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
+                     uint64_t *out_ids, float *out_dists, uint32_t *out_counts);
+
+/* Same, with queries and outputs already in device memory, enqueued on
+ * `stream` (a hipStream_t; NULL = the library's stream for this call).  The
+ * call returns after the work is enqueued and the host-side planning has
+ * synchronised once; outputs are final when `stream` completes. */
+int spf_search_batch_device(spf_index *ix, const float *d_queries, uint32_t nq, uint32_t dim,
+                            uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
+                            uint64_t *d_out_ids, float *d_out_dists, uint32_t *d_out_counts,
+                            void *stream);
+
+/* Profile of the last search on this thread that set SPF_FLAG_PROFILE. */
+int spf_last_profile(spf_profile *out);
+
+/* Thread-local message for the last error on this thread. */
+const char *spf_last_error(void);
+/* Library build string (arch, flags). */
+const char *spf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

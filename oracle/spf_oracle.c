@@ -1,0 +1,554 @@
+/*
+ * CPU oracle for the SPFresh batch-search path (test infrastructure; see
+ * spf_oracle.h).  Compiled with -ffp-contract=off so that every operation
+ * rounds exactly where the Rust source rounds (Rust never contracts a*b+c).
+ */
+#include "spf_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+static int g_fault = 0;
+void spf_o_set_fault(int fault) { g_fault = fault; }
+
+/* ------------------------------------------------------------------------ */
+/* total order on f32 bit patterns: identical to Rust's f32::total_cmp, which
+ * the reference uses for every final sort (src/spfresh.rs:367,370,724,768). */
+static inline uint32_t fkey(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+/* ---- f16 <-> f32, IEEE round-to-nearest-even.  The reference calls
+ * numkong::cast for this (src/sq.rs:51-55, 96-99). ---- */
+uint16_t spf_o_f32_to_f16(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t exp = (x >> 23) & 0xffu;
+    uint32_t mant = x & 0x7fffffu;
+    if (exp == 0xffu) {
+        if (mant) return (uint16_t)(sign | 0x7e00u | (mant >> 13));
+        return (uint16_t)(sign | 0x7c00u);
+    }
+    int e = (int)exp - 127 + 15;
+    if (e >= 0x1f) return (uint16_t)(sign | 0x7c00u);
+    if (e <= 0) {
+        if (e < -10) return (uint16_t)sign;
+        uint32_t m = mant | 0x800000u;
+        uint32_t shift = (uint32_t)(14 - e);
+        uint32_t h = m >> shift;
+        uint32_t rem = m & ((1u << shift) - 1u);
+        uint32_t half = 1u << (shift - 1);
+        if (g_fault == 4) return (uint16_t)(sign | h);
+        if (rem > half || (rem == half && (h & 1u))) h++;
+        return (uint16_t)(sign | h);
+    }
+    uint32_t h = ((uint32_t)e << 10) | (mant >> 13);
+    uint32_t rem = mant & 0x1fffu;
+    if (g_fault != 4 && (rem > 0x1000u || (rem == 0x1000u && (h & 1u)))) h++;
+    return (uint16_t)(sign | h);
+}
+
+float spf_o_f16_to_f32(uint16_t h) {
+    uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+    uint32_t exp = ((uint32_t)h >> 10) & 0x1fu;
+    uint32_t mant = (uint32_t)h & 0x3ffu;
+    uint32_t x;
+    float f;
+    if (exp == 0) {
+        f = (float)mant * 5.9604644775390625e-8f; /* mant * 2^-24, exact */
+        return sign ? -f : f;
+    }
+    if (exp == 0x1fu) x = sign | 0x7f800000u | (mant << 13);
+    else x = sign | ((exp + 112u) << 23) | (mant << 13);
+    memcpy(&f, &x, 4);
+    return f;
+}
+
+/* ---- Distances (src/metric.rs:16-39).  numkong's reduction order is not
+ * public; the canonical order is a sequential FMA chain from +0. ---- */
+float spf_o_l2sq(const float *a, const float *b, uint32_t d) {
+    float acc = 0.0f;
+    uint32_t n = (g_fault == 1 && d > 0) ? d - 1 : d;
+    for (uint32_t i = 0; i < n; i++) {
+        float t = a[i] - b[i];
+        acc = fmaf(t, t, acc);
+    }
+    return acc;
+}
+
+float spf_o_dot(const float *a, const float *b, uint32_t d) {
+    float acc = 0.0f;
+    for (uint32_t i = 0; i < d; i++) acc = fmaf(a[i], b[i], acc);
+    return acc;
+}
+
+/* Angular distance: numkong's special cases for zero norms are restated as
+ * 0 when both norms are zero and 1 when the dot product is zero. */
+static float cosine_from(float ab, float a2, float b2) {
+    if (a2 == 0.0f && b2 == 0.0f) return 0.0f;
+    if (ab == 0.0f) return 1.0f;
+    float r = 1.0f - ab / sqrtf(a2 * b2);
+    return r > 0.0f ? r : 0.0f;
+}
+
+float spf_o_eval(uint32_t metric, const float *a, const float *b, uint32_t d) {
+    switch (metric) {
+    case SPF_O_L2: return sqrtf(spf_o_l2sq(a, b, d));
+    case SPF_O_L2SQ: return spf_o_l2sq(a, b, d);
+    case SPF_O_DOT: return 1.0f - spf_o_dot(a, b, d);
+    case SPF_O_COSINE:
+        return cosine_from(spf_o_dot(a, b, d), spf_o_dot(a, a, d), spf_o_dot(b, b, d));
+    }
+    return NAN;
+}
+
+/* ---- Int8 global-affine codec (src/sq.rs:128-172, 208-226). ---- */
+static inline uint8_t sat_round_u8(float s) {
+    if (!(s > 0.0f)) return 0;          /* negatives and NaN saturate to 0 */
+    if (s >= 255.0f) return 255;
+    return (uint8_t)rintf(s);           /* nearest, ties to even */
+}
+
+void spf_o_int8_encode(const float *v, uint32_t d, float scale, float offset, uint8_t *out) {
+    float alpha = 1.0f / scale;
+    float beta = -offset / scale;
+    for (uint32_t i = 0; i < d; i++) {
+        float s = v[i] * alpha + beta;
+        out[i] = sat_round_u8(s);
+    }
+}
+
+float spf_o_int8_l2sq(const uint8_t *a, const uint8_t *b, uint32_t d, float scale) {
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < d; i++) {
+        int t = (int)a[i] - (int)b[i];
+        acc += (uint64_t)(t * t);
+    }
+    return (float)acc * scale * scale;
+}
+
+static void int8_decode(const uint8_t *c, uint32_t d, float scale, float offset, float *out) {
+    for (uint32_t i = 0; i < d; i++) out[i] = (float)c[i] * scale + offset;
+}
+
+/* ---- RaBitQ (src/rabitq.rs).  Pure scalar f32 arithmetic, restated in the
+ * same operation order. ---- */
+void spf_o_fht(float *v, uint32_t n) {
+    for (uint32_t h = 1; h < n; h *= 2)
+        for (uint32_t i = 0; i < n; i += 2 * h)
+            for (uint32_t j = i; j < i + h; j++) {
+                float x = v[j], y = v[j + h];
+                v[j] = x + y;
+                v[j + h] = x - y;
+            }
+}
+
+void spf_o_rotate(float *v, uint32_t padded, const float *signs) {
+    for (uint32_t r = 0; r < 3; r++) {
+        for (uint32_t i = 0; i < padded; i++) v[i] *= signs[r * padded + i];
+        spf_o_fht(v, padded);
+    }
+}
+
+void spf_o_unrotate(float *v, uint32_t padded, const float *signs) {
+    for (int r = 2; r >= 0; r--) {
+        spf_o_fht(v, padded);
+        for (uint32_t i = 0; i < padded; i++) v[i] *= signs[(uint32_t)r * padded + i];
+    }
+}
+
+/* prep_with, src/rabitq.rs:136-149 */
+static float rq_prep(const float *v, const float *c, uint32_t d, uint32_t padded,
+                     const float *signs, float *r) {
+    for (uint32_t i = 0; i < padded; i++) r[i] = i < d ? v[i] - c[i] : 0.0f;
+    float acc = 0.0f;
+    for (uint32_t i = 0; i < padded; i++) {
+        float sq = r[i] * r[i];
+        acc = acc + sq;
+    }
+    float norm = sqrtf(acc);
+    if (norm > 0.0f)
+        for (uint32_t i = 0; i < padded; i++) r[i] /= norm;
+    spf_o_rotate(r, padded, signs);
+    return norm;
+}
+
+/* encode_with + pack, src/rabitq.rs:218-256 */
+void spf_o_rabitq_encode(const float *v, const float *centroid, uint32_t d, uint32_t padded,
+                         const float *signs, uint8_t *code) {
+    float *r = (float *)malloc(sizeof(float) * padded);
+    float norm = rq_prep(v, centroid, d, padded, signs, r);
+    float ip = 0.0f;
+    for (uint32_t w = 0; w < padded / 64; w++) {
+        uint64_t bits = 0;
+        for (uint32_t i = 0; i < 64; i++) {
+            float x = r[w * 64 + i];
+            bits |= (uint64_t)(x > 0.0f) << i;
+            ip += fabsf(x);
+        }
+        memcpy(code + 8 * w, &bits, 8);
+    }
+    ip /= sqrtf((float)padded);
+    float fac = ip > 0.0f ? norm / ip : 0.0f;
+    float n2 = norm * norm;
+    memcpy(code + padded / 8, &n2, 4);
+    memcpy(code + padded / 8 + 4, &fac, 4);
+    free(r);
+}
+
+/* decode_with, src/rabitq.rs:224-238 */
+void spf_o_rabitq_decode(const uint8_t *code, const float *centroid, uint32_t d, uint32_t padded,
+                         const float *signs, float *out) {
+    float *r = (float *)malloc(sizeof(float) * padded);
+    float sd = sqrtf((float)padded);
+    for (uint32_t w = 0; w < padded / 64; w++) {
+        uint64_t bits;
+        memcpy(&bits, code + 8 * w, 8);
+        for (uint32_t i = 0; i < 64; i++)
+            r[w * 64 + i] = ((bits >> i) & 1u) ? 1.0f / sd : -1.0f / sd;
+    }
+    spf_o_unrotate(r, padded, signs);
+    float n2;
+    memcpy(&n2, code + padded / 8, 4);
+    float norm = sqrtf(n2);
+    for (uint32_t i = 0; i < d; i++) {
+        float t = norm * r[i];
+        out[i] = centroid[i] + t;
+    }
+    free(r);
+}
+
+/* query_with, src/rabitq.rs:157-187.  consts = {nq2, s, k1, k2, k3}. */
+void spf_o_rabitq_query(const float *q, const float *centroid, uint32_t d, uint32_t padded,
+                        const float *signs, uint64_t *planes, float *consts) {
+    float *r = (float *)malloc(sizeof(float) * padded);
+    float nq = rq_prep(q, centroid, d, padded, signs, r);
+    float lo = FLT_MAX, hi = -FLT_MAX;
+    for (uint32_t i = 0; i < padded; i++) {
+        lo = fminf(lo, r[i]);
+        hi = fmaxf(hi, r[i]);
+    }
+    const float levels = 15.0f;
+    float delta = (hi - lo) / levels;
+    float inv = delta > 0.0f ? 1.0f / delta : 0.0f;
+    uint32_t words = padded / 64;
+    memset(planes, 0, sizeof(uint64_t) * 4 * words);
+    uint32_t sum_u = 0;
+    for (uint32_t i = 0; i < padded; i++) {
+        float t = roundf((r[i] - lo) * inv);
+        uint32_t u = (t > 0.0f) ? (t >= 4294967295.0f ? 0xffffffffu : (uint32_t)t) : 0u;
+        if (u > 15u) u = 15u;
+        sum_u += u;
+        for (uint32_t j = 0; j < 4; j++)
+            if ((u >> j) & 1u) planes[j * words + i / 64] |= 1ull << (i % 64);
+    }
+    float sd = sqrtf((float)padded);
+    float a = delta * (float)sum_u;
+    float b = (float)padded * lo;
+    consts[0] = nq * nq;
+    consts[1] = 2.0f * nq;
+    consts[2] = 2.0f * delta / sd;
+    consts[3] = 2.0f * lo / sd;
+    consts[4] = -(a + b) / sd;
+    free(r);
+}
+
+/* distance_with_bound, src/rabitq.rs:191-215 */
+void spf_o_rabitq_dist(const uint64_t *planes, const float *consts, const uint8_t *code,
+                       uint32_t padded, float *est, float *err) {
+    uint32_t words = padded / 64;
+    uint32_t s = 0, c = 0;
+    for (uint32_t i = 0; i < words; i++) {
+        uint64_t b;
+        memcpy(&b, code + 8 * i, 8);
+        c += (uint32_t)__builtin_popcountll(b);
+        s += (uint32_t)__builtin_popcountll(b & planes[i]) +
+             ((uint32_t)__builtin_popcountll(b & planes[words + i]) << 1) +
+             ((uint32_t)__builtin_popcountll(b & planes[2 * words + i]) << 2) +
+             ((uint32_t)__builtin_popcountll(b & planes[3 * words + i]) << 3);
+    }
+    float n2, fac;
+    memcpy(&n2, code + padded / 8, 4);
+    memcpy(&fac, code + padded / 8 + 4, 4);
+    float nq2 = consts[0], sq = consts[1], k1 = consts[2], k2 = consts[3], k3 = consts[4];
+    float inner = k1 * (float)s + k2 * (float)c;
+    inner = inner + k3;
+    float sf = sq * fac;
+    float e = (n2 + nq2) - sf * inner;
+    *est = fmaxf(e, 0.0f);
+    float ip = fac > 0.0f ? sqrtf(n2) / fac : 0.0f;
+    if (ip > 0.0f) {
+        float one = fmaxf(1.0f - ip * ip, 0.0f);
+        float rad = sqrtf(one / ((float)padded - 1.0f));
+        *err = sf * 1.9f * rad;
+    } else {
+        *err = INFINITY;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+typedef struct { uint32_t key; uint32_t aux; uint64_t id; float dist; } cand_t;
+
+static int cand_cmp(const void *a, const void *b) {
+    const cand_t *x = (const cand_t *)a, *y = (const cand_t *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    if (x->id != y->id) return x->id < y->id ? -1 : 1;
+    return 0;
+}
+
+static inline int is_deleted(const spf_o_index *ix, uint64_t id) {
+    if (g_fault == 2 || !ix->deleted || id >= ix->n_raw) return 0;
+    return (ix->deleted[id >> 3] >> (id & 7)) & 1;
+}
+
+/* Inner::nearest, exact branch (src/spfresh.rs:358-372), then the probe_ratio
+ * take_while of Inner::search (src/spfresh.rs:683-687).  Ties in centroid
+ * distance are ordered by reference block index. */
+uint32_t spf_o_probe(const spf_o_index *ix, const float *q, uint32_t n_probe,
+                     uint32_t *out_post, float *out_dist) {
+    uint32_t n = n_probe < 1 ? 1 : n_probe;
+    if (ix->n_post == 0) return 0;
+    cand_t *all = (cand_t *)malloc(sizeof(cand_t) * ix->n_post);
+    for (uint32_t p = 0; p < ix->n_post; p++) {
+        float dd = spf_o_eval(ix->metric, q, ix->centroids + (size_t)p * ix->dim, ix->dim);
+        all[p].key = fkey(dd);
+        all[p].id = ix->block_ids[p];
+        all[p].aux = p;
+        all[p].dist = dd;
+    }
+    qsort(all, ix->n_post, sizeof(cand_t), cand_cmp);
+    if (n > ix->n_post) n = ix->n_post;
+    float d0 = all[0].dist, ratio = ix->probe_ratio;
+    uint32_t cnt = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        float dd = all[i].dist;
+        if (g_fault != 3 && isfinite(ratio) && d0 >= 0.0f && dd > d0 * fmaxf(ratio, 1.0f)) break;
+        out_post[cnt] = all[i].aux;
+        out_dist[cnt] = dd;
+        cnt++;
+    }
+    free(all);
+    return cnt;
+}
+
+/* Inner::code_dist (src/spfresh.rs:344-354) with quantized_distance_from_codes
+ * (src/quantized.rs:13-54) for the non-RaBitQ-L2 cases. */
+typedef struct {
+    uint16_t *q16;   /* F16 prepared query */
+    uint8_t *q8;     /* Int8 prepared query */
+    float *tmp;      /* decode buffer (>= padded floats) */
+} prep_t;
+
+static float code_dist(const spf_o_index *ix, const float *q, const uint8_t *code, uint32_t post,
+                       prep_t *pp) {
+    uint32_t d = ix->dim;
+    int l2 = ix->metric == SPF_O_L2, l2sq = ix->metric == SPF_O_L2SQ;
+    switch (ix->quant) {
+    case SPF_O_NONE:
+        return spf_o_eval(ix->metric, q, (const float *)code, d);
+    case SPF_O_F16: {
+        if (l2 || l2sq) {
+            float acc = 0.0f;
+            uint32_t n = (g_fault == 1 && d > 0) ? d - 1 : d;
+            for (uint32_t i = 0; i < n; i++) {
+                uint16_t h;
+                memcpy(&h, code + 2 * i, 2);
+                float t = spf_o_f16_to_f32(pp->q16[i]) - spf_o_f16_to_f32(h);
+                acc = fmaf(t, t, acc);
+            }
+            return l2 ? sqrtf(acc) : acc;
+        }
+        for (uint32_t i = 0; i < d; i++) {
+            uint16_t h;
+            memcpy(&h, code + 2 * i, 2);
+            pp->tmp[i] = spf_o_f16_to_f32(h);
+        }
+        return spf_o_eval(ix->metric, q, pp->tmp, d);
+    }
+    case SPF_O_INT8:
+        if (l2 || l2sq) {
+            float v = spf_o_int8_l2sq(pp->q8, code, d, ix->int8_scale);
+            return l2 ? sqrtf(v) : v;
+        }
+        int8_decode(code, d, ix->int8_scale, ix->int8_offset, pp->tmp);
+        return spf_o_eval(ix->metric, q, pp->tmp, d);
+    case SPF_O_RABITQ:
+        /* reached only for cosine / dot (L2 metrics take search_rabitq) */
+        spf_o_rabitq_decode(code, ix->centroids + (size_t)post * d, d, ix->rq_padded, ix->rq_signs,
+                            pp->tmp);
+        return spf_o_eval(ix->metric, q, pp->tmp, d);
+    }
+    return NAN;
+}
+
+/* search_rabitq (src/spfresh.rs:731-770).  The size-k max-heap keeps its
+ * entries sorted by (dist, id); an entry replaces the current k-th only when
+ * its exact distance is strictly smaller, as in the reference. */
+static uint32_t search_rabitq(const spf_o_index *ix, const float *q, uint32_t k,
+                              const uint32_t *probes, uint32_t np, uint64_t *out_ids,
+                              float *out_dists) {
+    uint32_t d = ix->dim, P = ix->rq_padded;
+    uint64_t *planes = (uint64_t *)malloc(sizeof(uint64_t) * 4 * (P / 64));
+    float consts[5];
+    cand_t *heap = (cand_t *)malloc(sizeof(cand_t) * (k + 1));
+    uint32_t cnt = 0;
+    for (uint32_t pi = 0; pi < np; pi++) {
+        uint32_t p = probes[pi];
+        spf_o_rabitq_query(q, ix->centroids + (size_t)p * d, d, P, ix->rq_signs, planes, consts);
+        for (uint64_t e = ix->post_off[p]; e < ix->post_off[p + 1]; e++) {
+            uint64_t id = ix->ids[e];
+            if (is_deleted(ix, id)) continue;
+            float est, err;
+            spf_o_rabitq_dist(planes, consts, ix->codes + e * ix->code_bytes, P, &est, &err);
+            float kth = cnt < k ? INFINITY : heap[cnt - 1].dist;
+            if (est - err < kth) {
+                float exact = spf_o_l2sq(q, ix->raw + id * d, d);
+                cand_t c = {fkey(exact), 0, id, exact};
+                if (cnt < k) {
+                    heap[cnt++] = c;
+                } else if (exact < kth) {
+                    heap[cnt - 1] = c;
+                } else {
+                    continue;
+                }
+                /* restore sorted order (insertion) */
+                for (uint32_t i = cnt - 1; i > 0 && cand_cmp(&heap[i], &heap[i - 1]) < 0; i--) {
+                    cand_t t = heap[i];
+                    heap[i] = heap[i - 1];
+                    heap[i - 1] = t;
+                }
+            }
+        }
+    }
+    for (uint32_t i = 0; i < cnt; i++) {
+        float v = ix->metric == SPF_O_L2 ? sqrtf(heap[i].dist) : heap[i].dist;
+        heap[i].dist = v;
+        heap[i].key = fkey(v);
+    }
+    qsort(heap, cnt, sizeof(cand_t), cand_cmp);
+    for (uint32_t i = 0; i < cnt; i++) {
+        out_ids[i] = heap[i].id;
+        out_dists[i] = heap[i].dist;
+    }
+    free(planes);
+    free(heap);
+    return cnt;
+}
+
+/* Inner::search (src/spfresh.rs:671-727).  The top-`want` heap is restated as
+ * "the `want` smallest by (dist, id)". */
+uint32_t spf_o_search(const spf_o_index *ix, const float *q, uint32_t k, uint32_t n_probe,
+                      uint64_t *out_ids, float *out_dists) {
+    uint32_t d = ix->dim;
+    uint32_t n = n_probe < 1 ? 1 : n_probe;
+    if (n > ix->n_post) n = ix->n_post;
+    if (ix->n_post == 0 || k == 0) return 0;
+    uint32_t *probes = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    float *pd = (float *)malloc(sizeof(float) * n);
+    uint32_t np = spf_o_probe(ix, q, n, probes, pd);
+    free(pd);
+    if (ix->quant == SPF_O_RABITQ && (ix->metric == SPF_O_L2 || ix->metric == SPF_O_L2SQ)) {
+        uint32_t r = search_rabitq(ix, q, k, probes, np, out_ids, out_dists);
+        free(probes);
+        return r;
+    }
+    int rerank = ix->quant != SPF_O_NONE && ix->rerank_size > 0;
+    uint32_t want = rerank ? (k > ix->rerank_size ? k : ix->rerank_size) : k;
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < np; i++) total += ix->post_off[probes[i] + 1] - ix->post_off[probes[i]];
+    cand_t *c = (cand_t *)malloc(sizeof(cand_t) * (total ? total : 1));
+    prep_t pp = {0};
+    uint32_t tmpn = ix->rq_padded > d ? ix->rq_padded : d;
+    pp.tmp = (float *)malloc(sizeof(float) * tmpn);
+    if (ix->quant == SPF_O_F16) {
+        pp.q16 = (uint16_t *)malloc(2 * d);
+        for (uint32_t i = 0; i < d; i++) pp.q16[i] = spf_o_f32_to_f16(q[i]);
+    }
+    if (ix->quant == SPF_O_INT8) {
+        pp.q8 = (uint8_t *)malloc(d);
+        spf_o_int8_encode(q, d, ix->int8_scale, ix->int8_offset, pp.q8);
+    }
+    uint64_t m = 0;
+    for (uint32_t i = 0; i < np; i++) {
+        uint32_t p = probes[i];
+        for (uint64_t e = ix->post_off[p]; e < ix->post_off[p + 1]; e++) {
+            uint64_t id = ix->ids[e];
+            if (is_deleted(ix, id)) continue;
+            float dd = code_dist(ix, q, ix->codes + e * ix->code_bytes, p, &pp);
+            c[m].key = fkey(dd);
+            c[m].id = id;
+            c[m].dist = dd;
+            c[m].aux = 0;
+            m++;
+        }
+    }
+    qsort(c, m, sizeof(cand_t), cand_cmp);
+    uint64_t keep = m < want ? m : want;
+    if (rerank) {
+        for (uint64_t i = 0; i < keep; i++) {
+            float dd = spf_o_eval(ix->metric, q, ix->raw + c[i].id * d, d);
+            c[i].dist = dd;
+            c[i].key = fkey(dd);
+        }
+        qsort(c, keep, sizeof(cand_t), cand_cmp);
+    }
+    uint32_t out = (uint32_t)(keep < k ? keep : k);
+    for (uint32_t i = 0; i < out; i++) {
+        out_ids[i] = c[i].id;
+        out_dists[i] = c[i].dist;
+    }
+    free(c);
+    free(pp.tmp);
+    free(pp.q16);
+    free(pp.q8);
+    free(probes);
+    return out;
+}
+
+/* ---- batch: one task per query on a thread pool (src/spfresh.rs:897-908) ---- */
+typedef struct {
+    const spf_o_index *ix;
+    const float *q;
+    uint32_t nq, k, n_probe;
+    uint64_t *ids;
+    float *dists;
+    uint32_t *counts;
+    uint32_t next;
+    pthread_mutex_t mu;
+} batch_t;
+
+static void *batch_worker(void *arg) {
+    batch_t *b = (batch_t *)arg;
+    for (;;) {
+        pthread_mutex_lock(&b->mu);
+        uint32_t i = b->next;
+        if (i < b->nq) b->next = i + 1;
+        pthread_mutex_unlock(&b->mu);
+        if (i >= b->nq) break;
+        b->counts[i] = spf_o_search(b->ix, b->q + (size_t)i * b->ix->dim, b->k, b->n_probe,
+                                    b->ids + (size_t)i * b->k, b->dists + (size_t)i * b->k);
+    }
+    return NULL;
+}
+
+void spf_o_search_batch(const spf_o_index *ix, const float *q, uint32_t nq, uint32_t k,
+                        uint32_t n_probe, uint32_t n_threads, uint64_t *out_ids,
+                        float *out_dists, uint32_t *out_counts) {
+    batch_t b = {ix, q, nq, k, n_probe, out_ids, out_dists, out_counts, 0,
+                 PTHREAD_MUTEX_INITIALIZER};
+    if (n_threads < 1) n_threads = 1;
+    if (n_threads == 1) {
+        batch_worker(&b);
+        return;
+    }
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * n_threads);
+    for (uint32_t t = 0; t < n_threads; t++) pthread_create(&th[t], NULL, batch_worker, &b);
+    for (uint32_t t = 0; t < n_threads; t++) pthread_join(th[t], NULL);
+    free(th);
+}

@@ -1,0 +1,132 @@
+"""pydiskann_amd -- MI355X batch search for SPFresh indexes.
+
+``SPFresh`` mirrors the search surface of the reference binding
+``pydiskann.SPFresh`` (python/src/lib.rs:303-361): ``build``, ``search``,
+``search_with_dists``, ``search_batch``, ``delete``, ``is_deleted``,
+``get_vector``, ``stats``, ``dim``, with the same argument names, defaults and
+error types.  Searches run on the GPU through libspf_hip.so.  Index mutation
+beyond tombstones (``insert``, ``gc``, ``compact``) belongs to the reference's
+CPU index and is not part of this path.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .index import IndexArrays, build_index, brute_force_f64, METRICS, QUANTS  # noqa: F401
+from ._lib import DeviceIndex, PREC_EXACT, PREC_MIXED, SpfError, lib  # noqa: F401
+
+__all__ = ["SPFresh", "IndexArrays", "build_index", "DeviceIndex", "PREC_EXACT", "PREC_MIXED"]
+
+
+class SPFresh:
+    """Search-side mirror of ``pydiskann.SPFresh``."""
+
+    def __init__(self, arrays: IndexArrays, path: str | None = None, device: int = 0):
+        self.arrays = arrays
+        self.path = path
+        self._dev = DeviceIndex(arrays, device=device)
+
+    # -- construction (python/src/lib.rs:308-319)
+    @staticmethod
+    def build(vectors, path: str, metric: str = "l2", quantizer=None, rerank_size: int = 0,
+              pq_subspaces: int = 8, max_posting_size: int = 128, min_posting_size: int = 16,
+              reassign_neighbors: int = 16, probe_ratio: float = math.inf,
+              n_postings: int | None = None, seed: int = 0) -> "SPFresh":
+        arr = build_index(np.asarray(vectors, dtype=np.float32), metric=metric,
+                          quantizer=quantizer, n_postings=n_postings,
+                          max_posting_size=max_posting_size, rerank_size=rerank_size,
+                          probe_ratio=probe_ratio, seed=seed)
+        return SPFresh(arr, path=path)
+
+    @staticmethod
+    def from_arrays(arrays: IndexArrays, device: int = 0) -> "SPFresh":
+        return SPFresh(arrays, device=device)
+
+    def save(self) -> None:
+        a = self.arrays
+        np.savez(self.path + ".spfgpu.npz", dim=a.dim, metric=a.metric, quant=a.quant,
+                 block_ids=a.block_ids, centroids=a.centroids, post_off=a.post_off, ids=a.ids,
+                 codes=a.codes, raw=a.raw, deleted=a.deleted, int8=np.array(
+                     [a.int8_scale, a.int8_offset], np.float32), rq_padded=a.rq_padded,
+                 rq_signs=a.rq_signs if a.rq_signs is not None else np.zeros(0, np.float32),
+                 probe_ratio=np.float32(a.probe_ratio), rerank_size=a.rerank_size)
+
+    @staticmethod
+    def open(path: str, metric: str = "l2") -> "SPFresh":
+        z = np.load(path + ".spfgpu.npz", allow_pickle=False)
+        arr = IndexArrays(dim=int(z["dim"]), metric=int(z["metric"]), quant=int(z["quant"]),
+                          block_ids=z["block_ids"], centroids=z["centroids"],
+                          post_off=z["post_off"], ids=z["ids"], codes=z["codes"], raw=z["raw"],
+                          deleted=z["deleted"], int8_scale=float(z["int8"][0]),
+                          int8_offset=float(z["int8"][1]), rq_padded=int(z["rq_padded"]),
+                          rq_signs=z["rq_signs"].reshape(3, -1) if z["rq_signs"].size else None,
+                          probe_ratio=float(z["probe_ratio"]), rerank_size=int(z["rerank_size"]))
+        return SPFresh(arr, path=path)
+
+    # -- search (python/src/lib.rs:326-337)
+    def _check_dim(self, n: int) -> None:
+        if n != self.arrays.dim:
+            raise RuntimeError(f"Query dim {n} != index dim {self.arrays.dim}")
+
+    def search_with_dists(self, query, k: int = 10, n_probe: int = 8) -> list[tuple[int, float]]:
+        q = np.asarray(query, dtype=np.float32).ravel()
+        self._check_dim(q.shape[0])
+        ids, dd, cnt = self._dev.search(q[None, :], k, n_probe)
+        return [(int(ids[0, i]), float(dd[0, i])) for i in range(int(cnt[0]))]
+
+    def search(self, query, k: int = 10, n_probe: int = 8) -> list[int]:
+        return [i for i, _ in self.search_with_dists(query, k, n_probe)]
+
+    def search_batch(self, queries, k: int = 10, n_probe: int = 8) -> list[list[int]]:
+        q = np.asarray(queries, dtype=np.float32)
+        if q.size == 0:
+            return []
+        if q.ndim != 2:
+            raise RuntimeError("queries must be a list of equal-length vectors")
+        self._check_dim(q.shape[1])
+        ids, _, cnt = self._dev.search(q, k, n_probe)
+        return [[int(x) for x in ids[i, : cnt[i]]] for i in range(q.shape[0])]
+
+    def search_batch_with_dists(self, queries, k: int = 10, n_probe: int = 8,
+                                precision: int = PREC_EXACT, rerank_size: int = 0):
+        """numpy form: (ids [nq,k] u64, dists [nq,k] f32, counts [nq])."""
+        q = np.asarray(queries, dtype=np.float32)
+        self._check_dim(q.shape[1])
+        return self._dev.search(q, k, n_probe, precision=precision, rerank_size=rerank_size)
+
+    # -- tombstones (python/src/lib.rs:324-325, 338)
+    def delete(self, ids) -> None:
+        ids = [int(i) for i in ids]
+        self.arrays.set_deleted(ids)
+        self._dev.delete(ids)
+
+    def is_deleted(self, id: int) -> bool:
+        return self.arrays.is_deleted(int(id))
+
+    def get_vector(self, id: int):
+        id = int(id)
+        if id >= self.arrays.n_raw or self.arrays.is_deleted(id):
+            return None
+        return [float(x) for x in self.arrays.raw[id]]
+
+    def stats(self) -> dict:
+        a = self.arrays
+        lens = a.posting_lengths()
+        dead = a.n_deleted()
+        return {"live": a.n_raw - dead, "deleted": dead, "postings": a.n_post,
+                "max_posting": int(lens.max()) if lens.size else 0}
+
+    @property
+    def dim(self) -> int:
+        return self.arrays.dim
+
+    def insert(self, vectors):
+        raise NotImplementedError("insert runs on the reference's CPU index")
+
+    def gc(self):
+        raise NotImplementedError("gc runs on the reference's CPU index")
+
+    def compact(self):
+        raise NotImplementedError("compact runs on the reference's CPU index")

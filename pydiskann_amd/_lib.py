@@ -1,0 +1,187 @@
+"""ctypes binding of libspf_hip.so (the C ABI declared in include/spf.h).
+
+The product search path: there is no fallback.  If the library is missing the
+import of a search entry point raises, naming the build step.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libspf_hip.so")
+
+SPF_OK, SPF_EINVAL, SPF_EDIM, SPF_EUNSUPPORTED, SPF_EDEVICE, SPF_ENOMEM = 0, -1, -2, -3, -4, -5
+PREC_EXACT, PREC_MIXED = 0, 1
+
+# every entry point of include/spf.h
+EXPORTS = ("spf_index_create", "spf_index_destroy", "spf_index_get_info", "spf_delete",
+           "spf_search_batch", "spf_search_batch_device", "spf_last_profile", "spf_last_error",
+           "spf_version")
+FLAG_PROFILE = 1
+STAGES = ("prep", "probe", "probe_select", "plan", "scan", "select", "rerank", "finalize")
+
+
+class IndexDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_uint32), ("metric", C.c_uint32), ("quantizer", C.c_uint32),
+        ("n_postings", C.c_uint32),
+        ("block_ids", C.c_void_p), ("centroids", C.c_void_p), ("posting_offsets", C.c_void_p),
+        ("entry_ids", C.c_void_p), ("codes", C.c_void_p), ("code_bytes", C.c_uint32),
+        ("raw", C.c_void_p), ("n_raw", C.c_uint64),
+        ("deleted_ids", C.c_void_p), ("n_deleted", C.c_uint64),
+        ("int8_scale", C.c_float), ("int8_offset", C.c_float),
+        ("rabitq_padded", C.c_uint32), ("rabitq_signs", C.c_void_p),
+        ("probe_ratio", C.c_float), ("rerank_size", C.c_uint32), ("device", C.c_int32),
+    ]
+
+
+class SearchOpts(C.Structure):
+    _fields_ = [("precision", C.c_uint32), ("rerank_size", C.c_uint32), ("flags", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
+
+
+class Profile(C.Structure):
+    _fields_ = [("stage_ms", C.c_float * 8), ("candidates", C.c_uint64),
+                ("scan_code_bytes", C.c_uint64), ("probe_pairs", C.c_uint64),
+                ("scan_items", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+class IndexInfo(C.Structure):
+    _fields_ = [("dim", C.c_uint32), ("metric", C.c_uint32), ("quantizer", C.c_uint32),
+                ("n_postings", C.c_uint32), ("n_entries", C.c_uint64), ("n_raw", C.c_uint64),
+                ("n_deleted", C.c_uint64), ("device_bytes", C.c_uint64),
+                ("max_posting", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+class SpfError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libspf_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                "g.build()'` (or make -C pydiskann_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        P, u32, u64 = C.c_void_p, C.c_uint32, C.c_uint64
+        L.spf_index_create.argtypes = [C.POINTER(IndexDesc), C.POINTER(C.c_void_p)]
+        L.spf_index_create.restype = C.c_int
+        L.spf_index_destroy.argtypes = [P]
+        L.spf_index_destroy.restype = None
+        L.spf_index_get_info.argtypes = [P, C.POINTER(IndexInfo)]
+        L.spf_index_get_info.restype = C.c_int
+        L.spf_delete.argtypes = [P, P, u64]
+        L.spf_delete.restype = C.c_int
+        L.spf_search_batch.argtypes = [P, P, u32, u32, u32, u32, C.POINTER(SearchOpts), P, P, P]
+        L.spf_search_batch.restype = C.c_int
+        L.spf_search_batch_device.argtypes = [P, P, u32, u32, u32, u32, C.POINTER(SearchOpts),
+                                              P, P, P, P]
+        L.spf_search_batch_device.restype = C.c_int
+        L.spf_last_profile.argtypes = [C.POINTER(Profile)]
+        L.spf_last_profile.restype = C.c_int
+        L.spf_last_error.restype = C.c_char_p
+        L.spf_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != SPF_OK:
+        msg = lib().spf_last_error().decode()
+        if rc == SPF_EINVAL:
+            raise ValueError(msg)
+        raise SpfError(rc, msg)
+
+
+def last_profile() -> dict:
+    pr = Profile()
+    check(lib().spf_last_profile(C.byref(pr)))
+    return {"stage_ms": dict(zip(STAGES, [float(x) for x in pr.stage_ms])),
+            "candidates": int(pr.candidates), "scan_code_bytes": int(pr.scan_code_bytes),
+            "probe_pairs": int(pr.probe_pairs), "scan_items": int(pr.scan_items)}
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceIndex:
+    """Device mirror of an ``IndexArrays`` (one GPU)."""
+
+    def __init__(self, ix, device: int = 0):
+        self._src = ix
+        d = IndexDesc()
+        d.dim, d.metric, d.quantizer, d.n_postings = ix.dim, ix.metric, ix.quant, ix.n_post
+        d.block_ids, d.centroids, d.posting_offsets = _p(ix.block_ids), _p(ix.centroids), _p(ix.post_off)
+        d.entry_ids, d.codes, d.code_bytes = _p(ix.ids), _p(ix.codes), ix.code_bytes
+        d.raw, d.n_raw = _p(ix.raw), ix.n_raw
+        dead = np.nonzero(np.unpackbits(ix.deleted, bitorder="little")[: ix.n_raw])[0].astype(np.uint64)
+        d.deleted_ids, d.n_deleted = _p(dead), dead.shape[0]
+        d.int8_scale, d.int8_offset = ix.int8_scale, ix.int8_offset
+        d.rabitq_padded, d.rabitq_signs = ix.rq_padded, _p(ix.rq_signs)
+        d.probe_ratio = ix.probe_ratio if ix.probe_ratio is not None else math.inf
+        d.rerank_size, d.device = ix.rerank_size, device
+        h = C.c_void_p()
+        check(lib().spf_index_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.dim = ix.dim
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().spf_index_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> IndexInfo:
+        inf = IndexInfo()
+        check(lib().spf_index_get_info(self.h, C.byref(inf)))
+        return inf
+
+    def delete(self, ids) -> None:
+        a = np.ascontiguousarray(np.asarray(ids, dtype=np.uint64).ravel())
+        check(lib().spf_delete(self.h, _p(a), a.shape[0]))
+
+    def search(self, queries, k: int, n_probe: int, precision: int = PREC_EXACT,
+               rerank_size: int = 0):
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim == 1:
+            q = q[None, :]
+        nq, dim = q.shape
+        ids = np.empty((nq, max(k, 1)), dtype=np.uint64)
+        dd = np.empty((nq, max(k, 1)), dtype=np.float32)
+        cnt = np.zeros(nq, dtype=np.uint32)
+        o = SearchOpts()
+        o.precision, o.rerank_size = precision, rerank_size
+        check(lib().spf_search_batch(self.h, _p(q), nq, dim, k, n_probe, C.byref(o), _p(ids),
+                                     _p(dd), _p(cnt)))
+        return ids, dd, cnt
+
+    def search_device(self, q_ptr: int, nq: int, dim: int, k: int, n_probe: int, ids_ptr: int,
+                      dists_ptr: int, counts_ptr: int, stream: int = 0,
+                      precision: int = PREC_EXACT, rerank_size: int = 0,
+                      profile: bool = False) -> None:
+        """Device-resident form: raw device pointers (e.g. torch .data_ptr())."""
+        o = SearchOpts()
+        o.precision, o.rerank_size = precision, rerank_size
+        o.flags = FLAG_PROFILE if profile else 0
+        check(lib().spf_search_batch_device(self.h, C.c_void_p(q_ptr), nq, dim, k, n_probe,
+                                            C.byref(o), C.c_void_p(ids_ptr),
+                                            C.c_void_p(dists_ptr), C.c_void_p(counts_ptr),
+                                            C.c_void_p(stream) if stream else None))

@@ -1,0 +1,712 @@
+// Host side of libspf_hip.so: device mirror of a search-ready SPFresh index,
+// per-call workspaces, and the batch-search pipeline behind the C ABI
+// (include/spf.h).
+//
+// Device layout (HBM):
+//   codes      tiled: 16-byte unit u of entry e at ((e/64)*units + u)*64 + e%64,
+//              so one wave64 load instruction reads 1 KiB contiguous; entries in
+//              CSR order, postings sorted by reference block index
+//   centroids  same tiled layout, f32
+//   rank       u32 per entry: position of its id among all ids (keys sort by
+//              (distance, id) without carrying 64-bit ids through the scan)
+//   rank_id    u64 per rank; raw f32 rows per rank (dpad stride) for rerank
+//   tombstones 1 bit per entry
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "spf.h"
+#include "spf_internal.h"
+
+using namespace spf;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local spf_profile g_prof;
+thread_local bool g_prof_valid = false;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                   \
+    do {                                                                            \
+        hipError_t _e = (x);                                                        \
+        if (_e != hipSuccess)                                                       \
+            return fail(_e == hipErrorOutOfMemory ? SPF_ENOMEM : SPF_EDEVICE,       \
+                        std::string(#x) + ": " + hipGetErrorString(_e));            \
+    } while (0)
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= cap && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t nn = n + n / 4 + 64;
+        hipError_t e = hipMalloc((void **)&p, nn * sizeof(T));
+        if (e == hipSuccess) cap = nn;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct Workspace {
+    hipStream_t stream = nullptr;
+    DBuf<float> qin, qv_probe, qv_scan, qnorm;
+    DBuf<uint32_t> q8, qsq;
+    DBuf<uint64_t> probe_keys, probe_sel;
+    DBuf<uint32_t> probe_cnt, np_q, probe_post;
+    DBuf<uint64_t> cand_len, cand_off, qp_slot, pair_slot;
+    DBuf<uint32_t> post_cnt, post_fill, pair_off, pair_q, item_cnt, item_off;
+    DBuf<uint64_t> cand, sel, fin;
+    DBuf<uint32_t> sel_cnt, fin_cnt;
+    DBuf<uint64_t> out_ids;
+    DBuf<float> out_d;
+    DBuf<uint32_t> out_c;
+    uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
+    hipEvent_t ev[SPF_N_STAGES + 1] = {};
+
+    void release() {
+        DBuf<float> *fs[] = {&qin, &qv_probe, &qv_scan, &qnorm, &out_d};
+        for (auto *b : fs) b->release();
+        DBuf<uint32_t> *us[] = {&q8, &qsq, &probe_cnt, &np_q, &probe_post, &post_cnt, &post_fill,
+                                &pair_off, &pair_q, &item_cnt, &item_off, &sel_cnt, &fin_cnt,
+                                &out_c};
+        for (auto *b : us) b->release();
+        DBuf<uint64_t> *ls[] = {&probe_keys, &probe_sel, &cand_len, &cand_off, &qp_slot,
+                                &pair_slot, &cand, &sel, &fin, &out_ids};
+        for (auto *b : ls) b->release();
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (h_summary) (void)hipHostFree(h_summary);
+        h_summary = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+inline uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+struct spf_index {
+    int device = 0;
+    uint32_t dim = 0, dpad = 0, metric = 0, quant = 0, P = 0, code_bytes = 0;
+    uint32_t units = 0, cent_units = 0, code_kind = 0, rerank_size = 0, max_len = 0;
+    uint64_t N = 0, n_raw = 0, n_deleted = 0;
+    float scale = 1.0f, offset = 0.0f, probe_ratio = INFINITY;
+
+    uint4 *d_cent = nullptr;
+    float *d_cnorm = nullptr;
+    uint32_t *d_post_off = nullptr, *d_post_len = nullptr;
+    uint4 *d_codes = nullptr;
+    uint32_t *d_rank = nullptr;
+    uint64_t *d_rank_id = nullptr;
+    float *d_raw = nullptr;
+    uint32_t *d_del = nullptr;
+    float *d_xnorm = nullptr;
+    uint32_t *d_xsq = nullptr;
+    uint64_t device_bytes = 0;
+
+    std::vector<uint64_t> rank_id;      // ids in ascending order
+    std::vector<uint32_t> rank_entry;   // entry holding each rank
+    std::vector<uint32_t> h_del;        // tombstone bit per entry
+    std::vector<uint8_t> id_del;        // tombstone bit per id < n_raw
+
+    std::mutex ws_mu;
+    std::vector<Workspace *> pool;
+
+    ~spf_index() {
+        void *ps[] = {d_cent, d_cnorm, d_post_off, d_post_len, d_codes, d_rank,
+                      d_rank_id, d_raw, d_del, d_xnorm, d_xsq};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+        for (Workspace *w : pool) {
+            w->release();
+            delete w;
+        }
+    }
+};
+
+namespace {
+
+template <class T>
+hipError_t upload(T **dst, const T *src, size_t n, uint64_t &bytes) {
+    hipError_t e = hipMalloc((void **)dst, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    bytes += std::max<size_t>(n, 1) * sizeof(T);
+    if (n) e = hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+
+// row-major rows of `row_bytes` -> tiled 16-byte units
+std::vector<uint4> tile_rows(const uint8_t *rows, uint64_t n, uint32_t row_bytes, uint32_t units,
+                             const uint32_t *order) {
+    const uint64_t tiles = (n + 63) / 64;
+    std::vector<uint4> t(tiles * units * 64, make_uint4(0, 0, 0, 0));
+    for (uint64_t e = 0; e < n; e++) {
+        const uint8_t *src = rows + (uint64_t)(order ? order[e] : e) * row_bytes;
+        for (uint32_t u = 0; u < units; u++) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            const uint32_t b0 = u * 16;
+            const uint32_t nb = b0 >= row_bytes ? 0 : std::min<uint32_t>(16, row_bytes - b0);
+            if (nb) std::memcpy(&v, src + b0, nb);
+            t[((e >> 6) * units + u) * 64 + (e & 63)] = v;
+        }
+    }
+    return t;
+}
+
+Workspace *acquire(spf_index *ix) {
+    {
+        std::lock_guard<std::mutex> g(ix->ws_mu);
+        if (!ix->pool.empty()) {
+            Workspace *w = ix->pool.back();
+            ix->pool.pop_back();
+            return w;
+        }
+    }
+    Workspace *w = new Workspace();
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&w->h_summary, 2 * sizeof(uint64_t)) != hipSuccess) {
+        w->release();
+        delete w;
+        return nullptr;
+    }
+    return w;
+}
+
+void release_ws(spf_index *ix, Workspace *w) {
+    std::lock_guard<std::mutex> g(ix->ws_mu);
+    ix->pool.push_back(w);
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+int create_impl(const spf_index_desc *d, spf_index **out) {
+    if (!d || !out) return fail(SPF_EINVAL, "null argument");
+    *out = nullptr;
+    if (d->quantizer == SPF_QUANT_PQ)
+        return fail(SPF_EUNSUPPORTED, "PQ postings have no device path");
+    if (d->quantizer == SPF_QUANT_RABITQ)
+        return fail(SPF_EUNSUPPORTED, "RaBitQ postings are not served by this build yet");
+    if (d->quantizer > SPF_QUANT_RABITQ) return fail(SPF_EINVAL, "unknown quantizer");
+    if (d->metric > SPF_METRIC_DOT) return fail(SPF_EINVAL, "unknown metric");
+    if (d->dim == 0 || d->dim > 4096) return fail(SPF_EINVAL, "dim must be in [1, 4096]");
+    const uint32_t P = d->n_postings;
+    if (P && (!d->block_ids || !d->centroids || !d->posting_offsets))
+        return fail(SPF_EINVAL, "postings arrays missing");
+    const uint64_t N = P ? d->posting_offsets[P] : 0;
+    if (N >= 0xffffffffull) return fail(SPF_EINVAL, "more than 2^32-2 entries");
+    if (N && (!d->entry_ids || !d->codes)) return fail(SPF_EINVAL, "entry arrays missing");
+    uint32_t want_cb = 0;
+    switch (d->quantizer) {
+    case SPF_QUANT_NONE: want_cb = 4 * d->dim; break;
+    case SPF_QUANT_F16: want_cb = 2 * d->dim; break;
+    case SPF_QUANT_INT8: want_cb = d->dim; break;
+    }
+    if (d->code_bytes != want_cb) return fail(SPF_EINVAL, "code_bytes does not match quantizer");
+    if (d->quantizer == SPF_QUANT_INT8 && !(d->int8_scale > 0.0f))
+        return fail(SPF_EINVAL, "int8 scale must be > 0");
+    const bool need_raw = d->quantizer != SPF_QUANT_NONE;
+    if (need_raw && !d->raw) return fail(SPF_EINVAL, "quantized index needs the raw store");
+    for (uint32_t p = 0; p < P; p++)
+        if (d->posting_offsets[p + 1] < d->posting_offsets[p])
+            return fail(SPF_EINVAL, "posting offsets not monotone");
+
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (d->device < 0 || d->device >= ndev) return fail(SPF_EINVAL, "bad device ordinal");
+    DeviceGuard dg(d->device);
+
+    spf_index *ix = new spf_index();
+    ix->device = d->device;
+    ix->dim = d->dim;
+    ix->dpad = round_up(d->dim, 16);
+    ix->metric = d->metric;
+    ix->quant = d->quantizer;
+    ix->P = P;
+    ix->N = N;
+    ix->code_bytes = d->code_bytes;
+    ix->n_raw = d->n_raw;
+    ix->scale = d->int8_scale;
+    ix->offset = d->int8_offset;
+    ix->probe_ratio = d->probe_ratio;
+    ix->rerank_size = d->rerank_size;
+    ix->code_kind = d->quantizer == SPF_QUANT_F16 ? CK_F16
+                    : d->quantizer == SPF_QUANT_INT8 ? CK_U8 : CK_F32;
+    ix->units = (d->code_bytes + 15) / 16;
+    ix->cent_units = (4 * d->dim + 15) / 16;
+    std::unique_ptr<spf_index> guard(ix);
+
+    // postings sorted by reference block index: equal centroid distances then
+    // order by block index through the low key bits
+    std::vector<uint32_t> perm(P);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::sort(perm.begin(), perm.end(),
+              [&](uint32_t a, uint32_t b) { return d->block_ids[a] < d->block_ids[b]; });
+    for (uint32_t i = 1; i < P; i++)
+        if (d->block_ids[perm[i]] == d->block_ids[perm[i - 1]])
+            return fail(SPF_EINVAL, "duplicate block id");
+    std::vector<uint32_t> post_off(P + 1, 0), post_len(P, 0), entry_src(N);
+    {
+        uint64_t o = 0;
+        for (uint32_t i = 0; i < P; i++) {
+            const uint32_t p = perm[i];
+            const uint64_t b = d->posting_offsets[p], e = d->posting_offsets[p + 1];
+            post_off[i] = (uint32_t)o;
+            post_len[i] = (uint32_t)(e - b);
+            ix->max_len = std::max(ix->max_len, post_len[i]);
+            for (uint64_t j = b; j < e; j++) entry_src[o++] = (uint32_t)j;
+        }
+        post_off[P] = (uint32_t)o;
+    }
+    // ranks
+    std::vector<uint32_t> by_id(N);
+    std::iota(by_id.begin(), by_id.end(), 0u);
+    std::sort(by_id.begin(), by_id.end(), [&](uint32_t a, uint32_t b) {
+        return d->entry_ids[entry_src[a]] < d->entry_ids[entry_src[b]];
+    });
+    ix->rank_id.resize(N);
+    ix->rank_entry.resize(N);
+    std::vector<uint32_t> entry_rank(N);
+    for (uint64_t r = 0; r < N; r++) {
+        const uint32_t e = by_id[r];
+        ix->rank_id[r] = d->entry_ids[entry_src[e]];
+        ix->rank_entry[r] = e;
+        entry_rank[e] = (uint32_t)r;
+        if (r && ix->rank_id[r] == ix->rank_id[r - 1])
+            return fail(SPF_EINVAL, "an id appears in more than one entry");
+        if (need_raw && ix->rank_id[r] >= d->n_raw)
+            return fail(SPF_EINVAL, "entry id beyond the raw store");
+    }
+
+    uint64_t &bytes = ix->device_bytes;
+    // centroids
+    {
+        std::vector<uint4> ct =
+            tile_rows((const uint8_t *)d->centroids, P, 4 * d->dim, ix->cent_units, perm.data());
+        HIPCHK(upload(&ix->d_cent, ct.data(), ct.size(), bytes));
+    }
+    HIPCHK(upload(&ix->d_post_off, post_off.data(), post_off.size(), bytes));
+    HIPCHK(upload(&ix->d_post_len, post_len.data(), post_len.size(), bytes));
+    {
+        std::vector<uint4> ct = tile_rows(d->codes, N, d->code_bytes, ix->units, entry_src.data());
+        HIPCHK(upload(&ix->d_codes, ct.data(), ct.size(), bytes));
+    }
+    HIPCHK(upload(&ix->d_rank, entry_rank.data(), entry_rank.size(), bytes));
+    HIPCHK(upload(&ix->d_rank_id, ix->rank_id.data(), ix->rank_id.size(), bytes));
+    if (need_raw) {
+        std::vector<float> rr((size_t)N * ix->dpad, 0.0f);
+        for (uint64_t r = 0; r < N; r++)
+            std::memcpy(&rr[(size_t)r * ix->dpad], d->raw + (size_t)ix->rank_id[r] * d->dim,
+                        4 * d->dim);
+        HIPCHK(upload(&ix->d_raw, rr.data(), rr.size(), bytes));
+    }
+    // tombstones
+    ix->id_del.assign((d->n_raw + 7) / 8, 0);
+    ix->h_del.assign((N + 31) / 32 + 1, 0);
+    for (uint64_t i = 0; i < d->n_deleted; i++) {
+        const uint64_t id = d->deleted_ids[i];
+        if (id >= d->n_raw || ((ix->id_del[id >> 3] >> (id & 7)) & 1)) continue;
+        ix->id_del[id >> 3] |= (uint8_t)(1u << (id & 7));
+        ix->n_deleted++;
+        auto it = std::lower_bound(ix->rank_id.begin(), ix->rank_id.end(), id);
+        if (it != ix->rank_id.end() && *it == id) {
+            const uint32_t e = ix->rank_entry[it - ix->rank_id.begin()];
+            ix->h_del[e >> 5] |= 1u << (e & 31);
+        }
+    }
+    HIPCHK(upload(&ix->d_del, ix->h_del.data(), ix->h_del.size(), bytes));
+    // norms for cosine, u8 squares for int8 L2
+    if (d->metric == SPF_METRIC_COSINE) {
+        HIPCHK(hipMalloc((void **)&ix->d_xnorm, std::max<uint64_t>(N, 1) * 4));
+        HIPCHK(hipMalloc((void **)&ix->d_cnorm, std::max<uint32_t>(P, 1) * 4));
+        bytes += 4 * (N + P);
+        HIPCHK(launch_entry_norms(ix->code_kind, ix->d_codes, ix->units, (uint32_t)N, d->dim,
+                                  ix->scale, ix->offset, ix->d_xnorm, nullptr, nullptr));
+        HIPCHK(launch_entry_norms(CK_F32, ix->d_cent, ix->cent_units, P, d->dim, 1.0f, 0.0f,
+                                  ix->d_cnorm, nullptr, nullptr));
+    }
+    if (ix->code_kind == CK_U8 &&
+        (d->metric == SPF_METRIC_L2 || d->metric == SPF_METRIC_L2SQ)) {
+        HIPCHK(hipMalloc((void **)&ix->d_xsq, std::max<uint64_t>(N, 1) * 4));
+        bytes += 4 * N;
+        HIPCHK(launch_entry_norms(CK_U8, ix->d_codes, ix->units, (uint32_t)N, d->dim, ix->scale,
+                                  ix->offset, nullptr, ix->d_xsq, nullptr));
+    }
+    HIPCHK(hipDeviceSynchronize());
+    *out = guard.release();
+    return SPF_OK;
+}
+
+int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, uint32_t nq,
+                uint32_t k, uint32_t n_probe, const spf_search_opts *opts, uint64_t *d_ids,
+                float *d_dists, uint32_t *d_counts) {
+    const uint32_t P = ix->P;
+    const uint32_t npr = std::min(std::max(n_probe, 1u), std::max(P, 1u));
+    const bool rerank = ix->quant != SPF_QUANT_NONE && ix->rerank_size > 0;
+    const uint32_t want = rerank ? std::max(k, ix->rerank_size) : k;
+    if (want > kMaxK) return fail(SPF_EINVAL, "k / rerank_size above 2048");
+    if (opts && opts->precision != SPF_PREC_EXACT)
+        return fail(SPF_EUNSUPPORTED, "mixed precision is not served by this build yet");
+    const uint32_t dpad = ix->dpad;
+    const uint32_t G = dpad <= 1024 ? 16 : 4;
+    const bool prof = opts && (opts->flags & SPF_FLAG_PROFILE);
+    if (prof)
+        for (auto &e : w->ev)
+            if (!e) HIPCHK(hipEventCreate(&e));
+    int next_mark = 0;
+    // event i closes stage i-1; stages that do not run get zero time
+    auto mark = [&](int stage) -> hipError_t {
+        if (!prof) return hipSuccess;
+        while (next_mark <= stage) {
+            hipError_t e = hipEventRecord(w->ev[next_mark], s);
+            if (e != hipSuccess) return e;
+            next_mark++;
+        }
+        return hipSuccess;
+    };
+    HIPCHK(mark(SPF_STAGE_PREP));
+
+    HIPCHK(w->qv_probe.ensure((size_t)nq * dpad));
+    HIPCHK(w->qnorm.ensure(nq));
+    const bool round16 = ix->quant == SPF_QUANT_F16 &&
+                         (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
+    const bool q8 = ix->quant == SPF_QUANT_INT8 &&
+                    (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
+    if (round16) HIPCHK(w->qv_scan.ensure((size_t)nq * dpad));
+    HIPCHK(w->q8.ensure((size_t)nq * dpad / 4));
+    HIPCHK(w->qsq.ensure(nq));
+    HIPCHK(w->sel.ensure((size_t)nq * want));
+    HIPCHK(w->sel_cnt.ensure(nq));
+
+    PrepParams pp{};
+    pp.q = d_q;
+    pp.nq = nq;
+    pp.dim = ix->dim;
+    pp.dpad = dpad;
+    pp.round_f16 = round16;
+    pp.want_q8 = q8;
+    pp.scale = ix->scale;
+    pp.offset = ix->offset;
+    pp.qv_probe = w->qv_probe.p;
+    pp.qv_scan = round16 ? w->qv_scan.p : w->qv_probe.p;
+    pp.q8 = w->q8.p;
+    pp.qnorm = w->qnorm.p;
+    pp.qsq = w->qsq.p;
+    HIPCHK(launch_prep(pp, s));
+    HIPCHK(mark(SPF_STAGE_PROBE));
+    const float *qv_scan = pp.qv_scan;
+
+    if (P == 0) {
+        HIPCHK(hipMemsetAsync(w->sel_cnt.p, 0, nq * sizeof(uint32_t), s));
+        HIPCHK(launch_finalize(w->sel.p, want, w->sel_cnt.p, nq, k, ix->d_rank_id, d_ids, d_dists,
+                               d_counts, s));
+        return SPF_OK;
+    }
+
+    // ---- probe: every query x every centroid, then top-npr per query
+    HIPCHK(w->probe_keys.ensure((size_t)nq * P));
+    HIPCHK(w->probe_sel.ensure((size_t)nq * npr));
+    HIPCHK(w->probe_cnt.ensure(nq));
+    {
+        ScanParams sp{};
+        sp.data = ix->d_cent;
+        sp.units = ix->cent_units;
+        sp.n_entries = P;
+        sp.xnorm = ix->d_cnorm;
+        sp.qv = w->qv_probe.p;
+        sp.qnorm = w->qnorm.p;
+        sp.dpad = dpad;
+        sp.out = w->probe_keys.p;
+        sp.nq = nq;
+        sp.n_echunks = (P + kEChunk - 1) / kEChunk;
+        sp.out_stride = P;
+        const uint64_t items = (uint64_t)((nq + G - 1) / G) * sp.n_echunks;
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(items, 16384);
+        HIPCHK(launch_scan(CK_F32, ix->metric, true, G, sp, grid, s));
+    }
+    HIPCHK(mark(SPF_STAGE_PROBE_SELECT));
+    HIPCHK(launch_select(w->probe_keys.p, nullptr, P, nq, npr, w->probe_sel.p, w->probe_cnt.p, s));
+    HIPCHK(mark(SPF_STAGE_PLAN));
+
+    // ---- plan: probe_ratio cut, candidate offsets, posting-major pairs
+    HIPCHK(w->np_q.ensure(nq));
+    HIPCHK(w->probe_post.ensure((size_t)nq * npr));
+    HIPCHK(w->cand_len.ensure(nq + 1));
+    HIPCHK(w->cand_off.ensure(nq + 1));
+    HIPCHK(w->post_cnt.ensure(P + 1));
+    HIPCHK(w->post_fill.ensure(P + 1));
+    HIPCHK(w->pair_off.ensure(P + 1));
+    HIPCHK(w->qp_slot.ensure((size_t)nq * npr));
+    HIPCHK(w->pair_q.ensure((size_t)nq * npr));
+    HIPCHK(w->pair_slot.ensure((size_t)nq * npr));
+    HIPCHK(w->item_cnt.ensure(P + 1));
+    HIPCHK(w->item_off.ensure(P + 1));
+    PlanParams pl{};
+    pl.nq = nq;
+    pl.n_post = P;
+    pl.npr = npr;
+    pl.probe_sel = w->probe_sel.p;
+    pl.probe_cnt = w->probe_cnt.p;
+    pl.probe_ratio = ix->probe_ratio;
+    pl.post_len = ix->d_post_len;
+    pl.np_q = w->np_q.p;
+    pl.probe_post = w->probe_post.p;
+    pl.cand_len = w->cand_len.p;
+    pl.cand_off = w->cand_off.p;
+    pl.post_cnt = w->post_cnt.p;
+    pl.post_fill = w->post_fill.p;
+    pl.pair_off = w->pair_off.p;
+    pl.qp_slot = w->qp_slot.p;
+    pl.pair_q = w->pair_q.p;
+    pl.pair_slot = w->pair_slot.p;
+    pl.item_cnt = w->item_cnt.p;
+    pl.G = G;
+    HIPCHK(launch_plan_probe(pl, s));
+    HIPCHK(launch_excl_scan_u64(w->cand_len.p, w->cand_off.p, nq, s));
+    HIPCHK(hipMemsetAsync(w->post_cnt.p, 0, (P + 1) * sizeof(uint32_t), s));
+    HIPCHK(hipMemsetAsync(w->post_fill.p, 0, (P + 1) * sizeof(uint32_t), s));
+    HIPCHK(launch_plan_pairs(pl, s));
+    HIPCHK(launch_excl_scan_u32(w->post_cnt.p, w->pair_off.p, P, s));
+    HIPCHK(launch_scatter_pairs(pl, s));
+    HIPCHK(launch_items(pl, s));
+    HIPCHK(launch_excl_scan_u32(w->item_cnt.p, w->item_off.p, P, s));
+    w->h_summary[1] = 0;
+    HIPCHK(hipMemcpyAsync(&w->h_summary[0], w->cand_off.p + nq, sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&w->h_summary[1], w->item_off.p + P, sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const uint64_t total = w->h_summary[0];
+    const uint32_t n_items = (uint32_t)w->h_summary[1];
+
+    // ---- posting scan
+    HIPCHK(w->cand.ensure(std::max<uint64_t>(total, 1)));
+    HIPCHK(mark(SPF_STAGE_SCAN));
+    {
+        ScanParams sp{};
+        sp.data = ix->d_codes;
+        sp.units = ix->units;
+        sp.n_entries = (uint32_t)ix->N;
+        sp.low = ix->d_rank;
+        sp.del = ix->d_del;
+        sp.xnorm = ix->d_xnorm;
+        sp.xsq = ix->d_xsq;
+        sp.qv = qv_scan;
+        sp.q8 = w->q8.p;
+        sp.qnorm = w->qnorm.p;
+        sp.qsq = w->qsq.p;
+        sp.dpad = dpad;
+        sp.scale = ix->scale;
+        sp.offset = ix->offset;
+        sp.out = w->cand.p;
+        sp.item_off = w->item_off.p;
+        sp.n_post = P;
+        sp.n_items = n_items;
+        sp.post_off = ix->d_post_off;
+        sp.pair_off = w->pair_off.p;
+        sp.pair_q = w->pair_q.p;
+        sp.pair_slot = w->pair_slot.p;
+        const uint32_t grid = std::min<uint32_t>(n_items, 16384);
+        HIPCHK(launch_scan(ix->code_kind, ix->metric, false, G, sp, grid, s));
+    }
+    // ---- top-`want` per query
+    HIPCHK(mark(SPF_STAGE_SELECT));
+    HIPCHK(launch_select(w->cand.p, w->cand_off.p, 0, nq, want, w->sel.p, w->sel_cnt.p, s));
+    HIPCHK(mark(SPF_STAGE_RERANK));
+    if (rerank) {
+        HIPCHK(w->fin.ensure((size_t)nq * k));
+        HIPCHK(w->fin_cnt.ensure(nq));
+        RerankParams rp{};
+        rp.sel = w->sel.p;
+        rp.sel_cnt = w->sel_cnt.p;
+        rp.want = want;
+        rp.k = k;
+        rp.nq = nq;
+        rp.raw = ix->d_raw;
+        rp.qv = w->qv_probe.p;
+        rp.qnorm = w->qnorm.p;
+        rp.dim = ix->dim;
+        rp.dpad = dpad;
+        rp.metric = ix->metric;
+        rp.out = w->fin.p;
+        rp.out_cnt = w->fin_cnt.p;
+        HIPCHK(launch_rerank(rp, s));
+        HIPCHK(mark(SPF_STAGE_FINALIZE));
+        HIPCHK(launch_finalize(w->fin.p, k, w->fin_cnt.p, nq, k, ix->d_rank_id, d_ids, d_dists,
+                               d_counts, s));
+    } else {
+        HIPCHK(mark(SPF_STAGE_FINALIZE));
+        HIPCHK(launch_finalize(w->sel.p, want, w->sel_cnt.p, nq, k, ix->d_rank_id, d_ids,
+                               d_dists, d_counts, s));
+    }
+    if (prof) {
+        HIPCHK(mark(SPF_N_STAGES));
+        HIPCHK(hipEventSynchronize(w->ev[SPF_N_STAGES]));
+        std::memset(&g_prof, 0, sizeof g_prof);
+        for (int i = 0; i < SPF_N_STAGES; i++)
+            HIPCHK(hipEventElapsedTime(&g_prof.stage_ms[i], w->ev[i], w->ev[i + 1]));
+        g_prof.candidates = total;
+        g_prof.scan_code_bytes = total * ix->code_bytes;
+        g_prof.probe_pairs = (uint64_t)nq * P;
+        g_prof.scan_items = n_items;
+        g_prof_valid = true;
+    }
+    return SPF_OK;
+}
+
+int check_search_args(spf_index *ix, uint32_t dim, uint32_t k) {
+    if (!ix) return fail(SPF_EINVAL, "null index");
+    if (dim != ix->dim) {
+        char m[96];
+        std::snprintf(m, sizeof m, "Query dim %u != index dim %u", dim, ix->dim);
+        return fail(SPF_EDIM, m);
+    }
+    if (k == 0) return fail(SPF_EINVAL, "k must be >= 1");
+    return SPF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spf_index_create(const spf_index_desc *desc, spf_index **out) {
+    try {
+        return create_impl(desc, out);
+    } catch (const std::bad_alloc &) {
+        return fail(SPF_ENOMEM, "host allocation failed");
+    }
+}
+
+void spf_index_destroy(spf_index *ix) {
+    if (!ix) return;
+    DeviceGuard dg(ix->device);
+    delete ix;
+}
+
+int spf_index_get_info(const spf_index *ix, spf_index_info *o) {
+    if (!ix || !o) return fail(SPF_EINVAL, "null argument");
+    std::memset(o, 0, sizeof *o);
+    o->dim = ix->dim;
+    o->metric = ix->metric;
+    o->quantizer = ix->quant;
+    o->n_postings = ix->P;
+    o->n_entries = ix->N;
+    o->n_raw = ix->n_raw;
+    o->n_deleted = ix->n_deleted;
+    o->device_bytes = ix->device_bytes;
+    o->max_posting = ix->max_len;
+    return SPF_OK;
+}
+
+int spf_delete(spf_index *ix, const uint64_t *ids, uint64_t n) {
+    if (!ix || (n && !ids)) return fail(SPF_EINVAL, "null argument");
+    DeviceGuard dg(ix->device);
+    bool changed = false;
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t id = ids[i];
+        if (id >= ix->n_raw || ((ix->id_del[id >> 3] >> (id & 7)) & 1)) continue;
+        ix->id_del[id >> 3] |= (uint8_t)(1u << (id & 7));
+        ix->n_deleted++;
+        auto it = std::lower_bound(ix->rank_id.begin(), ix->rank_id.end(), id);
+        if (it != ix->rank_id.end() && *it == id) {
+            const uint32_t e = ix->rank_entry[it - ix->rank_id.begin()];
+            ix->h_del[e >> 5] |= 1u << (e & 31);
+            changed = true;
+        }
+    }
+    if (changed)
+        HIPCHK(hipMemcpy(ix->d_del, ix->h_del.data(), ix->h_del.size() * 4,
+                         hipMemcpyHostToDevice));
+    return SPF_OK;
+}
+
+int spf_search_batch_device(spf_index *ix, const float *d_queries, uint32_t nq, uint32_t dim,
+                            uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
+                            uint64_t *d_out_ids, float *d_out_dists, uint32_t *d_out_counts,
+                            void *stream) {
+    int rc = check_search_args(ix, dim, k);
+    if (rc) return rc;
+    if (nq == 0) return SPF_OK;
+    DeviceGuard dg(ix->device);
+    Workspace *w = acquire(ix);
+    if (!w) return fail(SPF_EDEVICE, "workspace creation failed");
+    hipStream_t s = stream ? (hipStream_t)stream : w->stream;
+    rc = search_impl(ix, w, s, d_queries, nq, k, n_probe, opts, d_out_ids, d_out_dists,
+                     d_out_counts);
+    release_ws(ix, w);
+    return rc;
+}
+
+int spf_search_batch(spf_index *ix, const float *queries, uint32_t nq, uint32_t dim, uint32_t k,
+                     uint32_t n_probe, const spf_search_opts *opts, uint64_t *out_ids,
+                     float *out_dists, uint32_t *out_counts) {
+    int rc = check_search_args(ix, dim, k);
+    if (rc) return rc;
+    if (nq == 0) return SPF_OK;
+    if (!queries || !out_ids || !out_dists || !out_counts) return fail(SPF_EINVAL, "null buffer");
+    DeviceGuard dg(ix->device);
+    Workspace *w = acquire(ix);
+    if (!w) return fail(SPF_EDEVICE, "workspace creation failed");
+    hipStream_t s = w->stream;
+    auto run = [&]() -> int {
+        HIPCHK(w->qin.ensure((size_t)nq * dim));
+        HIPCHK(w->out_ids.ensure((size_t)nq * k));
+        HIPCHK(w->out_d.ensure((size_t)nq * k));
+        HIPCHK(w->out_c.ensure(nq));
+        HIPCHK(hipMemcpyAsync(w->qin.p, queries, (size_t)nq * dim * 4, hipMemcpyHostToDevice, s));
+        int r = search_impl(ix, w, s, w->qin.p, nq, k, n_probe, opts, w->out_ids.p, w->out_d.p,
+                            w->out_c.p);
+        if (r) return r;
+        HIPCHK(hipMemcpyAsync(out_ids, w->out_ids.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(out_dists, w->out_d.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(out_counts, w->out_c.p, (size_t)nq * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return SPF_OK;
+    };
+    rc = run();
+    release_ws(ix, w);
+    return rc;
+}
+
+int spf_last_profile(spf_profile *out) {
+    if (!out) return fail(SPF_EINVAL, "null argument");
+    if (!g_prof_valid) return fail(SPF_EINVAL, "no profiled search on this thread");
+    *out = g_prof;
+    return SPF_OK;
+}
+
+const char *spf_last_error(void) { return g_err.c_str(); }
+
+const char *spf_version(void) {
+    return "spf-hip 0.1 gfx950 (-O3 -ffp-contract=off, canonical sequential-fma order)";
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+// Internal interface between the host library (spf_host.cpp) and the gfx950
+// kernels (spf_kernels.hip).  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spf {
+
+enum CodeKind : uint32_t { CK_F32 = 0, CK_F16 = 1, CK_U8 = 2 };
+enum Metric : uint32_t { M_L2 = 0, M_L2SQ = 1, M_COS = 2, M_DOT = 3 };
+
+constexpr uint32_t kWG = 256;        // threads per workgroup (4 wave64)
+constexpr uint32_t kSelCap = 4096;   // LDS keys held by the selection kernel
+constexpr uint32_t kMaxK = 2048;     // largest k / rerank pool a query may ask for
+constexpr uint32_t kEChunk = 1024;   // entries of one posting per work item
+constexpr uint64_t kNoKey = ~0ull;   // "no candidate" (deleted / padding)
+
+// One scan launch: distances between staged queries and code rows held in
+// the tiled layout (16-byte unit u of entry e at ((e/64)*units + u)*64 + e%64).
+struct ScanParams {
+    const uint4 *data;
+    uint32_t units;
+    uint32_t n_entries;
+    const uint32_t *low;        // low 32 key bits per entry (rank of its id); null = entry index
+    const uint32_t *del;        // tombstone bit per entry; null = none
+    const float *xnorm;         // cosine: |x|^2 per entry (canonical order)
+    const uint32_t *xsq;        // int8 L2: sum c^2 per entry
+    const float *qv;            // [nq][dpad] query values as the metric consumes them
+    const uint32_t *q8;         // [nq][dpad/4] int8-encoded query, 4 codes per word
+    const float *qnorm;         // [nq] |q|^2
+    const uint32_t *qsq;        // [nq] sum q8^2
+    uint32_t dpad;              // padded dim (multiple of 16)
+    float scale, offset;        // int8 codec
+    uint64_t *out;              // candidate keys (fkey(dist) << 32 | low)
+    // dense mode (centroid probe): all queries x entries [0, n_entries)
+    uint32_t nq;
+    uint32_t n_echunks;
+    uint64_t out_stride;
+    // listed mode (posting scan): work items built by the planner
+    const uint32_t *item_off;   // [n_post + 1]
+    uint32_t n_post;
+    uint32_t n_items;
+    const uint32_t *post_off;   // [n_post + 1] entry offsets
+    const uint32_t *pair_off;   // [n_post + 1] (query, slot) pairs per posting
+    const uint32_t *pair_q;
+    const uint64_t *pair_slot;  // candidate-buffer position of the posting's first entry
+};
+
+struct PrepParams {
+    const float *q;             // [nq][dim] caller queries
+    uint32_t nq, dim, dpad;
+    uint32_t round_f16;         // F16 L2 scan: query rounded to f16 (src/sq.rs:51-55)
+    uint32_t want_q8;           // Int8 L2 scan: query encoded (src/sq.rs:208-220)
+    float scale, offset;
+    float *qv_probe;            // [nq][dpad] f32 query (zero padded)
+    float *qv_scan;             // [nq][dpad] (may alias qv_probe when not rounding)
+    uint32_t *q8;               // [nq][dpad/4]
+    float *qnorm;               // [nq]
+    uint32_t *qsq;              // [nq]
+};
+
+struct PlanParams {
+    uint32_t nq, n_post, npr;   // npr = probes requested per query (after clamp)
+    const uint64_t *probe_sel;  // [nq][npr] sorted probe keys
+    const uint32_t *probe_cnt;  // [nq]
+    float probe_ratio;
+    const uint32_t *post_len;   // [n_post]
+    uint32_t *np_q;             // [nq] probes kept after the probe_ratio cut
+    uint32_t *probe_post;       // [nq][npr]
+    uint64_t *cand_len;         // [nq + 1]
+    const uint64_t *cand_off;   // [nq + 1]
+    uint32_t *post_cnt;         // [n_post + 1]
+    uint32_t *post_fill;        // [n_post]
+    const uint32_t *pair_off;   // [n_post + 1]
+    uint64_t *qp_slot;          // [nq * npr] slot of each (query, probe) pair
+    uint32_t *pair_q;           // [nq * npr] posting-major pairs
+    uint64_t *pair_slot;        // [nq * npr]
+    uint32_t *item_cnt;         // [n_post + 1]
+    uint32_t G;                 // queries per work item
+};
+
+struct RerankParams {
+    const uint64_t *sel;        // [nq][want] sorted pool keys
+    const uint32_t *sel_cnt;    // [nq]
+    uint32_t want, k, nq;
+    const float *raw;           // [n][dim] raw vectors by rank
+    const float *qv;            // [nq][dpad] f32 queries
+    const float *qnorm;
+    uint32_t dim, dpad, metric;
+    uint64_t *out;              // [nq][k]
+    uint32_t *out_cnt;
+};
+
+// kernels launched from host code (spf_kernels.hip)
+hipError_t launch_prep(const PrepParams &p, hipStream_t s);
+hipError_t launch_scan(uint32_t code_kind, uint32_t metric, bool dense, uint32_t G,
+                       const ScanParams &p, uint32_t grid, hipStream_t s);
+hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, uint64_t seg_stride,
+                         uint32_t nseg, uint32_t K, uint64_t *out, uint32_t *out_cnt,
+                         hipStream_t s);
+hipError_t launch_plan_probe(const PlanParams &p, hipStream_t s);
+hipError_t launch_plan_pairs(const PlanParams &p, hipStream_t s);
+hipError_t launch_scatter_pairs(const PlanParams &p, hipStream_t s);
+hipError_t launch_items(const PlanParams &p, hipStream_t s);
+hipError_t launch_excl_scan_u64(const uint64_t *in, uint64_t *out, uint32_t n, hipStream_t s);
+hipError_t launch_excl_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t s);
+hipError_t launch_rerank(const RerankParams &p, hipStream_t s);
+hipError_t launch_finalize(const uint64_t *keys, uint32_t stride, const uint32_t *cnt, uint32_t nq,
+                           uint32_t k,
+                           const uint64_t *rank_id, uint64_t *out_ids, float *out_dists,
+                           uint32_t *out_counts, hipStream_t s);
+hipError_t launch_entry_norms(uint32_t code_kind, const uint4 *data, uint32_t units,
+                              uint32_t n, uint32_t dim, float scale, float offset, float *xnorm,
+                              uint32_t *xsq, hipStream_t s);
+
+}  // namespace spf

@@ -1,0 +1,749 @@
+// gfx950 kernels of the SPFresh batch-search path.
+//
+// Stages (one batch of queries, all device-resident):
+//   k_prep      per-query values the metric consumes (f16 rounding, int8 codes, norms)
+//   k_scan      distances: dense mode = centroid probe (every query x every
+//               centroid, Inner::nearest src/spfresh.rs:358-372); listed mode =
+//               posting scan over work items (posting, <=G queries, <=1024
+//               entries) grouped by posting so one posting is streamed once
+//               for all queries that probe it (Inner::search loop
+//               src/spfresh.rs:703-717)
+//   k_select    per-segment top-K of 64-bit keys (fkey(dist) << 32 | rank):
+//               radix select (11/11/10-bit digits) + LDS bitonic sort
+//   k_plan_*    probe_ratio cut (src/spfresh.rs:683-687), candidate offsets,
+//               posting-major inversion of the (query, probe) pairs
+//   k_rerank    exact f32 distance of the pool (src/spfresh.rs:719-723)
+//   k_finalize  rank -> id, key -> distance
+//
+// Distances follow one canonical order shared with the CPU oracle: each f32
+// reduction is a sequential fused multiply-add chain from +0 over dimensions
+// 0..d-1; int8 L2 is an exact integer sum.  Build with -ffp-contract=off so no
+// other multiply-add is fused.
+#include "spf_internal.h"
+
+namespace spf {
+
+__device__ __forceinline__ uint32_t fkey(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_to_float(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+}
+
+// Angular distance from (a.b, |a|^2, |b|^2); zero-norm cases as the oracle.
+__device__ __forceinline__ float cosine_from(float ab, float a2, float b2) {
+    if (a2 == 0.0f && b2 == 0.0f) return 0.0f;
+    if (ab == 0.0f) return 1.0f;
+    float r = 1.0f - ab / sqrtf(a2 * b2);
+    return r > 0.0f ? r : 0.0f;
+}
+
+template <int MET>
+__device__ __forceinline__ float finish_f(float acc, float qn, float xn) {
+    if (MET == M_L2SQ) return acc;
+    if (MET == M_L2) return sqrtf(acc);
+    if (MET == M_DOT) return 1.0f - acc;
+    return cosine_from(acc, qn, xn);
+}
+
+template <int MET>
+__device__ __forceinline__ void step(float &acc, float q, float x) {
+    if (MET == M_L2 || MET == M_L2SQ) {
+        float t = q - x;
+        acc = __builtin_fmaf(t, t, acc);
+    } else {
+        acc = __builtin_fmaf(q, x, acc);
+    }
+}
+
+__device__ __forceinline__ uint32_t next_pow2(uint32_t v) {
+    uint32_t m = 1;
+    while (m < v) m <<= 1;
+    return m;
+}
+
+// ---------------------------------------------------------------- k_prep
+__global__ __launch_bounds__(64) void k_prep(const PrepParams p) {
+    extern __shared__ float s_v[];   // dpad floats
+    const uint32_t q = blockIdx.x, lane = threadIdx.x;
+    const float *src = p.q + (size_t)q * p.dim;
+    for (uint32_t i = lane; i < p.dpad; i += 64) {
+        float v = i < p.dim ? src[i] : 0.0f;
+        s_v[i] = v;
+        p.qv_probe[(size_t)q * p.dpad + i] = v;
+        if (p.round_f16) p.qv_scan[(size_t)q * p.dpad + i] = (float)(_Float16)v;
+    }
+    __syncthreads();
+    if (p.want_q8) {
+        const float alpha = 1.0f / p.scale;
+        const float beta = -p.offset / p.scale;
+        for (uint32_t w = lane; w < p.dpad / 4; w += 64) {
+            uint32_t word = 0;
+            for (uint32_t j = 0; j < 4; j++) {
+                uint32_t i = 4 * w + j;
+                uint32_t c = 0;
+                if (i < p.dim) {
+                    float s = s_v[i] * alpha;
+                    s = s + beta;
+                    if (s > 0.0f) c = s >= 255.0f ? 255u : (uint32_t)rintf(s);
+                }
+                word |= c << (8 * j);
+            }
+            p.q8[(size_t)q * (p.dpad / 4) + w] = word;
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        float acc = 0.0f;
+        for (uint32_t i = 0; i < p.dim; i++) acc = __builtin_fmaf(s_v[i], s_v[i], acc);
+        p.qnorm[q] = acc;
+        if (p.want_q8) {
+            uint32_t s = 0;
+            for (uint32_t w = 0; w < p.dpad / 4; w++) {
+                uint32_t word = p.q8[(size_t)q * (p.dpad / 4) + w];
+                for (uint32_t j = 0; j < 4; j++) {
+                    uint32_t c = (word >> (8 * j)) & 0xffu;
+                    s += c * c;
+                }
+            }
+            p.qsq[q] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- k_scan
+template <int CK, int MET, int G, bool DENSE>
+__global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
+    extern __shared__ __align__(16) float s_q[];   // G * dpad f32 (or packed int8 words)
+    __shared__ uint64_t s_slot[G];
+    __shared__ uint32_t s_qi[G];
+    __shared__ float s_qn[G];
+    __shared__ uint32_t s_qs[G];
+    constexpr bool INT_L2 = (CK == CK_U8) && (MET == M_L2 || MET == M_L2SQ);
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t dq4 = p.dpad >> 2;   // float4 (or u32 words) per staged query row
+    const uint32_t n_items = DENSE ? ((p.nq + G - 1) / G) * p.n_echunks : p.n_items;
+
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        uint32_t e_begin, e_end, gcnt, base_e;
+        if (DENSE) {
+            const uint32_t qc = item / p.n_echunks, ec = item - qc * p.n_echunks;
+            const uint32_t q0 = qc * G;
+            gcnt = min((uint32_t)G, p.nq - q0);
+            e_begin = ec * kEChunk;
+            e_end = min(e_begin + kEChunk, p.n_entries);
+            base_e = 0;
+            if (tid < gcnt) {
+                s_qi[tid] = q0 + tid;
+                s_slot[tid] = (uint64_t)(q0 + tid) * p.out_stride;
+            }
+        } else {
+            uint32_t lo = 0, hi = p.n_post;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (p.item_off[mid] <= item) lo = mid; else hi = mid;
+            }
+            const uint32_t pp = lo;
+            const uint32_t local = item - p.item_off[pp];
+            const uint32_t pb = p.post_off[pp], pe = p.post_off[pp + 1];
+            const uint32_t nec = (pe - pb + kEChunk - 1) / kEChunk;
+            const uint32_t qc = local / nec, ec = local - qc * nec;
+            const uint32_t pr0 = p.pair_off[pp] + qc * G;
+            gcnt = min((uint32_t)G, p.pair_off[pp + 1] - pr0);
+            e_begin = pb + ec * kEChunk;
+            e_end = min(e_begin + kEChunk, pe);
+            base_e = pb;
+            if (tid < gcnt) {
+                s_qi[tid] = p.pair_q[pr0 + tid];
+                s_slot[tid] = p.pair_slot[pr0 + tid];
+            }
+        }
+        __syncthreads();
+        if (INT_L2) {
+            uint32_t *sw = reinterpret_cast<uint32_t *>(s_q);
+            for (uint32_t i = tid; i < gcnt * dq4; i += kWG) {
+                const uint32_t g = i / dq4;
+                sw[i] = p.q8[(size_t)s_qi[g] * dq4 + (i - g * dq4)];
+            }
+            if (tid < gcnt) s_qs[tid] = p.qsq[s_qi[tid]];
+        } else {
+            float4 *s4 = reinterpret_cast<float4 *>(s_q);
+            const float4 *q4 = reinterpret_cast<const float4 *>(p.qv);
+            for (uint32_t i = tid; i < gcnt * dq4; i += kWG) {
+                const uint32_t g = i / dq4;
+                s4[i] = q4[(size_t)s_qi[g] * dq4 + (i - g * dq4)];
+            }
+            if (MET == M_COS && tid < gcnt) s_qn[tid] = p.qnorm[s_qi[tid]];
+        }
+        __syncthreads();
+
+        for (uint32_t e0 = e_begin + wave * 64; e0 < e_end; e0 += kWG) {
+            const uint32_t e = e0 + lane;
+            const bool valid = e < e_end;
+            const uint4 *src = p.data + (size_t)(e >> 6) * p.units * 64 + (e & 63);
+            float accf[G];
+            uint32_t acci[G];
+#pragma unroll
+            for (int g = 0; g < G; g++) { accf[g] = 0.0f; acci[g] = 0u; }
+
+            if (CK == CK_F32) {
+                const float4 *s4 = reinterpret_cast<const float4 *>(s_q);
+#pragma unroll 2
+                for (uint32_t u = 0; u < p.units; u++) {
+                    const uint4 w = valid ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
+                    const float x0 = __uint_as_float(w.x), x1 = __uint_as_float(w.y);
+                    const float x2 = __uint_as_float(w.z), x3 = __uint_as_float(w.w);
+#pragma unroll
+                    for (int g = 0; g < G; g++) {
+                        const float4 qq = s4[g * dq4 + u];
+                        step<MET>(accf[g], qq.x, x0);
+                        step<MET>(accf[g], qq.y, x1);
+                        step<MET>(accf[g], qq.z, x2);
+                        step<MET>(accf[g], qq.w, x3);
+                    }
+                }
+            } else if (CK == CK_F16) {
+                const float4 *s4 = reinterpret_cast<const float4 *>(s_q);
+#pragma unroll 2
+                for (uint32_t u = 0; u < p.units; u++) {
+                    union { uint4 v; _Float16 h[8]; } cv;
+                    cv.v = valid ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
+                    float x[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) x[j] = (float)cv.h[j];
+#pragma unroll
+                    for (int g = 0; g < G; g++) {
+                        const float4 qa = s4[g * dq4 + 2 * u];
+                        const float4 qb = s4[g * dq4 + 2 * u + 1];
+                        step<MET>(accf[g], qa.x, x[0]);
+                        step<MET>(accf[g], qa.y, x[1]);
+                        step<MET>(accf[g], qa.z, x[2]);
+                        step<MET>(accf[g], qa.w, x[3]);
+                        step<MET>(accf[g], qb.x, x[4]);
+                        step<MET>(accf[g], qb.y, x[5]);
+                        step<MET>(accf[g], qb.z, x[6]);
+                        step<MET>(accf[g], qb.w, x[7]);
+                    }
+                }
+            } else if (INT_L2) {
+                const uint32_t *sw = reinterpret_cast<const uint32_t *>(s_q);
+#pragma unroll 2
+                for (uint32_t u = 0; u < p.units; u++) {
+                    const uint4 w = valid ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+                    for (int g = 0; g < G; g++) {
+                        const uint4 qq = *reinterpret_cast<const uint4 *>(sw + g * dq4 + 4 * u);
+                        uint32_t a = acci[g];
+                        a = __builtin_amdgcn_udot4(w.x, qq.x, a, false);
+                        a = __builtin_amdgcn_udot4(w.y, qq.y, a, false);
+                        a = __builtin_amdgcn_udot4(w.z, qq.z, a, false);
+                        a = __builtin_amdgcn_udot4(w.w, qq.w, a, false);
+                        acci[g] = a;
+                    }
+                }
+            } else {   // CK_U8 decode path (cosine / dot), src/quantized.rs:28-36
+                const float4 *s4 = reinterpret_cast<const float4 *>(s_q);
+                const float sc = p.scale, of = p.offset;
+#pragma unroll 1
+                for (uint32_t u = 0; u < p.units; u++) {
+                    const uint4 w = valid ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
+                    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+                    float x[16];
+#pragma unroll
+                    for (int j = 0; j < 16; j++) {
+                        const float c = (float)((ws[j >> 2] >> (8 * (j & 3))) & 0xffu);
+                        const float t = c * sc;
+                        x[j] = t + of;
+                    }
+#pragma unroll
+                    for (int g = 0; g < G; g++) {
+#pragma unroll
+                        for (int j4 = 0; j4 < 4; j4++) {
+                            const float4 qq = s4[g * dq4 + 4 * u + j4];
+                            step<MET>(accf[g], qq.x, x[4 * j4 + 0]);
+                            step<MET>(accf[g], qq.y, x[4 * j4 + 1]);
+                            step<MET>(accf[g], qq.z, x[4 * j4 + 2]);
+                            step<MET>(accf[g], qq.w, x[4 * j4 + 3]);
+                        }
+                    }
+                }
+            }
+
+            if (valid) {
+                const uint64_t low = p.low ? (uint64_t)p.low[e] : (uint64_t)e;
+                const bool dead = p.del && ((p.del[e >> 5] >> (e & 31)) & 1u);
+                const float xn = (MET == M_COS) ? p.xnorm[e] : 0.0f;
+                const uint32_t xs = INT_L2 ? p.xsq[e] : 0u;
+#pragma unroll
+                for (int g = 0; g < G; g++) {
+                    if (g < (int)gcnt) {
+                        float dist;
+                        if (INT_L2) {
+                            const uint32_t s = xs + s_qs[g] - 2u * acci[g];
+                            float v = (float)s;
+                            v = v * p.scale;
+                            v = v * p.scale;
+                            dist = (MET == M_L2) ? sqrtf(v) : v;
+                        } else {
+                            dist = finish_f<MET>(accf[g], MET == M_COS ? s_qn[g] : 0.0f, xn);
+                        }
+                        const uint64_t key = dead ? kNoKey : (((uint64_t)fkey(dist) << 32) | low);
+                        p.out[s_slot[g] + (e - base_e)] = key;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- selection
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t *part) {
+    const uint32_t tid = threadIdx.x;
+    part[tid] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < kWG; off <<= 1) {
+        const uint32_t t = tid >= off ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += t;
+        __syncthreads();
+    }
+    const uint32_t incl = part[tid];
+    __syncthreads();
+    return incl - v;
+}
+
+__device__ void bitonic_sort_lds(uint64_t *buf, uint32_t m) {
+    for (uint32_t k = 2; k <= m; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < (m >> 1); i += kWG) {
+                const uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                const uint32_t hi = lo + j;
+                const uint64_t a = buf[lo], b = buf[hi];
+                const bool up = (lo & k) == 0;
+                if ((a > b) == up) {
+                    buf[lo] = b;
+                    buf[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+}
+
+constexpr uint32_t kSelDirect = 1024;   // segments this short are sorted outright
+constexpr uint32_t kSelSlack = 256;     // extra keys tolerated in the final sort
+
+__global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint64_t *seg_off,
+                                                uint64_t seg_stride, uint32_t K, uint64_t *out,
+                                                uint32_t *out_cnt) {
+    __shared__ uint64_t buf[kSelCap];
+    __shared__ uint32_t hist[2048];
+    __shared__ uint32_t part[kWG];
+    __shared__ uint32_t s_u[4];
+    const uint32_t seg = blockIdx.x, tid = threadIdx.x;
+    const uint64_t b = seg_off ? seg_off[seg] : (uint64_t)seg * seg_stride;
+    const uint64_t e = seg_off ? seg_off[seg + 1] : b + seg_stride;
+    const uint64_t n = e - b;
+    const uint64_t *kp = keys + b;
+    uint64_t *o = out + (size_t)seg * K;
+
+    if (tid == 0) s_u[0] = 0;
+    __syncthreads();
+    uint32_t inval = 0;
+    for (uint64_t i = tid; i < n; i += kWG) inval += kp[i] == kNoKey;
+    if (inval) atomicAdd(&s_u[0], inval);
+    __syncthreads();
+    const uint64_t nvalid = n - s_u[0];
+    const uint32_t Kv = (uint32_t)min((uint64_t)K, nvalid);
+    __syncthreads();
+    if (Kv == 0) {
+        if (tid == 0) out_cnt[seg] = 0;
+        return;
+    }
+    uint32_t m;
+    if (n <= kSelDirect) {
+        for (uint32_t i = tid; i < n; i += kWG) buf[i] = kp[i];
+        m = (uint32_t)n;
+    } else {
+        const int shifts[6] = {53, 42, 32, 21, 10, 0};
+        const int widths[6] = {11, 11, 10, 11, 11, 10};
+        uint64_t prefix = 0, mask = 0;
+        uint32_t rem = Kv;
+        for (int pass = 0; pass < 6; pass++) {
+            const int sh = shifts[pass];
+            const uint32_t dm = (1u << widths[pass]) - 1u;
+            for (uint32_t i = tid; i < 2048; i += kWG) hist[i] = 0;
+            __syncthreads();
+            for (uint64_t i = tid; i < n; i += kWG) {
+                const uint64_t k = kp[i];
+                if ((k & mask) == prefix) atomicAdd(&hist[(uint32_t)(k >> sh) & dm], 1u);
+            }
+            __syncthreads();
+            uint32_t loc[8], s = 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                loc[j] = hist[tid * 8 + j];
+                s += loc[j];
+            }
+            const uint32_t excl = block_excl_scan256(s, part);
+            if (excl < rem && rem <= excl + s) {
+                uint32_t c = excl;
+                for (int j = 0; j < 8; j++) {
+                    if (rem <= c + loc[j]) {
+                        s_u[1] = tid * 8 + j;
+                        s_u[2] = c;
+                        s_u[3] = loc[j];
+                        break;
+                    }
+                    c += loc[j];
+                }
+            }
+            __syncthreads();
+            const uint32_t bin = s_u[1], below = s_u[2], cnt = s_u[3];
+            __syncthreads();
+            rem -= below;
+            prefix |= (uint64_t)bin << sh;
+            mask |= (uint64_t)dm << sh;
+            if ((Kv - rem) + cnt <= Kv + kSelSlack) break;
+        }
+        if (tid == 0) s_u[0] = 0;
+        __syncthreads();
+        for (uint64_t i = tid; i < n; i += kWG) {
+            const uint64_t k = kp[i];
+            if ((k & mask) <= prefix) {
+                const uint32_t pos = atomicAdd(&s_u[0], 1u);
+                if (pos < kSelCap) buf[pos] = k;
+            }
+        }
+        __syncthreads();
+        m = min(s_u[0], kSelCap);
+    }
+    const uint32_t m2 = next_pow2(m < 2 ? 2 : m);
+    for (uint32_t i = m + tid; i < m2; i += kWG) buf[i] = kNoKey;
+    __syncthreads();
+    bitonic_sort_lds(buf, m2);
+    for (uint32_t i = tid; i < Kv; i += kWG) o[i] = buf[i];
+    if (tid == 0) out_cnt[seg] = Kv;
+}
+
+// ---------------------------------------------------------------- planning
+// probe_ratio cut + per-query candidate count (one thread per query)
+__global__ void k_plan_probe(const PlanParams p) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= p.nq) return;
+    const uint64_t *sel = p.probe_sel + (size_t)q * p.npr;
+    const uint32_t c = p.probe_cnt[q];
+    uint32_t kept = 0;
+    uint64_t total = 0;
+    if (c > 0) {
+        const float d0 = key_to_float((uint32_t)(sel[0] >> 32));
+        const float ratio = p.probe_ratio;
+        const bool cut = isfinite(ratio) && d0 >= 0.0f;
+        const float lim = d0 * fmaxf(ratio, 1.0f);
+        for (uint32_t j = 0; j < c; j++) {
+            const float d = key_to_float((uint32_t)(sel[j] >> 32));
+            if (cut && d > lim) break;
+            const uint32_t post = (uint32_t)sel[j];
+            p.probe_post[(size_t)q * p.npr + kept] = post;
+            total += p.post_len[post];
+            kept++;
+        }
+    }
+    p.np_q[q] = kept;
+    p.cand_len[q] = total;
+}
+
+// (query, probe) pairs: candidate-buffer slot of each probed posting, and a
+// count of pairs per posting (one thread per query)
+__global__ void k_plan_pairs(const PlanParams p) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= p.nq) return;
+    uint64_t slot = p.cand_off[q];
+    const uint32_t kept = p.np_q[q];
+    for (uint32_t j = 0; j < kept; j++) {
+        const uint32_t post = p.probe_post[(size_t)q * p.npr + j];
+        p.qp_slot[(size_t)q * p.npr + j] = slot;
+        slot += p.post_len[post];
+        atomicAdd(&p.post_cnt[post], 1u);
+    }
+}
+
+// posting-major scatter of the pairs (one thread per (query, probe))
+__global__ void k_scatter_pairs(const PlanParams p) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.nq * p.npr) return;
+    const uint32_t q = i / p.npr, j = i - q * p.npr;
+    if (j >= p.np_q[q]) return;
+    const uint32_t post = p.probe_post[i];
+    const uint32_t pos = p.pair_off[post] + atomicAdd(&p.post_fill[post], 1u);
+    p.pair_q[pos] = q;
+    p.pair_slot[pos] = p.qp_slot[i];
+}
+
+__global__ void k_items(const PlanParams p) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= p.n_post) return;
+    const uint32_t cnt = p.pair_off[b + 1] - p.pair_off[b];
+    const uint32_t len = p.post_len[b];
+    p.item_cnt[b] = (cnt && len) ? ((cnt + p.G - 1) / p.G) * ((len + kEChunk - 1) / kEChunk) : 0u;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_excl_scan(const T *in, T *out, uint32_t n) {
+    __shared__ T s_w[16];
+    __shared__ T s_carry;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += 1024) {
+        const uint32_t i = base + tid;
+        const T v = i < n ? in[i] : (T)0;
+        T x = v;
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const T y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) s_w[wave] = x;
+        __syncthreads();
+        if (tid == 0) {
+            T c = 0;
+            for (int w = 0; w < 16; w++) {
+                const T t = s_w[w];
+                s_w[w] = c;
+                c += t;
+            }
+        }
+        __syncthreads();
+        const T carry = s_carry;
+        if (i < n) out[i] = carry + s_w[wave] + x - v;
+        __syncthreads();
+        if (tid == 1023) s_carry = carry + s_w[wave] + x;
+        __syncthreads();
+    }
+    if (tid == 0) out[n] = s_carry;
+}
+
+// ---------------------------------------------------------------- rerank
+template <int MET>
+__device__ float exact_row(const float *q, const float *x, uint32_t dpad, float qn) {
+    const float4 *q4 = reinterpret_cast<const float4 *>(q);
+    const float4 *x4 = reinterpret_cast<const float4 *>(x);
+    float acc = 0.0f, b2 = 0.0f;
+    for (uint32_t u = 0; u < dpad / 4; u++) {
+        const float4 a = q4[u], b = x4[u];
+        step<MET>(acc, a.x, b.x);
+        step<MET>(acc, a.y, b.y);
+        step<MET>(acc, a.z, b.z);
+        step<MET>(acc, a.w, b.w);
+        if (MET == M_COS) {
+            b2 = __builtin_fmaf(b.x, b.x, b2);
+            b2 = __builtin_fmaf(b.y, b.y, b2);
+            b2 = __builtin_fmaf(b.z, b.z, b2);
+            b2 = __builtin_fmaf(b.w, b.w, b2);
+        }
+    }
+    return finish_f<MET>(acc, qn, b2);
+}
+
+template <int MET>
+__global__ __launch_bounds__(256) void k_rerank(const RerankParams p) {
+    __shared__ uint64_t buf[kMaxK];
+    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    const uint32_t c = p.sel_cnt[q];
+    const uint64_t *sel = p.sel + (size_t)q * p.want;
+    const float *qv = p.qv + (size_t)q * p.dpad;
+    const float qn = p.qnorm[q];
+    for (uint32_t i = tid; i < c; i += kWG) {
+        const uint32_t rank = (uint32_t)sel[i];
+        const float d = exact_row<MET>(qv, p.raw + (size_t)rank * p.dpad, p.dpad, qn);
+        buf[i] = ((uint64_t)fkey(d) << 32) | rank;
+    }
+    const uint32_t m2 = next_pow2(c < 2 ? 2 : c);
+    for (uint32_t i = c + tid; i < m2; i += kWG) buf[i] = kNoKey;
+    __syncthreads();
+    bitonic_sort_lds(buf, m2);
+    const uint32_t kk = min(p.k, c);
+    for (uint32_t i = tid; i < kk; i += kWG) p.out[(size_t)q * p.k + i] = buf[i];
+    if (tid == 0) p.out_cnt[q] = kk;
+}
+
+__global__ void k_finalize(const uint64_t *keys, uint32_t stride, const uint32_t *cnt,
+                           uint32_t nq, uint32_t k, const uint64_t *rank_id, uint64_t *out_ids,
+                           float *out_dists, uint32_t *out_counts) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)nq * k) return;
+    const uint32_t q = (uint32_t)(i / k), j = (uint32_t)(i - (uint64_t)q * k);
+    const uint32_t c = min(cnt[q], k);
+    if (j < c) {
+        const uint64_t key = keys[(size_t)q * stride + j];
+        out_ids[i] = rank_id[(uint32_t)key];
+        out_dists[i] = key_to_float((uint32_t)(key >> 32));
+    } else {
+        out_ids[i] = ~0ull;
+        out_dists[i] = __builtin_inff();
+    }
+    if (j == 0) out_counts[q] = c;
+}
+
+// |x|^2 of every stored row in its decoded form (canonical order), and the
+// exact sum of squared u8 codes
+__global__ void k_entry_norms(uint32_t ck, const uint4 *data, uint32_t units, uint32_t n,
+                              uint32_t dim, float scale, float offset, float *xnorm,
+                              uint32_t *xsq) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint4 *src = data + (size_t)(e >> 6) * units * 64 + (e & 63);
+    float acc = 0.0f;
+    uint32_t si = 0, d = 0;
+    for (uint32_t u = 0; u < units; u++) {
+        const uint4 w = src[(size_t)u * 64];
+        if (ck == CK_F32) {
+            const float x[4] = {__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z),
+                                __uint_as_float(w.w)};
+            for (int j = 0; j < 4 && d < dim; j++, d++) acc = __builtin_fmaf(x[j], x[j], acc);
+        } else if (ck == CK_F16) {
+            union { uint4 v; _Float16 h[8]; } cv;
+            cv.v = w;
+            for (int j = 0; j < 8 && d < dim; j++, d++) {
+                const float x = (float)cv.h[j];
+                acc = __builtin_fmaf(x, x, acc);
+            }
+        } else {
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+            for (int j = 0; j < 16 && d < dim; j++, d++) {
+                const uint32_t c = (ws[j >> 2] >> (8 * (j & 3))) & 0xffu;
+                si += c * c;
+                float x = (float)c * scale;
+                x = x + offset;
+                acc = __builtin_fmaf(x, x, acc);
+            }
+        }
+    }
+    if (xnorm) xnorm[e] = acc;
+    if (xsq) xsq[e] = si;
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_prep(const PrepParams &p, hipStream_t s) {
+    if (p.nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_prep, dim3(p.nq), dim3(64), p.dpad * sizeof(float), s, p);
+    return hipGetLastError();
+}
+
+template <int CK, int MET, int G>
+static hipError_t scan_mode(bool dense, const ScanParams &p, uint32_t grid, hipStream_t s) {
+    const size_t lds = (size_t)G * p.dpad * sizeof(float);
+    if (dense)
+        hipLaunchKernelGGL((k_scan<CK, MET, G, true>), dim3(grid), dim3(kWG), lds, s, p);
+    else
+        hipLaunchKernelGGL((k_scan<CK, MET, G, false>), dim3(grid), dim3(kWG), lds, s, p);
+    return hipGetLastError();
+}
+
+template <int CK, int G>
+static hipError_t scan_metric(uint32_t metric, bool dense, const ScanParams &p, uint32_t grid,
+                              hipStream_t s) {
+    switch (metric) {
+    case M_L2: return scan_mode<CK, M_L2, G>(dense, p, grid, s);
+    case M_L2SQ: return scan_mode<CK, M_L2SQ, G>(dense, p, grid, s);
+    case M_COS: return scan_mode<CK, M_COS, G>(dense, p, grid, s);
+    case M_DOT: return scan_mode<CK, M_DOT, G>(dense, p, grid, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int G>
+static hipError_t scan_kind(uint32_t ck, uint32_t metric, bool dense, const ScanParams &p,
+                            uint32_t grid, hipStream_t s) {
+    switch (ck) {
+    case CK_F32: return scan_metric<CK_F32, G>(metric, dense, p, grid, s);
+    case CK_F16: return scan_metric<CK_F16, G>(metric, dense, p, grid, s);
+    case CK_U8: return scan_metric<CK_U8, G>(metric, dense, p, grid, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_scan(uint32_t ck, uint32_t metric, bool dense, uint32_t G, const ScanParams &p,
+                       uint32_t grid, hipStream_t s) {
+    if (grid == 0) return hipSuccess;
+    if (G == 16) return scan_kind<16>(ck, metric, dense, p, grid, s);
+    if (G == 4) return scan_kind<4>(ck, metric, dense, p, grid, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, uint64_t seg_stride,
+                         uint32_t nseg, uint32_t K, uint64_t *out, uint32_t *out_cnt,
+                         hipStream_t s) {
+    if (nseg == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_select, dim3(nseg), dim3(kWG), 0, s, keys, seg_off, seg_stride, K, out,
+                       out_cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_probe(const PlanParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_plan_probe, dim3((p.nq + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_pairs(const PlanParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_plan_pairs, dim3((p.nq + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_pairs(const PlanParams &p, hipStream_t s) {
+    const uint32_t n = p.nq * p.npr;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_pairs, dim3((n + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_items(const PlanParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_items, dim3((p.n_post + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_excl_scan_u64(const uint64_t *in, uint64_t *out, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_excl_scan<unsigned long long>, dim3(1), dim3(1024), 0, s,
+                       (const unsigned long long *)in, (unsigned long long *)out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_excl_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_excl_scan<uint32_t>, dim3(1), dim3(1024), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rerank(const RerankParams &p, hipStream_t s) {
+    if (p.nq == 0) return hipSuccess;
+    switch (p.metric) {
+    case M_L2: hipLaunchKernelGGL(k_rerank<M_L2>, dim3(p.nq), dim3(kWG), 0, s, p); break;
+    case M_L2SQ: hipLaunchKernelGGL(k_rerank<M_L2SQ>, dim3(p.nq), dim3(kWG), 0, s, p); break;
+    case M_COS: hipLaunchKernelGGL(k_rerank<M_COS>, dim3(p.nq), dim3(kWG), 0, s, p); break;
+    case M_DOT: hipLaunchKernelGGL(k_rerank<M_DOT>, dim3(p.nq), dim3(kWG), 0, s, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const uint64_t *keys, uint32_t stride, const uint32_t *cnt, uint32_t nq,
+                           uint32_t k, const uint64_t *rank_id, uint64_t *out_ids,
+                           float *out_dists, uint32_t *out_counts, hipStream_t s) {
+    const uint64_t n = (uint64_t)nq * k;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finalize, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, keys,
+                       stride, cnt, nq, k, rank_id, out_ids, out_dists, out_counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_entry_norms(uint32_t ck, const uint4 *data, uint32_t units, uint32_t n,
+                              uint32_t dim, float scale, float offset, float *xnorm,
+                              uint32_t *xsq, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_entry_norms, dim3((n + 255) / 256), dim3(256), 0, s, ck, data, units, n,
+                       dim, scale, offset, xnorm, xsq);
+    return hipGetLastError();
+}
+
+}  // namespace spf
