@@ -166,9 +166,12 @@ def kmeans(x: np.ndarray, k: int, iters: int = 10, seed: int = 0, sample: int = 
     c = xs[rng.choice(xs.shape[0], size=k, replace=False)].copy()
     for _ in range(iters):
         a = assign_nearest(xs, c, device=device)
-        sums = np.zeros_like(c, dtype=np.float64)
-        np.add.at(sums, a, xs)
+        order = np.argsort(a, kind="stable")
         cnt = np.bincount(a, minlength=k)
+        sums = np.zeros((k, x.shape[1]), dtype=np.float64)
+        nz = np.nonzero(cnt)[0]
+        starts = np.concatenate([[0], np.cumsum(cnt)[:-1]])[nz]
+        sums[nz] = np.add.reduceat(xs[order].astype(np.float64), starts, axis=0)
         empty = cnt == 0
         c = (sums / np.maximum(cnt, 1)[:, None]).astype(np.float32)
         if empty.any():
