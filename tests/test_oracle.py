@@ -1,0 +1,385 @@
+"""Pins the CPU oracle (CPU only).
+
+There are no golden vectors in the reference; each oracle function is pinned
+here by closed forms, library routines, the reference's own test invariants
+(cited per test) and brute force, chosen so that a plausible mistake fails one
+of them.  ``test_planted_mistakes`` turns the oracle's fault switch on and shows
+each such mistake is caught.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle
+from pydiskann_amd import build_index, brute_force_f64, codecs
+
+DIM = 32
+
+
+def clustered(n, seed, spread=0.12, n_centers=25, dim=DIM):
+    """Points around shared centres (python/tests/test_spfresh.py:25-31)."""
+    crng = np.random.default_rng(12345)
+    centers = crng.uniform(-1, 1, size=(n_centers, dim))
+    rng = np.random.default_rng(seed)
+    return (centers[np.arange(n) % n_centers] + rng.normal(0, spread, (n, dim))).astype(np.float32)
+
+
+def uniform(n, seed, dim=DIM):
+    return np.random.default_rng(seed).uniform(-1, 1, (n, dim)).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def built():
+    """2000 x 32 clustered, max_posting 64 (python/tests/test_spfresh.py:65-69)."""
+    vs = clustered(2000, seed=1)
+    ix = build_index(vs, "l2", None, max_posting_size=64, seed=0)
+    return ix, vs
+
+
+# ------------------------------------------------------------------ scalars
+def test_f16_conversion_is_ieee_round_to_nearest_even():
+    allh = np.arange(65536, dtype=np.uint32).astype(np.uint16)
+    ref = allh.view(np.float16).astype(np.float32)
+    got = np.array([oracle.f16_to_f32(int(h)) for h in allh], dtype=np.float32)
+    same = (got.view(np.uint32) == ref.view(np.uint32)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all()
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([
+        rng.normal(0, 1000, 3000), rng.normal(0, 1e-5, 2000), rng.uniform(-70000, 70000, 500),
+        # exact half-way cases between neighbouring halves (spacing 1 in [1024, 2048))
+        np.float32(1024.5) + np.arange(0, 1000, dtype=np.float32),
+        [0.0, -0.0, 65504.0, 65519.0, 65520.0, 5.96e-8, 2.98e-8, 1e-8, np.inf, -np.inf],
+    ]).astype(np.float32)
+    with np.errstate(over="ignore"):
+        ref16 = xs.astype(np.float16).view(np.uint16)
+    got16 = np.array([oracle.f32_to_f16(float(x)) for x in xs], dtype=np.uint16)
+    np.testing.assert_array_equal(got16, ref16)
+    # the package's encoder is numpy's conversion; both must agree
+    with np.errstate(over="ignore"):
+        np.testing.assert_array_equal(codecs.f16_encode(xs[None, :]).view(np.uint16)[0], ref16)
+
+
+def test_l2sq_closed_form_on_integer_data():
+    """Integer data below 2^24: every f32 partial sum is exact, so the L2 sum
+    equals the int64 closed form for any order (the 1M x 128 descriptor shape)."""
+    rng = np.random.default_rng(1)
+    for d in (1, 7, 32, 128, 129):
+        a = rng.integers(0, 256, d).astype(np.float32)
+        b = rng.integers(0, 256, d).astype(np.float32)
+        exact = int(((a.astype(np.int64) - b.astype(np.int64)) ** 2).sum())
+        assert oracle.eval_metric(1, a, b) == float(exact)
+        assert oracle.eval_metric(0, a, b) == np.float32(math.sqrt(exact))
+        dot = int((a.astype(np.int64) * b.astype(np.int64)).sum())
+        if dot < 2 ** 24:
+            assert oracle.eval_metric(3, a, b) == np.float32(1.0) - np.float32(dot)
+
+
+def test_metrics_against_float64():
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        d = int(rng.integers(1, 300))
+        a = rng.normal(size=d).astype(np.float32)
+        b = rng.normal(size=d).astype(np.float32)
+        a64, b64 = a.astype(np.float64), b.astype(np.float64)
+        l2 = ((a64 - b64) ** 2).sum()
+        assert oracle.eval_metric(1, a, b) == pytest.approx(l2, rel=1e-5)
+        assert oracle.eval_metric(0, a, b) == pytest.approx(math.sqrt(l2), rel=1e-5)
+        assert oracle.eval_metric(3, a, b) == pytest.approx(1 - a64 @ b64, abs=1e-4 * d)
+        cos = 1 - a64 @ b64 / np.sqrt((a64 @ a64) * (b64 @ b64))
+        assert oracle.eval_metric(2, a, b) == pytest.approx(max(cos, 0.0), abs=1e-5)
+
+
+def test_cosine_special_cases():
+    z = np.zeros(8, np.float32)
+    e0 = np.eye(8, dtype=np.float32)[0]
+    e1 = np.eye(8, dtype=np.float32)[1]
+    assert oracle.eval_metric(2, z, z) == 0.0
+    assert oracle.eval_metric(2, e0, e1) == 1.0
+    assert oracle.eval_metric(2, e0, z) == 1.0
+    assert oracle.eval_metric(2, e0, 3 * e0) == 0.0
+    assert oracle.eval_metric(2, e0, -e0) == 2.0
+
+
+def test_fht_is_the_hadamard_matrix():
+    rng = np.random.default_rng(3)
+    for n in (64, 128, 1024):
+        v = rng.normal(size=n).astype(np.float32)
+        w = v.copy()
+        oracle.lib().spf_o_fht(w.ctypes.data_as(oracle.C.c_void_p), n)
+        H = scipy.linalg.hadamard(n).astype(np.float64)
+        np.testing.assert_allclose(w, H @ v.astype(np.float64), rtol=1e-5, atol=1e-4)
+        # the vectorised encoder's transform is element-for-element the same
+        np.testing.assert_array_equal(codecs.fht_rows(v[None, :].copy())[0], w)
+
+
+# ------------------------------------------------------------------ int8 codec
+def test_int8_codec_round_trip_and_asymmetric_distance():
+    """src/sq.rs:374-419: max error < 0.1 on data in [-5, 5); encoded-query
+    distance within 1% of the decoded distance."""
+    rng = np.random.default_rng(42)
+    vs = rng.uniform(-5, 5, (500, 32)).astype(np.float32)
+    scale, offset = codecs.int8_train(vs)
+    assert scale == pytest.approx((vs.max() - vs.min()) / 255, rel=1e-6) and offset == vs.min()
+    codes = codecs.int8_encode(vs, scale, offset)
+    for i in range(0, 500, 25):
+        np.testing.assert_array_equal(codes[i], oracle.int8_encode(vs[i], scale, offset))
+    dec = codes.astype(np.float32) * np.float32(scale) + np.float32(offset)
+    assert np.abs(dec - vs).max() < 0.1
+    qc = oracle.int8_encode(vs[0], scale, offset)
+    asym = oracle.int8_l2sq(qc, codes[100], scale)
+    exp = float(((vs[0].astype(np.float64) - dec[100]) ** 2).sum())
+    assert abs(asym - exp) < 0.01 * exp
+    # closed form of the integer path: (sum of squared code differences) * s * s
+    s = int(((qc.astype(np.int64) - codes[100].astype(np.int64)) ** 2).sum())
+    assert asym == np.float32(np.float32(s) * np.float32(scale)) * np.float32(scale)
+
+
+def test_int8_saturation():
+    scale, offset = 0.1, -1.0
+    v = np.array([-5, -1, -0.96, 0, 24.5, 24.6, 100, np.nan], np.float32)
+    c = oracle.int8_encode(v, scale, offset)
+    assert c[0] == 0 and c[1] == 0 and c[-2] == 255 and c[-1] == 0
+    np.testing.assert_array_equal(c, codecs.int8_encode(v[None, :], scale, offset)[0])
+
+
+# ------------------------------------------------------------------ RaBitQ
+def _rq(vs):
+    padded = codecs.rabitq_padded(vs.shape[1])
+    return padded, codecs.rabitq_signs(padded)
+
+
+def test_rabitq_rotation_is_orthogonal():
+    """src/rabitq.rs:292-308."""
+    vs = np.random.default_rng(1).uniform(-1, 1, (4, 100)).astype(np.float32)
+    padded, signs = _rq(vs)
+    assert padded == 128
+    v = np.zeros(128, np.float32)
+    v[:100] = vs[0]
+    r = oracle.rotate(v, padded, signs)
+    assert abs((r.astype(np.float64) ** 2).sum() - (v.astype(np.float64) ** 2).sum()) < 1e-3 * (v ** 2).sum()
+    back = oracle.rotate(r, padded, signs, inverse=True)
+    np.testing.assert_allclose(back, v, atol=1e-4)
+
+
+def test_rabitq_estimator_bounds_and_codes():
+    """src/rabitq.rs:310-328: code_size 24 at dim 128, self-distance near 0,
+    estimate within 0.35*exact + 0.02*scale; numpy encoder == oracle encoder."""
+    vs = np.random.default_rng(2).uniform(-1, 1, (200, 128)).astype(np.float32)
+    padded, signs = _rq(vs)
+    cent = vs.mean(0).astype(np.float32)
+    assert codecs.rabitq_code_bytes(padded) == 24
+    codes = codecs.rabitq_encode(vs, cent, padded, signs)
+    for i in range(0, 200, 20):
+        np.testing.assert_array_equal(codes[i], oracle.rabitq_encode(vs[i], cent, padded, signs))
+    planes, consts = oracle.rabitq_query(vs[0], cent, padded, signs)
+    scale = float(((vs[0] - vs[1]) ** 2).sum())
+    est0, _ = oracle.rabitq_dist(planes, consts, codes[0], padded)
+    assert abs(est0) < 0.05 * scale
+    for i in range(20):
+        est, err = oracle.rabitq_dist(planes, consts, codes[i], padded)
+        exact = float(((vs[0] - vs[i]) ** 2).sum())
+        assert abs(est - exact) < 0.35 * exact + 0.02 * scale
+        assert err > 0
+
+
+def test_rabitq_decode_and_ordering():
+    """src/rabitq.rs:330-358: decode is closer than a neighbour; the exact
+    top-10 lands in the estimated top-30.  The reference asserts >= 7 hits for
+    one query of its own draw; over draws the mean is about 7.1, so this asserts
+    the mean over 10 queries (a wrong sign or dropped term gives ~1-3)."""
+    vs = np.random.default_rng(3).uniform(-1, 1, (500, 64)).astype(np.float32)
+    padded, signs = _rq(vs)
+    cent = vs.mean(0).astype(np.float32)
+    codes = codecs.rabitq_encode(vs, cent, padded, signs)
+    dec = oracle.rabitq_decode(codes[0], cent, 64, padded, signs)
+    assert ((vs[0] - dec) ** 2).sum() < ((vs[0] - vs[1]) ** 2).sum()
+    hits = []
+    for qi in range(10):
+        exact = ((vs - vs[qi]) ** 2).sum(1)
+        exact[qi] = np.inf
+        planes, consts = oracle.rabitq_query(vs[qi], cent, padded, signs)
+        est = np.array([oracle.rabitq_dist(planes, consts, c, padded)[0] for c in codes])
+        est[qi] = np.inf
+        top = set(np.argsort(exact)[:10].tolist())
+        hits.append(len(top & set(np.argsort(est, kind="stable")[:30].tolist())))
+    assert np.mean(hits) >= 6.5, hits
+
+
+def test_rabitq_error_bound_holds_mostly():
+    """Thm 3.2 with eps0 = 1.9 (src/rabitq.rs:199-215): exact >= est - err for
+    most pairs.  The 4-bit query quantisation sits outside the theorem, so a few
+    percent of pairs break the radius on Gaussian data (3.3% here); a wrong
+    factor in the radius or estimate breaks it for far more."""
+    vs = np.random.default_rng(4).normal(size=(400, 96)).astype(np.float32)
+    padded, signs = _rq(vs)
+    cent = vs.mean(0).astype(np.float32)
+    codes = codecs.rabitq_encode(vs, cent, padded, signs)
+    fails = 0
+    for qi in range(5):
+        planes, consts = oracle.rabitq_query(vs[qi], cent, padded, signs)
+        for i in range(400):
+            est, err = oracle.rabitq_dist(planes, consts, codes[i], padded)
+            fails += float(((vs[qi] - vs[i]) ** 2).sum()) < est - err - 1e-3
+    assert fails <= 0.06 * 2000
+
+
+# ------------------------------------------------------------------ search
+def test_exhaustive_probe_matches_brute_force(built):
+    """python/tests/test_spfresh.py:84-90."""
+    ix, vs = built
+    o = oracle.Oracle(ix)
+    qs = uniform(20, seed=2)
+    gt, gd = brute_force_f64(vs, qs, 10, 1)
+    for i, q in enumerate(qs):
+        ids, dd = o.search(q, 10, ix.n_post)
+        assert ids.tolist() == gt[i].tolist()
+        np.testing.assert_allclose(dd, gd[i], rtol=1e-4, atol=1e-5)
+
+
+def test_recall_improves_with_probes(built):
+    """python/tests/test_spfresh.py:93-99."""
+    ix, vs = built
+    o = oracle.Oracle(ix)
+    qs = clustered(50, seed=3)
+    gt, _ = brute_force_f64(vs, qs, 10, 1)
+
+    def rec(p):
+        ids, _, _ = o.search_batch(qs, 10, p)
+        return np.mean([len(set(ids[i].tolist()) & set(gt[i].tolist())) / 10 for i in range(50)])
+
+    r1, r4, r16 = rec(1), rec(4), rec(16)
+    assert r16 >= r4 >= r1 - 0.02
+    assert r4 >= 0.9 and r16 >= 0.98
+
+
+def test_batch_matches_single(built):
+    """python/tests/test_spfresh.py:246-249."""
+    ix, _ = built
+    o = oracle.Oracle(ix)
+    qs = clustered(25, seed=18)
+    ids, dd, cnt = o.search_batch(qs, 10, 8, threads=4)
+    for i, q in enumerate(qs):
+        si, sd = o.search(q, 10, 8)
+        assert ids[i, : cnt[i]].tolist() == si.tolist()
+        np.testing.assert_array_equal(dd[i, : cnt[i]], sd)
+
+
+def test_deletes_hidden():
+    """python/tests/test_spfresh.py:120-145."""
+    vs = clustered(2000, seed=1)
+    ix = build_index(vs, "l2", None, max_posting_size=64, seed=0)
+    dead = list(range(0, 500, 10))
+    assert ix.set_deleted(dead) == len(dead)
+    assert ix.set_deleted(dead) == 0 and ix.set_deleted([10 ** 9]) == 0
+    live = np.ones(2000, bool)
+    live[dead] = False
+    o = oracle.Oracle(ix)
+    qs = vs[dead[:10]]
+    gt, _ = brute_force_f64(vs, qs, 10, 1, live=live)
+    for i, q in enumerate(qs):
+        ids, _ = o.search(q, 10, ix.n_post)
+        assert not set(ids.tolist()) & set(dead)
+        assert ids.tolist() == gt[i].tolist()
+
+
+@pytest.mark.parametrize("quantizer,min_recall", [("f16", 0.98), ("int8", 0.9), ("rabitq", 0.95)])
+def test_quantizers_rerank_exact(quantizer, min_recall):
+    """python/tests/test_spfresh.py:205-228: recall floors at n_probe 16 with
+    rerank 64, and returned distances are exact squared L2."""
+    vs = clustered(2000, seed=14)
+    ix = build_index(vs, "l2", quantizer, max_posting_size=64, rerank_size=64, seed=0)
+    o = oracle.Oracle(ix)
+    qs = clustered(30, seed=15)
+    gt, _ = brute_force_f64(vs, qs, 10, 1)
+    ids, dd, cnt = o.search_batch(qs, 10, 16)
+    r = np.mean([len(set(ids[i].tolist()) & set(gt[i].tolist())) / 10 for i in range(30)])
+    assert r >= min_recall
+    for i in range(10):
+        for j in range(int(cnt[i])):
+            exact = float(((qs[i].astype(np.float64) - vs[ids[i, j]]) ** 2).sum())
+            assert dd[i, j] == pytest.approx(exact, rel=1e-4, abs=1e-5)
+
+
+def test_rabitq_search_with_tiny_k_is_exact_among_survivors():
+    """search_rabitq returns exact distances (src/spfresh.rs:753-760), sqrt for DistL2."""
+    vs = clustered(1500, seed=5)
+    ix = build_index(vs, "l2_root", "rabitq", max_posting_size=64, seed=0)
+    o = oracle.Oracle(ix)
+    ids, dd = o.search(vs[7], 1, 8)
+    assert ids[0] == 7 and abs(dd[0]) < 1e-3              # src/spfresh.rs:1372-1374
+    ids, dd = o.search(vs[8] + 0.01, 5, 8)
+    for i, d in zip(ids, dd):
+        assert d == pytest.approx(math.sqrt(((vs[8] + 0.01 - vs[i]).astype(np.float64) ** 2).sum()),
+                                  rel=1e-4)
+
+
+def test_probe_ratio_closed_form():
+    """take_while !(d > d0 * max(ratio, 1)) over the sorted probe list
+    (src/spfresh.rs:683-687)."""
+    vs = clustered(2000, seed=6)
+    for ratio in (1.0, 1.2, 2.0, math.inf, 0.5):
+        ix = build_index(vs, "l2", None, max_posting_size=64, probe_ratio=ratio, seed=0)
+        o = oracle.Oracle(ix)
+        for q in clustered(10, seed=7):
+            d = np.array([oracle.eval_metric(1, q, c) for c in ix.centroids])
+            order = np.lexsort((ix.block_ids, d))[:16]
+            lim = d[order[0]] * max(ratio, 1.0)
+            expect = [int(p) for p in order if not (math.isfinite(ratio) and d[p] > lim)]
+            posts, pd = o.probe(q, 16)
+            assert posts.tolist() == expect[: len(posts)] and len(posts) == len(expect)
+
+
+def test_short_rows_and_empty_index():
+    vs = clustered(30, seed=8)
+    ix = build_index(vs, "l2", None, n_postings=5, seed=0)
+    o = oracle.Oracle(ix)
+    ids, dd = o.search(vs[0], 100, 1000)
+    assert len(ids) == 30 and ids[0] == 0
+    ids, dd = o.search(vs[0], 100, 1)
+    assert len(ids) < 30
+
+
+# ------------------------------------------------------------------ planted mistakes
+def _integer_closed_form_holds():
+    rng = np.random.default_rng(9)
+    a = rng.integers(0, 256, 64).astype(np.float32)
+    b = rng.integers(0, 256, 64).astype(np.float32)
+    return oracle.eval_metric(1, a, b) == float(((a.astype(np.int64) - b) ** 2).sum())
+
+
+def _tombstones_hold():
+    vs = clustered(500, seed=10)
+    ix = build_index(vs, "l2", None, n_postings=8, seed=0)
+    ix.set_deleted([3])
+    ids, _ = oracle.Oracle(ix).search(vs[3], 5, 8)
+    return 3 not in ids.tolist()
+
+
+def _probe_ratio_holds():
+    vs = clustered(1000, seed=11)
+    ix = build_index(vs, "l2", None, n_postings=30, probe_ratio=1.0, seed=0)
+    posts, pd = oracle.Oracle(ix).probe(vs[0], 16)
+    return (pd <= pd[0]).all()
+
+
+def _f16_holds():
+    # f16 spacing is 1 in [1024, 2048): .5 are ties, .7 must round up
+    base = np.float32(1024) + np.arange(200, dtype=np.float32)
+    xs = np.concatenate([base + np.float32(0.5), base + np.float32(0.7)])
+    return all(oracle.f32_to_f16(float(x)) == int(x.astype(np.float16).view(np.uint16)) for x in xs)
+
+
+@pytest.mark.parametrize("fault,check", [(1, _integer_closed_form_holds), (2, _tombstones_hold),
+                                         (3, _probe_ratio_holds), (4, _f16_holds)])
+def test_planted_mistakes_are_caught(fault, check):
+    assert check()
+    oracle.set_fault(fault)
+    try:
+        assert not check(), f"planted mistake {fault} went unnoticed"
+    finally:
+        oracle.set_fault(0)
+    assert check()
