@@ -43,10 +43,10 @@ def build_workload(args, torch):
         n_post, n_probe, k = 4096, 64, 100
     log(f"[bench] data {base.shape} queries {queries.shape} in {time.time() - t0:.1f}s")
     t0 = time.time()
-    quant = None if args.precision == "exact" else "f16"
-    rerank = 0 if args.precision == "exact" else args.rerank
-    ix = build_index(base, "l2", quant, n_postings=n_post, rerank_size=rerank, seed=0,
-                     kmeans_iters=8, device="cuda")
+    # one f32 index serves both precisions: exact scans the f32 codes, mixed
+    # scans the library's f16 copy and reranks a pool from the f32 rows
+    ix = build_index(base, "l2", None, n_postings=n_post, seed=0, kmeans_iters=8,
+                     device="cuda")
     log(f"[bench] index {ix.n_post} postings, max len {int(ix.posting_lengths().max())} "
         f"in {time.time() - t0:.1f}s")
     return base, queries, ix, n_probe, k, np
@@ -89,7 +89,7 @@ def main() -> int:
     def step():
         dev.search_device(q_d.data_ptr(), nq, dim, k, n_probe, ids_d.data_ptr(),
                           dd_d.data_ptr(), cnt_d.data_ptr(), stream=stream.cuda_stream,
-                          precision=prec, profile=True)
+                          precision=prec, rerank_size=args.rerank if prec else 0, profile=True)
 
     for _ in range(args.warmup):
         step()
@@ -151,7 +151,8 @@ def main() -> int:
         "data": "synthetic (seeded numpy Gaussian mixture; index by seeded k-means)",
         "config": {"workload": f"{args.workload}: {ix.n_raw}x{dim} L2, {ix.n_post} postings, "
                                f"n_probe {n_probe}, k {k}, {nq} queries per step",
-                   "precision": args.precision, "rerank": ix.rerank_size},
+                   "precision": args.precision,
+                   "rerank_pool": args.rerank if args.precision == "mixed" else 0},
         "recall_at_10": rec, "p50_ms": round(p50, 3), "p99_ms": round(p99, 3),
         "stage_ms": {s: round(v, 4) for s, v in stage.items()},
         "roofline": roofline, "cpu_baseline": cpu,

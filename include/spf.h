@@ -147,6 +147,11 @@ int spf_search_batch_device(spf_index *ix, const float *d_queries, uint32_t nq, 
 /* Profile of the last search on this thread that set SPF_FLAG_PROFILE. */
 int spf_last_profile(spf_profile *out);
 
+/* Device self-check of the matrix-core fragment layout the mixed scan relies
+ * on (a 16x16x32 f16 and bf16 product against the host).  SPF_OK or
+ * SPF_EDEVICE with the first mismatch in spf_last_error. */
+int spf_selftest(int device);
+
 /* Thread-local message for the last error on this thread. */
 const char *spf_last_error(void);
 /* Library build string (arch, flags). */

@@ -19,8 +19,8 @@ PREC_EXACT, PREC_MIXED = 0, 1
 
 # every entry point of include/spf.h
 EXPORTS = ("spf_index_create", "spf_index_destroy", "spf_index_get_info", "spf_delete",
-           "spf_search_batch", "spf_search_batch_device", "spf_last_profile", "spf_last_error",
-           "spf_version")
+           "spf_search_batch", "spf_search_batch_device", "spf_last_profile", "spf_selftest",
+           "spf_last_error", "spf_version")
 FLAG_PROFILE = 1
 STAGES = ("prep", "probe", "probe_select", "plan", "scan", "select", "rerank", "finalize")
 
@@ -91,6 +91,8 @@ def lib():
         L.spf_search_batch_device.restype = C.c_int
         L.spf_last_profile.argtypes = [C.POINTER(Profile)]
         L.spf_last_profile.restype = C.c_int
+        L.spf_selftest.argtypes = [C.c_int]
+        L.spf_selftest.restype = C.c_int
         L.spf_last_error.restype = C.c_char_p
         L.spf_version.restype = C.c_char_p
         _lib = L

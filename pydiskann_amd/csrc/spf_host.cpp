@@ -79,10 +79,21 @@ struct Workspace {
     DBuf<uint64_t> out_ids;
     DBuf<float> out_d;
     DBuf<uint32_t> out_c;
+    DBuf<uint16_t> q16, cand16;
+    DBuf<float> qn16;
+    DBuf<uint32_t> bits, pool_cnt, pool_want;
+    DBuf<uint64_t> pool;
     uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
     hipEvent_t ev[SPF_N_STAGES + 1] = {};
 
     void release() {
+        q16.release();
+        cand16.release();
+        bits.release();
+        pool_cnt.release();
+        pool_want.release();
+        pool.release();
+        qn16.release();
         DBuf<float> *fs[] = {&qin, &qv_probe, &qv_scan, &qnorm, &out_d};
         for (auto *b : fs) b->release();
         DBuf<uint32_t> *us[] = {&q8, &qsq, &probe_cnt, &np_q, &probe_post, &post_cnt, &post_fill,
@@ -122,6 +133,12 @@ struct spf_index {
     uint32_t *d_del = nullptr;
     float *d_xnorm = nullptr;
     uint32_t *d_xsq = nullptr;
+    uint32_t *d_rank_entry = nullptr;   // entry of each rank
+    float *d_raw_e = nullptr;           // raw f32 rows in entry order, dpad stride
+    uint4 *d_codes16 = nullptr;         // tiled f16 codes for the matrix-core scan
+    bool codes16_owned = false;
+    float *d_xn16 = nullptr;
+    uint32_t units16 = 0;
     uint64_t device_bytes = 0;
 
     std::vector<uint64_t> rank_id;      // ids in ascending order
@@ -134,9 +151,10 @@ struct spf_index {
 
     ~spf_index() {
         void *ps[] = {d_cent, d_cnorm, d_post_off, d_post_len, d_codes, d_rank,
-                      d_rank_id, d_raw, d_del, d_xnorm, d_xsq};
+                      d_rank_id, d_raw, d_del, d_xnorm, d_xsq, d_rank_entry, d_raw_e, d_xn16};
         for (void *p : ps)
             if (p) (void)hipFree(p);
+        if (codes16_owned && d_codes16) (void)hipFree(d_codes16);
         for (Workspace *w : pool) {
             w->release();
             delete w;
@@ -321,12 +339,40 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
     }
     HIPCHK(upload(&ix->d_rank, entry_rank.data(), entry_rank.size(), bytes));
     HIPCHK(upload(&ix->d_rank_id, ix->rank_id.data(), ix->rank_id.size(), bytes));
-    if (need_raw) {
+    HIPCHK(upload(&ix->d_rank_entry, ix->rank_entry.data(), ix->rank_entry.size(), bytes));
+    {
+        // raw f32 rows in entry order: the rerank source of both precisions
         std::vector<float> rr((size_t)N * ix->dpad, 0.0f);
-        for (uint64_t r = 0; r < N; r++)
-            std::memcpy(&rr[(size_t)r * ix->dpad], d->raw + (size_t)ix->rank_id[r] * d->dim,
-                        4 * d->dim);
-        HIPCHK(upload(&ix->d_raw, rr.data(), rr.size(), bytes));
+        for (uint64_t e = 0; e < N; e++) {
+            const float *src = need_raw
+                                   ? d->raw + (size_t)d->entry_ids[entry_src[e]] * d->dim
+                                   : (const float *)(d->codes + (size_t)entry_src[e] * d->code_bytes);
+            std::memcpy(&rr[(size_t)e * ix->dpad], src, 4 * d->dim);
+        }
+        HIPCHK(upload(&ix->d_raw_e, rr.data(), rr.size(), bytes));
+    }
+    // f16 codes for the matrix-core scan: the index's own f16 codes, or an
+    // IEEE round-to-nearest-even copy of f32 codes
+    if (d->quantizer == SPF_QUANT_F16) {
+        ix->d_codes16 = ix->d_codes;
+        ix->units16 = ix->units;
+    } else if (d->quantizer == SPF_QUANT_NONE) {
+        ix->units16 = (2 * d->dim + 15) / 16;
+        std::vector<_Float16> h((size_t)N * d->dim);
+        for (uint64_t e = 0; e < N; e++) {
+            const float *src = (const float *)(d->codes + (size_t)entry_src[e] * d->code_bytes);
+            for (uint32_t i = 0; i < d->dim; i++) h[(size_t)e * d->dim + i] = (_Float16)src[i];
+        }
+        std::vector<uint4> ct = tile_rows((const uint8_t *)h.data(), N, 2 * d->dim, ix->units16,
+                                          nullptr);
+        HIPCHK(upload(&ix->d_codes16, ct.data(), ct.size(), bytes));
+        ix->codes16_owned = true;
+    }
+    if (ix->d_codes16) {
+        HIPCHK(hipMalloc((void **)&ix->d_xn16, std::max<uint64_t>(N, 1) * 4));
+        bytes += 4 * N;
+        HIPCHK(launch_entry_norms(CK_F16, ix->d_codes16, ix->units16, (uint32_t)N, d->dim, 1.0f,
+                                  0.0f, ix->d_xn16, nullptr, nullptr));
     }
     // tombstones
     ix->id_del.assign((d->n_raw + 7) / 8, 0);
@@ -365,18 +411,84 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
     return SPF_OK;
 }
 
+// Matrix-core scan -> 16-bit pool keys -> pool marks -> posting-major exact
+// rerank -> per-query sort.  Runs after probe and planning.
+template <class Mark>
+int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_t k,
+                 uint32_t want, uint64_t total, uint32_t n_items, uint64_t *d_ids,
+                 float *d_dists, uint32_t *d_counts, Mark &mark) {
+    const uint32_t dpad32 = round_up(ix->dim, 32);
+    HIPCHK(w->cand16.ensure(std::max<uint64_t>(total, 1)));
+    HIPCHK(w->bits.ensure(total / 32 + 4));
+    HIPCHK(w->pool.ensure((size_t)nq * want));
+    HIPCHK(w->pool_cnt.ensure(nq));
+    HIPCHK(w->pool_want.ensure(nq));
+    HIPCHK(w->fin.ensure((size_t)nq * k));
+    HIPCHK(w->fin_cnt.ensure(nq));
+    MixedParams mp{};
+    mp.item_off = w->item_off.p;
+    mp.n_post = ix->P;
+    mp.n_items = n_items;
+    mp.post_off = ix->d_post_off;
+    mp.pair_off = w->pair_off.p;
+    mp.pair_q = w->pair_q.p;
+    mp.pair_slot = w->pair_slot.p;
+    mp.q16 = (const uint4 *)w->q16.p;
+    mp.q16_units = dpad32 / 8;
+    mp.ks = dpad32 / 32;
+    mp.qn16 = w->qn16.p;
+    mp.codes16 = ix->d_codes16;
+    mp.units = ix->units16;
+    mp.xn16 = ix->d_xn16;
+    mp.del = ix->d_del;
+    mp.out16 = w->cand16.p;
+    mp.bits = w->bits.p;
+    mp.pool = w->pool.p;
+    mp.pool_cnt = w->pool_cnt.p;
+    mp.want = want;
+    mp.metric = ix->metric;
+    mp.dpad = ix->dpad;
+    mp.qv = w->qv_probe.p;
+    mp.qnorm = w->qnorm.p;
+    mp.raw_e = ix->d_raw_e;
+    mp.rank = ix->d_rank;
+    const uint32_t grid = std::min<uint32_t>(n_items, 16384);
+    HIPCHK(mark(SPF_STAGE_SCAN));
+    HIPCHK(launch_scan_mfma(mp, grid, s));
+    HIPCHK(mark(SPF_STAGE_SELECT));
+    HIPCHK(hipMemsetAsync(w->bits.p, 0, (total / 32 + 4) * 4, s));
+    HIPCHK(hipMemsetAsync(w->pool_cnt.p, 0, nq * 4, s));
+    HIPCHK(launch_pool_select(w->cand16.p, w->cand_off.p, nq, want, w->bits.p, w->pool_want.p, s));
+    HIPCHK(mark(SPF_STAGE_RERANK));
+    HIPCHK(launch_rerank_items(mp, grid, s));
+    HIPCHK(launch_sort_topk(w->pool.p, w->pool_cnt.p, nq, want, k, w->fin.p, w->fin_cnt.p, s));
+    HIPCHK(mark(SPF_STAGE_FINALIZE));
+    HIPCHK(launch_finalize(w->fin.p, k, w->fin_cnt.p, nq, k, ix->d_rank_id, d_ids, d_dists,
+                           d_counts, s));
+    return SPF_OK;
+}
+
 int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, uint32_t nq,
                 uint32_t k, uint32_t n_probe, const spf_search_opts *opts, uint64_t *d_ids,
                 float *d_dists, uint32_t *d_counts) {
     const uint32_t P = ix->P;
     const uint32_t npr = std::min(std::max(n_probe, 1u), std::max(P, 1u));
-    const bool rerank = ix->quant != SPF_QUANT_NONE && ix->rerank_size > 0;
-    const uint32_t want = rerank ? std::max(k, ix->rerank_size) : k;
+    const bool mixed = opts && opts->precision == SPF_PREC_MIXED;
+    if (opts && opts->precision > SPF_PREC_MIXED) return fail(SPF_EINVAL, "unknown precision");
+    if (mixed && (!ix->d_codes16 || ix->dim > 256))
+        return fail(SPF_EUNSUPPORTED,
+                    "mixed precision serves f32 and f16 postings with dim <= 256");
+    const bool rerank = !mixed && ix->quant != SPF_QUANT_NONE && ix->rerank_size > 0;
+    uint32_t want = rerank ? std::max(k, ix->rerank_size) : k;
+    if (mixed) {
+        const uint32_t pool = opts->rerank_size ? opts->rerank_size
+                              : ix->rerank_size ? ix->rerank_size : 1000;
+        want = std::max(k, pool);
+    }
     if (want > kMaxK) return fail(SPF_EINVAL, "k / rerank_size above 2048");
-    if (opts && opts->precision != SPF_PREC_EXACT)
-        return fail(SPF_EUNSUPPORTED, "mixed precision is not served by this build yet");
     const uint32_t dpad = ix->dpad;
-    const uint32_t G = dpad <= 1024 ? 16 : 4;
+    const uint32_t dpad32 = round_up(ix->dim, 32);
+    const uint32_t G = mixed ? 32 : (dpad <= 1024 ? 16 : 4);
     const bool prof = opts && (opts->flags & SPF_FLAG_PROFILE);
     if (prof)
         for (auto &e : w->ev)
@@ -420,6 +532,13 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pp.q8 = w->q8.p;
     pp.qnorm = w->qnorm.p;
     pp.qsq = w->qsq.p;
+    if (mixed) {
+        HIPCHK(w->q16.ensure((size_t)nq * dpad32));
+        HIPCHK(w->qn16.ensure(nq));
+        pp.q16 = w->q16.p;
+        pp.dpad32 = dpad32;
+        pp.qn16 = w->qn16.p;
+    }
     HIPCHK(launch_prep(pp, s));
     HIPCHK(mark(SPF_STAGE_PROBE));
     const float *qv_scan = pp.qv_scan;
@@ -507,6 +626,11 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     const uint64_t total = w->h_summary[0];
     const uint32_t n_items = (uint32_t)w->h_summary[1];
 
+    if (mixed) {
+        int rc = search_mixed(ix, w, s, nq, k, want, total, n_items, d_ids, d_dists, d_counts,
+                              mark);
+        if (rc) return rc;
+    } else {
     // ---- posting scan
     HIPCHK(w->cand.ensure(std::max<uint64_t>(total, 1)));
     HIPCHK(mark(SPF_STAGE_SCAN));
@@ -550,7 +674,8 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         rp.want = want;
         rp.k = k;
         rp.nq = nq;
-        rp.raw = ix->d_raw;
+        rp.raw = ix->d_raw_e;
+        rp.rank_entry = ix->d_rank_entry;
         rp.qv = w->qv_probe.p;
         rp.qnorm = w->qnorm.p;
         rp.dim = ix->dim;
@@ -567,6 +692,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         HIPCHK(launch_finalize(w->sel.p, want, w->sel_cnt.p, nq, k, ix->d_rank_id, d_ids,
                                d_dists, d_counts, s));
     }
+    }
     if (prof) {
         HIPCHK(mark(SPF_N_STAGES));
         HIPCHK(hipEventSynchronize(w->ev[SPF_N_STAGES]));
@@ -574,7 +700,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         for (int i = 0; i < SPF_N_STAGES; i++)
             HIPCHK(hipEventElapsedTime(&g_prof.stage_ms[i], w->ev[i], w->ev[i + 1]));
         g_prof.candidates = total;
-        g_prof.scan_code_bytes = total * ix->code_bytes;
+        g_prof.scan_code_bytes = total * (mixed ? 2ull * ix->dim : (uint64_t)ix->code_bytes);
         g_prof.probe_pairs = (uint64_t)nq * P;
         g_prof.scan_items = n_items;
         g_prof_valid = true;
@@ -694,6 +820,33 @@ int spf_search_batch(spf_index *ix, const float *queries, uint32_t nq, uint32_t 
     rc = run();
     release_ws(ix, w);
     return rc;
+}
+
+int spf_selftest(int device) {
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(SPF_EINVAL, "bad device ordinal");
+    DeviceGuard dg(device);
+    float *d = nullptr;
+    HIPCHK(hipMalloc((void **)&d, 2 * 256 * sizeof(float)));
+    std::vector<float> h(512);
+    hipError_t e = launch_selftest_mfma(d, d + 256, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), d, 512 * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    HIPCHK(e);
+    for (int m = 0; m < 16; m++)
+        for (int n = 0; n < 16; n++) {
+            float ref = 0.0f;
+            for (int k = 0; k < 32; k++)
+                ref += (float)(((3 * m + 5 * k) % 7) - 3) * (float)(((2 * k + 7 * n) % 5) - 2);
+            if (h[m * 16 + n] != ref || h[256 + m * 16 + n] != ref) {
+                char msg[128];
+                std::snprintf(msg, sizeof msg, "mfma 16x16x32 layout mismatch at (%d,%d): f16 %g bf16 %g want %g",
+                              m, n, h[m * 16 + n], h[256 + m * 16 + n], ref);
+                return fail(SPF_EDEVICE, msg);
+            }
+        }
+    return SPF_OK;
 }
 
 int spf_last_profile(spf_profile *out) {

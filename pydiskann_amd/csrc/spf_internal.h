@@ -57,6 +57,35 @@ struct PrepParams {
     uint32_t *q8;               // [nq][dpad/4]
     float *qnorm;               // [nq]
     uint32_t *qsq;              // [nq]
+    uint16_t *q16;              // [nq][dpad32] f16 query for the matrix-core scan (null = skip)
+    uint32_t dpad32;            // dim rounded up to 32
+    float *qn16;                // [nq] |f16(q)|^2
+};
+
+// Mixed-precision scan / rerank over the same posting-major work items.
+struct MixedParams {
+    const uint32_t *item_off;   // [n_post + 1]
+    uint32_t n_post, n_items;
+    const uint32_t *post_off;   // [n_post + 1]
+    const uint32_t *pair_off;   // [n_post + 1]
+    const uint32_t *pair_q;
+    const uint64_t *pair_slot;
+    const uint4 *q16;           // [nq][q16_units] f16 rows
+    uint32_t q16_units, ks;     // 16-byte units per query row; 32-dim K steps
+    const float *qn16;
+    const uint4 *codes16;       // tiled f16 codes
+    uint32_t units;             // 16-byte units per f16 code
+    const float *xn16;          // |x16|^2 per entry
+    const uint32_t *del;
+    uint16_t *out16;            // 16-bit candidate keys
+    uint32_t *bits;             // pool marks over candidate slots
+    uint64_t *pool;             // [nq][want] reranked keys
+    uint32_t *pool_cnt;         // [nq]
+    uint32_t want, metric, dpad;
+    const float *qv;            // [nq][dpad] f32 queries
+    const float *qnorm;         // [nq] |q|^2 (cosine)
+    const float *raw_e;         // [n_entries][dpad] f32 rows in entry order
+    const uint32_t *rank;       // per entry
 };
 
 struct PlanParams {
@@ -83,7 +112,8 @@ struct RerankParams {
     const uint64_t *sel;        // [nq][want] sorted pool keys
     const uint32_t *sel_cnt;    // [nq]
     uint32_t want, k, nq;
-    const float *raw;           // [n][dim] raw vectors by rank
+    const float *raw;           // [n][dpad] raw rows in entry order
+    const uint32_t *rank_entry; // entry of each rank
     const float *qv;            // [nq][dpad] f32 queries
     const float *qnorm;
     uint32_t dim, dpad, metric;
@@ -112,5 +142,14 @@ hipError_t launch_finalize(const uint64_t *keys, uint32_t stride, const uint32_t
 hipError_t launch_entry_norms(uint32_t code_kind, const uint4 *data, uint32_t units,
                               uint32_t n, uint32_t dim, float scale, float offset, float *xnorm,
                               uint32_t *xsq, hipStream_t s);
+
+hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t s);
+hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s);
+hipError_t launch_pool_select(const uint16_t *keys, const uint64_t *seg_off, uint32_t nq,
+                              uint32_t want, uint32_t *bits, uint32_t *pool_want, hipStream_t s);
+hipError_t launch_rerank_items(const MixedParams &p, uint32_t grid, hipStream_t s);
+hipError_t launch_sort_topk(const uint64_t *pool, const uint32_t *pool_cnt, uint32_t nq,
+                            uint32_t want, uint32_t k, uint64_t *out, uint32_t *out_cnt,
+                            hipStream_t s);
 
 }  // namespace spf

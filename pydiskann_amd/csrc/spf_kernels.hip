@@ -75,6 +75,10 @@ __global__ __launch_bounds__(64) void k_prep(const PrepParams p) {
         p.qv_probe[(size_t)q * p.dpad + i] = v;
         if (p.round_f16) p.qv_scan[(size_t)q * p.dpad + i] = (float)(_Float16)v;
     }
+    if (p.q16) {
+        for (uint32_t i = lane; i < p.dpad32; i += 64)
+            p.q16[(size_t)q * p.dpad32 + i] = __builtin_bit_cast(uint16_t, (_Float16)(i < p.dim ? src[i] : 0.0f));
+    }
     __syncthreads();
     if (p.want_q8) {
         const float alpha = 1.0f / p.scale;
@@ -99,6 +103,14 @@ __global__ __launch_bounds__(64) void k_prep(const PrepParams p) {
         float acc = 0.0f;
         for (uint32_t i = 0; i < p.dim; i++) acc = __builtin_fmaf(s_v[i], s_v[i], acc);
         p.qnorm[q] = acc;
+        if (p.q16) {
+            float a16 = 0.0f;
+            for (uint32_t i = 0; i < p.dim; i++) {
+                const float h = (float)(_Float16)s_v[i];
+                a16 = __builtin_fmaf(h, h, a16);
+            }
+            p.qn16[q] = a16;
+        }
         if (p.want_q8) {
             uint32_t s = 0;
             for (uint32_t w = 0; w < p.dpad / 4; w++) {
@@ -557,7 +569,8 @@ __global__ __launch_bounds__(256) void k_rerank(const RerankParams p) {
     const float qn = p.qnorm[q];
     for (uint32_t i = tid; i < c; i += kWG) {
         const uint32_t rank = (uint32_t)sel[i];
-        const float d = exact_row<MET>(qv, p.raw + (size_t)rank * p.dpad, p.dpad, qn);
+        const uint32_t e = p.rank_entry[rank];
+        const float d = exact_row<MET>(qv, p.raw + (size_t)e * p.dpad, p.dpad, qn);
         buf[i] = ((uint64_t)fkey(d) << 32) | rank;
     }
     const uint32_t m2 = next_pow2(c < 2 ? 2 : c);

@@ -1,0 +1,443 @@
+// Mixed-precision batch search kernels (gfx950 matrix cores).
+//
+//   k_scan_mfma     posting scan as a (queries x entries) f16 GEMM on
+//                   v_mfma_f32_16x16x32_f16 with f32 accumulate:
+//                   dist = |x|^2 + |q|^2 - 2 q.x, stored as a 16-bit key
+//                   (top half of the order-preserving f32 key) per candidate
+//   k_pool_select   per query: the `want` smallest 16-bit keys (ties by
+//                   candidate slot), marked in a bitmap over the candidate buffer
+//   k_rerank_items  posting-major exact f32 rerank of the marked candidates
+//                   (canonical sequential-fma order), one work item = one
+//                   posting x <= 32 queries, so a posting's raw rows are read once
+//                   for all queries that kept candidates in it
+//   k_sort_topk     per query: sort the reranked pool by (dist, id), keep k
+//
+// MFMA fragment layout (v_mfma_f32_16x16x32_{f16,bf16}, wave64), checked on the
+// device by spf_selftest: lane l holds A[l%16][8*(l/16) .. +7],
+// B[8*(l/16) .. +7][l%16], and D[4*(l/16) + j][l%16] for j = 0..3.
+#include "spf_internal.h"
+
+namespace spf {
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t fkey_m(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ half8_t as_half8(uint4 v) {
+    union { uint4 u; half8_t h; } c;
+    c.u = v;
+    return c.h;
+}
+
+// ---------------------------------------------------------------- self-test
+// A[m][k] = ((3m + 5k) % 7) - 3, B[k][n] = ((2k + 7n) % 5) - 2, D = A B (exact
+// in f16/bf16 and f32).  Each lane builds its fragments by the documented layout
+// and writes D back by the documented output layout.
+__global__ void k_selftest_mfma(float *out_f16, float *out_bf16) {
+    const int l = threadIdx.x;
+    half8_t a, b;
+    bf16x8_t a2, b2;
+    for (int i = 0; i < 8; i++) {
+        const int m = l % 16, k = 8 * (l / 16) + i, n = l % 16;
+        const float av = (float)(((3 * m + 5 * k) % 7) - 3);
+        const float bv = (float)(((2 * k + 7 * n) % 5) - 2);
+        a[i] = (_Float16)av;
+        b[i] = (_Float16)bv;
+        a2[i] = (__bf16)av;
+        b2[i] = (__bf16)bv;
+    }
+    floatx4_t acc = {0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc, 0, 0, 0);
+    floatx4_t acc2 = {0, 0, 0, 0};
+    acc2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2, acc2, 0, 0, 0);
+    for (int j = 0; j < 4; j++) {
+        const int m = 4 * (l / 16) + j, n = l % 16;
+        out_f16[m * 16 + n] = acc[j];
+        out_bf16[m * 16 + n] = acc2[j];
+    }
+}
+
+hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t s) {
+    hipLaunchKernelGGL(k_selftest_mfma, dim3(1), dim3(64), 0, s, d_out_f16, d_out_bf16);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- MFMA scan
+constexpr int kMixG = 32;      // queries per work item (two 16-row MFMA tiles)
+constexpr int kMixKS = 8;      // max K-steps of 32 dims held in registers (dim <= 256)
+
+__device__ __forceinline__ void item_decode(const MixedParams &p, uint32_t item, uint32_t &pp,
+                                            uint32_t &pb, uint32_t &e_begin, uint32_t &e_end,
+                                            uint32_t &pr0, uint32_t &gcnt) {
+    uint32_t lo = 0, hi = p.n_post;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (p.item_off[mid] <= item) lo = mid; else hi = mid;
+    }
+    pp = lo;
+    const uint32_t local = item - p.item_off[pp];
+    pb = p.post_off[pp];
+    const uint32_t pe = p.post_off[pp + 1];
+    const uint32_t nec = (pe - pb + kEChunk - 1) / kEChunk;
+    const uint32_t qc = local / nec, ec = local - qc * nec;
+    pr0 = p.pair_off[pp] + qc * kMixG;
+    gcnt = min((uint32_t)kMixG, p.pair_off[pp + 1] - pr0);
+    e_begin = pb + ec * kEChunk;
+    e_end = min(e_begin + kEChunk, pe);
+}
+
+__global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
+    __shared__ uint64_t s_slot[kMixG];
+    __shared__ uint32_t s_qi[kMixG];
+    __shared__ float s_qn[kMixG];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t col = lane & 15, kg = lane >> 4;
+    for (uint32_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
+        uint32_t pp, pb, e_begin, e_end, pr0, gcnt;
+        item_decode(p, item, pp, pb, e_begin, e_end, pr0, gcnt);
+        __syncthreads();
+        if (tid < kMixG) {
+            const bool ok = tid < gcnt;
+            const uint32_t q = ok ? p.pair_q[pr0 + tid] : 0u;
+            s_qi[tid] = q;
+            s_slot[tid] = ok ? p.pair_slot[pr0 + tid] : 0ull;
+            s_qn[tid] = ok ? p.qn16[q] : 0.0f;
+        }
+        __syncthreads();
+        // A fragments: queries (rows) x dims (K), two 16-row tiles
+        half8_t a[2][kMixKS];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const uint32_t g = 16 * t + col;
+            const bool ok = g < gcnt;
+            const uint4 *qrow = p.q16 + (size_t)s_qi[g] * p.q16_units;
+#pragma unroll
+            for (int s = 0; s < kMixKS; s++) {
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (ok && s < (int)p.ks) v = qrow[4 * s + kg];
+                a[t][s] = as_half8(v);
+            }
+        }
+        for (uint32_t e0 = e_begin + 16 * wave; e0 < e_end; e0 += 64) {
+            const uint32_t e = e0 + col;
+            const bool valid = e < e_end;
+            const uint4 *src = p.codes16 + (size_t)(e >> 6) * p.units * 64 + (e & 63);
+            floatx4_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < kMixKS; s++) {
+                if (s < (int)p.ks) {
+                    const uint32_t u = 4 * s + kg;
+                    uint4 v = make_uint4(0, 0, 0, 0);
+                    if (valid && u < p.units) v = src[(size_t)u * 64];
+                    const half8_t b = as_half8(v);
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b, acc1, 0, 0, 0);
+                }
+            }
+            if (valid) {
+                const float xn = p.xn16[e];
+                const bool dead = p.del && ((p.del[e >> 5] >> (e & 31)) & 1u);
+                const uint32_t off = e - pb;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t g0 = 4 * kg + j, g1 = 16 + 4 * kg + j;
+                    if (g0 < gcnt) {
+                        const float d = xn + s_qn[g0] - 2.0f * acc0[j];
+                        p.out16[s_slot[g0] + off] = dead ? (uint16_t)0xffff : (uint16_t)(fkey_m(d) >> 16);
+                    }
+                    if (g1 < gcnt) {
+                        const float d = xn + s_qn[g1] - 2.0f * acc1[j];
+                        p.out16[s_slot[g1] + off] = dead ? (uint16_t)0xffff : (uint16_t)(fkey_m(d) >> 16);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- pool select
+// Per query: mark the `want` smallest 16-bit keys (0xffff = no candidate).
+// Exactly-tied keys at the boundary are taken in slot order.
+constexpr uint32_t kPoolLds = 24576;   // keys cached in LDS (48 KiB)
+
+__device__ __forceinline__ uint32_t excl_scan256(uint32_t v, uint32_t *part) {
+    const uint32_t tid = threadIdx.x;
+    part[tid] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t t = tid >= off ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += t;
+        __syncthreads();
+    }
+    const uint32_t incl = part[tid];
+    __syncthreads();
+    return incl - v;
+}
+
+// find the bin where the cumulative count reaches `rem`; returns (bin, below)
+__device__ __forceinline__ void find_bin256(const uint32_t *hist, uint32_t rem, uint32_t *part,
+                                            uint32_t *s_out) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t h = hist[tid];
+    const uint32_t excl = excl_scan256(h, part);
+    if (excl < rem && rem <= excl + h) {
+        s_out[0] = tid;
+        s_out[1] = excl;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_pool_select(const uint16_t *keys, const uint64_t *seg_off,
+                                                     uint32_t want, uint32_t *bits,
+                                                     uint32_t *pool_want) {
+    __shared__ uint16_t s_k[kPoolLds];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t part[256];
+    __shared__ uint32_t s_u[4];
+    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    const uint64_t b = seg_off[q];
+    const uint32_t n = (uint32_t)(seg_off[q + 1] - b);
+    const uint16_t *kp = keys + b;
+    const bool cached = n <= kPoolLds;
+    hist[tid] = 0;
+    if (tid == 0) s_u[2] = 0;
+    __syncthreads();
+    uint32_t inval = 0;
+    for (uint32_t i = tid; i < n; i += 256) {
+        const uint16_t k = kp[i];
+        if (cached) s_k[i] = k;
+        if (k == 0xffff) inval++;
+        else atomicAdd(&hist[k >> 8], 1u);
+    }
+    if (inval) atomicAdd(&s_u[2], inval);
+    __syncthreads();
+    const uint32_t nvalid = n - s_u[2];
+    const uint32_t W = min(want, nvalid);
+    if (tid == 0) pool_want[q] = W;
+    if (W == 0) return;
+    uint32_t T = 0, need = 0;
+    if (W != nvalid) {
+        find_bin256(hist, W, part, s_u);
+        const uint32_t b1 = s_u[0], below1 = s_u[1];
+        __syncthreads();
+        hist[tid] = 0;
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += 256) {
+            const uint16_t k = cached ? s_k[i] : kp[i];
+            if ((uint32_t)(k >> 8) == b1 && k != 0xffff) atomicAdd(&hist[k & 0xff], 1u);
+        }
+        __syncthreads();
+        find_bin256(hist, W - below1, part, s_u);
+        const uint32_t b2 = s_u[0], below2 = s_u[1];
+        T = (b1 << 8) | b2;
+        need = W - below1 - below2;   // keys equal to T to take, in slot order
+    }
+    // collect: each thread owns a contiguous run of slots so tie ranks follow slot order
+    const bool all = W == nvalid;
+    const uint32_t run = (n + 255) / 256;
+    const uint32_t i0 = min(n, tid * run), i1 = min(n, i0 + run);
+    uint32_t ties = 0;
+    if (!all)
+        for (uint32_t i = i0; i < i1; i++) ties += (cached ? s_k[i] : kp[i]) == T;
+    uint32_t tr = excl_scan256(ties, part);
+    for (uint32_t i = i0; i < i1; i++) {
+        const uint16_t k = cached ? s_k[i] : kp[i];
+        bool sel;
+        if (all) {
+            sel = k != 0xffff;
+        } else if (k < T) {
+            sel = true;
+        } else if (k == T) {
+            sel = tr < need;
+            tr++;
+        } else {
+            sel = false;
+        }
+        if (sel) {
+            const uint64_t bit = b + i;
+            atomicOr(&bits[bit >> 5], 1u << (bit & 31));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- rerank
+constexpr uint32_t kListCap = 2048;
+
+__device__ float exact_dist(uint32_t metric, const float *q, const float *x, uint32_t dpad,
+                            float qn) {
+    const float4 *q4 = reinterpret_cast<const float4 *>(q);
+    const float4 *x4 = reinterpret_cast<const float4 *>(x);
+    float acc = 0.0f, b2 = 0.0f;
+    const bool l2 = metric == M_L2 || metric == M_L2SQ;
+    for (uint32_t u = 0; u < dpad / 4; u++) {
+        const float4 a = q4[u], b = x4[u];
+        if (l2) {
+            float t = a.x - b.x; acc = __builtin_fmaf(t, t, acc);
+            t = a.y - b.y; acc = __builtin_fmaf(t, t, acc);
+            t = a.z - b.z; acc = __builtin_fmaf(t, t, acc);
+            t = a.w - b.w; acc = __builtin_fmaf(t, t, acc);
+        } else {
+            acc = __builtin_fmaf(a.x, b.x, acc);
+            acc = __builtin_fmaf(a.y, b.y, acc);
+            acc = __builtin_fmaf(a.z, b.z, acc);
+            acc = __builtin_fmaf(a.w, b.w, acc);
+            b2 = __builtin_fmaf(b.x, b.x, b2);
+            b2 = __builtin_fmaf(b.y, b.y, b2);
+            b2 = __builtin_fmaf(b.z, b.z, b2);
+            b2 = __builtin_fmaf(b.w, b.w, b2);
+        }
+    }
+    switch (metric) {
+    case M_L2SQ: return acc;
+    case M_L2: return sqrtf(acc);
+    case M_DOT: return 1.0f - acc;
+    default: {
+        if (qn == 0.0f && b2 == 0.0f) return 0.0f;
+        if (acc == 0.0f) return 1.0f;
+        const float r = 1.0f - acc / sqrtf(qn * b2);
+        return r > 0.0f ? r : 0.0f;
+    }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rerank_items(const MixedParams p) {
+    __shared__ uint64_t s_slot[kMixG];
+    __shared__ uint32_t s_qi[kMixG];
+    __shared__ uint32_t s_list[kListCap];
+    __shared__ uint32_t part[256];
+    __shared__ uint32_t s_tot;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
+        uint32_t pp, pb, e_begin, e_end, pr0, gcnt;
+        item_decode(p, item, pp, pb, e_begin, e_end, pr0, gcnt);
+        __syncthreads();
+        if (tid < gcnt) {
+            s_qi[tid] = p.pair_q[pr0 + tid];
+            s_slot[tid] = p.pair_slot[pr0 + tid];
+        }
+        __syncthreads();
+        const uint32_t ne = e_end - e_begin;
+        const uint32_t nw = (ne + 31) / 32;
+        const uint32_t nwords = gcnt * nw;
+        // deterministic positions: thread t owns words t, t+256, ...
+        uint32_t mine = 0;
+        for (uint32_t w = tid; w < nwords; w += 256) {
+            const uint32_t g = w / nw, wi = w - g * nw;
+            const uint64_t bit0 = s_slot[g] + (e_begin - pb) + 32ull * wi;
+            const uint64_t lo = p.bits[bit0 >> 5];
+            const uint64_t hi = p.bits[(bit0 >> 5) + 1];
+            uint32_t word = (uint32_t)(((hi << 32) | lo) >> (bit0 & 31));
+            const uint32_t nb = min(32u, ne - 32 * wi);
+            if (nb < 32) word &= (1u << nb) - 1u;
+            mine += __popc(word);
+        }
+        const uint32_t base = excl_scan256(mine, part);
+        if (tid == 255) s_tot = base + mine;
+        __syncthreads();
+        const uint32_t total = s_tot;
+        for (uint32_t r0 = 0; r0 < total; r0 += kListCap) {
+            uint32_t pos = base;
+            for (uint32_t w = tid; w < nwords; w += 256) {
+                const uint32_t g = w / nw, wi = w - g * nw;
+                const uint64_t bit0 = s_slot[g] + (e_begin - pb) + 32ull * wi;
+                const uint64_t lo = p.bits[bit0 >> 5];
+                const uint64_t hi = p.bits[(bit0 >> 5) + 1];
+                uint32_t word = (uint32_t)(((hi << 32) | lo) >> (bit0 & 31));
+                const uint32_t nb = min(32u, ne - 32 * wi);
+                if (nb < 32) word &= (1u << nb) - 1u;
+                while (word) {
+                    const uint32_t bi = __ffs(word) - 1;
+                    word &= word - 1;
+                    if (pos >= r0 && pos < r0 + kListCap) s_list[pos - r0] = (g << 16) | (32 * wi + bi);
+                    pos++;
+                }
+            }
+            __syncthreads();
+            const uint32_t n = min(kListCap, total - r0);
+            for (uint32_t i = tid; i < n; i += 256) {
+                const uint32_t v = s_list[i];
+                const uint32_t g = v >> 16, e = e_begin + (v & 0xffff);
+                const uint32_t q = s_qi[g];
+                const float d = exact_dist(p.metric, p.qv + (size_t)q * p.dpad,
+                                           p.raw_e + (size_t)e * p.dpad, p.dpad,
+                                           p.qnorm ? p.qnorm[q] : 0.0f);
+                const uint64_t key = ((uint64_t)fkey_m(d) << 32) | p.rank[e];
+                const uint32_t at = atomicAdd(&p.pool_cnt[q], 1u);
+                if (at < p.want) p.pool[(size_t)q * p.want + at] = key;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------- pool -> top-k
+__device__ void bitonic_lds(uint64_t *buf, uint32_t m) {
+    for (uint32_t k = 2; k <= m; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < (m >> 1); i += 256) {
+                const uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                const uint32_t hi = lo + j;
+                const uint64_t a = buf[lo], b = buf[hi];
+                const bool up = (lo & k) == 0;
+                if ((a > b) == up) {
+                    buf[lo] = b;
+                    buf[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+}
+
+__global__ __launch_bounds__(256) void k_sort_topk(const uint64_t *pool, const uint32_t *pool_cnt,
+                                                   uint32_t want, uint32_t k, uint64_t *out,
+                                                   uint32_t *out_cnt) {
+    __shared__ uint64_t buf[kMaxK];
+    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    const uint32_t c = min(pool_cnt[q], want);
+    uint32_t m = 2;
+    while (m < c) m <<= 1;
+    for (uint32_t i = tid; i < m; i += 256) buf[i] = i < c ? pool[(size_t)q * want + i] : kNoKey;
+    __syncthreads();
+    bitonic_lds(buf, m);
+    const uint32_t kk = min(k, c);
+    for (uint32_t i = tid; i < kk; i += 256) out[(size_t)q * k + i] = buf[i];
+    if (tid == 0) out_cnt[q] = kk;
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) {
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scan_mfma, dim3(grid), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_pool_select(const uint16_t *keys, const uint64_t *seg_off, uint32_t nq,
+                              uint32_t want, uint32_t *bits, uint32_t *pool_want, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pool_select, dim3(nq), dim3(256), 0, s, keys, seg_off, want, bits,
+                       pool_want);
+    return hipGetLastError();
+}
+
+hipError_t launch_rerank_items(const MixedParams &p, uint32_t grid, hipStream_t s) {
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rerank_items, dim3(grid), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_topk(const uint64_t *pool, const uint32_t *pool_cnt, uint32_t nq,
+                            uint32_t want, uint32_t k, uint64_t *out, uint32_t *out_cnt,
+                            hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sort_topk, dim3(nq), dim3(256), 0, s, pool, pool_cnt, want, k, out,
+                       out_cnt);
+    return hipGetLastError();
+}
+
+}  // namespace spf
