@@ -488,7 +488,8 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     if (want > kMaxK) return fail(SPF_EINVAL, "k / rerank_size above 2048");
     const uint32_t dpad = ix->dpad;
     const uint32_t dpad32 = round_up(ix->dim, 32);
-    const uint32_t G = mixed ? 32 : (dpad <= 1024 ? 16 : 4);
+    const uint32_t Gv = dpad <= 1024 ? 16 : 4;   // queries per VALU scan item
+    const uint32_t G = mixed ? 32 : Gv;          // queries per posting-scan item
     const bool prof = opts && (opts->flags & SPF_FLAG_PROFILE);
     if (prof)
         for (auto &e : w->ev)
@@ -567,9 +568,9 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         sp.nq = nq;
         sp.n_echunks = (P + kEChunk - 1) / kEChunk;
         sp.out_stride = P;
-        const uint64_t items = (uint64_t)((nq + G - 1) / G) * sp.n_echunks;
+        const uint64_t items = (uint64_t)((nq + Gv - 1) / Gv) * sp.n_echunks;
         const uint32_t grid = (uint32_t)std::min<uint64_t>(items, 16384);
-        HIPCHK(launch_scan(CK_F32, ix->metric, true, G, sp, grid, s));
+        HIPCHK(launch_scan(CK_F32, ix->metric, true, Gv, sp, grid, s));
     }
     HIPCHK(mark(SPF_STAGE_PROBE_SELECT));
     HIPCHK(launch_select(w->probe_keys.p, nullptr, P, nq, npr, w->probe_sel.p, w->probe_cnt.p, s));
