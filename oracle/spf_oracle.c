@@ -71,21 +71,25 @@ float spf_o_f16_to_f32(uint16_t h) {
 }
 
 /* ---- Distances (src/metric.rs:16-39).  numkong's reduction order is not
- * public; the canonical order is a sequential FMA chain from +0. ---- */
+ * public.  The canonical order: four FMA chains from +0, chain r taking the
+ * dimensions i with (i / 4) % 4 == r in increasing i, combined as
+ * (c0 + c1) + (c2 + c3). ---- */
+static inline float combine4(const float c[4]) { return (c[0] + c[1]) + (c[2] + c[3]); }
+
 float spf_o_l2sq(const float *a, const float *b, uint32_t d) {
-    float acc = 0.0f;
+    float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t n = (g_fault == 1 && d > 0) ? d - 1 : d;
     for (uint32_t i = 0; i < n; i++) {
         float t = a[i] - b[i];
-        acc = fmaf(t, t, acc);
+        c[(i >> 2) & 3] = fmaf(t, t, c[(i >> 2) & 3]);
     }
-    return acc;
+    return combine4(c);
 }
 
 float spf_o_dot(const float *a, const float *b, uint32_t d) {
-    float acc = 0.0f;
-    for (uint32_t i = 0; i < d; i++) acc = fmaf(a[i], b[i], acc);
-    return acc;
+    float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t i = 0; i < d; i++) c[(i >> 2) & 3] = fmaf(a[i], b[i], c[(i >> 2) & 3]);
+    return combine4(c);
 }
 
 /* Angular distance: numkong's special cases for zero norms are restated as
@@ -354,14 +358,15 @@ static float code_dist(const spf_o_index *ix, const float *q, const uint8_t *cod
         return spf_o_eval(ix->metric, q, (const float *)code, d);
     case SPF_O_F16: {
         if (l2 || l2sq) {
-            float acc = 0.0f;
+            float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             uint32_t n = (g_fault == 1 && d > 0) ? d - 1 : d;
             for (uint32_t i = 0; i < n; i++) {
                 uint16_t h;
                 memcpy(&h, code + 2 * i, 2);
                 float t = spf_o_f16_to_f32(pp->q16[i]) - spf_o_f16_to_f32(h);
-                acc = fmaf(t, t, acc);
+                c[(i >> 2) & 3] = fmaf(t, t, c[(i >> 2) & 3]);
             }
+            float acc = combine4(c);
             return l2 ? sqrtf(acc) : acc;
         }
         for (uint32_t i = 0; i < d; i++) {

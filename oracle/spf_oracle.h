@@ -18,8 +18,9 @@
  * crate or to container internals (numkong 7.8.2 accumulation order, BinaryHeap
  * tie order, HashMap probe order) are fixed here and shared with the device
  * kernels:
- *   - every f32 reduction is a sequential fused multiply-add chain from +0
- *     over dimensions 0..d-1;
+ *   - every f32 reduction (sum of squares, dot product) runs as four fused
+ *     multiply-add chains from +0, chain r taking dimensions i with
+ *     (i / 4) % 4 == r in increasing order, combined as (c0 + c1) + (c2 + c3);
  *   - equal distances are ordered by id (posting candidates) or by reference
  *     block index (centroid probe).
  */

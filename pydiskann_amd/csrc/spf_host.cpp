@@ -73,7 +73,7 @@ struct Workspace {
     DBuf<uint64_t> probe_keys, probe_sel;
     DBuf<uint32_t> probe_cnt, np_q, probe_post;
     DBuf<uint64_t> cand_len, cand_off, qp_slot, pair_slot;
-    DBuf<uint32_t> post_cnt, post_fill, pair_off, pair_q, item_cnt, item_off;
+    DBuf<uint32_t> post_cnt, post_fill, pair_off, pair_q, item_cnt, item_off, item_post;
     DBuf<uint64_t> cand, sel, fin;
     DBuf<uint32_t> sel_cnt, fin_cnt;
     DBuf<uint64_t> out_ids;
@@ -81,7 +81,7 @@ struct Workspace {
     DBuf<uint32_t> out_c;
     DBuf<uint16_t> q16, cand16;
     DBuf<float> qn16;
-    DBuf<uint32_t> bits, pool_cnt, pool_want;
+    DBuf<uint32_t> bits, bit_prefix, pool_want, cand_cnt;
     DBuf<uint64_t> pool;
     uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
     hipEvent_t ev[SPF_N_STAGES + 1] = {};
@@ -90,14 +90,16 @@ struct Workspace {
         q16.release();
         cand16.release();
         bits.release();
-        pool_cnt.release();
+        bit_prefix.release();
+        cand_cnt.release();
         pool_want.release();
         pool.release();
         qn16.release();
         DBuf<float> *fs[] = {&qin, &qv_probe, &qv_scan, &qnorm, &out_d};
         for (auto *b : fs) b->release();
         DBuf<uint32_t> *us[] = {&q8, &qsq, &probe_cnt, &np_q, &probe_post, &post_cnt, &post_fill,
-                                &pair_off, &pair_q, &item_cnt, &item_off, &sel_cnt, &fin_cnt,
+                                &pair_off, &pair_q, &item_cnt, &item_off, &item_post, &sel_cnt,
+                                &fin_cnt,
                                 &out_c};
         for (auto *b : us) b->release();
         DBuf<uint64_t> *ls[] = {&probe_keys, &probe_sel, &cand_len, &cand_off, &qp_slot,
@@ -278,8 +280,15 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
     ix->rerank_size = d->rerank_size;
     ix->code_kind = d->quantizer == SPF_QUANT_F16 ? CK_F16
                     : d->quantizer == SPF_QUANT_INT8 ? CK_U8 : CK_F32;
-    ix->units = (d->code_bytes + 15) / 16;
-    ix->cent_units = (4 * d->dim + 15) / 16;
+    // every tiled layout covers dpad dims (zero padded), so f32 codes hold a
+    // multiple of 4 units and f16 codes an even count: the scans' four
+    // reduction chains then map to compile-time unit positions
+    switch (d->quantizer) {
+    case SPF_QUANT_NONE: ix->units = ix->dpad / 4; break;
+    case SPF_QUANT_F16: ix->units = ix->dpad / 8; break;
+    case SPF_QUANT_INT8: ix->units = ix->dpad / 16; break;
+    }
+    ix->cent_units = ix->dpad / 4;
     std::unique_ptr<spf_index> guard(ix);
 
     // postings sorted by reference block index: equal centroid distances then
@@ -357,7 +366,7 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         ix->d_codes16 = ix->d_codes;
         ix->units16 = ix->units;
     } else if (d->quantizer == SPF_QUANT_NONE) {
-        ix->units16 = (2 * d->dim + 15) / 16;
+        ix->units16 = ix->dpad / 8;
         std::vector<_Float16> h((size_t)N * d->dim);
         for (uint64_t e = 0; e < N; e++) {
             const float *src = (const float *)(d->codes + (size_t)entry_src[e] * d->code_bytes);
@@ -420,13 +429,14 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     const uint32_t dpad32 = round_up(ix->dim, 32);
     HIPCHK(w->cand16.ensure(std::max<uint64_t>(total, 1)));
     HIPCHK(w->bits.ensure(total / 32 + 4));
+    HIPCHK(w->bit_prefix.ensure(total / 32 + 4));
     HIPCHK(w->pool.ensure((size_t)nq * want));
-    HIPCHK(w->pool_cnt.ensure(nq));
     HIPCHK(w->pool_want.ensure(nq));
     HIPCHK(w->fin.ensure((size_t)nq * k));
     HIPCHK(w->fin_cnt.ensure(nq));
     MixedParams mp{};
     mp.item_off = w->item_off.p;
+    mp.item_post = w->item_post.p;
     mp.n_post = ix->P;
     mp.n_items = n_items;
     mp.post_off = ix->d_post_off;
@@ -443,8 +453,8 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     mp.del = ix->d_del;
     mp.out16 = w->cand16.p;
     mp.bits = w->bits.p;
+    mp.bit_prefix = w->bit_prefix.p;
     mp.pool = w->pool.p;
-    mp.pool_cnt = w->pool_cnt.p;
     mp.want = want;
     mp.metric = ix->metric;
     mp.dpad = ix->dpad;
@@ -456,12 +466,11 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     HIPCHK(mark(SPF_STAGE_SCAN));
     HIPCHK(launch_scan_mfma(mp, grid, s));
     HIPCHK(mark(SPF_STAGE_SELECT));
-    HIPCHK(hipMemsetAsync(w->bits.p, 0, (total / 32 + 4) * 4, s));
-    HIPCHK(hipMemsetAsync(w->pool_cnt.p, 0, nq * 4, s));
-    HIPCHK(launch_pool_select(w->cand16.p, w->cand_off.p, nq, want, w->bits.p, w->pool_want.p, s));
+    HIPCHK(launch_pool_select(w->cand16.p, w->cand_off.p, w->cand_cnt.p, nq, want, w->bits.p,
+                              w->bit_prefix.p, w->pool_want.p, s));
     HIPCHK(mark(SPF_STAGE_RERANK));
     HIPCHK(launch_rerank_items(mp, grid, s));
-    HIPCHK(launch_sort_topk(w->pool.p, w->pool_cnt.p, nq, want, k, w->fin.p, w->fin_cnt.p, s));
+    HIPCHK(launch_sort_topk(w->pool.p, w->pool_want.p, nq, want, k, w->fin.p, w->fin_cnt.p, s));
     HIPCHK(mark(SPF_STAGE_FINALIZE));
     HIPCHK(launch_finalize(w->fin.p, k, w->fin_cnt.p, nq, k, ix->d_rank_id, d_ids, d_dists,
                            d_counts, s));
@@ -573,13 +582,15 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         HIPCHK(launch_scan(CK_F32, ix->metric, true, Gv, sp, grid, s));
     }
     HIPCHK(mark(SPF_STAGE_PROBE_SELECT));
-    HIPCHK(launch_select(w->probe_keys.p, nullptr, P, nq, npr, w->probe_sel.p, w->probe_cnt.p, s));
+    HIPCHK(launch_select(w->probe_keys.p, nullptr, nullptr, P, nq, npr, w->probe_sel.p,
+                         w->probe_cnt.p, s));
     HIPCHK(mark(SPF_STAGE_PLAN));
 
     // ---- plan: probe_ratio cut, candidate offsets, posting-major pairs
     HIPCHK(w->np_q.ensure(nq));
     HIPCHK(w->probe_post.ensure((size_t)nq * npr));
     HIPCHK(w->cand_len.ensure(nq + 1));
+    HIPCHK(w->cand_cnt.ensure(nq));
     HIPCHK(w->cand_off.ensure(nq + 1));
     HIPCHK(w->post_cnt.ensure(P + 1));
     HIPCHK(w->post_fill.ensure(P + 1));
@@ -600,6 +611,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.np_q = w->np_q.p;
     pl.probe_post = w->probe_post.p;
     pl.cand_len = w->cand_len.p;
+    pl.cand_cnt = w->cand_cnt.p;
     pl.cand_off = w->cand_off.p;
     pl.post_cnt = w->post_cnt.p;
     pl.post_fill = w->post_fill.p;
@@ -626,6 +638,10 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(hipStreamSynchronize(s));
     const uint64_t total = w->h_summary[0];
     const uint32_t n_items = (uint32_t)w->h_summary[1];
+    HIPCHK(w->item_post.ensure(std::max<uint32_t>(n_items, 1)));
+    pl.item_off = w->item_off.p;
+    pl.item_post = w->item_post.p;
+    HIPCHK(launch_item_table(pl, s));
 
     if (mixed) {
         int rc = search_mixed(ix, w, s, nq, k, want, total, n_items, d_ids, d_dists, d_counts,
@@ -653,6 +669,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         sp.offset = ix->offset;
         sp.out = w->cand.p;
         sp.item_off = w->item_off.p;
+        sp.item_post = w->item_post.p;
         sp.n_post = P;
         sp.n_items = n_items;
         sp.post_off = ix->d_post_off;
@@ -664,7 +681,8 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     }
     // ---- top-`want` per query
     HIPCHK(mark(SPF_STAGE_SELECT));
-    HIPCHK(launch_select(w->cand.p, w->cand_off.p, 0, nq, want, w->sel.p, w->sel_cnt.p, s));
+    HIPCHK(launch_select(w->cand.p, w->cand_off.p, w->cand_cnt.p, 0, nq, want, w->sel.p,
+                         w->sel_cnt.p, s));
     HIPCHK(mark(SPF_STAGE_RERANK));
     if (rerank) {
         HIPCHK(w->fin.ensure((size_t)nq * k));

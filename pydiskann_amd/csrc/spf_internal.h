@@ -38,6 +38,7 @@ struct ScanParams {
     uint64_t out_stride;
     // listed mode (posting scan): work items built by the planner
     const uint32_t *item_off;   // [n_post + 1]
+    const uint32_t *item_post;  // [n_items] posting of each work item
     uint32_t n_post;
     uint32_t n_items;
     const uint32_t *post_off;   // [n_post + 1] entry offsets
@@ -65,6 +66,7 @@ struct PrepParams {
 // Mixed-precision scan / rerank over the same posting-major work items.
 struct MixedParams {
     const uint32_t *item_off;   // [n_post + 1]
+    const uint32_t *item_post;  // [n_items]
     uint32_t n_post, n_items;
     const uint32_t *post_off;   // [n_post + 1]
     const uint32_t *pair_off;   // [n_post + 1]
@@ -78,9 +80,9 @@ struct MixedParams {
     const float *xn16;          // |x16|^2 per entry
     const uint32_t *del;
     uint16_t *out16;            // 16-bit candidate keys
-    uint32_t *bits;             // pool marks over candidate slots
+    const uint32_t *bits;       // pool marks over candidate slots (segments 32-aligned)
+    const uint32_t *bit_prefix; // per bitmap word: marks before it in its query's segment
     uint64_t *pool;             // [nq][want] reranked keys
-    uint32_t *pool_cnt;         // [nq]
     uint32_t want, metric, dpad;
     const float *qv;            // [nq][dpad] f32 queries
     const float *qnorm;         // [nq] |q|^2 (cosine)
@@ -96,7 +98,8 @@ struct PlanParams {
     const uint32_t *post_len;   // [n_post]
     uint32_t *np_q;             // [nq] probes kept after the probe_ratio cut
     uint32_t *probe_post;       // [nq][npr]
-    uint64_t *cand_len;         // [nq + 1]
+    uint64_t *cand_len;         // [nq + 1] segment length, rounded up to 32 slots
+    uint32_t *cand_cnt;         // [nq] candidates actually written in the segment
     const uint64_t *cand_off;   // [nq + 1]
     uint32_t *post_cnt;         // [n_post + 1]
     uint32_t *post_fill;        // [n_post]
@@ -105,6 +108,8 @@ struct PlanParams {
     uint32_t *pair_q;           // [nq * npr] posting-major pairs
     uint64_t *pair_slot;        // [nq * npr]
     uint32_t *item_cnt;         // [n_post + 1]
+    const uint32_t *item_off;   // [n_post + 1]
+    uint32_t *item_post;        // [n_items]
     uint32_t G;                 // queries per work item
 };
 
@@ -125,13 +130,14 @@ struct RerankParams {
 hipError_t launch_prep(const PrepParams &p, hipStream_t s);
 hipError_t launch_scan(uint32_t code_kind, uint32_t metric, bool dense, uint32_t G,
                        const ScanParams &p, uint32_t grid, hipStream_t s);
-hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, uint64_t seg_stride,
-                         uint32_t nseg, uint32_t K, uint64_t *out, uint32_t *out_cnt,
-                         hipStream_t s);
+hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, const uint32_t *seg_cnt,
+                         uint64_t seg_stride, uint32_t nseg, uint32_t K, uint64_t *out,
+                         uint32_t *out_cnt, hipStream_t s);
 hipError_t launch_plan_probe(const PlanParams &p, hipStream_t s);
 hipError_t launch_plan_pairs(const PlanParams &p, hipStream_t s);
 hipError_t launch_scatter_pairs(const PlanParams &p, hipStream_t s);
 hipError_t launch_items(const PlanParams &p, hipStream_t s);
+hipError_t launch_item_table(const PlanParams &p, hipStream_t s);
 hipError_t launch_excl_scan_u64(const uint64_t *in, uint64_t *out, uint32_t n, hipStream_t s);
 hipError_t launch_excl_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t s);
 hipError_t launch_rerank(const RerankParams &p, hipStream_t s);
@@ -145,8 +151,9 @@ hipError_t launch_entry_norms(uint32_t code_kind, const uint4 *data, uint32_t un
 
 hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t s);
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s);
-hipError_t launch_pool_select(const uint16_t *keys, const uint64_t *seg_off, uint32_t nq,
-                              uint32_t want, uint32_t *bits, uint32_t *pool_want, hipStream_t s);
+hipError_t launch_pool_select(const uint16_t *keys, const uint64_t *seg_off,
+                              const uint32_t *seg_cnt, uint32_t nq, uint32_t want, uint32_t *bits,
+                              uint32_t *bit_prefix, uint32_t *pool_want, hipStream_t s);
 hipError_t launch_rerank_items(const MixedParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_sort_topk(const uint64_t *pool, const uint32_t *pool_cnt, uint32_t nq,
                             uint32_t want, uint32_t k, uint64_t *out, uint32_t *out_cnt,

@@ -100,9 +100,9 @@ __global__ __launch_bounds__(64) void k_prep(const PrepParams p) {
     }
     __syncthreads();
     if (lane == 0) {
-        float acc = 0.0f;
-        for (uint32_t i = 0; i < p.dim; i++) acc = __builtin_fmaf(s_v[i], s_v[i], acc);
-        p.qnorm[q] = acc;
+        float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t i = 0; i < p.dim; i++) c[(i >> 2) & 3] = __builtin_fmaf(s_v[i], s_v[i], c[(i >> 2) & 3]);
+        p.qnorm[q] = (c[0] + c[1]) + (c[2] + c[3]);
         if (p.q16) {
             float a16 = 0.0f;
             for (uint32_t i = 0; i < p.dim; i++) {
@@ -152,12 +152,7 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
                 s_slot[tid] = (uint64_t)(q0 + tid) * p.out_stride;
             }
         } else {
-            uint32_t lo = 0, hi = p.n_post;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (p.item_off[mid] <= item) lo = mid; else hi = mid;
-            }
-            const uint32_t pp = lo;
+            const uint32_t pp = p.item_post[item];
             const uint32_t local = item - p.item_off[pp];
             const uint32_t pb = p.post_off[pp], pe = p.post_off[pp + 1];
             const uint32_t nec = (pe - pb + kEChunk - 1) / kEChunk;
@@ -195,48 +190,61 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
             const uint32_t e = e0 + lane;
             const bool valid = e < e_end;
             const uint4 *src = p.data + (size_t)(e >> 6) * p.units * 64 + (e & 63);
-            float accf[G];
+            // four FMA chains per query: chain r takes dims i with (i/4)%4 == r
+            float accf[G][4];
             uint32_t acci[G];
 #pragma unroll
-            for (int g = 0; g < G; g++) { accf[g] = 0.0f; acci[g] = 0u; }
+            for (int g = 0; g < G; g++) {
+                accf[g][0] = accf[g][1] = accf[g][2] = accf[g][3] = 0.0f;
+                acci[g] = 0u;
+            }
 
-            if (CK == CK_F32) {
+            if (CK == CK_F32) {   // unit u = dims 4u..4u+3 -> chain u%4 (units % 4 == 0)
                 const float4 *s4 = reinterpret_cast<const float4 *>(s_q);
-#pragma unroll 2
-                for (uint32_t u = 0; u < p.units; u++) {
-                    const uint4 w = valid ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
-                    const float x0 = __uint_as_float(w.x), x1 = __uint_as_float(w.y);
-                    const float x2 = __uint_as_float(w.z), x3 = __uint_as_float(w.w);
+                for (uint32_t v = 0; v < p.units; v += 4) {
+                    uint4 w[4];
 #pragma unroll
-                    for (int g = 0; g < G; g++) {
-                        const float4 qq = s4[g * dq4 + u];
-                        step<MET>(accf[g], qq.x, x0);
-                        step<MET>(accf[g], qq.y, x1);
-                        step<MET>(accf[g], qq.z, x2);
-                        step<MET>(accf[g], qq.w, x3);
+                    for (int r = 0; r < 4; r++)
+                        w[r] = valid ? src[(size_t)(v + r) * 64] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const float x0 = __uint_as_float(w[r].x), x1 = __uint_as_float(w[r].y);
+                        const float x2 = __uint_as_float(w[r].z), x3 = __uint_as_float(w[r].w);
+#pragma unroll
+                        for (int g = 0; g < G; g++) {
+                            const float4 qq = s4[g * dq4 + v + r];
+                            step<MET>(accf[g][r], qq.x, x0);
+                            step<MET>(accf[g][r], qq.y, x1);
+                            step<MET>(accf[g][r], qq.z, x2);
+                            step<MET>(accf[g][r], qq.w, x3);
+                        }
                     }
                 }
-            } else if (CK == CK_F16) {
+            } else if (CK == CK_F16) {   // unit u = dims 8u..8u+7 -> chains 2(u%2), 2(u%2)+1
                 const float4 *s4 = reinterpret_cast<const float4 *>(s_q);
-#pragma unroll 2
-                for (uint32_t u = 0; u < p.units; u++) {
-                    union { uint4 v; _Float16 h[8]; } cv;
-                    cv.v = valid ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
-                    float x[8];
+                for (uint32_t v = 0; v < p.units; v += 2) {
+                    union { uint4 v; _Float16 h[8]; } cv[2];
 #pragma unroll
-                    for (int j = 0; j < 8; j++) x[j] = (float)cv.h[j];
+                    for (int r = 0; r < 2; r++)
+                        cv[r].v = valid ? src[(size_t)(v + r) * 64] : make_uint4(0, 0, 0, 0);
 #pragma unroll
-                    for (int g = 0; g < G; g++) {
-                        const float4 qa = s4[g * dq4 + 2 * u];
-                        const float4 qb = s4[g * dq4 + 2 * u + 1];
-                        step<MET>(accf[g], qa.x, x[0]);
-                        step<MET>(accf[g], qa.y, x[1]);
-                        step<MET>(accf[g], qa.z, x[2]);
-                        step<MET>(accf[g], qa.w, x[3]);
-                        step<MET>(accf[g], qb.x, x[4]);
-                        step<MET>(accf[g], qb.y, x[5]);
-                        step<MET>(accf[g], qb.z, x[6]);
-                        step<MET>(accf[g], qb.w, x[7]);
+                    for (int r = 0; r < 2; r++) {
+                        float x[8];
+#pragma unroll
+                        for (int j = 0; j < 8; j++) x[j] = (float)cv[r].h[j];
+#pragma unroll
+                        for (int g = 0; g < G; g++) {
+                            const float4 qa = s4[g * dq4 + 2 * (v + r)];
+                            const float4 qb = s4[g * dq4 + 2 * (v + r) + 1];
+                            step<MET>(accf[g][2 * r], qa.x, x[0]);
+                            step<MET>(accf[g][2 * r], qa.y, x[1]);
+                            step<MET>(accf[g][2 * r], qa.z, x[2]);
+                            step<MET>(accf[g][2 * r], qa.w, x[3]);
+                            step<MET>(accf[g][2 * r + 1], qb.x, x[4]);
+                            step<MET>(accf[g][2 * r + 1], qb.y, x[5]);
+                            step<MET>(accf[g][2 * r + 1], qb.z, x[6]);
+                            step<MET>(accf[g][2 * r + 1], qb.w, x[7]);
+                        }
                     }
                 }
             } else if (INT_L2) {
@@ -259,7 +267,7 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
                 const float4 *s4 = reinterpret_cast<const float4 *>(s_q);
                 const float sc = p.scale, of = p.offset;
 #pragma unroll 1
-                for (uint32_t u = 0; u < p.units; u++) {
+                for (uint32_t u = 0; u < p.units; u++) {   // unit u = dims 16u.. -> chains 0..3
                     const uint4 w = valid ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
                     const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
                     float x[16];
@@ -274,10 +282,10 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
 #pragma unroll
                         for (int j4 = 0; j4 < 4; j4++) {
                             const float4 qq = s4[g * dq4 + 4 * u + j4];
-                            step<MET>(accf[g], qq.x, x[4 * j4 + 0]);
-                            step<MET>(accf[g], qq.y, x[4 * j4 + 1]);
-                            step<MET>(accf[g], qq.z, x[4 * j4 + 2]);
-                            step<MET>(accf[g], qq.w, x[4 * j4 + 3]);
+                            step<MET>(accf[g][j4], qq.x, x[4 * j4 + 0]);
+                            step<MET>(accf[g][j4], qq.y, x[4 * j4 + 1]);
+                            step<MET>(accf[g][j4], qq.z, x[4 * j4 + 2]);
+                            step<MET>(accf[g][j4], qq.w, x[4 * j4 + 3]);
                         }
                     }
                 }
@@ -299,7 +307,8 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
                             v = v * p.scale;
                             dist = (MET == M_L2) ? sqrtf(v) : v;
                         } else {
-                            dist = finish_f<MET>(accf[g], MET == M_COS ? s_qn[g] : 0.0f, xn);
+                            const float acc = (accf[g][0] + accf[g][1]) + (accf[g][2] + accf[g][3]);
+                            dist = finish_f<MET>(acc, MET == M_COS ? s_qn[g] : 0.0f, xn);
                         }
                         const uint64_t key = dead ? kNoKey : (((uint64_t)fkey(dist) << 32) | low);
                         p.out[s_slot[g] + (e - base_e)] = key;
@@ -348,8 +357,8 @@ constexpr uint32_t kSelDirect = 1024;   // segments this short are sorted outrig
 constexpr uint32_t kSelSlack = 256;     // extra keys tolerated in the final sort
 
 __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint64_t *seg_off,
-                                                uint64_t seg_stride, uint32_t K, uint64_t *out,
-                                                uint32_t *out_cnt) {
+                                                const uint32_t *seg_cnt, uint64_t seg_stride,
+                                                uint32_t K, uint64_t *out, uint32_t *out_cnt) {
     __shared__ uint64_t buf[kSelCap];
     __shared__ uint32_t hist[2048];
     __shared__ uint32_t part[kWG];
@@ -357,7 +366,7 @@ __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint
     const uint32_t seg = blockIdx.x, tid = threadIdx.x;
     const uint64_t b = seg_off ? seg_off[seg] : (uint64_t)seg * seg_stride;
     const uint64_t e = seg_off ? seg_off[seg + 1] : b + seg_stride;
-    const uint64_t n = e - b;
+    const uint64_t n = seg_cnt ? (uint64_t)seg_cnt[seg] : e - b;
     const uint64_t *kp = keys + b;
     uint64_t *o = out + (size_t)seg * K;
 
@@ -379,13 +388,43 @@ __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint
         for (uint32_t i = tid; i < n; i += kWG) buf[i] = kp[i];
         m = (uint32_t)n;
     } else {
-        const int shifts[6] = {53, 42, 32, 21, 10, 0};
-        const int widths[6] = {11, 11, 10, 11, 11, 10};
-        uint64_t prefix = 0, mask = 0;
+        // digits start at the highest bit where the segment's valid keys differ,
+        // so the first histogram spreads over bins instead of piling into one
+        uint64_t lo = kNoKey, hi = 0;
+        for (uint64_t i = tid; i < n; i += kWG) {
+            const uint64_t k = kp[i];
+            if (k != kNoKey) {
+                lo = k < lo ? k : lo;
+                hi = k > hi ? k : hi;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t a = __shfl_xor(lo, off, 64), b2 = __shfl_xor(hi, off, 64);
+            lo = a < lo ? a : lo;
+            hi = b2 > hi ? b2 : hi;
+        }
+        __shared__ uint64_t s_lh[2][4];
+        if ((tid & 63) == 0) {
+            s_lh[0][tid >> 6] = lo;
+            s_lh[1][tid >> 6] = hi;
+        }
+        __syncthreads();
+        lo = s_lh[0][0];
+        hi = s_lh[1][0];
+        for (int w = 1; w < 4; w++) {
+            lo = s_lh[0][w] < lo ? s_lh[0][w] : lo;
+            hi = s_lh[1][w] > hi ? s_lh[1][w] : hi;
+        }
+        const int hb = lo == hi ? -1 : 63 - __clzll(lo ^ hi);
+        uint64_t mask = hb < 0 ? ~0ull : (hb >= 63 ? 0ull : ~((2ull << hb) - 1ull));
+        uint64_t prefix = lo & mask;
         uint32_t rem = Kv;
-        for (int pass = 0; pass < 6; pass++) {
-            const int sh = shifts[pass];
-            const uint32_t dm = (1u << widths[pass]) - 1u;
+        int top = hb;
+        __syncthreads();
+        while (top >= 0) {
+            const int wdt = top + 1 < 11 ? top + 1 : 11;
+            const int sh = top + 1 - wdt;
+            const uint32_t dm = (1u << wdt) - 1u;
             for (uint32_t i = tid; i < 2048; i += kWG) hist[i] = 0;
             __syncthreads();
             for (uint64_t i = tid; i < n; i += kWG) {
@@ -393,14 +432,14 @@ __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint
                 if ((k & mask) == prefix) atomicAdd(&hist[(uint32_t)(k >> sh) & dm], 1u);
             }
             __syncthreads();
-            uint32_t loc[8], s = 0;
+            uint32_t loc[8], sm = 0;
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 loc[j] = hist[tid * 8 + j];
-                s += loc[j];
+                sm += loc[j];
             }
-            const uint32_t excl = block_excl_scan256(s, part);
-            if (excl < rem && rem <= excl + s) {
+            const uint32_t excl = block_excl_scan256(sm, part);
+            if (excl < rem && rem <= excl + sm) {
                 uint32_t c = excl;
                 for (int j = 0; j < 8; j++) {
                     if (rem <= c + loc[j]) {
@@ -418,6 +457,7 @@ __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint
             rem -= below;
             prefix |= (uint64_t)bin << sh;
             mask |= (uint64_t)dm << sh;
+            top = sh - 1;
             if ((Kv - rem) + cnt <= Kv + kSelSlack) break;
         }
         if (tid == 0) s_u[0] = 0;
@@ -464,7 +504,8 @@ __global__ void k_plan_probe(const PlanParams p) {
         }
     }
     p.np_q[q] = kept;
-    p.cand_len[q] = total;
+    p.cand_cnt[q] = (uint32_t)total;
+    p.cand_len[q] = (total + 31) & ~31ull;   // 32-aligned segments: a query owns whole bitmap words
 }
 
 // (query, probe) pairs: candidate-buffer slot of each probed posting, and a
@@ -500,6 +541,13 @@ __global__ void k_items(const PlanParams p) {
     const uint32_t cnt = p.pair_off[b + 1] - p.pair_off[b];
     const uint32_t len = p.post_len[b];
     p.item_cnt[b] = (cnt && len) ? ((cnt + p.G - 1) / p.G) * ((len + kEChunk - 1) / kEChunk) : 0u;
+}
+
+// work item -> posting (one thread per posting)
+__global__ void k_item_table(const PlanParams p) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= p.n_post) return;
+    for (uint32_t i = p.item_off[b]; i < p.item_off[b + 1]; i++) p.item_post[i] = b;
 }
 
 template <typename T>
@@ -538,40 +586,67 @@ __global__ __launch_bounds__(1024) void k_excl_scan(const T *in, T *out, uint32_
 }
 
 // ---------------------------------------------------------------- rerank
-template <int MET>
-__device__ float exact_row(const float *q, const float *x, uint32_t dpad, float qn) {
-    const float4 *q4 = reinterpret_cast<const float4 *>(q);
-    const float4 *x4 = reinterpret_cast<const float4 *>(x);
+// Exact f32 distance with four lanes per candidate: lane r of the quad runs
+// chain r over units u = 4t + r (dims 16t+4r..16t+4r+3), so the quad reads 64
+// contiguous bytes per step; chains combine as (c0 + c1) + (c2 + c3).
+__device__ __forceinline__ float quad_combine(float c) {
+    const float c01 = c + __shfl_xor(c, 1, 64);       // lanes 0,1: c0+c1; lanes 2,3: c2+c3
+    return c01 + __shfl_xor(c01, 2, 64);               // (c0+c1) + (c2+c3) on every lane
+}
+
+__device__ __forceinline__ float quad_exact(uint32_t metric, const float4 *q4, const float4 *x4,
+                                            uint32_t units, uint32_t r, float qn) {
     float acc = 0.0f, b2 = 0.0f;
-    for (uint32_t u = 0; u < dpad / 4; u++) {
+    const bool l2 = metric == M_L2 || metric == M_L2SQ;
+    for (uint32_t u = r; u < units; u += 4) {
         const float4 a = q4[u], b = x4[u];
-        step<MET>(acc, a.x, b.x);
-        step<MET>(acc, a.y, b.y);
-        step<MET>(acc, a.z, b.z);
-        step<MET>(acc, a.w, b.w);
-        if (MET == M_COS) {
+        if (l2) {
+            float t = a.x - b.x; acc = __builtin_fmaf(t, t, acc);
+            t = a.y - b.y; acc = __builtin_fmaf(t, t, acc);
+            t = a.z - b.z; acc = __builtin_fmaf(t, t, acc);
+            t = a.w - b.w; acc = __builtin_fmaf(t, t, acc);
+        } else {
+            acc = __builtin_fmaf(a.x, b.x, acc);
+            acc = __builtin_fmaf(a.y, b.y, acc);
+            acc = __builtin_fmaf(a.z, b.z, acc);
+            acc = __builtin_fmaf(a.w, b.w, acc);
             b2 = __builtin_fmaf(b.x, b.x, b2);
             b2 = __builtin_fmaf(b.y, b.y, b2);
             b2 = __builtin_fmaf(b.z, b.z, b2);
             b2 = __builtin_fmaf(b.w, b.w, b2);
         }
     }
-    return finish_f<MET>(acc, qn, b2);
+    acc = quad_combine(acc);
+    switch (metric) {
+    case M_L2SQ: return acc;
+    case M_L2: return sqrtf(acc);
+    case M_DOT: return 1.0f - acc;
+    default:
+        b2 = quad_combine(b2);
+        return cosine_from(acc, qn, b2);
+    }
 }
 
-template <int MET>
 __global__ __launch_bounds__(256) void k_rerank(const RerankParams p) {
     __shared__ uint64_t buf[kMaxK];
+    extern __shared__ __align__(16) float s_qv[];   // dpad floats
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
     const uint32_t c = p.sel_cnt[q];
     const uint64_t *sel = p.sel + (size_t)q * p.want;
-    const float *qv = p.qv + (size_t)q * p.dpad;
+    for (uint32_t i = tid; i < p.dpad; i += kWG) s_qv[i] = p.qv[(size_t)q * p.dpad + i];
     const float qn = p.qnorm[q];
-    for (uint32_t i = tid; i < c; i += kWG) {
-        const uint32_t rank = (uint32_t)sel[i];
-        const uint32_t e = p.rank_entry[rank];
-        const float d = exact_row<MET>(qv, p.raw + (size_t)e * p.dpad, p.dpad, qn);
-        buf[i] = ((uint64_t)fkey(d) << 32) | rank;
+    __syncthreads();
+    const uint32_t r = tid & 3, units = p.dpad / 4;
+    const uint32_t nround = (c + 63) / 64;          // 64 quads per workgroup
+    for (uint32_t rd = 0; rd < nround; rd++) {
+        const uint32_t i = rd * 64 + (tid >> 2);
+        const bool ok = i < c;
+        const uint32_t rank = ok ? (uint32_t)sel[i] : 0u;
+        const uint32_t e = ok ? p.rank_entry[rank] : 0u;
+        const float d = quad_exact(p.metric, reinterpret_cast<const float4 *>(s_qv),
+                                   reinterpret_cast<const float4 *>(p.raw + (size_t)e * p.dpad),
+                                   units, r, qn);
+        if (ok && r == 0) buf[i] = ((uint64_t)fkey(d) << 32) | rank;
     }
     const uint32_t m2 = next_pow2(c < 2 ? 2 : c);
     for (uint32_t i = c + tid; i < m2; i += kWG) buf[i] = kNoKey;
@@ -608,20 +683,21 @@ __global__ void k_entry_norms(uint32_t ck, const uint4 *data, uint32_t units, ui
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const uint4 *src = data + (size_t)(e >> 6) * units * 64 + (e & 63);
-    float acc = 0.0f;
+    float c4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t si = 0, d = 0;
     for (uint32_t u = 0; u < units; u++) {
         const uint4 w = src[(size_t)u * 64];
         if (ck == CK_F32) {
             const float x[4] = {__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z),
                                 __uint_as_float(w.w)};
-            for (int j = 0; j < 4 && d < dim; j++, d++) acc = __builtin_fmaf(x[j], x[j], acc);
+            for (int j = 0; j < 4 && d < dim; j++, d++)
+                c4[(d >> 2) & 3] = __builtin_fmaf(x[j], x[j], c4[(d >> 2) & 3]);
         } else if (ck == CK_F16) {
             union { uint4 v; _Float16 h[8]; } cv;
             cv.v = w;
             for (int j = 0; j < 8 && d < dim; j++, d++) {
                 const float x = (float)cv.h[j];
-                acc = __builtin_fmaf(x, x, acc);
+                c4[(d >> 2) & 3] = __builtin_fmaf(x, x, c4[(d >> 2) & 3]);
             }
         } else {
             const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
@@ -630,11 +706,11 @@ __global__ void k_entry_norms(uint32_t ck, const uint4 *data, uint32_t units, ui
                 si += c * c;
                 float x = (float)c * scale;
                 x = x + offset;
-                acc = __builtin_fmaf(x, x, acc);
+                c4[(d >> 2) & 3] = __builtin_fmaf(x, x, c4[(d >> 2) & 3]);
             }
         }
     }
-    if (xnorm) xnorm[e] = acc;
+    if (xnorm) xnorm[e] = (c4[0] + c4[1]) + (c4[2] + c4[3]);
     if (xsq) xsq[e] = si;
 }
 
@@ -686,12 +762,12 @@ hipError_t launch_scan(uint32_t ck, uint32_t metric, bool dense, uint32_t G, con
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, uint64_t seg_stride,
-                         uint32_t nseg, uint32_t K, uint64_t *out, uint32_t *out_cnt,
-                         hipStream_t s) {
+hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, const uint32_t *seg_cnt,
+                         uint64_t seg_stride, uint32_t nseg, uint32_t K, uint64_t *out,
+                         uint32_t *out_cnt, hipStream_t s) {
     if (nseg == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_select, dim3(nseg), dim3(kWG), 0, s, keys, seg_off, seg_stride, K, out,
-                       out_cnt);
+    hipLaunchKernelGGL(k_select, dim3(nseg), dim3(kWG), 0, s, keys, seg_off, seg_cnt, seg_stride,
+                       K, out, out_cnt);
     return hipGetLastError();
 }
 
@@ -717,6 +793,11 @@ hipError_t launch_items(const PlanParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_item_table(const PlanParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_item_table, dim3((p.n_post + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
 hipError_t launch_excl_scan_u64(const uint64_t *in, uint64_t *out, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_excl_scan<unsigned long long>, dim3(1), dim3(1024), 0, s,
                        (const unsigned long long *)in, (unsigned long long *)out, n);
@@ -730,13 +811,7 @@ hipError_t launch_excl_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, h
 
 hipError_t launch_rerank(const RerankParams &p, hipStream_t s) {
     if (p.nq == 0) return hipSuccess;
-    switch (p.metric) {
-    case M_L2: hipLaunchKernelGGL(k_rerank<M_L2>, dim3(p.nq), dim3(kWG), 0, s, p); break;
-    case M_L2SQ: hipLaunchKernelGGL(k_rerank<M_L2SQ>, dim3(p.nq), dim3(kWG), 0, s, p); break;
-    case M_COS: hipLaunchKernelGGL(k_rerank<M_COS>, dim3(p.nq), dim3(kWG), 0, s, p); break;
-    case M_DOT: hipLaunchKernelGGL(k_rerank<M_DOT>, dim3(p.nq), dim3(kWG), 0, s, p); break;
-    default: return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL(k_rerank, dim3(p.nq), dim3(kWG), p.dpad * sizeof(float), s, p);
     return hipGetLastError();
 }
 

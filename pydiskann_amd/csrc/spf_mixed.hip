@@ -74,12 +74,7 @@ constexpr int kMixKS = 8;      // max K-steps of 32 dims held in registers (dim 
 __device__ __forceinline__ void item_decode(const MixedParams &p, uint32_t item, uint32_t &pp,
                                             uint32_t &pb, uint32_t &e_begin, uint32_t &e_end,
                                             uint32_t &pr0, uint32_t &gcnt) {
-    uint32_t lo = 0, hi = p.n_post;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (p.item_off[mid] <= item) lo = mid; else hi = mid;
-    }
-    pp = lo;
+    pp = p.item_post[item];
     const uint32_t local = item - p.item_off[pp];
     pb = p.post_off[pp];
     const uint32_t pe = p.post_off[pp + 1];
@@ -194,74 +189,122 @@ __device__ __forceinline__ void find_bin256(const uint32_t *hist, uint32_t rem, 
 }
 
 __global__ __launch_bounds__(256) void k_pool_select(const uint16_t *keys, const uint64_t *seg_off,
-                                                     uint32_t want, uint32_t *bits,
+                                                     const uint32_t *seg_cnt, uint32_t want,
+                                                     uint32_t *bits, uint32_t *bit_prefix,
                                                      uint32_t *pool_want) {
     __shared__ uint16_t s_k[kPoolLds];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t part[256];
     __shared__ uint32_t s_u[4];
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
-    const uint64_t b = seg_off[q];
-    const uint32_t n = (uint32_t)(seg_off[q + 1] - b);
+    const uint64_t b = seg_off[q];               // multiple of 32
+    const uint32_t n = seg_cnt[q];
     const uint16_t *kp = keys + b;
     const bool cached = n <= kPoolLds;
-    hist[tid] = 0;
-    if (tid == 0) s_u[2] = 0;
+    if (tid == 0) {
+        s_u[0] = 0xffff;
+        s_u[1] = 0;
+        s_u[2] = 0;
+    }
     __syncthreads();
-    uint32_t inval = 0;
+    uint32_t inval = 0, kmn = 0xffff, kmx = 0;
     for (uint32_t i = tid; i < n; i += 256) {
         const uint16_t k = kp[i];
         if (cached) s_k[i] = k;
-        if (k == 0xffff) inval++;
-        else atomicAdd(&hist[k >> 8], 1u);
+        if (k == 0xffff) {
+            inval++;
+        } else {
+            kmn = min(kmn, (uint32_t)k);
+            kmx = max(kmx, (uint32_t)k);
+        }
     }
     if (inval) atomicAdd(&s_u[2], inval);
+    if (kmn != 0xffff) atomicMin(&s_u[0], kmn);
+    if (kmx) atomicMax(&s_u[1], kmx);
     __syncthreads();
     const uint32_t nvalid = n - s_u[2];
     const uint32_t W = min(want, nvalid);
     if (tid == 0) pool_want[q] = W;
-    if (W == 0) return;
+    const bool all = W == nvalid;
     uint32_t T = 0, need = 0;
-    if (W != nvalid) {
-        find_bin256(hist, W, part, s_u);
-        const uint32_t b1 = s_u[0], below1 = s_u[1];
-        __syncthreads();
+    if (!all) {
+        // adaptive range: bins over [kmin, kmax] so the keys spread over 256 bins
+        const uint32_t kmin = s_u[0], kmax = s_u[1];
+        const uint32_t span = kmax - kmin;
+        const uint32_t sh = span < 256 ? 0 : 32 - __clz(span) - 8;
         hist[tid] = 0;
         __syncthreads();
         for (uint32_t i = tid; i < n; i += 256) {
             const uint16_t k = cached ? s_k[i] : kp[i];
-            if ((uint32_t)(k >> 8) == b1 && k != 0xffff) atomicAdd(&hist[k & 0xff], 1u);
+            if (k != 0xffff) atomicAdd(&hist[(k - kmin) >> sh], 1u);
         }
         __syncthreads();
-        find_bin256(hist, W - below1, part, s_u);
-        const uint32_t b2 = s_u[0], below2 = s_u[1];
-        T = (b1 << 8) | b2;
+        find_bin256(hist, W, part, s_u);
+        const uint32_t b1 = s_u[0], below1 = s_u[1];
+        __syncthreads();
+        uint32_t b2 = 0, below2 = 0;
+        if (sh > 0) {
+            hist[tid] = 0;
+            __syncthreads();
+            for (uint32_t i = tid; i < n; i += 256) {
+                const uint16_t k = cached ? s_k[i] : kp[i];
+                if (k != 0xffff && ((k - kmin) >> sh) == b1)
+                    atomicAdd(&hist[(k - kmin) & ((1u << sh) - 1u)], 1u);
+            }
+            __syncthreads();
+            find_bin256(hist, W - below1, part, s_u);
+            b2 = s_u[0];
+            below2 = s_u[1];
+        }
+        T = kmin + (b1 << sh) + b2;
         need = W - below1 - below2;   // keys equal to T to take, in slot order
     }
-    // collect: each thread owns a contiguous run of slots so tie ranks follow slot order
-    const bool all = W == nvalid;
-    const uint32_t run = (n + 255) / 256;
-    const uint32_t i0 = min(n, tid * run), i1 = min(n, i0 + run);
+    // each thread owns a run of whole words, so tie ranks and mark positions
+    // follow slot order and every word is written by exactly one thread
+    const uint32_t nwords = (n + 31) / 32;
+    const uint32_t run = (nwords + 255) / 256;
+    const uint32_t w0 = min(nwords, tid * run), w1 = min(nwords, w0 + run);
     uint32_t ties = 0;
     if (!all)
-        for (uint32_t i = i0; i < i1; i++) ties += (cached ? s_k[i] : kp[i]) == T;
+        for (uint32_t i = 32 * w0; i < min(n, 32 * w1); i++) ties += (cached ? s_k[i] : kp[i]) == T;
     uint32_t tr = excl_scan256(ties, part);
-    for (uint32_t i = i0; i < i1; i++) {
-        const uint16_t k = cached ? s_k[i] : kp[i];
-        bool sel;
-        if (all) {
-            sel = k != 0xffff;
-        } else if (k < T) {
-            sel = true;
-        } else if (k == T) {
-            sel = tr < need;
-            tr++;
-        } else {
-            sel = false;
+    uint32_t marks = 0;
+    uint32_t words[8];
+    const bool keep = run <= 8;
+    for (uint32_t w = w0; w < w1; w++) {
+        uint32_t word = 0;
+        for (uint32_t j = 0; j < 32; j++) {
+            const uint32_t i = 32 * w + j;
+            if (i >= n) break;
+            const uint16_t k = cached ? s_k[i] : kp[i];
+            bool sel;
+            if (all) {
+                sel = k != 0xffff;
+            } else if (k < T) {
+                sel = true;
+            } else if (k == T) {
+                sel = tr < need;
+                tr++;
+            } else {
+                sel = false;
+            }
+            word |= (uint32_t)sel << j;
         }
-        if (sel) {
-            const uint64_t bit = b + i;
-            atomicOr(&bits[bit >> 5], 1u << (bit & 31));
+        if (keep) words[w - w0] = word;
+        bits[(b >> 5) + w] = word;
+        marks += __popc(word);
+    }
+    uint32_t pre = excl_scan256(marks, part);
+    if (keep) {
+        for (uint32_t w = w0; w < w1; w++) {
+            bit_prefix[(b >> 5) + w] = pre;
+            pre += __popc(words[w - w0]);
+        }
+    } else {
+        __threadfence_block();
+        for (uint32_t w = w0; w < w1; w++) {
+            bit_prefix[(b >> 5) + w] = pre;
+            pre += __popc(bits[(b >> 5) + w]);
         }
     }
 }
@@ -269,13 +312,17 @@ __global__ __launch_bounds__(256) void k_pool_select(const uint16_t *keys, const
 // ---------------------------------------------------------------- rerank
 constexpr uint32_t kListCap = 2048;
 
-__device__ float exact_dist(uint32_t metric, const float *q, const float *x, uint32_t dpad,
-                            float qn) {
-    const float4 *q4 = reinterpret_cast<const float4 *>(q);
-    const float4 *x4 = reinterpret_cast<const float4 *>(x);
+__device__ __forceinline__ float quad_combine_m(float c) {
+    const float c01 = c + __shfl_xor(c, 1, 64);
+    return c01 + __shfl_xor(c01, 2, 64);
+}
+
+// exact f32 distance, four lanes per pair, canonical 4-chain order
+__device__ __forceinline__ float quad_exact_m(uint32_t metric, const float4 *q4, const float4 *x4,
+                                              uint32_t units, uint32_t r, float qn) {
     float acc = 0.0f, b2 = 0.0f;
     const bool l2 = metric == M_L2 || metric == M_L2SQ;
-    for (uint32_t u = 0; u < dpad / 4; u++) {
+    for (uint32_t u = r; u < units; u += 4) {
         const float4 a = q4[u], b = x4[u];
         if (l2) {
             float t = a.x - b.x; acc = __builtin_fmaf(t, t, acc);
@@ -293,15 +340,17 @@ __device__ float exact_dist(uint32_t metric, const float *q, const float *x, uin
             b2 = __builtin_fmaf(b.w, b.w, b2);
         }
     }
+    acc = quad_combine_m(acc);
     switch (metric) {
     case M_L2SQ: return acc;
     case M_L2: return sqrtf(acc);
     case M_DOT: return 1.0f - acc;
     default: {
+        b2 = quad_combine_m(b2);
         if (qn == 0.0f && b2 == 0.0f) return 0.0f;
         if (acc == 0.0f) return 1.0f;
-        const float r = 1.0f - acc / sqrtf(qn * b2);
-        return r > 0.0f ? r : 0.0f;
+        const float rr = 1.0f - acc / sqrtf(qn * b2);
+        return rr > 0.0f ? rr : 0.0f;
     }
     }
 }
@@ -309,10 +358,12 @@ __device__ float exact_dist(uint32_t metric, const float *q, const float *x, uin
 __global__ __launch_bounds__(256) void k_rerank_items(const MixedParams p) {
     __shared__ uint64_t s_slot[kMixG];
     __shared__ uint32_t s_qi[kMixG];
+    __shared__ float s_qn[kMixG];
     __shared__ uint32_t s_list[kListCap];
     __shared__ uint32_t part[256];
     __shared__ uint32_t s_tot;
-    const uint32_t tid = threadIdx.x;
+    extern __shared__ __align__(16) float s_q[];   // kMixG x dpad
+    const uint32_t tid = threadIdx.x, r = tid & 3, units = p.dpad / 4;
     for (uint32_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
         uint32_t pp, pb, e_begin, e_end, pr0, gcnt;
         item_decode(p, item, pp, pb, e_begin, e_end, pr0, gcnt);
@@ -320,6 +371,7 @@ __global__ __launch_bounds__(256) void k_rerank_items(const MixedParams p) {
         if (tid < gcnt) {
             s_qi[tid] = p.pair_q[pr0 + tid];
             s_slot[tid] = p.pair_slot[pr0 + tid];
+            s_qn[tid] = p.qnorm ? p.qnorm[s_qi[tid]] : 0.0f;
         }
         __syncthreads();
         const uint32_t ne = e_end - e_begin;
@@ -341,6 +393,15 @@ __global__ __launch_bounds__(256) void k_rerank_items(const MixedParams p) {
         if (tid == 255) s_tot = base + mine;
         __syncthreads();
         const uint32_t total = s_tot;
+        if (total == 0) continue;
+        {
+            float4 *s4 = reinterpret_cast<float4 *>(s_q);
+            const float4 *q4 = reinterpret_cast<const float4 *>(p.qv);
+            for (uint32_t i = tid; i < gcnt * units; i += 256) {
+                const uint32_t g = i / units;
+                s4[i] = q4[(size_t)s_qi[g] * units + (i - g * units)];
+            }
+        }
         for (uint32_t r0 = 0; r0 < total; r0 += kListCap) {
             uint32_t pos = base;
             for (uint32_t w = tid; w < nwords; w += 256) {
@@ -360,16 +421,23 @@ __global__ __launch_bounds__(256) void k_rerank_items(const MixedParams p) {
             }
             __syncthreads();
             const uint32_t n = min(kListCap, total - r0);
-            for (uint32_t i = tid; i < n; i += 256) {
-                const uint32_t v = s_list[i];
+            for (uint32_t i0 = 0; i0 < n; i0 += 64) {      // 64 quads per workgroup
+                const uint32_t i = i0 + (tid >> 2);
+                const bool ok = i < n;
+                const uint32_t v = ok ? s_list[i] : 0u;
                 const uint32_t g = v >> 16, e = e_begin + (v & 0xffff);
-                const uint32_t q = s_qi[g];
-                const float d = exact_dist(p.metric, p.qv + (size_t)q * p.dpad,
-                                           p.raw_e + (size_t)e * p.dpad, p.dpad,
-                                           p.qnorm ? p.qnorm[q] : 0.0f);
-                const uint64_t key = ((uint64_t)fkey_m(d) << 32) | p.rank[e];
-                const uint32_t at = atomicAdd(&p.pool_cnt[q], 1u);
-                if (at < p.want) p.pool[(size_t)q * p.want + at] = key;
+                const float d = quad_exact_m(p.metric,
+                                             reinterpret_cast<const float4 *>(s_q) + (size_t)g * units,
+                                             reinterpret_cast<const float4 *>(p.raw_e + (size_t)e * p.dpad),
+                                             units, r, s_qn[g]);
+                if (ok && r == 0) {
+                    const uint32_t q = s_qi[g];
+                    const uint64_t key = ((uint64_t)fkey_m(d) << 32) | p.rank[e];
+                    const uint64_t slot = s_slot[g] + (e - pb);
+                    const uint32_t wi = (uint32_t)(slot >> 5);
+                    const uint32_t at = p.bit_prefix[wi] + __popc(p.bits[wi] & ((1u << (slot & 31)) - 1u));
+                    if (at < p.want) p.pool[(size_t)q * p.want + at] = key;
+                }
             }
             __syncthreads();
         }
@@ -417,17 +485,19 @@ hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) 
     return hipGetLastError();
 }
 
-hipError_t launch_pool_select(const uint16_t *keys, const uint64_t *seg_off, uint32_t nq,
-                              uint32_t want, uint32_t *bits, uint32_t *pool_want, hipStream_t s) {
+hipError_t launch_pool_select(const uint16_t *keys, const uint64_t *seg_off,
+                              const uint32_t *seg_cnt, uint32_t nq, uint32_t want, uint32_t *bits,
+                              uint32_t *bit_prefix, uint32_t *pool_want, hipStream_t s) {
     if (nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pool_select, dim3(nq), dim3(256), 0, s, keys, seg_off, want, bits,
-                       pool_want);
+    hipLaunchKernelGGL(k_pool_select, dim3(nq), dim3(256), 0, s, keys, seg_off, seg_cnt, want,
+                       bits, bit_prefix, pool_want);
     return hipGetLastError();
 }
 
 hipError_t launch_rerank_items(const MixedParams &p, uint32_t grid, hipStream_t s) {
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rerank_items, dim3(grid), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_rerank_items, dim3(grid), dim3(256), (size_t)kMixG * p.dpad * sizeof(float),
+                       s, p);
     return hipGetLastError();
 }
 
