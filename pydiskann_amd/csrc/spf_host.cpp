@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -497,7 +498,13 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     if (want > kMaxK) return fail(SPF_EINVAL, "k / rerank_size above 2048");
     const uint32_t dpad = ix->dpad;
     const uint32_t dpad32 = round_up(ix->dim, 32);
-    const uint32_t Gv = dpad <= 1024 ? 16 : 4;   // queries per VALU scan item
+    // queries per VALU scan item (SPF_SCAN_G overrides: 4, 8 or 16)
+    static const uint32_t g_env = [] {
+        const char *e = std::getenv("SPF_SCAN_G");
+        const uint32_t v = e ? (uint32_t)std::atoi(e) : 0u;
+        return (v == 4 || v == 8 || v == 16) ? v : 0u;
+    }();
+    const uint32_t Gv = dpad > 1024 ? 4 : (g_env ? g_env : 8);
     const uint32_t G = mixed ? 32 : Gv;          // queries per posting-scan item
     const bool prof = opts && (opts->flags & SPF_FLAG_PROFILE);
     if (prof)

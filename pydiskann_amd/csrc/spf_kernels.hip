@@ -758,6 +758,7 @@ hipError_t launch_scan(uint32_t ck, uint32_t metric, bool dense, uint32_t G, con
                        uint32_t grid, hipStream_t s) {
     if (grid == 0) return hipSuccess;
     if (G == 16) return scan_kind<16>(ck, metric, dense, p, grid, s);
+    if (G == 8) return scan_kind<8>(ck, metric, dense, p, grid, s);
     if (G == 4) return scan_kind<4>(ck, metric, dense, p, grid, s);
     return hipErrorInvalidValue;
 }

@@ -175,12 +175,41 @@ __device__ __forceinline__ uint32_t excl_scan256(uint32_t v, uint32_t *part) {
     return incl - v;
 }
 
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// exclusive scan over a 256-thread block with two barriers; *tot = block sum
+__device__ __forceinline__ uint32_t block_scan_fast(uint32_t v, uint32_t *s_w, uint32_t *tot) {
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_incl_scan(v);
+    if ((threadIdx.x & 63) == 63) s_w[wave] = incl;
+    __syncthreads();
+    uint32_t base = 0, sum = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; w++) {
+        const uint32_t x = s_w[w];
+        base += w < wave ? x : 0u;
+        sum += x;
+    }
+    __syncthreads();
+    *tot = sum;
+    return base + incl - v;
+}
+
 // find the bin where the cumulative count reaches `rem`; returns (bin, below)
-__device__ __forceinline__ void find_bin256(const uint32_t *hist, uint32_t rem, uint32_t *part,
+__device__ __forceinline__ void find_bin256(const uint32_t *hist, uint32_t rem, uint32_t *s_w,
                                             uint32_t *s_out) {
     const uint32_t tid = threadIdx.x;
     const uint32_t h = hist[tid];
-    const uint32_t excl = excl_scan256(h, part);
+    uint32_t tot;
+    const uint32_t excl = block_scan_fast(h, s_w, &tot);
     if (excl < rem && rem <= excl + h) {
         s_out[0] = tid;
         s_out[1] = excl;
@@ -188,123 +217,142 @@ __device__ __forceinline__ void find_bin256(const uint32_t *hist, uint32_t rem, 
     __syncthreads();
 }
 
+// 32 keys of one bitmap word (64 bytes; segments are 32-slot aligned)
+__device__ __forceinline__ void load_word_keys(const uint16_t *kp, uint32_t w, uint32_t n,
+                                               uint16_t k[32]) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(kp + 32 * w);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        union { uint4 v; uint16_t h[8]; } c;
+        c.v = src[j];
+#pragma unroll
+        for (int t = 0; t < 8; t++) k[8 * j + t] = (32 * w + 8 * j + t < n) ? c.h[t] : (uint16_t)0xffff;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_pool_select(const uint16_t *keys, const uint64_t *seg_off,
                                                      const uint32_t *seg_cnt, uint32_t want,
                                                      uint32_t *bits, uint32_t *bit_prefix,
                                                      uint32_t *pool_want) {
-    __shared__ uint16_t s_k[kPoolLds];
     __shared__ uint32_t hist[256];
-    __shared__ uint32_t part[256];
+    __shared__ uint32_t s_w[4];
     __shared__ uint32_t s_u[4];
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
     const uint64_t b = seg_off[q];               // multiple of 32
     const uint32_t n = seg_cnt[q];
+    const uint32_t nwords = (n + 31) / 32;
     const uint16_t *kp = keys + b;
-    const bool cached = n <= kPoolLds;
-    if (tid == 0) {
-        s_u[0] = 0xffff;
-        s_u[1] = 0;
-        s_u[2] = 0;
-    }
-    __syncthreads();
+    uint16_t k[32];
+    // pass 1: invalid count, min and max key
     uint32_t inval = 0, kmn = 0xffff, kmx = 0;
-    for (uint32_t i = tid; i < n; i += 256) {
-        const uint16_t k = kp[i];
-        if (cached) s_k[i] = k;
-        if (k == 0xffff) {
-            inval++;
-        } else {
-            kmn = min(kmn, (uint32_t)k);
-            kmx = max(kmx, (uint32_t)k);
+    for (uint32_t w = tid; w < nwords; w += 256) {
+        load_word_keys(kp, w, n, k);
+#pragma unroll
+        for (int j = 0; j < 32; j++) {
+            if (32 * w + j >= n) continue;
+            if (k[j] == 0xffff) {
+                inval++;
+            } else {
+                kmn = min(kmn, (uint32_t)k[j]);
+                kmx = max(kmx, (uint32_t)k[j]);
+            }
         }
     }
-    if (inval) atomicAdd(&s_u[2], inval);
-    if (kmn != 0xffff) atomicMin(&s_u[0], kmn);
-    if (kmx) atomicMax(&s_u[1], kmx);
+    for (int off = 32; off > 0; off >>= 1) {
+        inval += __shfl_xor(inval, off, 64);
+        kmn = min(kmn, (uint32_t)__shfl_xor(kmn, off, 64));
+        kmx = max(kmx, (uint32_t)__shfl_xor(kmx, off, 64));
+    }
+    __shared__ uint32_t s_r[3][4];
+    if ((tid & 63) == 0) {
+        s_r[0][tid >> 6] = inval;
+        s_r[1][tid >> 6] = kmn;
+        s_r[2][tid >> 6] = kmx;
+    }
     __syncthreads();
-    const uint32_t nvalid = n - s_u[2];
+    inval = s_r[0][0] + s_r[0][1] + s_r[0][2] + s_r[0][3];
+    kmn = min(min(s_r[1][0], s_r[1][1]), min(s_r[1][2], s_r[1][3]));
+    kmx = max(max(s_r[2][0], s_r[2][1]), max(s_r[2][2], s_r[2][3]));
+    const uint32_t nvalid = n - inval;
     const uint32_t W = min(want, nvalid);
     if (tid == 0) pool_want[q] = W;
     const bool all = W == nvalid;
     uint32_t T = 0, need = 0;
     if (!all) {
         // adaptive range: bins over [kmin, kmax] so the keys spread over 256 bins
-        const uint32_t kmin = s_u[0], kmax = s_u[1];
-        const uint32_t span = kmax - kmin;
+        const uint32_t span = kmx - kmn;
         const uint32_t sh = span < 256 ? 0 : 32 - __clz(span) - 8;
         hist[tid] = 0;
         __syncthreads();
-        for (uint32_t i = tid; i < n; i += 256) {
-            const uint16_t k = cached ? s_k[i] : kp[i];
-            if (k != 0xffff) atomicAdd(&hist[(k - kmin) >> sh], 1u);
+        for (uint32_t w = tid; w < nwords; w += 256) {
+            load_word_keys(kp, w, n, k);
+#pragma unroll
+            for (int j = 0; j < 32; j++)
+                if (k[j] != 0xffff) atomicAdd(&hist[(k[j] - kmn) >> sh], 1u);
         }
         __syncthreads();
-        find_bin256(hist, W, part, s_u);
+        find_bin256(hist, W, s_w, s_u);
         const uint32_t b1 = s_u[0], below1 = s_u[1];
         __syncthreads();
         uint32_t b2 = 0, below2 = 0;
         if (sh > 0) {
             hist[tid] = 0;
             __syncthreads();
-            for (uint32_t i = tid; i < n; i += 256) {
-                const uint16_t k = cached ? s_k[i] : kp[i];
-                if (k != 0xffff && ((k - kmin) >> sh) == b1)
-                    atomicAdd(&hist[(k - kmin) & ((1u << sh) - 1u)], 1u);
+            for (uint32_t w = tid; w < nwords; w += 256) {
+                load_word_keys(kp, w, n, k);
+#pragma unroll
+                for (int j = 0; j < 32; j++)
+                    if (k[j] != 0xffff && ((k[j] - kmn) >> sh) == b1)
+                        atomicAdd(&hist[(k[j] - kmn) & ((1u << sh) - 1u)], 1u);
             }
             __syncthreads();
-            find_bin256(hist, W - below1, part, s_u);
+            find_bin256(hist, W - below1, s_w, s_u);
             b2 = s_u[0];
             below2 = s_u[1];
         }
-        T = kmin + (b1 << sh) + b2;
+        T = kmn + (b1 << sh) + b2;
         need = W - below1 - below2;   // keys equal to T to take, in slot order
     }
-    // each thread owns a run of whole words, so tie ranks and mark positions
-    // follow slot order and every word is written by exactly one thread
-    const uint32_t nwords = (n + 31) / 32;
-    const uint32_t run = (nwords + 255) / 256;
-    const uint32_t w0 = min(nwords, tid * run), w1 = min(nwords, w0 + run);
-    uint32_t ties = 0;
-    if (!all)
-        for (uint32_t i = 32 * w0; i < min(n, 32 * w1); i++) ties += (cached ? s_k[i] : kp[i]) == T;
-    uint32_t tr = excl_scan256(ties, part);
-    uint32_t marks = 0;
-    uint32_t words[8];
-    const bool keep = run <= 8;
-    for (uint32_t w = w0; w < w1; w++) {
+    // collect in chunks of 256 words: thread t of chunk c owns word 256c + t, so
+    // tie ranks and mark positions follow slot order
+    uint32_t tie_run = 0, mark_run = 0;
+    for (uint32_t c0 = 0; c0 < nwords; c0 += 256) {
+        const uint32_t w = c0 + tid;
+        const bool okw = w < nwords;
+        uint32_t ties = 0;
+        if (okw) {
+            load_word_keys(kp, w, n, k);
+            if (!all)
+#pragma unroll
+                for (int j = 0; j < 32; j++) ties += k[j] == T;
+        }
+        uint32_t tt;
+        uint32_t tr = tie_run + block_scan_fast(ties, s_w, &tt);
+        tie_run += tt;
         uint32_t word = 0;
-        for (uint32_t j = 0; j < 32; j++) {
-            const uint32_t i = 32 * w + j;
-            if (i >= n) break;
-            const uint16_t k = cached ? s_k[i] : kp[i];
-            bool sel;
-            if (all) {
-                sel = k != 0xffff;
-            } else if (k < T) {
-                sel = true;
-            } else if (k == T) {
-                sel = tr < need;
-                tr++;
-            } else {
-                sel = false;
+        if (okw) {
+#pragma unroll
+            for (int j = 0; j < 32; j++) {
+                bool sel;
+                if (all) {
+                    sel = k[j] != 0xffff;
+                } else if (k[j] < T) {
+                    sel = true;
+                } else if (k[j] == T) {
+                    sel = tr < need;
+                    tr++;
+                } else {
+                    sel = false;
+                }
+                word |= (uint32_t)sel << j;
             }
-            word |= (uint32_t)sel << j;
         }
-        if (keep) words[w - w0] = word;
-        bits[(b >> 5) + w] = word;
-        marks += __popc(word);
-    }
-    uint32_t pre = excl_scan256(marks, part);
-    if (keep) {
-        for (uint32_t w = w0; w < w1; w++) {
+        uint32_t mt;
+        const uint32_t pre = mark_run + block_scan_fast(__popc(word), s_w, &mt);
+        mark_run += mt;
+        if (okw) {
+            bits[(b >> 5) + w] = word;
             bit_prefix[(b >> 5) + w] = pre;
-            pre += __popc(words[w - w0]);
-        }
-    } else {
-        __threadfence_block();
-        for (uint32_t w = w0; w < w1; w++) {
-            bit_prefix[(b >> 5) + w] = pre;
-            pre += __popc(bits[(b >> 5) + w]);
         }
     }
 }
@@ -360,8 +408,8 @@ __global__ __launch_bounds__(256) void k_rerank_items(const MixedParams p) {
     __shared__ uint32_t s_qi[kMixG];
     __shared__ float s_qn[kMixG];
     __shared__ uint32_t s_list[kListCap];
-    __shared__ uint32_t part[256];
-    __shared__ uint32_t s_tot;
+    __shared__ uint32_t s_at[kListCap];
+    __shared__ uint32_t s_wsum[4];
     extern __shared__ __align__(16) float s_q[];   // kMixG x dpad
     const uint32_t tid = threadIdx.x, r = tid & 3, units = p.dpad / 4;
     for (uint32_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
@@ -389,10 +437,8 @@ __global__ __launch_bounds__(256) void k_rerank_items(const MixedParams p) {
             if (nb < 32) word &= (1u << nb) - 1u;
             mine += __popc(word);
         }
-        const uint32_t base = excl_scan256(mine, part);
-        if (tid == 255) s_tot = base + mine;
-        __syncthreads();
-        const uint32_t total = s_tot;
+        uint32_t total;
+        const uint32_t base = block_scan_fast(mine, s_wsum, &total);
         if (total == 0) continue;
         {
             float4 *s4 = reinterpret_cast<float4 *>(s_q);
@@ -412,31 +458,50 @@ __global__ __launch_bounds__(256) void k_rerank_items(const MixedParams p) {
                 uint32_t word = (uint32_t)(((hi << 32) | lo) >> (bit0 & 31));
                 const uint32_t nb = min(32u, ne - 32 * wi);
                 if (nb < 32) word &= (1u << nb) - 1u;
+                if (!word) continue;
+                // pool position of a marked slot: marks before its bitmap word in
+                // the query's segment + marks below it inside the word
+                const uint32_t wlo = (uint32_t)(bit0 >> 5), sft = (uint32_t)(bit0 & 31);
+                const uint32_t plo = p.bit_prefix[wlo], phi = p.bit_prefix[wlo + 1];
                 while (word) {
                     const uint32_t bi = __ffs(word) - 1;
                     word &= word - 1;
-                    if (pos >= r0 && pos < r0 + kListCap) s_list[pos - r0] = (g << 16) | (32 * wi + bi);
+                    if (pos >= r0 && pos < r0 + kListCap) {
+                        const uint32_t sb = sft + bi;   // bit within the (lo, hi) pair
+                        const uint32_t at = sb < 32
+                            ? plo + __popc((uint32_t)lo & ((1u << sb) - 1u))
+                            : phi + __popc((uint32_t)hi & ((1u << (sb - 32)) - 1u));
+                        s_list[pos - r0] = (g << 16) | (32 * wi + bi);
+                        s_at[pos - r0] = at;
+                    }
                     pos++;
                 }
             }
             __syncthreads();
             const uint32_t n = min(kListCap, total - r0);
-            for (uint32_t i0 = 0; i0 < n; i0 += 64) {      // 64 quads per workgroup
-                const uint32_t i = i0 + (tid >> 2);
-                const bool ok = i < n;
-                const uint32_t v = ok ? s_list[i] : 0u;
-                const uint32_t g = v >> 16, e = e_begin + (v & 0xffff);
-                const float d = quad_exact_m(p.metric,
-                                             reinterpret_cast<const float4 *>(s_q) + (size_t)g * units,
-                                             reinterpret_cast<const float4 *>(p.raw_e + (size_t)e * p.dpad),
-                                             units, r, s_qn[g]);
-                if (ok && r == 0) {
-                    const uint32_t q = s_qi[g];
-                    const uint64_t key = ((uint64_t)fkey_m(d) << 32) | p.rank[e];
-                    const uint64_t slot = s_slot[g] + (e - pb);
-                    const uint32_t wi = (uint32_t)(slot >> 5);
-                    const uint32_t at = p.bit_prefix[wi] + __popc(p.bits[wi] & ((1u << (slot & 31)) - 1u));
-                    if (at < p.want) p.pool[(size_t)q * p.want + at] = key;
+            for (uint32_t i0 = 0; i0 < n; i0 += 128) {     // 64 quads x 2 pairs in flight
+                uint32_t ii[2], gg[2], ee[2];
+                float dd[2];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    ii[h] = i0 + 64 * h + (tid >> 2);
+                    const uint32_t v = ii[h] < n ? s_list[ii[h]] : 0u;
+                    gg[h] = v >> 16;
+                    ee[h] = e_begin + (v & 0xffff);
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+                    dd[h] = quad_exact_m(p.metric,
+                                         reinterpret_cast<const float4 *>(s_q) + (size_t)gg[h] * units,
+                                         reinterpret_cast<const float4 *>(p.raw_e + (size_t)ee[h] * p.dpad),
+                                         units, r, s_qn[gg[h]]);
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (ii[h] < n && r == 0) {
+                        const uint32_t at = s_at[ii[h]];
+                        const uint64_t key = ((uint64_t)fkey_m(dd[h]) << 32) | p.rank[ee[h]];
+                        if (at < p.want) p.pool[(size_t)s_qi[gg[h]] * p.want + at] = key;
+                    }
                 }
             }
             __syncthreads();
