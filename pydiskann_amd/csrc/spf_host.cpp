@@ -82,7 +82,7 @@ struct Workspace {
     DBuf<uint32_t> out_c;
     DBuf<uint16_t> q16, cand16;
     DBuf<float> qn16;
-    DBuf<uint32_t> bits, bit_prefix, pool_want, cand_cnt;
+    DBuf<uint32_t> pool_entry, pool_want, cand_cnt, perm_cnt, perm_off, perm_fill, qperm;
     DBuf<uint64_t> pool;
     uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
     hipEvent_t ev[SPF_N_STAGES + 1] = {};
@@ -90,9 +90,12 @@ struct Workspace {
     void release() {
         q16.release();
         cand16.release();
-        bits.release();
-        bit_prefix.release();
+        pool_entry.release();
         cand_cnt.release();
+        perm_cnt.release();
+        perm_off.release();
+        perm_fill.release();
+        qperm.release();
         pool_want.release();
         pool.release();
         qn16.release();
@@ -425,16 +428,18 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
 // rerank -> per-query sort.  Runs after probe and planning.
 template <class Mark>
 int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_t k,
-                 uint32_t want, uint64_t total, uint32_t n_items, uint64_t *d_ids,
+                 uint32_t npr, uint32_t want, uint64_t total, uint32_t n_items, uint64_t *d_ids,
                  float *d_dists, uint32_t *d_counts, Mark &mark) {
     const uint32_t dpad32 = round_up(ix->dim, 32);
     HIPCHK(w->cand16.ensure(std::max<uint64_t>(total, 1)));
-    HIPCHK(w->bits.ensure(total / 32 + 4));
-    HIPCHK(w->bit_prefix.ensure(total / 32 + 4));
-    HIPCHK(w->pool.ensure((size_t)nq * want));
+    HIPCHK(w->pool_entry.ensure((size_t)nq * want));
     HIPCHK(w->pool_want.ensure(nq));
     HIPCHK(w->fin.ensure((size_t)nq * k));
     HIPCHK(w->fin_cnt.ensure(nq));
+    HIPCHK(w->perm_cnt.ensure(ix->P + 2));
+    HIPCHK(w->perm_off.ensure(ix->P + 2));
+    HIPCHK(w->perm_fill.ensure(ix->P + 2));
+    HIPCHK(w->qperm.ensure(nq));
     MixedParams mp{};
     mp.item_off = w->item_off.p;
     mp.item_post = w->item_post.p;
@@ -453,25 +458,42 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     mp.xn16 = ix->d_xn16;
     mp.del = ix->d_del;
     mp.out16 = w->cand16.p;
-    mp.bits = w->bits.p;
-    mp.bit_prefix = w->bit_prefix.p;
-    mp.pool = w->pool.p;
-    mp.want = want;
-    mp.metric = ix->metric;
-    mp.dpad = ix->dpad;
-    mp.qv = w->qv_probe.p;
-    mp.qnorm = w->qnorm.p;
-    mp.raw_e = ix->d_raw_e;
-    mp.rank = ix->d_rank;
-    const uint32_t grid = std::min<uint32_t>(n_items, 16384);
+    const uint32_t grid = xcd_grid(n_items, 16384);
     HIPCHK(mark(SPF_STAGE_SCAN));
     HIPCHK(launch_scan_mfma(mp, grid, s));
     HIPCHK(mark(SPF_STAGE_SELECT));
-    HIPCHK(launch_pool_select(w->cand16.p, w->cand_off.p, w->cand_cnt.p, nq, want, w->bits.p,
-                              w->bit_prefix.p, w->pool_want.p, s));
+    PoolParams pp{};
+    pp.keys = w->cand16.p;
+    pp.seg_off = w->cand_off.p;
+    pp.seg_cnt = w->cand_cnt.p;
+    pp.want = want;
+    pp.npr = npr;
+    pp.np_q = w->np_q.p;
+    pp.qp_slot = w->qp_slot.p;
+    pp.probe_post = w->probe_post.p;
+    pp.post_off = ix->d_post_off;
+    pp.pool_entry = w->pool_entry.p;
+    pp.pool_want = w->pool_want.p;
+    HIPCHK(launch_pool_select(pp, nq, s));
     HIPCHK(mark(SPF_STAGE_RERANK));
-    HIPCHK(launch_rerank_items(mp, grid, s));
-    HIPCHK(launch_sort_topk(w->pool.p, w->pool_want.p, nq, want, k, w->fin.p, w->fin_cnt.p, s));
+    HIPCHK(launch_query_perm(w->np_q.p, w->probe_post.p, npr, nq, ix->P, w->perm_cnt.p,
+                             w->perm_off.p, w->perm_fill.p, w->qperm.p, s));
+    RerankPoolParams rp{};
+    rp.pool_entry = w->pool_entry.p;
+    rp.pool_want = w->pool_want.p;
+    rp.qperm = w->qperm.p;
+    rp.nq = nq;
+    rp.want = want;
+    rp.k = k;
+    rp.metric = ix->metric;
+    rp.dpad = ix->dpad;
+    rp.qv = w->qv_probe.p;
+    rp.qnorm = w->qnorm.p;
+    rp.raw_e = ix->d_raw_e;
+    rp.rank = ix->d_rank;
+    rp.out = w->fin.p;
+    rp.out_cnt = w->fin_cnt.p;
+    HIPCHK(launch_rerank_pool(rp, s));
     HIPCHK(mark(SPF_STAGE_FINALIZE));
     HIPCHK(launch_finalize(w->fin.p, k, w->fin_cnt.p, nq, k, ix->d_rank_id, d_ids, d_dists,
                            d_counts, s));
@@ -585,7 +607,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         sp.n_echunks = (P + kEChunk - 1) / kEChunk;
         sp.out_stride = P;
         const uint64_t items = (uint64_t)((nq + Gv - 1) / Gv) * sp.n_echunks;
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(items, 16384);
+        const uint32_t grid = xcd_grid(items, 16384);
         HIPCHK(launch_scan(CK_F32, ix->metric, true, Gv, sp, grid, s));
     }
     HIPCHK(mark(SPF_STAGE_PROBE_SELECT));
@@ -651,8 +673,8 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(launch_item_table(pl, s));
 
     if (mixed) {
-        int rc = search_mixed(ix, w, s, nq, k, want, total, n_items, d_ids, d_dists, d_counts,
-                              mark);
+        int rc = search_mixed(ix, w, s, nq, k, npr, want, total, n_items, d_ids, d_dists,
+                              d_counts, mark);
         if (rc) return rc;
     } else {
     // ---- posting scan
@@ -683,7 +705,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         sp.pair_off = w->pair_off.p;
         sp.pair_q = w->pair_q.p;
         sp.pair_slot = w->pair_slot.p;
-        const uint32_t grid = std::min<uint32_t>(n_items, 16384);
+        const uint32_t grid = xcd_grid(n_items, 16384);
         HIPCHK(launch_scan(ix->code_kind, ix->metric, false, G, sp, grid, s));
     }
     // ---- top-`want` per query

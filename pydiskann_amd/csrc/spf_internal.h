@@ -14,6 +14,31 @@ constexpr uint32_t kSelCap = 4096;   // LDS keys held by the selection kernel
 constexpr uint32_t kMaxK = 2048;     // largest k / rerank pool a query may ask for
 constexpr uint32_t kEChunk = 1024;   // entries of one posting per work item
 constexpr uint64_t kNoKey = ~0ull;   // "no candidate" (deleted / padding)
+constexpr uint32_t kXcds = 8;        // MI355X: workgroup i runs on XCD i % 8
+
+// Work-item walk for a grid whose size is a multiple of kXcds: XCD x takes the
+// contiguous item range [x*chunk, (x+1)*chunk), so all items of one posting
+// (adjacent in item order) share one XCD's L2 instead of being spread round-robin.
+struct XcdWalk {
+    uint32_t item, end, step;
+    __device__ __forceinline__ XcdWalk(uint32_t n_items) {
+        const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+        step = gridDim.x / kXcds;
+        const uint32_t chunk = (n_items + kXcds - 1) / kXcds;
+        const uint32_t start = x * chunk;
+        end = min(n_items, start + chunk);
+        item = start + j;
+    }
+    __device__ __forceinline__ bool more() const { return item < end; }
+    __device__ __forceinline__ void next() { item += step; }
+};
+
+// grid size for an item walk: a multiple of kXcds, at most `cap`
+inline uint32_t xcd_grid(uint64_t n_items, uint32_t cap) {
+    uint64_t g = n_items < cap ? n_items : cap;
+    g = (g + kXcds - 1) / kXcds * kXcds;
+    return (uint32_t)(g ? g : 0);
+}
 
 // One scan launch: distances between staged queries and code rows held in
 // the tiled layout (16-byte unit u of entry e at ((e/64)*units + u)*64 + e%64).
@@ -80,14 +105,36 @@ struct MixedParams {
     const float *xn16;          // |x16|^2 per entry
     const uint32_t *del;
     uint16_t *out16;            // 16-bit candidate keys
-    const uint32_t *bits;       // pool marks over candidate slots (segments 32-aligned)
-    const uint32_t *bit_prefix; // per bitmap word: marks before it in its query's segment
-    uint64_t *pool;             // [nq][want] reranked keys
-    uint32_t want, metric, dpad;
-    const float *qv;            // [nq][dpad] f32 queries
-    const float *qnorm;         // [nq] |q|^2 (cosine)
-    const float *raw_e;         // [n_entries][dpad] f32 rows in entry order
+};
+
+// Pool selection over 16-bit keys: the `want` smallest per query, written as
+// entry indices (slot -> entry through the query's probe table).
+struct PoolParams {
+    const uint16_t *keys;
+    const uint64_t *seg_off;    // [nq + 1] 32-aligned segments
+    const uint32_t *seg_cnt;    // [nq]
+    uint32_t want, npr;
+    const uint32_t *np_q;       // [nq]
+    const uint64_t *qp_slot;    // [nq][npr] absolute slot of each probed posting
+    const uint32_t *probe_post; // [nq][npr]
+    const uint32_t *post_off;   // [n_post + 1]
+    uint32_t *pool_entry;       // [nq][want]
+    uint32_t *pool_want;        // [nq]
+};
+
+// Exact f32 rerank of each query's pool and the final top-k, one workgroup per
+// query, queries taken in `qperm` order.
+struct RerankPoolParams {
+    const uint32_t *pool_entry; // [nq][want]
+    const uint32_t *pool_want;  // [nq]
+    const uint32_t *qperm;      // [nq] processing order (by nearest posting)
+    uint32_t nq, want, k, metric, dpad;
+    const float *qv;            // [nq][dpad]
+    const float *qnorm;         // [nq]
+    const float *raw_e;         // [n_entries][dpad] rows in entry order
     const uint32_t *rank;       // per entry
+    uint64_t *out;              // [nq][k]
+    uint32_t *out_cnt;          // [nq]
 };
 
 struct PlanParams {
@@ -151,12 +198,10 @@ hipError_t launch_entry_norms(uint32_t code_kind, const uint4 *data, uint32_t un
 
 hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t s);
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s);
-hipError_t launch_pool_select(const uint16_t *keys, const uint64_t *seg_off,
-                              const uint32_t *seg_cnt, uint32_t nq, uint32_t want, uint32_t *bits,
-                              uint32_t *bit_prefix, uint32_t *pool_want, hipStream_t s);
-hipError_t launch_rerank_items(const MixedParams &p, uint32_t grid, hipStream_t s);
-hipError_t launch_sort_topk(const uint64_t *pool, const uint32_t *pool_cnt, uint32_t nq,
-                            uint32_t want, uint32_t k, uint64_t *out, uint32_t *out_cnt,
-                            hipStream_t s);
+hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s);
+hipError_t launch_rerank_pool(const RerankPoolParams &p, hipStream_t s);
+hipError_t launch_query_perm(const uint32_t *np_q, const uint32_t *probe_post, uint32_t npr,
+                             uint32_t nq, uint32_t n_post, uint32_t *cnt, uint32_t *off,
+                             uint32_t *fill, uint32_t *qperm, hipStream_t s);
 
 }  // namespace spf

@@ -138,7 +138,8 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
     const uint32_t dq4 = p.dpad >> 2;   // float4 (or u32 words) per staged query row
     const uint32_t n_items = DENSE ? ((p.nq + G - 1) / G) * p.n_echunks : p.n_items;
 
-    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    for (XcdWalk it(n_items); it.more(); it.next()) {
+        const uint32_t item = it.item;
         uint32_t e_begin, e_end, gcnt, base_e;
         if (DENSE) {
             const uint32_t qc = item / p.n_echunks, ec = item - qc * p.n_echunks;
@@ -356,6 +357,139 @@ __device__ void bitonic_sort_lds(uint64_t *buf, uint32_t m) {
 constexpr uint32_t kSelDirect = 1024;   // segments this short are sorted outright
 constexpr uint32_t kSelSlack = 256;     // extra keys tolerated in the final sort
 
+// Top-K of one segment.  R > 0: the segment (at most R * 256 keys) is held in
+// registers, key i = tid + 256 j in kr[j]; R == 0: every pass re-reads L2.
+template <int R>
+__device__ __forceinline__ void select_body(const uint64_t *kp, uint64_t n, uint32_t K,
+                                            uint64_t *o, uint32_t *cnt_out, uint64_t *buf,
+                                            uint32_t *hist, uint32_t *part, uint32_t *s_u,
+                                            uint64_t (*s_lh)[4]) {
+    const uint32_t tid = threadIdx.x;
+    uint64_t kr[R > 0 ? R : 1];
+    if (R > 0) {
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            const uint64_t i = tid + 256ull * j;
+            kr[j] = i < n ? kp[i] : kNoKey;
+        }
+    }
+    // visit every key of the segment (padding keys read as kNoKey)
+#define SPF_FOR_KEYS(BODY)                                                        \
+    if (R > 0) {                                                                  \
+        _Pragma("unroll") for (int j = 0; j < (R > 0 ? R : 1); j++) {             \
+            const uint64_t k = kr[j];                                             \
+            BODY                                                                  \
+        }                                                                         \
+    } else {                                                                      \
+        for (uint64_t i = tid; i < n; i += kWG) {                                 \
+            const uint64_t k = kp[i];                                             \
+            BODY                                                                  \
+        }                                                                         \
+    }
+    if (tid == 0) s_u[0] = 0;
+    __syncthreads();
+    // invalid count, min and max of the valid keys
+    uint32_t inval = 0;
+    uint64_t lo = kNoKey, hi = 0;
+    SPF_FOR_KEYS({
+        if (k == kNoKey) {
+            inval++;
+        } else {
+            lo = k < lo ? k : lo;
+            hi = k > hi ? k : hi;
+        }
+    })
+    for (int off = 32; off > 0; off >>= 1) {
+        inval += __shfl_xor(inval, off, 64);
+        const uint64_t a = __shfl_xor(lo, off, 64), b2 = __shfl_xor(hi, off, 64);
+        lo = a < lo ? a : lo;
+        hi = b2 > hi ? b2 : hi;
+    }
+    if ((tid & 63) == 0) {
+        s_lh[0][tid >> 6] = lo;
+        s_lh[1][tid >> 6] = hi;
+        atomicAdd(&s_u[0], inval);
+    }
+    __syncthreads();
+    lo = s_lh[0][0];
+    hi = s_lh[1][0];
+    for (int w = 1; w < 4; w++) {
+        lo = s_lh[0][w] < lo ? s_lh[0][w] : lo;
+        hi = s_lh[1][w] > hi ? s_lh[1][w] : hi;
+    }
+    // with registers, padding beyond n was counted as invalid
+    const uint64_t pad = R > 0 ? (uint64_t)R * 256 - n : 0;
+    const uint64_t nvalid = n + pad - s_u[0];
+    const uint32_t Kv = (uint32_t)min((uint64_t)K, nvalid);
+    __syncthreads();
+    if (Kv == 0) {
+        if (tid == 0) *cnt_out = 0;
+        return;
+    }
+    // digits start at the highest bit where the valid keys differ, so the first
+    // histogram spreads over bins instead of piling into one
+    const int hb = lo == hi ? -1 : 63 - __clzll(lo ^ hi);
+    uint64_t mask = hb < 0 ? ~0ull : (hb >= 63 ? 0ull : ~((2ull << hb) - 1ull));
+    uint64_t prefix = lo & mask;
+    uint32_t rem = Kv;
+    int top = hb;
+    while (top >= 0) {
+        const int wdt = top + 1 < 11 ? top + 1 : 11;
+        const int sh = top + 1 - wdt;
+        const uint32_t dm = (1u << wdt) - 1u;
+        for (uint32_t i = tid; i < 2048; i += kWG) hist[i] = 0;
+        __syncthreads();
+        SPF_FOR_KEYS({
+            if ((k & mask) == prefix) atomicAdd(&hist[(uint32_t)(k >> sh) & dm], 1u);
+        })
+        __syncthreads();
+        uint32_t loc[8], sm = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            loc[j] = hist[tid * 8 + j];
+            sm += loc[j];
+        }
+        const uint32_t excl = block_excl_scan256(sm, part);
+        if (excl < rem && rem <= excl + sm) {
+            uint32_t c = excl;
+            for (int j = 0; j < 8; j++) {
+                if (rem <= c + loc[j]) {
+                    s_u[1] = tid * 8 + j;
+                    s_u[2] = c;
+                    s_u[3] = loc[j];
+                    break;
+                }
+                c += loc[j];
+            }
+        }
+        __syncthreads();
+        const uint32_t bin = s_u[1], below = s_u[2], cnt = s_u[3];
+        __syncthreads();
+        rem -= below;
+        prefix |= (uint64_t)bin << sh;
+        mask |= (uint64_t)dm << sh;
+        top = sh - 1;
+        if ((Kv - rem) + cnt <= Kv + kSelSlack) break;
+    }
+    if (tid == 0) s_u[0] = 0;
+    __syncthreads();
+    SPF_FOR_KEYS({
+        if (k != kNoKey && (k & mask) <= prefix) {
+            const uint32_t pos = atomicAdd(&s_u[0], 1u);
+            if (pos < kSelCap) buf[pos] = k;
+        }
+    })
+#undef SPF_FOR_KEYS
+    __syncthreads();
+    const uint32_t m = min(s_u[0], kSelCap);
+    const uint32_t m2 = next_pow2(m < 2 ? 2 : m);
+    for (uint32_t i = m + tid; i < m2; i += kWG) buf[i] = kNoKey;
+    __syncthreads();
+    bitonic_sort_lds(buf, m2);
+    for (uint32_t i = tid; i < Kv; i += kWG) o[i] = buf[i];
+    if (tid == 0) *cnt_out = Kv;
+}
+
 __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint64_t *seg_off,
                                                 const uint32_t *seg_cnt, uint64_t seg_stride,
                                                 uint32_t K, uint64_t *out, uint32_t *out_cnt) {
@@ -363,121 +497,17 @@ __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint
     __shared__ uint32_t hist[2048];
     __shared__ uint32_t part[kWG];
     __shared__ uint32_t s_u[4];
-    const uint32_t seg = blockIdx.x, tid = threadIdx.x;
+    __shared__ uint64_t s_lh[2][4];
+    const uint32_t seg = blockIdx.x;
     const uint64_t b = seg_off ? seg_off[seg] : (uint64_t)seg * seg_stride;
     const uint64_t e = seg_off ? seg_off[seg + 1] : b + seg_stride;
     const uint64_t n = seg_cnt ? (uint64_t)seg_cnt[seg] : e - b;
-    const uint64_t *kp = keys + b;
-    uint64_t *o = out + (size_t)seg * K;
-
-    if (tid == 0) s_u[0] = 0;
-    __syncthreads();
-    uint32_t inval = 0;
-    for (uint64_t i = tid; i < n; i += kWG) inval += kp[i] == kNoKey;
-    if (inval) atomicAdd(&s_u[0], inval);
-    __syncthreads();
-    const uint64_t nvalid = n - s_u[0];
-    const uint32_t Kv = (uint32_t)min((uint64_t)K, nvalid);
-    __syncthreads();
-    if (Kv == 0) {
-        if (tid == 0) out_cnt[seg] = 0;
-        return;
-    }
-    uint32_t m;
-    if (n <= kSelDirect) {
-        for (uint32_t i = tid; i < n; i += kWG) buf[i] = kp[i];
-        m = (uint32_t)n;
-    } else {
-        // digits start at the highest bit where the segment's valid keys differ,
-        // so the first histogram spreads over bins instead of piling into one
-        uint64_t lo = kNoKey, hi = 0;
-        for (uint64_t i = tid; i < n; i += kWG) {
-            const uint64_t k = kp[i];
-            if (k != kNoKey) {
-                lo = k < lo ? k : lo;
-                hi = k > hi ? k : hi;
-            }
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            const uint64_t a = __shfl_xor(lo, off, 64), b2 = __shfl_xor(hi, off, 64);
-            lo = a < lo ? a : lo;
-            hi = b2 > hi ? b2 : hi;
-        }
-        __shared__ uint64_t s_lh[2][4];
-        if ((tid & 63) == 0) {
-            s_lh[0][tid >> 6] = lo;
-            s_lh[1][tid >> 6] = hi;
-        }
-        __syncthreads();
-        lo = s_lh[0][0];
-        hi = s_lh[1][0];
-        for (int w = 1; w < 4; w++) {
-            lo = s_lh[0][w] < lo ? s_lh[0][w] : lo;
-            hi = s_lh[1][w] > hi ? s_lh[1][w] : hi;
-        }
-        const int hb = lo == hi ? -1 : 63 - __clzll(lo ^ hi);
-        uint64_t mask = hb < 0 ? ~0ull : (hb >= 63 ? 0ull : ~((2ull << hb) - 1ull));
-        uint64_t prefix = lo & mask;
-        uint32_t rem = Kv;
-        int top = hb;
-        __syncthreads();
-        while (top >= 0) {
-            const int wdt = top + 1 < 11 ? top + 1 : 11;
-            const int sh = top + 1 - wdt;
-            const uint32_t dm = (1u << wdt) - 1u;
-            for (uint32_t i = tid; i < 2048; i += kWG) hist[i] = 0;
-            __syncthreads();
-            for (uint64_t i = tid; i < n; i += kWG) {
-                const uint64_t k = kp[i];
-                if ((k & mask) == prefix) atomicAdd(&hist[(uint32_t)(k >> sh) & dm], 1u);
-            }
-            __syncthreads();
-            uint32_t loc[8], sm = 0;
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                loc[j] = hist[tid * 8 + j];
-                sm += loc[j];
-            }
-            const uint32_t excl = block_excl_scan256(sm, part);
-            if (excl < rem && rem <= excl + sm) {
-                uint32_t c = excl;
-                for (int j = 0; j < 8; j++) {
-                    if (rem <= c + loc[j]) {
-                        s_u[1] = tid * 8 + j;
-                        s_u[2] = c;
-                        s_u[3] = loc[j];
-                        break;
-                    }
-                    c += loc[j];
-                }
-            }
-            __syncthreads();
-            const uint32_t bin = s_u[1], below = s_u[2], cnt = s_u[3];
-            __syncthreads();
-            rem -= below;
-            prefix |= (uint64_t)bin << sh;
-            mask |= (uint64_t)dm << sh;
-            top = sh - 1;
-            if ((Kv - rem) + cnt <= Kv + kSelSlack) break;
-        }
-        if (tid == 0) s_u[0] = 0;
-        __syncthreads();
-        for (uint64_t i = tid; i < n; i += kWG) {
-            const uint64_t k = kp[i];
-            if ((k & mask) <= prefix) {
-                const uint32_t pos = atomicAdd(&s_u[0], 1u);
-                if (pos < kSelCap) buf[pos] = k;
-            }
-        }
-        __syncthreads();
-        m = min(s_u[0], kSelCap);
-    }
-    const uint32_t m2 = next_pow2(m < 2 ? 2 : m);
-    for (uint32_t i = m + tid; i < m2; i += kWG) buf[i] = kNoKey;
-    __syncthreads();
-    bitonic_sort_lds(buf, m2);
-    for (uint32_t i = tid; i < Kv; i += kWG) o[i] = buf[i];
-    if (tid == 0) out_cnt[seg] = Kv;
+    if (n <= 16 * 256)
+        select_body<16>(keys + b, n, K, out + (size_t)seg * K, out_cnt + seg, buf, hist, part, s_u,
+                        s_lh);
+    else
+        select_body<0>(keys + b, n, K, out + (size_t)seg * K, out_cnt + seg, buf, hist, part, s_u,
+                       s_lh);
 }
 
 // ---------------------------------------------------------------- planning

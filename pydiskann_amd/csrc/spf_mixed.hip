@@ -92,7 +92,8 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
     __shared__ float s_qn[kMixG];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 15, kg = lane >> 4;
-    for (uint32_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
+    for (XcdWalk it(p.n_items); it.more(); it.next()) {
+        const uint32_t item = it.item;
         uint32_t pp, pb, e_begin, e_end, pr0, gcnt;
         item_decode(p, item, pp, pb, e_begin, e_end, pr0, gcnt);
         __syncthreads();
@@ -218,35 +219,85 @@ __device__ __forceinline__ void find_bin256(const uint32_t *hist, uint32_t rem, 
 }
 
 // 32 keys of one bitmap word (64 bytes; segments are 32-slot aligned)
-__device__ __forceinline__ void load_word_keys(const uint16_t *kp, uint32_t w, uint32_t n,
-                                               uint16_t k[32]) {
+__device__ __forceinline__ void load_word_raw(const uint16_t *kp, uint32_t w, uint4 r[4]) {
     const uint4 *src = reinterpret_cast<const uint4 *>(kp + 32 * w);
+#pragma unroll
+    for (int j = 0; j < 4; j++) r[j] = src[j];
+}
+
+__device__ __forceinline__ void unpack_word(const uint4 r[4], uint32_t w, uint32_t n, uint16_t k[32]) {
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         union { uint4 v; uint16_t h[8]; } c;
-        c.v = src[j];
+        c.v = r[j];
 #pragma unroll
         for (int t = 0; t < 8; t++) k[8 * j + t] = (32 * w + 8 * j + t < n) ? c.h[t] : (uint16_t)0xffff;
     }
 }
 
-__global__ __launch_bounds__(256) void k_pool_select(const uint16_t *keys, const uint64_t *seg_off,
-                                                     const uint32_t *seg_cnt, uint32_t want,
-                                                     uint32_t *bits, uint32_t *bit_prefix,
-                                                     uint32_t *pool_want) {
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t s_w[4];
-    __shared__ uint32_t s_u[4];
-    const uint32_t q = blockIdx.x, tid = threadIdx.x;
-    const uint64_t b = seg_off[q];               // multiple of 32
-    const uint32_t n = seg_cnt[q];
+// REG: segments of at most 1024 words keep their keys in registers (thread t
+// holds words t, t+256, t+512, t+768); longer segments re-read L2 each pass.
+constexpr uint32_t kPoolProbeLds = 1024;   // probe tables held in LDS up to this n_probe
+
+// The query's probe table: segment-relative first slot and first entry of each
+// probed posting, in LDS for up to kPoolProbeLds probes, else read from the
+// plan buffers.
+struct ProbeTab {
+    uint32_t np;
+    const uint32_t *pbase, *ebase;      // LDS form (null when global)
+    const uint64_t *qp_row;             // global form
+    const uint32_t *probe_row, *post_off;
+    uint64_t seg;
+    __device__ __forceinline__ uint32_t rel(uint32_t j) const {
+        return pbase ? pbase[j] : (uint32_t)(qp_row[j] - seg);
+    }
+    // index of the probed posting holding `slot` (relative to the query's segment)
+    __device__ __forceinline__ uint32_t find(uint32_t slot) const {
+        uint32_t lo = 0, hi = np;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (rel(mid) <= slot) lo = mid; else hi = mid;
+        }
+        return lo;
+    }
+    __device__ __forceinline__ uint32_t first_entry(uint32_t j) const {
+        return pbase ? ebase[j] : post_off[probe_row[j]];
+    }
+    // slot -> entry, walking forward from probe j (slots visited in increasing order)
+    __device__ __forceinline__ uint32_t entry_from(uint32_t &j, uint32_t slot) const {
+        while (j + 1 < np && rel(j + 1) <= slot) j++;
+        return first_entry(j) + (slot - rel(j));
+    }
+};
+
+template <bool REG>
+__device__ __forceinline__ void pool_select_body(const uint16_t *kp, uint32_t n, uint32_t want,
+                                                 uint32_t *pool_entry, uint32_t *pool_want,
+                                                 const ProbeTab &tab, uint32_t *hist,
+                                                 uint32_t *s_w, uint32_t *s_u,
+                                                 uint32_t (*s_r)[4]) {
+    const uint32_t tid = threadIdx.x, q = blockIdx.x;
     const uint32_t nwords = (n + 31) / 32;
-    const uint16_t *kp = keys + b;
+    const uint32_t M = REG ? 4u : (nwords + 255) / 256;
+    uint4 kr[REG ? 4 : 1][4];
     uint16_t k[32];
+    auto fetch = [&](uint32_t m, uint32_t w) {
+        if (REG) {
+            unpack_word(kr[REG ? m : 0], w, n, k);
+        } else {
+            uint4 r[4];
+            load_word_raw(kp, w, r);
+            unpack_word(r, w, n, k);
+        }
+    };
     // pass 1: invalid count, min and max key
     uint32_t inval = 0, kmn = 0xffff, kmx = 0;
-    for (uint32_t w = tid; w < nwords; w += 256) {
-        load_word_keys(kp, w, n, k);
+#pragma unroll
+    for (uint32_t m = 0; m < M; m++) {
+        const uint32_t w = tid + 256 * m;
+        if (w >= nwords) continue;
+        if (REG) load_word_raw(kp, w, kr[REG ? m : 0]);
+        fetch(m, w);
 #pragma unroll
         for (int j = 0; j < 32; j++) {
             if (32 * w + j >= n) continue;
@@ -263,7 +314,6 @@ __global__ __launch_bounds__(256) void k_pool_select(const uint16_t *keys, const
         kmn = min(kmn, (uint32_t)__shfl_xor(kmn, off, 64));
         kmx = max(kmx, (uint32_t)__shfl_xor(kmx, off, 64));
     }
-    __shared__ uint32_t s_r[3][4];
     if ((tid & 63) == 0) {
         s_r[0][tid >> 6] = inval;
         s_r[1][tid >> 6] = kmn;
@@ -284,8 +334,11 @@ __global__ __launch_bounds__(256) void k_pool_select(const uint16_t *keys, const
         const uint32_t sh = span < 256 ? 0 : 32 - __clz(span) - 8;
         hist[tid] = 0;
         __syncthreads();
-        for (uint32_t w = tid; w < nwords; w += 256) {
-            load_word_keys(kp, w, n, k);
+#pragma unroll
+        for (uint32_t m = 0; m < M; m++) {
+            const uint32_t w = tid + 256 * m;
+            if (w >= nwords) continue;
+            fetch(m, w);
 #pragma unroll
             for (int j = 0; j < 32; j++)
                 if (k[j] != 0xffff) atomicAdd(&hist[(k[j] - kmn) >> sh], 1u);
@@ -298,8 +351,11 @@ __global__ __launch_bounds__(256) void k_pool_select(const uint16_t *keys, const
         if (sh > 0) {
             hist[tid] = 0;
             __syncthreads();
-            for (uint32_t w = tid; w < nwords; w += 256) {
-                load_word_keys(kp, w, n, k);
+#pragma unroll
+            for (uint32_t m = 0; m < M; m++) {
+                const uint32_t w = tid + 256 * m;
+                if (w >= nwords) continue;
+                fetch(m, w);
 #pragma unroll
                 for (int j = 0; j < 32; j++)
                     if (k[j] != 0xffff && ((k[j] - kmn) >> sh) == b1)
@@ -313,15 +369,17 @@ __global__ __launch_bounds__(256) void k_pool_select(const uint16_t *keys, const
         T = kmn + (b1 << sh) + b2;
         need = W - below1 - below2;   // keys equal to T to take, in slot order
     }
-    // collect in chunks of 256 words: thread t of chunk c owns word 256c + t, so
-    // tie ranks and mark positions follow slot order
+    // collect: chunk m holds words 256m .. 256m+255 in thread order, so tie ranks
+    // and mark positions follow slot order
     uint32_t tie_run = 0, mark_run = 0;
-    for (uint32_t c0 = 0; c0 < nwords; c0 += 256) {
-        const uint32_t w = c0 + tid;
+#pragma unroll
+    for (uint32_t m = 0; m < M; m++) {
+        if (256 * m >= nwords) break;
+        const uint32_t w = tid + 256 * m;
         const bool okw = w < nwords;
         uint32_t ties = 0;
         if (okw) {
-            load_word_keys(kp, w, n, k);
+            fetch(m, w);
             if (!all)
 #pragma unroll
                 for (int j = 0; j < 32; j++) ties += k[j] == T;
@@ -348,168 +406,136 @@ __global__ __launch_bounds__(256) void k_pool_select(const uint16_t *keys, const
             }
         }
         uint32_t mt;
-        const uint32_t pre = mark_run + block_scan_fast(__popc(word), s_w, &mt);
+        uint32_t pre = mark_run + block_scan_fast(__popc(word), s_w, &mt);
         mark_run += mt;
-        if (okw) {
-            bits[(b >> 5) + w] = word;
-            bit_prefix[(b >> 5) + w] = pre;
+        if (word) {   // pool positions follow slot order
+            uint32_t pj = tab.find(32 * w + (__ffs(word) - 1));
+            while (word) {
+                const uint32_t j = __ffs(word) - 1;
+                word &= word - 1;
+                pool_entry[(size_t)q * want + pre++] = tab.entry_from(pj, 32 * w + j);
+            }
         }
     }
 }
 
-// ---------------------------------------------------------------- rerank
-constexpr uint32_t kListCap = 2048;
+__global__ __launch_bounds__(256) void k_pool_select(const PoolParams p) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_u[4];
+    __shared__ uint32_t s_r[3][4];
+    __shared__ uint32_t s_pbase[kPoolProbeLds], s_ebase[kPoolProbeLds];
+    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    const uint64_t b = p.seg_off[q];             // multiple of 32
+    const uint32_t n = p.seg_cnt[q];
+    ProbeTab tab;
+    tab.np = p.np_q[q];
+    tab.qp_row = p.qp_slot + (size_t)q * p.npr;
+    tab.probe_row = p.probe_post + (size_t)q * p.npr;
+    tab.post_off = p.post_off;
+    tab.seg = b;
+    tab.pbase = tab.ebase = nullptr;
+    if (tab.np <= kPoolProbeLds) {
+        for (uint32_t j = tid; j < tab.np; j += 256) {
+            s_pbase[j] = (uint32_t)(tab.qp_row[j] - b);
+            s_ebase[j] = p.post_off[tab.probe_row[j]];
+        }
+        tab.pbase = s_pbase;
+        tab.ebase = s_ebase;
+    }
+    __syncthreads();
+    if ((n + 31) / 32 <= 1024)
+        pool_select_body<true>(p.keys + b, n, p.want, p.pool_entry, p.pool_want, tab, hist, s_w,
+                               s_u, s_r);
+    else
+        pool_select_body<false>(p.keys + b, n, p.want, p.pool_entry, p.pool_want, tab, hist, s_w,
+                                s_u, s_r);
+}
 
-__device__ __forceinline__ float quad_combine_m(float c) {
+// ---------------------------------------------------------------- query order
+// Counting sort of the batch by nearest probed posting: queries that share
+// postings then run side by side on one XCD and share raw rows in its L2.
+__global__ void k_perm_count(const uint32_t *np_q, const uint32_t *probe_post, uint32_t npr,
+                             uint32_t nq, uint32_t n_post, uint32_t *cnt) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const uint32_t key = np_q[q] ? probe_post[(size_t)q * npr] : n_post;
+    atomicAdd(&cnt[key], 1u);
+}
+
+__global__ void k_perm_scatter(const uint32_t *np_q, const uint32_t *probe_post, uint32_t npr,
+                               uint32_t nq, uint32_t n_post, const uint32_t *off, uint32_t *fill,
+                               uint32_t *qperm) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const uint32_t key = np_q[q] ? probe_post[(size_t)q * npr] : n_post;
+    qperm[off[key] + atomicAdd(&fill[key], 1u)] = q;
+}
+
+// ---------------------------------------------------------------- rerank
+// Exact f32 distance, four lanes per pair (lane r of the quad runs chain r over
+// units r, r+4, ...), H pairs per quad in flight; canonical 4-chain order.
+__device__ __forceinline__ float quad_sum(float c) {
     const float c01 = c + __shfl_xor(c, 1, 64);
     return c01 + __shfl_xor(c01, 2, 64);
 }
 
-// exact f32 distance, four lanes per pair, canonical 4-chain order
-__device__ __forceinline__ float quad_exact_m(uint32_t metric, const float4 *q4, const float4 *x4,
-                                              uint32_t units, uint32_t r, float qn) {
-    float acc = 0.0f, b2 = 0.0f;
+template <int H>
+__device__ __forceinline__ void quad_exact_h(uint32_t metric, const float4 *q4,
+                                             const float4 *const x4[H], uint32_t units, uint32_t r,
+                                             float qn, float out[H]) {
+    float acc[H], b2[H];
+#pragma unroll
+    for (int h = 0; h < H; h++) acc[h] = b2[h] = 0.0f;
     const bool l2 = metric == M_L2 || metric == M_L2SQ;
     for (uint32_t u = r; u < units; u += 4) {
-        const float4 a = q4[u], b = x4[u];
-        if (l2) {
-            float t = a.x - b.x; acc = __builtin_fmaf(t, t, acc);
-            t = a.y - b.y; acc = __builtin_fmaf(t, t, acc);
-            t = a.z - b.z; acc = __builtin_fmaf(t, t, acc);
-            t = a.w - b.w; acc = __builtin_fmaf(t, t, acc);
+        const float4 a = q4[u];
+        float4 x[H];
+#pragma unroll
+        for (int h = 0; h < H; h++) x[h] = x4[h][u];
+#pragma unroll
+        for (int h = 0; h < H; h++) {
+            const float4 b = x[h];
+            if (l2) {
+                float t = a.x - b.x; acc[h] = __builtin_fmaf(t, t, acc[h]);
+                t = a.y - b.y; acc[h] = __builtin_fmaf(t, t, acc[h]);
+                t = a.z - b.z; acc[h] = __builtin_fmaf(t, t, acc[h]);
+                t = a.w - b.w; acc[h] = __builtin_fmaf(t, t, acc[h]);
+            } else {
+                acc[h] = __builtin_fmaf(a.x, b.x, acc[h]);
+                acc[h] = __builtin_fmaf(a.y, b.y, acc[h]);
+                acc[h] = __builtin_fmaf(a.z, b.z, acc[h]);
+                acc[h] = __builtin_fmaf(a.w, b.w, acc[h]);
+                b2[h] = __builtin_fmaf(b.x, b.x, b2[h]);
+                b2[h] = __builtin_fmaf(b.y, b.y, b2[h]);
+                b2[h] = __builtin_fmaf(b.z, b.z, b2[h]);
+                b2[h] = __builtin_fmaf(b.w, b.w, b2[h]);
+            }
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+        const float a = quad_sum(acc[h]);
+        float d;
+        if (metric == M_L2SQ) {
+            d = a;
+        } else if (metric == M_L2) {
+            d = sqrtf(a);
+        } else if (metric == M_DOT) {
+            d = 1.0f - a;
         } else {
-            acc = __builtin_fmaf(a.x, b.x, acc);
-            acc = __builtin_fmaf(a.y, b.y, acc);
-            acc = __builtin_fmaf(a.z, b.z, acc);
-            acc = __builtin_fmaf(a.w, b.w, acc);
-            b2 = __builtin_fmaf(b.x, b.x, b2);
-            b2 = __builtin_fmaf(b.y, b.y, b2);
-            b2 = __builtin_fmaf(b.z, b.z, b2);
-            b2 = __builtin_fmaf(b.w, b.w, b2);
+            const float bb = quad_sum(b2[h]);
+            if (qn == 0.0f && bb == 0.0f) d = 0.0f;
+            else if (a == 0.0f) d = 1.0f;
+            else {
+                const float rr = 1.0f - a / sqrtf(qn * bb);
+                d = rr > 0.0f ? rr : 0.0f;
+            }
         }
-    }
-    acc = quad_combine_m(acc);
-    switch (metric) {
-    case M_L2SQ: return acc;
-    case M_L2: return sqrtf(acc);
-    case M_DOT: return 1.0f - acc;
-    default: {
-        b2 = quad_combine_m(b2);
-        if (qn == 0.0f && b2 == 0.0f) return 0.0f;
-        if (acc == 0.0f) return 1.0f;
-        const float rr = 1.0f - acc / sqrtf(qn * b2);
-        return rr > 0.0f ? rr : 0.0f;
-    }
+        out[h] = d;
     }
 }
 
-__global__ __launch_bounds__(256) void k_rerank_items(const MixedParams p) {
-    __shared__ uint64_t s_slot[kMixG];
-    __shared__ uint32_t s_qi[kMixG];
-    __shared__ float s_qn[kMixG];
-    __shared__ uint32_t s_list[kListCap];
-    __shared__ uint32_t s_at[kListCap];
-    __shared__ uint32_t s_wsum[4];
-    extern __shared__ __align__(16) float s_q[];   // kMixG x dpad
-    const uint32_t tid = threadIdx.x, r = tid & 3, units = p.dpad / 4;
-    for (uint32_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
-        uint32_t pp, pb, e_begin, e_end, pr0, gcnt;
-        item_decode(p, item, pp, pb, e_begin, e_end, pr0, gcnt);
-        __syncthreads();
-        if (tid < gcnt) {
-            s_qi[tid] = p.pair_q[pr0 + tid];
-            s_slot[tid] = p.pair_slot[pr0 + tid];
-            s_qn[tid] = p.qnorm ? p.qnorm[s_qi[tid]] : 0.0f;
-        }
-        __syncthreads();
-        const uint32_t ne = e_end - e_begin;
-        const uint32_t nw = (ne + 31) / 32;
-        const uint32_t nwords = gcnt * nw;
-        // deterministic positions: thread t owns words t, t+256, ...
-        uint32_t mine = 0;
-        for (uint32_t w = tid; w < nwords; w += 256) {
-            const uint32_t g = w / nw, wi = w - g * nw;
-            const uint64_t bit0 = s_slot[g] + (e_begin - pb) + 32ull * wi;
-            const uint64_t lo = p.bits[bit0 >> 5];
-            const uint64_t hi = p.bits[(bit0 >> 5) + 1];
-            uint32_t word = (uint32_t)(((hi << 32) | lo) >> (bit0 & 31));
-            const uint32_t nb = min(32u, ne - 32 * wi);
-            if (nb < 32) word &= (1u << nb) - 1u;
-            mine += __popc(word);
-        }
-        uint32_t total;
-        const uint32_t base = block_scan_fast(mine, s_wsum, &total);
-        if (total == 0) continue;
-        {
-            float4 *s4 = reinterpret_cast<float4 *>(s_q);
-            const float4 *q4 = reinterpret_cast<const float4 *>(p.qv);
-            for (uint32_t i = tid; i < gcnt * units; i += 256) {
-                const uint32_t g = i / units;
-                s4[i] = q4[(size_t)s_qi[g] * units + (i - g * units)];
-            }
-        }
-        for (uint32_t r0 = 0; r0 < total; r0 += kListCap) {
-            uint32_t pos = base;
-            for (uint32_t w = tid; w < nwords; w += 256) {
-                const uint32_t g = w / nw, wi = w - g * nw;
-                const uint64_t bit0 = s_slot[g] + (e_begin - pb) + 32ull * wi;
-                const uint64_t lo = p.bits[bit0 >> 5];
-                const uint64_t hi = p.bits[(bit0 >> 5) + 1];
-                uint32_t word = (uint32_t)(((hi << 32) | lo) >> (bit0 & 31));
-                const uint32_t nb = min(32u, ne - 32 * wi);
-                if (nb < 32) word &= (1u << nb) - 1u;
-                if (!word) continue;
-                // pool position of a marked slot: marks before its bitmap word in
-                // the query's segment + marks below it inside the word
-                const uint32_t wlo = (uint32_t)(bit0 >> 5), sft = (uint32_t)(bit0 & 31);
-                const uint32_t plo = p.bit_prefix[wlo], phi = p.bit_prefix[wlo + 1];
-                while (word) {
-                    const uint32_t bi = __ffs(word) - 1;
-                    word &= word - 1;
-                    if (pos >= r0 && pos < r0 + kListCap) {
-                        const uint32_t sb = sft + bi;   // bit within the (lo, hi) pair
-                        const uint32_t at = sb < 32
-                            ? plo + __popc((uint32_t)lo & ((1u << sb) - 1u))
-                            : phi + __popc((uint32_t)hi & ((1u << (sb - 32)) - 1u));
-                        s_list[pos - r0] = (g << 16) | (32 * wi + bi);
-                        s_at[pos - r0] = at;
-                    }
-                    pos++;
-                }
-            }
-            __syncthreads();
-            const uint32_t n = min(kListCap, total - r0);
-            for (uint32_t i0 = 0; i0 < n; i0 += 128) {     // 64 quads x 2 pairs in flight
-                uint32_t ii[2], gg[2], ee[2];
-                float dd[2];
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    ii[h] = i0 + 64 * h + (tid >> 2);
-                    const uint32_t v = ii[h] < n ? s_list[ii[h]] : 0u;
-                    gg[h] = v >> 16;
-                    ee[h] = e_begin + (v & 0xffff);
-                }
-#pragma unroll
-                for (int h = 0; h < 2; h++)
-                    dd[h] = quad_exact_m(p.metric,
-                                         reinterpret_cast<const float4 *>(s_q) + (size_t)gg[h] * units,
-                                         reinterpret_cast<const float4 *>(p.raw_e + (size_t)ee[h] * p.dpad),
-                                         units, r, s_qn[gg[h]]);
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    if (ii[h] < n && r == 0) {
-                        const uint32_t at = s_at[ii[h]];
-                        const uint64_t key = ((uint64_t)fkey_m(dd[h]) << 32) | p.rank[ee[h]];
-                        if (at < p.want) p.pool[(size_t)s_qi[gg[h]] * p.want + at] = key;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// ---------------------------------------------------------------- pool -> top-k
 __device__ void bitonic_lds(uint64_t *buf, uint32_t m) {
     for (uint32_t k = 2; k <= m; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -527,20 +553,52 @@ __device__ void bitonic_lds(uint64_t *buf, uint32_t m) {
         }
 }
 
-__global__ __launch_bounds__(256) void k_sort_topk(const uint64_t *pool, const uint32_t *pool_cnt,
-                                                   uint32_t want, uint32_t k, uint64_t *out,
-                                                   uint32_t *out_cnt) {
-    __shared__ uint64_t buf[kMaxK];
-    const uint32_t q = blockIdx.x, tid = threadIdx.x;
-    const uint32_t c = min(pool_cnt[q], want);
+constexpr int kRerankH = 4;   // pairs per quad in flight
+
+__global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
+    __shared__ uint64_t s_keys[kMaxK];
+    extern __shared__ __align__(16) float s_qv[];   // dpad floats
+    // XCD x takes a contiguous range of the nearest-posting order
+    const uint32_t x = blockIdx.x % kXcds, jx = blockIdx.x / kXcds;
+    const uint32_t per = (p.nq + kXcds - 1) / kXcds;
+    const uint32_t idx = x * per + jx;
+    if (jx >= per || idx >= p.nq) return;
+    const uint32_t q = p.qperm[idx];
+    const uint32_t tid = threadIdx.x, r = tid & 3, quad = tid >> 2, units = p.dpad / 4;
+    const uint32_t c = min(p.pool_want[q], p.want);
+    for (uint32_t i = tid; i < p.dpad; i += 256) s_qv[i] = p.qv[(size_t)q * p.dpad + i];
+    const float qn = p.qnorm ? p.qnorm[q] : 0.0f;
+    __syncthreads();
+    const uint32_t *pe = p.pool_entry + (size_t)q * p.want;
+    for (uint32_t i0 = 0; i0 < c; i0 += 64 * kRerankH) {
+        uint32_t ii[kRerankH], ee[kRerankH];
+        const float4 *xr[kRerankH];
+#pragma unroll
+        for (int h = 0; h < kRerankH; h++) {
+            ii[h] = i0 + 64 * h + quad;
+            ee[h] = ii[h] < c ? pe[ii[h]] : 0u;
+            xr[h] = reinterpret_cast<const float4 *>(p.raw_e + (size_t)ee[h] * p.dpad);
+        }
+        float d[kRerankH];
+        quad_exact_h<kRerankH>(p.metric, reinterpret_cast<const float4 *>(s_qv), xr, units, r, qn, d);
+        if (r == 0) {
+#pragma unroll
+            for (int h = 0; h < kRerankH; h++)
+                if (ii[h] < c) {
+                    const uint32_t u = __float_as_uint(d[h]);
+                    const uint32_t fk = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+                    s_keys[ii[h]] = ((uint64_t)fk << 32) | p.rank[ee[h]];
+                }
+        }
+    }
     uint32_t m = 2;
     while (m < c) m <<= 1;
-    for (uint32_t i = tid; i < m; i += 256) buf[i] = i < c ? pool[(size_t)q * want + i] : kNoKey;
+    for (uint32_t i = c + tid; i < m; i += 256) s_keys[i] = kNoKey;
     __syncthreads();
-    bitonic_lds(buf, m);
-    const uint32_t kk = min(k, c);
-    for (uint32_t i = tid; i < kk; i += 256) out[(size_t)q * k + i] = buf[i];
-    if (tid == 0) out_cnt[q] = kk;
+    bitonic_lds(s_keys, m);
+    const uint32_t kk = min(p.k, c);
+    for (uint32_t i = tid; i < kk; i += 256) p.out[(size_t)q * p.k + i] = s_keys[i];
+    if (tid == 0) p.out_cnt[q] = kk;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -550,28 +608,33 @@ hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) 
     return hipGetLastError();
 }
 
-hipError_t launch_pool_select(const uint16_t *keys, const uint64_t *seg_off,
-                              const uint32_t *seg_cnt, uint32_t nq, uint32_t want, uint32_t *bits,
-                              uint32_t *bit_prefix, uint32_t *pool_want, hipStream_t s) {
+hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s) {
     if (nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pool_select, dim3(nq), dim3(256), 0, s, keys, seg_off, seg_cnt, want,
-                       bits, bit_prefix, pool_want);
+    hipLaunchKernelGGL(k_pool_select, dim3(nq), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
-hipError_t launch_rerank_items(const MixedParams &p, uint32_t grid, hipStream_t s) {
-    if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rerank_items, dim3(grid), dim3(256), (size_t)kMixG * p.dpad * sizeof(float),
-                       s, p);
+hipError_t launch_rerank_pool(const RerankPoolParams &p, hipStream_t s) {
+    if (p.nq == 0) return hipSuccess;
+    const uint32_t grid = (p.nq + kXcds - 1) / kXcds * kXcds;
+    hipLaunchKernelGGL(k_rerank_pool, dim3(grid), dim3(256), p.dpad * sizeof(float), s, p);
     return hipGetLastError();
 }
 
-hipError_t launch_sort_topk(const uint64_t *pool, const uint32_t *pool_cnt, uint32_t nq,
-                            uint32_t want, uint32_t k, uint64_t *out, uint32_t *out_cnt,
-                            hipStream_t s) {
+hipError_t launch_query_perm(const uint32_t *np_q, const uint32_t *probe_post, uint32_t npr,
+                             uint32_t nq, uint32_t n_post, uint32_t *cnt, uint32_t *off,
+                             uint32_t *fill, uint32_t *qperm, hipStream_t s) {
     if (nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sort_topk, dim3(nq), dim3(256), 0, s, pool, pool_cnt, want, k, out,
-                       out_cnt);
+    hipError_t e = hipMemsetAsync(cnt, 0, (n_post + 2) * sizeof(uint32_t), s);
+    if (e != hipSuccess) return e;
+    e = hipMemsetAsync(fill, 0, (n_post + 2) * sizeof(uint32_t), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_perm_count, dim3((nq + 255) / 256), dim3(256), 0, s, np_q, probe_post,
+                       npr, nq, n_post, cnt);
+    e = launch_excl_scan_u32(cnt, off, n_post + 1, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_perm_scatter, dim3((nq + 255) / 256), dim3(256), 0, s, np_q, probe_post,
+                       npr, nq, n_post, off, fill, qperm);
     return hipGetLastError();
 }
 
