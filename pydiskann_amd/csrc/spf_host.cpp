@@ -80,8 +80,9 @@ struct Workspace {
     DBuf<uint64_t> out_ids;
     DBuf<float> out_d;
     DBuf<uint32_t> out_c;
-    DBuf<uint16_t> q16, cand16;
-    DBuf<float> qn16;
+    DBuf<uint16_t> q16, cand16, qbf, probe16;
+    DBuf<float> qn16, qnbf;
+    DBuf<uint32_t> probe_pool, probe_pool_cnt;
     DBuf<uint32_t> pool_entry, pool_want, cand_cnt, perm_cnt, perm_off, perm_fill, qperm;
     DBuf<uint64_t> pool;
     uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
@@ -90,6 +91,11 @@ struct Workspace {
     void release() {
         q16.release();
         cand16.release();
+        qbf.release();
+        probe16.release();
+        qnbf.release();
+        probe_pool.release();
+        probe_pool_cnt.release();
         pool_entry.release();
         cand_cnt.release();
         perm_cnt.release();
@@ -145,6 +151,10 @@ struct spf_index {
     bool codes16_owned = false;
     float *d_xn16 = nullptr;
     uint32_t units16 = 0;
+    // matrix-core probe: bf16 centroids centred on their mean, f32 rows for the recheck
+    float *d_mu = nullptr, *d_cnbf = nullptr, *d_cent_rows = nullptr;
+    uint4 *d_cbf = nullptr;
+    uint32_t cbf_units = 0;
     uint64_t device_bytes = 0;
 
     std::vector<uint64_t> rank_id;      // ids in ascending order
@@ -157,7 +167,8 @@ struct spf_index {
 
     ~spf_index() {
         void *ps[] = {d_cent, d_cnorm, d_post_off, d_post_len, d_codes, d_rank,
-                      d_rank_id, d_raw, d_del, d_xnorm, d_xsq, d_rank_entry, d_raw_e, d_xn16};
+                      d_rank_id, d_raw, d_del, d_xnorm, d_xsq, d_rank_entry, d_raw_e, d_xn16,
+                      d_mu, d_cnbf, d_cent_rows, d_cbf};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         if (codes16_owned && d_codes16) (void)hipFree(d_codes16);
@@ -344,6 +355,44 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
             tile_rows((const uint8_t *)d->centroids, P, 4 * d->dim, ix->cent_units, perm.data());
         HIPCHK(upload(&ix->d_cent, ct.data(), ct.size(), bytes));
     }
+    {
+        // bf16 probe operands: c - mu (mu = mean centroid) keeps the products small
+        const uint32_t dpad32 = round_up(d->dim, 32);
+        std::vector<double> mu(d->dim, 0.0);
+        for (uint32_t i = 0; i < P; i++)
+            for (uint32_t j = 0; j < d->dim; j++) mu[j] += d->centroids[(size_t)perm[i] * d->dim + j];
+        std::vector<float> muf(ix->dpad, 0.0f);
+        for (uint32_t j = 0; j < d->dim; j++) muf[j] = P ? (float)(mu[j] / P) : 0.0f;
+        auto bf16_rne = [](float f) -> uint16_t {
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+            return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+        };
+        std::vector<uint16_t> cb((size_t)P * dpad32, 0);
+        std::vector<float> cn(std::max<uint32_t>(P, 1), 0.0f), rows((size_t)P * ix->dpad, 0.0f);
+        for (uint32_t i = 0; i < P; i++) {
+            const float *c = d->centroids + (size_t)perm[i] * d->dim;
+            float acc = 0.0f;
+            for (uint32_t j = 0; j < d->dim; j++) {
+                const uint16_t h = bf16_rne(c[j] - muf[j]);
+                cb[(size_t)i * dpad32 + j] = h;
+                const uint32_t u = (uint32_t)h << 16;
+                float v;
+                std::memcpy(&v, &u, 4);
+                acc += v * v;
+                rows[(size_t)i * ix->dpad + j] = c[j];
+            }
+            cn[i] = acc;
+        }
+        ix->cbf_units = dpad32 / 8;
+        std::vector<uint4> ct = tile_rows((const uint8_t *)cb.data(), P, 2 * dpad32, ix->cbf_units,
+                                          nullptr);
+        HIPCHK(upload(&ix->d_cbf, ct.data(), ct.size(), bytes));
+        HIPCHK(upload(&ix->d_mu, muf.data(), muf.size(), bytes));
+        HIPCHK(upload(&ix->d_cnbf, cn.data(), cn.size(), bytes));
+        HIPCHK(upload(&ix->d_cent_rows, rows.data(), rows.size(), bytes));
+    }
     HIPCHK(upload(&ix->d_post_off, post_off.data(), post_off.size(), bytes));
     HIPCHK(upload(&ix->d_post_len, post_len.data(), post_len.size(), bytes));
     {
@@ -466,6 +515,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     pp.keys = w->cand16.p;
     pp.seg_off = w->cand_off.p;
     pp.seg_cnt = w->cand_cnt.p;
+    pp.map = 1;
     pp.want = want;
     pp.npr = npr;
     pp.np_q = w->np_q.p;
@@ -571,12 +621,23 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pp.q8 = w->q8.p;
     pp.qnorm = w->qnorm.p;
     pp.qsq = w->qsq.p;
+    // matrix-core probe for L2 metrics when it has something to prune
+    const bool mprobe = mixed && (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ) &&
+                        P > npr && ix->dim <= 256 && npr + npr / 4 + 32 <= kMaxK;
     if (mixed) {
         HIPCHK(w->q16.ensure((size_t)nq * dpad32));
         HIPCHK(w->qn16.ensure(nq));
         pp.q16 = w->q16.p;
         pp.dpad32 = dpad32;
         pp.qn16 = w->qn16.p;
+    }
+    if (mprobe) {
+        HIPCHK(w->qbf.ensure((size_t)nq * dpad32));
+        HIPCHK(w->qnbf.ensure(nq));
+        pp.mu = ix->d_mu;
+        pp.qbf = w->qbf.p;
+        pp.qnbf = w->qnbf.p;
+        pp.dpad32 = dpad32;
     }
     HIPCHK(launch_prep(pp, s));
     HIPCHK(mark(SPF_STAGE_PROBE));
@@ -590,10 +651,55 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     }
 
     // ---- probe: every query x every centroid, then top-npr per query
-    HIPCHK(w->probe_keys.ensure((size_t)nq * P));
     HIPCHK(w->probe_sel.ensure((size_t)nq * npr));
     HIPCHK(w->probe_cnt.ensure(nq));
-    {
+    if (mprobe) {
+        // bf16 matrix-core distances -> best npr + margin by 16-bit key -> exact
+        // f32 recheck of those centroids -> sorted probe list
+        const uint32_t Pp = round_up(P, 32);
+        const uint32_t band = std::min<uint32_t>(P, std::min<uint32_t>(kMaxK, std::max(npr + 32, npr + npr / 4)));
+        HIPCHK(w->probe16.ensure((size_t)nq * Pp));
+        HIPCHK(w->probe_pool.ensure((size_t)nq * band));
+        HIPCHK(w->probe_pool_cnt.ensure(nq));
+        ProbeMfmaParams pm{};
+        pm.qbf = (const uint4 *)w->qbf.p;
+        pm.q_units = dpad32 / 8;
+        pm.ks = dpad32 / 32;
+        pm.nq = nq;
+        pm.qnbf = w->qnbf.p;
+        pm.cbf = ix->d_cbf;
+        pm.c_units = ix->cbf_units;
+        pm.n_post = P;
+        pm.cnbf = ix->d_cnbf;
+        pm.out = w->probe16.p;
+        pm.out_stride = Pp;
+        HIPCHK(launch_probe_mfma(pm, s));
+        HIPCHK(mark(SPF_STAGE_PROBE_SELECT));
+        PoolParams pq{};
+        pq.keys = w->probe16.p;
+        pq.seg_stride = Pp;
+        pq.seg_len = P;
+        pq.map = 0;
+        pq.want = band;
+        pq.pool_entry = w->probe_pool.p;
+        pq.pool_want = w->probe_pool_cnt.p;
+        HIPCHK(launch_pool_select(pq, nq, s));
+        RerankPoolParams rc{};
+        rc.pool_entry = w->probe_pool.p;
+        rc.pool_want = w->probe_pool_cnt.p;
+        rc.nq = nq;
+        rc.want = band;
+        rc.k = npr;
+        rc.metric = ix->metric;
+        rc.dpad = dpad;
+        rc.qv = w->qv_probe.p;
+        rc.qnorm = w->qnorm.p;
+        rc.raw_e = ix->d_cent_rows;
+        rc.out = w->probe_sel.p;
+        rc.out_cnt = w->probe_cnt.p;
+        HIPCHK(launch_rerank_pool(rc, s));
+    } else {
+        HIPCHK(w->probe_keys.ensure((size_t)nq * P));
         ScanParams sp{};
         sp.data = ix->d_cent;
         sp.units = ix->cent_units;
@@ -609,10 +715,10 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         const uint64_t items = (uint64_t)((nq + Gv - 1) / Gv) * sp.n_echunks;
         const uint32_t grid = xcd_grid(items, 16384);
         HIPCHK(launch_scan(CK_F32, ix->metric, true, Gv, sp, grid, s));
+        HIPCHK(mark(SPF_STAGE_PROBE_SELECT));
+        HIPCHK(launch_select(w->probe_keys.p, nullptr, nullptr, P, nq, npr, w->probe_sel.p,
+                             w->probe_cnt.p, s));
     }
-    HIPCHK(mark(SPF_STAGE_PROBE_SELECT));
-    HIPCHK(launch_select(w->probe_keys.p, nullptr, nullptr, P, nq, npr, w->probe_sel.p,
-                         w->probe_cnt.p, s));
     HIPCHK(mark(SPF_STAGE_PLAN));
 
     // ---- plan: probe_ratio cut, candidate offsets, posting-major pairs

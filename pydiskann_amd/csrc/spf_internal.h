@@ -86,6 +86,22 @@ struct PrepParams {
     uint16_t *q16;              // [nq][dpad32] f16 query for the matrix-core scan (null = skip)
     uint32_t dpad32;            // dim rounded up to 32
     float *qn16;                // [nq] |f16(q)|^2
+    const float *mu;            // [dim] centring vector of the bf16 probe (null = skip)
+    uint16_t *qbf;              // [nq][dpad32] bf16(q - mu)
+    float *qnbf;                // [nq] |bf16(q - mu)|^2
+};
+
+// Matrix-core centroid probe: 16-bit keys of |c-mu|^2 + |q-mu|^2 - 2(q-mu).(c-mu)
+// in bf16 with f32 accumulate.
+struct ProbeMfmaParams {
+    const uint4 *qbf;           // [nq][q_units] bf16 rows
+    uint32_t q_units, ks, nq;
+    const float *qnbf;
+    const uint4 *cbf;           // tiled bf16 centred centroids
+    uint32_t c_units, n_post;
+    const float *cnbf;          // [n_post]
+    uint16_t *out;              // [nq][out_stride]
+    uint32_t out_stride;
 };
 
 // Mixed-precision scan / rerank over the same posting-major work items.
@@ -111,8 +127,11 @@ struct MixedParams {
 // entry indices (slot -> entry through the query's probe table).
 struct PoolParams {
     const uint16_t *keys;
-    const uint64_t *seg_off;    // [nq + 1] 32-aligned segments
-    const uint32_t *seg_cnt;    // [nq]
+    const uint64_t *seg_off;    // [nq + 1] 32-aligned segments (null: q * seg_stride)
+    const uint32_t *seg_cnt;    // [nq] (null: seg_len)
+    uint64_t seg_stride;
+    uint32_t seg_len;
+    uint32_t map;               // 1: write entries via the probe table; 0: write slots
     uint32_t want, npr;
     const uint32_t *np_q;       // [nq]
     const uint64_t *qp_slot;    // [nq][npr] absolute slot of each probed posting
@@ -132,7 +151,7 @@ struct RerankPoolParams {
     const float *qv;            // [nq][dpad]
     const float *qnorm;         // [nq]
     const float *raw_e;         // [n_entries][dpad] rows in entry order
-    const uint32_t *rank;       // per entry
+    const uint32_t *rank;       // per entry (null: the entry index itself)
     uint64_t *out;              // [nq][k]
     uint32_t *out_cnt;          // [nq]
 };
@@ -200,6 +219,7 @@ hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s);
 hipError_t launch_rerank_pool(const RerankPoolParams &p, hipStream_t s);
+hipError_t launch_probe_mfma(const ProbeMfmaParams &p, hipStream_t s);
 hipError_t launch_query_perm(const uint32_t *np_q, const uint32_t *probe_post, uint32_t npr,
                              uint32_t nq, uint32_t n_post, uint32_t *cnt, uint32_t *off,
                              uint32_t *fill, uint32_t *qperm, hipStream_t s);

@@ -79,6 +79,12 @@ __global__ __launch_bounds__(64) void k_prep(const PrepParams p) {
         for (uint32_t i = lane; i < p.dpad32; i += 64)
             p.q16[(size_t)q * p.dpad32 + i] = __builtin_bit_cast(uint16_t, (_Float16)(i < p.dim ? src[i] : 0.0f));
     }
+    if (p.qbf) {
+        for (uint32_t i = lane; i < p.dpad32; i += 64) {
+            const float v = i < p.dim ? src[i] - p.mu[i] : 0.0f;
+            p.qbf[(size_t)q * p.dpad32 + i] = __builtin_bit_cast(uint16_t, (__bf16)v);
+        }
+    }
     __syncthreads();
     if (p.want_q8) {
         const float alpha = 1.0f / p.scale;
@@ -110,6 +116,14 @@ __global__ __launch_bounds__(64) void k_prep(const PrepParams p) {
                 a16 = __builtin_fmaf(h, h, a16);
             }
             p.qn16[q] = a16;
+        }
+        if (p.qbf) {
+            float ab = 0.0f;
+            for (uint32_t i = 0; i < p.dim; i++) {
+                const float h = (float)(__bf16)(s_v[i] - p.mu[i]);
+                ab = __builtin_fmaf(h, h, ab);
+            }
+            p.qnbf[q] = ab;
         }
         if (p.want_q8) {
             uint32_t s = 0;

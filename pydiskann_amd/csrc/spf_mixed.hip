@@ -15,6 +15,8 @@
 // MFMA fragment layout (v_mfma_f32_16x16x32_{f16,bf16}, wave64), checked on the
 // device by spf_selftest: lane l holds A[l%16][8*(l/16) .. +7],
 // B[8*(l/16) .. +7][l%16], and D[4*(l/16) + j][l%16] for j = 0..3.
+#include <cstdlib>
+
 #include "spf_internal.h"
 
 namespace spf {
@@ -86,6 +88,7 @@ __device__ __forceinline__ void item_decode(const MixedParams &p, uint32_t item,
     e_end = min(e_begin + kEChunk, pe);
 }
 
+template <int KS>
 __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
     __shared__ uint64_t s_slot[kMixG];
     __shared__ uint32_t s_qi[kMixG];
@@ -106,36 +109,47 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
         }
         __syncthreads();
         // A fragments: queries (rows) x dims (K), two 16-row tiles
-        half8_t a[2][kMixKS];
+        half8_t a[2][KS];
 #pragma unroll
         for (int t = 0; t < 2; t++) {
             const uint32_t g = 16 * t + col;
             const bool ok = g < gcnt;
             const uint4 *qrow = p.q16 + (size_t)s_qi[g] * p.q16_units;
 #pragma unroll
-            for (int s = 0; s < kMixKS; s++) {
+            for (int s = 0; s < KS; s++) {
                 uint4 v = make_uint4(0, 0, 0, 0);
-                if (ok && s < (int)p.ks) v = qrow[4 * s + kg];
+                if (ok && s < (int)p.ks) v = qrow[4 * s + kg];   // rows hold 4*ks units
                 a[t][s] = as_half8(v);
             }
         }
-        for (uint32_t e0 = e_begin + 16 * wave; e0 < e_end; e0 += 64) {
+        // B fragments of this wave's 16-entry tiles, next tile loaded ahead
+        auto load_b = [&](uint32_t e0, uint4 (&bv)[KS]) {
             const uint32_t e = e0 + col;
             const bool valid = e < e_end;
             const uint4 *src = p.codes16 + (size_t)(e >> 6) * p.units * 64 + (e & 63);
+#pragma unroll
+            for (int s = 0; s < KS; s++) {
+                const uint32_t u = 4 * s + kg;
+                bv[s] = (valid && u < p.units) ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
+            }
+        };
+        uint32_t e0 = e_begin + 16 * wave;
+        uint4 bnext[KS];
+        if (e0 < e_end) load_b(e0, bnext);
+        for (; e0 < e_end; e0 += 64) {
+            uint4 bcur[KS];
+#pragma unroll
+            for (int s = 0; s < KS; s++) bcur[s] = bnext[s];
+            if (e0 + 64 < e_end) load_b(e0 + 64, bnext);
             floatx4_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
 #pragma unroll
-            for (int s = 0; s < kMixKS; s++) {
-                if (s < (int)p.ks) {
-                    const uint32_t u = 4 * s + kg;
-                    uint4 v = make_uint4(0, 0, 0, 0);
-                    if (valid && u < p.units) v = src[(size_t)u * 64];
-                    const half8_t b = as_half8(v);
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b, acc1, 0, 0, 0);
-                }
+            for (int s = 0; s < KS; s++) {
+                const half8_t b = as_half8(bcur[s]);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b, acc1, 0, 0, 0);
             }
-            if (valid) {
+            const uint32_t e = e0 + col;
+            if (e < e_end) {
                 const float xn = p.xn16[e];
                 const bool dead = p.del && ((p.del[e >> 5] >> (e & 31)) & 1u);
                 const uint32_t off = e - pb;
@@ -150,6 +164,65 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
                         const float d = xn + s_qn[g1] - 2.0f * acc1[j];
                         p.out16[s_slot[g1] + off] = dead ? (uint16_t)0xffff : (uint16_t)(fkey_m(d) >> 16);
                     }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- probe
+// One workgroup = 16 queries x 1024 centroids; each wave takes 16-centroid
+// tiles.  bf16 operands, f32 accumulate, 16-bit keys out.
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ bf16x8v as_bf8(uint4 v) {
+    union { uint4 u; bf16x8v h; } c;
+    c.u = v;
+    return c.h;
+}
+
+__global__ __launch_bounds__(256) void k_probe_mfma(const ProbeMfmaParams p) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t col = lane & 15, kg = lane >> 4;
+    const uint32_t nch = (p.n_post + kEChunk - 1) / kEChunk;
+    const uint32_t qt = blockIdx.x / nch, ch = blockIdx.x - qt * nch;
+    const uint32_t q0 = qt * 16;
+    const uint32_t c_begin = ch * kEChunk, c_end = min(c_begin + kEChunk, p.n_post);
+    bf16x8v a[kMixKS];
+    const uint32_t qa = q0 + col;
+#pragma unroll
+    for (int s = 0; s < kMixKS; s++) {
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (qa < p.nq && s < (int)p.ks) v = p.qbf[(size_t)qa * p.q_units + 4 * s + kg];
+        a[s] = as_bf8(v);
+    }
+    float qn[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t q = q0 + 4 * kg + j;
+        qn[j] = q < p.nq ? p.qnbf[q] : 0.0f;
+    }
+    for (uint32_t c0 = c_begin + 16 * wave; c0 < c_end; c0 += 64) {
+        const uint32_t c = c0 + col;
+        const bool valid = c < c_end;
+        const uint4 *src = p.cbf + (size_t)(c >> 6) * p.c_units * 64 + (c & 63);
+        floatx4_t acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < kMixKS; s++) {
+            if (s < (int)p.ks) {
+                const uint32_t u = 4 * s + kg;
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (valid && u < p.c_units) v = src[(size_t)u * 64];
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], as_bf8(v), acc, 0, 0, 0);
+            }
+        }
+        if (valid) {
+            const float cn = p.cnbf[c];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t q = q0 + 4 * kg + j;
+                if (q < p.nq) {
+                    const float d = cn + qn[j] - 2.0f * acc[j];
+                    p.out[(size_t)q * p.out_stride + c] = (uint16_t)(fkey_m(d) >> 16);
                 }
             }
         }
@@ -235,13 +308,8 @@ __device__ __forceinline__ void unpack_word(const uint4 r[4], uint32_t w, uint32
     }
 }
 
-// REG: segments of at most 1024 words keep their keys in registers (thread t
-// holds words t, t+256, t+512, t+768); longer segments re-read L2 each pass.
-constexpr uint32_t kPoolProbeLds = 1024;   // probe tables held in LDS up to this n_probe
-
 // The query's probe table: segment-relative first slot and first entry of each
-// probed posting, in LDS for up to kPoolProbeLds probes, else read from the
-// plan buffers.
+// probed posting, in LDS for short probe lists, else read from the plan buffers.
 struct ProbeTab {
     uint32_t np;
     const uint32_t *pbase, *ebase;      // LDS form (null when global)
@@ -270,15 +338,63 @@ struct ProbeTab {
     }
 };
 
-template <bool REG>
-__device__ __forceinline__ void pool_select_body(const uint16_t *kp, uint32_t n, uint32_t want,
-                                                 uint32_t *pool_entry, uint32_t *pool_want,
-                                                 const ProbeTab &tab, uint32_t *hist,
-                                                 uint32_t *s_w, uint32_t *s_u,
-                                                 uint32_t (*s_r)[4]) {
-    const uint32_t tid = threadIdx.x, q = blockIdx.x;
+// One wavefront per query (four queries per workgroup), so no step needs a
+// workgroup barrier: scans are wave shuffles and each wave owns its LDS
+// histogram.  Lane l holds bitmap-words l, l+64, ... of the segment (word w =
+// 32 candidate slots); REG keeps up to 4 words per lane in registers.
+__device__ __forceinline__ uint32_t wave_excl(uint32_t v, uint32_t *tot) {
+    const uint32_t incl = wave_incl_scan(v);
+    *tot = __shfl(incl, 63, 64);
+    return incl - v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// bin where the running count over hist[0..255] reaches rem: (bin, below)
+__device__ __forceinline__ void wave_find_bin(const uint32_t *hist, uint32_t rem, uint32_t &bin,
+                                              uint32_t &below) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t h[4], sm = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        h[i] = hist[4 * lane + i];
+        sm += h[i];
+    }
+    uint32_t tot;
+    const uint32_t excl = wave_excl(sm, &tot);
+    uint32_t b = 0, bl = 0;
+    const bool mine = excl < rem && rem <= excl + sm;
+    if (mine) {
+        uint32_t c = excl;
+        for (int i = 0; i < 4; i++) {
+            if (rem <= c + h[i]) {
+                b = 4 * lane + i;
+                bl = c;
+                break;
+            }
+            c += h[i];
+        }
+    }
+    const uint64_t who = __ballot(mine);
+    const int src = who ? __ffsll((unsigned long long)who) - 1 : 0;
+    bin = __shfl(b, src, 64);
+    below = __shfl(bl, src, 64);
+}
+
+// MODE 1: keys in registers (<= 256 words); 2: keys staged in LDS (`lds`,
+// the caller's copy of the segment); 0: keys re-read from global memory.
+template <int MODE>
+__device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32_t n,
+                                          uint32_t want, uint32_t *pool_entry,
+                                          uint32_t *pool_want, bool map, const ProbeTab &tab,
+                                          uint32_t *hist, const uint16_t *lds = nullptr) {
+    constexpr bool REG = MODE == 1;
+    const uint32_t lane = threadIdx.x & 63;
     const uint32_t nwords = (n + 31) / 32;
-    const uint32_t M = REG ? 4u : (nwords + 255) / 256;
+    const uint32_t M = REG ? 4u : (nwords + 63) / 64;
     uint4 kr[REG ? 4 : 1][4];
     uint16_t k[32];
     auto fetch = [&](uint32_t m, uint32_t w) {
@@ -286,15 +402,14 @@ __device__ __forceinline__ void pool_select_body(const uint16_t *kp, uint32_t n,
             unpack_word(kr[REG ? m : 0], w, n, k);
         } else {
             uint4 r[4];
-            load_word_raw(kp, w, r);
+            load_word_raw(MODE == 2 ? lds : kp, w, r);
             unpack_word(r, w, n, k);
         }
     };
-    // pass 1: invalid count, min and max key
     uint32_t inval = 0, kmn = 0xffff, kmx = 0;
 #pragma unroll
     for (uint32_t m = 0; m < M; m++) {
-        const uint32_t w = tid + 256 * m;
+        const uint32_t w = lane + 64 * m;
         if (w >= nwords) continue;
         if (REG) load_word_raw(kp, w, kr[REG ? m : 0]);
         fetch(m, w);
@@ -314,46 +429,40 @@ __device__ __forceinline__ void pool_select_body(const uint16_t *kp, uint32_t n,
         kmn = min(kmn, (uint32_t)__shfl_xor(kmn, off, 64));
         kmx = max(kmx, (uint32_t)__shfl_xor(kmx, off, 64));
     }
-    if ((tid & 63) == 0) {
-        s_r[0][tid >> 6] = inval;
-        s_r[1][tid >> 6] = kmn;
-        s_r[2][tid >> 6] = kmx;
-    }
-    __syncthreads();
-    inval = s_r[0][0] + s_r[0][1] + s_r[0][2] + s_r[0][3];
-    kmn = min(min(s_r[1][0], s_r[1][1]), min(s_r[1][2], s_r[1][3]));
-    kmx = max(max(s_r[2][0], s_r[2][1]), max(s_r[2][2], s_r[2][3]));
     const uint32_t nvalid = n - inval;
     const uint32_t W = min(want, nvalid);
-    if (tid == 0) pool_want[q] = W;
+    if (lane == 0) pool_want[q] = W;
     const bool all = W == nvalid;
     uint32_t T = 0, need = 0;
     if (!all) {
-        // adaptive range: bins over [kmin, kmax] so the keys spread over 256 bins
         const uint32_t span = kmx - kmn;
         const uint32_t sh = span < 256 ? 0 : 32 - __clz(span) - 8;
-        hist[tid] = 0;
-        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; i++) hist[4 * lane + i] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (uint32_t m = 0; m < M; m++) {
-            const uint32_t w = tid + 256 * m;
+            const uint32_t w = lane + 64 * m;
             if (w >= nwords) continue;
             fetch(m, w);
 #pragma unroll
             for (int j = 0; j < 32; j++)
                 if (k[j] != 0xffff) atomicAdd(&hist[(k[j] - kmn) >> sh], 1u);
         }
-        __syncthreads();
-        find_bin256(hist, W, s_w, s_u);
-        const uint32_t b1 = s_u[0], below1 = s_u[1];
-        __syncthreads();
-        uint32_t b2 = 0, below2 = 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        uint32_t b1, below1, b2 = 0, below2 = 0;
+        wave_find_bin(hist, W, b1, below1);
         if (sh > 0) {
-            hist[tid] = 0;
-            __syncthreads();
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int i = 0; i < 4; i++) hist[4 * lane + i] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (uint32_t m = 0; m < M; m++) {
-                const uint32_t w = tid + 256 * m;
+                const uint32_t w = lane + 64 * m;
                 if (w >= nwords) continue;
                 fetch(m, w);
 #pragma unroll
@@ -361,21 +470,19 @@ __device__ __forceinline__ void pool_select_body(const uint16_t *kp, uint32_t n,
                     if (k[j] != 0xffff && ((k[j] - kmn) >> sh) == b1)
                         atomicAdd(&hist[(k[j] - kmn) & ((1u << sh) - 1u)], 1u);
             }
-            __syncthreads();
-            find_bin256(hist, W - below1, s_w, s_u);
-            b2 = s_u[0];
-            below2 = s_u[1];
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            wave_find_bin(hist, W - below1, b2, below2);
         }
         T = kmn + (b1 << sh) + b2;
         need = W - below1 - below2;   // keys equal to T to take, in slot order
     }
-    // collect: chunk m holds words 256m .. 256m+255 in thread order, so tie ranks
-    // and mark positions follow slot order
+    // collect: chunk m = words 64m .. 64m+63 in lane order (slot order)
     uint32_t tie_run = 0, mark_run = 0;
 #pragma unroll
     for (uint32_t m = 0; m < M; m++) {
-        if (256 * m >= nwords) break;
-        const uint32_t w = tid + 256 * m;
+        if (64 * m >= nwords) break;
+        const uint32_t w = lane + 64 * m;
         const bool okw = w < nwords;
         uint32_t ties = 0;
         if (okw) {
@@ -385,7 +492,7 @@ __device__ __forceinline__ void pool_select_body(const uint16_t *kp, uint32_t n,
                 for (int j = 0; j < 32; j++) ties += k[j] == T;
         }
         uint32_t tt;
-        uint32_t tr = tie_run + block_scan_fast(ties, s_w, &tt);
+        uint32_t tr = tie_run + wave_excl(ties, &tt);
         tie_run += tt;
         uint32_t word = 0;
         if (okw) {
@@ -406,50 +513,51 @@ __device__ __forceinline__ void pool_select_body(const uint16_t *kp, uint32_t n,
             }
         }
         uint32_t mt;
-        uint32_t pre = mark_run + block_scan_fast(__popc(word), s_w, &mt);
+        uint32_t pre = mark_run + wave_excl(__popc(word), &mt);
         mark_run += mt;
         if (word) {   // pool positions follow slot order
-            uint32_t pj = tab.find(32 * w + (__ffs(word) - 1));
+            uint32_t pj = map ? tab.find(32 * w + (__ffs(word) - 1)) : 0u;
             while (word) {
                 const uint32_t j = __ffs(word) - 1;
                 word &= word - 1;
-                pool_entry[(size_t)q * want + pre++] = tab.entry_from(pj, 32 * w + j);
+                pool_entry[(size_t)q * want + pre++] = map ? tab.entry_from(pj, 32 * w + j)
+                                                           : 32 * w + j;
             }
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_pool_select(const PoolParams p) {
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t s_w[4];
-    __shared__ uint32_t s_u[4];
-    __shared__ uint32_t s_r[3][4];
-    __shared__ uint32_t s_pbase[kPoolProbeLds], s_ebase[kPoolProbeLds];
-    const uint32_t q = blockIdx.x, tid = threadIdx.x;
-    const uint64_t b = p.seg_off[q];             // multiple of 32
-    const uint32_t n = p.seg_cnt[q];
+constexpr uint32_t kWaveTab = 256;   // probe tables held in LDS up to this n_probe
+
+__global__ __launch_bounds__(256) void k_pool_select(const PoolParams p, uint32_t nq) {
+    __shared__ uint32_t hist[4][256];
+    __shared__ uint32_t s_pbase[4][kWaveTab], s_ebase[4][kWaveTab];
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t q = blockIdx.x * 4 + wv;
+    if (q >= nq) return;   // whole wave: nothing below synchronises the workgroup
+    const uint64_t b = p.seg_off ? p.seg_off[q] : (uint64_t)q * p.seg_stride;   // multiple of 32
+    const uint32_t n = p.seg_cnt ? p.seg_cnt[q] : p.seg_len;
     ProbeTab tab;
-    tab.np = p.np_q[q];
+    tab.np = p.map ? p.np_q[q] : 0u;
     tab.qp_row = p.qp_slot + (size_t)q * p.npr;
     tab.probe_row = p.probe_post + (size_t)q * p.npr;
     tab.post_off = p.post_off;
     tab.seg = b;
     tab.pbase = tab.ebase = nullptr;
-    if (tab.np <= kPoolProbeLds) {
-        for (uint32_t j = tid; j < tab.np; j += 256) {
-            s_pbase[j] = (uint32_t)(tab.qp_row[j] - b);
-            s_ebase[j] = p.post_off[tab.probe_row[j]];
+    if (tab.np && tab.np <= kWaveTab) {
+        for (uint32_t j = lane; j < tab.np; j += 64) {
+            s_pbase[wv][j] = (uint32_t)(tab.qp_row[j] - b);
+            s_ebase[wv][j] = p.post_off[tab.probe_row[j]];
         }
-        tab.pbase = s_pbase;
-        tab.ebase = s_ebase;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        tab.pbase = s_pbase[wv];
+        tab.ebase = s_ebase[wv];
     }
-    __syncthreads();
-    if ((n + 31) / 32 <= 1024)
-        pool_select_body<true>(p.keys + b, n, p.want, p.pool_entry, p.pool_want, tab, hist, s_w,
-                               s_u, s_r);
+    if ((n + 31) / 32 <= 256)
+        pool_wave<1>(p.keys + b, q, n, p.want, p.pool_entry, p.pool_want, p.map, tab, hist[wv]);
     else
-        pool_select_body<false>(p.keys + b, n, p.want, p.pool_entry, p.pool_want, tab, hist, s_w,
-                                s_u, s_r);
+        pool_wave<0>(p.keys + b, q, n, p.want, p.pool_entry, p.pool_want, p.map, tab, hist[wv]);
 }
 
 // ---------------------------------------------------------------- query order
@@ -563,7 +671,7 @@ __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
     const uint32_t per = (p.nq + kXcds - 1) / kXcds;
     const uint32_t idx = x * per + jx;
     if (jx >= per || idx >= p.nq) return;
-    const uint32_t q = p.qperm[idx];
+    const uint32_t q = p.qperm ? p.qperm[idx] : idx;
     const uint32_t tid = threadIdx.x, r = tid & 3, quad = tid >> 2, units = p.dpad / 4;
     const uint32_t c = min(p.pool_want[q], p.want);
     for (uint32_t i = tid; i < p.dpad; i += 256) s_qv[i] = p.qv[(size_t)q * p.dpad + i];
@@ -587,7 +695,7 @@ __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
                 if (ii[h] < c) {
                     const uint32_t u = __float_as_uint(d[h]);
                     const uint32_t fk = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-                    s_keys[ii[h]] = ((uint64_t)fk << 32) | p.rank[ee[h]];
+                    s_keys[ii[h]] = ((uint64_t)fk << 32) | (p.rank ? p.rank[ee[h]] : ee[h]);
                 }
         }
     }
@@ -604,13 +712,21 @@ __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
 // ---------------------------------------------------------------- launchers
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) {
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scan_mfma, dim3(grid), dim3(256), 0, s, p);
+    switch (p.ks) {   // 32-dim K steps: dim <= 32, 64, 96, 128, 256
+    case 1: hipLaunchKernelGGL(k_scan_mfma<1>, dim3(grid), dim3(256), 0, s, p); break;
+    case 2: hipLaunchKernelGGL(k_scan_mfma<2>, dim3(grid), dim3(256), 0, s, p); break;
+    case 3: hipLaunchKernelGGL(k_scan_mfma<3>, dim3(grid), dim3(256), 0, s, p); break;
+    case 4: hipLaunchKernelGGL(k_scan_mfma<4>, dim3(grid), dim3(256), 0, s, p); break;
+    case 5: case 6: case 7: case 8:
+        hipLaunchKernelGGL(k_scan_mfma<8>, dim3(grid), dim3(256), 0, s, p); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s) {
     if (nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pool_select, dim3(nq), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_pool_select, dim3((nq + 3) / 4), dim3(256), 0, s, p, nq);
     return hipGetLastError();
 }
 
@@ -618,6 +734,14 @@ hipError_t launch_rerank_pool(const RerankPoolParams &p, hipStream_t s) {
     if (p.nq == 0) return hipSuccess;
     const uint32_t grid = (p.nq + kXcds - 1) / kXcds * kXcds;
     hipLaunchKernelGGL(k_rerank_pool, dim3(grid), dim3(256), p.dpad * sizeof(float), s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_mfma(const ProbeMfmaParams &p, hipStream_t s) {
+    const uint32_t nch = (p.n_post + kEChunk - 1) / kEChunk;
+    const uint32_t grid = ((p.nq + 15) / 16) * nch;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_probe_mfma, dim3(grid), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
