@@ -85,6 +85,9 @@ struct Workspace {
     DBuf<uint32_t> probe_pool, probe_pool_cnt;
     DBuf<uint32_t> pool_entry, pool_want, cand_cnt, perm_cnt, perm_off, perm_fill, qperm;
     DBuf<uint64_t> pool;
+    DBuf<uint64_t> rq_planes;
+    DBuf<float> rq_consts;
+    DBuf<uint32_t> pair_src;
     uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
     hipEvent_t ev[SPF_N_STAGES + 1] = {};
 
@@ -104,6 +107,9 @@ struct Workspace {
         qperm.release();
         pool_want.release();
         pool.release();
+        rq_planes.release();
+        rq_consts.release();
+        pair_src.release();
         qn16.release();
         DBuf<float> *fs[] = {&qin, &qv_probe, &qv_scan, &qnorm, &out_d};
         for (auto *b : fs) b->release();
@@ -155,6 +161,11 @@ struct spf_index {
     float *d_mu = nullptr, *d_cnbf = nullptr, *d_cent_rows = nullptr;
     uint4 *d_cbf = nullptr;
     uint32_t cbf_units = 0;
+    // RaBitQ: rotation signs; with an L2 metric d_codes holds the bit codes
+    // (rq_l2), otherwise the decoded f32 vectors
+    float *d_rq_signs = nullptr;
+    uint32_t rq_padded = 0;
+    bool rq_l2 = false;
     uint64_t device_bytes = 0;
 
     std::vector<uint64_t> rank_id;      // ids in ascending order
@@ -168,7 +179,7 @@ struct spf_index {
     ~spf_index() {
         void *ps[] = {d_cent, d_cnorm, d_post_off, d_post_len, d_codes, d_rank,
                       d_rank_id, d_raw, d_del, d_xnorm, d_xsq, d_rank_entry, d_raw_e, d_xn16,
-                      d_mu, d_cnbf, d_cent_rows, d_cbf};
+                      d_mu, d_cnbf, d_cent_rows, d_cbf, d_rq_signs};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         if (codes16_owned && d_codes16) (void)hipFree(d_codes16);
@@ -248,8 +259,6 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
     *out = nullptr;
     if (d->quantizer == SPF_QUANT_PQ)
         return fail(SPF_EUNSUPPORTED, "PQ postings have no device path");
-    if (d->quantizer == SPF_QUANT_RABITQ)
-        return fail(SPF_EUNSUPPORTED, "RaBitQ postings are not served by this build yet");
     if (d->quantizer > SPF_QUANT_RABITQ) return fail(SPF_EINVAL, "unknown quantizer");
     if (d->metric > SPF_METRIC_DOT) return fail(SPF_EINVAL, "unknown metric");
     if (d->dim == 0 || d->dim > 4096) return fail(SPF_EINVAL, "dim must be in [1, 4096]");
@@ -264,6 +273,15 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
     case SPF_QUANT_NONE: want_cb = 4 * d->dim; break;
     case SPF_QUANT_F16: want_cb = 2 * d->dim; break;
     case SPF_QUANT_INT8: want_cb = d->dim; break;
+    case SPF_QUANT_RABITQ: {
+        uint32_t pad = 64;
+        while (pad < d->dim) pad *= 2;
+        if (d->rabitq_padded != pad)
+            return fail(SPF_EINVAL, "rabitq_padded must be max(dim, 64) rounded up to a power of two");
+        if (!d->rabitq_signs) return fail(SPF_EINVAL, "RaBitQ index needs its rotation signs");
+        want_cb = pad / 8 + 8;
+        break;
+    }
     }
     if (d->code_bytes != want_cb) return fail(SPF_EINVAL, "code_bytes does not match quantizer");
     if (d->quantizer == SPF_QUANT_INT8 && !(d->int8_scale > 0.0f))
@@ -302,6 +320,13 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
     case SPF_QUANT_NONE: ix->units = ix->dpad / 4; break;
     case SPF_QUANT_F16: ix->units = ix->dpad / 8; break;
     case SPF_QUANT_INT8: ix->units = ix->dpad / 16; break;
+    case SPF_QUANT_RABITQ:
+        ix->rq_padded = d->rabitq_padded;
+        ix->rq_l2 = d->metric == SPF_METRIC_L2 || d->metric == SPF_METRIC_L2SQ;
+        // bit codes: padded/64 words then (|r|^2, factor), 16-byte units;
+        // cosine / dot search decoded f32 vectors (src/spfresh.rs:346-348)
+        ix->units = ix->rq_l2 ? (d->code_bytes + 15) / 16 : ix->dpad / 4;
+        break;
     }
     ix->cent_units = ix->dpad / 4;
     std::unique_ptr<spf_index> guard(ix);
@@ -395,7 +420,48 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
     }
     HIPCHK(upload(&ix->d_post_off, post_off.data(), post_off.size(), bytes));
     HIPCHK(upload(&ix->d_post_len, post_len.data(), post_len.size(), bytes));
-    {
+    if (d->quantizer == SPF_QUANT_RABITQ) {
+        HIPCHK(upload(&ix->d_rq_signs, d->rabitq_signs, 3 * (size_t)ix->rq_padded, bytes));
+        const uint32_t bit_units = (d->code_bytes + 15) / 16;
+        std::vector<uint4> ct = tile_rows(d->codes, N, d->code_bytes, bit_units, entry_src.data());
+        if (ix->rq_l2) {
+            HIPCHK(upload(&ix->d_codes, ct.data(), ct.size(), bytes));
+        } else {
+            // decode_with once per entry into tiled f32 codes
+            uint4 *d_bits = nullptr;
+            uint32_t *d_entry_post = nullptr;
+            uint64_t tmp_bytes = 0;
+            std::vector<uint32_t> entry_post(N);
+            for (uint32_t i = 0; i < P; i++)
+                for (uint32_t e = post_off[i]; e < post_off[i + 1]; e++) entry_post[e] = i;
+            const size_t n_units = ((N + 63) / 64) * ix->units * 64;
+            hipError_t he = upload(&d_bits, ct.data(), ct.size(), tmp_bytes);
+            if (he == hipSuccess) he = upload(&d_entry_post, entry_post.data(), entry_post.size(), tmp_bytes);
+            if (he == hipSuccess) he = hipMalloc((void **)&ix->d_codes, std::max<size_t>(n_units, 1) * 16);
+            if (he == hipSuccess) {
+                bytes += std::max<size_t>(n_units, 1) * 16;
+                he = hipMemset(ix->d_codes, 0, std::max<size_t>(n_units, 1) * 16);
+            }
+            if (he == hipSuccess) {
+                RqDecodeParams dp{};
+                dp.codes = d_bits;
+                dp.units = bit_units;
+                dp.padded = ix->rq_padded;
+                dp.dpad = ix->dpad;
+                dp.dim = d->dim;
+                dp.out_units = ix->units;
+                dp.signs = ix->d_rq_signs;
+                dp.cent = ix->d_cent_rows;
+                dp.entry_post = d_entry_post;
+                dp.out = ix->d_codes;
+                he = launch_rq_decode(dp, (uint32_t)N, nullptr);
+            }
+            if (he == hipSuccess) he = hipDeviceSynchronize();
+            if (d_bits) (void)hipFree(d_bits);
+            if (d_entry_post) (void)hipFree(d_entry_post);
+            HIPCHK(he);
+        }
+    } else {
         std::vector<uint4> ct = tile_rows(d->codes, N, d->code_bytes, ix->units, entry_src.data());
         HIPCHK(upload(&ix->d_codes, ct.data(), ct.size(), bytes));
     }
@@ -478,7 +544,7 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
 template <class Mark>
 int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_t k,
                  uint32_t npr, uint32_t want, uint64_t total, uint32_t n_items, uint64_t *d_ids,
-                 float *d_dists, uint32_t *d_counts, Mark &mark) {
+                 float *d_dists, uint32_t *d_counts, Mark &mark, RqParams *rq) {
     const uint32_t dpad32 = round_up(ix->dim, 32);
     HIPCHK(w->cand16.ensure(std::max<uint64_t>(total, 1)));
     HIPCHK(w->pool_entry.ensure((size_t)nq * want));
@@ -509,7 +575,13 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     mp.out16 = w->cand16.p;
     const uint32_t grid = xcd_grid(n_items, 16384);
     HIPCHK(mark(SPF_STAGE_SCAN));
-    HIPCHK(launch_scan_mfma(mp, grid, s));
+    if (rq) {
+        rq->out16 = w->cand16.p;
+        HIPCHK(launch_rq_prep(*rq, nq * npr, s));
+        HIPCHK(launch_rq_scan(*rq, grid, s));
+    } else {
+        HIPCHK(launch_scan_mfma(mp, grid, s));
+    }
     HIPCHK(mark(SPF_STAGE_SELECT));
     PoolParams pp{};
     pp.keys = w->cand16.p;
@@ -557,10 +629,15 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     const uint32_t npr = std::min(std::max(n_probe, 1u), std::max(P, 1u));
     const bool mixed = opts && opts->precision == SPF_PREC_MIXED;
     if (opts && opts->precision > SPF_PREC_MIXED) return fail(SPF_EINVAL, "unknown precision");
-    if (mixed && (!ix->d_codes16 || ix->dim > 256))
+    if (mixed && !ix->rq_l2 && (!ix->d_codes16 || ix->dim > 256))
         return fail(SPF_EUNSUPPORTED,
-                    "mixed precision serves f32 and f16 postings with dim <= 256");
-    const bool rerank = !mixed && ix->quant != SPF_QUANT_NONE && ix->rerank_size > 0;
+                    "mixed precision serves f32 and f16 postings with dim <= 256 and RaBitQ "
+                    "postings with an L2 metric");
+    // RaBitQ + L2 takes search_rabitq (src/spfresh.rs:697-703): no rerank_size
+    const bool rerank =
+        !mixed && !ix->rq_l2 && ix->quant != SPF_QUANT_NONE && ix->rerank_size > 0;
+    if (ix->rq_l2 && !mixed && k > 1024)
+        return fail(SPF_EINVAL, "k above 1024 on RaBitQ postings");
     uint32_t want = rerank ? std::max(k, ix->rerank_size) : k;
     if (mixed) {
         const uint32_t pool = opts->rerank_size ? opts->rerank_size
@@ -577,7 +654,9 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         return (v == 4 || v == 8 || v == 16) ? v : 0u;
     }();
     const uint32_t Gv = dpad > 1024 ? 4 : (g_env ? g_env : 8);
-    const uint32_t G = mixed ? 32 : Gv;          // queries per posting-scan item
+    // queries per posting-scan item: matrix-core tiles take 32, the RaBitQ
+    // estimator scan 8
+    const uint32_t G = mixed ? (ix->rq_l2 ? 8u : 32u) : Gv;
     const bool prof = opts && (opts->flags & SPF_FLAG_PROFILE);
     if (prof)
         for (auto &e : w->ev)
@@ -733,6 +812,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(w->qp_slot.ensure((size_t)nq * npr));
     HIPCHK(w->pair_q.ensure((size_t)nq * npr));
     HIPCHK(w->pair_slot.ensure((size_t)nq * npr));
+    if (ix->rq_l2) HIPCHK(w->pair_src.ensure((size_t)nq * npr));
     HIPCHK(w->item_cnt.ensure(P + 1));
     HIPCHK(w->item_off.ensure(P + 1));
     PlanParams pl{};
@@ -755,6 +835,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.pair_q = w->pair_q.p;
     pl.pair_slot = w->pair_slot.p;
     pl.item_cnt = w->item_cnt.p;
+    pl.pair_src = ix->rq_l2 ? w->pair_src.p : nullptr;
     pl.G = G;
     HIPCHK(launch_plan_probe(pl, s));
     HIPCHK(launch_excl_scan_u64(w->cand_len.p, w->cand_off.p, nq, s));
@@ -778,10 +859,55 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.item_post = w->item_post.p;
     HIPCHK(launch_item_table(pl, s));
 
+    RqParams rq{};
+    if (ix->rq_l2) {
+        const uint32_t words = ix->rq_padded / 64;
+        HIPCHK(w->rq_planes.ensure((size_t)nq * npr * 4 * words));
+        HIPCHK(w->rq_consts.ensure((size_t)nq * npr * 8));
+        rq.qv = w->qv_probe.p;
+        rq.dpad = dpad;
+        rq.dim = ix->dim;
+        rq.padded = ix->rq_padded;
+        rq.words = words;
+        rq.cent = ix->d_cent_rows;
+        rq.signs = ix->d_rq_signs;
+        rq.probe_post = w->probe_post.p;
+        rq.np_q = w->np_q.p;
+        rq.npr = npr;
+        rq.nq = nq;
+        rq.planes = w->rq_planes.p;
+        rq.consts = w->rq_consts.p;
+        rq.post_off = ix->d_post_off;
+        rq.codes = ix->d_codes;
+        rq.units = ix->units;
+        rq.del = ix->d_del;
+        rq.rank = ix->d_rank;
+        rq.raw_e = ix->d_raw_e;
+        rq.k = k;
+        rq.metric = ix->metric;
+        rq.n_items = n_items;
+        rq.item_post = w->item_post.p;
+        rq.item_off = w->item_off.p;
+        rq.pair_off = w->pair_off.p;
+        rq.pair_src = w->pair_src.p;
+        rq.pair_slot = w->pair_slot.p;
+    }
     if (mixed) {
         int rc = search_mixed(ix, w, s, nq, k, npr, want, total, n_items, d_ids, d_dists,
-                              d_counts, mark);
+                              d_counts, mark, ix->rq_l2 ? &rq : nullptr);
         if (rc) return rc;
+    } else if (ix->rq_l2) {
+        // search_rabitq: per-pair query prep, then one wave per query
+        HIPCHK(w->fin.ensure((size_t)nq * k));
+        HIPCHK(w->fin_cnt.ensure(nq));
+        HIPCHK(mark(SPF_STAGE_SCAN));
+        HIPCHK(launch_rq_prep(rq, nq * npr, s));
+        rq.out = w->fin.p;
+        rq.out_cnt = w->fin_cnt.p;
+        HIPCHK(launch_rq_search_exact(rq, s));
+        HIPCHK(mark(SPF_STAGE_FINALIZE));
+        HIPCHK(launch_finalize(w->fin.p, k, w->fin_cnt.p, nq, k, ix->d_rank_id, d_ids, d_dists,
+                               d_counts, s));
     } else {
     // ---- posting scan
     HIPCHK(w->cand.ensure(std::max<uint64_t>(total, 1)));
@@ -854,7 +980,9 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         for (int i = 0; i < SPF_N_STAGES; i++)
             HIPCHK(hipEventElapsedTime(&g_prof.stage_ms[i], w->ev[i], w->ev[i + 1]));
         g_prof.candidates = total;
-        g_prof.scan_code_bytes = total * (mixed ? 2ull * ix->dim : (uint64_t)ix->code_bytes);
+        g_prof.scan_code_bytes =
+            total * (ix->quant == SPF_QUANT_RABITQ ? (uint64_t)ix->code_bytes
+                     : mixed ? 2ull * ix->dim : (uint64_t)ix->code_bytes);
         g_prof.probe_pairs = (uint64_t)nq * P;
         g_prof.scan_items = n_items;
         g_prof_valid = true;

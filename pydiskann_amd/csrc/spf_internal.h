@@ -123,6 +123,42 @@ struct MixedParams {
     uint16_t *out16;            // 16-bit candidate keys
 };
 
+// RaBitQ: per-pair query prep, exact search, mixed estimator scan.
+struct RqParams {
+    const float *qv;            // [nq][dpad] f32 queries
+    uint32_t dpad, dim, padded, words;   // words = padded / 64
+    const float *cent;          // [n_post][dpad] centroid rows
+    const float *signs;         // [3][padded]
+    const uint32_t *probe_post; // [nq][npr]
+    const uint32_t *np_q;       // [nq]
+    uint32_t npr, nq;
+    uint64_t *planes;           // [nq*npr][4*words]
+    float *consts;              // [nq*npr][8]
+    const uint32_t *post_off;   // [n_post + 1]
+    const uint4 *codes;         // tiled RaBitQ codes
+    uint32_t units;
+    const uint32_t *del;
+    const uint32_t *rank;
+    const float *raw_e;         // [n_entries][dpad]
+    uint32_t k, metric;
+    uint64_t *out;              // [nq][k]
+    uint32_t *out_cnt;
+    // mixed scan over posting-major items
+    uint32_t n_items;
+    const uint32_t *item_post, *item_off, *pair_off, *pair_src;
+    const uint64_t *pair_slot;
+    uint16_t *out16;
+};
+
+struct RqDecodeParams {
+    const uint4 *codes;
+    uint32_t units, padded, dpad, dim, out_units;
+    const float *signs;
+    const float *cent;          // [n_post][dpad]
+    const uint32_t *entry_post; // posting of each entry
+    uint4 *out;                 // tiled f32 codes (out_units per entry)
+};
+
 // Pool selection over 16-bit keys: the `want` smallest per query, written as
 // entry indices (slot -> entry through the query's probe table).
 struct PoolParams {
@@ -173,6 +209,7 @@ struct PlanParams {
     uint64_t *qp_slot;          // [nq * npr] slot of each (query, probe) pair
     uint32_t *pair_q;           // [nq * npr] posting-major pairs
     uint64_t *pair_slot;        // [nq * npr]
+    uint32_t *pair_src;         // [nq * npr] (query * npr + probe) of each posting-major pair
     uint32_t *item_cnt;         // [n_post + 1]
     const uint32_t *item_off;   // [n_post + 1]
     uint32_t *item_post;        // [n_items]
@@ -220,6 +257,10 @@ hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s);
 hipError_t launch_rerank_pool(const RerankPoolParams &p, hipStream_t s);
 hipError_t launch_probe_mfma(const ProbeMfmaParams &p, hipStream_t s);
+hipError_t launch_rq_prep(const RqParams &p, uint32_t n_pairs, hipStream_t s);
+hipError_t launch_rq_search_exact(const RqParams &p, hipStream_t s);
+hipError_t launch_rq_decode(const RqDecodeParams &p, uint32_t n, hipStream_t s);
+hipError_t launch_rq_scan(const RqParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_query_perm(const uint32_t *np_q, const uint32_t *probe_post, uint32_t npr,
                              uint32_t nq, uint32_t n_post, uint32_t *cnt, uint32_t *off,
                              uint32_t *fill, uint32_t *qperm, hipStream_t s);

@@ -577,6 +577,7 @@ __global__ void k_scatter_pairs(const PlanParams p) {
     const uint32_t pos = p.pair_off[post] + atomicAdd(&p.post_fill[post], 1u);
     p.pair_q[pos] = q;
     p.pair_slot[pos] = p.qp_slot[i];
+    if (p.pair_src) p.pair_src[pos] = i;
 }
 
 __global__ void k_items(const PlanParams p) {

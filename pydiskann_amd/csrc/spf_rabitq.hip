@@ -1,0 +1,411 @@
+// RaBitQ postings on gfx950 (src/rabitq.rs, src/spfresh.rs:731-770).
+//
+//   k_rq_prep         query_with(q, centroid_b) for every (query, probed posting)
+//                     pair: residual, sequential norm, 3 x (sign flip + FHT) in
+//                     LDS with the reference's butterfly pairing (so every element
+//                     is the same IEEE sum), 4-bit quantisation, bit planes by
+//                     wave ballot, estimator constants
+//   k_rq_search_exact search_rabitq semantics: one wave per query walks its
+//                     probes and entries in scan order; estimates, bounds and
+//                     exact distances run in parallel per 64-entry chunk, heap
+//                     updates are applied in entry order (the bound test of an
+//                     entry sees every earlier update, as in the reference)
+//   k_rq_decode       decode_with for every entry (cosine / dot postings are
+//                     searched on the decoded f32 vectors, src/spfresh.rs:346-348)
+//   k_rq_scan         mixed precision: posting-major estimator scan writing the
+//                     16-bit key of each estimate, for pool selection + exact rerank
+#include "spf_internal.h"
+
+namespace spf {
+
+__device__ __forceinline__ uint32_t fkey_r(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float_r(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// unnormalised fast Hadamard transform of v[0..n) in LDS, one wave;
+// pair (j, j+h) of stage h is the reference's (src/rabitq.rs:31-43)
+__device__ __forceinline__ void wave_fht(float *v, uint32_t n) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t h = 1; h < n; h *= 2) {
+        for (uint32_t idx = lane; idx < n / 2; idx += 64) {
+            const uint32_t j = (idx / h) * 2 * h + (idx % h);
+            const float x = v[j], y = v[j + h];
+            v[j] = x + y;
+            v[j + h] = x - y;
+        }
+        wave_sync_lds();
+    }
+}
+
+// ---------------------------------------------------------------- query prep
+__global__ __launch_bounds__(64) void k_rq_prep(const RqParams p) {
+    extern __shared__ float s_r[];   // padded floats
+    __shared__ float s_norm;
+    const uint32_t pair = blockIdx.x, lane = threadIdx.x;
+    const uint32_t q = pair / p.npr, j = pair - q * p.npr;
+    if (j >= p.np_q[q]) return;
+    const uint32_t post = p.probe_post[pair];
+    const float *qv = p.qv + (size_t)q * p.dpad;
+    const float *c = p.cent + (size_t)post * p.dpad;
+    const uint32_t P = p.padded;
+    for (uint32_t i = lane; i < P; i += 64) s_r[i] = i < p.dim ? qv[i] - c[i] : 0.0f;
+    wave_sync_lds();
+    if (lane == 0) {   // r.iter().map(|x| x * x).sum::<f32>(): sequential, unfused
+        float acc = 0.0f;
+        for (uint32_t i = 0; i < P; i++) {
+            const float sq = s_r[i] * s_r[i];
+            acc = acc + sq;
+        }
+        s_norm = sqrtf(acc);
+    }
+    wave_sync_lds();
+    const float norm = s_norm;
+    if (norm > 0.0f)
+        for (uint32_t i = lane; i < P; i += 64) s_r[i] = s_r[i] / norm;
+    wave_sync_lds();
+    for (uint32_t rd = 0; rd < 3; rd++) {
+        for (uint32_t i = lane; i < P; i += 64) s_r[i] = s_r[i] * p.signs[rd * P + i];
+        wave_sync_lds();
+        wave_fht(s_r, P);
+    }
+    float lo = 3.402823466e+38f, hi = -3.402823466e+38f;
+    for (uint32_t i = lane; i < P; i += 64) {
+        lo = fminf(lo, s_r[i]);
+        hi = fmaxf(hi, s_r[i]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, off, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+    }
+    const float levels = 15.0f;
+    const float delta = (hi - lo) / levels;
+    const float inv = delta > 0.0f ? 1.0f / delta : 0.0f;
+    const uint32_t W = P / 64;
+    uint64_t *planes = p.planes + (size_t)pair * 4 * p.words;
+    uint32_t sum_u = 0;
+    for (uint32_t m = 0; m < W; m++) {   // element i = 64 m + lane -> bit `lane` of word m
+        const float t = roundf((s_r[64 * m + lane] - lo) * inv);
+        uint32_t u = t > 0.0f ? (t >= 4294967295.0f ? 0xffffffffu : (uint32_t)t) : 0u;
+        u = u > 15u ? 15u : u;
+        sum_u += u;
+#pragma unroll
+        for (uint32_t b = 0; b < 4; b++) {
+            const uint64_t bits = __ballot((u >> b) & 1u);
+            if (lane == 0) planes[b * W + m] = bits;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) sum_u += __shfl_xor(sum_u, off, 64);
+    if (lane == 0) {
+        const float sd = sqrtf((float)P);
+        const float a = delta * (float)sum_u;
+        const float b = (float)P * lo;
+        float *cs = p.consts + (size_t)pair * 8;
+        cs[0] = norm * norm;
+        cs[1] = 2.0f * norm;
+        cs[2] = 2.0f * delta / sd;
+        cs[3] = 2.0f * lo / sd;
+        cs[4] = -(a + b) / sd;
+    }
+}
+
+// estimate and error radius of one code against prepared planes
+// (distance_with_bound, src/rabitq.rs:202-215); bits in 64-bit words
+template <uint32_t W>
+__device__ __forceinline__ void rq_est(const uint64_t *code_words, float n2, float fac,
+                                       const uint64_t *planes, const float *cs, uint32_t P,
+                                       float &est, float &err) {
+    uint32_t S = 0, C = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < W; i++) {
+        const uint64_t b = code_words[i];
+        C += __popcll(b);
+        S += __popcll(b & planes[i]) + (__popcll(b & planes[W + i]) << 1) +
+             (__popcll(b & planes[2 * W + i]) << 2) + (__popcll(b & planes[3 * W + i]) << 3);
+    }
+    const float nq2 = cs[0], sq = cs[1], k1 = cs[2], k2 = cs[3], k3 = cs[4];
+    float inner = k1 * (float)S + k2 * (float)C;
+    inner = inner + k3;
+    const float sf = sq * fac;
+    const float e = (n2 + nq2) - sf * inner;
+    est = fmaxf(e, 0.0f);
+    const float ip = fac > 0.0f ? sqrtf(n2) / fac : 0.0f;
+    if (ip > 0.0f) {
+        const float one = fmaxf(1.0f - ip * ip, 0.0f);
+        const float rad = sqrtf(one / ((float)P - 1.0f));
+        err = sf * 1.9f * rad;
+    } else {
+        err = __builtin_inff();
+    }
+}
+
+// code of entry e from the tiled layout: bit words and (n2, fac)
+template <uint32_t W>
+__device__ __forceinline__ void rq_load_code(const uint4 *codes, uint32_t units, uint32_t e,
+                                             uint64_t *words, float &n2, float &fac) {
+    const uint4 *src = codes + (size_t)(e >> 6) * units * 64 + (e & 63);
+#pragma unroll
+    for (uint32_t u = 0; u < W / 2; u++) {
+        const uint4 v = src[(size_t)u * 64];
+        words[2 * u] = ((uint64_t)v.y << 32) | v.x;
+        words[2 * u + 1] = ((uint64_t)v.w << 32) | v.z;
+    }
+    if (W & 1) {   // padded = 64: one word, then n2/fac in the same unit
+        const uint4 v = src[0];
+        words[0] = ((uint64_t)v.y << 32) | v.x;
+        n2 = __uint_as_float(v.z);
+        fac = __uint_as_float(v.w);
+        return;
+    }
+    const uint4 v = src[(size_t)(W / 2) * 64];
+    n2 = __uint_as_float(v.x);
+    fac = __uint_as_float(v.y);
+}
+
+// exact squared L2 of a raw row, canonical 4-chain order
+__device__ __forceinline__ float l2sq_row(const float *q, const float *x, uint32_t dpad) {
+    float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const float4 *q4 = reinterpret_cast<const float4 *>(q);
+    const float4 *x4 = reinterpret_cast<const float4 *>(x);
+    for (uint32_t u = 0; u < dpad / 4; u += 4) {
+#pragma unroll
+        for (uint32_t r = 0; r < 4; r++) {
+            const float4 a = q4[u + r], b = x4[u + r];
+            float t = a.x - b.x; c[r] = __builtin_fmaf(t, t, c[r]);
+            t = a.y - b.y; c[r] = __builtin_fmaf(t, t, c[r]);
+            t = a.z - b.z; c[r] = __builtin_fmaf(t, t, c[r]);
+            t = a.w - b.w; c[r] = __builtin_fmaf(t, t, c[r]);
+        }
+    }
+    return (c[0] + c[1]) + (c[2] + c[3]);
+}
+
+// ---------------------------------------------------------------- exact search
+constexpr uint32_t kRqMaxK = 1024;
+
+template <uint32_t W>
+__global__ __launch_bounds__(64) void k_rq_search_exact(const RqParams p) {
+    __shared__ uint64_t s_heap[kRqMaxK];
+    __shared__ uint64_t s_pl[4 * W];
+    __shared__ float s_cs[8];
+    __shared__ uint32_t s_cnt;
+    const uint32_t q = blockIdx.x, lane = threadIdx.x;
+    const uint32_t P = p.padded, k = p.k;
+    const float *qv = p.qv + (size_t)q * p.dpad;
+    if (lane == 0) s_cnt = 0;
+    wave_sync_lds();
+    float kth = __builtin_inff();
+    const uint32_t np = p.np_q[q];
+    for (uint32_t jp = 0; jp < np; jp++) {
+        const uint32_t pair = q * p.npr + jp;
+        const uint32_t post = p.probe_post[pair];
+        for (uint32_t i = lane; i < 4 * W; i += 64) s_pl[i] = p.planes[(size_t)pair * 4 * W + i];
+        if (lane < 5) s_cs[lane] = p.consts[(size_t)pair * 8 + lane];
+        wave_sync_lds();
+        const uint32_t eb = p.post_off[post], ee = p.post_off[post + 1];
+        for (uint32_t e0 = eb; e0 < ee; e0 += 64) {
+            const uint32_t e = e0 + lane;
+            const bool valid = e < ee && !(p.del && ((p.del[e >> 5] >> (e & 31)) & 1u));
+            float lb = __builtin_inff(), exact = 0.0f;
+            if (valid) {
+                uint64_t words[W];
+                float n2, fac, est, err;
+                rq_load_code<W>(p.codes, p.units, e, words, n2, fac);
+                rq_est<W>(words, n2, fac, s_pl, s_cs, P, est, err);
+                lb = est - err;
+                if (lb < kth) exact = l2sq_row(qv, p.raw_e + (size_t)e * p.dpad, p.dpad);
+            }
+            uint64_t mask = __ballot(valid && lb < kth);
+            const uint32_t rank = valid ? p.rank[e] : 0u;
+            while (mask) {   // heap updates in entry order
+                const int src = __ffsll((unsigned long long)mask) - 1;
+                mask &= mask - 1;
+                const float lb_i = __shfl(lb, src, 64);
+                const float ex_i = __shfl(exact, src, 64);
+                const uint32_t rk_i = __shfl(rank, src, 64);
+                if (!(lb_i < kth)) continue;
+                const uint64_t key = ((uint64_t)fkey_r(ex_i) << 32) | rk_i;
+                if (lane == 0) {
+                    uint32_t cnt = s_cnt;
+                    bool ins = false;
+                    if (cnt < k) {
+                        s_heap[cnt] = key;
+                        cnt++;
+                        ins = true;
+                    } else if (ex_i < kth) {
+                        s_heap[cnt - 1] = key;
+                        ins = true;
+                    }
+                    if (ins)
+                        for (uint32_t i = cnt - 1; i > 0 && s_heap[i] < s_heap[i - 1]; i--) {
+                            const uint64_t t = s_heap[i];
+                            s_heap[i] = s_heap[i - 1];
+                            s_heap[i - 1] = t;
+                        }
+                    s_cnt = cnt;
+                }
+                wave_sync_lds();
+                const uint32_t cnt = s_cnt;
+                kth = cnt < k ? __builtin_inff() : key_float_r((uint32_t)(s_heap[cnt - 1] >> 32));
+            }
+        }
+        wave_sync_lds();
+    }
+    const uint32_t cnt = s_cnt;
+    // DistL2 returns the root (src/spfresh.rs:766); re-key and re-sort by (dist, id)
+    if (p.metric == M_L2) {
+        for (uint32_t i = lane; i < cnt; i += 64) {
+            const float v = sqrtf(key_float_r((uint32_t)(s_heap[i] >> 32)));
+            s_heap[i] = ((uint64_t)fkey_r(v) << 32) | (uint32_t)s_heap[i];
+        }
+        wave_sync_lds();
+        if (lane == 0)
+            for (uint32_t i = 1; i < cnt; i++)
+                for (uint32_t t = i; t > 0 && s_heap[t] < s_heap[t - 1]; t--) {
+                    const uint64_t x = s_heap[t];
+                    s_heap[t] = s_heap[t - 1];
+                    s_heap[t - 1] = x;
+                }
+        wave_sync_lds();
+    }
+    for (uint32_t i = lane; i < cnt; i += 64) p.out[(size_t)q * k + i] = s_heap[i];
+    if (lane == 0) p.out_cnt[q] = cnt;
+}
+
+// ---------------------------------------------------------------- decode
+// decode_with (src/rabitq.rs:224-238) of entry e into tiled f32 codes
+template <uint32_t W>
+__global__ __launch_bounds__(64) void k_rq_decode(const RqDecodeParams p) {
+    extern __shared__ float s_r[];
+    const uint32_t e = blockIdx.x, lane = threadIdx.x;
+    const uint32_t P = p.padded;
+    uint64_t words[W];
+    float n2, fac;
+    rq_load_code<W>(p.codes, p.units, e, words, n2, fac);
+    const float sd = sqrtf((float)P);
+    for (uint32_t i = lane; i < P; i += 64) {
+        const uint64_t w = words[i >> 6];
+        s_r[i] = ((w >> (i & 63)) & 1u) ? 1.0f / sd : -1.0f / sd;
+    }
+    wave_sync_lds();
+    for (int rd = 2; rd >= 0; rd--) {
+        wave_fht(s_r, P);
+        for (uint32_t i = lane; i < P; i += 64) s_r[i] = s_r[i] * p.signs[(uint32_t)rd * P + i];
+        wave_sync_lds();
+    }
+    const float norm = sqrtf(n2);
+    const float *c = p.cent + (size_t)p.entry_post[e] * p.dpad;
+    uint32_t *dst = reinterpret_cast<uint32_t *>(p.out);
+    for (uint32_t i = lane; i < p.out_units * 4; i += 64) {
+        float v = 0.0f;
+        if (i < p.dim) {
+            const float t = norm * s_r[i];
+            v = c[i] + t;
+        }
+        // tiled f32 layout: unit i/4 of entry e, word i%4
+        dst[(((size_t)(e >> 6) * p.out_units + i / 4) * 64 + (e & 63)) * 4 + (i & 3)] = __float_as_uint(v);
+    }
+}
+
+// ---------------------------------------------------------------- mixed scan
+// Posting-major items of <= 8 queries: lane = entry, popcount estimator per
+// (entry, query), 16-bit key of the estimate out.
+constexpr uint32_t kRqG = 8;
+
+template <uint32_t W>
+__global__ __launch_bounds__(256) void k_rq_scan(const RqParams p) {
+    __shared__ uint64_t s_pl[kRqG][4 * W];
+    __shared__ float s_cs[kRqG][8];
+    __shared__ uint64_t s_slot[kRqG];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t P = p.padded;
+    for (XcdWalk it(p.n_items); it.more(); it.next()) {
+        const uint32_t item = it.item;
+        const uint32_t pp = p.item_post[item];
+        const uint32_t local = item - p.item_off[pp];
+        const uint32_t pb = p.post_off[pp], pe = p.post_off[pp + 1];
+        const uint32_t nec = (pe - pb + kEChunk - 1) / kEChunk;
+        const uint32_t qc = local / nec, ec = local - qc * nec;
+        const uint32_t pr0 = p.pair_off[pp] + qc * kRqG;
+        const uint32_t gcnt = min(kRqG, p.pair_off[pp + 1] - pr0);
+        const uint32_t e_begin = pb + ec * kEChunk, e_end = min(e_begin + kEChunk, pe);
+        __syncthreads();
+        for (uint32_t i = tid; i < gcnt * 4 * W; i += 256) {
+            const uint32_t g = i / (4 * W), w = i - g * 4 * W;
+            s_pl[g][w] = p.planes[(size_t)p.pair_src[pr0 + g] * 4 * W + w];
+        }
+        if (tid < gcnt * 8) {
+            const uint32_t g = tid / 8, c = tid - g * 8;
+            s_cs[g][c] = p.consts[(size_t)p.pair_src[pr0 + g] * 8 + c];
+        }
+        if (tid < gcnt) s_slot[tid] = p.pair_slot[pr0 + tid];
+        __syncthreads();
+        for (uint32_t e0 = e_begin + 64 * wave; e0 < e_end; e0 += 256) {
+            const uint32_t e = e0 + lane;
+            if (e >= e_end) continue;
+            uint64_t words[W];
+            float n2, fac;
+            rq_load_code<W>(p.codes, p.units, e, words, n2, fac);
+            const bool dead = p.del && ((p.del[e >> 5] >> (e & 31)) & 1u);
+            for (uint32_t g = 0; g < gcnt; g++) {
+                float est, err;
+                rq_est<W>(words, n2, fac, s_pl[g], s_cs[g], P, est, err);
+                p.out16[s_slot[g] + (e - pb)] = dead ? (uint16_t)0xffff : (uint16_t)(fkey_r(est) >> 16);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+#define SPF_RQ_DISPATCH(W_, LAUNCH)                                               \
+    switch (W_) {                                                                 \
+    case 1: LAUNCH(1); break;                                                     \
+    case 2: LAUNCH(2); break;                                                     \
+    case 4: LAUNCH(4); break;                                                     \
+    case 8: LAUNCH(8); break;                                                     \
+    case 16: LAUNCH(16); break;                                                   \
+    case 32: LAUNCH(32); break;                                                   \
+    case 64: LAUNCH(64); break;                                                   \
+    default: return hipErrorInvalidValue;                                         \
+    }
+
+hipError_t launch_rq_prep(const RqParams &p, uint32_t n_pairs, hipStream_t s) {
+    if (n_pairs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rq_prep, dim3(n_pairs), dim3(64), p.padded * sizeof(float), s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_rq_search_exact(const RqParams &p, hipStream_t s) {
+    if (p.nq == 0) return hipSuccess;
+#define L(W) hipLaunchKernelGGL(k_rq_search_exact<W>, dim3(p.nq), dim3(64), 0, s, p)
+    SPF_RQ_DISPATCH(p.words, L)
+#undef L
+    return hipGetLastError();
+}
+
+hipError_t launch_rq_decode(const RqDecodeParams &p, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+#define L(W) hipLaunchKernelGGL(k_rq_decode<W>, dim3(n), dim3(64), p.padded * sizeof(float), s, p)
+    SPF_RQ_DISPATCH(p.padded / 64, L)
+#undef L
+    return hipGetLastError();
+}
+
+hipError_t launch_rq_scan(const RqParams &p, uint32_t grid, hipStream_t s) {
+    if (grid == 0) return hipSuccess;
+#define L(W) hipLaunchKernelGGL(k_rq_scan<W>, dim3(grid), dim3(256), 0, s, p)
+    SPF_RQ_DISPATCH(p.words, L)
+#undef L
+    return hipGetLastError();
+}
+
+}  // namespace spf

@@ -65,6 +65,13 @@ def test_argument_errors_without_a_device():
     d = _lib.IndexDesc()
     d.quantizer = 4
     assert L.spf_index_create(C.byref(d), C.byref(h)) == _lib.SPF_EUNSUPPORTED
+    # RaBitQ: padded width must be max(dim, 64) rounded up to a power of two
+    d.quantizer, d.metric, d.dim, d.rabitq_padded = 3, 1, 100, 100
+    assert L.spf_index_create(C.byref(d), C.byref(h)) == _lib.SPF_EINVAL
+    assert L.spf_last_error().startswith(b"rabitq_padded")
+    d.rabitq_padded = 128
+    assert L.spf_index_create(C.byref(d), C.byref(h)) == _lib.SPF_EINVAL
+    assert L.spf_last_error() == b"RaBitQ index needs its rotation signs"
     assert L.spf_search_batch(None, None, 1, 4, 10, 8, None, None, None, None) == _lib.SPF_EINVAL
     assert L.spf_delete(None, None, 0) == _lib.SPF_EINVAL
     assert L.spf_index_get_info(None, None) == _lib.SPF_EINVAL
