@@ -6,9 +6,15 @@ Gaussians, centres U[0,200], sigma 20, clamped to [0,255]), 10k queries from the
 same mixture, L2, 4,096 postings, n_probe 64, k 100.  One step = one batch
 search of all 10k queries with queries and outputs resident in HBM.
 
-Prints one JSON line: QPS, per-batch p50/p99, recall@10 against an FP64
-brute-force oracle on a query sample, the posting-scan roofline measured with
-HIP events inside the timed region, and the CPU oracle timed on this host.
+The headline precision is mixed (bf16 matrix-core probe, f16 matrix-core scan
+with f32 accumulate, exact f32 rerank of a 1,000-candidate pool); the plain f32
+GPU path is timed in the same run on the same index and queries, and the line
+carries both QPS values and their ratio.  Prints one JSON line: QPS, per-batch
+p50/p99, recall@10 against an FP64 brute-force oracle on a query sample, the
+roofline of the dominant kernel measured with HIP events inside the timed
+region (HBM traffic per launch from the committed rocprofv3 counter summary of
+the same kernel and workload, when there is one), and the CPU oracle timed on
+this host.
 """
 from __future__ import annotations
 
@@ -23,6 +29,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "QPS at recall@10>=0.95, 1M x 128, mixed-precision vs fp32 GPU; p99 ms"
 HBM_PEAK_GBS = 8000.0
+RECALL_FLOOR = 0.95
+# stage -> (kernel, algorithmic bytes per launch) for the roofline
+STAGE_KERNELS = {
+    "exact": {"scan": "k_scan", "select": "k_select", "rerank": "k_rerank"},
+    "mixed": {"scan": "k_scan_mfma", "select": "k_pool_select", "rerank": "k_rerank_pruned"},
+}
 
 
 def log(*a):
@@ -52,13 +64,29 @@ def build_workload(args, torch):
     return base, queries, ix, n_probe, k, np
 
 
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary profiles/<kernel>_<workload>.md (a line
+    `traffic_bytes_per_launch: N`), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1].strip().split()[0]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["c1", "c2"], default="c2")
-    ap.add_argument("--precision", choices=["exact", "mixed"], default="exact")
+    ap.add_argument("--precision", choices=["exact", "mixed"], default="mixed")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip timing the other precision")
     ap.add_argument("--rerank", type=int, default=1000)
     ap.add_argument("--n-base", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
@@ -79,70 +107,107 @@ def main() -> int:
     base, queries, ix, n_probe, k, np = build_workload(args, torch)
     nq, dim = queries.shape
     dev = DeviceIndex(ix, device=0)
-    prec = PREC_EXACT if args.precision == "exact" else PREC_MIXED
     q_d = torch.from_numpy(queries).cuda()
     ids_d = torch.empty((nq, k), dtype=torch.int64, device="cuda")
     dd_d = torch.empty((nq, k), dtype=torch.float32, device="cuda")
     cnt_d = torch.empty((nq,), dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
-
-    def step():
-        dev.search_device(q_d.data_ptr(), nq, dim, k, n_probe, ids_d.data_ptr(),
-                          dd_d.data_ptr(), cnt_d.data_ptr(), stream=stream.cuda_stream,
-                          precision=prec, rerank_size=args.rerank if prec else 0, profile=True)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    log(f"[bench] warmup done ({args.warmup} steps)")
-    lat, profs = [], []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        step()
-        torch.cuda.synchronize()
-        lat.append((time.perf_counter() - t0) * 1e3)
-        profs.append(last_profile())
-    torch.cuda.synchronize()
-    total = time.perf_counter() - t_start
-    qps = nq * args.steps / total
-    log(f"[bench] {args.steps} steps in {total:.3f}s -> {qps:,.0f} QPS")
-
-    # recall@10 vs FP64 brute force on a query sample
-    ids = ids_d.cpu().numpy().view(np.uint64)
     ns = min(args.recall_sample, nq)
     gt, _ = brute_force_f64(base, queries[:ns], 10, 1)
-    rec = float(np.mean([len(set(ids[i, :10].tolist()) & set(gt[i].tolist())) / 10.0
-                         for i in range(ns)]))
-    log(f"[bench] recall@10 {rec:.4f} on {ns} queries")
 
-    stage = {s: float(np.mean([p["stage_ms"][s] for p in profs])) for s in profs[0]["stage_ms"]}
-    scan_ms = stage["scan"]
-    scan_bytes = profs[-1]["scan_code_bytes"]
-    achieved = scan_bytes / (scan_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_scan (posting scan)", "scan_ms": round(scan_ms, 4),
-                "algorithmic_bytes_per_launch": scan_bytes,
-                "candidates_per_launch": profs[-1]["candidates"]}
+    def run(precision: str):
+        prec = PREC_EXACT if precision == "exact" else PREC_MIXED
+
+        def step():
+            dev.search_device(q_d.data_ptr(), nq, dim, k, n_probe, ids_d.data_ptr(),
+                              dd_d.data_ptr(), cnt_d.data_ptr(), stream=stream.cuda_stream,
+                              precision=prec, rerank_size=args.rerank if prec else 0,
+                              profile=True)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        log(f"[bench] {precision}: warmup done ({args.warmup} steps)")
+        lat, profs = [], []
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            step()
+            torch.cuda.synchronize()
+            lat.append((time.perf_counter() - t0) * 1e3)
+            profs.append(last_profile())
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t_start
+        qps = nq * args.steps / total
+        ids = ids_d.cpu().numpy().view(np.uint64)
+        rec = float(np.mean([len(set(ids[i, :10].tolist()) & set(gt[i].tolist())) / 10.0
+                             for i in range(ns)]))
+        log(f"[bench] {precision}: {args.steps} steps in {total:.3f}s -> {qps:,.0f} QPS, "
+            f"recall@10 {rec:.4f} on {ns} queries")
+        stage = {s: float(np.mean([p["stage_ms"][s] for p in profs]))
+                 for s in profs[0]["stage_ms"]}
+        lat_sorted = sorted(lat)
+        if precision == "mixed":
+            log(f"[bench] mixed: rerank gathered {profs[-1]['rerank_rows']:,} rows of "
+                f"{profs[-1]['pool_members']:,} pool members")
+        return {"qps": qps, "total": total, "recall": rec, "stage": stage, "prof": profs[-1],
+                "p50": lat_sorted[len(lat) // 2],
+                "p99": lat_sorted[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]}
+
+    head = run(args.precision)
+    other = None
+    if not args.no_compare:
+        other = run("exact" if args.precision == "mixed" else "mixed")
+    if head["recall"] < RECALL_FLOOR:
+        log(f"[bench] WARNING: recall@10 {head['recall']:.4f} below {RECALL_FLOOR}")
+
+    # roofline of the dominant kernel of the headline run
+    stage, prof = head["stage"], head["prof"]
+    want = max(k, args.rerank) if args.precision == "mixed" else k
+    cand = prof["candidates"]
+    if args.precision == "mixed":
+        # select: 16-bit keys read, pool (entry, key) written; rerank: pool
+        # read, exact rows actually gathered, results written
+        alg = {"scan": prof["scan_code_bytes"], "select": cand * 2 + nq * want * 6,
+               "rerank": nq * want * 6 + prof["rerank_rows"] * dim * 4 + nq * k * 8}
+    else:
+        alg = {"scan": prof["scan_code_bytes"], "select": cand * 8 + nq * want * 8,
+               "rerank": nq * want * dim * 4 if ix.quant else 0}
+    kernels = {}
+    for st in ("scan", "select", "rerank"):
+        ms = stage[st]
+        if ms <= 0 or alg[st] <= 0:
+            continue
+        ach = alg[st] / (ms * 1e-3) / 1e9
+        kernels[st] = {"kernel": STAGE_KERNELS[args.precision][st], "ms": round(ms, 4),
+                       "algorithmic_bytes": int(alg[st]), "achieved_GBs": round(ach, 1)}
+    dom = max(kernels, key=lambda s: kernels[s]["ms"])
+    dk = kernels[dom]
+    wl = f"{args.workload}_{args.precision}_{ix.n_raw // 1000}kx{dim}"
+    traffic, traffic_src = read_traffic(dk["kernel"], wl)
+    roofline = {"bound": "hbm", "achieved": dk["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(dk["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "kernel": dk["kernel"], "kernel_ms": dk["ms"],
+                "algorithmic_bytes_per_launch": dk["algorithmic_bytes"],
+                "traffic_source": traffic_src, "stages": kernels,
+                "candidates_per_launch": cand}
 
     cpu = None
     if not args.no_cpu:
         import oracle
         cores = min(16, os.cpu_count() or 1)
-        ns = min(args.cpu_sample, nq)
+        nsc = min(args.cpu_sample, nq)
         o = oracle.Oracle(ix)
         t0 = time.perf_counter()
-        o.search_batch(queries[:ns], k, n_probe, threads=cores)
+        o.search_batch(queries[:nsc], k, n_probe, threads=cores)
         dt = time.perf_counter() - t0
-        cpu = {"value": round(ns / dt, 1), "unit": "queries/s", "cores": cores, "kind": "port",
-               "sample": f"first {ns} of the {nq} queries, same index, C restatement of "
-                         f"Inner::search, {cores} threads"}
-        log(f"[bench] cpu oracle {ns / dt:,.1f} QPS on {cores} threads")
+        cpu = {"value": round(nsc / dt, 1), "unit": "queries/s", "cores": cores, "kind": "port",
+               "sample": f"first {nsc} of the {nq} queries, same index, C restatement of "
+                         f"Inner::search (exact f32), {cores} threads"}
+        log(f"[bench] cpu oracle {nsc / dt:,.1f} QPS on {cores} threads")
 
-    lat_sorted = sorted(lat)
-    p50 = lat_sorted[len(lat) // 2]
-    p99 = lat_sorted[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]
+    qps, total = head["qps"], head["total"]
     out = {
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(total * 1e3 / args.steps, 3),
@@ -153,10 +218,19 @@ def main() -> int:
                                f"n_probe {n_probe}, k {k}, {nq} queries per step",
                    "precision": args.precision,
                    "rerank_pool": args.rerank if args.precision == "mixed" else 0},
-        "recall_at_10": rec, "p50_ms": round(p50, 3), "p99_ms": round(p99, 3),
+        "recall_at_10": round(head["recall"], 4), "recall_floor_met": head["recall"] >= RECALL_FLOOR,
+        "p50_ms": round(head["p50"], 3), "p99_ms": round(head["p99"], 3),
         "stage_ms": {s: round(v, 4) for s, v in stage.items()},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if other is not None:
+        oname = "exact" if args.precision == "mixed" else "mixed"
+        out[f"{oname}_gpu"] = {"value": round(other["qps"], 1), "unit": "queries/s",
+                               "recall_at_10": round(other["recall"], 4),
+                               "p50_ms": round(other["p50"], 3), "p99_ms": round(other["p99"], 3),
+                               "stage_ms": {s: round(v, 4) for s, v in other["stage"].items()}}
+        if args.precision == "mixed":
+            out["speedup_vs_fp32_gpu"] = round(qps / other["qps"], 3)
     print(json.dumps(out), flush=True)
     return 0
 

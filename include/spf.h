@@ -83,6 +83,9 @@ typedef struct spf_index_desc {
 
 #define SPF_FLAG_PROFILE 1u     /* time each stage with HIP events on the call's stream;
                                    the call then waits for its work (spf_last_profile) */
+#define SPF_FLAG_FULL_RERANK 2u /* MIXED: rerank every pool member exactly instead of
+                                   skipping members whose proven lower bound exceeds the
+                                   k-th exact distance (same results, more work) */
 
 typedef struct spf_search_opts {
     uint32_t precision;             /* SPF_PREC_* (default EXACT) */
@@ -108,7 +111,9 @@ typedef struct spf_profile {
     uint64_t scan_code_bytes;       /* candidates * code bytes: the scan's algorithmic reads */
     uint64_t probe_pairs;           /* (query, centroid) distances of the probe */
     uint32_t scan_items;            /* posting-scan work items */
-    uint32_t reserved[3];
+    uint32_t rerank_rows;           /* MIXED: raw rows gathered by the exact rerank */
+    uint32_t pool_members;          /* MIXED: candidates in the rerank pools */
+    uint32_t reserved[1];
 } spf_profile;
 
 typedef struct spf_index_info {

@@ -22,6 +22,7 @@ EXPORTS = ("spf_index_create", "spf_index_destroy", "spf_index_get_info", "spf_d
            "spf_search_batch", "spf_search_batch_device", "spf_last_profile", "spf_selftest",
            "spf_last_error", "spf_version")
 FLAG_PROFILE = 1
+FLAG_FULL_RERANK = 2
 STAGES = ("prep", "probe", "probe_select", "plan", "scan", "select", "rerank", "finalize")
 
 
@@ -47,7 +48,8 @@ class SearchOpts(C.Structure):
 class Profile(C.Structure):
     _fields_ = [("stage_ms", C.c_float * 8), ("candidates", C.c_uint64),
                 ("scan_code_bytes", C.c_uint64), ("probe_pairs", C.c_uint64),
-                ("scan_items", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("scan_items", C.c_uint32), ("rerank_rows", C.c_uint32),
+                ("pool_members", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
 
 class IndexInfo(C.Structure):
@@ -112,7 +114,8 @@ def last_profile() -> dict:
     check(lib().spf_last_profile(C.byref(pr)))
     return {"stage_ms": dict(zip(STAGES, [float(x) for x in pr.stage_ms])),
             "candidates": int(pr.candidates), "scan_code_bytes": int(pr.scan_code_bytes),
-            "probe_pairs": int(pr.probe_pairs), "scan_items": int(pr.scan_items)}
+            "probe_pairs": int(pr.probe_pairs), "scan_items": int(pr.scan_items),
+            "rerank_rows": int(pr.rerank_rows), "pool_members": int(pr.pool_members)}
 
 
 def _p(a):
@@ -161,7 +164,7 @@ class DeviceIndex:
         check(lib().spf_delete(self.h, _p(a), a.shape[0]))
 
     def search(self, queries, k: int, n_probe: int, precision: int = PREC_EXACT,
-               rerank_size: int = 0):
+               rerank_size: int = 0, flags: int = 0):
         q = np.ascontiguousarray(queries, dtype=np.float32)
         if q.ndim == 1:
             q = q[None, :]
@@ -170,7 +173,7 @@ class DeviceIndex:
         dd = np.empty((nq, max(k, 1)), dtype=np.float32)
         cnt = np.zeros(nq, dtype=np.uint32)
         o = SearchOpts()
-        o.precision, o.rerank_size = precision, rerank_size
+        o.precision, o.rerank_size, o.flags = precision, rerank_size, flags
         check(lib().spf_search_batch(self.h, _p(q), nq, dim, k, n_probe, C.byref(o), _p(ids),
                                      _p(dd), _p(cnt)))
         return ids, dd, cnt

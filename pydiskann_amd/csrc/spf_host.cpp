@@ -80,7 +80,7 @@ struct Workspace {
     DBuf<uint64_t> out_ids;
     DBuf<float> out_d;
     DBuf<uint32_t> out_c;
-    DBuf<uint16_t> q16, cand16, qbf, probe16;
+    DBuf<uint16_t> q16, cand16, qbf, probe16, pool_key;
     DBuf<float> qn16, qnbf;
     DBuf<uint32_t> probe_pool, probe_pool_cnt;
     DBuf<uint32_t> pool_entry, pool_want, cand_cnt, perm_cnt, perm_off, perm_fill, qperm;
@@ -88,6 +88,8 @@ struct Workspace {
     DBuf<uint64_t> rq_planes;
     DBuf<float> rq_consts;
     DBuf<uint32_t> pair_src;
+    DBuf<float> eq16;
+    DBuf<uint32_t> rr_stats;
     uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
     hipEvent_t ev[SPF_N_STAGES + 1] = {};
 
@@ -96,6 +98,7 @@ struct Workspace {
         cand16.release();
         qbf.release();
         probe16.release();
+        pool_key.release();
         qnbf.release();
         probe_pool.release();
         probe_pool_cnt.release();
@@ -110,6 +113,8 @@ struct Workspace {
         rq_planes.release();
         rq_consts.release();
         pair_src.release();
+        eq16.release();
+        rr_stats.release();
         qn16.release();
         DBuf<float> *fs[] = {&qin, &qv_probe, &qv_scan, &qnorm, &out_d};
         for (auto *b : fs) b->release();
@@ -157,6 +162,8 @@ struct spf_index {
     bool codes16_owned = false;
     float *d_xn16 = nullptr;
     uint32_t units16 = 0;
+    // per posting: max |x - f16(x)| over raw rows, max |f16(x)|^2 (rerank bound)
+    float *d_post_ex = nullptr, *d_post_xn = nullptr;
     // matrix-core probe: bf16 centroids centred on their mean, f32 rows for the recheck
     float *d_mu = nullptr, *d_cnbf = nullptr, *d_cent_rows = nullptr;
     uint4 *d_cbf = nullptr;
@@ -179,7 +186,7 @@ struct spf_index {
     ~spf_index() {
         void *ps[] = {d_cent, d_cnorm, d_post_off, d_post_len, d_codes, d_rank,
                       d_rank_id, d_raw, d_del, d_xnorm, d_xsq, d_rank_entry, d_raw_e, d_xn16,
-                      d_mu, d_cnbf, d_cent_rows, d_cbf, d_rq_signs};
+                      d_mu, d_cnbf, d_cent_rows, d_cbf, d_rq_signs, d_post_ex, d_post_xn};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         if (codes16_owned && d_codes16) (void)hipFree(d_codes16);
@@ -497,6 +504,31 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         ix->codes16_owned = true;
     }
     if (ix->d_codes16) {
+        // bound inputs of the pruned rerank, in double and rounded up
+        std::vector<float> pex(std::max<uint32_t>(P, 1), 0.0f), pxn(std::max<uint32_t>(P, 1), 0.0f);
+        for (uint32_t i = 0; i < P; i++) {
+            double ex = 0.0, xn = 0.0;
+            for (uint32_t e = post_off[i]; e < post_off[i + 1]; e++) {
+                const float *raw = d->quantizer == SPF_QUANT_NONE
+                    ? (const float *)(d->codes + (size_t)entry_src[e] * d->code_bytes)
+                    : d->raw + (size_t)d->entry_ids[entry_src[e]] * d->dim;
+                const _Float16 *c16 = d->quantizer == SPF_QUANT_F16
+                    ? (const _Float16 *)(d->codes + (size_t)entry_src[e] * d->code_bytes) : nullptr;
+                double s2 = 0.0, n2 = 0.0;
+                for (uint32_t j = 0; j < d->dim; j++) {
+                    const double h = c16 ? (double)(float)c16[j] : (double)(float)(_Float16)raw[j];
+                    const double t = (double)raw[j] - h;
+                    s2 += t * t;
+                    n2 += h * h;
+                }
+                ex = std::max(ex, std::sqrt(s2));
+                xn = std::max(xn, n2);
+            }
+            pex[i] = std::nextafter((float)(ex * (1.0 + 1e-12)), INFINITY);
+            pxn[i] = std::nextafter((float)(xn * (1.0 + 1e-12)), INFINITY);
+        }
+        HIPCHK(upload(&ix->d_post_ex, pex.data(), pex.size(), bytes));
+        HIPCHK(upload(&ix->d_post_xn, pxn.data(), pxn.size(), bytes));
         HIPCHK(hipMalloc((void **)&ix->d_xn16, std::max<uint64_t>(N, 1) * 4));
         bytes += 4 * N;
         HIPCHK(launch_entry_norms(CK_F16, ix->d_codes16, ix->units16, (uint32_t)N, d->dim, 1.0f,
@@ -539,15 +571,18 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
     return SPF_OK;
 }
 
-// Matrix-core scan -> 16-bit pool keys -> pool marks -> posting-major exact
-// rerank -> per-query sort.  Runs after probe and planning.
+// Matrix-core (or RaBitQ estimator) scan -> 16-bit candidate keys -> per
+// query pool select -> bound-pruned exact rerank -> ids.  Runs after probe
+// and planning.
 template <class Mark>
 int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_t k,
                  uint32_t npr, uint32_t want, uint64_t total, uint32_t n_items, uint64_t *d_ids,
-                 float *d_dists, uint32_t *d_counts, Mark &mark, RqParams *rq) {
+                 float *d_dists, uint32_t *d_counts, Mark &mark, RqParams *rq, bool full_rerank,
+                 bool stats) {
     const uint32_t dpad32 = round_up(ix->dim, 32);
     HIPCHK(w->cand16.ensure(std::max<uint64_t>(total, 1)));
     HIPCHK(w->pool_entry.ensure((size_t)nq * want));
+    HIPCHK(w->pool_key.ensure((size_t)nq * want));
     HIPCHK(w->pool_want.ensure(nq));
     HIPCHK(w->fin.ensure((size_t)nq * k));
     HIPCHK(w->fin_cnt.ensure(nq));
@@ -572,6 +607,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     mp.units = ix->units16;
     mp.xn16 = ix->d_xn16;
     mp.del = ix->d_del;
+    mp.metric = ix->metric;
     mp.out16 = w->cand16.p;
     const uint32_t grid = xcd_grid(n_items, 16384);
     HIPCHK(mark(SPF_STAGE_SCAN));
@@ -582,6 +618,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     } else {
         HIPCHK(launch_scan_mfma(mp, grid, s));
     }
+    // per query: the `want` smallest keys (pool) with their keys
     HIPCHK(mark(SPF_STAGE_SELECT));
     PoolParams pp{};
     pp.keys = w->cand16.p;
@@ -595,27 +632,47 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     pp.probe_post = w->probe_post.p;
     pp.post_off = ix->d_post_off;
     pp.pool_entry = w->pool_entry.p;
+    pp.pool_key = w->pool_key.p;
     pp.pool_want = w->pool_want.p;
     HIPCHK(launch_pool_select(pp, nq, s));
+    // nearest-posting query order, then the (pruned) exact rerank of each pool
     HIPCHK(mark(SPF_STAGE_RERANK));
     HIPCHK(launch_query_perm(w->np_q.p, w->probe_post.p, npr, nq, ix->P, w->perm_cnt.p,
                              w->perm_off.p, w->perm_fill.p, w->qperm.p, s));
-    RerankPoolParams rp{};
+    RerankPrunedParams rp{};
     rp.pool_entry = w->pool_entry.p;
+    rp.pool_key = w->pool_key.p;
     rp.pool_want = w->pool_want.p;
-    rp.qperm = w->qperm.p;
     rp.nq = nq;
     rp.want = want;
     rp.k = k;
+    rp.ca = std::min(want, std::max(k, round_up(k + k / 4 + 32, 64)));
     rp.metric = ix->metric;
     rp.dpad = ix->dpad;
+    rp.dim = ix->dim;
+    rp.qperm = w->qperm.p;
+    rp.np_q = w->np_q.p;
+    rp.probe_post = w->probe_post.p;
+    rp.npr = npr;
     rp.qv = w->qv_probe.p;
     rp.qnorm = w->qnorm.p;
     rp.raw_e = ix->d_raw_e;
     rp.rank = ix->d_rank;
+    rp.prune = !rq && !full_rerank && ix->d_post_ex &&
+               (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
+    rp.kdim = dpad32;
+    rp.qn16 = w->qn16.p;
+    rp.eq16 = w->eq16.p;
+    rp.post_ex = ix->d_post_ex;
+    rp.post_xn = ix->d_post_xn;
     rp.out = w->fin.p;
     rp.out_cnt = w->fin_cnt.p;
-    HIPCHK(launch_rerank_pool(rp, s));
+    if (stats) {
+        HIPCHK(w->rr_stats.ensure(2));
+        HIPCHK(hipMemsetAsync(w->rr_stats.p, 0, 2 * sizeof(uint32_t), s));
+        rp.stats = w->rr_stats.p;
+    }
+    HIPCHK(launch_rerank_pruned(rp, s));
     HIPCHK(mark(SPF_STAGE_FINALIZE));
     HIPCHK(launch_finalize(w->fin.p, k, w->fin_cnt.p, nq, k, ix->d_rank_id, d_ids, d_dists,
                            d_counts, s));
@@ -706,9 +763,11 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     if (mixed) {
         HIPCHK(w->q16.ensure((size_t)nq * dpad32));
         HIPCHK(w->qn16.ensure(nq));
+        HIPCHK(w->eq16.ensure(nq));
         pp.q16 = w->q16.p;
         pp.dpad32 = dpad32;
         pp.qn16 = w->qn16.p;
+        pp.eq16 = w->eq16.p;
     }
     if (mprobe) {
         HIPCHK(w->qbf.ensure((size_t)nq * dpad32));
@@ -894,7 +953,8 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     }
     if (mixed) {
         int rc = search_mixed(ix, w, s, nq, k, npr, want, total, n_items, d_ids, d_dists,
-                              d_counts, mark, ix->rq_l2 ? &rq : nullptr);
+                              d_counts, mark, ix->rq_l2 ? &rq : nullptr,
+                              opts->flags & SPF_FLAG_FULL_RERANK, prof);
         if (rc) return rc;
     } else if (ix->rq_l2) {
         // search_rabitq: per-pair query prep, then one wave per query
@@ -985,6 +1045,12 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                      : mixed ? 2ull * ix->dim : (uint64_t)ix->code_bytes);
         g_prof.probe_pairs = (uint64_t)nq * P;
         g_prof.scan_items = n_items;
+        if (mixed) {
+            uint32_t st[2] = {0, 0};
+            HIPCHK(hipMemcpy(st, w->rr_stats.p, sizeof st, hipMemcpyDeviceToHost));
+            g_prof.rerank_rows = st[0];
+            g_prof.pool_members = st[1];
+        }
         g_prof_valid = true;
     }
     return SPF_OK;

@@ -86,6 +86,7 @@ struct PrepParams {
     uint16_t *q16;              // [nq][dpad32] f16 query for the matrix-core scan (null = skip)
     uint32_t dpad32;            // dim rounded up to 32
     float *qn16;                // [nq] |f16(q)|^2
+    float *eq16;                // [nq] |q - f16(q)| (null = skip)
     const float *mu;            // [dim] centring vector of the bf16 probe (null = skip)
     uint16_t *qbf;              // [nq][dpad32] bf16(q - mu)
     float *qnbf;                // [nq] |bf16(q - mu)|^2
@@ -120,7 +121,34 @@ struct MixedParams {
     uint32_t units;             // 16-byte units per f16 code
     const float *xn16;          // |x16|^2 per entry
     const uint32_t *del;
+    uint32_t metric;            // key: L2 |q16-x16|^2, DOT 1-q16.x16, COSINE 1-cos(q16,x16)
     uint16_t *out16;            // 16-bit candidate keys
+};
+
+// Exact rerank of each query's pool (entries + 16-bit keys from pool
+// selection).  With prune, only the `ca` smallest keys and the members whose
+// key does not prove an exact distance above the k-th of those are reranked
+// (L2 metrics over f16 keys); the result equals reranking the whole pool.
+struct RerankPrunedParams {
+    const uint32_t *pool_entry; // [nq][want]
+    const uint16_t *pool_key;   // [nq][want]
+    const uint32_t *pool_want;  // [nq]
+    uint32_t nq, want, k, ca, metric, dpad, dim;
+    const uint32_t *qperm;      // processing order (null = identity)
+    const uint32_t *np_q;       // [nq]
+    const uint32_t *probe_post; // [nq][npr]
+    uint32_t npr;
+    const float *qv, *qnorm;    // [nq][dpad], [nq]
+    const float *raw_e;         // [n_entries][dpad]
+    const uint32_t *rank;       // entry -> rank
+    uint32_t prune;
+    uint32_t kdim;              // terms in each f16 dot product (dim rounded to 32)
+    const float *qn16, *eq16;   // per query |q16|^2, |q - q16|
+    const float *post_ex;       // per posting max |x - x16|
+    const float *post_xn;       // per posting max |x16|^2
+    uint64_t *out;              // [nq][k] (dist key << 32 | rank)
+    uint32_t *out_cnt;
+    uint32_t *stats;            // optional [2]: rows gathered, pool members
 };
 
 // RaBitQ: per-pair query prep, exact search, mixed estimator scan.
@@ -174,6 +202,7 @@ struct PoolParams {
     const uint32_t *probe_post; // [nq][npr]
     const uint32_t *post_off;   // [n_post + 1]
     uint32_t *pool_entry;       // [nq][want]
+    uint16_t *pool_key;         // [nq][want] keys of the members (null = not written)
     uint32_t *pool_want;        // [nq]
 };
 
@@ -256,6 +285,7 @@ hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s);
 hipError_t launch_rerank_pool(const RerankPoolParams &p, hipStream_t s);
+hipError_t launch_rerank_pruned(const RerankPrunedParams &p, hipStream_t s);
 hipError_t launch_probe_mfma(const ProbeMfmaParams &p, hipStream_t s);
 hipError_t launch_rq_prep(const RqParams &p, uint32_t n_pairs, hipStream_t s);
 hipError_t launch_rq_search_exact(const RqParams &p, hipStream_t s);

@@ -110,12 +110,15 @@ __global__ __launch_bounds__(64) void k_prep(const PrepParams p) {
         for (uint32_t i = 0; i < p.dim; i++) c[(i >> 2) & 3] = __builtin_fmaf(s_v[i], s_v[i], c[(i >> 2) & 3]);
         p.qnorm[q] = (c[0] + c[1]) + (c[2] + c[3]);
         if (p.q16) {
-            float a16 = 0.0f;
+            float a16 = 0.0f, e16 = 0.0f;
             for (uint32_t i = 0; i < p.dim; i++) {
                 const float h = (float)(_Float16)s_v[i];
                 a16 = __builtin_fmaf(h, h, a16);
+                const float t = s_v[i] - h;   // exact: h is s_v[i] rounded to 11 bits
+                e16 = __builtin_fmaf(t, t, e16);
             }
             p.qn16[q] = a16;
+            if (p.eq16) p.eq16[q] = sqrtf(e16);
         }
         if (p.qbf) {
             float ab = 0.0f;
@@ -368,7 +371,6 @@ __device__ void bitonic_sort_lds(uint64_t *buf, uint32_t m) {
         }
 }
 
-constexpr uint32_t kSelDirect = 1024;   // segments this short are sorted outright
 constexpr uint32_t kSelSlack = 256;     // extra keys tolerated in the final sort
 
 // Top-K of one segment.  R > 0: the segment (at most R * 256 keys) is held in

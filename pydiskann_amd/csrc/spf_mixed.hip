@@ -29,6 +29,10 @@ __device__ __forceinline__ uint32_t fkey_m(float f) {
     uint32_t u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
+__device__ __forceinline__ float key_float_m(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+}
 
 __device__ __forceinline__ half8_t as_half8(uint4 v) {
     union { uint4 u; half8_t h; } c;
@@ -105,7 +109,10 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
             const uint32_t q = ok ? p.pair_q[pr0 + tid] : 0u;
             s_qi[tid] = q;
             s_slot[tid] = ok ? p.pair_slot[pr0 + tid] : 0ull;
-            s_qn[tid] = ok ? p.qn16[q] : 0.0f;
+            float qn = ok ? p.qn16[q] : 0.0f;
+            if (p.metric == M_DOT) qn = 1.0f;
+            else if (p.metric == M_COS) qn = qn > 0.0f ? 1.0f / sqrtf(qn) : 0.0f;
+            s_qn[tid] = qn;
         }
         __syncthreads();
         // A fragments: queries (rows) x dims (K), two 16-row tiles
@@ -153,15 +160,20 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
                 const float xn = p.xn16[e];
                 const bool dead = p.del && ((p.del[e >> 5] >> (e & 31)) & 1u);
                 const uint32_t off = e - pb;
+                const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
+                // cosine: scale by the inverse norms (s_qn holds them)
+                const float rx = p.metric == M_COS ? (xn > 0.0f ? 1.0f / sqrtf(xn) : 0.0f) : 1.0f;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const uint32_t g0 = 4 * kg + j, g1 = 16 + 4 * kg + j;
                     if (g0 < gcnt) {
-                        const float d = xn + s_qn[g0] - 2.0f * acc0[j];
+                        const float d = l2 ? xn + s_qn[g0] - 2.0f * acc0[j]
+                                           : 1.0f - acc0[j] * s_qn[g0] * rx;
                         p.out16[s_slot[g0] + off] = dead ? (uint16_t)0xffff : (uint16_t)(fkey_m(d) >> 16);
                     }
                     if (g1 < gcnt) {
-                        const float d = xn + s_qn[g1] - 2.0f * acc1[j];
+                        const float d = l2 ? xn + s_qn[g1] - 2.0f * acc1[j]
+                                           : 1.0f - acc1[j] * s_qn[g1] * rx;
                         p.out16[s_slot[g1] + off] = dead ? (uint16_t)0xffff : (uint16_t)(fkey_m(d) >> 16);
                     }
                 }
@@ -232,7 +244,6 @@ __global__ __launch_bounds__(256) void k_probe_mfma(const ProbeMfmaParams p) {
 // ---------------------------------------------------------------- pool select
 // Per query: mark the `want` smallest 16-bit keys (0xffff = no candidate).
 // Exactly-tied keys at the boundary are taken in slot order.
-constexpr uint32_t kPoolLds = 24576;   // keys cached in LDS (48 KiB)
 
 __device__ __forceinline__ uint32_t excl_scan256(uint32_t v, uint32_t *part) {
     const uint32_t tid = threadIdx.x;
@@ -384,43 +395,76 @@ __device__ __forceinline__ void wave_find_bin(const uint32_t *hist, uint32_t rem
     below = __shfl(bl, src, 64);
 }
 
-// MODE 1: keys in registers (<= 256 words); 2: keys staged in LDS (`lds`,
-// the caller's copy of the segment); 0: keys re-read from global memory.
-template <int MODE>
+// bit masks of one 32-key word: live keys, keys below T, keys equal to T
+__device__ __forceinline__ void word_masks(const uint4 r[4], uint32_t w, uint32_t n, uint32_t T,
+                                           uint32_t &live, uint32_t &lt, uint32_t &eq) {
+    uint32_t lv = 0, l = 0, e = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const uint32_t j = 8 * c + t;
+            const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+            const uint32_t bit = (uint32_t)(kk != 0xffffu && 32 * w + j < n) << j;
+            lv |= bit;
+            l |= kk < T ? bit : 0u;
+            e |= kk == T ? bit : 0u;
+        }
+    }
+    live = lv;
+    lt = l;
+    eq = e;
+}
+
+// key j (runtime) of a 32-key word
+__device__ __forceinline__ uint32_t word_key(const uint4 r[4], uint32_t j) {
+    const uint32_t c = j >> 3, t = (j >> 1) & 3;
+    const uint4 v = c == 0 ? r[0] : c == 1 ? r[1] : c == 2 ? r[2] : r[3];
+    const uint32_t d = t == 0 ? v.x : t == 1 ? v.y : t == 2 ? v.z : v.w;
+    return (d >> (16 * (j & 1))) & 0xffffu;
+}
+
+// the lowest `c` set bits of m
+__device__ __forceinline__ uint32_t lowest_bits(uint32_t m, uint32_t c) {
+    uint32_t out = 0;
+    while (c && m) {
+        const uint32_t lo = m & (~m + 1u);
+        out |= lo;
+        m ^= lo;
+        c--;
+    }
+    return out;
+}
+
+// One wavefront per query: the `want` smallest 16-bit keys of its segment,
+// exactly-tied keys at the threshold taken in slot order, written as entries
+// (map) or slots, with their keys (pool_key, optional).  Lane l owns words
+// l, l+64, ... (word w = slots 32w..32w+31) and re-reads them from L2 in each
+// of the four passes; a word's four loads go out together.
 __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32_t n,
                                           uint32_t want, uint32_t *pool_entry,
-                                          uint32_t *pool_want, bool map, const ProbeTab &tab,
-                                          uint32_t *hist, const uint16_t *lds = nullptr) {
-    constexpr bool REG = MODE == 1;
+                                          uint16_t *pool_key, uint32_t *pool_want, bool map,
+                                          const ProbeTab &tab, uint32_t *hist) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nwords = (n + 31) / 32;
-    const uint32_t M = REG ? 4u : (nwords + 63) / 64;
-    uint4 kr[REG ? 4 : 1][4];
-    uint16_t k[32];
-    auto fetch = [&](uint32_t m, uint32_t w) {
-        if (REG) {
-            unpack_word(kr[REG ? m : 0], w, n, k);
-        } else {
-            uint4 r[4];
-            load_word_raw(MODE == 2 ? lds : kp, w, r);
-            unpack_word(r, w, n, k);
-        }
-    };
-    uint32_t inval = 0, kmn = 0xffff, kmx = 0;
+    uint32_t inval = 0, kmn = 0xffffu, kmx = 0u;
+    for (uint32_t w = lane; w < nwords; w += 64) {
+        uint4 r[4];
+        load_word_raw(kp, w, r);
 #pragma unroll
-    for (uint32_t m = 0; m < M; m++) {
-        const uint32_t w = lane + 64 * m;
-        if (w >= nwords) continue;
-        if (REG) load_word_raw(kp, w, kr[REG ? m : 0]);
-        fetch(m, w);
+        for (int c = 0; c < 4; c++) {
+            const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
 #pragma unroll
-        for (int j = 0; j < 32; j++) {
-            if (32 * w + j >= n) continue;
-            if (k[j] == 0xffff) {
-                inval++;
-            } else {
-                kmn = min(kmn, (uint32_t)k[j]);
-                kmx = max(kmx, (uint32_t)k[j]);
+            for (int t = 0; t < 8; t++) {
+                const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+                if (32 * w + 8 * c + t >= n) continue;
+                if (kk == 0xffffu) {
+                    inval++;
+                } else {
+                    kmn = min(kmn, kk);
+                    kmx = max(kmx, kk);
+                }
             }
         }
     }
@@ -433,7 +477,7 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
     const uint32_t W = min(want, nvalid);
     if (lane == 0) pool_want[q] = W;
     const bool all = W == nvalid;
-    uint32_t T = 0, need = 0;
+    uint32_t T = 0xffffu, need = 0;
     if (!all) {
         const uint32_t span = kmx - kmn;
         const uint32_t sh = span < 256 ? 0 : 32 - __clz(span) - 8;
@@ -441,14 +485,18 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
         for (int i = 0; i < 4; i++) hist[4 * lane + i] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        for (uint32_t w = lane; w < nwords; w += 64) {
+            uint4 r[4];
+            load_word_raw(kp, w, r);
 #pragma unroll
-        for (uint32_t m = 0; m < M; m++) {
-            const uint32_t w = lane + 64 * m;
-            if (w >= nwords) continue;
-            fetch(m, w);
+            for (int c = 0; c < 4; c++) {
+                const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
 #pragma unroll
-            for (int j = 0; j < 32; j++)
-                if (k[j] != 0xffff) atomicAdd(&hist[(k[j] - kmn) >> sh], 1u);
+                for (int t = 0; t < 8; t++) {
+                    const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+                    if (kk != 0xffffu && 32 * w + 8 * c + t < n) atomicAdd(&hist[(kk - kmn) >> sh], 1u);
+                }
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -460,15 +508,20 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
             for (int i = 0; i < 4; i++) hist[4 * lane + i] = 0;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
+            const uint32_t mask = (1u << sh) - 1u;
+            for (uint32_t w = lane; w < nwords; w += 64) {
+                uint4 r[4];
+                load_word_raw(kp, w, r);
 #pragma unroll
-            for (uint32_t m = 0; m < M; m++) {
-                const uint32_t w = lane + 64 * m;
-                if (w >= nwords) continue;
-                fetch(m, w);
+                for (int c = 0; c < 4; c++) {
+                    const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
 #pragma unroll
-                for (int j = 0; j < 32; j++)
-                    if (k[j] != 0xffff && ((k[j] - kmn) >> sh) == b1)
-                        atomicAdd(&hist[(k[j] - kmn) & ((1u << sh) - 1u)], 1u);
+                    for (int t = 0; t < 8; t++) {
+                        const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+                        if (kk != 0xffffu && 32 * w + 8 * c + t < n && ((kk - kmn) >> sh) == b1)
+                            atomicAdd(&hist[(kk - kmn) & mask], 1u);
+                    }
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -477,51 +530,32 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
         T = kmn + (b1 << sh) + b2;
         need = W - below1 - below2;   // keys equal to T to take, in slot order
     }
-    // collect: chunk m = words 64m .. 64m+63 in lane order (slot order)
+    // collect: round m = words 64m .. 64m+63 in lane order (slot order)
     uint32_t tie_run = 0, mark_run = 0;
-#pragma unroll
-    for (uint32_t m = 0; m < M; m++) {
-        if (64 * m >= nwords) break;
+    for (uint32_t m = 0; 64 * m < nwords; m++) {
         const uint32_t w = lane + 64 * m;
-        const bool okw = w < nwords;
-        uint32_t ties = 0;
-        if (okw) {
-            fetch(m, w);
-            if (!all)
-#pragma unroll
-                for (int j = 0; j < 32; j++) ties += k[j] == T;
+        uint4 r[4];
+        uint32_t live = 0, lt = 0, eq = 0;
+        if (w < nwords) {
+            load_word_raw(kp, w, r);
+            word_masks(r, w, n, T, live, lt, eq);
         }
         uint32_t tt;
-        uint32_t tr = tie_run + wave_excl(ties, &tt);
+        const uint32_t tr = tie_run + wave_excl(all ? 0u : __popc(eq), &tt);
         tie_run += tt;
-        uint32_t word = 0;
-        if (okw) {
-#pragma unroll
-            for (int j = 0; j < 32; j++) {
-                bool sel;
-                if (all) {
-                    sel = k[j] != 0xffff;
-                } else if (k[j] < T) {
-                    sel = true;
-                } else if (k[j] == T) {
-                    sel = tr < need;
-                    tr++;
-                } else {
-                    sel = false;
-                }
-                word |= (uint32_t)sel << j;
-            }
-        }
+        const uint32_t sel = all ? live : lt | (tr < need ? lowest_bits(eq, need - tr) : 0u);
         uint32_t mt;
-        uint32_t pre = mark_run + wave_excl(__popc(word), &mt);
+        uint32_t pre = mark_run + wave_excl(__popc(sel), &mt);
         mark_run += mt;
-        if (word) {   // pool positions follow slot order
+        if (sel) {   // pool positions follow slot order
+            uint32_t word = sel;
             uint32_t pj = map ? tab.find(32 * w + (__ffs(word) - 1)) : 0u;
             while (word) {
                 const uint32_t j = __ffs(word) - 1;
                 word &= word - 1;
-                pool_entry[(size_t)q * want + pre++] = map ? tab.entry_from(pj, 32 * w + j)
-                                                           : 32 * w + j;
+                const size_t o = (size_t)q * want + pre++;
+                pool_entry[o] = map ? tab.entry_from(pj, 32 * w + j) : 32 * w + j;
+                if (pool_key) pool_key[o] = (uint16_t)word_key(r, j);
             }
         }
     }
@@ -554,10 +588,8 @@ __global__ __launch_bounds__(256) void k_pool_select(const PoolParams p, uint32_
         tab.pbase = s_pbase[wv];
         tab.ebase = s_ebase[wv];
     }
-    if ((n + 31) / 32 <= 256)
-        pool_wave<1>(p.keys + b, q, n, p.want, p.pool_entry, p.pool_want, p.map, tab, hist[wv]);
-    else
-        pool_wave<0>(p.keys + b, q, n, p.want, p.pool_entry, p.pool_want, p.map, tab, hist[wv]);
+    pool_wave(p.keys + b, q, n, p.want, p.pool_entry, p.pool_key, p.pool_want, p.map, tab,
+              hist[wv]);
 }
 
 // ---------------------------------------------------------------- query order
@@ -588,7 +620,8 @@ __device__ __forceinline__ float quad_sum(float c) {
     return c01 + __shfl_xor(c01, 2, 64);
 }
 
-template <int H>
+// H rows per quad; UC units of each row are loaded together per lane
+template <int H, int UC>
 __device__ __forceinline__ void quad_exact_h(uint32_t metric, const float4 *q4,
                                              const float4 *const x4[H], uint32_t units, uint32_t r,
                                              float qn, float out[H]) {
@@ -596,14 +629,23 @@ __device__ __forceinline__ void quad_exact_h(uint32_t metric, const float4 *q4,
 #pragma unroll
     for (int h = 0; h < H; h++) acc[h] = b2[h] = 0.0f;
     const bool l2 = metric == M_L2 || metric == M_L2SQ;
-    for (uint32_t u = r; u < units; u += 4) {
-        const float4 a = q4[u];
-        float4 x[H];
+    // UC units of each row are loaded together, then consumed in chain
+    // order (units is a multiple of 4, so the bound is uniform over the quad)
+    for (uint32_t u0 = r; u0 < units; u0 += 4 * UC) {
+        float4 xc[UC][H];
 #pragma unroll
-        for (int h = 0; h < H; h++) x[h] = x4[h][u];
+        for (int c = 0; c < UC; c++)
+#pragma unroll
+            for (int h = 0; h < H; h++)
+                xc[c][h] = u0 + 4 * c < units ? x4[h][u0 + 4 * c] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int c = 0; c < UC; c++) {
+        const uint32_t u = u0 + 4 * c;
+        if (u >= units) break;
+        const float4 a = q4[u];
 #pragma unroll
         for (int h = 0; h < H; h++) {
-            const float4 b = x[h];
+            const float4 b = xc[c][h];
             if (l2) {
                 float t = a.x - b.x; acc[h] = __builtin_fmaf(t, t, acc[h]);
                 t = a.y - b.y; acc[h] = __builtin_fmaf(t, t, acc[h]);
@@ -619,6 +661,7 @@ __device__ __forceinline__ void quad_exact_h(uint32_t metric, const float4 *q4,
                 b2[h] = __builtin_fmaf(b.z, b.z, b2[h]);
                 b2[h] = __builtin_fmaf(b.w, b.w, b2[h]);
             }
+        }
         }
     }
 #pragma unroll
@@ -644,7 +687,7 @@ __device__ __forceinline__ void quad_exact_h(uint32_t metric, const float4 *q4,
     }
 }
 
-__device__ void bitonic_lds(uint64_t *buf, uint32_t m) {
+__device__ __forceinline__ void bitonic_lds(uint64_t *buf, uint32_t m) {
     for (uint32_t k = 2; k <= m; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             for (uint32_t i = threadIdx.x; i < (m >> 1); i += 256) {
@@ -688,7 +731,7 @@ __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
             xr[h] = reinterpret_cast<const float4 *>(p.raw_e + (size_t)ee[h] * p.dpad);
         }
         float d[kRerankH];
-        quad_exact_h<kRerankH>(p.metric, reinterpret_cast<const float4 *>(s_qv), xr, units, r, qn, d);
+        quad_exact_h<kRerankH, 1>(p.metric, reinterpret_cast<const float4 *>(s_qv), xr, units, r, qn, d);
         if (r == 0) {
 #pragma unroll
             for (int h = 0; h < kRerankH; h++)
@@ -707,6 +750,282 @@ __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
     const uint32_t kk = min(p.k, c);
     for (uint32_t i = tid; i < kk; i += 256) p.out[(size_t)q * p.k + i] = s_keys[i];
     if (tid == 0) p.out_cnt[q] = kk;
+}
+
+// ---------------------------------------------------------------- pruned rerank
+// One workgroup per query (nearest-posting order, XCD-contiguous) over its
+// pool of `c` (entry, 16-bit key) members, in pool order:
+//   1. A = the `ca` smallest keys (ties in pool order), by two radix passes
+//   2. A is reranked exactly; its k-th exact distance bounds the answer
+//   3. members outside A whose key proves an exact distance above that bound
+//      are dropped (prune), the rest reranked
+//   4. the k smallest exact keys of A and the survivors are written.
+// Dropped members cannot enter the top k, so the result is that of reranking
+// the whole pool.  Without prune every member is reranked.
+constexpr uint32_t kPrCount = 512;    // lists up to this size are ranked by counting
+constexpr int kPrH = 4, kPrUC = 2;    // rows per quad, row units per lane in flight
+
+__device__ __forceinline__ uint32_t blk_excl(uint32_t v, uint32_t *s_w) {
+    uint32_t tot;
+    return block_scan_fast(v, s_w, &tot);
+}
+
+// bin b holding the R-th smallest (R >= 1) and the count below it
+__device__ __forceinline__ void find_rank_bin(uint32_t h, uint32_t excl, uint32_t R, uint32_t *out) {
+    if (R && excl < R && R <= excl + h) {
+        out[0] = threadIdx.x;
+        out[1] = excl;
+    }
+}
+
+// exact keys of `cnt` entries listed in LDS -> keys_out[0..cnt)
+__device__ __forceinline__ void rerank_list(const RerankPrunedParams &p, const uint32_t *ents,
+                                            uint32_t cnt, const float *s_qv, float qn,
+                                            uint64_t *keys_out) {
+    const uint32_t tid = threadIdx.x, r = tid & 3, quad = tid >> 2, units = p.dpad / 4;
+    for (uint32_t i0 = 0; i0 < cnt; i0 += 64 * kPrH) {
+        uint32_t ii[kPrH], ee[kPrH];
+        const float4 *xr[kPrH];
+#pragma unroll
+        for (int h = 0; h < kPrH; h++) {
+            ii[h] = i0 + 64 * h + quad;
+            ee[h] = ii[h] < cnt ? ents[ii[h]] : ents[0];
+            xr[h] = reinterpret_cast<const float4 *>(p.raw_e + (size_t)ee[h] * p.dpad);
+        }
+        float d[kPrH];
+        quad_exact_h<kPrH, kPrUC>(p.metric, reinterpret_cast<const float4 *>(s_qv), xr, units, r, qn, d);
+        if (r == 0) {
+#pragma unroll
+            for (int h = 0; h < kPrH; h++)
+                if (ii[h] < cnt) keys_out[ii[h]] = ((uint64_t)fkey_m(d[h]) << 32) | p.rank[ee[h]];
+        }
+    }
+}
+
+// the k smallest of keys[0..cnt) (distinct) in ascending order -> out[0..min(k, cnt))
+__device__ __forceinline__ void smallest_k(uint64_t *keys, uint32_t cnt, uint32_t k, uint64_t *out) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t kk = min(k, cnt);
+    if (cnt <= kPrCount) {   // rank by counting, scatter
+        for (uint32_t i = tid; i < cnt; i += 256) {
+            const uint64_t v = keys[i];
+            uint32_t c = 0;
+            for (uint32_t j = 0; j < cnt; j++) c += keys[j] < v;
+            if (c < kk) out[c] = v;
+        }
+    } else {
+        uint32_t m = 2;
+        while (m < cnt) m <<= 1;
+        for (uint32_t i = cnt + tid; i < m; i += 256) keys[i] = kNoKey;
+        __syncthreads();
+        bitonic_lds(keys, m);
+        for (uint32_t i = tid; i < kk; i += 256) out[i] = keys[i];
+    }
+}
+
+template <uint32_t CAP>
+struct RerankLds {
+    union {
+        uint64_t ex[CAP];        // exact keys (A, then survivors)
+        uint32_t hist[2][256];   // radix histograms (step 1)
+    };
+    uint32_t ent[CAP];           // A at [0, nA), others from nA; later survivors
+    uint16_t bk[CAP];            // keys of the others
+    uint32_t w[8];
+    uint32_t m[16];
+    uint64_t kth;
+};
+
+template <uint32_t CAP>
+__global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams p) {
+    __shared__ RerankLds<CAP> L;
+    extern __shared__ __align__(16) float s_qv[];   // dpad floats
+    enum { MN, MX, EXQ, XNQ, BA, BEA, CB2, THR };
+    const uint32_t x = blockIdx.x % kXcds, jx = blockIdx.x / kXcds;
+    const uint32_t per = (p.nq + kXcds - 1) / kXcds;
+    const uint32_t idx = x * per + jx;
+    if (jx >= per || idx >= p.nq) return;   // whole workgroup
+    const uint32_t q = p.qperm ? p.qperm[idx] : idx;
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t c = min(p.pool_want[q], p.want);
+    const uint32_t *pe = p.pool_entry + (size_t)q * p.want;
+    const uint16_t *pk = p.pool_key + (size_t)q * p.want;
+    uint32_t *s_ent = L.ent, *s_w = L.w, *s_m = L.m;
+    uint64_t *s_ex = L.ex;
+    uint16_t *s_bk = L.bk;
+    uint64_t *out = p.out + (size_t)q * p.k;
+    const uint32_t nA = min(p.ca, c);
+    const bool prune = p.prune && nA < c && nA >= p.k;
+    if (tid == 0) {
+        s_m[MN] = 0xffffu; s_m[MX] = 0u; s_m[EXQ] = 0u; s_m[XNQ] = 0u;
+        s_m[THR] = 0xffffu;
+        p.out_cnt[q] = min(p.k, c);
+    }
+    for (uint32_t i = tid; i < p.dpad; i += 256) s_qv[i] = p.qv[(size_t)q * p.dpad + i];
+    const float qn = p.qnorm ? p.qnorm[q] : 0.0f;
+    if (!prune) {   // rerank the whole pool
+        for (uint32_t i = tid; i < c; i += 256) s_ent[i] = pe[i];
+        __syncthreads();
+        rerank_list(p, s_ent, c, s_qv, qn, s_ex);
+        __syncthreads();
+        smallest_k(s_ex, c, p.k, out);
+        return;
+    }
+    // 1. A = the nA smallest keys; thread t owns pool positions [tE, tE+E)
+    const uint32_t E = (c + 255) / 256, i0 = tid * E, i1 = min(i0 + E, c);
+    for (uint32_t i = tid; i < 2 * 256; i += 256) (&L.hist[0][0])[i] = 0u;
+    float exq = 0.0f, xnq = 0.0f;
+    for (uint32_t j = tid; j < p.np_q[q]; j += 256) {
+        const uint32_t post = p.probe_post[(size_t)q * p.npr + j];
+        exq = fmaxf(exq, p.post_ex[post]);
+        xnq = fmaxf(xnq, p.post_xn[post]);
+    }
+    uint32_t kmn = 0xffffu, kmx = 0u;
+    for (uint32_t i = i0; i < i1; i++) {
+        const uint32_t kk = pk[i];
+        kmn = min(kmn, kk);
+        kmx = max(kmx, kk);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        kmn = min(kmn, (uint32_t)__shfl_xor(kmn, off, 64));
+        kmx = max(kmx, (uint32_t)__shfl_xor(kmx, off, 64));
+        exq = fmaxf(exq, __shfl_xor(exq, off, 64));
+        xnq = fmaxf(xnq, __shfl_xor(xnq, off, 64));
+    }
+    __syncthreads();   // s_m initialised, histograms zeroed
+    if (lane == 0) {
+        atomicMin(&s_m[MN], kmn);
+        atomicMax(&s_m[MX], kmx);
+        atomicMax(&s_m[EXQ], __float_as_uint(exq));   // non-negative floats order as bits
+        atomicMax(&s_m[XNQ], __float_as_uint(xnq));
+    }
+    __syncthreads();
+    kmn = s_m[MN];
+    kmx = s_m[MX];
+    const uint32_t span = kmx - kmn;
+    const uint32_t sh = span < 256 ? 0 : 32 - __clz(span) - 8;
+    for (uint32_t i = i0; i < i1; i++) atomicAdd(&L.hist[0][(pk[i] - kmn) >> sh], 1u);
+    __syncthreads();
+    {
+        const uint32_t h = L.hist[0][tid];
+        find_rank_bin(h, blk_excl(h, s_w), nA, &s_m[BA]);
+    }
+    __syncthreads();
+    const uint32_t b1 = s_m[BA], below1 = s_m[BEA];
+    uint32_t b2 = 0, below2 = 0;
+    if (sh > 0) {
+        const uint32_t mask = (1u << sh) - 1u;
+        for (uint32_t i = i0; i < i1; i++) {
+            const uint32_t kk = pk[i];
+            if (((kk - kmn) >> sh) == b1) atomicAdd(&L.hist[1][(kk - kmn) & mask], 1u);
+        }
+        __syncthreads();
+        const uint32_t h = L.hist[1][tid];
+        find_rank_bin(h, blk_excl(h, s_w), nA - below1, &s_m[BA]);
+        __syncthreads();
+        b2 = s_m[BA];
+        below2 = s_m[BEA];
+    }
+    const uint32_t TA = kmn + (b1 << sh) + b2, needA = nA - below1 - below2;
+    // split in pool order: per-thread counts, block scans, then the writes
+    uint32_t lt = 0, eq = 0;
+    for (uint32_t i = i0; i < i1; i++) {
+        const uint32_t kk = pk[i];
+        lt += kk < TA;
+        eq += kk == TA;
+    }
+    uint32_t ra = blk_excl(eq, s_w);
+    const uint32_t cA = lt + (ra >= needA ? 0u : min(eq, needA - ra));
+    uint32_t pA = blk_excl(cA, s_w), pB = blk_excl((i1 > i0 ? i1 - i0 : 0u) - cA, s_w);
+    for (uint32_t i = i0; i < i1; i++) {
+        const uint32_t kk = pk[i];
+        bool inA = kk < TA;
+        if (kk == TA) {
+            inA = ra < needA;
+            ra++;
+        }
+        if (inA) {
+            s_ent[pA++] = pe[i];
+        } else {
+            s_ent[nA + pB] = pe[i];
+            s_bk[nA + pB] = (uint16_t)kk;
+            pB++;
+        }
+    }
+    __syncthreads();
+    const uint32_t nB = c - nA;
+
+    // 2. rerank A; its k-th exact key (keys are distinct: the rank is in the low bits)
+    rerank_list(p, s_ent, nA, s_qv, qn, s_ex);
+    __syncthreads();
+    if (nA <= kPrCount) {
+        for (uint32_t i = tid; i < nA; i += 256) {
+            const uint64_t v = s_ex[i];
+            uint32_t cnt = 0;
+            for (uint32_t j = 0; j < nA; j++) cnt += s_ex[j] < v;
+            if (cnt == p.k - 1) L.kth = v;
+        }
+    } else {
+        uint32_t mA = 2;
+        while (mA < nA) mA <<= 1;
+        for (uint32_t i = nA + tid; i < mA; i += 256) s_ex[i] = kNoKey;
+        __syncthreads();
+        bitonic_lds(s_ex, mA);
+        if (tid == 0) L.kth = s_ex[p.k - 1];
+    }
+    __syncthreads();
+
+    // 3. bound: a key above thr proves an exact distance above A's k-th
+    if (tid == 0) {
+        uint32_t thr = 0xffffu;
+        const float kth = key_float_m((uint32_t)(L.kth >> 32));
+        if (kth >= 0.0f && kth < __builtin_inff()) {
+            const double u = 1.0 / 8388608.0;   // 2^-23
+            double ksq = (double)kth;
+            if (p.metric == M_L2) ksq = ksq * ksq * (1.0 + 1.0 / 1048576.0);
+            // the f32 exact sum is at most (dim + 8) 2^-24 relative below the true value
+            const double s = sqrt(ksq * (1.0 + (p.dim + 16) * u));
+            const double eqd = (double)p.eq16[q] * (1.0 + 1e-5) + 1e-30;
+            const double r = s + eqd + (double)__uint_as_float(s_m[EXQ]) * (1.0 + 1e-6);
+            const double qn16 = p.qn16[q], xn16 = __uint_as_float(s_m[XNQ]);
+            // f32 evaluation of |x16|^2 + |q16|^2 - 2 q16.x16 (exact products)
+            const double e1 = (p.kdim + 16) * u * (qn16 + xn16 + 2.0 * sqrt(qn16 * xn16)) * 2.0;
+            const double v = (r * r + e1) * (1.0 + 1e-9);
+            if (v < 3.0e38) {
+                float vf = (float)v;   // round up
+                if ((double)vf < v) vf = __uint_as_float(__float_as_uint(vf) + 1u);
+                thr = min(0x8000u | (__float_as_uint(vf) >> 16), 0xfffeu);
+            }
+        }
+        s_m[THR] = thr;
+    }
+    __syncthreads();
+    const uint32_t thr = s_m[THR];
+    // survivors, compacted to the front of ent (A's entries are consumed):
+    // read into registers, written after the scan's barriers
+    constexpr uint32_t kPer = CAP / 256;
+    uint32_t be[kPer], nb = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; r++) {
+        const uint32_t i = tid + 256 * r;
+        const bool ok = i < nB && s_bk[nA + i] <= thr;
+        be[r] = ok ? s_ent[nA + i] : 0xffffffffu;
+        nb += ok;
+    }
+    uint32_t pos = blk_excl(nb, s_w);
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; r++)
+        if (be[r] != 0xffffffffu) s_ent[pos++] = be[r];
+    if (tid == 255) s_m[CB2] = pos;   // the last thread's prefix + count = total
+    __syncthreads();
+    const uint32_t nB2 = s_m[CB2];
+    rerank_list(p, s_ent, nB2, s_qv, qn, s_ex + nA);
+    __syncthreads();
+    smallest_k(s_ex, nA + nB2, p.k, out);
+    if (p.stats && tid == 0) {
+        atomicAdd(&p.stats[0], nA + nB2);
+        atomicAdd(&p.stats[1], c);
+    }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -734,6 +1053,17 @@ hipError_t launch_rerank_pool(const RerankPoolParams &p, hipStream_t s) {
     if (p.nq == 0) return hipSuccess;
     const uint32_t grid = (p.nq + kXcds - 1) / kXcds * kXcds;
     hipLaunchKernelGGL(k_rerank_pool, dim3(grid), dim3(256), p.dpad * sizeof(float), s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_rerank_pruned(const RerankPrunedParams &p, hipStream_t s) {
+    if (p.nq == 0) return hipSuccess;
+    if (p.k > p.ca || p.ca > p.want || p.want > kMaxK) return hipErrorInvalidValue;
+    const uint32_t grid = (p.nq + kXcds - 1) / kXcds * kXcds;
+    if (p.want <= 1024)
+        hipLaunchKernelGGL(k_rerank_pruned<1024>, dim3(grid), dim3(256), p.dpad * sizeof(float), s, p);
+    else
+        hipLaunchKernelGGL(k_rerank_pruned<kMaxK>, dim3(grid), dim3(256), p.dpad * sizeof(float), s, p);
     return hipGetLastError();
 }
 

@@ -120,3 +120,44 @@ def test_mixed_short_rows():
     np.testing.assert_array_equal(cnt, e_cnt)
     for i in range(3):
         np.testing.assert_array_equal(ids[i, : cnt[i]], e_ids[i, : cnt[i]])
+
+
+@pytest.mark.parametrize("metric", ["l2", "l2_root"])
+@pytest.mark.parametrize("data", ["float", "integer", "wide"])
+@pytest.mark.parametrize("k,pool", [(1, 64), (10, 200), (100, 1000)])
+def test_pruned_rerank_equals_full_pool_rerank(metric, data, k, pool):
+    """The bound-pruned rerank returns exactly what reranking every pool member
+    returns, while gathering fewer raw rows."""
+    rng = np.random.default_rng(11)
+    if data == "integer":
+        c = rng.uniform(0, 200, size=(40, 64))
+        x = np.clip(np.rint(c[rng.integers(0, 40, 8000)] + rng.normal(0, 20, (8000, 64))), 0, 255)
+        q = np.clip(np.rint(c[rng.integers(0, 40, 64)] + rng.normal(0, 20, (64, 64))), 0, 255)
+    elif data == "wide":   # large magnitudes: f16 rounding error matters
+        x = clustered(8000, 48, 30, seed=12) * 3000.0
+        q = clustered(64, 48, 30, seed=13) * 3000.0
+    else:
+        x = clustered(8000, 100, 30, seed=12)
+        q = clustered(64, 100, 30, seed=13)
+    x, q = x.astype(np.float32), q.astype(np.float32)
+    ix = build_index(x, metric, None, n_postings=64, seed=0)
+    dev = DeviceIndex(ix)
+    print(f"[case] prune metric={metric} data={data} k={k} pool={pool}", flush=True)
+    a_ids, a_d, a_c = dev.search(q, k, 24, precision=PREC_MIXED, rerank_size=pool)
+    b_ids, b_d, b_c = dev.search(q, k, 24, precision=PREC_MIXED, rerank_size=pool,
+                                 flags=_lib.FLAG_FULL_RERANK)
+    np.testing.assert_array_equal(a_c, b_c)
+    np.testing.assert_array_equal(a_ids, b_ids)
+    np.testing.assert_array_equal(a_d.view(np.uint32), b_d.view(np.uint32))
+
+
+def test_pruned_rerank_gathers_fewer_rows():
+    x = clustered(20000, 64, 40, seed=14)
+    q = clustered(256, 64, 40, seed=15)
+    ix = build_index(x, "l2", None, n_postings=128, seed=0)
+    dev = DeviceIndex(ix)
+    dev.search(q, 10, 16, precision=PREC_MIXED, rerank_size=500, flags=_lib.FLAG_PROFILE)
+    pr = _lib.last_profile()
+    print(f"rerank rows {pr['rerank_rows']} of {pr['pool_members']} pool members")
+    assert pr["pool_members"] == 256 * 500
+    assert 256 * 10 <= pr["rerank_rows"] < pr["pool_members"] // 2
