@@ -74,7 +74,7 @@ struct Workspace {
     DBuf<uint64_t> probe_keys, probe_sel;
     DBuf<uint32_t> probe_cnt, np_q, probe_post;
     DBuf<uint64_t> cand_len, cand_off, qp_slot, pair_slot;
-    DBuf<uint32_t> post_cnt, post_fill, pair_off, pair_q, item_cnt, item_off, item_post;
+    DBuf<uint32_t> post_cnt, qp_rel, prank, pair_off, pair_q, item_cnt, item_off, item_post;
     DBuf<uint64_t> cand, sel, fin;
     DBuf<uint32_t> sel_cnt, fin_cnt;
     DBuf<uint64_t> out_ids;
@@ -89,7 +89,7 @@ struct Workspace {
     DBuf<float> rq_consts;
     DBuf<uint32_t> pair_src;
     DBuf<float> eq16;
-    DBuf<uint32_t> rr_stats;
+    DBuf<uint32_t> rr_stats, sel_bits;
     uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
     hipEvent_t ev[SPF_N_STAGES + 1] = {};
 
@@ -115,10 +115,11 @@ struct Workspace {
         pair_src.release();
         eq16.release();
         rr_stats.release();
+        sel_bits.release();
         qn16.release();
         DBuf<float> *fs[] = {&qin, &qv_probe, &qv_scan, &qnorm, &out_d};
         for (auto *b : fs) b->release();
-        DBuf<uint32_t> *us[] = {&q8, &qsq, &probe_cnt, &np_q, &probe_post, &post_cnt, &post_fill,
+        DBuf<uint32_t> *us[] = {&q8, &qsq, &probe_cnt, &np_q, &probe_post, &post_cnt, &qp_rel, &prank,
                                 &pair_off, &pair_q, &item_cnt, &item_off, &item_post, &sel_cnt,
                                 &fin_cnt,
                                 &out_c};
@@ -581,8 +582,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
                  bool stats) {
     const uint32_t dpad32 = round_up(ix->dim, 32);
     HIPCHK(w->cand16.ensure(std::max<uint64_t>(total, 1)));
-    HIPCHK(w->pool_entry.ensure((size_t)nq * want));
-    HIPCHK(w->pool_key.ensure((size_t)nq * want));
+    HIPCHK(w->sel_bits.ensure(std::max<uint64_t>(total / 32, 1)));
     HIPCHK(w->pool_want.ensure(nq));
     HIPCHK(w->fin.ensure((size_t)nq * k));
     HIPCHK(w->fin_cnt.ensure(nq));
@@ -631,22 +631,26 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     pp.qp_slot = w->qp_slot.p;
     pp.probe_post = w->probe_post.p;
     pp.post_off = ix->d_post_off;
-    pp.pool_entry = w->pool_entry.p;
-    pp.pool_key = w->pool_key.p;
     pp.pool_want = w->pool_want.p;
+    pp.sel_bits = w->sel_bits.p;
     HIPCHK(launch_pool_select(pp, nq, s));
     // nearest-posting query order, then the (pruned) exact rerank of each pool
     HIPCHK(mark(SPF_STAGE_RERANK));
     HIPCHK(launch_query_perm(w->np_q.p, w->probe_post.p, npr, nq, ix->P, w->perm_cnt.p,
                              w->perm_off.p, w->perm_fill.p, w->qperm.p, s));
     RerankPrunedParams rp{};
-    rp.pool_entry = w->pool_entry.p;
-    rp.pool_key = w->pool_key.p;
+    rp.keys = w->cand16.p;
+    rp.seg_off = w->cand_off.p;
+    rp.seg_cnt = w->cand_cnt.p;
+    rp.sel_bits = w->sel_bits.p;
     rp.pool_want = w->pool_want.p;
+    rp.qp_slot = w->qp_slot.p;
+    rp.post_off = ix->d_post_off;
     rp.nq = nq;
     rp.want = want;
     rp.k = k;
-    rp.ca = std::min(want, std::max(k, round_up(k + k / 4 + 32, 64)));
+    // phase-A size: k plus a margin for the 16-bit key resolution
+    rp.ca = std::min(want, std::max(k, round_up(k + k / 8 + 8, 32)));
     rp.metric = ix->metric;
     rp.dpad = ix->dpad;
     rp.dim = ix->dim;
@@ -866,7 +870,8 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(w->cand_cnt.ensure(nq));
     HIPCHK(w->cand_off.ensure(nq + 1));
     HIPCHK(w->post_cnt.ensure(P + 1));
-    HIPCHK(w->post_fill.ensure(P + 1));
+    HIPCHK(w->qp_rel.ensure((size_t)nq * npr));
+    HIPCHK(w->prank.ensure((size_t)nq * npr));
     HIPCHK(w->pair_off.ensure(P + 1));
     HIPCHK(w->qp_slot.ensure((size_t)nq * npr));
     HIPCHK(w->pair_q.ensure((size_t)nq * npr));
@@ -888,7 +893,8 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.cand_cnt = w->cand_cnt.p;
     pl.cand_off = w->cand_off.p;
     pl.post_cnt = w->post_cnt.p;
-    pl.post_fill = w->post_fill.p;
+    pl.qp_rel = w->qp_rel.p;
+    pl.prank = w->prank.p;
     pl.pair_off = w->pair_off.p;
     pl.qp_slot = w->qp_slot.p;
     pl.pair_q = w->pair_q.p;
@@ -896,11 +902,9 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.item_cnt = w->item_cnt.p;
     pl.pair_src = ix->rq_l2 ? w->pair_src.p : nullptr;
     pl.G = G;
-    HIPCHK(launch_plan_probe(pl, s));
-    HIPCHK(launch_excl_scan_u64(w->cand_len.p, w->cand_off.p, nq, s));
     HIPCHK(hipMemsetAsync(w->post_cnt.p, 0, (P + 1) * sizeof(uint32_t), s));
-    HIPCHK(hipMemsetAsync(w->post_fill.p, 0, (P + 1) * sizeof(uint32_t), s));
-    HIPCHK(launch_plan_pairs(pl, s));
+    HIPCHK(launch_plan_q(pl, s));
+    HIPCHK(launch_excl_scan_u64(w->cand_len.p, w->cand_off.p, nq, s));
     HIPCHK(launch_excl_scan_u32(w->post_cnt.p, w->pair_off.p, P, s));
     HIPCHK(launch_scatter_pairs(pl, s));
     HIPCHK(launch_items(pl, s));

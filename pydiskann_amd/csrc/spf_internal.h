@@ -125,14 +125,18 @@ struct MixedParams {
     uint16_t *out16;            // 16-bit candidate keys
 };
 
-// Exact rerank of each query's pool (entries + 16-bit keys from pool
-// selection).  With prune, only the `ca` smallest keys and the members whose
+// Exact rerank of each query's pool (selection masks over the candidate
+// keys from pool selection).  With prune, only the `ca` smallest keys and the members whose
 // key does not prove an exact distance above the k-th of those are reranked
 // (L2 metrics over f16 keys); the result equals reranking the whole pool.
 struct RerankPrunedParams {
-    const uint32_t *pool_entry; // [nq][want]
-    const uint16_t *pool_key;   // [nq][want]
-    const uint32_t *pool_want;  // [nq]
+    const uint16_t *keys;       // candidate keys (segments at seg_off)
+    const uint64_t *seg_off;    // [nq] first slot (multiple of 32)
+    const uint32_t *seg_cnt;    // [nq]
+    const uint32_t *sel_bits;   // pool membership, one mask per 32-slot word
+    const uint32_t *pool_want;  // [nq] pool size
+    const uint64_t *qp_slot;    // [nq][npr] absolute first slot of each probed posting
+    const uint32_t *post_off;
     uint32_t nq, want, k, ca, metric, dpad, dim;
     const uint32_t *qperm;      // processing order (null = identity)
     const uint32_t *np_q;       // [nq]
@@ -204,6 +208,7 @@ struct PoolParams {
     uint32_t *pool_entry;       // [nq][want]
     uint16_t *pool_key;         // [nq][want] keys of the members (null = not written)
     uint32_t *pool_want;        // [nq]
+    uint32_t *sel_bits;         // non-null: write per-word selection masks instead
 };
 
 // Exact f32 rerank of each query's pool and the final top-k, one workgroup per
@@ -232,8 +237,9 @@ struct PlanParams {
     uint64_t *cand_len;         // [nq + 1] segment length, rounded up to 32 slots
     uint32_t *cand_cnt;         // [nq] candidates actually written in the segment
     const uint64_t *cand_off;   // [nq + 1]
-    uint32_t *post_cnt;         // [n_post + 1]
-    uint32_t *post_fill;        // [n_post]
+    uint32_t *post_cnt;         // [n_post + 1] pairs per posting
+    uint32_t *qp_rel;           // [nq * npr] slot of each pair within its query's segment
+    uint32_t *prank;            // [nq * npr] rank of each pair within its posting
     const uint32_t *pair_off;   // [n_post + 1]
     uint64_t *qp_slot;          // [nq * npr] slot of each (query, probe) pair
     uint32_t *pair_q;           // [nq * npr] posting-major pairs
@@ -265,8 +271,7 @@ hipError_t launch_scan(uint32_t code_kind, uint32_t metric, bool dense, uint32_t
 hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, const uint32_t *seg_cnt,
                          uint64_t seg_stride, uint32_t nseg, uint32_t K, uint64_t *out,
                          uint32_t *out_cnt, hipStream_t s);
-hipError_t launch_plan_probe(const PlanParams &p, hipStream_t s);
-hipError_t launch_plan_pairs(const PlanParams &p, hipStream_t s);
+hipError_t launch_plan_q(const PlanParams &p, hipStream_t s);
 hipError_t launch_scatter_pairs(const PlanParams &p, hipStream_t s);
 hipError_t launch_items(const PlanParams &p, hipStream_t s);
 hipError_t launch_item_table(const PlanParams &p, hipStream_t s);

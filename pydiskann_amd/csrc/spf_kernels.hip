@@ -527,58 +527,74 @@ __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint
 }
 
 // ---------------------------------------------------------------- planning
-// probe_ratio cut + per-query candidate count (one thread per query)
-__global__ void k_plan_probe(const PlanParams p) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= p.nq) return;
+// One wavefront per query (four per workgroup): the probe_ratio cut (the first
+// probe beyond d0 * max(ratio, 1) ends the list), the kept postings, the
+// query's candidate count, each probe's slot relative to the query's segment
+// (wave prefix sums in probe order), and the pair's rank within its posting
+// (the posting-major counting sort's count, taken with the returned value).
+__global__ __launch_bounds__(256) void k_plan_q(const PlanParams p) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= p.nq) return;   // whole wave
     const uint64_t *sel = p.probe_sel + (size_t)q * p.npr;
     const uint32_t c = p.probe_cnt[q];
-    uint32_t kept = 0;
-    uint64_t total = 0;
+    uint32_t kept = c;
     if (c > 0) {
         const float d0 = key_to_float((uint32_t)(sel[0] >> 32));
         const float ratio = p.probe_ratio;
         const bool cut = isfinite(ratio) && d0 >= 0.0f;
         const float lim = d0 * fmaxf(ratio, 1.0f);
-        for (uint32_t j = 0; j < c; j++) {
-            const float d = key_to_float((uint32_t)(sel[j] >> 32));
-            if (cut && d > lim) break;
+        if (cut)
+            for (uint32_t j0 = 0; j0 < c; j0 += 64) {
+                const uint32_t j = j0 + lane;
+                const bool beyond = j < c && key_to_float((uint32_t)(sel[j] >> 32)) > lim;
+                const uint64_t m = __ballot(beyond);
+                if (m) {
+                    kept = j0 + (uint32_t)(__ffsll((unsigned long long)m) - 1);
+                    break;
+                }
+            }
+    }
+    uint32_t run = 0;   // slots of the probes before this round
+    for (uint32_t j0 = 0; j0 < kept; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        uint32_t len = 0;
+        if (j < kept) {
             const uint32_t post = (uint32_t)sel[j];
-            p.probe_post[(size_t)q * p.npr + kept] = post;
-            total += p.post_len[post];
-            kept++;
+            len = p.post_len[post];
+            const size_t i = (size_t)q * p.npr + j;
+            p.probe_post[i] = post;
+            p.prank[i] = atomicAdd(&p.post_cnt[post], 1u);
         }
+        uint32_t incl = len;
+#pragma unroll
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        if (j < kept) p.qp_rel[(size_t)q * p.npr + j] = run + incl - len;
+        run += __shfl(incl, 63, 64);
     }
-    p.np_q[q] = kept;
-    p.cand_cnt[q] = (uint32_t)total;
-    p.cand_len[q] = (total + 31) & ~31ull;   // 32-aligned segments: a query owns whole bitmap words
-}
-
-// (query, probe) pairs: candidate-buffer slot of each probed posting, and a
-// count of pairs per posting (one thread per query)
-__global__ void k_plan_pairs(const PlanParams p) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= p.nq) return;
-    uint64_t slot = p.cand_off[q];
-    const uint32_t kept = p.np_q[q];
-    for (uint32_t j = 0; j < kept; j++) {
-        const uint32_t post = p.probe_post[(size_t)q * p.npr + j];
-        p.qp_slot[(size_t)q * p.npr + j] = slot;
-        slot += p.post_len[post];
-        atomicAdd(&p.post_cnt[post], 1u);
+    if (lane == 0) {
+        p.np_q[q] = kept;
+        p.cand_cnt[q] = run;
+        p.cand_len[q] = (run + 31) & ~31ull;   // 32-aligned segments: a query owns whole bitmap words
     }
 }
 
-// posting-major scatter of the pairs (one thread per (query, probe))
+// posting-major pairs, the absolute candidate slot of each (query, probe)
+// pair (one thread per pair)
 __global__ void k_scatter_pairs(const PlanParams p) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.nq * p.npr) return;
     const uint32_t q = i / p.npr, j = i - q * p.npr;
     if (j >= p.np_q[q]) return;
     const uint32_t post = p.probe_post[i];
-    const uint32_t pos = p.pair_off[post] + atomicAdd(&p.post_fill[post], 1u);
+    const uint64_t slot = p.cand_off[q] + p.qp_rel[i];
+    const uint32_t pos = p.pair_off[post] + p.prank[i];
+    p.qp_slot[i] = slot;
     p.pair_q[pos] = q;
-    p.pair_slot[pos] = p.qp_slot[i];
+    p.pair_slot[pos] = slot;
     if (p.pair_src) p.pair_src[pos] = i;
 }
 
@@ -819,13 +835,9 @@ hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, const ui
     return hipGetLastError();
 }
 
-hipError_t launch_plan_probe(const PlanParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_plan_probe, dim3((p.nq + 255) / 256), dim3(256), 0, s, p);
-    return hipGetLastError();
-}
-
-hipError_t launch_plan_pairs(const PlanParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_plan_pairs, dim3((p.nq + 255) / 256), dim3(256), 0, s, p);
+hipError_t launch_plan_q(const PlanParams &p, hipStream_t s) {
+    if (p.nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plan_q, dim3((p.nq + 3) / 4), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

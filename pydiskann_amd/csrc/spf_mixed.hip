@@ -396,7 +396,7 @@ __device__ __forceinline__ void wave_find_bin(const uint32_t *hist, uint32_t rem
 }
 
 // bit masks of one 32-key word: live keys, keys below T, keys equal to T
-__device__ __forceinline__ void word_masks(const uint4 r[4], uint32_t w, uint32_t n, uint32_t T,
+__device__ __forceinline__ void word_masks(const uint4 (&r)[4], uint32_t w, uint32_t n, uint32_t T,
                                            uint32_t &live, uint32_t &lt, uint32_t &eq) {
     uint32_t lv = 0, l = 0, e = 0;
 #pragma unroll
@@ -418,11 +418,13 @@ __device__ __forceinline__ void word_masks(const uint4 r[4], uint32_t w, uint32_
 }
 
 // key j (runtime) of a 32-key word
-__device__ __forceinline__ uint32_t word_key(const uint4 r[4], uint32_t j) {
+__device__ __forceinline__ uint32_t word_key(const uint4 (&r)[4], uint32_t j) {
     const uint32_t c = j >> 3, t = (j >> 1) & 3;
-    const uint4 v = c == 0 ? r[0] : c == 1 ? r[1] : c == 2 ? r[2] : r[3];
-    const uint32_t d = t == 0 ? v.x : t == 1 ? v.y : t == 2 ? v.z : v.w;
-    return (d >> (16 * (j & 1))) & 0xffffu;
+    uint32_t d[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) d[i] = t == 0 ? r[i].x : t == 1 ? r[i].y : t == 2 ? r[i].z : r[i].w;
+    const uint32_t v = c == 0 ? d[0] : c == 1 ? d[1] : c == 2 ? d[2] : d[3];
+    return (v >> (16 * (j & 1))) & 0xffffu;
 }
 
 // the lowest `c` set bits of m
@@ -438,20 +440,32 @@ __device__ __forceinline__ uint32_t lowest_bits(uint32_t m, uint32_t c) {
 }
 
 // One wavefront per query: the `want` smallest 16-bit keys of its segment,
-// exactly-tied keys at the threshold taken in slot order, written as entries
-// (map) or slots, with their keys (pool_key, optional).  Lane l owns words
+// exactly-tied keys at the threshold taken in slot order, written as one
+// selection mask per 32-slot word (sel_bits, at the segment's word offset),
+// or else as entries (map) or slots, with their keys (pool_key, optional).
+// Lane l owns words
 // l, l+64, ... (word w = slots 32w..32w+31) and re-reads them from L2 in each
 // of the four passes; a word's four loads go out together.
 __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32_t n,
                                           uint32_t want, uint32_t *pool_entry,
                                           uint16_t *pool_key, uint32_t *pool_want, bool map,
-                                          const ProbeTab &tab, uint32_t *hist) {
+                                          const ProbeTab &tab, uint32_t *hist,
+                                          uint32_t *sel_bits) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nwords = (n + 31) / 32;
+    // the lane's words two at a time: both words' loads go out before use
+    auto sweep = [&](auto &&body) {
+        for (uint32_t w = lane; w < nwords; w += 128) {
+            uint4 r0[4], r1[4];
+            const uint32_t w1 = w + 64;
+            load_word_raw(kp, w, r0);
+            if (w1 < nwords) load_word_raw(kp, w1, r1);
+            body(r0, w);
+            if (w1 < nwords) body(r1, w1);
+        }
+    };
     uint32_t inval = 0, kmn = 0xffffu, kmx = 0u;
-    for (uint32_t w = lane; w < nwords; w += 64) {
-        uint4 r[4];
-        load_word_raw(kp, w, r);
+    sweep([&](const uint4 (&r)[4], uint32_t w) {
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
@@ -467,7 +481,7 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
                 }
             }
         }
-    }
+    });
     for (int off = 32; off > 0; off >>= 1) {
         inval += __shfl_xor(inval, off, 64);
         kmn = min(kmn, (uint32_t)__shfl_xor(kmn, off, 64));
@@ -485,9 +499,7 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
         for (int i = 0; i < 4; i++) hist[4 * lane + i] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        for (uint32_t w = lane; w < nwords; w += 64) {
-            uint4 r[4];
-            load_word_raw(kp, w, r);
+        sweep([&](const uint4 (&r)[4], uint32_t w) {
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
@@ -497,7 +509,7 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
                     if (kk != 0xffffu && 32 * w + 8 * c + t < n) atomicAdd(&hist[(kk - kmn) >> sh], 1u);
                 }
             }
-        }
+        });
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
         uint32_t b1, below1, b2 = 0, below2 = 0;
@@ -509,9 +521,7 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             const uint32_t mask = (1u << sh) - 1u;
-            for (uint32_t w = lane; w < nwords; w += 64) {
-                uint4 r[4];
-                load_word_raw(kp, w, r);
+            sweep([&](const uint4 (&r)[4], uint32_t w) {
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
                     const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
@@ -522,7 +532,7 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
                             atomicAdd(&hist[(kk - kmn) & mask], 1u);
                     }
                 }
-            }
+            });
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
             wave_find_bin(hist, W - below1, b2, below2);
@@ -532,18 +542,24 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
     }
     // collect: round m = words 64m .. 64m+63 in lane order (slot order)
     uint32_t tie_run = 0, mark_run = 0;
+    uint4 rn[4];   // next round's word, loaded ahead
+    if (lane < nwords) load_word_raw(kp, lane, rn);
     for (uint32_t m = 0; 64 * m < nwords; m++) {
         const uint32_t w = lane + 64 * m;
         uint4 r[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) r[c] = rn[c];
+        if (w + 64 < nwords) load_word_raw(kp, w + 64, rn);
         uint32_t live = 0, lt = 0, eq = 0;
-        if (w < nwords) {
-            load_word_raw(kp, w, r);
-            word_masks(r, w, n, T, live, lt, eq);
-        }
+        if (w < nwords) word_masks(r, w, n, T, live, lt, eq);
         uint32_t tt;
         const uint32_t tr = tie_run + wave_excl(all ? 0u : __popc(eq), &tt);
         tie_run += tt;
         const uint32_t sel = all ? live : lt | (tr < need ? lowest_bits(eq, need - tr) : 0u);
+        if (sel_bits) {   // one coalesced mask store per word
+            if (w < nwords) sel_bits[w] = sel;
+            continue;
+        }
         uint32_t mt;
         uint32_t pre = mark_run + wave_excl(__popc(sel), &mt);
         mark_run += mt;
@@ -589,7 +605,7 @@ __global__ __launch_bounds__(256) void k_pool_select(const PoolParams p, uint32_
         tab.ebase = s_ebase[wv];
     }
     pool_wave(p.keys + b, q, n, p.want, p.pool_entry, p.pool_key, p.pool_want, p.map, tab,
-              hist[wv]);
+              hist[wv], p.sel_bits ? p.sel_bits + b / 32 : nullptr);
 }
 
 // ---------------------------------------------------------------- query order
@@ -826,11 +842,19 @@ __device__ __forceinline__ void smallest_k(uint64_t *keys, uint32_t cnt, uint32_
 template <uint32_t CAP>
 struct RerankLds {
     union {
-        uint64_t ex[CAP];        // exact keys (A, then survivors)
-        uint32_t hist[2][256];   // radix histograms (step 1)
+        struct {
+            uint64_t ex[CAP];    // exact keys (A, then survivors)
+            uint32_t ent[CAP];   // A at [0, nA), others from nA; later survivors
+            uint16_t bk[CAP];    // keys of the others
+        };
+        struct {                 // step 0: non-empty mask words
+            uint32_t nzw[CAP], nzm[CAP], nzp[CAP];   // index, mask, first member
+        };
+        uint32_t hist[2][256];   // step 1: radix histograms
     };
-    uint32_t ent[CAP];           // A at [0, nA), others from nA; later survivors
-    uint16_t bk[CAP];            // keys of the others
+    uint32_t pe[CAP];            // pool entries, in slot order
+    uint16_t pk[CAP];            // pool keys
+    uint32_t pbase[kWaveTab], ebase[kWaveTab];
     uint32_t w[8];
     uint32_t m[16];
     uint64_t kth;
@@ -840,7 +864,7 @@ template <uint32_t CAP>
 __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams p) {
     __shared__ RerankLds<CAP> L;
     extern __shared__ __align__(16) float s_qv[];   // dpad floats
-    enum { MN, MX, EXQ, XNQ, BA, BEA, CB2, THR };
+    enum { MN, MX, EXQ, XNQ, BA, BEA, CB2, THR, NZ };
     const uint32_t x = blockIdx.x % kXcds, jx = blockIdx.x / kXcds;
     const uint32_t per = (p.nq + kXcds - 1) / kXcds;
     const uint32_t idx = x * per + jx;
@@ -848,8 +872,8 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
     const uint32_t q = p.qperm ? p.qperm[idx] : idx;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     const uint32_t c = min(p.pool_want[q], p.want);
-    const uint32_t *pe = p.pool_entry + (size_t)q * p.want;
-    const uint16_t *pk = p.pool_key + (size_t)q * p.want;
+    const uint32_t *pe = L.pe;
+    const uint16_t *pk = L.pk;
     uint32_t *s_ent = L.ent, *s_w = L.w, *s_m = L.m;
     uint64_t *s_ex = L.ex;
     uint16_t *s_bk = L.bk;
@@ -863,16 +887,76 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
     }
     for (uint32_t i = tid; i < p.dpad; i += 256) s_qv[i] = p.qv[(size_t)q * p.dpad + i];
     const float qn = p.qnorm ? p.qnorm[q] : 0.0f;
-    if (!prune) {   // rerank the whole pool
-        for (uint32_t i = tid; i < c; i += 256) s_ent[i] = pe[i];
+    // 0. the pool from the selection masks, in slot order: thread t owns a
+    // contiguous word range; slots map to entries through the probe table
+    {
+        const uint64_t b = p.seg_off[q];
+        const uint32_t n = p.seg_cnt[q];
+        const uint32_t nwords = (n + 31) / 32, Mw = (nwords + 255) / 256;
+        const uint32_t w0 = tid * Mw, w1 = min(w0 + Mw, nwords);
+        const uint32_t *bits = p.sel_bits + b / 32;
+        ProbeTab tab;
+        tab.np = p.np_q[q];
+        tab.qp_row = p.qp_slot + (size_t)q * p.npr;
+        tab.probe_row = p.probe_post + (size_t)q * p.npr;
+        tab.post_off = p.post_off;
+        tab.seg = b;
+        tab.pbase = tab.ebase = nullptr;
+        if (tab.np <= kWaveTab) {
+            for (uint32_t j = tid; j < tab.np; j += 256) {
+                L.pbase[j] = (uint32_t)(tab.qp_row[j] - b);
+                L.ebase[j] = p.post_off[tab.probe_row[j]];
+            }
+            tab.pbase = L.pbase;
+            tab.ebase = L.ebase;
+        }
+        // non-empty words in slot order with the pool position of their first member
+        uint32_t nz = 0, cnt = 0;
+        for (uint32_t w = w0; w < w1; w++) {
+            const uint32_t m = bits[w];
+            nz += m != 0u;
+            cnt += __popc(m);
+        }
+        uint32_t zpos = blk_excl(nz, s_w);   // its barriers also publish the table
+        uint32_t ppos = blk_excl(cnt, s_w);
+        for (uint32_t w = w0; w < w1; w++) {
+            const uint32_t m = bits[w];
+            if (m && zpos < CAP) {
+                L.nzw[zpos] = w;
+                L.nzm[zpos] = m;
+                L.nzp[zpos] = ppos;
+                zpos++;
+                ppos += __popc(m);
+            }
+        }
+        if (tid == 255) s_m[NZ] = zpos;   // the last thread's prefix + count = total
         __syncthreads();
-        rerank_list(p, s_ent, c, s_qv, qn, s_ex);
+        const uint32_t nzt = s_m[NZ];
+        // pool member i: its word by binary search, then its bit in the word
+        for (uint32_t i = tid; i < c; i += 256) {
+            uint32_t lo = 0, hi = nzt;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (L.nzp[mid] <= i) lo = mid; else hi = mid;
+            }
+            uint32_t m = L.nzm[lo];
+            for (uint32_t r = i - L.nzp[lo]; r; r--) m &= m - 1;   // drop the lower members
+            const uint32_t slot = 32 * L.nzw[lo] + (__ffs(m) - 1);
+            uint32_t pj = tab.find(slot);
+            L.pe[i] = tab.entry_from(pj, slot);
+            L.pk[i] = p.keys[b + slot];
+        }
+    }
+    if (!prune) {   // rerank the whole pool
+        __syncthreads();
+        rerank_list(p, pe, c, s_qv, qn, s_ex);
         __syncthreads();
         smallest_k(s_ex, c, p.k, out);
         return;
     }
     // 1. A = the nA smallest keys; thread t owns pool positions [tE, tE+E)
     const uint32_t E = (c + 255) / 256, i0 = tid * E, i1 = min(i0 + E, c);
+    __syncthreads();   // pool complete
     for (uint32_t i = tid; i < 2 * 256; i += 256) (&L.hist[0][0])[i] = 0u;
     float exq = 0.0f, xnq = 0.0f;
     for (uint32_t j = tid; j < p.np_q[q]; j += 256) {
