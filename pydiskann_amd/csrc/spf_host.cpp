@@ -89,7 +89,7 @@ struct Workspace {
     DBuf<float> rq_consts;
     DBuf<uint32_t> pair_src;
     DBuf<float> eq16;
-    DBuf<uint32_t> rr_stats, sel_bits;
+    DBuf<uint32_t> rr_stats, sel_bits, a_bits;
     uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
     hipEvent_t ev[SPF_N_STAGES + 1] = {};
 
@@ -116,6 +116,7 @@ struct Workspace {
         eq16.release();
         rr_stats.release();
         sel_bits.release();
+        a_bits.release();
         qn16.release();
         DBuf<float> *fs[] = {&qin, &qv_probe, &qv_scan, &qnorm, &out_d};
         for (auto *b : fs) b->release();
@@ -583,6 +584,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     const uint32_t dpad32 = round_up(ix->dim, 32);
     HIPCHK(w->cand16.ensure(std::max<uint64_t>(total, 1)));
     HIPCHK(w->sel_bits.ensure(std::max<uint64_t>(total / 32, 1)));
+    HIPCHK(w->a_bits.ensure(std::max<uint64_t>(total / 32, 1)));
     HIPCHK(w->pool_want.ensure(nq));
     HIPCHK(w->fin.ensure((size_t)nq * k));
     HIPCHK(w->fin_cnt.ensure(nq));
@@ -633,6 +635,9 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     pp.post_off = ix->d_post_off;
     pp.pool_want = w->pool_want.p;
     pp.sel_bits = w->sel_bits.p;
+    pp.a_bits = w->a_bits.p;
+    // phase-A size of the rerank: k plus a margin for the 16-bit key resolution
+    pp.ca = std::min(want, std::max(k, round_up(k + k / 8 + 8, 32)));
     HIPCHK(launch_pool_select(pp, nq, s));
     // nearest-posting query order, then the (pruned) exact rerank of each pool
     HIPCHK(mark(SPF_STAGE_RERANK));
@@ -643,14 +648,14 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     rp.seg_off = w->cand_off.p;
     rp.seg_cnt = w->cand_cnt.p;
     rp.sel_bits = w->sel_bits.p;
+    rp.a_bits = w->a_bits.p;
     rp.pool_want = w->pool_want.p;
     rp.qp_slot = w->qp_slot.p;
     rp.post_off = ix->d_post_off;
     rp.nq = nq;
     rp.want = want;
     rp.k = k;
-    // phase-A size: k plus a margin for the 16-bit key resolution
-    rp.ca = std::min(want, std::max(k, round_up(k + k / 8 + 8, 32)));
+    rp.ca = pp.ca;
     rp.metric = ix->metric;
     rp.dpad = ix->dpad;
     rp.dim = ix->dim;
@@ -672,8 +677,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     rp.out = w->fin.p;
     rp.out_cnt = w->fin_cnt.p;
     if (stats) {
-        HIPCHK(w->rr_stats.ensure(2));
-        HIPCHK(hipMemsetAsync(w->rr_stats.p, 0, 2 * sizeof(uint32_t), s));
+        HIPCHK(w->rr_stats.ensure(nq));
         rp.stats = w->rr_stats.p;
     }
     HIPCHK(launch_rerank_pruned(rp, s));
@@ -1049,11 +1053,19 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                      : mixed ? 2ull * ix->dim : (uint64_t)ix->code_bytes);
         g_prof.probe_pairs = (uint64_t)nq * P;
         g_prof.scan_items = n_items;
-        if (mixed) {
-            uint32_t st[2] = {0, 0};
-            HIPCHK(hipMemcpy(st, w->rr_stats.p, sizeof st, hipMemcpyDeviceToHost));
-            g_prof.rerank_rows = st[0];
-            g_prof.pool_members = st[1];
+        if (mixed) {   // per-query counts, summed here
+            std::vector<uint32_t> rows(nq), pool(nq);
+            HIPCHK(hipMemcpy(rows.data(), w->rr_stats.p, nq * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(pool.data(), w->pool_want.p, nq * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost));
+            uint64_t sr = 0, sp = 0;
+            for (uint32_t i = 0; i < nq; i++) {
+                sr += rows[i];
+                sp += std::min(pool[i], want);
+            }
+            g_prof.rerank_rows = (uint32_t)std::min<uint64_t>(sr, 0xffffffffu);
+            g_prof.pool_members = (uint32_t)std::min<uint64_t>(sp, 0xffffffffu);
         }
         g_prof_valid = true;
     }

@@ -134,6 +134,7 @@ struct RerankPrunedParams {
     const uint64_t *seg_off;    // [nq] first slot (multiple of 32)
     const uint32_t *seg_cnt;    // [nq]
     const uint32_t *sel_bits;   // pool membership, one mask per 32-slot word
+    const uint32_t *a_bits;     // members of phase A (the `ca` smallest keys)
     const uint32_t *pool_want;  // [nq] pool size
     const uint64_t *qp_slot;    // [nq][npr] absolute first slot of each probed posting
     const uint32_t *post_off;
@@ -152,7 +153,7 @@ struct RerankPrunedParams {
     const float *post_xn;       // per posting max |x16|^2
     uint64_t *out;              // [nq][k] (dist key << 32 | rank)
     uint32_t *out_cnt;
-    uint32_t *stats;            // optional [2]: rows gathered, pool members
+    uint32_t *stats;            // optional [nq]: raw rows gathered per query
 };
 
 // RaBitQ: per-pair query prep, exact search, mixed estimator scan.
@@ -209,6 +210,8 @@ struct PoolParams {
     uint16_t *pool_key;         // [nq][want] keys of the members (null = not written)
     uint32_t *pool_want;        // [nq]
     uint32_t *sel_bits;         // non-null: write per-word selection masks instead
+    uint32_t *a_bits;           // with sel_bits: masks of the `ca` smallest pool members
+    uint32_t ca;
 };
 
 // Exact f32 rerank of each query's pool and the final top-k, one workgroup per

@@ -417,6 +417,34 @@ __device__ __forceinline__ void word_masks(const uint4 (&r)[4], uint32_t w, uint
     eq = e;
 }
 
+// the same for two thresholds T and TA
+__device__ __forceinline__ void word_masks2(const uint4 (&r)[4], uint32_t w, uint32_t n,
+                                            uint32_t T, uint32_t TA, uint32_t &live,
+                                            uint32_t &lt, uint32_t &eq, uint32_t &ltA,
+                                            uint32_t &eqA) {
+    uint32_t lv = 0, l = 0, e = 0, la = 0, ea = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const uint32_t j = 8 * c + t;
+            const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+            const uint32_t bit = (uint32_t)(kk != 0xffffu && 32 * w + j < n) << j;
+            lv |= bit;
+            l |= kk < T ? bit : 0u;
+            e |= kk == T ? bit : 0u;
+            la |= kk < TA ? bit : 0u;
+            ea |= kk == TA ? bit : 0u;
+        }
+    }
+    live = lv;
+    lt = l;
+    eq = e;
+    ltA = la;
+    eqA = ea;
+}
+
 // key j (runtime) of a 32-key word
 __device__ __forceinline__ uint32_t word_key(const uint4 (&r)[4], uint32_t j) {
     const uint32_t c = j >> 3, t = (j >> 1) & 3;
@@ -441,8 +469,9 @@ __device__ __forceinline__ uint32_t lowest_bits(uint32_t m, uint32_t c) {
 
 // One wavefront per query: the `want` smallest 16-bit keys of its segment,
 // exactly-tied keys at the threshold taken in slot order, written as one
-// selection mask per 32-slot word (sel_bits, at the segment's word offset),
-// or else as entries (map) or slots, with their keys (pool_key, optional).
+// selection mask per 32-slot word (sel_bits, at the segment's word offset;
+// a_bits marks the `ca` smallest of them the same way), or else as entries
+// (map) or slots, with their keys (pool_key, optional).  hist: 768 words.
 // Lane l owns words
 // l, l+64, ... (word w = slots 32w..32w+31) and re-reads them from L2 in each
 // of the four passes; a word's four loads go out together.
@@ -450,18 +479,15 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
                                           uint32_t want, uint32_t *pool_entry,
                                           uint16_t *pool_key, uint32_t *pool_want, bool map,
                                           const ProbeTab &tab, uint32_t *hist,
-                                          uint32_t *sel_bits) {
+                                          uint32_t *sel_bits, uint32_t *a_bits, uint32_t ca) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nwords = (n + 31) / 32;
-    // the lane's words two at a time: both words' loads go out before use
+    // the lane's words, one word's four loads together
     auto sweep = [&](auto &&body) {
-        for (uint32_t w = lane; w < nwords; w += 128) {
-            uint4 r0[4], r1[4];
-            const uint32_t w1 = w + 64;
+        for (uint32_t w = lane; w < nwords; w += 64) {
+            uint4 r0[4];
             load_word_raw(kp, w, r0);
-            if (w1 < nwords) load_word_raw(kp, w1, r1);
             body(r0, w);
-            if (w1 < nwords) body(r1, w1);
         }
     };
     uint32_t inval = 0, kmn = 0xffffu, kmx = 0u;
@@ -491,12 +517,16 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
     const uint32_t W = min(want, nvalid);
     if (lane == 0) pool_want[q] = W;
     const bool all = W == nvalid;
-    uint32_t T = 0xffffu, need = 0;
-    if (!all) {
+    // phase-A rank (a_bits): the `ca` smallest of the pool, same tie rule
+    const uint32_t nA = a_bits ? min(ca, W) : W;
+    const bool allA = nA == nvalid;
+    uint32_t T = 0xffffu, need = 0, TA = 0xffffu, needA = 0;
+    if (!allA) {   // (all implies allA)
+        uint32_t *h1 = hist, *h2 = hist + 256, *h3 = hist + 512;
         const uint32_t span = kmx - kmn;
         const uint32_t sh = span < 256 ? 0 : 32 - __clz(span) - 8;
 #pragma unroll
-        for (int i = 0; i < 4; i++) hist[4 * lane + i] = 0;
+        for (int i = 0; i < 12; i++) hist[12 * lane + i] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         sweep([&](const uint4 (&r)[4], uint32_t w) {
@@ -506,20 +536,17 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
 #pragma unroll
                 for (int t = 0; t < 8; t++) {
                     const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
-                    if (kk != 0xffffu && 32 * w + 8 * c + t < n) atomicAdd(&hist[(kk - kmn) >> sh], 1u);
+                    if (kk != 0xffffu && 32 * w + 8 * c + t < n) atomicAdd(&h1[(kk - kmn) >> sh], 1u);
                 }
             }
         });
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        uint32_t b1, below1, b2 = 0, below2 = 0;
-        wave_find_bin(hist, W, b1, below1);
+        uint32_t bW = 0, belowW = 0, bA = 0, belowA = 0;
+        if (!all) wave_find_bin(h1, W, bW, belowW);
+        wave_find_bin(h1, nA, bA, belowA);
+        uint32_t b2W = 0, below2W = 0, b2A = 0, below2A = 0;
         if (sh > 0) {
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int i = 0; i < 4; i++) hist[4 * lane + i] = 0;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
             const uint32_t mask = (1u << sh) - 1u;
             sweep([&](const uint4 (&r)[4], uint32_t w) {
 #pragma unroll
@@ -528,20 +555,27 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
 #pragma unroll
                     for (int t = 0; t < 8; t++) {
                         const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
-                        if (kk != 0xffffu && 32 * w + 8 * c + t < n && ((kk - kmn) >> sh) == b1)
-                            atomicAdd(&hist[(kk - kmn) & mask], 1u);
+                        if (kk == 0xffffu || 32 * w + 8 * c + t >= n) continue;
+                        const uint32_t bin = (kk - kmn) >> sh;
+                        if (!all && bin == bW) atomicAdd(&h2[(kk - kmn) & mask], 1u);
+                        if (bin == bA) atomicAdd(&h3[(kk - kmn) & mask], 1u);
                     }
                 }
             });
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            wave_find_bin(hist, W - below1, b2, below2);
+            if (!all) wave_find_bin(h2, W - belowW, b2W, below2W);
+            wave_find_bin(h3, nA - belowA, b2A, below2A);
         }
-        T = kmn + (b1 << sh) + b2;
-        need = W - below1 - below2;   // keys equal to T to take, in slot order
+        if (!all) {
+            T = kmn + (bW << sh) + b2W;
+            need = W - belowW - below2W;   // keys equal to T to take, in slot order
+        }
+        TA = kmn + (bA << sh) + b2A;
+        needA = nA - belowA - below2A;
     }
     // collect: round m = words 64m .. 64m+63 in lane order (slot order)
-    uint32_t tie_run = 0, mark_run = 0;
+    uint32_t tie_run = 0, tieA_run = 0, mark_run = 0;
     uint4 rn[4];   // next round's word, loaded ahead
     if (lane < nwords) load_word_raw(kp, lane, rn);
     for (uint32_t m = 0; 64 * m < nwords; m++) {
@@ -550,14 +584,21 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
 #pragma unroll
         for (int c = 0; c < 4; c++) r[c] = rn[c];
         if (w + 64 < nwords) load_word_raw(kp, w + 64, rn);
-        uint32_t live = 0, lt = 0, eq = 0;
-        if (w < nwords) word_masks(r, w, n, T, live, lt, eq);
+        uint32_t live = 0, lt = 0, eq = 0, ltA = 0, eqA = 0;
+        if (w < nwords) word_masks2(r, w, n, T, TA, live, lt, eq, ltA, eqA);
+        // tie counts of this round (each at most 2048) packed in one scan
         uint32_t tt;
-        const uint32_t tr = tie_run + wave_excl(all ? 0u : __popc(eq), &tt);
-        tie_run += tt;
+        const uint32_t pk = wave_excl((all ? 0u : __popc(eq)) | ((allA ? 0u : __popc(eqA)) << 16), &tt);
+        const uint32_t tr = tie_run + (pk & 0xffffu), trA = tieA_run + (pk >> 16);
+        tie_run += tt & 0xffffu;
+        tieA_run += tt >> 16;
         const uint32_t sel = all ? live : lt | (tr < need ? lowest_bits(eq, need - tr) : 0u);
         if (sel_bits) {   // one coalesced mask store per word
-            if (w < nwords) sel_bits[w] = sel;
+            if (w < nwords) {
+                sel_bits[w] = sel;
+                if (a_bits)
+                    a_bits[w] = allA ? live : ltA | (trA < needA ? lowest_bits(eqA, needA - trA) : 0u);
+            }
             continue;
         }
         uint32_t mt;
@@ -580,7 +621,7 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
 constexpr uint32_t kWaveTab = 256;   // probe tables held in LDS up to this n_probe
 
 __global__ __launch_bounds__(256) void k_pool_select(const PoolParams p, uint32_t nq) {
-    __shared__ uint32_t hist[4][256];
+    __shared__ uint32_t hist[4][768];
     __shared__ uint32_t s_pbase[4][kWaveTab], s_ebase[4][kWaveTab];
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t q = blockIdx.x * 4 + wv;
@@ -605,7 +646,8 @@ __global__ __launch_bounds__(256) void k_pool_select(const PoolParams p, uint32_
         tab.ebase = s_ebase[wv];
     }
     pool_wave(p.keys + b, q, n, p.want, p.pool_entry, p.pool_key, p.pool_want, p.map, tab,
-              hist[wv], p.sel_bits ? p.sel_bits + b / 32 : nullptr);
+              hist[wv], p.sel_bits ? p.sel_bits + b / 32 : nullptr,
+              p.a_bits ? p.a_bits + b / 32 : nullptr, p.ca);
 }
 
 // ---------------------------------------------------------------- query order
@@ -779,7 +821,7 @@ __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
 // Dropped members cannot enter the top k, so the result is that of reranking
 // the whole pool.  Without prune every member is reranked.
 constexpr uint32_t kPrCount = 512;    // lists up to this size are ranked by counting
-constexpr int kPrH = 4, kPrUC = 2;    // rows per quad, row units per lane in flight
+constexpr int kPrH = 2, kPrUC = 8;    // rows per quad, row units per lane in flight
 
 __device__ __forceinline__ uint32_t blk_excl(uint32_t v, uint32_t *s_w) {
     uint32_t tot;
@@ -841,30 +883,70 @@ __device__ __forceinline__ void smallest_k(uint64_t *keys, uint32_t cnt, uint32_
 
 template <uint32_t CAP>
 struct RerankLds {
-    union {
-        struct {
-            uint64_t ex[CAP];    // exact keys (A, then survivors)
-            uint32_t ent[CAP];   // A at [0, nA), others from nA; later survivors
-            uint16_t bk[CAP];    // keys of the others
-        };
-        struct {                 // step 0: non-empty mask words
-            uint32_t nzw[CAP], nzm[CAP], nzp[CAP];   // index, mask, first member
-        };
-        uint32_t hist[2][256];   // step 1: radix histograms
-    };
-    uint32_t pe[CAP];            // pool entries, in slot order
-    uint16_t pk[CAP];            // pool keys
+    uint64_t ex[CAP];            // exact keys: A, then survivors
+    uint32_t ent[CAP];           // entries: A, then survivors
+    uint32_t nzw[CAP], nzm[CAP], nzp[CAP];   // non-empty mask words: index, mask, first member
     uint32_t pbase[kWaveTab], ebase[kWaveTab];
     uint32_t w[8];
     uint32_t m[16];
     uint64_t kth;
 };
 
+// Members of the masks mask_of(w) over the query's words, in slot order, as
+// entries out[0..): non-empty words are listed with the position of their
+// first member, then each thread takes members i = tid, tid + 256, ... (word
+// by binary search, bit by rank), so the work is even however the members
+// cluster.  Returns the member count.  Thread t owns words [w0, w1).
+template <uint32_t CAP, class MaskOf>
+__device__ __forceinline__ uint32_t rebuild_members(RerankLds<CAP> &L, uint32_t w0, uint32_t w1,
+                                                    MaskOf &&mask_of, const ProbeTab &tab,
+                                                    uint32_t *out, uint32_t cap) {
+    const uint32_t tid = threadIdx.x;
+    uint32_t nz = 0, cnt = 0;
+    for (uint32_t w = w0; w < w1; w++) {
+        const uint32_t m = mask_of(w);
+        nz += m != 0u;
+        cnt += __popc(m);
+    }
+    uint32_t zpos = blk_excl(nz, L.w);
+    uint32_t ppos = blk_excl(cnt, L.w);
+    for (uint32_t w = w0; w < w1 && nz; w++) {
+        const uint32_t m = mask_of(w);
+        if (m && zpos < CAP) {
+            L.nzw[zpos] = w;
+            L.nzm[zpos] = m;
+            L.nzp[zpos] = ppos;
+            zpos++;
+            ppos += __popc(m);
+        }
+    }
+    if (tid == 255) {   // the last thread's prefix + count = totals
+        L.m[14] = zpos;
+        L.m[15] = ppos;
+    }
+    __syncthreads();
+    const uint32_t nzt = L.m[14], total = min(L.m[15], cap);
+    for (uint32_t i = tid; i < total; i += 256) {
+        uint32_t lo = 0, hi = nzt;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (L.nzp[mid] <= i) lo = mid; else hi = mid;
+        }
+        uint32_t m = L.nzm[lo];
+        for (uint32_t r = i - L.nzp[lo]; r; r--) m &= m - 1;   // drop the lower members
+        const uint32_t slot = 32 * L.nzw[lo] + (__ffs(m) - 1);
+        uint32_t pj = tab.find(slot);
+        out[i] = tab.entry_from(pj, slot);
+    }
+    __syncthreads();
+    return total;
+}
+
 template <uint32_t CAP>
 __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams p) {
     __shared__ RerankLds<CAP> L;
     extern __shared__ __align__(16) float s_qv[];   // dpad floats
-    enum { MN, MX, EXQ, XNQ, BA, BEA, CB2, THR, NZ };
+    enum { EXQ, XNQ, THR };
     const uint32_t x = blockIdx.x % kXcds, jx = blockIdx.x / kXcds;
     const uint32_t per = (p.nq + kXcds - 1) / kXcds;
     const uint32_t idx = x * per + jx;
@@ -872,194 +954,105 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
     const uint32_t q = p.qperm ? p.qperm[idx] : idx;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     const uint32_t c = min(p.pool_want[q], p.want);
-    const uint32_t *pe = L.pe;
-    const uint16_t *pk = L.pk;
-    uint32_t *s_ent = L.ent, *s_w = L.w, *s_m = L.m;
-    uint64_t *s_ex = L.ex;
-    uint16_t *s_bk = L.bk;
     uint64_t *out = p.out + (size_t)q * p.k;
     const uint32_t nA = min(p.ca, c);
     const bool prune = p.prune && nA < c && nA >= p.k;
+    const uint64_t b = p.seg_off[q];
+    const uint32_t n = p.seg_cnt[q];
+    const uint32_t nwords = (n + 31) / 32, Mw = (nwords + 255) / 256;
+    const uint32_t w0 = tid * Mw, w1 = min(w0 + Mw, nwords);
+    const uint32_t *sel = p.sel_bits + b / 32, *abits = p.a_bits + b / 32;
+    // the thread's mask words, loaded with the table below (up to 4 in registers)
+    const bool cached = Mw <= 4;
+    uint32_t sm[4] = {0u, 0u, 0u, 0u}, am[4] = {0u, 0u, 0u, 0u};
+    if (cached)
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++)
+            if (w0 + i < w1) {
+                sm[i] = sel[w0 + i];
+                am[i] = prune ? abits[w0 + i] : 0u;
+            }
+    auto pick = [&](const uint32_t (&v)[4], uint32_t w) {
+        const uint32_t i = w - w0;
+        return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
+    };
+    auto sel_w = [&](uint32_t w) { return cached ? pick(sm, w) : sel[w]; };
+    auto a_w = [&](uint32_t w) { return cached ? pick(am, w) : abits[w]; };
     if (tid == 0) {
-        s_m[MN] = 0xffffu; s_m[MX] = 0u; s_m[EXQ] = 0u; s_m[XNQ] = 0u;
-        s_m[THR] = 0xffffu;
+        L.m[EXQ] = 0u;
+        L.m[XNQ] = 0u;
+        L.m[THR] = 0xffffu;
+        L.kth = kNoKey;   // no bound unless phase A yields its k-th key
         p.out_cnt[q] = min(p.k, c);
     }
     for (uint32_t i = tid; i < p.dpad; i += 256) s_qv[i] = p.qv[(size_t)q * p.dpad + i];
     const float qn = p.qnorm ? p.qnorm[q] : 0.0f;
-    // 0. the pool from the selection masks, in slot order: thread t owns a
-    // contiguous word range; slots map to entries through the probe table
-    {
-        const uint64_t b = p.seg_off[q];
-        const uint32_t n = p.seg_cnt[q];
-        const uint32_t nwords = (n + 31) / 32, Mw = (nwords + 255) / 256;
-        const uint32_t w0 = tid * Mw, w1 = min(w0 + Mw, nwords);
-        const uint32_t *bits = p.sel_bits + b / 32;
-        ProbeTab tab;
-        tab.np = p.np_q[q];
-        tab.qp_row = p.qp_slot + (size_t)q * p.npr;
-        tab.probe_row = p.probe_post + (size_t)q * p.npr;
-        tab.post_off = p.post_off;
-        tab.seg = b;
-        tab.pbase = tab.ebase = nullptr;
+    // probe table (segment-relative first slot, first entry of each probe)
+    ProbeTab tab;
+    tab.np = p.np_q[q];
+    tab.qp_row = p.qp_slot + (size_t)q * p.npr;
+    tab.probe_row = p.probe_post + (size_t)q * p.npr;
+    tab.post_off = p.post_off;
+    tab.seg = b;
+    tab.pbase = tab.ebase = nullptr;
+    float exq = 0.0f, xnq = 0.0f;
+    for (uint32_t j = tid; j < tab.np; j += 256) {
+        const uint32_t post = tab.probe_row[j];
         if (tab.np <= kWaveTab) {
-            for (uint32_t j = tid; j < tab.np; j += 256) {
-                L.pbase[j] = (uint32_t)(tab.qp_row[j] - b);
-                L.ebase[j] = p.post_off[tab.probe_row[j]];
-            }
-            tab.pbase = L.pbase;
-            tab.ebase = L.ebase;
+            L.pbase[j] = (uint32_t)(tab.qp_row[j] - b);
+            L.ebase[j] = p.post_off[post];
         }
-        // non-empty words in slot order with the pool position of their first member
-        uint32_t nz = 0, cnt = 0;
-        for (uint32_t w = w0; w < w1; w++) {
-            const uint32_t m = bits[w];
-            nz += m != 0u;
-            cnt += __popc(m);
-        }
-        uint32_t zpos = blk_excl(nz, s_w);   // its barriers also publish the table
-        uint32_t ppos = blk_excl(cnt, s_w);
-        for (uint32_t w = w0; w < w1; w++) {
-            const uint32_t m = bits[w];
-            if (m && zpos < CAP) {
-                L.nzw[zpos] = w;
-                L.nzm[zpos] = m;
-                L.nzp[zpos] = ppos;
-                zpos++;
-                ppos += __popc(m);
-            }
-        }
-        if (tid == 255) s_m[NZ] = zpos;   // the last thread's prefix + count = total
-        __syncthreads();
-        const uint32_t nzt = s_m[NZ];
-        // pool member i: its word by binary search, then its bit in the word
-        for (uint32_t i = tid; i < c; i += 256) {
-            uint32_t lo = 0, hi = nzt;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (L.nzp[mid] <= i) lo = mid; else hi = mid;
-            }
-            uint32_t m = L.nzm[lo];
-            for (uint32_t r = i - L.nzp[lo]; r; r--) m &= m - 1;   // drop the lower members
-            const uint32_t slot = 32 * L.nzw[lo] + (__ffs(m) - 1);
-            uint32_t pj = tab.find(slot);
-            L.pe[i] = tab.entry_from(pj, slot);
-            L.pk[i] = p.keys[b + slot];
+        if (prune) {
+            exq = fmaxf(exq, p.post_ex[post]);
+            xnq = fmaxf(xnq, p.post_xn[post]);
         }
     }
-    if (!prune) {   // rerank the whole pool
+    if (tab.np <= kWaveTab) {
+        tab.pbase = L.pbase;
+        tab.ebase = L.ebase;
+    }
+    if (!prune) {   // rerank the whole pool (rebuild_members' barriers publish the table)
+        const uint32_t cnt = rebuild_members(L, w0, w1, [&](uint32_t w) { return sel_w(w); }, tab,
+                                             L.ent, c);
+        rerank_list(p, L.ent, cnt, s_qv, qn, L.ex);
         __syncthreads();
-        rerank_list(p, pe, c, s_qv, qn, s_ex);
-        __syncthreads();
-        smallest_k(s_ex, c, p.k, out);
+        smallest_k(L.ex, cnt, p.k, out);
+        if (p.stats && tid == 0) p.stats[q] = cnt;
         return;
     }
-    // 1. A = the nA smallest keys; thread t owns pool positions [tE, tE+E)
-    const uint32_t E = (c + 255) / 256, i0 = tid * E, i1 = min(i0 + E, c);
-    __syncthreads();   // pool complete
-    for (uint32_t i = tid; i < 2 * 256; i += 256) (&L.hist[0][0])[i] = 0u;
-    float exq = 0.0f, xnq = 0.0f;
-    for (uint32_t j = tid; j < p.np_q[q]; j += 256) {
-        const uint32_t post = p.probe_post[(size_t)q * p.npr + j];
-        exq = fmaxf(exq, p.post_ex[post]);
-        xnq = fmaxf(xnq, p.post_xn[post]);
-    }
-    uint32_t kmn = 0xffffu, kmx = 0u;
-    for (uint32_t i = i0; i < i1; i++) {
-        const uint32_t kk = pk[i];
-        kmn = min(kmn, kk);
-        kmx = max(kmx, kk);
-    }
     for (int off = 32; off > 0; off >>= 1) {
-        kmn = min(kmn, (uint32_t)__shfl_xor(kmn, off, 64));
-        kmx = max(kmx, (uint32_t)__shfl_xor(kmx, off, 64));
         exq = fmaxf(exq, __shfl_xor(exq, off, 64));
         xnq = fmaxf(xnq, __shfl_xor(xnq, off, 64));
     }
-    __syncthreads();   // s_m initialised, histograms zeroed
-    if (lane == 0) {
-        atomicMin(&s_m[MN], kmn);
-        atomicMax(&s_m[MX], kmx);
-        atomicMax(&s_m[EXQ], __float_as_uint(exq));   // non-negative floats order as bits
-        atomicMax(&s_m[XNQ], __float_as_uint(xnq));
+    __syncthreads();   // L.m initialised
+    if (lane == 0) {   // non-negative floats order as their bits
+        atomicMax(&L.m[EXQ], __float_as_uint(exq));
+        atomicMax(&L.m[XNQ], __float_as_uint(xnq));
     }
+    // 1. phase A (the `ca` smallest keys, marked by pool selection): rerank,
+    // then its k-th exact key (keys are distinct: the rank is in the low bits)
+    const uint32_t na = rebuild_members(L, w0, w1, [&](uint32_t w) { return a_w(w); }, tab,
+                                        L.ent, nA);
+    rerank_list(p, L.ent, na, s_qv, qn, L.ex);
     __syncthreads();
-    kmn = s_m[MN];
-    kmx = s_m[MX];
-    const uint32_t span = kmx - kmn;
-    const uint32_t sh = span < 256 ? 0 : 32 - __clz(span) - 8;
-    for (uint32_t i = i0; i < i1; i++) atomicAdd(&L.hist[0][(pk[i] - kmn) >> sh], 1u);
-    __syncthreads();
-    {
-        const uint32_t h = L.hist[0][tid];
-        find_rank_bin(h, blk_excl(h, s_w), nA, &s_m[BA]);
-    }
-    __syncthreads();
-    const uint32_t b1 = s_m[BA], below1 = s_m[BEA];
-    uint32_t b2 = 0, below2 = 0;
-    if (sh > 0) {
-        const uint32_t mask = (1u << sh) - 1u;
-        for (uint32_t i = i0; i < i1; i++) {
-            const uint32_t kk = pk[i];
-            if (((kk - kmn) >> sh) == b1) atomicAdd(&L.hist[1][(kk - kmn) & mask], 1u);
-        }
-        __syncthreads();
-        const uint32_t h = L.hist[1][tid];
-        find_rank_bin(h, blk_excl(h, s_w), nA - below1, &s_m[BA]);
-        __syncthreads();
-        b2 = s_m[BA];
-        below2 = s_m[BEA];
-    }
-    const uint32_t TA = kmn + (b1 << sh) + b2, needA = nA - below1 - below2;
-    // split in pool order: per-thread counts, block scans, then the writes
-    uint32_t lt = 0, eq = 0;
-    for (uint32_t i = i0; i < i1; i++) {
-        const uint32_t kk = pk[i];
-        lt += kk < TA;
-        eq += kk == TA;
-    }
-    uint32_t ra = blk_excl(eq, s_w);
-    const uint32_t cA = lt + (ra >= needA ? 0u : min(eq, needA - ra));
-    uint32_t pA = blk_excl(cA, s_w), pB = blk_excl((i1 > i0 ? i1 - i0 : 0u) - cA, s_w);
-    for (uint32_t i = i0; i < i1; i++) {
-        const uint32_t kk = pk[i];
-        bool inA = kk < TA;
-        if (kk == TA) {
-            inA = ra < needA;
-            ra++;
-        }
-        if (inA) {
-            s_ent[pA++] = pe[i];
-        } else {
-            s_ent[nA + pB] = pe[i];
-            s_bk[nA + pB] = (uint16_t)kk;
-            pB++;
-        }
-    }
-    __syncthreads();
-    const uint32_t nB = c - nA;
-
-    // 2. rerank A; its k-th exact key (keys are distinct: the rank is in the low bits)
-    rerank_list(p, s_ent, nA, s_qv, qn, s_ex);
-    __syncthreads();
-    if (nA <= kPrCount) {
-        for (uint32_t i = tid; i < nA; i += 256) {
-            const uint64_t v = s_ex[i];
+    if (na <= kPrCount) {
+        for (uint32_t i = tid; i < na; i += 256) {
+            const uint64_t v = L.ex[i];
             uint32_t cnt = 0;
-            for (uint32_t j = 0; j < nA; j++) cnt += s_ex[j] < v;
+            for (uint32_t j = 0; j < na; j++) cnt += L.ex[j] < v;
             if (cnt == p.k - 1) L.kth = v;
         }
     } else {
         uint32_t mA = 2;
-        while (mA < nA) mA <<= 1;
-        for (uint32_t i = nA + tid; i < mA; i += 256) s_ex[i] = kNoKey;
+        while (mA < na) mA <<= 1;
+        for (uint32_t i = na + tid; i < mA; i += 256) L.ex[i] = kNoKey;
         __syncthreads();
-        bitonic_lds(s_ex, mA);
-        if (tid == 0) L.kth = s_ex[p.k - 1];
+        bitonic_lds(L.ex, mA);
+        if (tid == 0) L.kth = L.ex[p.k - 1];
     }
     __syncthreads();
 
-    // 3. bound: a key above thr proves an exact distance above A's k-th
+    // 2. bound: a key above thr proves an exact distance above A's k-th
     if (tid == 0) {
         uint32_t thr = 0xffffu;
         const float kth = key_float_m((uint32_t)(L.kth >> 32));
@@ -1070,8 +1063,8 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
             // the f32 exact sum is at most (dim + 8) 2^-24 relative below the true value
             const double s = sqrt(ksq * (1.0 + (p.dim + 16) * u));
             const double eqd = (double)p.eq16[q] * (1.0 + 1e-5) + 1e-30;
-            const double r = s + eqd + (double)__uint_as_float(s_m[EXQ]) * (1.0 + 1e-6);
-            const double qn16 = p.qn16[q], xn16 = __uint_as_float(s_m[XNQ]);
+            const double r = s + eqd + (double)__uint_as_float(L.m[EXQ]) * (1.0 + 1e-6);
+            const double qn16 = p.qn16[q], xn16 = __uint_as_float(L.m[XNQ]);
             // f32 evaluation of |x16|^2 + |q16|^2 - 2 q16.x16 (exact products)
             const double e1 = (p.kdim + 16) * u * (qn16 + xn16 + 2.0 * sqrt(qn16 * xn16)) * 2.0;
             const double v = (r * r + e1) * (1.0 + 1e-9);
@@ -1081,35 +1074,25 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
                 thr = min(0x8000u | (__float_as_uint(vf) >> 16), 0xfffeu);
             }
         }
-        s_m[THR] = thr;
+        L.m[THR] = thr;
     }
     __syncthreads();
-    const uint32_t thr = s_m[THR];
-    // survivors, compacted to the front of ent (A's entries are consumed):
-    // read into registers, written after the scan's barriers
-    constexpr uint32_t kPer = CAP / 256;
-    uint32_t be[kPer], nb = 0;
-#pragma unroll
-    for (uint32_t r = 0; r < kPer; r++) {
-        const uint32_t i = tid + 256 * r;
-        const bool ok = i < nB && s_bk[nA + i] <= thr;
-        be[r] = ok ? s_ent[nA + i] : 0xffffffffu;
-        nb += ok;
-    }
-    uint32_t pos = blk_excl(nb, s_w);
-#pragma unroll
-    for (uint32_t r = 0; r < kPer; r++)
-        if (be[r] != 0xffffffffu) s_ent[pos++] = be[r];
-    if (tid == 255) s_m[CB2] = pos;   // the last thread's prefix + count = total
+    const uint32_t thr = L.m[THR];
+    // 3. survivors: pool members outside A whose key is at most thr
+    const uint16_t *kp = p.keys + b;
+    const uint32_t nb = rebuild_members(L, w0, w1, [&](uint32_t w) {
+        const uint32_t rest = sel_w(w) & ~a_w(w);
+        if (!rest) return 0u;
+        uint4 r[4];
+        load_word_raw(kp, w, r);
+        uint32_t live, lt, eq;
+        word_masks(r, w, n, thr, live, lt, eq);
+        return rest & (lt | eq);
+    }, tab, L.ent + na, c - na);
+    rerank_list(p, L.ent + na, nb, s_qv, qn, L.ex + na);
     __syncthreads();
-    const uint32_t nB2 = s_m[CB2];
-    rerank_list(p, s_ent, nB2, s_qv, qn, s_ex + nA);
-    __syncthreads();
-    smallest_k(s_ex, nA + nB2, p.k, out);
-    if (p.stats && tid == 0) {
-        atomicAdd(&p.stats[0], nA + nB2);
-        atomicAdd(&p.stats[1], c);
-    }
+    smallest_k(L.ex, na + nb, p.k, out);
+    if (p.stats && tid == 0) p.stats[q] = na + nb;
 }
 
 // ---------------------------------------------------------------- launchers
