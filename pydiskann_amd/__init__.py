@@ -16,6 +16,7 @@ import numpy as np
 
 from .index import IndexArrays, build_index, brute_force_f64, METRICS, QUANTS  # noqa: F401
 from ._lib import DeviceIndex, PREC_EXACT, PREC_MIXED, SpfError, lib  # noqa: F401
+from .refformat import read_index, write_index  # noqa: F401
 
 __all__ = ["SPFresh", "IndexArrays", "build_index", "DeviceIndex", "PREC_EXACT", "PREC_MIXED"]
 
@@ -45,24 +46,18 @@ class SPFresh:
         return SPFresh(arrays, device=device)
 
     def save(self) -> None:
-        a = self.arrays
-        np.savez(self.path + ".spfgpu.npz", dim=a.dim, metric=a.metric, quant=a.quant,
-                 block_ids=a.block_ids, centroids=a.centroids, post_off=a.post_off, ids=a.ids,
-                 codes=a.codes, raw=a.raw, deleted=a.deleted, int8=np.array(
-                     [a.int8_scale, a.int8_offset], np.float32), rq_padded=a.rq_padded,
-                 rq_signs=a.rq_signs if a.rq_signs is not None else np.zeros(0, np.float32),
-                 probe_ratio=np.float32(a.probe_ratio), rerank_size=a.rerank_size)
+        """Write {path}.spf / .postings / .vectors in the reference's layout
+        (SPFresh::save, src/spfresh.rs:852-858; the centroid graph file is not
+        written: this path probes centroids exactly)."""
+        if not self.path:
+            raise RuntimeError("index has no path")
+        write_index(self.arrays, self.path)
 
     @staticmethod
     def open(path: str, metric: str = "l2") -> "SPFresh":
-        z = np.load(path + ".spfgpu.npz", allow_pickle=False)
-        arr = IndexArrays(dim=int(z["dim"]), metric=int(z["metric"]), quant=int(z["quant"]),
-                          block_ids=z["block_ids"], centroids=z["centroids"],
-                          post_off=z["post_off"], ids=z["ids"], codes=z["codes"], raw=z["raw"],
-                          deleted=z["deleted"], int8_scale=float(z["int8"][0]),
-                          int8_offset=float(z["int8"][1]), rq_padded=int(z["rq_padded"]),
-                          rq_signs=z["rq_signs"].reshape(3, -1) if z["rq_signs"].size else None,
-                          probe_ratio=float(z["probe_ratio"]), rerank_size=int(z["rerank_size"]))
+        """Open an index the reference saved at `path` (SPFresh::open,
+        src/spfresh.rs:819-850; python/src/lib.rs:315-319)."""
+        arr, _ = read_index(path, metric)
         return SPFresh(arr, path=path)
 
     # -- search (python/src/lib.rs:326-337)
