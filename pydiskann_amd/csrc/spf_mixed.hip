@@ -475,6 +475,7 @@ __device__ __forceinline__ uint32_t lowest_bits(uint32_t m, uint32_t c) {
 // Lane l owns words
 // l, l+64, ... (word w = slots 32w..32w+31) and re-reads them from L2 in each
 // of the four passes; a word's four loads go out together.
+template <bool BITS>
 __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32_t n,
                                           uint32_t want, uint32_t *pool_entry,
                                           uint16_t *pool_key, uint32_t *pool_want, bool map,
@@ -482,29 +483,33 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
                                           uint32_t *sel_bits, uint32_t *a_bits, uint32_t ca) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nwords = (n + 31) / 32;
-    // the lane's words, one word's four loads together
+    // the segment's 16-byte units (8 keys each), lane-interleaved, four units
+    // per lane in flight: body(unit, index)
+    const uint4 *kp4 = reinterpret_cast<const uint4 *>(kp);
+    const uint32_t nun = (n + 7) / 8;
     auto sweep = [&](auto &&body) {
-        for (uint32_t w = lane; w < nwords; w += 64) {
-            uint4 r0[4];
-            load_word_raw(kp, w, r0);
-            body(r0, w);
+        for (uint32_t u0 = lane; u0 < nun; u0 += 256) {
+            uint4 v[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (u0 + 64 * i < nun) v[i] = kp4[u0 + 64 * i];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (u0 + 64 * i < nun) body(v[i], u0 + 64 * i);
         }
     };
     uint32_t inval = 0, kmn = 0xffffu, kmx = 0u;
-    sweep([&](const uint4 (&r)[4], uint32_t w) {
+    sweep([&](const uint4 v, uint32_t u) {
+        const uint32_t v4[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
-#pragma unroll
-            for (int t = 0; t < 8; t++) {
-                const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
-                if (32 * w + 8 * c + t >= n) continue;
-                if (kk == 0xffffu) {
-                    inval++;
-                } else {
-                    kmn = min(kmn, kk);
-                    kmx = max(kmx, kk);
-                }
+        for (int t = 0; t < 8; t++) {
+            const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+            if (8 * u + t >= n) continue;
+            if (kk == 0xffffu) {
+                inval++;
+            } else {
+                kmn = min(kmn, kk);
+                kmx = max(kmx, kk);
             }
         }
     });
@@ -529,15 +534,12 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
         for (int i = 0; i < 12; i++) hist[12 * lane + i] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        sweep([&](const uint4 (&r)[4], uint32_t w) {
+        sweep([&](const uint4 v, uint32_t u) {
+            const uint32_t v4[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
-#pragma unroll
-                for (int t = 0; t < 8; t++) {
-                    const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
-                    if (kk != 0xffffu && 32 * w + 8 * c + t < n) atomicAdd(&h1[(kk - kmn) >> sh], 1u);
-                }
+            for (int t = 0; t < 8; t++) {
+                const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+                if (kk != 0xffffu && 8 * u + t < n) atomicAdd(&h1[(kk - kmn) >> sh], 1u);
             }
         });
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -548,18 +550,15 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
         uint32_t b2W = 0, below2W = 0, b2A = 0, below2A = 0;
         if (sh > 0) {
             const uint32_t mask = (1u << sh) - 1u;
-            sweep([&](const uint4 (&r)[4], uint32_t w) {
+            sweep([&](const uint4 v, uint32_t u) {
+                const uint32_t v4[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
-#pragma unroll
-                    for (int t = 0; t < 8; t++) {
-                        const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
-                        if (kk == 0xffffu || 32 * w + 8 * c + t >= n) continue;
-                        const uint32_t bin = (kk - kmn) >> sh;
-                        if (!all && bin == bW) atomicAdd(&h2[(kk - kmn) & mask], 1u);
-                        if (bin == bA) atomicAdd(&h3[(kk - kmn) & mask], 1u);
-                    }
+                for (int t = 0; t < 8; t++) {
+                    const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+                    if (kk == 0xffffu || 8 * u + t >= n) continue;
+                    const uint32_t bin = (kk - kmn) >> sh;
+                    if (!all && bin == bW) atomicAdd(&h2[(kk - kmn) & mask], 1u);
+                    if (bin == bA) atomicAdd(&h3[(kk - kmn) & mask], 1u);
                 }
             });
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -576,14 +575,10 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
     }
     // collect: round m = words 64m .. 64m+63 in lane order (slot order)
     uint32_t tie_run = 0, tieA_run = 0, mark_run = 0;
-    uint4 rn[4];   // next round's word, loaded ahead
-    if (lane < nwords) load_word_raw(kp, lane, rn);
     for (uint32_t m = 0; 64 * m < nwords; m++) {
         const uint32_t w = lane + 64 * m;
         uint4 r[4];
-#pragma unroll
-        for (int c = 0; c < 4; c++) r[c] = rn[c];
-        if (w + 64 < nwords) load_word_raw(kp, w + 64, rn);
+        if (w < nwords) load_word_raw(kp, w, r);
         uint32_t live = 0, lt = 0, eq = 0, ltA = 0, eqA = 0;
         if (w < nwords) word_masks2(r, w, n, T, TA, live, lt, eq, ltA, eqA);
         // tie counts of this round (each at most 2048) packed in one scan
@@ -593,7 +588,7 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
         tie_run += tt & 0xffffu;
         tieA_run += tt >> 16;
         const uint32_t sel = all ? live : lt | (tr < need ? lowest_bits(eq, need - tr) : 0u);
-        if (sel_bits) {   // one coalesced mask store per word
+        if (BITS) {   // one coalesced mask store per word
             if (w < nwords) {
                 sel_bits[w] = sel;
                 if (a_bits)
@@ -620,22 +615,23 @@ __device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32
 
 constexpr uint32_t kWaveTab = 256;   // probe tables held in LDS up to this n_probe
 
-__global__ __launch_bounds__(256) void k_pool_select(const PoolParams p, uint32_t nq) {
+template <bool BITS>
+__global__ __launch_bounds__(256, 8) void k_pool_select(const PoolParams p, uint32_t nq) {
     __shared__ uint32_t hist[4][768];
-    __shared__ uint32_t s_pbase[4][kWaveTab], s_ebase[4][kWaveTab];
+    __shared__ uint32_t s_pbase[BITS ? 1 : 4][BITS ? 1 : kWaveTab], s_ebase[BITS ? 1 : 4][BITS ? 1 : kWaveTab];
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t q = blockIdx.x * 4 + wv;
     if (q >= nq) return;   // whole wave: nothing below synchronises the workgroup
     const uint64_t b = p.seg_off ? p.seg_off[q] : (uint64_t)q * p.seg_stride;   // multiple of 32
     const uint32_t n = p.seg_cnt ? p.seg_cnt[q] : p.seg_len;
     ProbeTab tab;
-    tab.np = p.map ? p.np_q[q] : 0u;
+    tab.np = (!BITS && p.map) ? p.np_q[q] : 0u;
     tab.qp_row = p.qp_slot + (size_t)q * p.npr;
     tab.probe_row = p.probe_post + (size_t)q * p.npr;
     tab.post_off = p.post_off;
     tab.seg = b;
     tab.pbase = tab.ebase = nullptr;
-    if (tab.np && tab.np <= kWaveTab) {
+    if (!BITS && tab.np && tab.np <= kWaveTab) {
         for (uint32_t j = lane; j < tab.np; j += 64) {
             s_pbase[wv][j] = (uint32_t)(tab.qp_row[j] - b);
             s_ebase[wv][j] = p.post_off[tab.probe_row[j]];
@@ -645,9 +641,9 @@ __global__ __launch_bounds__(256) void k_pool_select(const PoolParams p, uint32_
         tab.pbase = s_pbase[wv];
         tab.ebase = s_ebase[wv];
     }
-    pool_wave(p.keys + b, q, n, p.want, p.pool_entry, p.pool_key, p.pool_want, p.map, tab,
-              hist[wv], p.sel_bits ? p.sel_bits + b / 32 : nullptr,
-              p.a_bits ? p.a_bits + b / 32 : nullptr, p.ca);
+    pool_wave<BITS>(p.keys + b, q, n, p.want, p.pool_entry, p.pool_key, p.pool_want,
+                    !BITS && p.map, tab, hist[wv], BITS ? p.sel_bits + b / 32 : nullptr,
+                    BITS && p.a_bits ? p.a_bits + b / 32 : nullptr, p.ca);
 }
 
 // ---------------------------------------------------------------- query order
@@ -1112,7 +1108,10 @@ hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) 
 
 hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s) {
     if (nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pool_select, dim3((nq + 3) / 4), dim3(256), 0, s, p, nq);
+    if (p.sel_bits)
+        hipLaunchKernelGGL(k_pool_select<true>, dim3((nq + 3) / 4), dim3(256), 0, s, p, nq);
+    else
+        hipLaunchKernelGGL(k_pool_select<false>, dim3((nq + 3) / 4), dim3(256), 0, s, p, nq);
     return hipGetLastError();
 }
 
