@@ -610,6 +610,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     mp.xn16 = ix->d_xn16;
     mp.del = ix->d_del;
     mp.metric = ix->metric;
+    mp.echunk = mfma_chunk(dpad32 / 32);
     mp.out16 = w->cand16.p;
     const uint32_t grid = xcd_grid(n_items, 16384);
     HIPCHK(mark(SPF_STAGE_SCAN));
@@ -719,9 +720,12 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         return (v == 4 || v == 8 || v == 16) ? v : 0u;
     }();
     const uint32_t Gv = dpad > 1024 ? 4 : (g_env ? g_env : 8);
-    // queries per posting-scan item: matrix-core tiles take 32, the RaBitQ
-    // estimator scan 8
-    const uint32_t G = mixed ? (ix->rq_l2 ? 8u : 32u) : Gv;
+    // queries and entries per posting-scan item: the matrix-core scan takes
+    // kMixQ queries over mfma_chunk entries, the RaBitQ estimator scan 8
+    // queries, the VALU scan Gv; both of the latter over kEChunk entries
+    const bool mfma_scan = mixed && !ix->rq_l2;
+    const uint32_t G = mfma_scan ? kMixQ : mixed ? 8u : Gv;
+    const uint32_t echunk = mfma_scan ? mfma_chunk(dpad32 / 32) : kEChunk;
     const bool prof = opts && (opts->flags & SPF_FLAG_PROFILE);
     if (prof)
         for (auto &e : w->ev)
@@ -906,6 +910,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.item_cnt = w->item_cnt.p;
     pl.pair_src = ix->rq_l2 ? w->pair_src.p : nullptr;
     pl.G = G;
+    pl.echunk = echunk;
     HIPCHK(hipMemsetAsync(w->post_cnt.p, 0, (P + 1) * sizeof(uint32_t), s));
     HIPCHK(launch_plan_q(pl, s));
     HIPCHK(launch_excl_scan_u64(w->cand_len.p, w->cand_off.p, nq, s));

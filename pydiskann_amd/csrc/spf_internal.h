@@ -122,8 +122,15 @@ struct MixedParams {
     const float *xn16;          // |x16|^2 per entry
     const uint32_t *del;
     uint32_t metric;            // key: L2 |q16-x16|^2, DOT 1-q16.x16, COSINE 1-cos(q16,x16)
+    uint32_t echunk;            // entries per work item (64 x tiles per wave)
     uint16_t *out16;            // 16-bit candidate keys
 };
+
+// Matrix-core scan work items: (posting, `mfma_chunk(ks)` entries, up to
+// kMixQ queries).  ks = 32-dim K steps.
+constexpr uint32_t kMixQ = 128;
+inline uint32_t mfma_tiles(uint32_t ks) { return ks <= 4 ? 4u : 2u; }   // 16-entry tiles per wave
+inline uint32_t mfma_chunk(uint32_t ks) { return 64u * mfma_tiles(ks); }
 
 // Exact rerank of each query's pool (selection masks over the candidate
 // keys from pool selection).  With prune, only the `ca` smallest keys and the members whose
@@ -252,6 +259,7 @@ struct PlanParams {
     const uint32_t *item_off;   // [n_post + 1]
     uint32_t *item_post;        // [n_items]
     uint32_t G;                 // queries per work item
+    uint32_t echunk;            // entries per work item
 };
 
 struct RerankParams {

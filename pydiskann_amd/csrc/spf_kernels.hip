@@ -603,7 +603,7 @@ __global__ void k_items(const PlanParams p) {
     if (b >= p.n_post) return;
     const uint32_t cnt = p.pair_off[b + 1] - p.pair_off[b];
     const uint32_t len = p.post_len[b];
-    p.item_cnt[b] = (cnt && len) ? ((cnt + p.G - 1) / p.G) * ((len + kEChunk - 1) / kEChunk) : 0u;
+    p.item_cnt[b] = (cnt && len) ? ((cnt + p.G - 1) / p.G) * ((len + p.echunk - 1) / p.echunk) : 0u;
 }
 
 // work item -> posting (one thread per posting)

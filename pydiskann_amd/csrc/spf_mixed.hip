@@ -74,37 +74,32 @@ hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t
 }
 
 // ---------------------------------------------------------------- MFMA scan
-constexpr int kMixG = 32;      // queries per work item (two 16-row MFMA tiles)
-constexpr int kMixKS = 8;      // max K-steps of 32 dims held in registers (dim <= 256)
-
-__device__ __forceinline__ void item_decode(const MixedParams &p, uint32_t item, uint32_t &pp,
-                                            uint32_t &pb, uint32_t &e_begin, uint32_t &e_end,
-                                            uint32_t &pr0, uint32_t &gcnt) {
-    pp = p.item_post[item];
-    const uint32_t local = item - p.item_off[pp];
-    pb = p.post_off[pp];
-    const uint32_t pe = p.post_off[pp + 1];
-    const uint32_t nec = (pe - pb + kEChunk - 1) / kEChunk;
-    const uint32_t qc = local / nec, ec = local - qc * nec;
-    pr0 = p.pair_off[pp] + qc * kMixG;
-    gcnt = min((uint32_t)kMixG, p.pair_off[pp + 1] - pr0);
-    e_begin = pb + ec * kEChunk;
-    e_end = min(e_begin + kEChunk, pe);
-}
-
+// Work item = (posting, chunk of 64 T entries, up to kMixQ queries that probe
+// it).  Wave w owns the chunk's 16-entry tiles w, w+4, ...: their B fragments
+// (f16 codes) are loaded once into registers, then reused by every 32-query
+// group of the item (two 16-row A tiles per group, 16x16x32 f16 MFMA, f32
+// accumulate).  Each (query, entry) writes the 16-bit key of its distance.
 template <int KS>
 __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
-    __shared__ uint64_t s_slot[kMixG];
-    __shared__ uint32_t s_qi[kMixG];
-    __shared__ float s_qn[kMixG];
+    constexpr int T = KS <= 4 ? 4 : 2;   // tiles per wave, mfma_tiles()
+    __shared__ uint64_t s_slot[kMixQ];
+    __shared__ uint32_t s_qi[kMixQ];
+    __shared__ float s_qn[kMixQ];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 15, kg = lane >> 4;
+    const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
     for (XcdWalk it(p.n_items); it.more(); it.next()) {
         const uint32_t item = it.item;
-        uint32_t pp, pb, e_begin, e_end, pr0, gcnt;
-        item_decode(p, item, pp, pb, e_begin, e_end, pr0, gcnt);
-        __syncthreads();
-        if (tid < kMixG) {
+        const uint32_t pp = p.item_post[item];
+        const uint32_t local = item - p.item_off[pp];
+        const uint32_t pb = p.post_off[pp], pe = p.post_off[pp + 1];
+        const uint32_t nec = (pe - pb + p.echunk - 1) / p.echunk;
+        const uint32_t qc = local / nec, ec = local - qc * nec;
+        const uint32_t pr0 = p.pair_off[pp] + qc * kMixQ;
+        const uint32_t gcnt = min(kMixQ, p.pair_off[pp + 1] - pr0);
+        const uint32_t e_begin = pb + ec * p.echunk, e_end = min(e_begin + p.echunk, pe);
+        __syncthreads();   // the previous item is done with the query table
+        if (tid < kMixQ) {
             const bool ok = tid < gcnt;
             const uint32_t q = ok ? p.pair_q[pr0 + tid] : 0u;
             s_qi[tid] = q;
@@ -114,67 +109,67 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
             else if (p.metric == M_COS) qn = qn > 0.0f ? 1.0f / sqrtf(qn) : 0.0f;
             s_qn[tid] = qn;
         }
-        __syncthreads();
-        // A fragments: queries (rows) x dims (K), two 16-row tiles
-        half8_t a[2][KS];
+        // the wave's B fragments, and per-entry norm / tombstone of its column
+        uint4 bf[T][KS];
+        float xn[T], rx[T];
+        bool live[T];
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
-            const uint32_t g = 16 * t + col;
-            const bool ok = g < gcnt;
-            const uint4 *qrow = p.q16 + (size_t)s_qi[g] * p.q16_units;
-#pragma unroll
-            for (int s = 0; s < KS; s++) {
-                uint4 v = make_uint4(0, 0, 0, 0);
-                if (ok && s < (int)p.ks) v = qrow[4 * s + kg];   // rows hold 4*ks units
-                a[t][s] = as_half8(v);
-            }
-        }
-        // B fragments of this wave's 16-entry tiles, next tile loaded ahead
-        auto load_b = [&](uint32_t e0, uint4 (&bv)[KS]) {
-            const uint32_t e = e0 + col;
+        for (int t = 0; t < T; t++) {
+            const uint32_t e = e_begin + 16 * (wave + 4 * t) + col;
             const bool valid = e < e_end;
             const uint4 *src = p.codes16 + (size_t)(e >> 6) * p.units * 64 + (e & 63);
 #pragma unroll
             for (int s = 0; s < KS; s++) {
                 const uint32_t u = 4 * s + kg;
-                bv[s] = (valid && u < p.units) ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
+                bf[t][s] = (valid && u < p.units) ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
             }
-        };
-        uint32_t e0 = e_begin + 16 * wave;
-        uint4 bnext[KS];
-        if (e0 < e_end) load_b(e0, bnext);
-        for (; e0 < e_end; e0 += 64) {
-            uint4 bcur[KS];
+            xn[t] = valid ? p.xn16[e] : 0.0f;
+            live[t] = valid && !(p.del && ((p.del[e >> 5] >> (e & 31)) & 1u));
+            rx[t] = p.metric == M_COS ? (xn[t] > 0.0f ? 1.0f / sqrtf(xn[t]) : 0.0f) : 1.0f;
+        }
+        __syncthreads();
+        for (uint32_t g0 = 0; g0 < gcnt; g0 += 32) {
+            // A fragments: 32 queries (two 16-row tiles) x dims
+            half8_t a[2][KS];
 #pragma unroll
-            for (int s = 0; s < KS; s++) bcur[s] = bnext[s];
-            if (e0 + 64 < e_end) load_b(e0 + 64, bnext);
-            floatx4_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+            for (int h = 0; h < 2; h++) {
+                const uint32_t g = g0 + 16 * h + col;
+                const bool ok = g < gcnt;
+                const uint4 *qrow = p.q16 + (size_t)s_qi[ok ? g : 0] * p.q16_units;
 #pragma unroll
-            for (int s = 0; s < KS; s++) {
-                const half8_t b = as_half8(bcur[s]);
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b, acc1, 0, 0, 0);
+                for (int s = 0; s < KS; s++) {
+                    uint4 v = make_uint4(0, 0, 0, 0);
+                    if (ok && s < (int)p.ks) v = qrow[4 * s + kg];   // rows hold 4*ks units
+                    a[h][s] = as_half8(v);
+                }
             }
-            const uint32_t e = e0 + col;
-            if (e < e_end) {
-                const float xn = p.xn16[e];
-                const bool dead = p.del && ((p.del[e >> 5] >> (e & 31)) & 1u);
-                const uint32_t off = e - pb;
-                const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
-                // cosine: scale by the inverse norms (s_qn holds them)
-                const float rx = p.metric == M_COS ? (xn > 0.0f ? 1.0f / sqrtf(xn) : 0.0f) : 1.0f;
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const uint32_t g0 = 4 * kg + j, g1 = 16 + 4 * kg + j;
-                    if (g0 < gcnt) {
-                        const float d = l2 ? xn + s_qn[g0] - 2.0f * acc0[j]
-                                           : 1.0f - acc0[j] * s_qn[g0] * rx;
-                        p.out16[s_slot[g0] + off] = dead ? (uint16_t)0xffff : (uint16_t)(fkey_m(d) >> 16);
-                    }
-                    if (g1 < gcnt) {
-                        const float d = l2 ? xn + s_qn[g1] - 2.0f * acc1[j]
-                                           : 1.0f - acc1[j] * s_qn[g1] * rx;
-                        p.out16[s_slot[g1] + off] = dead ? (uint16_t)0xffff : (uint16_t)(fkey_m(d) >> 16);
+            for (int t = 0; t < T; t++) {
+                const uint32_t e0 = e_begin + 16 * (wave + 4 * t);
+                if (e0 >= e_end) break;   // later tiles are further out
+                floatx4_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < KS; s++) {
+                    const half8_t b = as_half8(bf[t][s]);
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b, acc1, 0, 0, 0);
+                }
+                const uint32_t e = e0 + col;
+                if (e < e_end) {
+                    const uint32_t off = e - pb;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const uint32_t r0 = g0 + 4 * kg + j, r1 = r0 + 16;
+                        if (r0 < gcnt) {
+                            const float d = l2 ? xn[t] + s_qn[r0] - 2.0f * acc0[j]
+                                               : 1.0f - acc0[j] * s_qn[r0] * rx[t];
+                            p.out16[s_slot[r0] + off] = live[t] ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
+                        }
+                        if (r1 < gcnt) {
+                            const float d = l2 ? xn[t] + s_qn[r1] - 2.0f * acc1[j]
+                                               : 1.0f - acc1[j] * s_qn[r1] * rx[t];
+                            p.out16[s_slot[r1] + off] = live[t] ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
+                        }
                     }
                 }
             }
@@ -183,6 +178,8 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
 }
 
 // ---------------------------------------------------------------- probe
+constexpr int kMixKS = 8;      // max K-steps of 32 dims held in registers (dim <= 256)
+
 // One workgroup = 16 queries x 1024 centroids; each wave takes 16-centroid
 // tiles.  bf16 operands, f32 accumulate, 16-bit keys out.
 typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
