@@ -192,6 +192,11 @@ def main() -> int:
                 "algorithmic_bytes_per_launch": dk["algorithmic_bytes"],
                 "traffic_source": traffic_src, "stages": kernels,
                 "candidates_per_launch": cand}
+    if traffic:
+        # DRAM bytes the kernel actually moved (rocprofv3 counters) over its live time
+        tgb = traffic / (dk["ms"] * 1e-3) / 1e9
+        roofline["traffic_GBs"] = round(tgb, 1)
+        roofline["traffic_frac"] = round(tgb / HBM_PEAK_GBS, 4)
 
     cpu = None
     if not args.no_cpu:
