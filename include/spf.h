@@ -149,6 +149,15 @@ int spf_search_batch_device(spf_index *ix, const float *d_queries, uint32_t nq, 
                             uint64_t *d_out_ids, float *d_out_dists, uint32_t *d_out_counts,
                             void *stream);
 
+/* Insert routing: for each of n vectors, the reference block index of its
+ * nearest live posting and the metric distance to that posting's centroid
+ * (exact over all live centroids; equal distances -> lower block index).
+ * Replaces the parallel part of Inner::insert, `nearest(v, 1)` per vector of a
+ * chunk (src/spfresh.rs:437-444, nearest :358-372).  out_block gets UINT32_MAX
+ * and out_dist +inf when the index has no live posting. */
+int spf_nearest_postings(spf_index *ix, const float *vectors, uint32_t n, uint32_t dim,
+                         uint32_t *out_block, float *out_dist);
+
 /* Profile of the last search on this thread that set SPF_FLAG_PROFILE. */
 int spf_last_profile(spf_profile *out);
 

@@ -19,8 +19,8 @@ PREC_EXACT, PREC_MIXED = 0, 1
 
 # every entry point of include/spf.h
 EXPORTS = ("spf_index_create", "spf_index_destroy", "spf_index_get_info", "spf_delete",
-           "spf_search_batch", "spf_search_batch_device", "spf_last_profile", "spf_selftest",
-           "spf_last_error", "spf_version")
+           "spf_search_batch", "spf_search_batch_device", "spf_nearest_postings",
+           "spf_last_profile", "spf_selftest", "spf_last_error", "spf_version")
 FLAG_PROFILE = 1
 FLAG_FULL_RERANK = 2
 STAGES = ("prep", "probe", "probe_select", "plan", "scan", "select", "rerank", "finalize")
@@ -91,6 +91,8 @@ def lib():
         L.spf_search_batch_device.argtypes = [P, P, u32, u32, u32, u32, C.POINTER(SearchOpts),
                                               P, P, P, P]
         L.spf_search_batch_device.restype = C.c_int
+        L.spf_nearest_postings.argtypes = [P, P, u32, u32, P, P]
+        L.spf_nearest_postings.restype = C.c_int
         L.spf_last_profile.argtypes = [C.POINTER(Profile)]
         L.spf_last_profile.restype = C.c_int
         L.spf_selftest.argtypes = [C.c_int]
@@ -177,6 +179,18 @@ class DeviceIndex:
         check(lib().spf_search_batch(self.h, _p(q), nq, dim, k, n_probe, C.byref(o), _p(ids),
                                      _p(dd), _p(cnt)))
         return ids, dd, cnt
+
+    def nearest_postings(self, vectors):
+        """(block index, distance) of each vector's nearest live posting:
+        the routing step of Inner::insert (src/spfresh.rs:437-444)."""
+        v = np.ascontiguousarray(vectors, dtype=np.float32)
+        if v.ndim == 1:
+            v = v[None, :]
+        n, dim = v.shape
+        blocks = np.empty(n, dtype=np.uint32)
+        dd = np.empty(n, dtype=np.float32)
+        check(lib().spf_nearest_postings(self.h, _p(v), n, dim, _p(blocks), _p(dd)))
+        return blocks, dd
 
     def search_device(self, q_ptr: int, nq: int, dim: int, k: int, n_probe: int, ids_ptr: int,
                       dists_ptr: int, counts_ptr: int, stream: int = 0,

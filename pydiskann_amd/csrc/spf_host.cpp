@@ -151,6 +151,7 @@ struct spf_index {
     uint4 *d_cent = nullptr;
     float *d_cnorm = nullptr;
     uint32_t *d_post_off = nullptr, *d_post_len = nullptr;
+    uint32_t *d_block_sorted = nullptr;   // reference block index of each (sorted) posting
     uint4 *d_codes = nullptr;
     uint32_t *d_rank = nullptr;
     uint64_t *d_rank_id = nullptr;
@@ -186,7 +187,7 @@ struct spf_index {
     std::vector<Workspace *> pool;
 
     ~spf_index() {
-        void *ps[] = {d_cent, d_cnorm, d_post_off, d_post_len, d_codes, d_rank,
+        void *ps[] = {d_cent, d_cnorm, d_post_off, d_post_len, d_block_sorted, d_codes, d_rank,
                       d_rank_id, d_raw, d_del, d_xnorm, d_xsq, d_rank_entry, d_raw_e, d_xn16,
                       d_mu, d_cnbf, d_cent_rows, d_cbf, d_rq_signs, d_post_ex, d_post_xn};
         for (void *p : ps)
@@ -428,6 +429,11 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         HIPCHK(upload(&ix->d_cent_rows, rows.data(), rows.size(), bytes));
     }
     HIPCHK(upload(&ix->d_post_off, post_off.data(), post_off.size(), bytes));
+    {
+        std::vector<uint32_t> bs(std::max<uint32_t>(P, 1), 0u);
+        for (uint32_t i = 0; i < P; i++) bs[i] = d->block_ids[perm[i]];
+        HIPCHK(upload(&ix->d_block_sorted, bs.data(), bs.size(), bytes));
+    }
     HIPCHK(upload(&ix->d_post_len, post_len.data(), post_len.size(), bytes));
     if (d->quantizer == SPF_QUANT_RABITQ) {
         HIPCHK(upload(&ix->d_rq_signs, d->rabitq_signs, 3 * (size_t)ix->rq_padded, bytes));
@@ -1077,6 +1083,54 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     return SPF_OK;
 }
 
+// exact nearest live posting per vector: prep + dense f32 probe + top 1
+int nearest_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_v, uint32_t n,
+                 uint32_t *d_block, float *d_dist) {
+    const uint32_t dpad = ix->dpad, P = ix->P;
+    HIPCHK(w->qv_probe.ensure((size_t)n * dpad));
+    HIPCHK(w->qnorm.ensure(n));
+    HIPCHK(w->q8.ensure((size_t)n * dpad / 4));
+    HIPCHK(w->qsq.ensure(n));
+    HIPCHK(w->probe_sel.ensure(n));
+    HIPCHK(w->probe_cnt.ensure(n));
+    PrepParams pp{};
+    pp.q = d_v;
+    pp.nq = n;
+    pp.dim = ix->dim;
+    pp.dpad = dpad;
+    pp.qv_probe = w->qv_probe.p;
+    pp.qv_scan = w->qv_probe.p;
+    pp.q8 = w->q8.p;
+    pp.qnorm = w->qnorm.p;
+    pp.qsq = w->qsq.p;
+    HIPCHK(launch_prep(pp, s));
+    if (P == 0) {
+        HIPCHK(hipMemsetAsync(w->probe_cnt.p, 0, n * sizeof(uint32_t), s));
+    } else {
+        HIPCHK(w->probe_keys.ensure((size_t)n * P));
+        ScanParams sp{};
+        sp.data = ix->d_cent;
+        sp.units = ix->cent_units;
+        sp.n_entries = P;
+        sp.xnorm = ix->d_cnorm;
+        sp.qv = w->qv_probe.p;
+        sp.qnorm = w->qnorm.p;
+        sp.dpad = dpad;
+        sp.out = w->probe_keys.p;
+        sp.nq = n;
+        sp.n_echunks = (P + kEChunk - 1) / kEChunk;
+        sp.out_stride = P;
+        const uint32_t G = dpad > 1024 ? 4 : 8;
+        const uint64_t items = (uint64_t)((n + G - 1) / G) * sp.n_echunks;
+        HIPCHK(launch_scan(CK_F32, ix->metric, true, G, sp, xcd_grid(items, 16384), s));
+        HIPCHK(launch_select(w->probe_keys.p, nullptr, nullptr, P, n, 1, w->probe_sel.p,
+                             w->probe_cnt.p, s));
+    }
+    HIPCHK(launch_nearest_out(w->probe_sel.p, w->probe_cnt.p, n, ix->d_block_sorted, d_block,
+                              d_dist, s));
+    return SPF_OK;
+}
+
 int check_search_args(spf_index *ix, uint32_t dim, uint32_t k) {
     if (!ix) return fail(SPF_EINVAL, "null index");
     if (dim != ix->dim) {
@@ -1183,6 +1237,33 @@ int spf_search_batch(spf_index *ix, const float *queries, uint32_t nq, uint32_t 
         HIPCHK(hipMemcpyAsync(out_ids, w->out_ids.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(out_dists, w->out_d.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(out_counts, w->out_c.p, (size_t)nq * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return SPF_OK;
+    };
+    rc = run();
+    release_ws(ix, w);
+    return rc;
+}
+
+int spf_nearest_postings(spf_index *ix, const float *vectors, uint32_t n, uint32_t dim,
+                         uint32_t *out_block, float *out_dist) {
+    int rc = check_search_args(ix, dim, 1);
+    if (rc) return rc;
+    if (n == 0) return SPF_OK;
+    if (!vectors || !out_block || !out_dist) return fail(SPF_EINVAL, "null buffer");
+    DeviceGuard dg(ix->device);
+    Workspace *w = acquire(ix);
+    if (!w) return fail(SPF_EDEVICE, "workspace creation failed");
+    hipStream_t s = w->stream;
+    auto run = [&]() -> int {
+        HIPCHK(w->qin.ensure((size_t)n * dim));
+        HIPCHK(w->out_c.ensure(n));
+        HIPCHK(w->out_d.ensure(n));
+        HIPCHK(hipMemcpyAsync(w->qin.p, vectors, (size_t)n * dim * 4, hipMemcpyHostToDevice, s));
+        int r = nearest_impl(ix, w, s, w->qin.p, n, w->out_c.p, w->out_d.p);
+        if (r) return r;
+        HIPCHK(hipMemcpyAsync(out_block, w->out_c.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(out_dist, w->out_d.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         return SPF_OK;
     };

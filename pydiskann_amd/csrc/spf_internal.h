@@ -289,6 +289,9 @@ hipError_t launch_item_table(const PlanParams &p, hipStream_t s);
 hipError_t launch_excl_scan_u64(const uint64_t *in, uint64_t *out, uint32_t n, hipStream_t s);
 hipError_t launch_excl_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t s);
 hipError_t launch_rerank(const RerankParams &p, hipStream_t s);
+hipError_t launch_nearest_out(const uint64_t *probe_sel, const uint32_t *probe_cnt, uint32_t n,
+                              const uint32_t *block_sorted, uint32_t *out_block, float *out_dist,
+                              hipStream_t s);
 hipError_t launch_finalize(const uint64_t *keys, uint32_t stride, const uint32_t *cnt, uint32_t nq,
                            uint32_t k,
                            const uint64_t *rank_id, uint64_t *out_ids, float *out_dists,

@@ -835,6 +835,30 @@ hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, const ui
     return hipGetLastError();
 }
 
+// nearest posting of each vector: probe key -> (block index, distance)
+__global__ void k_nearest_out(const uint64_t *probe_sel, const uint32_t *probe_cnt, uint32_t n,
+                              const uint32_t *block_sorted, uint32_t *out_block, float *out_dist) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (probe_cnt[i] == 0) {
+        out_block[i] = 0xffffffffu;
+        out_dist[i] = __builtin_inff();
+        return;
+    }
+    const uint64_t key = probe_sel[i];
+    out_block[i] = block_sorted[(uint32_t)key];
+    out_dist[i] = key_to_float((uint32_t)(key >> 32));
+}
+
+hipError_t launch_nearest_out(const uint64_t *probe_sel, const uint32_t *probe_cnt, uint32_t n,
+                              const uint32_t *block_sorted, uint32_t *out_block, float *out_dist,
+                              hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nearest_out, dim3((n + 255) / 256), dim3(256), 0, s, probe_sel, probe_cnt,
+                       n, block_sorted, out_block, out_dist);
+    return hipGetLastError();
+}
+
 hipError_t launch_plan_q(const PlanParams &p, hipStream_t s) {
     if (p.nq == 0) return hipSuccess;
     hipLaunchKernelGGL(k_plan_q, dim3((p.nq + 3) / 4), dim3(256), 0, s, p);

@@ -74,6 +74,7 @@ def test_argument_errors_without_a_device():
     assert L.spf_last_error() == b"RaBitQ index needs its rotation signs"
     assert L.spf_search_batch(None, None, 1, 4, 10, 8, None, None, None, None) == _lib.SPF_EINVAL
     assert L.spf_delete(None, None, 0) == _lib.SPF_EINVAL
+    assert L.spf_nearest_postings(None, None, 1, 4, None, None) == _lib.SPF_EINVAL
     assert L.spf_index_get_info(None, None) == _lib.SPF_EINVAL
 
 
