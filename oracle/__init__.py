@@ -24,6 +24,7 @@ class OIndex(C.Structure):
         ("block_ids", C.c_void_p), ("centroids", C.c_void_p), ("post_off", C.c_void_p),
         ("ids", C.c_void_p), ("codes", C.c_void_p), ("raw", C.c_void_p),
         ("n_raw", C.c_uint64), ("deleted", C.c_void_p), ("rq_signs", C.c_void_p),
+        ("labels", C.c_void_p),
     ]
 
 
@@ -67,6 +68,7 @@ def lib():
         L.spf_o_search.restype = u32
         L.spf_o_search.argtypes = [P, P, u32, u32, P, P]
         L.spf_o_search_batch.argtypes = [P, P, u32, u32, u32, u32, P, P, P]
+        L.spf_o_search_batch_filtered.argtypes = [P, P, P, u32, u32, u32, u32, P, P, P]
         L.spf_o_set_fault.argtypes = [C.c_int]
         _lib = L
     return _lib
@@ -93,6 +95,11 @@ class Oracle:
         s.n_raw = ix.n_raw
         s.deleted = _p(ix.deleted)
         s.rq_signs = _p(ix.rq_signs)
+        self._labels = None
+        if getattr(ix, "labels", None) is not None:
+            self._labels = np.ascontiguousarray(ix.labels, dtype=np.uint8)
+            assert self._labels.shape[0] == ix.n_raw
+            s.labels = _p(self._labels)
         self.s = s
 
     def search(self, q, k: int, n_probe: int):
@@ -110,14 +117,21 @@ class Oracle:
         c = lib().spf_o_probe(C.byref(self.s), _p(q), n_probe, _p(posts), _p(dd))
         return posts[:c].copy(), dd[:c].copy()
 
-    def search_batch(self, qs, k: int, n_probe: int, threads: int = 1):
+    def search_batch(self, qs, k: int, n_probe: int, threads: int = 1, masks=None):
+        """masks: per-query u64 label masks (filtered search) or None."""
         qs = np.ascontiguousarray(qs, dtype=np.float32)
         nq = qs.shape[0]
         ids = np.zeros((nq, k), dtype=np.uint64)
         dd = np.zeros((nq, k), dtype=np.float32)
         cnt = np.zeros(nq, dtype=np.uint32)
-        lib().spf_o_search_batch(C.byref(self.s), _p(qs), nq, k, n_probe, threads, _p(ids),
-                                 _p(dd), _p(cnt))
+        if masks is None:
+            lib().spf_o_search_batch(C.byref(self.s), _p(qs), nq, k, n_probe, threads, _p(ids),
+                                     _p(dd), _p(cnt))
+        else:
+            m = np.ascontiguousarray(masks, dtype=np.uint64)
+            assert m.shape == (nq,)
+            lib().spf_o_search_batch_filtered(C.byref(self.s), _p(qs), _p(m), nq, k, n_probe,
+                                              threads, _p(ids), _p(dd), _p(cnt))
         return ids, dd, cnt
 
 

@@ -311,6 +311,14 @@ static inline int is_deleted(const spf_o_index *ix, uint64_t id) {
     return (ix->deleted[id >> 3] >> (id & 7)) & 1;
 }
 
+/* entry skipped by the scan: tombstoned, or (filtered) label not in the mask */
+static inline int skipped(const spf_o_index *ix, uint64_t id, int filtered, uint64_t mask) {
+    if (is_deleted(ix, id)) return 1;
+    if (!filtered || !ix->labels || g_fault == 5) return 0;
+    uint8_t l = id < ix->n_raw ? ix->labels[id] : 255;
+    return l >= 64 || !((mask >> l) & 1u);
+}
+
 /* Inner::nearest, exact branch (src/spfresh.rs:358-372), then the probe_ratio
  * take_while of Inner::search (src/spfresh.rs:683-687).  Ties in centroid
  * distance are ordered by reference block index. */
@@ -397,7 +405,7 @@ static float code_dist(const spf_o_index *ix, const float *q, const uint8_t *cod
  * its exact distance is strictly smaller, as in the reference. */
 static uint32_t search_rabitq(const spf_o_index *ix, const float *q, uint32_t k,
                               const uint32_t *probes, uint32_t np, uint64_t *out_ids,
-                              float *out_dists) {
+                              float *out_dists, int filtered, uint64_t mask) {
     uint32_t d = ix->dim, P = ix->rq_padded;
     uint64_t *planes = (uint64_t *)malloc(sizeof(uint64_t) * 4 * (P / 64));
     float consts[5];
@@ -408,7 +416,7 @@ static uint32_t search_rabitq(const spf_o_index *ix, const float *q, uint32_t k,
         spf_o_rabitq_query(q, ix->centroids + (size_t)p * d, d, P, ix->rq_signs, planes, consts);
         for (uint64_t e = ix->post_off[p]; e < ix->post_off[p + 1]; e++) {
             uint64_t id = ix->ids[e];
-            if (is_deleted(ix, id)) continue;
+            if (skipped(ix, id, filtered, mask)) continue;
             float est, err;
             spf_o_rabitq_dist(planes, consts, ix->codes + e * ix->code_bytes, P, &est, &err);
             float kth = cnt < k ? INFINITY : heap[cnt - 1].dist;
@@ -448,8 +456,8 @@ static uint32_t search_rabitq(const spf_o_index *ix, const float *q, uint32_t k,
 
 /* Inner::search (src/spfresh.rs:671-727).  The top-`want` heap is restated as
  * "the `want` smallest by (dist, id)". */
-uint32_t spf_o_search(const spf_o_index *ix, const float *q, uint32_t k, uint32_t n_probe,
-                      uint64_t *out_ids, float *out_dists) {
+static uint32_t search_impl(const spf_o_index *ix, const float *q, uint32_t k, uint32_t n_probe,
+                            uint64_t *out_ids, float *out_dists, int filtered, uint64_t mask) {
     uint32_t d = ix->dim;
     uint32_t n = n_probe < 1 ? 1 : n_probe;
     if (n > ix->n_post) n = ix->n_post;
@@ -459,7 +467,7 @@ uint32_t spf_o_search(const spf_o_index *ix, const float *q, uint32_t k, uint32_
     uint32_t np = spf_o_probe(ix, q, n, probes, pd);
     free(pd);
     if (ix->quant == SPF_O_RABITQ && (ix->metric == SPF_O_L2 || ix->metric == SPF_O_L2SQ)) {
-        uint32_t r = search_rabitq(ix, q, k, probes, np, out_ids, out_dists);
+        uint32_t r = search_rabitq(ix, q, k, probes, np, out_ids, out_dists, filtered, mask);
         free(probes);
         return r;
     }
@@ -484,7 +492,7 @@ uint32_t spf_o_search(const spf_o_index *ix, const float *q, uint32_t k, uint32_
         uint32_t p = probes[i];
         for (uint64_t e = ix->post_off[p]; e < ix->post_off[p + 1]; e++) {
             uint64_t id = ix->ids[e];
-            if (is_deleted(ix, id)) continue;
+            if (skipped(ix, id, filtered, mask)) continue;
             float dd = code_dist(ix, q, ix->codes + e * ix->code_bytes, p, &pp);
             c[m].key = fkey(dd);
             c[m].id = id;
@@ -516,10 +524,21 @@ uint32_t spf_o_search(const spf_o_index *ix, const float *q, uint32_t k, uint32_
     return out;
 }
 
+uint32_t spf_o_search(const spf_o_index *ix, const float *q, uint32_t k, uint32_t n_probe,
+                      uint64_t *out_ids, float *out_dists) {
+    return search_impl(ix, q, k, n_probe, out_ids, out_dists, 0, 0);
+}
+
+uint32_t spf_o_search_filtered(const spf_o_index *ix, const float *q, uint64_t mask, uint32_t k,
+                               uint32_t n_probe, uint64_t *out_ids, float *out_dists) {
+    return search_impl(ix, q, k, n_probe, out_ids, out_dists, 1, mask);
+}
+
 /* ---- batch: one task per query on a thread pool (src/spfresh.rs:897-908) ---- */
 typedef struct {
     const spf_o_index *ix;
     const float *q;
+    const uint64_t *masks;      /* NULL = unfiltered */
     uint32_t nq, k, n_probe;
     uint64_t *ids;
     float *dists;
@@ -536,16 +555,17 @@ static void *batch_worker(void *arg) {
         if (i < b->nq) b->next = i + 1;
         pthread_mutex_unlock(&b->mu);
         if (i >= b->nq) break;
-        b->counts[i] = spf_o_search(b->ix, b->q + (size_t)i * b->ix->dim, b->k, b->n_probe,
-                                    b->ids + (size_t)i * b->k, b->dists + (size_t)i * b->k);
+        b->counts[i] = search_impl(b->ix, b->q + (size_t)i * b->ix->dim, b->k, b->n_probe,
+                                   b->ids + (size_t)i * b->k, b->dists + (size_t)i * b->k,
+                                   b->masks != NULL, b->masks ? b->masks[i] : 0);
     }
     return NULL;
 }
 
-void spf_o_search_batch(const spf_o_index *ix, const float *q, uint32_t nq, uint32_t k,
-                        uint32_t n_probe, uint32_t n_threads, uint64_t *out_ids,
-                        float *out_dists, uint32_t *out_counts) {
-    batch_t b = {ix, q, nq, k, n_probe, out_ids, out_dists, out_counts, 0,
+static void batch_run(const spf_o_index *ix, const float *q, const uint64_t *masks, uint32_t nq,
+                      uint32_t k, uint32_t n_probe, uint32_t n_threads, uint64_t *out_ids,
+                      float *out_dists, uint32_t *out_counts) {
+    batch_t b = {ix, q, masks, nq, k, n_probe, out_ids, out_dists, out_counts, 0,
                  PTHREAD_MUTEX_INITIALIZER};
     if (n_threads < 1) n_threads = 1;
     if (n_threads == 1) {
@@ -556,4 +576,16 @@ void spf_o_search_batch(const spf_o_index *ix, const float *q, uint32_t nq, uint
     for (uint32_t t = 0; t < n_threads; t++) pthread_create(&th[t], NULL, batch_worker, &b);
     for (uint32_t t = 0; t < n_threads; t++) pthread_join(th[t], NULL);
     free(th);
+}
+
+void spf_o_search_batch(const spf_o_index *ix, const float *q, uint32_t nq, uint32_t k,
+                        uint32_t n_probe, uint32_t n_threads, uint64_t *out_ids,
+                        float *out_dists, uint32_t *out_counts) {
+    batch_run(ix, q, NULL, nq, k, n_probe, n_threads, out_ids, out_dists, out_counts);
+}
+
+void spf_o_search_batch_filtered(const spf_o_index *ix, const float *q, const uint64_t *masks,
+                                 uint32_t nq, uint32_t k, uint32_t n_probe, uint32_t n_threads,
+                                 uint64_t *out_ids, float *out_dists, uint32_t *out_counts) {
+    batch_run(ix, q, masks, nq, k, n_probe, n_threads, out_ids, out_dists, out_counts);
 }

@@ -55,6 +55,7 @@ typedef struct spf_o_index {
     uint64_t n_raw;
     const uint8_t *deleted;     /* bitmap over ids < n_raw (may be NULL) */
     const float *rq_signs;      /* [3 * rq_padded] */
+    const uint8_t *labels;      /* [n_raw] label of each id for filtered search (may be NULL) */
 } spf_o_index;
 
 /* ---- scalar building blocks (exported so tests can pin each one) ---- */
@@ -92,9 +93,20 @@ void spf_o_search_batch(const spf_o_index *ix, const float *q, uint32_t nq, uint
                         uint32_t n_probe, uint32_t n_threads, uint64_t *out_ids,
                         float *out_dists, uint32_t *out_counts);
 
+/* Label-filtered search: Filter::LabelIn on one label field
+ * (src/filtered.rs:120-135) applied where the scan skips tombstones
+ * (src/spfresh.rs:707,745): an entry takes part only if bit labels[id] of the
+ * query's mask is set (labels >= 64 never match).  Same outputs as above. */
+uint32_t spf_o_search_filtered(const spf_o_index *ix, const float *q, uint64_t mask, uint32_t k,
+                               uint32_t n_probe, uint64_t *out_ids, float *out_dists);
+void spf_o_search_batch_filtered(const spf_o_index *ix, const float *q, const uint64_t *masks,
+                                 uint32_t nq, uint32_t k, uint32_t n_probe, uint32_t n_threads,
+                                 uint64_t *out_ids, float *out_dists, uint32_t *out_counts);
+
 /* Planted-mistake switch for the tests that show the oracle checks have teeth.
  * 0 = faithful.  1 = drop the last dimension from f32 L2 sums.  2 = ignore
- * tombstones.  3 = skip the probe_ratio cut.  4 = round f16 by truncation. */
+ * tombstones.  3 = skip the probe_ratio cut.  4 = round f16 by truncation.
+ * 5 = ignore the label filter. */
 void spf_o_set_fault(int fault);
 
 #ifdef __cplusplus

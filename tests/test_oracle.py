@@ -383,3 +383,61 @@ def test_planted_mistakes_are_caught(fault, check):
     finally:
         oracle.set_fault(0)
     assert check()
+
+
+# ---- label-filtered search (Filter::LabelIn, src/filtered.rs:120-135) ----
+def _labelled_index(quant=None, metric="l2", seed=0):
+    from pydiskann_amd import build_index
+    rng = np.random.default_rng(seed)
+    c = rng.normal(0, 2, size=(15, 16))
+    x = (c[rng.integers(0, 15, 2500)] + rng.normal(0, 1, size=(2500, 16))).astype(np.float32)
+    ix = build_index(x, metric, quant, n_postings=24, rerank_size=32 if quant else 0, seed=0)
+    ix.labels = rng.integers(0, 64, size=ix.n_raw).astype(np.uint8)
+    q = (c[rng.integers(0, 15, 40)] + rng.normal(0, 1, size=(40, 16))).astype(np.float32)
+    masks = np.zeros(40, dtype=np.uint64)
+    for i in range(40):
+        for b in rng.choice(64, size=rng.integers(1, 9), replace=False):
+            masks[i] |= np.uint64(1) << np.uint64(b)
+    return ix, q, masks
+
+
+def test_filtered_exhaustive_equals_fp64_over_matching_ids():
+    from pydiskann_amd import brute_force_f64
+    ix, q, masks = _labelled_index()
+    ids, dd, cnt = oracle.Oracle(ix).search_batch(q, 10, ix.n_post, masks=masks)
+    for i in range(q.shape[0]):
+        match = ((masks[i] >> ix.labels.astype(np.uint64)) & np.uint64(1)).astype(bool)
+        gt, gd = brute_force_f64(ix.raw, q[i:i + 1], 10, ix.metric, live=match)
+        c = int(cnt[i])
+        assert c == min(10, int(match.sum()))
+        assert match[ids[i, :c].astype(np.int64)].all()
+        np.testing.assert_allclose(dd[i, :c], gd[0, :c], rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("quant", [None, "int8", "rabitq"])
+def test_filter_equals_tombstoning_the_rest(quant):
+    ix, q, masks = _labelled_index(quant)
+    o = oracle.Oracle(ix)
+    for i in range(0, 40, 7):
+        a = o.search_batch(q[i:i + 1], 10, 6, masks=masks[i:i + 1])
+        ix2, _, _ = _labelled_index(quant)
+        match = ((masks[i] >> ix2.labels.astype(np.uint64)) & np.uint64(1)).astype(bool)
+        ix2.set_deleted(np.nonzero(~match)[0].astype(np.uint64))
+        b = oracle.Oracle(ix2).search_batch(q[i:i + 1], 10, 6)
+        for u, v in zip(a, b):
+            np.testing.assert_array_equal(u, v)
+
+
+def test_all_labels_mask_is_unfiltered_and_fault_is_caught():
+    ix, q, masks = _labelled_index()
+    o = oracle.Oracle(ix)
+    full = np.full(40, np.uint64((1 << 64) - 1))
+    a = o.search_batch(q, 10, 6, masks=full)
+    b = o.search_batch(q, 10, 6)
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
+    oracle.set_fault(5)
+    ids, _, cnt = o.search_batch(q, 10, 6, masks=masks)
+    bad = any(not ((masks[i] >> np.uint64(ix.labels[int(j)])) & np.uint64(1))
+              for i in range(40) for j in ids[i, :cnt[i]])
+    assert bad   # ignoring the filter returns non-matching ids
