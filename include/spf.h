@@ -79,6 +79,7 @@ typedef struct spf_index_desc {
     float probe_ratio;              /* SPFreshConfig::probe_ratio (inf = off) */
     uint32_t rerank_size;           /* SPFreshConfig::rerank_size */
     int32_t device;                 /* HIP device ordinal */
+    const uint8_t *labels;          /* [n_raw] label of each id for filtered search (NULL = none) */
 } spf_index_desc;
 
 #define SPF_FLAG_PROFILE 1u     /* time each stage with HIP events on the call's stream;
@@ -141,13 +142,33 @@ int spf_search_batch(spf_index *ix, const float *queries, uint32_t nq, uint32_t 
                      uint64_t *out_ids, float *out_dists, uint32_t *out_counts);
 
 /* Same, with queries and outputs already in device memory, enqueued on
- * `stream` (a hipStream_t; NULL = the library's stream for this call).  The
- * call returns after the work is enqueued and the host-side planning has
- * synchronised once; outputs are final when `stream` completes. */
+ * `stream` (a hipStream_t).  The call returns after the work is enqueued and
+ * the host-side planning has synchronised once; outputs are final when
+ * `stream` completes.  stream NULL = the library's own stream, and the call
+ * returns with the outputs final. */
 int spf_search_batch_device(spf_index *ix, const float *d_queries, uint32_t nq, uint32_t dim,
                             uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
                             uint64_t *d_out_ids, float *d_out_dists, uint32_t *d_out_counts,
                             void *stream);
+
+/* Label-filtered search (Filter::LabelIn on one label field,
+ * src/filtered.rs:120-135, applied where the scan skips tombstones,
+ * src/spfresh.rs:707,745): query i only sees entries whose id's label l
+ * (spf_index_desc.labels) has bit l set in masks[i]; labels >= 64 never match.
+ * Otherwise identical to spf_search_batch.  SPF_EINVAL if the index has no
+ * labels.  masks: host array [nq]. */
+int spf_search_batch_filtered(spf_index *ix, const float *queries, uint32_t nq, uint32_t dim,
+                              uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
+                              const uint64_t *masks, uint64_t *out_ids, float *out_dists,
+                              uint32_t *out_counts);
+
+/* Same with device-resident queries, masks and outputs (see
+ * spf_search_batch_device); d_masks NULL = unfiltered. */
+int spf_search_batch_filtered_device(spf_index *ix, const float *d_queries, uint32_t nq,
+                                     uint32_t dim, uint32_t k, uint32_t n_probe,
+                                     const spf_search_opts *opts, const uint64_t *d_masks,
+                                     uint64_t *d_out_ids, float *d_out_dists,
+                                     uint32_t *d_out_counts, void *stream);
 
 /* Insert routing: for each of n vectors, the reference block index of its
  * nearest live posting and the metric distance to that posting's centroid

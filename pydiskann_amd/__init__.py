@@ -18,7 +18,27 @@ from .index import IndexArrays, build_index, brute_force_f64, METRICS, QUANTS  #
 from ._lib import DeviceIndex, PREC_EXACT, PREC_MIXED, SpfError, lib  # noqa: F401
 from .refformat import read_index, write_index  # noqa: F401
 
-__all__ = ["SPFresh", "IndexArrays", "build_index", "DeviceIndex", "PREC_EXACT", "PREC_MIXED"]
+__all__ = ["SPFresh", "IndexArrays", "build_index", "DeviceIndex", "PREC_EXACT", "PREC_MIXED",
+           "label_masks"]
+
+MAX_LABEL = 64   # labels 0..63 are filterable; an id whose label is >= 64 never matches
+
+
+def label_masks(values, nq: int) -> np.ndarray:
+    """u64 [nq] masks for Filter::LabelIn (src/filtered.rs:65,124-126): bit l
+    set = label l allowed.  ``values``: one iterable of labels for every query,
+    or a list of nq iterables.  Values must lie in [0, 64)."""
+    vals = list(values)
+    per_query = len(vals) == nq and all(not np.isscalar(v) for v in vals) and nq > 0
+    groups = vals if per_query else [vals] * nq
+    out = np.zeros(nq, dtype=np.uint64)
+    for i, g in enumerate(groups):
+        for v in g:
+            v = int(v)
+            if not 0 <= v < MAX_LABEL:
+                raise ValueError(f"label value {v} outside [0, {MAX_LABEL})")
+            out[i] |= np.uint64(1) << np.uint64(v)
+    return out
 
 
 class SPFresh:
@@ -27,6 +47,7 @@ class SPFresh:
     def __init__(self, arrays: IndexArrays, path: str | None = None, device: int = 0):
         self.arrays = arrays
         self.path = path
+        self._dev_ordinal = device
         self._dev = DeviceIndex(arrays, device=device)
 
     # -- construction (python/src/lib.rs:308-319)
@@ -90,6 +111,32 @@ class SPFresh:
         q = np.asarray(queries, dtype=np.float32)
         self._check_dim(q.shape[1])
         return self._dev.search(q, k, n_probe, precision=precision, rerank_size=rerank_size)
+
+    # -- label filter (Filter::LabelIn semantics, src/filtered.rs:65,124-126;
+    #    the reference's SPFresh has none, so this surface is new)
+    def set_labels(self, labels) -> None:
+        """One label (u8) per id; re-creates the device mirror."""
+        lab = np.asarray(labels)
+        if lab.shape != (self.arrays.n_raw,) or lab.min(initial=0) < 0 or lab.max(initial=0) > 255:
+            raise ValueError("labels must be one value in [0, 255] per id")
+        self.arrays.labels = lab.astype(np.uint8)
+        dev = DeviceIndex(self.arrays, device=self._dev_ordinal)
+        self._dev.close()
+        self._dev = dev
+
+    def search_batch_filtered(self, queries, values, k: int = 10, n_probe: int = 8,
+                              precision: int = PREC_EXACT, rerank_size: int = 0):
+        """Search only ids whose label is in ``values`` (one set for every
+        query, or one per query): (ids, dists, counts) as
+        search_batch_with_dists."""
+        q = np.asarray(queries, dtype=np.float32)
+        if q.ndim != 2:
+            raise RuntimeError("queries must be a list of equal-length vectors")
+        self._check_dim(q.shape[1])
+        if self.arrays.labels is None:
+            raise RuntimeError("index has no labels (set_labels first)")
+        return self._dev.search(q, k, n_probe, precision=precision, rerank_size=rerank_size,
+                                masks=label_masks(values, q.shape[0]))
 
     # -- tombstones (python/src/lib.rs:324-325, 338)
     def delete(self, ids) -> None:

@@ -19,7 +19,8 @@ PREC_EXACT, PREC_MIXED = 0, 1
 
 # every entry point of include/spf.h
 EXPORTS = ("spf_index_create", "spf_index_destroy", "spf_index_get_info", "spf_delete",
-           "spf_search_batch", "spf_search_batch_device", "spf_nearest_postings",
+           "spf_search_batch", "spf_search_batch_device", "spf_search_batch_filtered",
+           "spf_search_batch_filtered_device", "spf_nearest_postings",
            "spf_last_profile", "spf_selftest", "spf_last_error", "spf_version")
 FLAG_PROFILE = 1
 FLAG_FULL_RERANK = 2
@@ -37,6 +38,7 @@ class IndexDesc(C.Structure):
         ("int8_scale", C.c_float), ("int8_offset", C.c_float),
         ("rabitq_padded", C.c_uint32), ("rabitq_signs", C.c_void_p),
         ("probe_ratio", C.c_float), ("rerank_size", C.c_uint32), ("device", C.c_int32),
+        ("labels", C.c_void_p),
     ]
 
 
@@ -91,6 +93,12 @@ def lib():
         L.spf_search_batch_device.argtypes = [P, P, u32, u32, u32, u32, C.POINTER(SearchOpts),
                                               P, P, P, P]
         L.spf_search_batch_device.restype = C.c_int
+        L.spf_search_batch_filtered.argtypes = [P, P, u32, u32, u32, u32, C.POINTER(SearchOpts),
+                                                P, P, P, P]
+        L.spf_search_batch_filtered.restype = C.c_int
+        L.spf_search_batch_filtered_device.argtypes = [P, P, u32, u32, u32, u32,
+                                                       C.POINTER(SearchOpts), P, P, P, P, P]
+        L.spf_search_batch_filtered_device.restype = C.c_int
         L.spf_nearest_postings.argtypes = [P, P, u32, u32, P, P]
         L.spf_nearest_postings.restype = C.c_int
         L.spf_last_profile.argtypes = [C.POINTER(Profile)]
@@ -140,6 +148,12 @@ class DeviceIndex:
         d.rabitq_padded, d.rabitq_signs = ix.rq_padded, _p(ix.rq_signs)
         d.probe_ratio = ix.probe_ratio if ix.probe_ratio is not None else math.inf
         d.rerank_size, d.device = ix.rerank_size, device
+        self._labels = None
+        if getattr(ix, "labels", None) is not None:
+            self._labels = np.ascontiguousarray(ix.labels, dtype=np.uint8)
+            if self._labels.shape[0] != ix.n_raw:
+                raise ValueError("labels must hold one u8 per id (n_raw)")
+            d.labels = _p(self._labels)
         h = C.c_void_p()
         check(lib().spf_index_create(C.byref(d), C.byref(h)))
         self.h = h
@@ -166,7 +180,9 @@ class DeviceIndex:
         check(lib().spf_delete(self.h, _p(a), a.shape[0]))
 
     def search(self, queries, k: int, n_probe: int, precision: int = PREC_EXACT,
-               rerank_size: int = 0, flags: int = 0):
+               rerank_size: int = 0, flags: int = 0, masks=None):
+        """Batch search; ``masks`` (u64 [nq], bit l = label l allowed) makes it
+        the label-filtered search."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         if q.ndim == 1:
             q = q[None, :]
@@ -176,8 +192,13 @@ class DeviceIndex:
         cnt = np.zeros(nq, dtype=np.uint32)
         o = SearchOpts()
         o.precision, o.rerank_size, o.flags = precision, rerank_size, flags
-        check(lib().spf_search_batch(self.h, _p(q), nq, dim, k, n_probe, C.byref(o), _p(ids),
-                                     _p(dd), _p(cnt)))
+        if masks is None:
+            check(lib().spf_search_batch(self.h, _p(q), nq, dim, k, n_probe, C.byref(o),
+                                         _p(ids), _p(dd), _p(cnt)))
+        else:
+            m = np.ascontiguousarray(np.broadcast_to(np.asarray(masks, dtype=np.uint64), (nq,)))
+            check(lib().spf_search_batch_filtered(self.h, _p(q), nq, dim, k, n_probe,
+                                                  C.byref(o), _p(m), _p(ids), _p(dd), _p(cnt)))
         return ids, dd, cnt
 
     def nearest_postings(self, vectors):
@@ -195,12 +216,20 @@ class DeviceIndex:
     def search_device(self, q_ptr: int, nq: int, dim: int, k: int, n_probe: int, ids_ptr: int,
                       dists_ptr: int, counts_ptr: int, stream: int = 0,
                       precision: int = PREC_EXACT, rerank_size: int = 0,
-                      profile: bool = False) -> None:
-        """Device-resident form: raw device pointers (e.g. torch .data_ptr())."""
+                      profile: bool = False, masks_ptr: int = 0) -> None:
+        """Device-resident form: raw device pointers (e.g. torch .data_ptr());
+        ``masks_ptr`` (u64 [nq] on the device) makes it the filtered search."""
         o = SearchOpts()
         o.precision, o.rerank_size = precision, rerank_size
         o.flags = FLAG_PROFILE if profile else 0
-        check(lib().spf_search_batch_device(self.h, C.c_void_p(q_ptr), nq, dim, k, n_probe,
-                                            C.byref(o), C.c_void_p(ids_ptr),
-                                            C.c_void_p(dists_ptr), C.c_void_p(counts_ptr),
-                                            C.c_void_p(stream) if stream else None))
+        st = C.c_void_p(stream) if stream else None
+        if masks_ptr:
+            check(lib().spf_search_batch_filtered_device(
+                self.h, C.c_void_p(q_ptr), nq, dim, k, n_probe, C.byref(o),
+                C.c_void_p(masks_ptr), C.c_void_p(ids_ptr), C.c_void_p(dists_ptr),
+                C.c_void_p(counts_ptr), st))
+        else:
+            check(lib().spf_search_batch_device(self.h, C.c_void_p(q_ptr), nq, dim, k, n_probe,
+                                                C.byref(o), C.c_void_p(ids_ptr),
+                                                C.c_void_p(dists_ptr), C.c_void_p(counts_ptr),
+                                                st))

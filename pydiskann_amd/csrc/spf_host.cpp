@@ -90,6 +90,7 @@ struct Workspace {
     DBuf<uint32_t> pair_src;
     DBuf<float> eq16;
     DBuf<uint32_t> rr_stats, sel_bits, a_bits;
+    DBuf<uint64_t> qmask;
     uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
     hipEvent_t ev[SPF_N_STAGES + 1] = {};
 
@@ -117,6 +118,7 @@ struct Workspace {
         rr_stats.release();
         sel_bits.release();
         a_bits.release();
+        qmask.release();
         qn16.release();
         DBuf<float> *fs[] = {&qin, &qv_probe, &qv_scan, &qnorm, &out_d};
         for (auto *b : fs) b->release();
@@ -152,6 +154,7 @@ struct spf_index {
     float *d_cnorm = nullptr;
     uint32_t *d_post_off = nullptr, *d_post_len = nullptr;
     uint32_t *d_block_sorted = nullptr;   // reference block index of each (sorted) posting
+    uint8_t *d_label_e = nullptr;         // label of each entry (filtered search); null = none
     uint4 *d_codes = nullptr;
     uint32_t *d_rank = nullptr;
     uint64_t *d_rank_id = nullptr;
@@ -189,7 +192,8 @@ struct spf_index {
     ~spf_index() {
         void *ps[] = {d_cent, d_cnorm, d_post_off, d_post_len, d_block_sorted, d_codes, d_rank,
                       d_rank_id, d_raw, d_del, d_xnorm, d_xsq, d_rank_entry, d_raw_e, d_xn16,
-                      d_mu, d_cnbf, d_cent_rows, d_cbf, d_rq_signs, d_post_ex, d_post_xn};
+                      d_mu, d_cnbf, d_cent_rows, d_cbf, d_rq_signs, d_post_ex, d_post_xn,
+                      d_label_e};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         if (codes16_owned && d_codes16) (void)hipFree(d_codes16);
@@ -481,6 +485,15 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         HIPCHK(upload(&ix->d_codes, ct.data(), ct.size(), bytes));
     }
     HIPCHK(upload(&ix->d_rank, entry_rank.data(), entry_rank.size(), bytes));
+    if (d->labels) {   // label of each entry's id (the raw store covers every id)
+        std::vector<uint8_t> lab(std::max<uint64_t>(N, 1), 0);
+        for (uint64_t e = 0; e < N; e++) {
+            const uint64_t id = d->entry_ids[entry_src[e]];
+            if (id >= d->n_raw) return fail(SPF_EINVAL, "entry id beyond the label array");
+            lab[e] = d->labels[id];
+        }
+        HIPCHK(upload(&ix->d_label_e, lab.data(), lab.size(), bytes));
+    }
     HIPCHK(upload(&ix->d_rank_id, ix->rank_id.data(), ix->rank_id.size(), bytes));
     HIPCHK(upload(&ix->d_rank_entry, ix->rank_entry.data(), ix->rank_entry.size(), bytes));
     {
@@ -586,7 +599,7 @@ template <class Mark>
 int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_t k,
                  uint32_t npr, uint32_t want, uint64_t total, uint32_t n_items, uint64_t *d_ids,
                  float *d_dists, uint32_t *d_counts, Mark &mark, RqParams *rq, bool full_rerank,
-                 bool stats) {
+                 bool stats, const uint64_t *d_masks) {
     const uint32_t dpad32 = round_up(ix->dim, 32);
     HIPCHK(w->cand16.ensure(std::max<uint64_t>(total, 1)));
     HIPCHK(w->sel_bits.ensure(std::max<uint64_t>(total / 32, 1)));
@@ -617,6 +630,8 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     mp.del = ix->d_del;
     mp.metric = ix->metric;
     mp.echunk = mfma_chunk(dpad32 / 32);
+    mp.label_e = d_masks ? ix->d_label_e : nullptr;
+    mp.qmask = d_masks;
     mp.out16 = w->cand16.p;
     const uint32_t grid = xcd_grid(n_items, 16384);
     HIPCHK(mark(SPF_STAGE_SCAN));
@@ -696,8 +711,9 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
 
 int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, uint32_t nq,
                 uint32_t k, uint32_t n_probe, const spf_search_opts *opts, uint64_t *d_ids,
-                float *d_dists, uint32_t *d_counts) {
+                float *d_dists, uint32_t *d_counts, const uint64_t *d_masks) {
     const uint32_t P = ix->P;
+    if (d_masks && !ix->d_label_e) return fail(SPF_EINVAL, "filtered search on an index without labels");
     const uint32_t npr = std::min(std::max(n_probe, 1u), std::max(P, 1u));
     const bool mixed = opts && opts->precision == SPF_PREC_MIXED;
     if (opts && opts->precision > SPF_PREC_MIXED) return fail(SPF_EINVAL, "unknown precision");
@@ -969,11 +985,13 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         rq.pair_off = w->pair_off.p;
         rq.pair_src = w->pair_src.p;
         rq.pair_slot = w->pair_slot.p;
+        rq.label_e = d_masks ? ix->d_label_e : nullptr;
+        rq.qmask = d_masks;
     }
     if (mixed) {
         int rc = search_mixed(ix, w, s, nq, k, npr, want, total, n_items, d_ids, d_dists,
                               d_counts, mark, ix->rq_l2 ? &rq : nullptr,
-                              opts->flags & SPF_FLAG_FULL_RERANK, prof);
+                              opts->flags & SPF_FLAG_FULL_RERANK, prof, d_masks);
         if (rc) return rc;
     } else if (ix->rq_l2) {
         // search_rabitq: per-pair query prep, then one wave per query
@@ -1016,6 +1034,8 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         sp.pair_off = w->pair_off.p;
         sp.pair_q = w->pair_q.p;
         sp.pair_slot = w->pair_slot.p;
+        sp.label_e = d_masks ? ix->d_label_e : nullptr;
+        sp.qmask = d_masks;
         const uint32_t grid = xcd_grid(n_items, 16384);
         HIPCHK(launch_scan(ix->code_kind, ix->metric, false, G, sp, grid, s));
     }
@@ -1197,10 +1217,11 @@ int spf_delete(spf_index *ix, const uint64_t *ids, uint64_t n) {
     return SPF_OK;
 }
 
-int spf_search_batch_device(spf_index *ix, const float *d_queries, uint32_t nq, uint32_t dim,
-                            uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
-                            uint64_t *d_out_ids, float *d_out_dists, uint32_t *d_out_counts,
-                            void *stream) {
+int spf_search_batch_filtered_device(spf_index *ix, const float *d_queries, uint32_t nq,
+                                     uint32_t dim, uint32_t k, uint32_t n_probe,
+                                     const spf_search_opts *opts, const uint64_t *d_masks,
+                                     uint64_t *d_out_ids, float *d_out_dists,
+                                     uint32_t *d_out_counts, void *stream) {
     int rc = check_search_args(ix, dim, k);
     if (rc) return rc;
     if (nq == 0) return SPF_OK;
@@ -1209,14 +1230,25 @@ int spf_search_batch_device(spf_index *ix, const float *d_queries, uint32_t nq, 
     if (!w) return fail(SPF_EDEVICE, "workspace creation failed");
     hipStream_t s = stream ? (hipStream_t)stream : w->stream;
     rc = search_impl(ix, w, s, d_queries, nq, k, n_probe, opts, d_out_ids, d_out_dists,
-                     d_out_counts);
+                     d_out_counts, d_masks);
+    if (rc == SPF_OK && !stream && hipStreamSynchronize(s) != hipSuccess)
+        rc = fail(SPF_EDEVICE, "stream synchronize failed");
     release_ws(ix, w);
     return rc;
 }
 
-int spf_search_batch(spf_index *ix, const float *queries, uint32_t nq, uint32_t dim, uint32_t k,
-                     uint32_t n_probe, const spf_search_opts *opts, uint64_t *out_ids,
-                     float *out_dists, uint32_t *out_counts) {
+int spf_search_batch_device(spf_index *ix, const float *d_queries, uint32_t nq, uint32_t dim,
+                            uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
+                            uint64_t *d_out_ids, float *d_out_dists, uint32_t *d_out_counts,
+                            void *stream) {
+    return spf_search_batch_filtered_device(ix, d_queries, nq, dim, k, n_probe, opts, nullptr,
+                                            d_out_ids, d_out_dists, d_out_counts, stream);
+}
+
+int spf_search_batch_filtered(spf_index *ix, const float *queries, uint32_t nq, uint32_t dim,
+                              uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
+                              const uint64_t *masks, uint64_t *out_ids, float *out_dists,
+                              uint32_t *out_counts) {
     int rc = check_search_args(ix, dim, k);
     if (rc) return rc;
     if (nq == 0) return SPF_OK;
@@ -1231,8 +1263,12 @@ int spf_search_batch(spf_index *ix, const float *queries, uint32_t nq, uint32_t 
         HIPCHK(w->out_d.ensure((size_t)nq * k));
         HIPCHK(w->out_c.ensure(nq));
         HIPCHK(hipMemcpyAsync(w->qin.p, queries, (size_t)nq * dim * 4, hipMemcpyHostToDevice, s));
+        if (masks) {
+            HIPCHK(w->qmask.ensure(nq));
+            HIPCHK(hipMemcpyAsync(w->qmask.p, masks, (size_t)nq * 8, hipMemcpyHostToDevice, s));
+        }
         int r = search_impl(ix, w, s, w->qin.p, nq, k, n_probe, opts, w->out_ids.p, w->out_d.p,
-                            w->out_c.p);
+                            w->out_c.p, masks ? w->qmask.p : nullptr);
         if (r) return r;
         HIPCHK(hipMemcpyAsync(out_ids, w->out_ids.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(out_dists, w->out_d.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, s));
@@ -1243,6 +1279,13 @@ int spf_search_batch(spf_index *ix, const float *queries, uint32_t nq, uint32_t 
     rc = run();
     release_ws(ix, w);
     return rc;
+}
+
+int spf_search_batch(spf_index *ix, const float *queries, uint32_t nq, uint32_t dim, uint32_t k,
+                     uint32_t n_probe, const spf_search_opts *opts, uint64_t *out_ids,
+                     float *out_dists, uint32_t *out_counts) {
+    return spf_search_batch_filtered(ix, queries, nq, dim, k, n_probe, opts, nullptr, out_ids,
+                                     out_dists, out_counts);
 }
 
 int spf_nearest_postings(spf_index *ix, const float *vectors, uint32_t n, uint32_t dim,

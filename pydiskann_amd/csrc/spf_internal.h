@@ -70,7 +70,15 @@ struct ScanParams {
     const uint32_t *pair_off;   // [n_post + 1] (query, slot) pairs per posting
     const uint32_t *pair_q;
     const uint64_t *pair_slot;  // candidate-buffer position of the posting's first entry
+    // label filter (listed mode): entry label, per-query mask; null = unfiltered
+    const uint8_t *label_e;
+    const uint64_t *qmask;
 };
+
+// entry of label l passes mask m (labels >= 64 never do)
+__device__ __forceinline__ bool label_ok(uint64_t m, uint32_t l) {
+    return l < 64u && ((m >> l) & 1ull);
+}
 
 struct PrepParams {
     const float *q;             // [nq][dim] caller queries
@@ -123,6 +131,8 @@ struct MixedParams {
     const uint32_t *del;
     uint32_t metric;            // key: L2 |q16-x16|^2, DOT 1-q16.x16, COSINE 1-cos(q16,x16)
     uint32_t echunk;            // entries per work item (64 x tiles per wave)
+    const uint8_t *label_e;     // label filter: entry label, per-query mask (null = off)
+    const uint64_t *qmask;
     uint16_t *out16;            // 16-bit candidate keys
 };
 
@@ -188,6 +198,8 @@ struct RqParams {
     const uint32_t *item_post, *item_off, *pair_off, *pair_src;
     const uint64_t *pair_slot;
     uint16_t *out16;
+    const uint8_t *label_e;     // label filter: entry label, per-query mask (null = off)
+    const uint64_t *qmask;
 };
 
 struct RqDecodeParams {

@@ -150,6 +150,7 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
     __shared__ uint32_t s_qi[G];
     __shared__ float s_qn[G];
     __shared__ uint32_t s_qs[G];
+    __shared__ uint64_t s_mask[G];
     constexpr bool INT_L2 = (CK == CK_U8) && (MET == M_L2 || MET == M_L2SQ);
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t dq4 = p.dpad >> 2;   // float4 (or u32 words) per staged query row
@@ -183,6 +184,7 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
             if (tid < gcnt) {
                 s_qi[tid] = p.pair_q[pr0 + tid];
                 s_slot[tid] = p.pair_slot[pr0 + tid];
+                if (p.qmask) s_mask[tid] = p.qmask[s_qi[tid]];
             }
         }
         __syncthreads();
@@ -314,9 +316,11 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
                 const bool dead = p.del && ((p.del[e >> 5] >> (e & 31)) & 1u);
                 const float xn = (MET == M_COS) ? p.xnorm[e] : 0.0f;
                 const uint32_t xs = INT_L2 ? p.xsq[e] : 0u;
+                const uint32_t lab = (!DENSE && p.qmask) ? p.label_e[e] : 0u;
 #pragma unroll
                 for (int g = 0; g < G; g++) {
                     if (g < (int)gcnt) {
+                        const bool skip = dead || (!DENSE && p.qmask && !label_ok(s_mask[g], lab));
                         float dist;
                         if (INT_L2) {
                             const uint32_t s = xs + s_qs[g] - 2u * acci[g];
@@ -328,7 +332,7 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
                             const float acc = (accf[g][0] + accf[g][1]) + (accf[g][2] + accf[g][3]);
                             dist = finish_f<MET>(acc, MET == M_COS ? s_qn[g] : 0.0f, xn);
                         }
-                        const uint64_t key = dead ? kNoKey : (((uint64_t)fkey(dist) << 32) | low);
+                        const uint64_t key = skip ? kNoKey : (((uint64_t)fkey(dist) << 32) | low);
                         p.out[s_slot[g] + (e - base_e)] = key;
                     }
                 }

@@ -85,6 +85,7 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
     __shared__ uint64_t s_slot[kMixQ];
     __shared__ uint32_t s_qi[kMixQ];
     __shared__ float s_qn[kMixQ];
+    __shared__ uint64_t s_mask[kMixQ];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 15, kg = lane >> 4;
     const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
@@ -108,11 +109,13 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
             if (p.metric == M_DOT) qn = 1.0f;
             else if (p.metric == M_COS) qn = qn > 0.0f ? 1.0f / sqrtf(qn) : 0.0f;
             s_qn[tid] = qn;
+            if (p.qmask) s_mask[tid] = ok ? p.qmask[q] : 0ull;
         }
         // the wave's B fragments, and per-entry norm / tombstone of its column
         uint4 bf[T][KS];
         float xn[T], rx[T];
         bool live[T];
+        uint32_t lab[T];
 #pragma unroll
         for (int t = 0; t < T; t++) {
             const uint32_t e = e_begin + 16 * (wave + 4 * t) + col;
@@ -125,6 +128,7 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
             }
             xn[t] = valid ? p.xn16[e] : 0.0f;
             live[t] = valid && !(p.del && ((p.del[e >> 5] >> (e & 31)) & 1u));
+            lab[t] = (valid && p.qmask) ? p.label_e[e] : 0u;
             rx[t] = p.metric == M_COS ? (xn[t] > 0.0f ? 1.0f / sqrtf(xn[t]) : 0.0f) : 1.0f;
         }
         __syncthreads();
@@ -163,12 +167,14 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
                         if (r0 < gcnt) {
                             const float d = l2 ? xn[t] + s_qn[r0] - 2.0f * acc0[j]
                                                : 1.0f - acc0[j] * s_qn[r0] * rx[t];
-                            p.out16[s_slot[r0] + off] = live[t] ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
+                            const bool ok = live[t] && (!p.qmask || label_ok(s_mask[r0], lab[t]));
+                            p.out16[s_slot[r0] + off] = ok ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
                         }
                         if (r1 < gcnt) {
                             const float d = l2 ? xn[t] + s_qn[r1] - 2.0f * acc1[j]
                                                : 1.0f - acc1[j] * s_qn[r1] * rx[t];
-                            p.out16[s_slot[r1] + off] = live[t] ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
+                            const bool ok = live[t] && (!p.qmask || label_ok(s_mask[r1], lab[t]));
+                            p.out16[s_slot[r1] + off] = ok ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
                         }
                     }
                 }

@@ -205,6 +205,7 @@ __global__ __launch_bounds__(64) void k_rq_search_exact(const RqParams p) {
     wave_sync_lds();
     float kth = __builtin_inff();
     const uint32_t np = p.np_q[q];
+    const uint64_t qm = p.qmask ? p.qmask[q] : ~0ull;
     for (uint32_t jp = 0; jp < np; jp++) {
         const uint32_t pair = q * p.npr + jp;
         const uint32_t post = p.probe_post[pair];
@@ -214,7 +215,8 @@ __global__ __launch_bounds__(64) void k_rq_search_exact(const RqParams p) {
         const uint32_t eb = p.post_off[post], ee = p.post_off[post + 1];
         for (uint32_t e0 = eb; e0 < ee; e0 += 64) {
             const uint32_t e = e0 + lane;
-            const bool valid = e < ee && !(p.del && ((p.del[e >> 5] >> (e & 31)) & 1u));
+            const bool valid = e < ee && !(p.del && ((p.del[e >> 5] >> (e & 31)) & 1u)) &&
+                               (!p.qmask || label_ok(qm, p.label_e[e]));
             float lb = __builtin_inff(), exact = 0.0f;
             if (valid) {
                 uint64_t words[W];
@@ -326,6 +328,7 @@ __global__ __launch_bounds__(256) void k_rq_scan(const RqParams p) {
     __shared__ uint64_t s_pl[kRqG][4 * W];
     __shared__ float s_cs[kRqG][8];
     __shared__ uint64_t s_slot[kRqG];
+    __shared__ uint64_t s_mask[kRqG];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t P = p.padded;
     for (XcdWalk it(p.n_items); it.more(); it.next()) {
@@ -347,7 +350,10 @@ __global__ __launch_bounds__(256) void k_rq_scan(const RqParams p) {
             const uint32_t g = tid / 8, c = tid - g * 8;
             s_cs[g][c] = p.consts[(size_t)p.pair_src[pr0 + g] * 8 + c];
         }
-        if (tid < gcnt) s_slot[tid] = p.pair_slot[pr0 + tid];
+        if (tid < gcnt) {
+            s_slot[tid] = p.pair_slot[pr0 + tid];
+            if (p.qmask) s_mask[tid] = p.qmask[p.pair_src[pr0 + tid] / p.npr];
+        }
         __syncthreads();
         for (uint32_t e0 = e_begin + 64 * wave; e0 < e_end; e0 += 256) {
             const uint32_t e = e0 + lane;
@@ -356,10 +362,12 @@ __global__ __launch_bounds__(256) void k_rq_scan(const RqParams p) {
             float n2, fac;
             rq_load_code<W>(p.codes, p.units, e, words, n2, fac);
             const bool dead = p.del && ((p.del[e >> 5] >> (e & 31)) & 1u);
+            const uint32_t lab = p.qmask ? p.label_e[e] : 0u;
             for (uint32_t g = 0; g < gcnt; g++) {
                 float est, err;
                 rq_est<W>(words, n2, fac, s_pl[g], s_cs[g], P, est, err);
-                p.out16[s_slot[g] + (e - pb)] = dead ? (uint16_t)0xffff : (uint16_t)(fkey_r(est) >> 16);
+                const bool skip = dead || (p.qmask && !label_ok(s_mask[g], lab));
+                p.out16[s_slot[g] + (e - pb)] = skip ? (uint16_t)0xffff : (uint16_t)(fkey_r(est) >> 16);
             }
         }
     }

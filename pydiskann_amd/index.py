@@ -76,7 +76,7 @@ class IndexArrays:
     rq_signs: np.ndarray = None    # f32 [3, padded]
     probe_ratio: float = math.inf
     rerank_size: int = 0
-    labels: np.ndarray = None      # optional u8/u32 label per id (filtered serving)
+    labels: np.ndarray = None      # optional u8 [n_raw] label per id (label-filtered search)
 
     def __post_init__(self):
         self.block_ids = np.ascontiguousarray(self.block_ids, dtype=np.uint32)

@@ -42,10 +42,11 @@ def test_every_declared_symbol_is_exported():
 def test_struct_layouts_match_the_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "spf.h"\n'
-                   "int main(void){printf(\"%zu %zu %zu %zu %zu %zu\\n\","
+                   "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu\\n\","
                    "sizeof(spf_index_desc), offsetof(spf_index_desc, probe_ratio),"
                    "sizeof(spf_search_opts), sizeof(spf_profile), sizeof(spf_index_info),"
-                   "offsetof(spf_index_desc, rabitq_signs));return 0;}\n")
+                   "offsetof(spf_index_desc, rabitq_signs),"
+                   "offsetof(spf_index_desc, labels));return 0;}\n")
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
@@ -53,7 +54,7 @@ def test_struct_layouts_match_the_header(tmp_path):
                                           check=True).stdout.split()]
     want = [C.sizeof(_lib.IndexDesc), _lib.IndexDesc.probe_ratio.offset,
             C.sizeof(_lib.SearchOpts), C.sizeof(_lib.Profile), C.sizeof(_lib.IndexInfo),
-            _lib.IndexDesc.rabitq_signs.offset]
+            _lib.IndexDesc.rabitq_signs.offset, _lib.IndexDesc.labels.offset]
     assert got == want
 
 

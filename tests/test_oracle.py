@@ -441,3 +441,24 @@ def test_all_labels_mask_is_unfiltered_and_fault_is_caught():
     bad = any(not ((masks[i] >> np.uint64(ix.labels[int(j)])) & np.uint64(1))
               for i in range(40) for j in ids[i, :cnt[i]])
     assert bad   # ignoring the filter returns non-matching ids
+
+
+def test_label_masks_follow_label_in_set_membership():
+    """label_masks + the oracle's mask test = `values.contains(label)` of
+    Filter::LabelIn (src/filtered.rs:124-126) for labels below 64."""
+    from pydiskann_amd import label_masks
+    m = label_masks({3, 0, 63}, 2)
+    assert m.tolist() == [(1 << 63) | 9] * 2
+    per = label_masks([[1], [], [5, 6]], 3)
+    assert per.tolist() == [2, 0, 96]
+    with pytest.raises(ValueError):
+        label_masks([64], 1)
+    with pytest.raises(ValueError):
+        label_masks([-1], 1)
+    ix, q, _ = _labelled_index()
+    sets = [set(range(i % 7, 64, 5)) for i in range(40)]
+    ids, _, cnt = oracle.Oracle(ix).search_batch(q, 20, ix.n_post, masks=label_masks(sets, 40))
+    for i in range(40):
+        assert all(int(ix.labels[int(j)]) in sets[i] for j in ids[i, :cnt[i]])
+        n_match = sum(int(l) in sets[i] for l in ix.labels)
+        assert int(cnt[i]) == min(20, n_match)
