@@ -33,7 +33,9 @@ extern "C" {
                                   "Query dim {} != index dim {}" (src/spfresh.rs:672-678) */
 #define SPF_EUNSUPPORTED (-3)  /* no device path (PQ postings): caller keeps its CPU search */
 #define SPF_EDEVICE (-4)       /* HIP runtime error */
-#define SPF_ENOMEM (-5)        /* device allocation failed */
+#define SPF_ENOMEM (-5)        /* device allocation failed, or a request above the
+                                  server's device-memory budget */
+#define SPF_EAGAIN (-6)        /* spf_server_wait: request not complete yet */
 
 /* metrics: the Distance impls of src/metric.rs:16-39 */
 #define SPF_METRIC_L2 0        /* DistL2    sqrt(sum (a-b)^2) */
@@ -178,6 +180,61 @@ int spf_search_batch_filtered_device(spf_index *ix, const float *d_queries, uint
  * and out_dist +inf when the index has no live posting. */
 int spf_nearest_postings(spf_index *ix, const float *vectors, uint32_t n, uint32_t dim,
                          uint32_t *out_block, float *out_dist);
+
+/* ---- Search server: continuous batching under a device-memory budget.
+ * No counterpart in the reference (its SPFresh serves one call at a time
+ * under a read lock, src/spfresh.rs:898); the serving loop of a filtered
+ * SPFresh deployment.  Requests (query rows with one k / n_probe / options /
+ * filter) queue in arrival order; each of n_slots workers (one HIP stream and
+ * one working-memory set each) takes the oldest request plus every later one
+ * with the same settings, up to max_batch rows, as one device batch, so
+ * batches of several slots run on the device at once.  A slot starts a batch
+ * only if the working memory all slots hold or have reserved plus the batch's
+ * upper bound stays within hbm_budget_bytes; requests whose single query
+ * exceeds the budget fail at submit with SPF_ENOMEM.  Every row's result is
+ * that of spf_search_batch(_filtered) on the same query.  The index must
+ * outlive the server and must not be modified (spf_delete) while requests
+ * are in flight. */
+typedef struct spf_server spf_server;
+
+typedef struct {
+    uint32_t n_slots;           /* concurrent device batches; 0 = 3 */
+    uint32_t max_batch;         /* rows per device batch; 0 = 1024 */
+    uint64_t hbm_budget_bytes;  /* working memory of all slots; 0 = 8 GiB */
+    uint32_t reserved[4];
+} spf_server_opts;
+
+typedef struct {
+    uint64_t submit_ns, start_ns, done_ns; /* steady clock: submit, first batch start, done */
+    uint32_t batches;                      /* device batches that carried its rows */
+    uint32_t reserved[3];
+} spf_request_info;
+
+typedef struct {
+    uint64_t requests_done, rows_done, batches, rejected;
+    uint64_t peak_reserved_bytes; /* max working memory held + reserved at once */
+    uint64_t budget_overruns;     /* batches whose memory exceeded their reservation (0) */
+    uint64_t admission_waits;     /* times the oldest request did not fit the budget */
+    uint64_t trims;               /* idle slots that gave their memory back */
+    uint32_t max_concurrent;      /* most batches on the device at once */
+    uint32_t reserved[3];
+} spf_server_stats;
+
+int spf_server_create(spf_index *ix, const spf_server_opts *opts, spf_server **out);
+/* Enqueue nq query rows (masks: NULL = unfiltered, else [nq] as in
+ * spf_search_batch_filtered).  All host buffers must stay valid until
+ * spf_server_wait returns for *ticket.  SPF_FLAG_PROFILE is ignored. */
+int spf_server_submit(spf_server *sv, const float *queries, uint32_t nq, uint32_t dim,
+                      uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
+                      const uint64_t *masks, uint64_t *out_ids, float *out_dists,
+                      uint32_t *out_counts, uint64_t *ticket);
+/* Wait for a request: timeout_us < 0 = until done; SPF_EAGAIN on timeout
+ * (ticket stays valid); otherwise the request's status, and the ticket is
+ * released.  info may be NULL. */
+int spf_server_wait(spf_server *sv, uint64_t ticket, int64_t timeout_us, spf_request_info *info);
+int spf_server_get_stats(spf_server *sv, spf_server_stats *out);
+/* Completes every submitted request, then frees the server. */
+void spf_server_destroy(spf_server *sv);
 
 /* Profile of the last search on this thread that set SPF_FLAG_PROFILE. */
 int spf_last_profile(spf_profile *out);

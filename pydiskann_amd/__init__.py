@@ -15,11 +15,11 @@ import math
 import numpy as np
 
 from .index import IndexArrays, build_index, brute_force_f64, METRICS, QUANTS  # noqa: F401
-from ._lib import DeviceIndex, PREC_EXACT, PREC_MIXED, SpfError, lib  # noqa: F401
+from ._lib import DeviceIndex, Server, PREC_EXACT, PREC_MIXED, SpfError, lib  # noqa: F401
 from .refformat import read_index, write_index  # noqa: F401
 
-__all__ = ["SPFresh", "IndexArrays", "build_index", "DeviceIndex", "PREC_EXACT", "PREC_MIXED",
-           "label_masks"]
+__all__ = ["SPFresh", "IndexArrays", "build_index", "DeviceIndex", "Server", "PREC_EXACT",
+           "PREC_MIXED", "label_masks"]
 
 MAX_LABEL = 64   # labels 0..63 are filterable; an id whose label is >= 64 never matches
 

@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import threading
 
 import numpy as np
 
@@ -15,12 +16,14 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libspf_hip.so")
 
 SPF_OK, SPF_EINVAL, SPF_EDIM, SPF_EUNSUPPORTED, SPF_EDEVICE, SPF_ENOMEM = 0, -1, -2, -3, -4, -5
+SPF_EAGAIN = -6
 PREC_EXACT, PREC_MIXED = 0, 1
 
 # every entry point of include/spf.h
 EXPORTS = ("spf_index_create", "spf_index_destroy", "spf_index_get_info", "spf_delete",
            "spf_search_batch", "spf_search_batch_device", "spf_search_batch_filtered",
-           "spf_search_batch_filtered_device", "spf_nearest_postings",
+           "spf_search_batch_filtered_device", "spf_nearest_postings", "spf_server_create",
+           "spf_server_submit", "spf_server_wait", "spf_server_get_stats", "spf_server_destroy",
            "spf_last_profile", "spf_selftest", "spf_last_error", "spf_version")
 FLAG_PROFILE = 1
 FLAG_FULL_RERANK = 2
@@ -59,6 +62,24 @@ class IndexInfo(C.Structure):
                 ("n_postings", C.c_uint32), ("n_entries", C.c_uint64), ("n_raw", C.c_uint64),
                 ("n_deleted", C.c_uint64), ("device_bytes", C.c_uint64),
                 ("max_posting", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("n_slots", C.c_uint32), ("max_batch", C.c_uint32),
+                ("hbm_budget_bytes", C.c_uint64), ("reserved", C.c_uint32 * 4)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("submit_ns", C.c_uint64), ("start_ns", C.c_uint64), ("done_ns", C.c_uint64),
+                ("batches", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests_done", C.c_uint64), ("rows_done", C.c_uint64),
+                ("batches", C.c_uint64), ("rejected", C.c_uint64),
+                ("peak_reserved_bytes", C.c_uint64), ("budget_overruns", C.c_uint64),
+                ("admission_waits", C.c_uint64), ("trims", C.c_uint64),
+                ("max_concurrent", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class SpfError(RuntimeError):
@@ -101,6 +122,17 @@ def lib():
         L.spf_search_batch_filtered_device.restype = C.c_int
         L.spf_nearest_postings.argtypes = [P, P, u32, u32, P, P]
         L.spf_nearest_postings.restype = C.c_int
+        L.spf_server_create.argtypes = [P, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]
+        L.spf_server_create.restype = C.c_int
+        L.spf_server_submit.argtypes = [P, P, u32, u32, u32, u32, C.POINTER(SearchOpts), P, P, P,
+                                        P, C.POINTER(C.c_uint64)]
+        L.spf_server_submit.restype = C.c_int
+        L.spf_server_wait.argtypes = [P, u64, C.c_int64, C.POINTER(RequestInfo)]
+        L.spf_server_wait.restype = C.c_int
+        L.spf_server_get_stats.argtypes = [P, C.POINTER(ServerStats)]
+        L.spf_server_get_stats.restype = C.c_int
+        L.spf_server_destroy.argtypes = [P]
+        L.spf_server_destroy.restype = None
         L.spf_last_profile.argtypes = [C.POINTER(Profile)]
         L.spf_last_profile.restype = C.c_int
         L.spf_selftest.argtypes = [C.c_int]
@@ -233,3 +265,81 @@ class DeviceIndex:
                                                 C.byref(o), C.c_void_p(ids_ptr),
                                                 C.c_void_p(dists_ptr), C.c_void_p(counts_ptr),
                                                 st))
+
+
+class Ticket:
+    """A submitted request: holds its input and output arrays until waited."""
+
+    def __init__(self, tid: int, q, masks, ids, dists, counts):
+        self.id, self.q, self.masks = tid, q, masks
+        self.ids, self.dists, self.counts = ids, dists, counts
+
+
+class Server:
+    """Continuous-batching search server over one DeviceIndex (spf_server_*):
+    requests from any thread are merged into device batches that run on
+    several streams at once, admitted by a device-memory budget."""
+
+    def __init__(self, dev: "DeviceIndex", n_slots: int = 0, max_batch: int = 0,
+                 hbm_budget: int = 0):
+        o = ServerOpts()
+        o.n_slots, o.max_batch, o.hbm_budget_bytes = n_slots, max_batch, hbm_budget
+        h = C.c_void_p()
+        check(lib().spf_server_create(dev.h, C.byref(o), C.byref(h)))
+        self.h, self.dev, self.dim = h, dev, dev.dim
+        self._live = {}   # submitted tickets: the device writes into their arrays
+        self._mu = threading.Lock()
+
+    def submit(self, queries, k: int, n_probe: int, precision: int = PREC_EXACT,
+               rerank_size: int = 0, flags: int = 0, masks=None) -> Ticket:
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim == 1:
+            q = q[None, :]
+        nq, dim = q.shape
+        m = None
+        if masks is not None:
+            m = np.ascontiguousarray(np.broadcast_to(np.asarray(masks, dtype=np.uint64), (nq,)))
+        ids = np.empty((nq, max(k, 1)), dtype=np.uint64)
+        dd = np.empty((nq, max(k, 1)), dtype=np.float32)
+        cnt = np.zeros(nq, dtype=np.uint32)
+        o = SearchOpts()
+        o.precision, o.rerank_size, o.flags = precision, rerank_size, flags
+        t = C.c_uint64()
+        check(lib().spf_server_submit(self.h, _p(q), nq, dim, k, n_probe, C.byref(o), _p(m),
+                                      _p(ids), _p(dd), _p(cnt), C.byref(t)))
+        tk = Ticket(t.value, q, m, ids, dd, cnt)
+        with self._mu:
+            self._live[tk.id] = tk
+        return tk
+
+    def wait(self, ticket: Ticket, timeout_us: int = -1):
+        """(ids, dists, counts, info) once the request is done; None if
+        timeout_us passes first."""
+        info = RequestInfo()
+        rc = lib().spf_server_wait(self.h, ticket.id, timeout_us, C.byref(info))
+        if rc == SPF_EAGAIN:
+            return None
+        with self._mu:
+            self._live.pop(ticket.id, None)
+        check(rc)
+        return ticket.ids, ticket.dists, ticket.counts, {
+            "submit_ns": int(info.submit_ns), "start_ns": int(info.start_ns),
+            "done_ns": int(info.done_ns), "batches": int(info.batches)}
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().spf_server_get_stats(self.h, C.byref(st)))
+        return {f: int(getattr(st, f)) for f, _ in ServerStats._fields_ if f != "reserved"}
+
+    def close(self) -> None:
+        """Completes every submitted request, then frees the server."""
+        if getattr(self, "h", None):
+            lib().spf_server_destroy(self.h)
+            self.h = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

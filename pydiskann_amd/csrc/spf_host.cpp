@@ -24,6 +24,7 @@
 
 #include "spf.h"
 #include "spf_internal.h"
+#include "spf_sched.h"
 
 using namespace spf;
 
@@ -67,69 +68,46 @@ struct DBuf {
     }
 };
 
+// every device buffer of a Workspace: (element type, name)
+#define SPF_WS_BUFFERS(X)                                                                  \
+    X(float, qin) X(float, qv_probe) X(float, qv_scan) X(float, qnorm) X(uint32_t, q8)    \
+    X(uint32_t, qsq) X(uint64_t, probe_keys) X(uint64_t, probe_sel) X(uint32_t, probe_cnt) \
+    X(uint32_t, np_q) X(uint32_t, probe_post) X(uint64_t, cand_len) X(uint64_t, cand_off)  \
+    X(uint64_t, qp_slot) X(uint64_t, pair_slot) X(uint32_t, post_cnt) X(uint32_t, qp_rel)  \
+    X(uint32_t, prank) X(uint32_t, pair_off) X(uint32_t, pair_q) X(uint32_t, item_cnt)     \
+    X(uint32_t, item_off) X(uint32_t, item_post) X(uint64_t, cand) X(uint64_t, sel)        \
+    X(uint64_t, fin) X(uint32_t, sel_cnt) X(uint32_t, fin_cnt) X(uint64_t, out_ids)        \
+    X(float, out_d) X(uint32_t, out_c) X(uint16_t, q16) X(uint16_t, cand16) X(uint16_t, qbf) \
+    X(uint16_t, probe16) X(float, qn16) X(float, qnbf) X(uint32_t, probe_pool)             \
+    X(uint32_t, probe_pool_cnt) X(uint32_t, pool_want) X(uint32_t, cand_cnt)               \
+    X(uint32_t, perm_cnt) X(uint32_t, perm_off) X(uint32_t, perm_fill) X(uint32_t, qperm)  \
+    X(uint64_t, rq_planes) X(float, rq_consts) X(uint32_t, pair_src) X(float, eq16)        \
+    X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)
+
 struct Workspace {
     hipStream_t stream = nullptr;
-    DBuf<float> qin, qv_probe, qv_scan, qnorm;
-    DBuf<uint32_t> q8, qsq;
-    DBuf<uint64_t> probe_keys, probe_sel;
-    DBuf<uint32_t> probe_cnt, np_q, probe_post;
-    DBuf<uint64_t> cand_len, cand_off, qp_slot, pair_slot;
-    DBuf<uint32_t> post_cnt, qp_rel, prank, pair_off, pair_q, item_cnt, item_off, item_post;
-    DBuf<uint64_t> cand, sel, fin;
-    DBuf<uint32_t> sel_cnt, fin_cnt;
-    DBuf<uint64_t> out_ids;
-    DBuf<float> out_d;
-    DBuf<uint32_t> out_c;
-    DBuf<uint16_t> q16, cand16, qbf, probe16, pool_key;
-    DBuf<float> qn16, qnbf;
-    DBuf<uint32_t> probe_pool, probe_pool_cnt;
-    DBuf<uint32_t> pool_entry, pool_want, cand_cnt, perm_cnt, perm_off, perm_fill, qperm;
-    DBuf<uint64_t> pool;
-    DBuf<uint64_t> rq_planes;
-    DBuf<float> rq_consts;
-    DBuf<uint32_t> pair_src;
-    DBuf<float> eq16;
-    DBuf<uint32_t> rr_stats, sel_bits, a_bits;
-    DBuf<uint64_t> qmask;
+#define SPF_WS_DECL(T, n) DBuf<T> n;
+    SPF_WS_BUFFERS(SPF_WS_DECL)
+#undef SPF_WS_DECL
     uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
     hipEvent_t ev[SPF_N_STAGES + 1] = {};
 
+    // device bytes the buffers hold now
+    uint64_t bytes() const {
+        uint64_t b = 0;
+#define SPF_WS_BYTES(T, n) b += (uint64_t)n.cap * sizeof(T);
+        SPF_WS_BUFFERS(SPF_WS_BYTES)
+#undef SPF_WS_BYTES
+        return b;
+    }
+    // free the device buffers (the stream, events and pinned words stay)
+    void trim() {
+#define SPF_WS_FREE(T, n) n.release();
+        SPF_WS_BUFFERS(SPF_WS_FREE)
+#undef SPF_WS_FREE
+    }
     void release() {
-        q16.release();
-        cand16.release();
-        qbf.release();
-        probe16.release();
-        pool_key.release();
-        qnbf.release();
-        probe_pool.release();
-        probe_pool_cnt.release();
-        pool_entry.release();
-        cand_cnt.release();
-        perm_cnt.release();
-        perm_off.release();
-        perm_fill.release();
-        qperm.release();
-        pool_want.release();
-        pool.release();
-        rq_planes.release();
-        rq_consts.release();
-        pair_src.release();
-        eq16.release();
-        rr_stats.release();
-        sel_bits.release();
-        a_bits.release();
-        qmask.release();
-        qn16.release();
-        DBuf<float> *fs[] = {&qin, &qv_probe, &qv_scan, &qnorm, &out_d};
-        for (auto *b : fs) b->release();
-        DBuf<uint32_t> *us[] = {&q8, &qsq, &probe_cnt, &np_q, &probe_post, &post_cnt, &qp_rel, &prank,
-                                &pair_off, &pair_q, &item_cnt, &item_off, &item_post, &sel_cnt,
-                                &fin_cnt,
-                                &out_c};
-        for (auto *b : us) b->release();
-        DBuf<uint64_t> *ls[] = {&probe_keys, &probe_sel, &cand_len, &cand_off, &qp_slot,
-                                &pair_slot, &cand, &sel, &fin, &out_ids};
-        for (auto *b : ls) b->release();
+        trim();
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         if (h_summary) (void)hipHostFree(h_summary);
@@ -180,6 +158,10 @@ struct spf_index {
     uint32_t rq_padded = 0;
     bool rq_l2 = false;
     uint64_t device_bytes = 0;
+    // working-memory bound inputs: top_len[j] = entries in the j longest
+    // postings; posting chunk counts at the VALU and matrix-core chunk sizes
+    std::vector<uint64_t> top_len;
+    uint64_t chunks_v = 0, chunks_m = 0;
 
     std::vector<uint64_t> rank_id;      // ids in ascending order
     std::vector<uint32_t> rank_entry;   // entry holding each rank
@@ -233,6 +215,17 @@ std::vector<uint4> tile_rows(const uint8_t *rows, uint64_t n, uint32_t row_bytes
     return t;
 }
 
+Workspace *new_workspace() {
+    Workspace *w = new Workspace();
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&w->h_summary, 2 * sizeof(uint64_t)) != hipSuccess) {
+        w->release();
+        delete w;
+        return nullptr;
+    }
+    return w;
+}
+
 Workspace *acquire(spf_index *ix) {
     {
         std::lock_guard<std::mutex> g(ix->ws_mu);
@@ -242,14 +235,7 @@ Workspace *acquire(spf_index *ix) {
             return w;
         }
     }
-    Workspace *w = new Workspace();
-    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void **)&w->h_summary, 2 * sizeof(uint64_t)) != hipSuccess) {
-        w->release();
-        delete w;
-        return nullptr;
-    }
-    return w;
+    return new_workspace();
 }
 
 void release_ws(spf_index *ix, Workspace *w) {
@@ -439,6 +425,17 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         HIPCHK(upload(&ix->d_block_sorted, bs.data(), bs.size(), bytes));
     }
     HIPCHK(upload(&ix->d_post_len, post_len.data(), post_len.size(), bytes));
+    {
+        std::vector<uint32_t> sorted(post_len);
+        std::sort(sorted.begin(), sorted.end(), std::greater<uint32_t>());
+        ix->top_len.assign(P + 1, 0);
+        for (uint32_t i = 0; i < P; i++) ix->top_len[i + 1] = ix->top_len[i] + sorted[i];
+        const uint32_t cm = mfma_chunk(round_up(d->dim, 32) / 32);
+        for (uint32_t len : post_len) {
+            ix->chunks_v += (len + kEChunk - 1) / kEChunk;
+            ix->chunks_m += (len + cm - 1) / cm;
+        }
+    }
     if (d->quantizer == SPF_QUANT_RABITQ) {
         HIPCHK(upload(&ix->d_rq_signs, d->rabitq_signs, 3 * (size_t)ix->rq_padded, bytes));
         const uint32_t bit_units = (d->code_bytes + 15) / 16;
@@ -709,45 +706,162 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     return SPF_OK;
 }
 
-int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, uint32_t nq,
-                uint32_t k, uint32_t n_probe, const spf_search_opts *opts, uint64_t *d_ids,
-                float *d_dists, uint32_t *d_counts, const uint64_t *d_masks) {
+// Launch shape of one search call: derived from the index and the options
+// alone (not from the queries), so the working-memory bound can use it too.
+struct Shape {
+    uint32_t npr, want, dpad, dpad32, Gv, G, echunk, band, Pp;
+    bool mixed, rerank, mfma_scan, mprobe, round16, q8;
+};
+
+int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
+                 Shape *sh) {
     const uint32_t P = ix->P;
-    if (d_masks && !ix->d_label_e) return fail(SPF_EINVAL, "filtered search on an index without labels");
-    const uint32_t npr = std::min(std::max(n_probe, 1u), std::max(P, 1u));
-    const bool mixed = opts && opts->precision == SPF_PREC_MIXED;
+    Shape &z = *sh;
+    z.npr = std::min(std::max(n_probe, 1u), std::max(P, 1u));
+    z.mixed = opts && opts->precision == SPF_PREC_MIXED;
     if (opts && opts->precision > SPF_PREC_MIXED) return fail(SPF_EINVAL, "unknown precision");
-    if (mixed && !ix->rq_l2 && (!ix->d_codes16 || ix->dim > 256))
+    if (z.mixed && !ix->rq_l2 && (!ix->d_codes16 || ix->dim > 256))
         return fail(SPF_EUNSUPPORTED,
                     "mixed precision serves f32 and f16 postings with dim <= 256 and RaBitQ "
                     "postings with an L2 metric");
     // RaBitQ + L2 takes search_rabitq (src/spfresh.rs:697-703): no rerank_size
-    const bool rerank =
-        !mixed && !ix->rq_l2 && ix->quant != SPF_QUANT_NONE && ix->rerank_size > 0;
-    if (ix->rq_l2 && !mixed && k > 1024)
+    z.rerank = !z.mixed && !ix->rq_l2 && ix->quant != SPF_QUANT_NONE && ix->rerank_size > 0;
+    if (ix->rq_l2 && !z.mixed && k > 1024)
         return fail(SPF_EINVAL, "k above 1024 on RaBitQ postings");
-    uint32_t want = rerank ? std::max(k, ix->rerank_size) : k;
-    if (mixed) {
+    z.want = z.rerank ? std::max(k, ix->rerank_size) : k;
+    if (z.mixed) {
         const uint32_t pool = opts->rerank_size ? opts->rerank_size
                               : ix->rerank_size ? ix->rerank_size : 1000;
-        want = std::max(k, pool);
+        z.want = std::max(k, pool);
     }
-    if (want > kMaxK) return fail(SPF_EINVAL, "k / rerank_size above 2048");
-    const uint32_t dpad = ix->dpad;
-    const uint32_t dpad32 = round_up(ix->dim, 32);
+    if (z.want > kMaxK) return fail(SPF_EINVAL, "k / rerank_size above 2048");
+    z.dpad = ix->dpad;
+    z.dpad32 = round_up(ix->dim, 32);
     // queries per VALU scan item (SPF_SCAN_G overrides: 4, 8 or 16)
     static const uint32_t g_env = [] {
         const char *e = std::getenv("SPF_SCAN_G");
         const uint32_t v = e ? (uint32_t)std::atoi(e) : 0u;
         return (v == 4 || v == 8 || v == 16) ? v : 0u;
     }();
-    const uint32_t Gv = dpad > 1024 ? 4 : (g_env ? g_env : 8);
+    z.Gv = z.dpad > 1024 ? 4 : (g_env ? g_env : 8);
     // queries and entries per posting-scan item: the matrix-core scan takes
     // kMixQ queries over mfma_chunk entries, the RaBitQ estimator scan 8
     // queries, the VALU scan Gv; both of the latter over kEChunk entries
-    const bool mfma_scan = mixed && !ix->rq_l2;
-    const uint32_t G = mfma_scan ? kMixQ : mixed ? 8u : Gv;
-    const uint32_t echunk = mfma_scan ? mfma_chunk(dpad32 / 32) : kEChunk;
+    z.mfma_scan = z.mixed && !ix->rq_l2;
+    z.G = z.mfma_scan ? kMixQ : z.mixed ? 8u : z.Gv;
+    z.echunk = z.mfma_scan ? mfma_chunk(z.dpad32 / 32) : kEChunk;
+    z.round16 = ix->quant == SPF_QUANT_F16 &&
+                (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
+    z.q8 = ix->quant == SPF_QUANT_INT8 &&
+           (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
+    // matrix-core probe for L2 metrics when it has something to prune
+    z.mprobe = z.mixed && (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ) &&
+               P > z.npr && ix->dim <= 256 && z.npr + z.npr / 4 + 32 <= kMaxK;
+    z.Pp = round_up(P, 32);
+    z.band = std::min<uint32_t>(P, std::min<uint32_t>(kMaxK, std::max(z.npr + 32, z.npr + z.npr / 4)));
+    return SPF_OK;
+}
+
+// Upper bound on the device bytes one search call of nq queries adds to a
+// Workspace: every buffer it ensures, at DBuf's growth rule, with the
+// candidate count bounded by nq x the npr longest postings and the work-item
+// count by the posting chunk counts.  host_io adds the staging buffers of the
+// host-memory entry points.
+uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, bool host_io,
+                  bool filtered) {
+    const uint64_t q = nq, P = ix->P, npr = z.npr, pairs = q * npr;
+    uint64_t b = 0;
+    auto add = [&](uint64_t n, uint64_t size) { b += (n + n / 4 + 64) * size; };
+    if (host_io) {
+        add(q * ix->dim, 4);
+        add(q * k, 8);
+        add(q * k, 4);
+        add(q, 4);
+        if (filtered) add(q, 8);
+    }
+    add(q * z.dpad, 4);          // qv_probe
+    add(q, 4);                   // qnorm
+    if (z.round16) add(q * z.dpad, 4);
+    add(q * z.dpad / 4, 4);      // q8
+    add(q, 4);                   // qsq
+    add(q * z.want, 8);          // sel
+    add(q, 4);                   // sel_cnt
+    if (z.mixed) {
+        add(q * z.dpad32, 2);
+        add(q, 4);
+        add(q, 4);
+    }
+    if (z.mprobe) {
+        add(q * z.dpad32, 2);
+        add(q, 4);
+    }
+    add(pairs, 8);               // probe_sel
+    add(q, 4);                   // probe_cnt
+    if (z.mprobe) {
+        add(q * z.Pp, 2);
+        add(q * z.band, 4);
+        add(q, 4);
+    } else {
+        add(q * P, 8);           // probe_keys
+    }
+    add(q, 4);                   // np_q
+    add(pairs, 4);               // probe_post
+    add(q + 1, 8);               // cand_len
+    add(q, 4);                   // cand_cnt
+    add(q + 1, 8);               // cand_off
+    add(P + 1, 4);               // post_cnt
+    add(pairs, 4);               // qp_rel
+    add(pairs, 4);               // prank
+    add(P + 1, 4);               // pair_off
+    add(pairs, 8);               // qp_slot
+    add(pairs, 4);               // pair_q
+    add(pairs, 8);               // pair_slot
+    if (ix->rq_l2) add(pairs, 4);
+    add(P + 1, 4);               // item_cnt
+    add(P + 1, 4);               // item_off
+    // items of posting p = ceil(pairs_p / G) * ceil(len_p / echunk), pairs_p <= nq
+    const uint64_t cmax = (ix->max_len + z.echunk - 1) / z.echunk;
+    const uint64_t call = z.echunk == kEChunk ? ix->chunks_v : ix->chunks_m;
+    const uint64_t items = (pairs + z.G - 1) / z.G * cmax + std::min(call, pairs * cmax);
+    add(std::max<uint64_t>(items, 1), 4);   // item_post
+    if (ix->rq_l2) {
+        add(pairs * 4 * (ix->rq_padded / 64), 8);
+        add(pairs * 8, 4);
+    }
+    const uint64_t total = q * ix->top_len[std::min<uint64_t>(npr, P)];
+    if (z.mixed) {
+        add(std::max<uint64_t>(total, 1), 2);       // cand16
+        add(std::max<uint64_t>(total / 32, 1), 4);  // sel_bits
+        add(std::max<uint64_t>(total / 32, 1), 4);  // a_bits
+        add(q, 4);                                  // pool_want
+        add(q * k, 8);                              // fin
+        add(q, 4);                                  // fin_cnt
+        add(P + 2, 4 * 3);                          // perm_cnt, perm_off, perm_fill
+        add(q, 4);                                  // qperm
+        add(q, 4);                                  // rr_stats
+    } else if (ix->rq_l2) {
+        add(q * k, 8);
+        add(q, 4);
+    } else {
+        add(std::max<uint64_t>(total, 1), 8);       // cand
+        if (z.rerank) {
+            add(q * k, 8);
+            add(q, 4);
+        }
+    }
+    return b;
+}
+
+int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, uint32_t nq,
+                uint32_t k, uint32_t n_probe, const spf_search_opts *opts, uint64_t *d_ids,
+                float *d_dists, uint32_t *d_counts, const uint64_t *d_masks) {
+    const uint32_t P = ix->P;
+    if (d_masks && !ix->d_label_e) return fail(SPF_EINVAL, "filtered search on an index without labels");
+    Shape z;
+    if (int rc = search_shape(ix, k, n_probe, opts, &z)) return rc;
+    const uint32_t npr = z.npr, want = z.want, dpad = z.dpad, dpad32 = z.dpad32, Gv = z.Gv;
+    const uint32_t G = z.G, echunk = z.echunk;
+    const bool mixed = z.mixed, rerank = z.rerank;
     const bool prof = opts && (opts->flags & SPF_FLAG_PROFILE);
     if (prof)
         for (auto &e : w->ev)
@@ -767,10 +881,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
 
     HIPCHK(w->qv_probe.ensure((size_t)nq * dpad));
     HIPCHK(w->qnorm.ensure(nq));
-    const bool round16 = ix->quant == SPF_QUANT_F16 &&
-                         (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
-    const bool q8 = ix->quant == SPF_QUANT_INT8 &&
-                    (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
+    const bool round16 = z.round16, q8 = z.q8;
     if (round16) HIPCHK(w->qv_scan.ensure((size_t)nq * dpad));
     HIPCHK(w->q8.ensure((size_t)nq * dpad / 4));
     HIPCHK(w->qsq.ensure(nq));
@@ -791,9 +902,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pp.q8 = w->q8.p;
     pp.qnorm = w->qnorm.p;
     pp.qsq = w->qsq.p;
-    // matrix-core probe for L2 metrics when it has something to prune
-    const bool mprobe = mixed && (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ) &&
-                        P > npr && ix->dim <= 256 && npr + npr / 4 + 32 <= kMaxK;
+    const bool mprobe = z.mprobe;
     if (mixed) {
         HIPCHK(w->q16.ensure((size_t)nq * dpad32));
         HIPCHK(w->qn16.ensure(nq));
@@ -828,8 +937,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     if (mprobe) {
         // bf16 matrix-core distances -> best npr + margin by 16-bit key -> exact
         // f32 recheck of those centroids -> sorted probe list
-        const uint32_t Pp = round_up(P, 32);
-        const uint32_t band = std::min<uint32_t>(P, std::min<uint32_t>(kMaxK, std::max(npr + 32, npr + npr / 4)));
+        const uint32_t Pp = z.Pp, band = z.band;
         HIPCHK(w->probe16.ensure((size_t)nq * Pp));
         HIPCHK(w->probe_pool.ensure((size_t)nq * band));
         HIPCHK(w->probe_pool_cnt.ensure(nq));
@@ -1162,7 +1270,149 @@ int check_search_args(spf_index *ix, uint32_t dim, uint32_t k) {
     return SPF_OK;
 }
 
+// Host-memory batch on one workspace: queries (and masks) up, search, results
+// down; returns after the stream has drained.
+int host_batch(spf_index *ix, Workspace *w, const float *queries, uint32_t nq, uint32_t k,
+               uint32_t n_probe, const spf_search_opts *opts, const uint64_t *masks,
+               uint64_t *out_ids, float *out_dists, uint32_t *out_counts) {
+    hipStream_t s = w->stream;
+    const uint32_t dim = ix->dim;
+    HIPCHK(w->qin.ensure((size_t)nq * dim));
+    HIPCHK(w->out_ids.ensure((size_t)nq * k));
+    HIPCHK(w->out_d.ensure((size_t)nq * k));
+    HIPCHK(w->out_c.ensure(nq));
+    HIPCHK(hipMemcpyAsync(w->qin.p, queries, (size_t)nq * dim * 4, hipMemcpyHostToDevice, s));
+    if (masks) {
+        HIPCHK(w->qmask.ensure(nq));
+        HIPCHK(hipMemcpyAsync(w->qmask.p, masks, (size_t)nq * 8, hipMemcpyHostToDevice, s));
+    }
+    int r = search_impl(ix, w, s, w->qin.p, nq, k, n_probe, opts, w->out_ids.p, w->out_d.p,
+                        w->out_c.p, masks ? w->qmask.p : nullptr);
+    if (r) {
+        (void)hipStreamSynchronize(s);
+        return r;
+    }
+    HIPCHK(hipMemcpyAsync(out_ids, w->out_ids.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(out_dists, w->out_d.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(out_counts, w->out_c.p, (size_t)nq * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SPF_OK;
+}
+
+// pinned host staging that grows by a quarter past the request
+template <class T>
+hipError_t pin_ensure(T **p, size_t *cap, size_t n) {
+    if (n <= *cap && *p) return hipSuccess;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    const size_t nn = n + n / 4 + 64;
+    hipError_t e = hipHostMalloc((void **)p, nn * sizeof(T));
+    if (e == hipSuccess) *cap = nn;
+    return e;
+}
+
+spf_search_opts opts_of(const spf_sched::Key &key) {
+    spf_search_opts o{};
+    o.precision = key.precision;
+    o.rerank_size = key.rerank;
+    o.flags = key.flags;
+    return o;
+}
+
+// Device side of the search server: one workspace, stream and pinned staging
+// set per slot; the scheduler (spf_sched.h) decides what each slot runs.
+struct ServerExec {
+    struct Slot {
+        Workspace *w = nullptr;
+        float *h_q = nullptr, *h_d = nullptr;
+        uint64_t *h_m = nullptr, *h_ids = nullptr;
+        uint32_t *h_c = nullptr;
+        size_t c_q = 0, c_d = 0, c_m = 0, c_ids = 0, c_c = 0;
+    };
+    spf_index *ix = nullptr;
+    std::vector<Slot> slots;
+
+    int validate(const spf_sched::Key &key, uint32_t dim, std::string *err) {
+        const spf_search_opts o = opts_of(key);
+        Shape z;
+        int rc = check_search_args(ix, dim, key.k);
+        if (!rc) rc = search_shape(ix, key.k, key.n_probe, &o, &z);
+        if (!rc && key.filtered && !ix->d_label_e)
+            rc = fail(SPF_EINVAL, "filtered search on an index without labels");
+        if (rc) *err = g_err;
+        return rc;
+    }
+    uint64_t bound(const spf_sched::Key &key, uint32_t nq) {
+        const spf_search_opts o = opts_of(key);
+        Shape z;
+        if (search_shape(ix, key.k, key.n_probe, &o, &z)) return UINT64_MAX;
+        return ws_bound(ix, z, nq, key.k, true, key.filtered);
+    }
+    uint64_t held(int s) { return slots[s].w->bytes(); }
+    void trim(int s) {
+        DeviceGuard dg(ix->device);
+        slots[s].w->trim();
+    }
+    int run(int s, const spf_sched::Batch &b, std::string *err) {
+        DeviceGuard dg(ix->device);
+        Slot &sl = slots[s];
+        const uint32_t nq = b.nq, dim = ix->dim, k = b.key.k;
+        auto stage = [&]() -> int {
+            HIPCHK(pin_ensure(&sl.h_q, &sl.c_q, (size_t)nq * dim));
+            HIPCHK(pin_ensure(&sl.h_ids, &sl.c_ids, (size_t)nq * k));
+            HIPCHK(pin_ensure(&sl.h_d, &sl.c_d, (size_t)nq * k));
+            HIPCHK(pin_ensure(&sl.h_c, &sl.c_c, nq));
+            if (b.key.filtered) HIPCHK(pin_ensure(&sl.h_m, &sl.c_m, nq));
+            return SPF_OK;
+        };
+        int rc = stage();
+        if (!rc) {
+            size_t row = 0;
+            for (const spf_sched::Piece &p : b.pieces) {
+                std::memcpy(sl.h_q + row * dim, p.r->q + (size_t)p.row0 * dim,
+                            (size_t)p.n * dim * 4);
+                if (b.key.filtered)
+                    std::memcpy(sl.h_m + row, p.r->masks + p.row0, (size_t)p.n * 8);
+                row += p.n;
+            }
+            const spf_search_opts o = opts_of(b.key);
+            rc = host_batch(ix, sl.w, sl.h_q, nq, k, b.key.n_probe, &o,
+                            b.key.filtered ? sl.h_m : nullptr, sl.h_ids, sl.h_d, sl.h_c);
+        }
+        if (rc) {
+            *err = g_err;
+            return rc;
+        }
+        size_t row = 0;
+        for (const spf_sched::Piece &p : b.pieces) {
+            std::memcpy(p.r->ids + (size_t)p.row0 * k, sl.h_ids + row * k, (size_t)p.n * k * 8);
+            std::memcpy(p.r->dists + (size_t)p.row0 * k, sl.h_d + row * k, (size_t)p.n * k * 4);
+            std::memcpy(p.r->counts + p.row0, sl.h_c + row, (size_t)p.n * 4);
+            row += p.n;
+        }
+        return SPF_OK;
+    }
+    void free_slots() {
+        for (Slot &sl : slots) {
+            if (sl.w) {
+                sl.w->release();
+                delete sl.w;
+            }
+            void *hs[] = {sl.h_q, sl.h_d, sl.h_m, sl.h_ids, sl.h_c};
+            for (void *h : hs)
+                if (h) (void)hipHostFree(h);
+            sl = Slot();
+        }
+    }
+};
+
 }  // namespace
+
+struct spf_server {
+    ServerExec ex;
+    std::unique_ptr<spf_sched::Scheduler<ServerExec>> sch;
+};
 
 extern "C" {
 
@@ -1256,27 +1506,7 @@ int spf_search_batch_filtered(spf_index *ix, const float *queries, uint32_t nq, 
     DeviceGuard dg(ix->device);
     Workspace *w = acquire(ix);
     if (!w) return fail(SPF_EDEVICE, "workspace creation failed");
-    hipStream_t s = w->stream;
-    auto run = [&]() -> int {
-        HIPCHK(w->qin.ensure((size_t)nq * dim));
-        HIPCHK(w->out_ids.ensure((size_t)nq * k));
-        HIPCHK(w->out_d.ensure((size_t)nq * k));
-        HIPCHK(w->out_c.ensure(nq));
-        HIPCHK(hipMemcpyAsync(w->qin.p, queries, (size_t)nq * dim * 4, hipMemcpyHostToDevice, s));
-        if (masks) {
-            HIPCHK(w->qmask.ensure(nq));
-            HIPCHK(hipMemcpyAsync(w->qmask.p, masks, (size_t)nq * 8, hipMemcpyHostToDevice, s));
-        }
-        int r = search_impl(ix, w, s, w->qin.p, nq, k, n_probe, opts, w->out_ids.p, w->out_d.p,
-                            w->out_c.p, masks ? w->qmask.p : nullptr);
-        if (r) return r;
-        HIPCHK(hipMemcpyAsync(out_ids, w->out_ids.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(out_dists, w->out_d.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(out_counts, w->out_c.p, (size_t)nq * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        return SPF_OK;
-    };
-    rc = run();
+    rc = host_batch(ix, w, queries, nq, k, n_probe, opts, masks, out_ids, out_dists, out_counts);
     release_ws(ix, w);
     return rc;
 }
@@ -1340,6 +1570,90 @@ int spf_selftest(int device) {
             }
         }
     return SPF_OK;
+}
+
+int spf_server_create(spf_index *ix, const spf_server_opts *opts, spf_server **out) {
+    if (!ix || !out) return fail(SPF_EINVAL, "null argument");
+    *out = nullptr;
+    const uint32_t n_slots = opts && opts->n_slots ? opts->n_slots : 3u;
+    const uint32_t max_batch = opts && opts->max_batch ? opts->max_batch : 1024u;
+    const uint64_t budget = opts && opts->hbm_budget_bytes ? opts->hbm_budget_bytes : (8ull << 30);
+    if (n_slots > 16) return fail(SPF_EINVAL, "n_slots above 16");
+    DeviceGuard dg(ix->device);
+    std::unique_ptr<spf_server> sv(new spf_server());
+    sv->ex.ix = ix;
+    sv->ex.slots.resize(n_slots);
+    for (auto &sl : sv->ex.slots) {
+        sl.w = new_workspace();
+        if (!sl.w) {
+            sv->ex.free_slots();
+            return fail(SPF_EDEVICE, "workspace creation failed");
+        }
+    }
+    sv->sch.reset(new spf_sched::Scheduler<ServerExec>(&sv->ex, (int)n_slots, max_batch, budget));
+    *out = sv.release();
+    return SPF_OK;
+}
+
+int spf_server_submit(spf_server *sv, const float *queries, uint32_t nq, uint32_t dim,
+                      uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
+                      const uint64_t *masks, uint64_t *out_ids, float *out_dists,
+                      uint32_t *out_counts, uint64_t *ticket) {
+    if (!sv || !ticket) return fail(SPF_EINVAL, "null argument");
+    if (nq && (!queries || !out_ids || !out_dists || !out_counts))
+        return fail(SPF_EINVAL, "null buffer");
+    spf_sched::Key key;
+    key.k = k;
+    key.n_probe = n_probe;
+    key.precision = opts ? opts->precision : SPF_PREC_EXACT;
+    key.rerank = opts ? opts->rerank_size : 0;
+    key.flags = opts ? (opts->flags & ~SPF_FLAG_PROFILE) : 0;
+    key.filtered = masks != nullptr;
+    std::string err;
+    const int rc = sv->sch->submit(key, queries, nq, dim, masks, out_ids, out_dists, out_counts,
+                                   ticket, &err);
+    return rc ? fail(rc, err) : SPF_OK;
+}
+
+int spf_server_wait(spf_server *sv, uint64_t ticket, int64_t timeout_us, spf_request_info *info) {
+    if (!sv) return fail(SPF_EINVAL, "null server");
+    spf_sched::Request r;
+    std::string err;
+    const int rc = sv->sch->wait(ticket, timeout_us, &r, &err);
+    if (info && rc != SPF_EAGAIN && !(rc == SPF_EINVAL && err == "unknown ticket")) {
+        *info = spf_request_info{};
+        info->submit_ns = r.t_submit;
+        info->start_ns = r.t_start;
+        info->done_ns = r.t_done;
+        info->batches = r.batches;
+    }
+    return rc ? fail(rc, err) : SPF_OK;
+}
+
+int spf_server_get_stats(spf_server *sv, spf_server_stats *out) {
+    if (!sv || !out) return fail(SPF_EINVAL, "null argument");
+    const spf_sched::Stats st = sv->sch->stats();
+    *out = spf_server_stats{};
+    out->requests_done = st.requests_done;
+    out->rows_done = st.rows_done;
+    out->batches = st.batches;
+    out->rejected = st.rejected;
+    out->peak_reserved_bytes = st.peak_reserved;
+    out->budget_overruns = st.overruns;
+    out->admission_waits = st.admission_waits;
+    out->trims = st.trims;
+    out->max_concurrent = st.max_concurrent;
+    return SPF_OK;
+}
+
+void spf_server_destroy(spf_server *sv) {
+    if (!sv) return;
+    sv->sch.reset();
+    {
+        DeviceGuard dg(sv->ex.ix->device);
+        sv->ex.free_slots();
+    }
+    delete sv;
 }
 
 int spf_last_profile(spf_profile *out) {

@@ -42,11 +42,12 @@ def test_every_declared_symbol_is_exported():
 def test_struct_layouts_match_the_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "spf.h"\n'
-                   "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu\\n\","
+                   "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\","
                    "sizeof(spf_index_desc), offsetof(spf_index_desc, probe_ratio),"
                    "sizeof(spf_search_opts), sizeof(spf_profile), sizeof(spf_index_info),"
                    "offsetof(spf_index_desc, rabitq_signs),"
-                   "offsetof(spf_index_desc, labels));return 0;}\n")
+                   "offsetof(spf_index_desc, labels), sizeof(spf_server_opts),"
+                   "sizeof(spf_request_info), sizeof(spf_server_stats));return 0;}\n")
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
@@ -54,7 +55,8 @@ def test_struct_layouts_match_the_header(tmp_path):
                                           check=True).stdout.split()]
     want = [C.sizeof(_lib.IndexDesc), _lib.IndexDesc.probe_ratio.offset,
             C.sizeof(_lib.SearchOpts), C.sizeof(_lib.Profile), C.sizeof(_lib.IndexInfo),
-            _lib.IndexDesc.rabitq_signs.offset, _lib.IndexDesc.labels.offset]
+            _lib.IndexDesc.rabitq_signs.offset, _lib.IndexDesc.labels.offset,
+            C.sizeof(_lib.ServerOpts), C.sizeof(_lib.RequestInfo), C.sizeof(_lib.ServerStats)]
     assert got == want
 
 
@@ -77,6 +79,15 @@ def test_argument_errors_without_a_device():
     assert L.spf_delete(None, None, 0) == _lib.SPF_EINVAL
     assert L.spf_nearest_postings(None, None, 1, 4, None, None) == _lib.SPF_EINVAL
     assert L.spf_index_get_info(None, None) == _lib.SPF_EINVAL
+    assert L.spf_search_batch_filtered(None, None, 1, 4, 10, 8, None, None, None, None,
+                                       None) == _lib.SPF_EINVAL
+    assert L.spf_server_create(None, None, C.byref(h)) == _lib.SPF_EINVAL
+    t = C.c_uint64()
+    assert L.spf_server_submit(None, None, 1, 4, 10, 8, None, None, None, None, None,
+                               C.byref(t)) == _lib.SPF_EINVAL
+    assert L.spf_server_wait(None, 1, 0, None) == _lib.SPF_EINVAL
+    assert L.spf_server_get_stats(None, None) == _lib.SPF_EINVAL
+    L.spf_server_destroy(None)
 
 
 def test_python_mirror_argument_errors():
