@@ -37,6 +37,10 @@ STAGE_KERNELS = {
 }
 
 
+METRIC_NAMES = {0: "L2 (root)", 1: "L2", 2: "cosine", 3: "dot"}
+QUANT_NAMES = {0: "f32", 1: "f16", 2: "int8", 3: "rabitq"}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -44,14 +48,26 @@ def log(*a):
 def build_workload(args, torch):
     import numpy as np
     from pydiskann_amd import build_index
-    from pydiskann_amd.datasets import config1, config2
+    from pydiskann_amd.datasets import config1, config2, config3, labels
 
     t0 = time.time()
+    if args.workload in ("c3", "c5"):
+        # 10M x 96 unit vectors, one posting per mixture component (Zipf
+        # sizes), int8 codes, inner product, exact rerank of 400
+        base, queries, mu = config3(n=args.n_base or 10_000_000, nq=args.nq, return_means=True)
+        log(f"[bench] data {base.shape} queries {queries.shape} in {time.time() - t0:.1f}s")
+        t0 = time.time()
+        ix = build_index(base, "dot", "int8", centroids=mu, rerank_size=400, device="cuda")
+        if args.workload == "c5":
+            ix.labels = labels(ix.n_raw)
+        log(f"[bench] index {ix.n_post} postings, max len {int(ix.posting_lengths().max())} "
+            f"in {time.time() - t0:.1f}s")
+        return base, queries, ix, 32, 10, np
     if args.workload == "c1":
         base, queries = config1()
         n_post, n_probe, k = 256, 16, 10
     else:
-        base, queries = config2(n=args.n_base, nq=args.nq)
+        base, queries = config2(n=args.n_base or 1_000_000, nq=args.nq)
         n_post, n_probe, k = 4096, 64, 100
     log(f"[bench] data {base.shape} queries {queries.shape} in {time.time() - t0:.1f}s")
     t0 = time.time()
@@ -78,21 +94,135 @@ def read_traffic(kernel: str, workload: str):
     return None, None
 
 
+def serve(args, queries, ix, np) -> int:
+    """Filtered serving over the c3 index: requests of U{1..256} queries with
+    k in {10, 100}, n_probe in {16, 64} and a 1-8 label allow mask per query,
+    through the continuous-batching server.  First a closed loop (every
+    request submitted at once) measures capacity; then open-loop Poisson
+    arrivals at the target query rate(s) give per-request latency (server
+    clock, submit to last row back) and the achieved rate."""
+    import queue
+    import threading
+
+    from pydiskann_amd import DeviceIndex, Server, PREC_EXACT
+    from pydiskann_amd.datasets import label_masks
+
+    rng = np.random.default_rng(5)
+    nq_all = queries.shape[0]
+    masks_all = label_masks(nq_all)
+    dev = DeviceIndex(ix, device=0)
+
+    def make_requests(n, seed):
+        r = np.random.default_rng(seed)
+        out = []
+        for _ in range(n):
+            m = int(r.integers(1, 257))
+            rows = r.integers(0, nq_all, size=m)
+            out.append((np.ascontiguousarray(queries[rows]), np.ascontiguousarray(masks_all[rows]),
+                        int(r.choice([10, 100])), int(r.choice([16, 64]))))
+        return out
+
+    def run(reqs, rate_qps):
+        """Submit reqs (all at once if rate_qps is 0, else Poisson arrivals at
+        rate_qps queries/s); returns latencies (ms), wall seconds, stats."""
+        srv = Server(dev, n_slots=args.slots, max_batch=args.max_batch,
+                     hbm_budget=int(args.budget_gib * (1 << 30)))
+        tickets = queue.Queue()
+        lat = []
+
+        def waiter():
+            while True:
+                t = tickets.get()
+                if t is None:
+                    return
+                _, _, _, info = srv.wait(t)
+                lat.append((info["done_ns"] - info["submit_ns"]) * 1e-6)
+
+        wt = threading.Thread(target=waiter)
+        wt.start()
+        mean_rows = 128.5
+        gaps = rng.exponential(mean_rows / rate_qps, size=len(reqs)) if rate_qps else None
+        t0 = time.perf_counter()
+        due = t0
+        for i, (q, m, k, npr) in enumerate(reqs):
+            if gaps is not None:
+                due += gaps[i]
+                d = due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+            tickets.put(srv.submit(q, k, npr, precision=PREC_EXACT, masks=m))
+        tickets.put(None)
+        wt.join()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        return lat, wall, st
+
+    def summary(lat, wall, rows):
+        s = sorted(lat)
+        pick = lambda p: s[min(len(s) - 1, int(round(p * (len(s) - 1))))]
+        return {"qps": round(rows / wall, 1), "p50_ms": round(pick(0.5), 3),
+                "p99_ms": round(pick(0.99), 3), "mean_ms": round(float(np.mean(s)), 3)}
+
+    warm = make_requests(64, 1)
+    run(warm, 0)   # grows every slot's working memory once
+    log("[bench] c5: warmup done")
+    reqs = make_requests(args.requests, 2)
+    rows = sum(q.shape[0] for q, _, _, _ in reqs)
+    lat, wall, st = run(reqs, 0)
+    cap = summary(lat, wall, rows)
+    log(f"[bench] c5 closed loop: {rows} queries in {args.requests} requests, {cap}, {st}")
+    rates = [args.target_qps] if args.target_qps else [0.5 * cap["qps"], 0.8 * cap["qps"]]
+    points = []
+    for rate in rates:
+        lat, wall, st2 = run(reqs, rate)
+        p = summary(lat, wall, rows)
+        p["target_qps"] = round(rate, 1)
+        p["batches"], p["max_concurrent"] = st2["batches"], st2["max_concurrent"]
+        points.append(p)
+        log(f"[bench] c5 open loop at {rate:,.0f} QPS: {p}")
+    head = points[-1]
+    out = {
+        "metric": "filtered serving: throughput and per-request p50/p99 latency under Poisson "
+                  "arrivals (config 5)",
+        "value": head["qps"], "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
+        "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"], "target_qps": head["target_qps"],
+        "load_points": points, "closed_loop": cap, "server_stats_closed_loop": st,
+        "dtype": "int8 scan (int32 sums), f32 rerank",
+        "data": "synthetic (seeded vMF-like mixture, labels U{0..63}, masks of 1-8 labels)",
+        "config": {"workload": f"c5: {ix.n_raw}x{ix.dim} dot int8, {ix.n_post} postings, "
+                               f"rerank 400, requests of U{{1..256}} queries, k in {{10,100}}, "
+                               f"n_probe in {{16,64}}, {args.requests} requests per point",
+                   "slots": args.slots, "max_batch": args.max_batch,
+                   "budget_gib": args.budget_gib},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["c1", "c2"], default="c2")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c5"], default="c2",
+                    help="c2: the headline batch search; c3: int8 inner-product batch search "
+                         "over 10M x 96; c5: filtered serving over the c3 index")
     ap.add_argument("--precision", choices=["exact", "mixed"], default="mixed")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip timing the other precision")
     ap.add_argument("--rerank", type=int, default=1000)
-    ap.add_argument("--n-base", type=int, default=1_000_000)
+    ap.add_argument("--n-base", type=int, default=0, help="base size (0 = the workload's)")
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--recall-sample", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--requests", type=int, default=4000, help="c5: requests per load point")
+    ap.add_argument("--target-qps", type=float, default=0,
+                    help="c5: open-loop query rate (0 = 50%% and 80%% of measured capacity)")
+    ap.add_argument("--slots", type=int, default=3, help="c5: concurrent device batches")
+    ap.add_argument("--max-batch", type=int, default=1024, help="c5: rows per device batch")
+    ap.add_argument("--budget-gib", type=float, default=16.0, help="c5: working-memory budget")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not "
@@ -105,6 +235,11 @@ def main() -> int:
     from pydiskann_amd.index import brute_force_f64
 
     base, queries, ix, n_probe, k, np = build_workload(args, torch)
+    if args.workload == "c5":
+        return serve(args, queries, ix, np)
+    if args.workload == "c3" and args.precision == "mixed":
+        log("[bench] c3 postings are int8: the exact path (int8 scan + f32 rerank) is timed")
+        args.precision, args.no_compare = "exact", True
     nq, dim = queries.shape
     dev = DeviceIndex(ix, device=0)
     q_d = torch.from_numpy(queries).cuda()
@@ -113,7 +248,7 @@ def main() -> int:
     cnt_d = torch.empty((nq,), dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
     ns = min(args.recall_sample, nq)
-    gt, _ = brute_force_f64(base, queries[:ns], 10, 1)
+    gt, _ = brute_force_f64(base, queries[:ns], 10, ix.metric, chunk=16)
 
     def run(precision: str):
         prec = PREC_EXACT if precision == "exact" else PREC_MIXED
@@ -164,7 +299,8 @@ def main() -> int:
 
     # roofline of the dominant kernel of the headline run
     stage, prof = head["stage"], head["prof"]
-    want = max(k, args.rerank) if args.precision == "mixed" else k
+    want = max(k, args.rerank) if args.precision == "mixed" else \
+        (max(k, ix.rerank_size) if ix.quant and ix.rerank_size else k)
     cand = prof["candidates"]
     if args.precision == "mixed":
         # select: 16-bit keys read, pool (entry, key) written; rerank: pool
@@ -209,7 +345,7 @@ def main() -> int:
         dt = time.perf_counter() - t0
         cpu = {"value": round(nsc / dt, 1), "unit": "queries/s", "cores": cores, "kind": "port",
                "sample": f"first {nsc} of the {nq} queries, same index, C restatement of "
-                         f"Inner::search (exact f32), {cores} threads"}
+                         f"Inner::search (exact path), {cores} threads"}
         log(f"[bench] cpu oracle {nsc / dt:,.1f} QPS on {cores} threads")
 
     qps, total = head["qps"], head["total"]
@@ -218,8 +354,10 @@ def main() -> int:
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(total * 1e3 / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.precision == "exact" else "f16/bf16 scan+probe, f32 rerank",
-        "data": "synthetic (seeded numpy Gaussian mixture; index by seeded k-means)",
-        "config": {"workload": f"{args.workload}: {ix.n_raw}x{dim} L2, {ix.n_post} postings, "
+        "data": "synthetic (seeded numpy mixture; index by seeded k-means or, c3, one "
+                "posting per mixture component)",
+        "config": {"workload": f"{args.workload}: {ix.n_raw}x{dim} {METRIC_NAMES[ix.metric]} "
+                               f"{QUANT_NAMES[ix.quant]}, {ix.n_post} postings, "
                                f"n_probe {n_probe}, k {k}, {nq} queries per step",
                    "precision": args.precision,
                    "rerank_pool": args.rerank if args.precision == "mixed" else 0},

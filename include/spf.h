@@ -58,6 +58,27 @@ extern "C" {
 
 typedef struct spf_index spf_index;
 
+/* Centroid routing graph: the reference's IncrementalDiskANN over the posting
+ * centroids ({path}.centroids: DANN base + delta layer + tombstones,
+ * src/incremental.rs:1213-1233, src/lib.rs:90-98), which Inner::nearest
+ * searches instead of scanning every centroid above 4,096 live postings
+ * (src/spfresh.rs:20,374-378).  Node ids are unified: base nodes
+ * 0..base_n-1, then delta nodes (UnifiedView, src/incremental.rs:357-487). */
+typedef struct spf_graph_desc {
+    uint32_t n_nodes;
+    uint32_t n_seeds;               /* entry points: base medoid, then base_n + delta entry */
+    uint32_t seeds[2];
+    uint32_t centroid_beam;         /* SPFreshConfig::centroid_beam */
+    uint32_t reserved;
+    uint64_t n_tombstones;          /* size of the graph's tombstone set */
+    const float *vectors;           /* [n_nodes * dim] node vectors */
+    const uint32_t *adj_off;        /* [n_nodes + 1] */
+    const uint32_t *adj;            /* [adj_off[n_nodes]] neighbor lists, stored order, no PAD */
+    const uint8_t *live;            /* [n_nodes] 0 = tombstoned */
+    const int64_t *node_post;       /* [n_nodes] posting (index into this descriptor's
+                                       postings) of the node's cid, -1 = none */
+} spf_graph_desc;
+
 typedef struct spf_index_desc {
     uint32_t dim;
     uint32_t metric;                /* SPF_METRIC_* */
@@ -82,6 +103,7 @@ typedef struct spf_index_desc {
     uint32_t rerank_size;           /* SPFreshConfig::rerank_size */
     int32_t device;                 /* HIP device ordinal */
     const uint8_t *labels;          /* [n_raw] label of each id for filtered search (NULL = none) */
+    const spf_graph_desc *graph;    /* routing graph (NULL = exact probe at any posting count) */
 } spf_index_desc;
 
 #define SPF_FLAG_PROFILE 1u     /* time each stage with HIP events on the call's stream;

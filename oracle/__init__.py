@@ -24,7 +24,16 @@ class OIndex(C.Structure):
         ("block_ids", C.c_void_p), ("centroids", C.c_void_p), ("post_off", C.c_void_p),
         ("ids", C.c_void_p), ("codes", C.c_void_p), ("raw", C.c_void_p),
         ("n_raw", C.c_uint64), ("deleted", C.c_void_p), ("rq_signs", C.c_void_p),
-        ("labels", C.c_void_p),
+        ("labels", C.c_void_p), ("graph", C.c_void_p),
+    ]
+
+
+class OGraph(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_uint32), ("n_seeds", C.c_uint32), ("seeds", C.c_uint32 * 2),
+        ("centroid_beam", C.c_uint32), ("n_tombstones", C.c_uint64),
+        ("vectors", C.c_void_p), ("adj_off", C.c_void_p), ("adj", C.c_void_p),
+        ("live", C.c_void_p), ("node_post", C.c_void_p),
     ]
 
 
@@ -70,12 +79,32 @@ def lib():
         L.spf_o_search_batch.argtypes = [P, P, u32, u32, u32, u32, P, P, P]
         L.spf_o_search_batch_filtered.argtypes = [P, P, P, u32, u32, u32, u32, P, P, P]
         L.spf_o_set_fault.argtypes = [C.c_int]
+        L.spf_o_graph_search.restype = u32
+        L.spf_o_graph_search.argtypes = [P, u32, u32, P, u32, u32, P, P]
         _lib = L
     return _lib
 
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def graph_struct(g):
+    """(OGraph, kept arrays) for a pydiskann_amd.index.CentroidGraph."""
+    keep = [np.ascontiguousarray(g.vectors, dtype=np.float32),
+            np.ascontiguousarray(g.adj_off, dtype=np.uint32),
+            np.ascontiguousarray(g.adj, dtype=np.uint32),
+            np.ascontiguousarray(g.live, dtype=np.uint8),
+            np.ascontiguousarray(g.node_post, dtype=np.int64)]
+    o = OGraph()
+    o.n_nodes = keep[0].shape[0]
+    o.n_seeds = len(g.seeds)
+    for i, sd in enumerate(g.seeds):
+        o.seeds[i] = int(sd)
+    o.centroid_beam = g.centroid_beam
+    o.n_tombstones = g.n_tombstones
+    o.vectors, o.adj_off, o.adj, o.live, o.node_post = (_p(a) for a in keep)
+    return o, keep
 
 
 class Oracle:
@@ -100,7 +129,21 @@ class Oracle:
             self._labels = np.ascontiguousarray(ix.labels, dtype=np.uint8)
             assert self._labels.shape[0] == ix.n_raw
             s.labels = _p(self._labels)
+        self._graph = None
+        if getattr(ix, "graph", None) is not None:
+            self._graph = graph_struct(ix.graph)
+            s.graph = C.cast(C.byref(self._graph[0]), C.c_void_p)
         self.s = s
+
+    def graph_search(self, q, k: int, beam: int):
+        """IncrementalDiskANN::search_with_dists over the routing graph:
+        (unified node ids, distances)."""
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        nodes = np.zeros(max(k, 1), dtype=np.uint32)
+        dd = np.zeros(max(k, 1), dtype=np.float32)
+        c = lib().spf_o_graph_search(C.byref(self._graph[0]), self.ix.metric, self.ix.dim, _p(q),
+                                     k, beam, _p(nodes), _p(dd))
+        return nodes[:c].copy(), dd[:c].copy()
 
     def search(self, q, k: int, n_probe: int):
         q = np.ascontiguousarray(q, dtype=np.float32)

@@ -319,6 +319,217 @@ static inline int skipped(const spf_o_index *ix, uint64_t id, int filtered, uint
     return l >= 64 || !((mask >> l) & 1u);
 }
 
+/* ---- routing graph (Inner::nearest above BRUTE_FORCE_CENTROIDS = 4096 live
+ * postings, src/spfresh.rs:20,358-378) ---- */
+
+/* Candidate (src/lib.rs:100-122) compares by dist alone; `a <= b` is
+ * PartialOrd::le, false when either dist is NaN. */
+typedef struct { float d; uint32_t id; } gcand_t;
+
+/* std::collections::BinaryHeap restated: push = append + sift_up; pop = take
+ * the last element, put it at the root, sift_down_to_bottom (always towards the
+ * child that is not <= its sibling, i.e. the right one on ties), then sift_up.
+ * `rev` = the heap of Reverse<Candidate> (a min-heap): a <= b becomes b <= a. */
+typedef struct { gcand_t *v; uint32_t n; int rev; } gheap_t;
+
+static inline int gle(const gheap_t *h, gcand_t a, gcand_t b) {
+    return h->rev ? (b.d <= a.d) : (a.d <= b.d);
+}
+
+static uint32_t gheap_sift_up(gheap_t *h, uint32_t start, uint32_t pos) {
+    gcand_t e = h->v[pos];
+    while (pos > start) {
+        uint32_t parent = (pos - 1) / 2;
+        if (gle(h, e, h->v[parent])) break;
+        h->v[pos] = h->v[parent];
+        pos = parent;
+    }
+    h->v[pos] = e;
+    return pos;
+}
+
+static void gheap_push(gheap_t *h, gcand_t c) {
+    h->v[h->n] = c;
+    h->n++;
+    gheap_sift_up(h, 0, h->n - 1);
+}
+
+static gcand_t gheap_pop(gheap_t *h) {
+    gcand_t item = h->v[h->n - 1];
+    h->n--;
+    if (h->n == 0) return item;
+    gcand_t top = h->v[0];
+    h->v[0] = item;
+    /* sift_down_to_bottom(0) */
+    uint32_t end = h->n, pos = 0, child = 1;
+    gcand_t e = h->v[0];
+    while (end >= 2 && child <= end - 2) {
+        child += gle(h, h->v[child], h->v[child + 1]) ? 1u : 0u;
+        h->v[pos] = h->v[child];
+        pos = child;
+        child = 2 * pos + 1;
+    }
+    if (child == end - 1) {
+        h->v[pos] = h->v[child];
+        pos = child;
+    }
+    h->v[pos] = e;
+    gheap_sift_up(h, 0, pos);
+    return top;
+}
+
+/* f32::total_cmp as an unsigned key */
+static inline uint32_t total_key(float f) { return fkey(f); }
+
+typedef struct { uint32_t key; uint32_t pos; gcand_t c; } gsort_t;
+static int gsort_cmp(const void *a, const void *b) {   /* stable: ties by position */
+    const gsort_t *x = (const gsort_t *)a, *y = (const gsort_t *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->pos < y->pos ? -1 : (x->pos > y->pos);
+}
+
+static float graph_dist(const spf_o_graph *g, uint32_t metric, uint32_t dim, const float *q,
+                        uint32_t node) {
+    return spf_o_eval(metric, q, g->vectors + (size_t)node * dim, dim);
+}
+
+/* beam_search (src/lib.rs:198-303) with UnifiedView's seeds, neighbors and
+ * liveness and tombstone_config (src/incremental.rs:383-393,441-487,489-499). */
+uint32_t spf_o_graph_search(const spf_o_graph *g, uint32_t metric, uint32_t dim, const float *q,
+                            uint32_t k, uint32_t beam, uint32_t *out_node, float *out_dist) {
+    if (!g || g->n_seeds == 0 || g->n_nodes == 0) return 0;
+    const int filtered = g->n_tombstones > 0;
+    uint64_t wb = beam;
+    double max_it = INFINITY, early = INFINITY;
+    if (filtered) {
+        uint64_t t = g->n_tombstones < beam ? g->n_tombstones : beam;
+        uint64_t e1 = (uint64_t)beam * 2, e2 = (uint64_t)k + t;
+        wb = e1 > e2 ? e1 : e2;
+        max_it = (double)(wb * 2);
+        early = 1.5;
+    }
+    uint8_t *visited = (uint8_t *)calloc(g->n_nodes, 1);
+    gcand_t *wv = (gcand_t *)malloc(sizeof(gcand_t) * (size_t)(g->n_nodes + 1));
+    gcand_t *fv = (gcand_t *)malloc(sizeof(gcand_t) * (size_t)(g->n_nodes + 1));
+    gcand_t *res = (gcand_t *)malloc(sizeof(gcand_t) * (size_t)(k + 2));
+    gheap_t w = {wv, 0, 0}, fr = {fv, 0, 1};
+    uint32_t nres = 0;
+    uint32_t n_seeds = (g_fault == 7 && g->n_seeds > 1) ? 1 : g->n_seeds;
+    for (uint32_t i = 0; i < n_seeds; i++) {
+        uint32_t sid = g->seeds[i];
+        if (visited[sid]) continue;
+        visited[sid] = 1;
+        gcand_t c = {graph_dist(g, metric, dim, q, sid), sid};
+        gheap_push(&fr, c);
+        gheap_push(&w, c);
+        if (filtered && g->live[sid]) res[nres++] = c;
+    }
+    if (filtered) {   /* sort_by(total_cmp) (stable), truncate(k) */
+        for (uint32_t i = 1; i < nres; i++) {
+            gcand_t c = res[i];
+            uint32_t j = i;
+            while (j > 0 && total_key(res[j - 1].d) > total_key(c.d)) {
+                res[j] = res[j - 1];
+                j--;
+            }
+            res[j] = c;
+        }
+        if (nres > k) nres = k;
+    }
+    double iterations = 0;
+    while (fr.n > 0) {
+        gcand_t best = fr.v[0];
+        iterations += 1;
+        if (iterations > max_it) break;
+        if (filtered && nres >= k && nres > 0) {
+            float worst = res[nres - 1].d;
+            if (isfinite(early) && worst >= 0.0f && best.d > worst * (float)fmax(early, 1.0)) break;
+        }
+        if (w.n >= wb) {
+            gcand_t worst = w.v[0];
+            if (g_fault == 6 ? best.d > worst.d : best.d >= worst.d) break;
+        }
+        gcand_t cur = gheap_pop(&fr);
+        for (uint32_t a = g->adj_off[cur.id]; a < g->adj_off[cur.id + 1]; a++) {
+            uint32_t nb = g->adj[a];
+            if (visited[nb]) continue;
+            visited[nb] = 1;
+            gcand_t c = {graph_dist(g, metric, dim, q, nb), nb};
+            if (w.n < wb) {
+                gheap_push(&w, c);
+                gheap_push(&fr, c);
+            } else if (c.d < w.v[0].d) {
+                gheap_pop(&w);
+                gheap_push(&w, c);
+                gheap_push(&fr, c);
+            }
+            if (filtered && g->live[nb]) {
+                uint32_t pos = nres;
+                for (uint32_t i = 0; i < nres; i++)
+                    if (c.d < res[i].d) {
+                        pos = i;
+                        break;
+                    }
+                if (pos < k) {
+                    for (uint32_t i = nres; i > pos; i--) res[i] = res[i - 1];
+                    res[pos] = c;
+                    nres++;
+                    if (nres > k) nres--;
+                }
+            }
+        }
+    }
+    uint32_t cnt = 0;
+    if (filtered) {
+        for (uint32_t i = 0; i < nres; i++) {
+            out_node[cnt] = res[i].id;
+            out_dist[cnt] = res[i].d;
+            cnt++;
+        }
+    } else {   /* w.into_vec(), stable sort by total_cmp, truncate(k) */
+        gsort_t *s = (gsort_t *)malloc(sizeof(gsort_t) * (size_t)(w.n + 1));
+        for (uint32_t i = 0; i < w.n; i++) {
+            s[i].key = total_key(w.v[i].d);
+            s[i].pos = i;
+            s[i].c = w.v[i];
+        }
+        qsort(s, w.n, sizeof(gsort_t), gsort_cmp);
+        for (uint32_t i = 0; i < w.n && cnt < k; i++) {
+            out_node[cnt] = s[i].c.id;
+            out_dist[cnt] = s[i].c.d;
+            cnt++;
+        }
+        free(s);
+    }
+    free(visited);
+    free(wv);
+    free(fv);
+    free(res);
+    return cnt;
+}
+
+/* Inner::nearest over the graph (src/spfresh.rs:374-378): n results with
+ * beam max(centroid_beam, n), mapped to postings (nodes without one dropped). */
+static uint32_t graph_probe(const spf_o_index *ix, const float *q, uint32_t n, uint32_t *out_post,
+                            float *out_dist) {
+    const spf_o_graph *g = ix->graph;
+    uint32_t beam = g->centroid_beam > n ? g->centroid_beam : n;
+    uint32_t *nodes = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1));
+    float *dd = (float *)malloc(sizeof(float) * (size_t)(n + 1));
+    uint32_t c = spf_o_graph_search(g, ix->metric, ix->dim, q, n, beam, nodes, dd);
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < c; i++) {
+        int64_t p = g->node_post[nodes[i]];
+        if (p < 0) continue;
+        out_post[m] = (uint32_t)p;
+        out_dist[m] = dd[i];
+        m++;
+    }
+    free(nodes);
+    free(dd);
+    return m;
+}
+
 /* Inner::nearest, exact branch (src/spfresh.rs:358-372), then the probe_ratio
  * take_while of Inner::search (src/spfresh.rs:683-687).  Ties in centroid
  * distance are ordered by reference block index. */
@@ -326,6 +537,23 @@ uint32_t spf_o_probe(const spf_o_index *ix, const float *q, uint32_t n_probe,
                      uint32_t *out_post, float *out_dist) {
     uint32_t n = n_probe < 1 ? 1 : n_probe;
     if (ix->n_post == 0) return 0;
+    if (ix->graph && ix->n_post > 4096) {
+        uint32_t *gp = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1));
+        float *gd = (float *)malloc(sizeof(float) * (size_t)(n + 1));
+        uint32_t m = graph_probe(ix, q, n, gp, gd);
+        uint32_t cnt = 0;
+        float ratio = ix->probe_ratio;
+        for (uint32_t i = 0; i < m; i++) {
+            if (g_fault != 3 && isfinite(ratio) && gd[0] >= 0.0f && gd[i] > gd[0] * fmaxf(ratio, 1.0f))
+                break;
+            out_post[cnt] = gp[i];
+            out_dist[cnt] = gd[i];
+            cnt++;
+        }
+        free(gp);
+        free(gd);
+        return cnt;
+    }
     cand_t *all = (cand_t *)malloc(sizeof(cand_t) * ix->n_post);
     for (uint32_t p = 0; p < ix->n_post; p++) {
         float dd = spf_o_eval(ix->metric, q, ix->centroids + (size_t)p * ix->dim, ix->dim);
@@ -460,7 +688,8 @@ static uint32_t search_impl(const spf_o_index *ix, const float *q, uint32_t k, u
                             uint64_t *out_ids, float *out_dists, int filtered, uint64_t mask) {
     uint32_t d = ix->dim;
     uint32_t n = n_probe < 1 ? 1 : n_probe;
-    if (n > ix->n_post) n = ix->n_post;
+    /* the graph's beam depends on the unclamped probe count */
+    if (n > ix->n_post && !(ix->graph && ix->n_post > 4096)) n = ix->n_post;
     if (ix->n_post == 0 || k == 0) return 0;
     uint32_t *probes = (uint32_t *)malloc(sizeof(uint32_t) * n);
     float *pd = (float *)malloc(sizeof(float) * n);

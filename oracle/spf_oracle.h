@@ -7,6 +7,11 @@
  * search algorithm:
  *   Inner::search            src/spfresh.rs:671-727
  *   Inner::nearest (exact)   src/spfresh.rs:358-372
+ *   Inner::nearest (graph)   src/spfresh.rs:374-378, IncrementalDiskANN::
+ *                            search_with_dists src/incremental.rs:934-1003,
+ *                            UnifiedView src/incremental.rs:357-480,
+ *                            tombstone_config src/incremental.rs:489-499,
+ *                            beam_search src/lib.rs:198-303
  *   Inner::code_dist         src/spfresh.rs:344-354
  *   Inner::search_rabitq     src/spfresh.rs:731-770
  *   quantized_distance_from_codes src/quantized.rs:13-54
@@ -36,6 +41,22 @@ extern "C" {
 enum { SPF_O_L2 = 0, SPF_O_L2SQ = 1, SPF_O_COSINE = 2, SPF_O_DOT = 3 };
 enum { SPF_O_NONE = 0, SPF_O_F16 = 1, SPF_O_INT8 = 2, SPF_O_RABITQ = 3 };
 
+/* Centroid routing graph (the reference's IncrementalDiskANN over posting
+ * centroids: DANN base + delta layer + tombstones), in unified node ids: base
+ * nodes 0..base_n-1, then delta nodes (UnifiedView, src/incremental.rs:357-487). */
+typedef struct spf_o_graph {
+    uint32_t n_nodes;
+    uint32_t n_seeds;           /* entry points: base medoid, then base_n + delta entry */
+    uint32_t seeds[2];
+    uint32_t centroid_beam;     /* SPFreshConfig::centroid_beam */
+    uint64_t n_tombstones;      /* size of the tombstone set (tombstone_config) */
+    const float *vectors;       /* [n_nodes * dim] */
+    const uint32_t *adj_off;    /* [n_nodes + 1] */
+    const uint32_t *adj;        /* neighbor lists in stored order, PAD removed */
+    const uint8_t *live;        /* [n_nodes] 0 = tombstoned */
+    const int64_t *node_post;   /* [n_nodes] posting index of the node's cid, -1 = none */
+} spf_o_graph;
+
 typedef struct spf_o_index {
     uint32_t dim;
     uint32_t metric;
@@ -56,6 +77,7 @@ typedef struct spf_o_index {
     const uint8_t *deleted;     /* bitmap over ids < n_raw (may be NULL) */
     const float *rq_signs;      /* [3 * rq_padded] */
     const uint8_t *labels;      /* [n_raw] label of each id for filtered search (may be NULL) */
+    const spf_o_graph *graph;   /* routing graph used above 4,096 live postings (may be NULL) */
 } spf_o_index;
 
 /* ---- scalar building blocks (exported so tests can pin each one) ---- */
@@ -84,6 +106,10 @@ void spf_o_unrotate(float *v, uint32_t padded, const float *signs);
  * closest first after the probe_ratio cut; returns the count. */
 uint32_t spf_o_probe(const spf_o_index *ix, const float *q, uint32_t n_probe,
                      uint32_t *out_post, float *out_dist);
+/* IncrementalDiskANN::search_with_dists over the routing graph: up to k
+ * (unified node, distance) pairs in the reference's order; returns the count. */
+uint32_t spf_o_graph_search(const spf_o_graph *g, uint32_t metric, uint32_t dim, const float *q,
+                            uint32_t k, uint32_t beam, uint32_t *out_node, float *out_dist);
 /* One query (Inner::search).  Writes <= k results, returns the count. */
 uint32_t spf_o_search(const spf_o_index *ix, const float *q, uint32_t k, uint32_t n_probe,
                       uint64_t *out_ids, float *out_dists);
@@ -106,7 +132,8 @@ void spf_o_search_batch_filtered(const spf_o_index *ix, const float *q, const ui
 /* Planted-mistake switch for the tests that show the oracle checks have teeth.
  * 0 = faithful.  1 = drop the last dimension from f32 L2 sums.  2 = ignore
  * tombstones.  3 = skip the probe_ratio cut.  4 = round f16 by truncation.
- * 5 = ignore the label filter. */
+ * 5 = ignore the label filter.  6 = graph search stops on best > worst instead
+ * of best >= worst.  7 = graph search ignores the delta layer's entry point. */
 void spf_o_set_fault(int fault);
 
 #ifdef __cplusplus

@@ -30,6 +30,15 @@ FLAG_FULL_RERANK = 2
 STAGES = ("prep", "probe", "probe_select", "plan", "scan", "select", "rerank", "finalize")
 
 
+class GraphDesc(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_uint32), ("n_seeds", C.c_uint32), ("seeds", C.c_uint32 * 2),
+        ("centroid_beam", C.c_uint32), ("reserved", C.c_uint32), ("n_tombstones", C.c_uint64),
+        ("vectors", C.c_void_p), ("adj_off", C.c_void_p), ("adj", C.c_void_p),
+        ("live", C.c_void_p), ("node_post", C.c_void_p),
+    ]
+
+
 class IndexDesc(C.Structure):
     _fields_ = [
         ("dim", C.c_uint32), ("metric", C.c_uint32), ("quantizer", C.c_uint32),
@@ -41,7 +50,7 @@ class IndexDesc(C.Structure):
         ("int8_scale", C.c_float), ("int8_offset", C.c_float),
         ("rabitq_padded", C.c_uint32), ("rabitq_signs", C.c_void_p),
         ("probe_ratio", C.c_float), ("rerank_size", C.c_uint32), ("device", C.c_int32),
-        ("labels", C.c_void_p),
+        ("labels", C.c_void_p), ("graph", C.POINTER(GraphDesc)),
     ]
 
 
@@ -186,6 +195,22 @@ class DeviceIndex:
             if self._labels.shape[0] != ix.n_raw:
                 raise ValueError("labels must hold one u8 per id (n_raw)")
             d.labels = _p(self._labels)
+        self._graph = None
+        g = getattr(ix, "graph", None)
+        if g is not None:
+            keep = [np.ascontiguousarray(g.vectors, dtype=np.float32),
+                    np.ascontiguousarray(g.adj_off, dtype=np.uint32),
+                    np.ascontiguousarray(g.adj, dtype=np.uint32),
+                    np.ascontiguousarray(g.live, dtype=np.uint8),
+                    np.ascontiguousarray(g.node_post, dtype=np.int64)]
+            gd = GraphDesc()
+            gd.n_nodes, gd.n_seeds = keep[0].shape[0], len(g.seeds)
+            for i, sd in enumerate(g.seeds):
+                gd.seeds[i] = int(sd)
+            gd.centroid_beam, gd.n_tombstones = g.centroid_beam, g.n_tombstones
+            gd.vectors, gd.adj_off, gd.adj, gd.live, gd.node_post = (_p(a) for a in keep)
+            self._graph = (gd, keep)
+            d.graph = C.pointer(gd)
         h = C.c_void_p()
         check(lib().spf_index_create(C.byref(d), C.byref(h)))
         self.h = h

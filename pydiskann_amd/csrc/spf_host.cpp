@@ -82,14 +82,15 @@ struct DBuf {
     X(uint32_t, probe_pool_cnt) X(uint32_t, pool_want) X(uint32_t, cand_cnt)               \
     X(uint32_t, perm_cnt) X(uint32_t, perm_off) X(uint32_t, perm_fill) X(uint32_t, qperm)  \
     X(uint64_t, rq_planes) X(float, rq_consts) X(uint32_t, pair_src) X(float, eq16)        \
-    X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)
+    X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)        \
+    X(uint32_t, gflag) X(uint32_t, gscr)
 
 struct Workspace {
     hipStream_t stream = nullptr;
 #define SPF_WS_DECL(T, n) DBuf<T> n;
     SPF_WS_BUFFERS(SPF_WS_DECL)
 #undef SPF_WS_DECL
-    uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items]
+    uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items, -, graph overflows]
     hipEvent_t ev[SPF_N_STAGES + 1] = {};
 
     // device bytes the buffers hold now
@@ -133,6 +134,14 @@ struct spf_index {
     uint32_t *d_post_off = nullptr, *d_post_len = nullptr;
     uint32_t *d_block_sorted = nullptr;   // reference block index of each (sorted) posting
     uint8_t *d_label_e = nullptr;         // label of each entry (filtered search); null = none
+    // routing graph (probe above kBruteCentroids live postings); see spf_graph_desc
+    float *d_gvec = nullptr;              // [n_nodes][dpad]
+    uint32_t *d_gadj_off = nullptr, *d_gadj = nullptr;
+    uint8_t *d_glive = nullptr;
+    int32_t *d_gpost = nullptr;           // device posting of each node, -1 = none
+    uint32_t g_nodes = 0, g_n_seeds = 0, g_seeds[2] = {0, 0}, g_beam = 64;
+    uint64_t g_tomb = 0;
+    bool has_graph = false;
     uint4 *d_codes = nullptr;
     uint32_t *d_rank = nullptr;
     uint64_t *d_rank_id = nullptr;
@@ -175,7 +184,7 @@ struct spf_index {
         void *ps[] = {d_cent, d_cnorm, d_post_off, d_post_len, d_block_sorted, d_codes, d_rank,
                       d_rank_id, d_raw, d_del, d_xnorm, d_xsq, d_rank_entry, d_raw_e, d_xn16,
                       d_mu, d_cnbf, d_cent_rows, d_cbf, d_rq_signs, d_post_ex, d_post_xn,
-                      d_label_e};
+                      d_label_e, d_gvec, d_gadj_off, d_gadj, d_glive, d_gpost};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         if (codes16_owned && d_codes16) (void)hipFree(d_codes16);
@@ -218,7 +227,7 @@ std::vector<uint4> tile_rows(const uint8_t *rows, uint64_t n, uint32_t row_bytes
 Workspace *new_workspace() {
     Workspace *w = new Workspace();
     if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void **)&w->h_summary, 2 * sizeof(uint64_t)) != hipSuccess) {
+        hipHostMalloc((void **)&w->h_summary, 4 * sizeof(uint64_t)) != hipSuccess) {
         w->release();
         delete w;
         return nullptr;
@@ -491,6 +500,44 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         }
         HIPCHK(upload(&ix->d_label_e, lab.data(), lab.size(), bytes));
     }
+    if (d->graph) {   // routing graph, checked and mapped to device posting indices
+        const spf_graph_desc *g = d->graph;
+        const uint32_t G = g->n_nodes;
+        if (!g->vectors || !g->adj_off || !g->adj || !g->live || !g->node_post)
+            return fail(SPF_EINVAL, "graph: null array");
+        if (g->n_seeds > 2 || (G == 0 && g->n_seeds)) return fail(SPF_EINVAL, "graph: bad seeds");
+        for (uint32_t i = 0; i < g->n_seeds; i++)
+            if (g->seeds[i] >= G) return fail(SPF_EINVAL, "graph: seed beyond the nodes");
+        if (g->adj_off[0] != 0) return fail(SPF_EINVAL, "graph: adj_off[0] != 0");
+        for (uint32_t u = 0; u < G; u++)
+            if (g->adj_off[u + 1] < g->adj_off[u]) return fail(SPF_EINVAL, "graph: adj_off decreasing");
+        const uint32_t E = G ? g->adj_off[G] : 0;
+        for (uint32_t e = 0; e < E; e++)
+            if (g->adj[e] >= G) return fail(SPF_EINVAL, "graph: neighbor beyond the nodes");
+        std::vector<uint32_t> inv(std::max<uint32_t>(P, 1), 0);
+        for (uint32_t i = 0; i < P; i++) inv[perm[i]] = i;
+        std::vector<int32_t> post(std::max<uint32_t>(G, 1), -1);
+        for (uint32_t u = 0; u < G; u++) {
+            const int64_t pi = g->node_post[u];
+            if (pi >= (int64_t)P) return fail(SPF_EINVAL, "graph: node posting beyond the postings");
+            post[u] = pi < 0 ? -1 : (int32_t)inv[(uint32_t)pi];
+        }
+        std::vector<float> vec((size_t)std::max<uint32_t>(G, 1) * ix->dpad, 0.0f);
+        for (uint32_t u = 0; u < G; u++)
+            std::memcpy(&vec[(size_t)u * ix->dpad], g->vectors + (size_t)u * d->dim, d->dim * 4);
+        HIPCHK(upload(&ix->d_gvec, vec.data(), vec.size(), bytes));
+        HIPCHK(upload(&ix->d_gadj_off, g->adj_off, (size_t)G + 1, bytes));
+        HIPCHK(upload(&ix->d_gadj, g->adj, E, bytes));
+        HIPCHK(upload(&ix->d_glive, g->live, G, bytes));
+        HIPCHK(upload(&ix->d_gpost, post.data(), post.size(), bytes));
+        ix->g_nodes = G;
+        ix->g_n_seeds = g->n_seeds;
+        ix->g_seeds[0] = g->seeds[0];
+        ix->g_seeds[1] = g->seeds[1];
+        ix->g_beam = g->centroid_beam;
+        ix->g_tomb = g->n_tombstones;
+        ix->has_graph = true;
+    }
     HIPCHK(upload(&ix->d_rank_id, ix->rank_id.data(), ix->rank_id.size(), bytes));
     HIPCHK(upload(&ix->d_rank_entry, ix->rank_entry.data(), ix->rank_entry.size(), bytes));
     {
@@ -711,13 +758,20 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
 struct Shape {
     uint32_t npr, want, dpad, dpad32, Gv, G, echunk, band, Pp;
     bool mixed, rerank, mfma_scan, mprobe, round16, q8;
+    bool graph;   // probe through the routing graph (above kBruteCentroids live postings)
+    uint32_t nraw;   // the probe count as asked (the graph's result count and beam)
 };
+
+// Inner::nearest scans every centroid up to this many live postings and
+// searches the routing graph above it (src/spfresh.rs:20,358-378)
+constexpr uint32_t kBruteCentroids = 4096;
 
 int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
                  Shape *sh) {
     const uint32_t P = ix->P;
     Shape &z = *sh;
     z.npr = std::min(std::max(n_probe, 1u), std::max(P, 1u));
+    z.nraw = std::max(n_probe, 1u);
     z.mixed = opts && opts->precision == SPF_PREC_MIXED;
     if (opts && opts->precision > SPF_PREC_MIXED) return fail(SPF_EINVAL, "unknown precision");
     if (z.mixed && !ix->rq_l2 && (!ix->d_codes16 || ix->dim > 256))
@@ -755,12 +809,17 @@ int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_se
     z.q8 = ix->quant == SPF_QUANT_INT8 &&
            (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
     // matrix-core probe for L2 metrics when it has something to prune
-    z.mprobe = z.mixed && (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ) &&
+    z.graph = ix->has_graph && P > kBruteCentroids;
+    z.mprobe = !z.graph && z.mixed && (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ) &&
                P > z.npr && ix->dim <= 256 && z.npr + z.npr / 4 + 32 <= kMaxK;
     z.Pp = round_up(P, 32);
     z.band = std::min<uint32_t>(P, std::min<uint32_t>(kMaxK, std::max(z.npr + 32, z.npr + z.npr / 4)));
     return SPF_OK;
 }
+
+bool graph_shape(const spf_index *ix, uint32_t n, GraphParams *gp);
+uint64_t graph_scratch_words(const spf_index *ix, const GraphParams &g);
+extern const uint64_t kGraphScratchBytes;
 
 // Upper bound on the device bytes one search call of nq queries adds to a
 // Workspace: every buffer it ensures, at DBuf's growth rule, with the
@@ -797,7 +856,14 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     }
     add(pairs, 8);               // probe_sel
     add(q, 4);                   // probe_cnt
-    if (z.mprobe) {
+    if (z.graph) {
+        add(1, 4);               // gflag
+        GraphParams g{};
+        if (graph_shape(ix, z.nraw, &g)) {   // global-mode scratch, at most one launch's worth
+            const uint64_t words = graph_scratch_words(ix, g);
+            add(std::min<uint64_t>(q, std::max<uint64_t>(1, kGraphScratchBytes / (words * 4))) * words, 4);
+        }
+    } else if (z.mprobe) {
         add(q * z.Pp, 2);
         add(q * z.band, 4);
         add(q, 4);
@@ -850,6 +916,104 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
         }
     }
     return b;
+}
+
+// Graph-probe sizes for n asked results (the unclamped probe count): beam
+// max(centroid_beam, n), tombstone_config (src/incremental.rs:489-499), LDS
+// layout.  Returns false if the beam is beyond any capacity.
+bool graph_shape(const spf_index *ix, uint32_t n, GraphParams *gp) {
+    GraphParams &g = *gp;
+    g.k = n;
+    const uint32_t beam = std::max(ix->g_beam, n);
+    g.filtered = ix->g_tomb > 0;
+    if (g.filtered) {
+        const uint64_t t = std::min<uint64_t>(ix->g_tomb, beam);
+        const uint64_t wb = std::max<uint64_t>(2ull * beam, (uint64_t)n + t);
+        if (wb > (1u << 24)) return false;
+        g.wb = (uint32_t)wb;
+        g.max_it = (uint32_t)std::min<uint64_t>(2 * wb, 0xffffffffull);
+        g.early = 1.5f;
+    } else {
+        g.wb = beam;
+        g.max_it = 0xffffffffu;
+        g.early = INFINITY;
+    }
+    auto even = [](uint64_t v) { return (uint32_t)((v + 1) & ~1ull); };
+    g.wcap = even(std::max(g.wb, 2u));
+    g.rcap = even((uint64_t)n + 2);
+    g.bitmap_words = (ix->g_nodes + 127) / 128 * 4;
+    g.dpad = ix->dpad;
+    // LDS mode: the frontier takes what 64 KiB (less the static LDS) leaves;
+    // at least the w heap's size (the slot-by-rank table reuses it)
+    g.scratch = nullptr;
+    g.fcap = 0;
+    const size_t limit = 65536 - 256, used = graph_lds_bytes(g);
+    if (used < limit) g.fcap = std::min<uint32_t>(even(ix->g_nodes + 2), (uint32_t)((limit - used) / 8)) & ~1u;
+    return true;
+}
+
+// global-mode scratch words per query: the frontier holds every node at most once
+uint64_t graph_scratch_words(const spf_index *ix, const GraphParams &g) {
+    GraphParams t = g;
+    t.fcap = (ix->g_nodes + 3) & ~1u;
+    t.fcap = std::max(t.fcap, t.wcap);
+    return graph_state_bytes(t) / 4;
+}
+
+const uint64_t kGraphScratchBytes = 256ull << 20;   // global-mode scratch per launch
+
+// Routing-graph probe of nq prepared queries (qv_probe, qnorm) into out /
+// out_cnt (rows of npr).  The state of a query lives in LDS when it fits; a
+// query whose frontier outgrows its LDS share (or a beam too wide for LDS)
+// runs again with its state in global scratch, where the frontier holds every
+// node, so the result never depends on the capacity.
+int graph_probe(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_t n,
+                uint32_t npr, uint64_t *out, uint32_t *out_cnt) {
+    GraphParams g{};
+    if (!graph_shape(ix, n, &g)) return fail(SPF_EUNSUPPORTED, "routing graph: beam too wide");
+    g.vec = ix->d_gvec;
+    g.adj_off = ix->d_gadj_off;
+    g.adj = ix->d_gadj;
+    g.live = ix->d_glive;
+    g.node_post = ix->d_gpost;
+    g.n_seeds = ix->g_n_seeds;
+    g.seed0 = ix->g_seeds[0];
+    g.seed1 = ix->g_seeds[1];
+    g.metric = ix->metric;
+    g.qv = w->qv_probe.p;
+    g.qnorm = w->qnorm.p;
+    g.nq = nq;
+    g.npr = npr;
+    g.out = out;
+    g.out_cnt = out_cnt;
+    HIPCHK(w->gflag.ensure(1));
+    g.overflow = w->gflag.p;
+    bool global = g.fcap < g.wcap;
+    if (!global) {
+        HIPCHK(hipMemsetAsync(w->gflag.p, 0, sizeof(uint32_t), s));
+        g.q0 = 0;
+        HIPCHK(launch_graph_probe(g, nq, s));
+        if (g.fcap < ix->g_nodes + 2) {   // the frontier could have outgrown LDS
+            w->h_summary[3] = 0;
+            HIPCHK(hipMemcpyAsync(&w->h_summary[3], w->gflag.p, sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            global = w->h_summary[3] != 0;
+        }
+    }
+    if (global) {
+        g.fcap = std::max<uint32_t>((ix->g_nodes + 3) & ~1u, g.wcap);
+        g.scratch_words = graph_scratch_words(ix, g);
+        const uint64_t per = g.scratch_words * 4;
+        const uint32_t chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nq, kGraphScratchBytes / per));
+        HIPCHK(w->gscr.ensure((size_t)chunk * g.scratch_words));
+        g.scratch = w->gscr.p;
+        for (uint32_t q0 = 0; q0 < nq; q0 += chunk) {
+            g.q0 = q0;
+            HIPCHK(launch_graph_probe(g, std::min(chunk, nq - q0), s));
+        }
+    }
+    return SPF_OK;
 }
 
 int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, uint32_t nq,
@@ -934,7 +1098,12 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     // ---- probe: every query x every centroid, then top-npr per query
     HIPCHK(w->probe_sel.ensure((size_t)nq * npr));
     HIPCHK(w->probe_cnt.ensure(nq));
-    if (mprobe) {
+    if (z.graph) {
+        HIPCHK(mark(SPF_STAGE_PROBE_SELECT));
+        if (int rc = graph_probe(ix, w, s, nq, std::max(n_probe, 1u), npr, w->probe_sel.p,
+                                 w->probe_cnt.p))
+            return rc;
+    } else if (mprobe) {
         // bf16 matrix-core distances -> best npr + margin by 16-bit key -> exact
         // f32 recheck of those centroids -> sorted probe list
         const uint32_t Pp = z.Pp, band = z.band;
@@ -1234,6 +1403,8 @@ int nearest_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_v, u
     HIPCHK(launch_prep(pp, s));
     if (P == 0) {
         HIPCHK(hipMemsetAsync(w->probe_cnt.p, 0, n * sizeof(uint32_t), s));
+    } else if (ix->has_graph && P > kBruteCentroids) {   // nearest(v, 1) over the graph
+        if (int rc = graph_probe(ix, w, s, n, 1, 1, w->probe_sel.p, w->probe_cnt.p)) return rc;
     } else {
         HIPCHK(w->probe_keys.ensure((size_t)n * P));
         ScanParams sp{};

@@ -100,6 +100,44 @@ struct PrepParams {
     float *qnbf;                // [nq] |bf16(q - mu)|^2
 };
 
+// Routing-graph probe (one wave per query): beam search over the centroid
+// graph with the reference's heap order; writes probe keys like the exact probe.
+struct GraphParams {
+    const float *vec;           // [n_nodes][dpad] node vectors, zero padded
+    const uint32_t *adj_off;    // [n_nodes + 1]
+    const uint32_t *adj;        // neighbor lists in stored order
+    const uint8_t *live;        // [n_nodes] 0 = tombstoned
+    const int32_t *node_post;   // [n_nodes] device posting index, -1 = none
+    uint32_t n_seeds, seed0, seed1;
+    uint32_t metric, dpad;
+    uint32_t k;                 // results asked (the unclamped probe count)
+    uint32_t wb;                // working beam
+    uint32_t wcap, fcap, rcap;  // LDS entries of w, frontier, results (even)
+    uint32_t bitmap_words;      // visited bitmap words (multiple of 4)
+    uint32_t filtered;          // tombstones present: expanded beam, live results
+    uint32_t max_it;
+    float early;                // early-termination factor (inf = none)
+    const float *qv;            // [nq][dpad] queries
+    const float *qnorm;         // [nq] |q|^2
+    uint32_t nq, npr;           // out row stride
+    uint64_t *out;              // [nq][npr] fkey(dist) << 32 | posting
+    uint32_t *out_cnt;          // [nq]
+    uint32_t *overflow;         // queries whose frontier outgrew fcap
+    // global mode (state too large for LDS): bitmap and heaps of query
+    // q0 + blockIdx.x at scratch + blockIdx.x * scratch_words
+    uint32_t *scratch;          // null = LDS mode
+    uint64_t scratch_words;
+    uint32_t q0;
+};
+
+// bytes of the visited bitmap and the three heaps of one query
+inline size_t graph_state_bytes(const GraphParams &p) {
+    return (size_t)p.bitmap_words * 4 + (size_t)(p.wcap + p.fcap + (p.filtered ? p.rcap : 0)) * 8;
+}
+inline size_t graph_lds_bytes(const GraphParams &p) {
+    return (p.scratch ? 0 : graph_state_bytes(p)) + (size_t)p.dpad * 4 + 64 * 8;
+}
+
 // Matrix-core centroid probe: 16-bit keys of |c-mu|^2 + |q-mu|^2 - 2(q-mu).(c-mu)
 // in bf16 with f32 accumulate.
 struct ProbeMfmaParams {
@@ -301,6 +339,7 @@ hipError_t launch_item_table(const PlanParams &p, hipStream_t s);
 hipError_t launch_excl_scan_u64(const uint64_t *in, uint64_t *out, uint32_t n, hipStream_t s);
 hipError_t launch_excl_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t s);
 hipError_t launch_rerank(const RerankParams &p, hipStream_t s);
+hipError_t launch_graph_probe(const GraphParams &p, uint32_t n, hipStream_t s);
 hipError_t launch_nearest_out(const uint64_t *probe_sel, const uint32_t *probe_cnt, uint32_t n,
                               const uint32_t *block_sorted, uint32_t *out_block, float *out_dist,
                               hipStream_t s);

@@ -11,6 +11,8 @@ reproducible on any host.
            (concentration 50), component weights Zipf(1.1).
 * config4  5M x 768 unit vectors, covariance eigenvalues i^-0.8 under a random
            rotation.
+* labels   one of 64 uniform labels per base vector, and per-query allow masks
+           of 1-8 random labels (the filtered serving workload over config3).
 """
 from __future__ import annotations
 
@@ -54,7 +56,10 @@ def _normalize(x):
 
 
 def config3(n: int = 10_000_000, nq: int = 10_000, dim: int = 96, comps: int = 16_384,
-            kappa: float = 50.0, zipf: float = 1.1, chunk: int = 500_000):
+            kappa: float = 50.0, zipf: float = 1.1, chunk: int = 500_000,
+            return_means: bool = False):
+    """(base, queries[, component mean directions]); with one posting per
+    component the posting sizes follow the Zipf weights."""
     r_c, r_w, r_b, r_q = _streams(4)
     mu = _normalize(r_c.normal(size=(comps, dim)))
     w = 1.0 / np.arange(1, comps + 1) ** zipf
@@ -71,7 +76,23 @@ def config3(n: int = 10_000_000, nq: int = 10_000, dim: int = 96, comps: int = 1
         return out
 
     # queries uniform over components (each component equally likely)
-    return draw(r_b, n, w), draw(r_q, nq, np.full(comps, 1.0 / comps))
+    base, queries = draw(r_b, n, w), draw(r_q, nq, np.full(comps, 1.0 / comps))
+    return (base, queries, mu) if return_means else (base, queries)
+
+
+def labels(n: int, n_labels: int = 64):
+    """One uniform label in [0, n_labels) per base vector (u8)."""
+    return _streams(5)[4].integers(0, n_labels, size=n).astype(np.uint8)
+
+
+def label_masks(nq: int, seed: int = 1, lo: int = 1, hi: int = 8, n_labels: int = 64):
+    """Per-query u64 allow masks of lo..hi distinct random labels."""
+    r = _streams(6 + seed)[5 + seed]
+    out = np.zeros(nq, dtype=np.uint64)
+    for i in range(nq):
+        for b in r.choice(n_labels, size=int(r.integers(lo, hi + 1)), replace=False):
+            out[i] |= np.uint64(1) << np.uint64(int(b))
+    return out
 
 
 def config4(n: int = 5_000_000, nq: int = 10_000, dim: int = 768, chunk: int = 250_000):

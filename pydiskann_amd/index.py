@@ -58,6 +58,35 @@ def code_bytes_for(quant: int, dim: int, rq_padded: int = 0) -> int:
     raise ValueError(quant)
 
 
+DELTA_ID_OFFSET = 1 << 48   # global id of delta node i = DELTA_ID_OFFSET + i (src/incremental.rs:334)
+
+
+@dataclass
+class CentroidGraph:
+    """The reference's centroid routing graph (an IncrementalDiskANN over the
+    posting centroids: DANN base + delta layer + tombstones), which
+    Inner::nearest searches above 4,096 live postings (src/spfresh.rs:20,
+    374-378), in unified node ids: base nodes 0..base_n-1, then delta nodes
+    (UnifiedView, src/incremental.rs:357-487)."""
+    vectors: np.ndarray            # f32 [n, d] node vectors (base, then delta)
+    adj_off: np.ndarray            # u32 [n + 1]
+    adj: np.ndarray                # u32 [E] neighbor lists in stored order, PAD removed
+    live: np.ndarray               # u8 [n] 0 = tombstoned
+    seeds: list                    # entry points: base medoid, then base_n + delta entry
+    node_post: np.ndarray          # i64 [n] posting index of the node's cid, -1 = none
+    n_tombstones: int = 0          # size of the tombstone set
+    centroid_beam: int = 64        # SPFreshConfig::centroid_beam
+    base_n: int = 0
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.vectors.shape[0])
+
+    def cid(self, node: int) -> int:
+        """Global u64 id of a unified node (UnifiedView::to_global_u64)."""
+        return node if node < self.base_n else DELTA_ID_OFFSET + node - self.base_n
+
+
 @dataclass
 class IndexArrays:
     dim: int
@@ -77,6 +106,7 @@ class IndexArrays:
     probe_ratio: float = math.inf
     rerank_size: int = 0
     labels: np.ndarray = None      # optional u8 [n_raw] label per id (label-filtered search)
+    graph: CentroidGraph = None    # routing graph above 4,096 postings (None = exact probe)
 
     def __post_init__(self):
         self.block_ids = np.ascontiguousarray(self.block_ids, dtype=np.uint32)
