@@ -212,8 +212,12 @@ int spf_nearest_postings(spf_index *ix, const float *vectors, uint32_t n, uint32
  * with the same settings, up to max_batch rows, as one device batch, so
  * batches of several slots run on the device at once.  A slot starts a batch
  * only if the working memory all slots hold or have reserved plus the batch's
- * upper bound stays within hbm_budget_bytes; requests whose single query
- * exceeds the budget fail at submit with SPF_ENOMEM.  Every row's result is
+ * upper bound before its candidate buffers stays within hbm_budget_bytes;
+ * once the probe has fixed the candidate count the batch reserves exactly what
+ * those buffers add, waiting for running batches to finish if needed (a batch
+ * that would wait on batches that are themselves waiting goes back to the
+ * queue; one too large for the budget alone is split).  A single query that
+ * cannot fit fails with SPF_ENOMEM (at submit when its bound already shows it).  Every row's result is
  * that of spf_search_batch(_filtered) on the same query.  The index must
  * outlive the server and must not be modified (spf_delete) while requests
  * are in flight. */
@@ -239,7 +243,9 @@ typedef struct {
     uint64_t admission_waits;     /* times the oldest request did not fit the budget */
     uint64_t trims;               /* idle slots that gave their memory back */
     uint32_t max_concurrent;      /* most batches on the device at once */
-    uint32_t reserved[3];
+    uint32_t requeues;            /* batches handed back while others held the memory */
+    uint32_t splits;              /* batches handed back to run as smaller batches */
+    uint32_t reserved[1];
 } spf_server_stats;
 
 int spf_server_create(spf_index *ix, const spf_server_opts *opts, spf_server **out);

@@ -88,7 +88,8 @@ class ServerStats(C.Structure):
                 ("batches", C.c_uint64), ("rejected", C.c_uint64),
                 ("peak_reserved_bytes", C.c_uint64), ("budget_overruns", C.c_uint64),
                 ("admission_waits", C.c_uint64), ("trims", C.c_uint64),
-                ("max_concurrent", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("max_concurrent", C.c_uint32), ("requeues", C.c_uint32),
+                ("splits", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
 
 class SpfError(RuntimeError):

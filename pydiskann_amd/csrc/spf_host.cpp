@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -827,7 +828,7 @@ extern const uint64_t kGraphScratchBytes;
 // count by the posting chunk counts.  host_io adds the staging buffers of the
 // host-memory entry points.
 uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, bool host_io,
-                  bool filtered) {
+                  bool filtered, bool candidates = true) {
     const uint64_t q = nq, P = ix->P, npr = z.npr, pairs = q * npr;
     uint64_t b = 0;
     auto add = [&](uint64_t n, uint64_t size) { b += (n + n / 4 + 64) * size; };
@@ -889,16 +890,18 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     const uint64_t cmax = (ix->max_len + z.echunk - 1) / z.echunk;
     const uint64_t call = z.echunk == kEChunk ? ix->chunks_v : ix->chunks_m;
     const uint64_t items = (pairs + z.G - 1) / z.G * cmax + std::min(call, pairs * cmax);
-    add(std::max<uint64_t>(items, 1), 4);   // item_post
+    if (candidates) add(std::max<uint64_t>(items, 1), 4);   // item_post
     if (ix->rq_l2) {
         add(pairs * 4 * (ix->rq_padded / 64), 8);
         add(pairs * 8, 4);
     }
     const uint64_t total = q * ix->top_len[std::min<uint64_t>(npr, P)];
     if (z.mixed) {
-        add(std::max<uint64_t>(total, 1), 2);       // cand16
-        add(std::max<uint64_t>(total / 32, 1), 4);  // sel_bits
-        add(std::max<uint64_t>(total / 32, 1), 4);  // a_bits
+        if (candidates) {
+            add(std::max<uint64_t>(total, 1), 2);       // cand16
+            add(std::max<uint64_t>(total / 32, 1), 4);  // sel_bits
+            add(std::max<uint64_t>(total / 32, 1), 4);  // a_bits
+        }
         add(q, 4);                                  // pool_want
         add(q * k, 8);                              // fin
         add(q, 4);                                  // fin_cnt
@@ -909,7 +912,7 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
         add(q * k, 8);
         add(q, 4);
     } else {
-        add(std::max<uint64_t>(total, 1), 8);       // cand
+        if (candidates) add(std::max<uint64_t>(total, 1), 8);   // cand
         if (z.rerank) {
             add(q * k, 8);
             add(q, 4);
@@ -1016,9 +1019,15 @@ int graph_probe(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_
     return SPF_OK;
 }
 
+// Called once the candidate count is known, with the bytes the candidate
+// buffers are about to add to the workspace; a nonzero return ends the search
+// with that code before anything is allocated.
+using Reserve = std::function<int(uint64_t)>;
+
 int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, uint32_t nq,
                 uint32_t k, uint32_t n_probe, const spf_search_opts *opts, uint64_t *d_ids,
-                float *d_dists, uint32_t *d_counts, const uint64_t *d_masks) {
+                float *d_dists, uint32_t *d_counts, const uint64_t *d_masks,
+                const Reserve *reserve = nullptr) {
     const uint32_t P = ix->P;
     if (d_masks && !ix->d_label_e) return fail(SPF_EINVAL, "filtered search on an index without labels");
     Shape z;
@@ -1225,6 +1234,22 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(hipStreamSynchronize(s));
     const uint64_t total = w->h_summary[0];
     const uint32_t n_items = (uint32_t)w->h_summary[1];
+    if (reserve) {   // growth of the candidate-sized buffers, at DBuf's rule
+        uint64_t extra = 0;
+        auto grow = [&](const auto &b, uint64_t n, uint64_t size) {
+            if (n > b.cap || !b.p) extra += (n + n / 4 + 64) * size - (uint64_t)b.cap * size;
+        };
+        grow(w->item_post, std::max<uint32_t>(n_items, 1), 4);
+        if (mixed) {
+            grow(w->cand16, std::max<uint64_t>(total, 1), 2);
+            grow(w->sel_bits, std::max<uint64_t>(total / 32, 1), 4);
+            grow(w->a_bits, std::max<uint64_t>(total / 32, 1), 4);
+        } else if (!ix->rq_l2) {
+            grow(w->cand, std::max<uint64_t>(total, 1), 8);
+        }
+        if (extra)
+            if (int rc = (*reserve)(extra)) return rc;
+    }
     HIPCHK(w->item_post.ensure(std::max<uint32_t>(n_items, 1)));
     pl.item_off = w->item_off.p;
     pl.item_post = w->item_post.p;
@@ -1445,7 +1470,8 @@ int check_search_args(spf_index *ix, uint32_t dim, uint32_t k) {
 // down; returns after the stream has drained.
 int host_batch(spf_index *ix, Workspace *w, const float *queries, uint32_t nq, uint32_t k,
                uint32_t n_probe, const spf_search_opts *opts, const uint64_t *masks,
-               uint64_t *out_ids, float *out_dists, uint32_t *out_counts) {
+               uint64_t *out_ids, float *out_dists, uint32_t *out_counts,
+               const Reserve *reserve = nullptr) {
     hipStream_t s = w->stream;
     const uint32_t dim = ix->dim;
     HIPCHK(w->qin.ensure((size_t)nq * dim));
@@ -1458,7 +1484,7 @@ int host_batch(spf_index *ix, Workspace *w, const float *queries, uint32_t nq, u
         HIPCHK(hipMemcpyAsync(w->qmask.p, masks, (size_t)nq * 8, hipMemcpyHostToDevice, s));
     }
     int r = search_impl(ix, w, s, w->qin.p, nq, k, n_probe, opts, w->out_ids.p, w->out_d.p,
-                        w->out_c.p, masks ? w->qmask.p : nullptr);
+                        w->out_c.p, masks ? w->qmask.p : nullptr, reserve);
     if (r) {
         (void)hipStreamSynchronize(s);
         return r;
@@ -1518,14 +1544,15 @@ struct ServerExec {
         const spf_search_opts o = opts_of(key);
         Shape z;
         if (search_shape(ix, key.k, key.n_probe, &o, &z)) return UINT64_MAX;
-        return ws_bound(ix, z, nq, key.k, true, key.filtered);
+        return ws_bound(ix, z, nq, key.k, true, key.filtered, false);
     }
     uint64_t held(int s) { return slots[s].w->bytes(); }
     void trim(int s) {
         DeviceGuard dg(ix->device);
         slots[s].w->trim();
     }
-    int run(int s, const spf_sched::Batch &b, std::string *err) {
+    template <class Grow>
+    int run(int s, const spf_sched::Batch &b, std::string *err, Grow &grow) {
         DeviceGuard dg(ix->device);
         Slot &sl = slots[s];
         const uint32_t nq = b.nq, dim = ix->dim, k = b.key.k;
@@ -1548,9 +1575,11 @@ struct ServerExec {
                 row += p.n;
             }
             const spf_search_opts o = opts_of(b.key);
+            const Reserve reserve = [&grow](uint64_t extra) { return grow(extra); };
             rc = host_batch(ix, sl.w, sl.h_q, nq, k, b.key.n_probe, &o,
-                            b.key.filtered ? sl.h_m : nullptr, sl.h_ids, sl.h_d, sl.h_c);
+                            b.key.filtered ? sl.h_m : nullptr, sl.h_ids, sl.h_d, sl.h_c, &reserve);
         }
+        if (rc == spf_sched::kRequeue || rc == spf_sched::kSplit) return rc;
         if (rc) {
             *err = g_err;
             return rc;
@@ -1814,6 +1843,8 @@ int spf_server_get_stats(spf_server *sv, spf_server_stats *out) {
     out->admission_waits = st.admission_waits;
     out->trims = st.trims;
     out->max_concurrent = st.max_concurrent;
+    out->requeues = st.requeues;
+    out->splits = st.splits;
     return SPF_OK;
 }
 

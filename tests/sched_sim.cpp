@@ -28,12 +28,12 @@ static int g_fail = 0;
 struct SimExec {
     uint64_t budget;
     std::mutex mu;
-    std::vector<uint64_t> held_b;
+    std::vector<uint64_t> held_b, cand_b;   // all buffers; the candidate part of them
     uint64_t over_budget = 0;
     uint32_t fail_probe = 999;   // n_probe that makes a batch fail
     int sleep_us = 200;
 
-    explicit SimExec(int slots, uint64_t b) : budget(b), held_b(slots, 0) {}
+    explicit SimExec(int slots, uint64_t b) : budget(b), held_b(slots, 0), cand_b(slots, 0) {}
     int validate(const Key &key, uint32_t dim, std::string *err) {
         if (dim != 4) {
             *err = "dim";
@@ -53,13 +53,38 @@ struct SimExec {
     void trim(int s) {
         std::lock_guard<std::mutex> g(mu);
         held_b[s] = 0;
+        cand_b[s] = 0;
     }
-    int run(int s, const Batch &b, std::string *err) {
-        {   // buffers grow to this batch's need, as a workspace does
+    // candidate bytes of a row, known only once the batch runs (row value
+    // 1e9 marks a row too large for any budget)
+    static uint64_t cand_bytes(const float *q) { return q[2] >= 1e9f ? (1ull << 40) : (uint64_t)q[2]; }
+    void grow_to(int s, uint64_t v) {
+        std::lock_guard<std::mutex> g(mu);
+        held_b[s] = std::max(held_b[s], v);
+        uint64_t t = 0;
+        for (uint64_t x : held_b) t += x;
+        if (t > budget) over_budget++;
+    }
+    template <class Grow>
+    int run(int s, const Batch &b, std::string *err, Grow &grow) {
+        // phase-1 buffers grow to this batch's need, as a workspace does
+        grow_to(s, bound(b.key, b.nq) - 1024);
+        uint64_t extra = 0;
+        for (const Piece &p : b.pieces)
+            for (uint32_t i = 0; i < p.n; i++) extra += cand_bytes(p.r->q + (size_t)(p.row0 + i) * 4);
+        uint64_t more;   // candidate buffers grow to this batch's need
+        {
             std::lock_guard<std::mutex> g(mu);
-            held_b[s] = std::max(held_b[s], bound(b.key, b.nq) - 1024);
+            more = extra > cand_b[s] ? extra - cand_b[s] : 0;
+        }
+        if (more)
+            if (int g = grow(more)) return g;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            held_b[s] += more;
+            cand_b[s] += more;
             uint64_t t = 0;
-            for (uint64_t v : held_b) t += v;
+            for (uint64_t x : held_b) t += x;
             if (t > budget) over_budget++;
         }
         std::this_thread::sleep_for(std::chrono::microseconds(sleep_us));
@@ -102,6 +127,7 @@ static Req make_req(std::mt19937_64 &rng, uint32_t nq, uint32_t k, uint32_t n_pr
     r.nq = nq;
     r.q.resize((size_t)nq * 4);
     for (auto &v : r.q) v = (float)(rng() % 100000);
+    for (uint32_t i = 0; i < nq; i++) r.q[(size_t)i * 4 + 2] = (float)(rng() % 2000);
     if (filt) {
         r.masks.resize(nq);
         for (auto &m : r.masks) m = rng();
@@ -246,12 +272,66 @@ static void tight_budget_trims() {
     mixed_load(3, budget, 60, 3, 11, false);
 }
 
+static void candidate_heavy() {
+    // rows whose candidate bytes dominate: batches must wait, yield or split
+    SimExec probe(1, 0);
+    Key k10;
+    k10.k = 10;
+    const uint64_t budget = probe.bound(k10, 256) * 3 + 300000;
+    SimExec ex(3, budget);
+    std::mutex mu;
+    std::vector<Req> reqs;
+    std::mt19937_64 rng(21);
+    for (int i = 0; i < 80; i++) {
+        Req r = make_req(rng, 1 + (uint32_t)(rng() % 256), 10, 16, i % 2);
+        for (uint32_t j = 0; j < r.nq; j++) r.q[(size_t)j * 4 + 2] = (float)(rng() % 4000);
+        reqs.push_back(std::move(r));
+    }
+    Req huge = make_req(rng, 3, 10, 16, false);
+    huge.q[4 + 2] = 1e9f;   // row 1 can never fit
+    {
+        Scheduler<SimExec> sch(&ex, 3, 1024, budget);
+        std::vector<std::thread> th;
+        for (int t = 0; t < 4; t++)
+            th.emplace_back([&, t] {
+                for (int i = t; i < (int)reqs.size(); i += 4) {
+                    Req &r = reqs[i];
+                    std::string err;
+                    CHECK(sch.submit(r.key, r.q.data(), r.nq, 4,
+                                     r.key.filtered ? r.masks.data() : nullptr, r.ids.data(),
+                                     r.d.data(), r.c.data(), &r.ticket, &err) == SPF_OK);
+                }
+                for (int i = t; i < (int)reqs.size(); i += 4) {
+                    std::string err;
+                    CHECK(sch.wait(reqs[i].ticket, -1, nullptr, &err) == SPF_OK);
+                    CHECK(rows_ok(reqs[i]));
+                }
+            });
+        std::string err;
+        CHECK(sch.submit(huge.key, huge.q.data(), huge.nq, 4, nullptr, huge.ids.data(),
+                         huge.d.data(), huge.c.data(), &huge.ticket, &err) == SPF_OK);
+        CHECK(sch.wait(huge.ticket, -1, nullptr, &err) == SPF_ENOMEM);
+        for (auto &t : th) t.join();
+        const spf_sched::Stats st = sch.stats();
+        CHECK(st.overruns == 0);
+        CHECK(st.peak_reserved <= budget);
+        CHECK(st.splits > 0);
+        std::printf("candidate-heavy: %llu batches, requeues %u, splits %u, waits %llu, "
+                    "peak %llu of %llu\n",
+                    (unsigned long long)st.batches, st.requeues, st.splits,
+                    (unsigned long long)st.admission_waits,
+                    (unsigned long long)st.peak_reserved, (unsigned long long)budget);
+    }
+    CHECK(ex.over_budget == 0);
+}
+
 int main() {
     mixed_load(3, 64ull << 20, 400, 6, 1);
     mixed_load(1, 64ull << 20, 100, 2, 2);
     tight_budget_trims();
     edge_cases();
     drain_on_destroy();
+    candidate_heavy();
     if (g_fail) return 1;
     std::printf("sched ok\n");
     return 0;
