@@ -45,6 +45,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def knn_routing_graph(ix, torch, r: int):
+    """Routing graph for the graph-probe path: each centroid linked to its r
+    nearest centroids (closest first), entry point the centroid nearest the
+    mean.  The reference builds a Vamana graph instead (src/lib.rs:436-489);
+    any graph exercises the same walk."""
+    import numpy as np
+    from pydiskann_amd.index import CentroidGraph
+    c = torch.from_numpy(ix.centroids).cuda()
+    n = c.shape[0]
+    nbr = torch.empty((n, r), dtype=torch.int64)
+    cn = (c * c).sum(1)
+    for s in range(0, n, 4096):
+        d = cn[s:s + 4096, None] + cn[None, :] - 2 * c[s:s + 4096] @ c.T
+        d[torch.arange(d.shape[0]), torch.arange(s, s + d.shape[0])] = float("inf")
+        nbr[s:s + 4096] = torch.topk(d, r, dim=1, largest=False).indices.cpu()
+    medoid = int(((c - c.mean(0)) ** 2).sum(1).argmin())
+    adj_off = np.arange(0, (n + 1) * r, r, dtype=np.uint32)
+    return CentroidGraph(vectors=ix.centroids.copy(), adj_off=adj_off,
+                         adj=nbr.numpy().astype(np.uint32).ravel(), live=np.ones(n, np.uint8),
+                         seeds=[medoid], node_post=np.arange(n, dtype=np.int64), n_tombstones=0,
+                         centroid_beam=64, base_n=n)
+
+
 def build_workload(args, torch):
     import numpy as np
     from pydiskann_amd import build_index
@@ -60,6 +83,9 @@ def build_workload(args, torch):
         ix = build_index(base, "dot", "int8", centroids=mu, rerank_size=400, device="cuda")
         if args.workload == "c5":
             ix.labels = labels(ix.n_raw)
+        if args.graph:
+            ix.graph = knn_routing_graph(ix, torch, args.graph)
+            log(f"[bench] routing graph: {ix.graph.n_nodes} nodes, {args.graph} neighbours each")
         log(f"[bench] index {ix.n_post} postings, max len {int(ix.posting_lengths().max())} "
             f"in {time.time() - t0:.1f}s")
         return base, queries, ix, 32, 10, np
@@ -217,6 +243,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--recall-sample", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="c3/c5: probe through a routing graph of this many neighbours per "
+                         "centroid (the reference's path above 4,096 postings)")
     ap.add_argument("--requests", type=int, default=4000, help="c5: requests per load point")
     ap.add_argument("--target-qps", type=float, default=0,
                     help="c5: open-loop query rate (0 = 50%% and 80%% of measured capacity)")
@@ -360,7 +389,9 @@ def main() -> int:
                                f"{QUANT_NAMES[ix.quant]}, {ix.n_post} postings, "
                                f"n_probe {n_probe}, k {k}, {nq} queries per step",
                    "precision": args.precision,
-                   "rerank_pool": args.rerank if args.precision == "mixed" else 0},
+                   "probe": f"routing graph ({args.graph} neighbours)" if args.graph else "exact",
+                   "rerank_pool": args.rerank if args.precision == "mixed" else
+                   (ix.rerank_size if ix.quant else 0)},
         "recall_at_10": round(head["recall"], 4), "recall_floor_met": head["recall"] >= RECALL_FLOOR,
         "p50_ms": round(head["p50"], 3), "p99_ms": round(head["p99"], 3),
         "stage_ms": {s: round(v, 4) for s, v in stage.items()},

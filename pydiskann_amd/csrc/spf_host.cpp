@@ -84,7 +84,8 @@ struct DBuf {
     X(uint32_t, perm_cnt) X(uint32_t, perm_off) X(uint32_t, perm_fill) X(uint32_t, qperm)  \
     X(uint64_t, rq_planes) X(float, rq_consts) X(uint32_t, pair_src) X(float, eq16)        \
     X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)        \
-    X(uint32_t, gflag) X(uint32_t, gscr)
+    X(uint32_t, gflag) X(uint32_t, gscr) X(uint32_t, sel_ccnt) X(uint32_t, sel_coff)            \
+    X(uint64_t, sel_tmp)
 
 struct Workspace {
     hipStream_t stream = nullptr;
@@ -822,6 +823,10 @@ bool graph_shape(const spf_index *ix, uint32_t n, GraphParams *gp);
 uint64_t graph_scratch_words(const spf_index *ix, const GraphParams &g);
 extern const uint64_t kGraphScratchBytes;
 
+// chunks of the long-segment select: a segment above 65,536 candidates takes
+// ceil(n / 65,536) <= 2n / 65,536 chunks
+inline uint64_t sel_chunks(uint64_t total) { return total > 65536 ? 2 * total / 65536 + 1 : 0; }
+
 // Upper bound on the device bytes one search call of nq queries adds to a
 // Workspace: every buffer it ensures, at DBuf's growth rule, with the
 // candidate count bounded by nq x the npr longest postings and the work-item
@@ -912,7 +917,11 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
         add(q * k, 8);
         add(q, 4);
     } else {
-        if (candidates) add(std::max<uint64_t>(total, 1), 8);   // cand
+        if (candidates) {
+            add(std::max<uint64_t>(total, 1), 8);   // cand
+            if (const uint64_t mc = sel_chunks(total)) add(mc * z.want, 8);   // sel_tmp
+        }
+        add(q + 1, 4 * 2);                          // sel_ccnt, sel_coff
         if (z.rerank) {
             add(q * k, 8);
             add(q, 4);
@@ -1246,6 +1255,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
             grow(w->a_bits, std::max<uint64_t>(total / 32, 1), 4);
         } else if (!ix->rq_l2) {
             grow(w->cand, std::max<uint64_t>(total, 1), 8);
+            if (const uint64_t mc = sel_chunks(total)) grow(w->sel_tmp, mc * want, 8);
         }
         if (extra)
             if (int rc = (*reserve)(extra)) return rc;
@@ -1343,8 +1353,15 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     }
     // ---- top-`want` per query
     HIPCHK(mark(SPF_STAGE_SELECT));
-    HIPCHK(launch_select(w->cand.p, w->cand_off.p, w->cand_cnt.p, 0, nq, want, w->sel.p,
-                         w->sel_cnt.p, s));
+    {
+        const uint64_t max_chunks = sel_chunks(total);
+        HIPCHK(w->sel_ccnt.ensure(nq + 1));
+        HIPCHK(w->sel_coff.ensure(nq + 1));
+        if (max_chunks) HIPCHK(w->sel_tmp.ensure((size_t)max_chunks * want));
+        HIPCHK(launch_select_long(w->cand.p, w->cand_off.p, w->cand_cnt.p, nq, want, w->sel.p,
+                                  w->sel_cnt.p, w->sel_ccnt.p, w->sel_coff.p, w->sel_tmp.p,
+                                  max_chunks, s));
+    }
     HIPCHK(mark(SPF_STAGE_RERANK));
     if (rerank) {
         HIPCHK(w->fin.ensure((size_t)nq * k));

@@ -376,12 +376,14 @@ __device__ void bitonic_sort_lds(uint64_t *buf, uint32_t m) {
 }
 
 constexpr uint32_t kSelSlack = 256;     // extra keys tolerated in the final sort
+constexpr uint64_t kSelChunk = 65536;   // segments longer than this select in chunks
 
 // Top-K of one segment.  R > 0: the segment (at most R * 256 keys) is held in
 // registers, key i = tid + 256 j in kr[j]; R == 0: every pass re-reads L2.
 template <int R>
 __device__ __forceinline__ void select_body(const uint64_t *kp, uint64_t n, uint32_t K,
-                                            uint64_t *o, uint32_t *cnt_out, uint64_t *buf,
+                                            uint32_t pad_to, uint64_t *o, uint32_t *cnt_out,
+                                            uint64_t *buf,
                                             uint32_t *hist, uint32_t *part, uint32_t *s_u,
                                             uint64_t (*s_lh)[4]) {
     const uint32_t tid = threadIdx.x;
@@ -443,7 +445,8 @@ __device__ __forceinline__ void select_body(const uint64_t *kp, uint64_t n, uint
     const uint32_t Kv = (uint32_t)min((uint64_t)K, nvalid);
     __syncthreads();
     if (Kv == 0) {
-        if (tid == 0) *cnt_out = 0;
+        for (uint32_t i = tid; i < pad_to; i += kWG) o[i] = kNoKey;
+        if (tid == 0 && cnt_out) *cnt_out = 0;
         return;
     }
     // digits start at the highest bit where the valid keys differ, so the first
@@ -507,12 +510,14 @@ __device__ __forceinline__ void select_body(const uint64_t *kp, uint64_t n, uint
     __syncthreads();
     bitonic_sort_lds(buf, m2);
     for (uint32_t i = tid; i < Kv; i += kWG) o[i] = buf[i];
-    if (tid == 0) *cnt_out = Kv;
+    for (uint32_t i = Kv + tid; i < pad_to; i += kWG) o[i] = kNoKey;
+    if (tid == 0 && cnt_out) *cnt_out = Kv;
 }
 
 __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint64_t *seg_off,
                                                 const uint32_t *seg_cnt, uint64_t seg_stride,
-                                                uint32_t K, uint64_t *out, uint32_t *out_cnt) {
+                                                uint32_t K, uint64_t *out, uint32_t *out_cnt,
+                                                uint64_t skip_above) {
     __shared__ uint64_t buf[kSelCap];
     __shared__ uint32_t hist[2048];
     __shared__ uint32_t part[kWG];
@@ -522,12 +527,67 @@ __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint
     const uint64_t b = seg_off ? seg_off[seg] : (uint64_t)seg * seg_stride;
     const uint64_t e = seg_off ? seg_off[seg + 1] : b + seg_stride;
     const uint64_t n = seg_cnt ? (uint64_t)seg_cnt[seg] : e - b;
+    if (skip_above && n > skip_above) return;   // a long segment: its chunks do it
     if (n <= 16 * 256)
-        select_body<16>(keys + b, n, K, out + (size_t)seg * K, out_cnt + seg, buf, hist, part, s_u,
-                        s_lh);
+        select_body<16>(keys + b, n, K, 0, out + (size_t)seg * K, out_cnt + seg, buf, hist, part,
+                        s_u, s_lh);
     else
-        select_body<0>(keys + b, n, K, out + (size_t)seg * K, out_cnt + seg, buf, hist, part, s_u,
-                       s_lh);
+        select_body<0>(keys + b, n, K, 0, out + (size_t)seg * K, out_cnt + seg, buf, hist, part,
+                       s_u, s_lh);
+}
+
+// Long segments (more than kSelChunk keys) in two rounds, so one query's
+// Zipf-sized candidate list does not leave a single workgroup streaming it:
+// chunk counts per segment, then the top K of every chunk (padded to K with
+// "no key"), then the top K of each long segment's chunk results.
+__global__ void k_sel_chunk_count(const uint64_t *seg_off, const uint32_t *seg_cnt, uint32_t nseg,
+                                  uint32_t *chunk_cnt) {
+    const uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x;
+    if (seg >= nseg) return;
+    const uint64_t n = seg_cnt ? (uint64_t)seg_cnt[seg] : seg_off[seg + 1] - seg_off[seg];
+    chunk_cnt[seg] = n > kSelChunk ? (uint32_t)((n + kSelChunk - 1) / kSelChunk) : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_select_chunk(const uint64_t *keys, const uint64_t *seg_off,
+                                                      const uint32_t *seg_cnt,
+                                                      const uint32_t *chunk_off, uint32_t nseg,
+                                                      uint32_t K, uint64_t *tmp) {
+    __shared__ uint64_t buf[kSelCap];
+    __shared__ uint32_t hist[2048];
+    __shared__ uint32_t part[kWG];
+    __shared__ uint32_t s_u[4];
+    __shared__ uint64_t s_lh[2][4];
+    const uint32_t c = blockIdx.x;
+    if (c >= chunk_off[nseg]) return;
+    uint32_t lo = 0, hi = nseg;   // the segment whose chunk range holds c
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (chunk_off[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t seg = lo;
+    const uint64_t j = c - chunk_off[seg];
+    const uint64_t n = seg_cnt ? (uint64_t)seg_cnt[seg] : seg_off[seg + 1] - seg_off[seg];
+    const uint64_t b = seg_off[seg] + j * kSelChunk;
+    const uint64_t m = min((uint64_t)kSelChunk, n - j * kSelChunk);
+    select_body<0>(keys + b, m, K, K, tmp + (size_t)c * K, nullptr, buf, hist, part, s_u, s_lh);
+}
+
+__global__ __launch_bounds__(256) void k_select_merge(const uint64_t *tmp, const uint32_t *chunk_off,
+                                                      uint32_t K, uint64_t *out, uint32_t *out_cnt) {
+    __shared__ uint64_t buf[kSelCap];
+    __shared__ uint32_t hist[2048];
+    __shared__ uint32_t part[kWG];
+    __shared__ uint32_t s_u[4];
+    __shared__ uint64_t s_lh[2][4];
+    const uint32_t seg = blockIdx.x;
+    const uint32_t c0 = chunk_off[seg], c1 = chunk_off[seg + 1];
+    if (c1 == c0) return;   // done whole
+    const uint64_t n = (uint64_t)(c1 - c0) * K;
+    select_body<0>(tmp + (size_t)c0 * K, n, K, 0, out + (size_t)seg * K, out_cnt + seg, buf, hist,
+                   part, s_u, s_lh);
 }
 
 // ---------------------------------------------------------------- planning
@@ -835,7 +895,27 @@ hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, const ui
                          uint32_t *out_cnt, hipStream_t s) {
     if (nseg == 0) return hipSuccess;
     hipLaunchKernelGGL(k_select, dim3(nseg), dim3(kWG), 0, s, keys, seg_off, seg_cnt, seg_stride,
-                       K, out, out_cnt);
+                       K, out, out_cnt, (uint64_t)0);
+    return hipGetLastError();
+}
+
+hipError_t launch_select_long(const uint64_t *keys, const uint64_t *seg_off,
+                              const uint32_t *seg_cnt, uint32_t nseg, uint32_t K, uint64_t *out,
+                              uint32_t *out_cnt, uint32_t *chunk_cnt, uint32_t *chunk_off,
+                              uint64_t *tmp, uint64_t max_chunks, hipStream_t s) {
+    if (nseg == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_select, dim3(nseg), dim3(kWG), 0, s, keys, seg_off, seg_cnt,
+                       (uint64_t)0, K, out, out_cnt, (uint64_t)kSelChunk);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || max_chunks == 0) return e;
+    hipLaunchKernelGGL(k_sel_chunk_count, dim3((nseg + 255) / 256), dim3(256), 0, s, seg_off,
+                       seg_cnt, nseg, chunk_cnt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_excl_scan_u32(chunk_cnt, chunk_off, nseg, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_select_chunk, dim3((uint32_t)max_chunks), dim3(kWG), 0, s, keys, seg_off,
+                       seg_cnt, chunk_off, nseg, K, tmp);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_select_merge, dim3(nseg), dim3(kWG), 0, s, tmp, chunk_off, K, out, out_cnt);
     return hipGetLastError();
 }
 

@@ -184,3 +184,17 @@ def test_batch_matches_single_and_errors():
     assert e.value.code == SPF_EDIM and "Query dim 31 != index dim 32" in str(e.value)
     with pytest.raises(ValueError):
         dev.search(q, 0, 8)
+
+
+@pytest.mark.parametrize("quant,k,rerank", [(None, 10, 0), (None, 1000, 0), ("int8", 50, 400)])
+def test_long_candidate_lists_select_in_chunks(quant, k, rerank):
+    """Candidate lists far above 65,536 keys per query (the chunked select)."""
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=(300000, 16)).astype(np.float32)
+    cents = np.concatenate([rng.normal(size=(3, 16)) * 0.01, rng.normal(size=(40, 16)) * 3])
+    ix = build_index(x, "l2", quant, centroids=cents.astype(np.float32), rerank_size=rerank, seed=0)
+    assert int(ix.posting_lengths().max()) > 65536
+    dev = DeviceIndex(ix)
+    q = np.concatenate([x[:12], (rng.normal(size=(12, 16)) * 3).astype(np.float32)])
+    compare(dev, ix, q, k, 3)
+    compare(dev, ix, q, k, ix.n_post)
