@@ -1263,6 +1263,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(w->item_post.ensure(std::max<uint32_t>(n_items, 1)));
     pl.item_off = w->item_off.p;
     pl.item_post = w->item_post.p;
+    pl.n_items = n_items;
     HIPCHK(launch_item_table(pl, s));
 
     RqParams rq{};

@@ -308,6 +308,7 @@ struct PlanParams {
     uint32_t *item_cnt;         // [n_post + 1]
     const uint32_t *item_off;   // [n_post + 1]
     uint32_t *item_post;        // [n_items]
+    uint32_t n_items;           // item table size (known after the item scan)
     uint32_t G;                 // queries per work item
     uint32_t echunk;            // entries per work item
 };

@@ -670,11 +670,20 @@ __global__ void k_items(const PlanParams p) {
     p.item_cnt[b] = (cnt && len) ? ((cnt + p.G - 1) / p.G) * ((len + p.echunk - 1) / p.echunk) : 0u;
 }
 
-// work item -> posting (one thread per posting)
+// work item -> posting, one thread per item: the posting b with
+// item_off[b] <= i < item_off[b + 1] (a long posting's items spread over threads)
 __global__ void k_item_table(const PlanParams p) {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= p.n_post) return;
-    for (uint32_t i = p.item_off[b]; i < p.item_off[b + 1]; i++) p.item_post[i] = b;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n_items) return;
+    uint32_t lo = 0, hi = p.n_post;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (p.item_off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    p.item_post[i] = lo;
 }
 
 template <typename T>
@@ -962,7 +971,8 @@ hipError_t launch_items(const PlanParams &p, hipStream_t s) {
 }
 
 hipError_t launch_item_table(const PlanParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_item_table, dim3((p.n_post + 255) / 256), dim3(256), 0, s, p);
+    if (p.n_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_item_table, dim3((p.n_items + 255) / 256), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
