@@ -85,7 +85,7 @@ struct DBuf {
     X(uint64_t, rq_planes) X(float, rq_consts) X(uint32_t, pair_src) X(float, eq16)        \
     X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)        \
     X(uint32_t, gflag) X(uint32_t, gscr) X(uint32_t, sel_ccnt) X(uint32_t, sel_coff)            \
-    X(uint64_t, sel_tmp)
+    X(uint64_t, sel_tmp) X(uint64_t, probe_tmp)
 
 struct Workspace {
     hipStream_t stream = nullptr;
@@ -825,7 +825,9 @@ extern const uint64_t kGraphScratchBytes;
 
 // chunks of the long-segment select: a segment above 65,536 candidates takes
 // ceil(n / 65,536) <= 2n / 65,536 chunks
-inline uint64_t sel_chunks(uint64_t total) { return total > 65536 ? 2 * total / 65536 + 1 : 0; }
+inline uint64_t sel_chunks(uint64_t total) {
+    return total > kSelChunk ? 2 * total / kSelChunk + 1 : 0;
+}
 
 // Upper bound on the device bytes one search call of nq queries adds to a
 // Workspace: every buffer it ensures, at DBuf's growth rule, with the
@@ -875,6 +877,8 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
         add(q, 4);
     } else {
         add(q * P, 8);           // probe_keys
+        if (P > kProbeChunk) add(q * ((P + kProbeChunk - 1) / kProbeChunk) * npr, 8);   // probe_tmp
+        add(q + 1, 4 * 2);       // sel_ccnt, sel_coff
     }
     add(q, 4);                   // np_q
     add(pairs, 4);               // probe_post
@@ -1183,8 +1187,14 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         const uint32_t grid = xcd_grid(items, 16384);
         HIPCHK(launch_scan(CK_F32, ix->metric, true, Gv, sp, grid, s));
         HIPCHK(mark(SPF_STAGE_PROBE_SELECT));
-        HIPCHK(launch_select(w->probe_keys.p, nullptr, nullptr, P, nq, npr, w->probe_sel.p,
-                             w->probe_cnt.p, s));
+        // rows of P keys: above kProbeChunk in register-sized chunks, then a merge
+        const uint64_t pch = P > kProbeChunk ? (uint64_t)nq * ((P + kProbeChunk - 1) / kProbeChunk) : 0;
+        HIPCHK(w->sel_ccnt.ensure(nq + 1));
+        HIPCHK(w->sel_coff.ensure(nq + 1));
+        if (pch) HIPCHK(w->probe_tmp.ensure((size_t)pch * npr));
+        HIPCHK(launch_select_long(w->probe_keys.p, nullptr, nullptr, P, nq, kProbeChunk, npr,
+                                  w->probe_sel.p, w->probe_cnt.p, w->sel_ccnt.p, w->sel_coff.p,
+                                  w->probe_tmp.p, pch, s));
     }
     HIPCHK(mark(SPF_STAGE_PLAN));
 
@@ -1359,9 +1369,9 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         HIPCHK(w->sel_ccnt.ensure(nq + 1));
         HIPCHK(w->sel_coff.ensure(nq + 1));
         if (max_chunks) HIPCHK(w->sel_tmp.ensure((size_t)max_chunks * want));
-        HIPCHK(launch_select_long(w->cand.p, w->cand_off.p, w->cand_cnt.p, nq, want, w->sel.p,
-                                  w->sel_cnt.p, w->sel_ccnt.p, w->sel_coff.p, w->sel_tmp.p,
-                                  max_chunks, s));
+        HIPCHK(launch_select_long(w->cand.p, w->cand_off.p, w->cand_cnt.p, 0, nq, kSelChunk,
+                                  want, w->sel.p, w->sel_cnt.p, w->sel_ccnt.p, w->sel_coff.p,
+                                  w->sel_tmp.p, max_chunks, s));
     }
     HIPCHK(mark(SPF_STAGE_RERANK));
     if (rerank) {

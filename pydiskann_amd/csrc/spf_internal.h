@@ -330,13 +330,17 @@ struct RerankParams {
 hipError_t launch_prep(const PrepParams &p, hipStream_t s);
 hipError_t launch_scan(uint32_t code_kind, uint32_t metric, bool dense, uint32_t G,
                        const ScanParams &p, uint32_t grid, hipStream_t s);
-// top K of each segment; segments longer than 65,536 keys in chunks (the top K
-// of each chunk into tmp, then of the chunk results); max_chunks bounds the
-// chunk count (0 = no segment that long)
+// top K of each segment; segments longer than `chunk` keys in chunks (the top
+// K of each chunk into tmp, then of the chunk results); max_chunks bounds the
+// chunk count (0 = no segment that long).  Segments by seg_off, or seg_stride
+// apart when seg_off is null.
 hipError_t launch_select_long(const uint64_t *keys, const uint64_t *seg_off,
-                              const uint32_t *seg_cnt, uint32_t nseg, uint32_t K, uint64_t *out,
-                              uint32_t *out_cnt, uint32_t *chunk_cnt, uint32_t *chunk_off,
-                              uint64_t *tmp, uint64_t max_chunks, hipStream_t s);
+                              const uint32_t *seg_cnt, uint64_t seg_stride, uint32_t nseg,
+                              uint32_t chunk, uint32_t K, uint64_t *out, uint32_t *out_cnt,
+                              uint32_t *chunk_cnt, uint32_t *chunk_off, uint64_t *tmp,
+                              uint64_t max_chunks, hipStream_t s);
+constexpr uint32_t kSelChunk = 65536;      // candidate lists: keys per chunk
+constexpr uint32_t kProbeChunk = 4096;     // dense probe rows: keys per chunk (in registers)
 hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, const uint32_t *seg_cnt,
                          uint64_t seg_stride, uint32_t nseg, uint32_t K, uint64_t *out,
                          uint32_t *out_cnt, hipStream_t s);

@@ -376,7 +376,6 @@ __device__ void bitonic_sort_lds(uint64_t *buf, uint32_t m) {
 }
 
 constexpr uint32_t kSelSlack = 256;     // extra keys tolerated in the final sort
-constexpr uint64_t kSelChunk = 65536;   // segments longer than this select in chunks
 
 // Top-K of one segment.  R > 0: the segment (at most R * 256 keys) is held in
 // registers, key i = tid + 256 j in kr[j]; R == 0: every pass re-reads L2.
@@ -540,18 +539,20 @@ __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint
 // Zipf-sized candidate list does not leave a single workgroup streaming it:
 // chunk counts per segment, then the top K of every chunk (padded to K with
 // "no key"), then the top K of each long segment's chunk results.
-__global__ void k_sel_chunk_count(const uint64_t *seg_off, const uint32_t *seg_cnt, uint32_t nseg,
+__global__ void k_sel_chunk_count(const uint64_t *seg_off, const uint32_t *seg_cnt,
+                                  uint64_t seg_stride, uint32_t nseg, uint32_t chunk,
                                   uint32_t *chunk_cnt) {
     const uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x;
     if (seg >= nseg) return;
-    const uint64_t n = seg_cnt ? (uint64_t)seg_cnt[seg] : seg_off[seg + 1] - seg_off[seg];
-    chunk_cnt[seg] = n > kSelChunk ? (uint32_t)((n + kSelChunk - 1) / kSelChunk) : 0u;
+    const uint64_t n = seg_cnt ? (uint64_t)seg_cnt[seg]
+                       : seg_off ? seg_off[seg + 1] - seg_off[seg] : seg_stride;
+    chunk_cnt[seg] = n > chunk ? (uint32_t)((n + chunk - 1) / chunk) : 0u;
 }
 
 __global__ __launch_bounds__(256) void k_select_chunk(const uint64_t *keys, const uint64_t *seg_off,
-                                                      const uint32_t *seg_cnt,
+                                                      const uint32_t *seg_cnt, uint64_t seg_stride,
                                                       const uint32_t *chunk_off, uint32_t nseg,
-                                                      uint32_t K, uint64_t *tmp) {
+                                                      uint32_t chunk, uint32_t K, uint64_t *tmp) {
     __shared__ uint64_t buf[kSelCap];
     __shared__ uint32_t hist[2048];
     __shared__ uint32_t part[kWG];
@@ -569,10 +570,14 @@ __global__ __launch_bounds__(256) void k_select_chunk(const uint64_t *keys, cons
     }
     const uint32_t seg = lo;
     const uint64_t j = c - chunk_off[seg];
-    const uint64_t n = seg_cnt ? (uint64_t)seg_cnt[seg] : seg_off[seg + 1] - seg_off[seg];
-    const uint64_t b = seg_off[seg] + j * kSelChunk;
-    const uint64_t m = min((uint64_t)kSelChunk, n - j * kSelChunk);
-    select_body<0>(keys + b, m, K, K, tmp + (size_t)c * K, nullptr, buf, hist, part, s_u, s_lh);
+    const uint64_t s0 = seg_off ? seg_off[seg] : (uint64_t)seg * seg_stride;
+    const uint64_t n = seg_cnt ? (uint64_t)seg_cnt[seg] : seg_off ? seg_off[seg + 1] - s0 : seg_stride;
+    const uint64_t b = s0 + j * chunk;
+    const uint64_t m = min((uint64_t)chunk, n - j * chunk);
+    if (m <= 16 * 256)
+        select_body<16>(keys + b, m, K, K, tmp + (size_t)c * K, nullptr, buf, hist, part, s_u, s_lh);
+    else
+        select_body<0>(keys + b, m, K, K, tmp + (size_t)c * K, nullptr, buf, hist, part, s_u, s_lh);
 }
 
 __global__ __launch_bounds__(256) void k_select_merge(const uint64_t *tmp, const uint32_t *chunk_off,
@@ -586,8 +591,12 @@ __global__ __launch_bounds__(256) void k_select_merge(const uint64_t *tmp, const
     const uint32_t c0 = chunk_off[seg], c1 = chunk_off[seg + 1];
     if (c1 == c0) return;   // done whole
     const uint64_t n = (uint64_t)(c1 - c0) * K;
-    select_body<0>(tmp + (size_t)c0 * K, n, K, 0, out + (size_t)seg * K, out_cnt + seg, buf, hist,
-                   part, s_u, s_lh);
+    if (n <= 16 * 256)
+        select_body<16>(tmp + (size_t)c0 * K, n, K, 0, out + (size_t)seg * K, out_cnt + seg, buf,
+                        hist, part, s_u, s_lh);
+    else
+        select_body<0>(tmp + (size_t)c0 * K, n, K, 0, out + (size_t)seg * K, out_cnt + seg, buf,
+                       hist, part, s_u, s_lh);
 }
 
 // ---------------------------------------------------------------- planning
@@ -909,20 +918,21 @@ hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, const ui
 }
 
 hipError_t launch_select_long(const uint64_t *keys, const uint64_t *seg_off,
-                              const uint32_t *seg_cnt, uint32_t nseg, uint32_t K, uint64_t *out,
-                              uint32_t *out_cnt, uint32_t *chunk_cnt, uint32_t *chunk_off,
-                              uint64_t *tmp, uint64_t max_chunks, hipStream_t s) {
+                              const uint32_t *seg_cnt, uint64_t seg_stride, uint32_t nseg,
+                              uint32_t chunk, uint32_t K, uint64_t *out, uint32_t *out_cnt,
+                              uint32_t *chunk_cnt, uint32_t *chunk_off, uint64_t *tmp,
+                              uint64_t max_chunks, hipStream_t s) {
     if (nseg == 0) return hipSuccess;
     hipLaunchKernelGGL(k_select, dim3(nseg), dim3(kWG), 0, s, keys, seg_off, seg_cnt,
-                       (uint64_t)0, K, out, out_cnt, (uint64_t)kSelChunk);
+                       seg_stride, K, out, out_cnt, (uint64_t)chunk);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || max_chunks == 0) return e;
     hipLaunchKernelGGL(k_sel_chunk_count, dim3((nseg + 255) / 256), dim3(256), 0, s, seg_off,
-                       seg_cnt, nseg, chunk_cnt);
+                       seg_cnt, seg_stride, nseg, chunk, chunk_cnt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = launch_excl_scan_u32(chunk_cnt, chunk_off, nseg, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_select_chunk, dim3((uint32_t)max_chunks), dim3(kWG), 0, s, keys, seg_off,
-                       seg_cnt, chunk_off, nseg, K, tmp);
+                       seg_cnt, seg_stride, chunk_off, nseg, chunk, K, tmp);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_select_merge, dim3(nseg), dim3(kWG), 0, s, tmp, chunk_off, K, out, out_cnt);
     return hipGetLastError();

@@ -198,3 +198,17 @@ def test_long_candidate_lists_select_in_chunks(quant, k, rerank):
     q = np.concatenate([x[:12], (rng.normal(size=(12, 16)) * 3).astype(np.float32)])
     compare(dev, ix, q, k, 3)
     compare(dev, ix, q, k, ix.n_post)
+
+
+@pytest.mark.parametrize("n_probe", [1, 64, 700])
+def test_exact_probe_above_4096_postings(n_probe):
+    """Exact probe over 9,000 centroids (the register-chunked probe select),
+    including integer centroids with equal distances (block-order ties)."""
+    rng = np.random.default_rng(12)
+    x = rng.integers(-4, 5, size=(60000, 8)).astype(np.float32)
+    cents = x[rng.choice(60000, 9000, replace=False)] + rng.integers(0, 2, size=(9000, 8))
+    ix = build_index(x, "l2", None, centroids=cents.astype(np.float32), seed=0)
+    assert ix.n_post > 4096
+    dev = DeviceIndex(ix)
+    q = rng.integers(-4, 5, size=(40, 8)).astype(np.float32)
+    compare(dev, ix, q, 10, n_probe)
