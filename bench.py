@@ -246,6 +246,11 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=0,
                     help="c3/c5: probe through a routing graph of this many neighbours per "
                          "centroid (the reference's path above 4,096 postings)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes (by default each precision is first "
+                         "autotuned on the batch itself, untimed)")
+    ap.add_argument("--tune-cache", default="",
+                    help="reuse / record the autotuner's choice in this file")
     ap.add_argument("--requests", type=int, default=4000, help="c5: requests per load point")
     ap.add_argument("--target-qps", type=float, default=0,
                     help="c5: open-loop query rate (0 = 50%% and 80%% of measured capacity)")
@@ -281,6 +286,12 @@ def main() -> int:
 
     def run(precision: str):
         prec = PREC_EXACT if precision == "exact" else PREC_MIXED
+        tune = None
+        if not args.no_tune:
+            tune = dev.autotune(queries, k, n_probe, precision=prec,
+                                rerank_size=args.rerank if prec else 0,
+                                cache_path=args.tune_cache or None)
+            log(f"[bench] {precision}: autotune {tune}")
 
         def step():
             dev.search_device(q_d.data_ptr(), nq, dim, k, n_probe, ids_d.data_ptr(),
@@ -316,6 +327,7 @@ def main() -> int:
             log(f"[bench] mixed: rerank gathered {profs[-1]['rerank_rows']:,} rows of "
                 f"{profs[-1]['pool_members']:,} pool members")
         return {"qps": qps, "total": total, "recall": rec, "stage": stage, "prof": profs[-1],
+                "tune": tune,
                 "p50": lat_sorted[len(lat) // 2],
                 "p99": lat_sorted[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]}
 
@@ -391,7 +403,8 @@ def main() -> int:
                    "precision": args.precision,
                    "probe": f"routing graph ({args.graph} neighbours)" if args.graph else "exact",
                    "rerank_pool": args.rerank if args.precision == "mixed" else
-                   (ix.rerank_size if ix.quant else 0)},
+                   (ix.rerank_size if ix.quant else 0),
+                   "launch_shapes": head["tune"] or "defaults"},
         "recall_at_10": round(head["recall"], 4), "recall_floor_met": head["recall"] >= RECALL_FLOOR,
         "p50_ms": round(head["p50"], 3), "p99_ms": round(head["p99"], 3),
         "stage_ms": {s: round(v, 4) for s, v in stage.items()},
@@ -402,6 +415,7 @@ def main() -> int:
         out[f"{oname}_gpu"] = {"value": round(other["qps"], 1), "unit": "queries/s",
                                "recall_at_10": round(other["recall"], 4),
                                "p50_ms": round(other["p50"], 3), "p99_ms": round(other["p99"], 3),
+                               "launch_shapes": other["tune"] or "defaults",
                                "stage_ms": {s: round(v, 4) for s, v in other["stage"].items()}}
         if args.precision == "mixed":
             out["speedup_vs_fp32_gpu"] = round(qps / other["qps"], 3)

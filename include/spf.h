@@ -264,6 +264,27 @@ int spf_server_get_stats(spf_server *sv, spf_server_stats *out);
 /* Completes every submitted request, then frees the server. */
 void spf_server_destroy(spf_server *sv);
 
+/* Launch-shape autotuner.  No counterpart in the reference (its search has no
+ * device launch shapes).  Times the search of the given sample queries under
+ * each candidate shape -- VALU scan (EXACT): 4 / 8 / 16 queries x 256 / 512 /
+ * 1,024 / 2,048 entries per work item; matrix-core scan (MIXED): 32 / 64 / 128
+ * queries per work item -- and keeps the fastest on the index for later
+ * searches.  With cache_path (may be NULL), a line keyed by the device, the
+ * index shape and the search settings is looked up first (nothing is timed
+ * then) and appended after measuring.  Results never depend on the shape.
+ * Must not run concurrently with searches on the same index. */
+typedef struct {
+    uint32_t scan_g, scan_chunk, mix_g;  /* shapes in effect after the call */
+    uint32_t from_cache;                 /* 1: read from cache_path */
+    float best_ms, default_ms;           /* sample search at the chosen / default shapes */
+    uint32_t tried;                      /* shapes timed */
+    uint32_t reserved[1];
+} spf_tune_result;
+
+int spf_autotune(spf_index *ix, const float *queries, uint32_t nq, uint32_t dim, uint32_t k,
+                 uint32_t n_probe, const spf_search_opts *opts, const char *cache_path,
+                 spf_tune_result *out);
+
 /* Profile of the last search on this thread that set SPF_FLAG_PROFILE. */
 int spf_last_profile(spf_profile *out);
 

@@ -24,6 +24,7 @@ EXPORTS = ("spf_index_create", "spf_index_destroy", "spf_index_get_info", "spf_d
            "spf_search_batch", "spf_search_batch_device", "spf_search_batch_filtered",
            "spf_search_batch_filtered_device", "spf_nearest_postings", "spf_server_create",
            "spf_server_submit", "spf_server_wait", "spf_server_get_stats", "spf_server_destroy",
+           "spf_autotune",
            "spf_last_profile", "spf_selftest", "spf_last_error", "spf_version")
 FLAG_PROFILE = 1
 FLAG_FULL_RERANK = 2
@@ -92,6 +93,12 @@ class ServerStats(C.Structure):
                 ("splits", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("scan_g", C.c_uint32), ("scan_chunk", C.c_uint32), ("mix_g", C.c_uint32),
+                ("from_cache", C.c_uint32), ("best_ms", C.c_float), ("default_ms", C.c_float),
+                ("tried", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+
+
 class SpfError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[{code}] {msg}")
@@ -143,6 +150,9 @@ def lib():
         L.spf_server_get_stats.restype = C.c_int
         L.spf_server_destroy.argtypes = [P]
         L.spf_server_destroy.restype = None
+        L.spf_autotune.argtypes = [P, P, u32, u32, u32, u32, C.POINTER(SearchOpts), C.c_char_p,
+                                   C.POINTER(TuneResult)]
+        L.spf_autotune.restype = C.c_int
         L.spf_last_profile.argtypes = [C.POINTER(Profile)]
         L.spf_last_profile.restype = C.c_int
         L.spf_selftest.argtypes = [C.c_int]
@@ -258,6 +268,21 @@ class DeviceIndex:
             check(lib().spf_search_batch_filtered(self.h, _p(q), nq, dim, k, n_probe,
                                                   C.byref(o), _p(m), _p(ids), _p(dd), _p(cnt)))
         return ids, dd, cnt
+
+    def autotune(self, queries, k: int, n_probe: int, precision: int = PREC_EXACT,
+                 rerank_size: int = 0, cache_path: str | None = None) -> dict:
+        """Time the launch shapes on these sample queries and keep the fastest
+        (spf_autotune); with cache_path, reuse / record the choice."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim == 1:
+            q = q[None, :]
+        o = SearchOpts()
+        o.precision, o.rerank_size = precision, rerank_size
+        r = TuneResult()
+        check(lib().spf_autotune(self.h, _p(q), q.shape[0], q.shape[1], k, n_probe, C.byref(o),
+                                 cache_path.encode() if cache_path else None, C.byref(r)))
+        return {f: (float(getattr(r, f)) if f.endswith("_ms") else int(getattr(r, f)))
+                for f, _ in TuneResult._fields_ if f != "reserved"}
 
     def nearest_postings(self, vectors):
         """(block index, distance) of each vector's nearest live posting:

@@ -172,7 +172,12 @@ struct spf_index {
     // working-memory bound inputs: top_len[j] = entries in the j longest
     // postings; posting chunk counts at the VALU and matrix-core chunk sizes
     std::vector<uint64_t> top_len;
-    uint64_t chunks_v = 0, chunks_m = 0;
+    std::vector<uint32_t> h_post_len;   // entries per (device-order) posting
+    uint64_t chunks_m = 0;
+    uint64_t chunks_v[4] = {0, 0, 0, 0};   // posting chunks at 256, 512, 1024, 2048 entries
+    // launch shapes (spf_autotune): VALU scan queries and entries per item,
+    // matrix-core scan queries per item
+    uint32_t scan_g = 8, scan_chunk = kEChunk, mix_g = kMixQ;
 
     std::vector<uint64_t> rank_id;      // ids in ascending order
     std::vector<uint32_t> rank_entry;   // entry holding each rank
@@ -437,13 +442,14 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
     }
     HIPCHK(upload(&ix->d_post_len, post_len.data(), post_len.size(), bytes));
     {
+        ix->h_post_len = post_len;
         std::vector<uint32_t> sorted(post_len);
         std::sort(sorted.begin(), sorted.end(), std::greater<uint32_t>());
         ix->top_len.assign(P + 1, 0);
         for (uint32_t i = 0; i < P; i++) ix->top_len[i + 1] = ix->top_len[i] + sorted[i];
         const uint32_t cm = mfma_chunk(round_up(d->dim, 32) / 32);
         for (uint32_t len : post_len) {
-            ix->chunks_v += (len + kEChunk - 1) / kEChunk;
+            for (int c = 0; c < 4; c++) ix->chunks_v[c] += (len + (256u << c) - 1) / (256u << c);
             ix->chunks_m += (len + cm - 1) / cm;
         }
     }
@@ -676,6 +682,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     mp.del = ix->d_del;
     mp.metric = ix->metric;
     mp.echunk = mfma_chunk(dpad32 / 32);
+    mp.qg = ix->mix_g;
     mp.label_e = d_masks ? ix->d_label_e : nullptr;
     mp.qmask = d_masks;
     mp.out16 = w->cand16.p;
@@ -748,10 +755,12 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
         HIPCHK(w->rr_stats.ensure(nq));
         rp.stats = w->rr_stats.p;
     }
+    rp.rank_id = ix->d_rank_id;   // the rerank writes the caller's rows itself
+    rp.ids = d_ids;
+    rp.dists = d_dists;
+    rp.counts = d_counts;
     HIPCHK(launch_rerank_pruned(rp, s));
     HIPCHK(mark(SPF_STAGE_FINALIZE));
-    HIPCHK(launch_finalize(w->fin.p, k, w->fin_cnt.p, nq, k, ix->d_rank_id, d_ids, d_dists,
-                           d_counts, s));
     return SPF_OK;
 }
 
@@ -793,19 +802,16 @@ int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_se
     if (z.want > kMaxK) return fail(SPF_EINVAL, "k / rerank_size above 2048");
     z.dpad = ix->dpad;
     z.dpad32 = round_up(ix->dim, 32);
-    // queries per VALU scan item (SPF_SCAN_G overrides: 4, 8 or 16)
-    static const uint32_t g_env = [] {
-        const char *e = std::getenv("SPF_SCAN_G");
-        const uint32_t v = e ? (uint32_t)std::atoi(e) : 0u;
-        return (v == 4 || v == 8 || v == 16) ? v : 0u;
-    }();
-    z.Gv = z.dpad > 1024 ? 4 : (g_env ? g_env : 8);
+    // queries per VALU scan item: the tuned value while its staged queries
+    // (G x dpad f32) fit 32 KiB of LDS
+    z.Gv = ix->scan_g;
+    while (z.Gv > 4 && z.Gv * z.dpad > 8192) z.Gv /= 2;
     // queries and entries per posting-scan item: the matrix-core scan takes
     // kMixQ queries over mfma_chunk entries, the RaBitQ estimator scan 8
     // queries, the VALU scan Gv; both of the latter over kEChunk entries
     z.mfma_scan = z.mixed && !ix->rq_l2;
-    z.G = z.mfma_scan ? kMixQ : z.mixed ? 8u : z.Gv;
-    z.echunk = z.mfma_scan ? mfma_chunk(z.dpad32 / 32) : kEChunk;
+    z.G = z.mfma_scan ? ix->mix_g : z.mixed ? 8u : z.Gv;
+    z.echunk = z.mfma_scan ? mfma_chunk(z.dpad32 / 32) : (z.mixed ? kEChunk : ix->scan_chunk);
     z.round16 = ix->quant == SPF_QUANT_F16 &&
                 (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
     z.q8 = ix->quant == SPF_QUANT_INT8 &&
@@ -897,14 +903,17 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     add(P + 1, 4);               // item_off
     // items of posting p = ceil(pairs_p / G) * ceil(len_p / echunk), pairs_p <= nq
     const uint64_t cmax = (ix->max_len + z.echunk - 1) / z.echunk;
-    const uint64_t call = z.echunk == kEChunk ? ix->chunks_v : ix->chunks_m;
+    uint64_t call = ix->chunks_m;
+    for (int c = 0; c < 4; c++)
+        if (z.echunk == (256u << c)) call = ix->chunks_v[c];
     const uint64_t items = (pairs + z.G - 1) / z.G * cmax + std::min(call, pairs * cmax);
     if (candidates) add(std::max<uint64_t>(items, 1), 4);   // item_post
     if (ix->rq_l2) {
         add(pairs * 4 * (ix->rq_padded / 64), 8);
         add(pairs * 8, 4);
     }
-    const uint64_t total = q * ix->top_len[std::min<uint64_t>(npr, P)];
+    // (+ up to 7 alignment slots per probe for the matrix-core scan)
+    const uint64_t total = q * ix->top_len[std::min<uint64_t>(npr, P)] + (z.mfma_scan ? 7 * pairs : 0);
     if (z.mixed) {
         if (candidates) {
             add(std::max<uint64_t>(total, 1), 2);       // cand16
@@ -1238,6 +1247,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.pair_src = ix->rq_l2 ? w->pair_src.p : nullptr;
     pl.G = G;
     pl.echunk = echunk;
+    pl.align = z.mfma_scan ? 8u : 1u;   // the matrix-core scan stores aligned 16-byte rows
     HIPCHK(hipMemsetAsync(w->post_cnt.p, 0, (P + 1) * sizeof(uint32_t), s));
     HIPCHK(launch_plan_q(pl, s));
     HIPCHK(launch_excl_scan_u64(w->cand_len.p, w->cand_off.p, nq, s));
@@ -1359,6 +1369,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         sp.pair_slot = w->pair_slot.p;
         sp.label_e = d_masks ? ix->d_label_e : nullptr;
         sp.qmask = d_masks;
+        sp.echunk = echunk;
         const uint32_t grid = xcd_grid(n_items, 16384);
         HIPCHK(launch_scan(ix->code_kind, ix->metric, false, G, sp, grid, s));
     }
@@ -1408,10 +1419,19 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         std::memset(&g_prof, 0, sizeof g_prof);
         for (int i = 0; i < SPF_N_STAGES; i++)
             HIPCHK(hipEventElapsedTime(&g_prof.stage_ms[i], w->ev[i], w->ev[i + 1]));
-        g_prof.candidates = total;
+        uint64_t real = total;   // candidates without the alignment slots
+        if (pl.align > 1) {
+            std::vector<uint32_t> npq(nq), pp((size_t)nq * npr);
+            HIPCHK(hipMemcpy(npq.data(), w->np_q.p, nq * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(pp.data(), w->probe_post.p, pp.size() * 4, hipMemcpyDeviceToHost));
+            real = 0;
+            for (uint32_t q = 0; q < nq; q++)
+                for (uint32_t j = 0; j < npq[q]; j++) real += ix->h_post_len[pp[(size_t)q * npr + j]];
+        }
+        g_prof.candidates = real;
         g_prof.scan_code_bytes =
-            total * (ix->quant == SPF_QUANT_RABITQ ? (uint64_t)ix->code_bytes
-                     : mixed ? 2ull * ix->dim : (uint64_t)ix->code_bytes);
+            real * (ix->quant == SPF_QUANT_RABITQ ? (uint64_t)ix->code_bytes
+                    : mixed ? 2ull * ix->dim : (uint64_t)ix->code_bytes);
         g_prof.probe_pairs = (uint64_t)nq * P;
         g_prof.scan_items = n_items;
         if (mixed) {   // per-query counts, summed here
@@ -1797,6 +1817,136 @@ int spf_selftest(int device) {
                 return fail(SPF_EDEVICE, msg);
             }
         }
+    return SPF_OK;
+}
+
+int spf_autotune(spf_index *ix, const float *queries, uint32_t nq, uint32_t dim, uint32_t k,
+                 uint32_t n_probe, const spf_search_opts *opts, const char *cache_path,
+                 spf_tune_result *out) {
+    int rc = check_search_args(ix, dim, k);
+    if (rc) return rc;
+    if (!queries || nq == 0 || !out) return fail(SPF_EINVAL, "tuning needs sample queries");
+    Shape z;
+    if ((rc = search_shape(ix, k, n_probe, opts, &z))) return rc;
+    DeviceGuard dg(ix->device);
+    *out = spf_tune_result{};
+    // key: device, index shape, search settings
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ix->device) != hipSuccess)
+        return fail(SPF_EDEVICE, "hipGetDeviceProperties failed");
+    char key[512];
+    std::snprintf(key, sizeof key,
+                  "spf-tune-v1 %s/%d dim=%u metric=%u quant=%u P=%u N=%llu maxlen=%u nq=%u k=%u "
+                  "n_probe=%u prec=%u rerank=%u",
+                  prop.gcnArchName, prop.multiProcessorCount, ix->dim, ix->metric, ix->quant,
+                  ix->P, (unsigned long long)ix->N, ix->max_len, nq, k, n_probe,
+                  opts ? opts->precision : 0u, opts ? opts->rerank_size : 0u);
+    auto valid = [&](uint32_t g, uint32_t c, uint32_t m) {
+        return (g == 4 || g == 8 || g == 16) && (c == 256 || c == 512 || c == 1024 || c == 2048) &&
+               (m == 32 || m == 64 || m == 128);
+    };
+    auto finish = [&](uint32_t g, uint32_t c, uint32_t m) {
+        ix->scan_g = g;
+        ix->scan_chunk = c;
+        ix->mix_g = m;
+        out->scan_g = g;
+        out->scan_chunk = c;
+        out->mix_g = m;
+    };
+    if (cache_path) {
+        if (FILE *f = std::fopen(cache_path, "r")) {
+            char line[768];
+            bool hit = false;
+            uint32_t g = 0, c = 0, m = 0;
+            while (std::fgets(line, sizeof line, f)) {
+                const size_t kl = std::strlen(key);
+                if (std::strncmp(line, key, kl) != 0 || line[kl] != ' ') continue;
+                uint32_t a, b, e;
+                if (std::sscanf(line + kl, " scan_g=%u scan_chunk=%u mix_g=%u", &a, &b, &e) == 3 &&
+                    valid(a, b, e)) {
+                    hit = true;
+                    g = a;
+                    c = b;
+                    m = e;
+                }
+            }
+            std::fclose(f);
+            if (hit) {
+                finish(g, c, m);
+                out->from_cache = 1;
+                return SPF_OK;
+            }
+        }
+    }
+    Workspace *w = acquire(ix);
+    if (!w) return fail(SPF_EDEVICE, "workspace creation failed");
+    hipStream_t s = w->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const uint32_t g0 = ix->scan_g, c0 = ix->scan_chunk, m0 = ix->mix_g;
+    auto run = [&]() -> int {
+        HIPCHK(w->qin.ensure((size_t)nq * dim));
+        HIPCHK(w->out_ids.ensure((size_t)nq * k));
+        HIPCHK(w->out_d.ensure((size_t)nq * k));
+        HIPCHK(w->out_c.ensure(nq));
+        HIPCHK(hipMemcpyAsync(w->qin.p, queries, (size_t)nq * dim * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        // shapes to time: the default first
+        struct Cand { uint32_t g, c, m; };
+        std::vector<Cand> cands{{8, kEChunk, kMixQ}};
+        if (z.mfma_scan) {
+            for (uint32_t m : {32u, 64u})
+                cands.push_back({8, kEChunk, m});
+        } else if (!z.mixed && !ix->rq_l2) {
+            for (uint32_t g : {4u, 8u, 16u})
+                for (uint32_t c : {256u, 512u, 1024u, 2048u})
+                    if (!(g == 8 && c == kEChunk) && g * z.dpad <= 8192) cands.push_back({g, c, kMixQ});
+        }
+        float best = INFINITY;
+        Cand pick = cands[0];
+        for (const Cand &cd : cands) {
+            ix->scan_g = cd.g;
+            ix->scan_chunk = cd.c;
+            ix->mix_g = cd.m;
+            float t[3];
+            for (int r = -1; r < 3; r++) {   // one untimed run, then three
+                HIPCHK(hipEventRecord(e0, s));
+                int x = search_impl(ix, w, s, w->qin.p, nq, k, n_probe, opts, w->out_ids.p,
+                                    w->out_d.p, w->out_c.p, nullptr);
+                if (x) return x;
+                HIPCHK(hipEventRecord(e1, s));
+                HIPCHK(hipEventSynchronize(e1));
+                if (r >= 0) HIPCHK(hipEventElapsedTime(&t[r], e0, e1));
+            }
+            std::sort(t, t + 3);
+            if (&cd == &cands[0]) out->default_ms = t[1];
+            if (t[1] < best) {
+                best = t[1];
+                pick = cd;
+            }
+            out->tried++;
+        }
+        out->best_ms = best;
+        finish(pick.g, pick.c, pick.m);
+        return SPF_OK;
+    };
+    rc = run();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    release_ws(ix, w);
+    if (rc) {
+        ix->scan_g = g0;
+        ix->scan_chunk = c0;
+        ix->mix_g = m0;
+        return rc;
+    }
+    if (cache_path) {
+        if (FILE *f = std::fopen(cache_path, "a")) {
+            std::fprintf(f, "%s scan_g=%u scan_chunk=%u mix_g=%u best_ms=%.4f default_ms=%.4f\n", key,
+                         ix->scan_g, ix->scan_chunk, ix->mix_g, out->best_ms, out->default_ms);
+            std::fclose(f);
+        }
+    }
     return SPF_OK;
 }
 

@@ -70,6 +70,7 @@ struct ScanParams {
     const uint32_t *pair_off;   // [n_post + 1] (query, slot) pairs per posting
     const uint32_t *pair_q;
     const uint64_t *pair_slot;  // candidate-buffer position of the posting's first entry
+    uint32_t echunk;            // entries per work item (listed mode)
     // label filter (listed mode): entry label, per-query mask; null = unfiltered
     const uint8_t *label_e;
     const uint64_t *qmask;
@@ -169,6 +170,10 @@ struct MixedParams {
     const uint32_t *del;
     uint32_t metric;            // key: L2 |q16-x16|^2, DOT 1-q16.x16, COSINE 1-cos(q16,x16)
     uint32_t echunk;            // entries per work item (64 x tiles per wave)
+    uint32_t qg;                // queries per work item (a multiple of 32, at most kMixQ)
+    // probe slot ranges start at multiples of 8 (plan align 8): keys leave
+    // through an LDS tile as aligned 16-byte rows, the posting's last item
+    // filling its range's tail with "no key"
     const uint8_t *label_e;     // label filter: entry label, per-query mask (null = off)
     const uint64_t *qmask;
     uint16_t *out16;            // 16-bit candidate keys
@@ -208,6 +213,11 @@ struct RerankPrunedParams {
     const float *post_xn;       // per posting max |x16|^2
     uint64_t *out;              // [nq][k] (dist key << 32 | rank)
     uint32_t *out_cnt;
+    // final outputs instead (when ids is set): rank -> id, key -> distance
+    const uint64_t *rank_id;
+    uint64_t *ids;
+    float *dists;
+    uint32_t *counts;
     uint32_t *stats;            // optional [nq]: raw rows gathered per query
 };
 
@@ -311,6 +321,7 @@ struct PlanParams {
     uint32_t n_items;           // item table size (known after the item scan)
     uint32_t G;                 // queries per work item
     uint32_t echunk;            // entries per work item
+    uint32_t align;             // each probe's candidate slots start at a multiple of this
 };
 
 struct RerankParams {

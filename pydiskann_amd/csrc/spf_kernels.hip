@@ -174,12 +174,12 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
             const uint32_t pp = p.item_post[item];
             const uint32_t local = item - p.item_off[pp];
             const uint32_t pb = p.post_off[pp], pe = p.post_off[pp + 1];
-            const uint32_t nec = (pe - pb + kEChunk - 1) / kEChunk;
+            const uint32_t nec = (pe - pb + p.echunk - 1) / p.echunk;
             const uint32_t qc = local / nec, ec = local - qc * nec;
             const uint32_t pr0 = p.pair_off[pp] + qc * G;
             gcnt = min((uint32_t)G, p.pair_off[pp + 1] - pr0);
-            e_begin = pb + ec * kEChunk;
-            e_end = min(e_begin + kEChunk, pe);
+            e_begin = pb + ec * p.echunk;
+            e_end = min(e_begin + p.echunk, pe);
             base_e = pb;
             if (tid < gcnt) {
                 s_qi[tid] = p.pair_q[pr0 + tid];
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(256) void k_plan_q(const PlanParams p) {
         uint32_t len = 0;
         if (j < kept) {
             const uint32_t post = (uint32_t)sel[j];
-            len = p.post_len[post];
+            len = (p.post_len[post] + p.align - 1) / p.align * p.align;   // slots reserved
             const size_t i = (size_t)q * p.npr + j;
             p.probe_post[i] = post;
             p.prank[i] = atomicAdd(&p.post_cnt[post], 1u);

@@ -82,6 +82,8 @@ hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t
 template <int KS>
 __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
     constexpr int T = KS <= 4 ? 4 : 2;   // tiles per wave, mfma_tiles()
+    constexpr uint32_t EW = 64 * T;      // entries per item
+    __shared__ __align__(16) uint16_t s_tile[32][EW];   // one query group's keys
     __shared__ uint64_t s_slot[kMixQ];
     __shared__ uint32_t s_qi[kMixQ];
     __shared__ float s_qn[kMixQ];
@@ -96,8 +98,8 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
         const uint32_t pb = p.post_off[pp], pe = p.post_off[pp + 1];
         const uint32_t nec = (pe - pb + p.echunk - 1) / p.echunk;
         const uint32_t qc = local / nec, ec = local - qc * nec;
-        const uint32_t pr0 = p.pair_off[pp] + qc * kMixQ;
-        const uint32_t gcnt = min(kMixQ, p.pair_off[pp + 1] - pr0);
+        const uint32_t pr0 = p.pair_off[pp] + qc * p.qg;
+        const uint32_t gcnt = min(p.qg, p.pair_off[pp + 1] - pr0);
         const uint32_t e_begin = pb + ec * p.echunk, e_end = min(e_begin + p.echunk, pe);
         __syncthreads();   // the previous item is done with the query table
         if (tid < kMixQ) {
@@ -158,27 +160,40 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
                     acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b, acc0, 0, 0, 0);
                     acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b, acc1, 0, 0, 0);
                 }
-                const uint32_t e = e0 + col;
-                if (e < e_end) {
-                    const uint32_t off = e - pb;
+                // keys into the group tile (beyond e_end: "no key", the
+                // aligned range's tail)
+                const uint32_t e = e0 + col, el = e - e_begin;
+                const bool in = e < e_end;
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const uint32_t r0 = g0 + 4 * kg + j, r1 = r0 + 16;
-                        if (r0 < gcnt) {
-                            const float d = l2 ? xn[t] + s_qn[r0] - 2.0f * acc0[j]
-                                               : 1.0f - acc0[j] * s_qn[r0] * rx[t];
-                            const bool ok = live[t] && (!p.qmask || label_ok(s_mask[r0], lab[t]));
-                            p.out16[s_slot[r0] + off] = ok ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
-                        }
-                        if (r1 < gcnt) {
-                            const float d = l2 ? xn[t] + s_qn[r1] - 2.0f * acc1[j]
-                                               : 1.0f - acc1[j] * s_qn[r1] * rx[t];
-                            const bool ok = live[t] && (!p.qmask || label_ok(s_mask[r1], lab[t]));
-                            p.out16[s_slot[r1] + off] = ok ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
-                        }
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t r0 = g0 + 4 * kg + j, r1 = r0 + 16;
+                    if (r0 < gcnt) {
+                        const float d = l2 ? xn[t] + s_qn[r0] - 2.0f * acc0[j]
+                                           : 1.0f - acc0[j] * s_qn[r0] * rx[t];
+                        const bool ok = in && live[t] && (!p.qmask || label_ok(s_mask[r0], lab[t]));
+                        s_tile[r0 - g0][el] = ok ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
+                    }
+                    if (r1 < gcnt) {
+                        const float d = l2 ? xn[t] + s_qn[r1] - 2.0f * acc1[j]
+                                           : 1.0f - acc1[j] * s_qn[r1] * rx[t];
+                        const bool ok = in && live[t] && (!p.qmask || label_ok(s_mask[r1], lab[t]));
+                        s_tile[r1 - g0][el] = ok ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
                     }
                 }
             }
+            __syncthreads();
+            // the tile's rows as aligned 16-byte stores: each probe's slot
+            // range starts at a multiple of 8 and is reserved up to the next
+            // multiple of 8 past the posting's end
+            const uint32_t len = e_end - e_begin;
+            const uint32_t lp = e_end == pe ? (len + 7) & ~7u : len;
+            const uint32_t cpr = lp / 8, rows = min(32u, gcnt - g0);
+            for (uint32_t c = tid; c < rows * cpr; c += 256) {
+                const uint32_t r = c / cpr, ch = c - r * cpr;
+                const uint4 v = *reinterpret_cast<const uint4 *>(&s_tile[r][8 * ch]);
+                *reinterpret_cast<uint4 *>(p.out16 + s_slot[g0 + r] + (e_begin - pb) + 8 * ch) = v;
+            }
+            __syncthreads();   // the next group reuses the tile
         }
     }
 }
@@ -860,7 +875,9 @@ __device__ __forceinline__ void rerank_list(const RerankPrunedParams &p, const u
 }
 
 // the k smallest of keys[0..cnt) (distinct) in ascending order -> out[0..min(k, cnt))
-__device__ __forceinline__ void smallest_k(uint64_t *keys, uint32_t cnt, uint32_t k, uint64_t *out) {
+// emit(position, key) for the k smallest keys
+template <class Emit>
+__device__ __forceinline__ void smallest_k(uint64_t *keys, uint32_t cnt, uint32_t k, Emit &&emit) {
     const uint32_t tid = threadIdx.x;
     const uint32_t kk = min(k, cnt);
     if (cnt <= kPrCount) {   // rank by counting, scatter
@@ -868,7 +885,7 @@ __device__ __forceinline__ void smallest_k(uint64_t *keys, uint32_t cnt, uint32_
             const uint64_t v = keys[i];
             uint32_t c = 0;
             for (uint32_t j = 0; j < cnt; j++) c += keys[j] < v;
-            if (c < kk) out[c] = v;
+            if (c < kk) emit(c, v);
         }
     } else {
         uint32_t m = 2;
@@ -876,7 +893,7 @@ __device__ __forceinline__ void smallest_k(uint64_t *keys, uint32_t cnt, uint32_
         for (uint32_t i = cnt + tid; i < m; i += 256) keys[i] = kNoKey;
         __syncthreads();
         bitonic_lds(keys, m);
-        for (uint32_t i = tid; i < kk; i += 256) out[i] = keys[i];
+        for (uint32_t i = tid; i < kk; i += 256) emit(i, keys[i]);
     }
 }
 
@@ -982,8 +999,26 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
         L.m[XNQ] = 0u;
         L.m[THR] = 0xffffu;
         L.kth = kNoKey;   // no bound unless phase A yields its k-th key
-        p.out_cnt[q] = min(p.k, c);
+        if (p.ids)
+            p.counts[q] = min(p.k, c);
+        else
+            p.out_cnt[q] = min(p.k, c);
     }
+    // the k smallest: keys, or (with p.ids) the caller's ids and distances
+    // with the row padded by UINT64_MAX / +inf
+    auto emit = [&](uint32_t pos, uint64_t v) {
+        if (p.ids) {
+            p.ids[(size_t)q * p.k + pos] = p.rank_id[(uint32_t)v];
+            p.dists[(size_t)q * p.k + pos] = key_float_m((uint32_t)(v >> 32));
+        } else {
+            out[pos] = v;
+        }
+    };
+    if (p.ids)
+        for (uint32_t i = min(p.k, c) + tid; i < p.k; i += 256) {
+            p.ids[(size_t)q * p.k + i] = ~0ull;
+            p.dists[(size_t)q * p.k + i] = __builtin_inff();
+        }
     for (uint32_t i = tid; i < p.dpad; i += 256) s_qv[i] = p.qv[(size_t)q * p.dpad + i];
     const float qn = p.qnorm ? p.qnorm[q] : 0.0f;
     // probe table (segment-relative first slot, first entry of each probe)
@@ -1015,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
                                              L.ent, c);
         rerank_list(p, L.ent, cnt, s_qv, qn, L.ex);
         __syncthreads();
-        smallest_k(L.ex, cnt, p.k, out);
+        smallest_k(L.ex, cnt, p.k, emit);
         if (p.stats && tid == 0) p.stats[q] = cnt;
         return;
     }
@@ -1090,7 +1125,7 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
     }, tab, L.ent + na, c - na);
     rerank_list(p, L.ent + na, nb, s_qv, qn, L.ex + na);
     __syncthreads();
-    smallest_k(L.ex, na + nb, p.k, out);
+    smallest_k(L.ex, na + nb, p.k, emit);
     if (p.stats && tid == 0) p.stats[q] = na + nb;
 }
 

@@ -42,13 +42,14 @@ def test_every_declared_symbol_is_exported():
 def test_struct_layouts_match_the_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "spf.h"\n'
-                   "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\","
+                   "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\","
                    "sizeof(spf_index_desc), offsetof(spf_index_desc, probe_ratio),"
                    "sizeof(spf_search_opts), sizeof(spf_profile), sizeof(spf_index_info),"
                    "offsetof(spf_index_desc, rabitq_signs),"
                    "offsetof(spf_index_desc, labels), sizeof(spf_server_opts),"
                    "sizeof(spf_request_info), sizeof(spf_server_stats),"
-                   "sizeof(spf_graph_desc), offsetof(spf_index_desc, graph));return 0;}\n")
+                   "sizeof(spf_graph_desc), offsetof(spf_index_desc, graph),"
+                   "sizeof(spf_tune_result));return 0;}\n")
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
@@ -58,7 +59,7 @@ def test_struct_layouts_match_the_header(tmp_path):
             C.sizeof(_lib.SearchOpts), C.sizeof(_lib.Profile), C.sizeof(_lib.IndexInfo),
             _lib.IndexDesc.rabitq_signs.offset, _lib.IndexDesc.labels.offset,
             C.sizeof(_lib.ServerOpts), C.sizeof(_lib.RequestInfo), C.sizeof(_lib.ServerStats),
-            C.sizeof(_lib.GraphDesc), _lib.IndexDesc.graph.offset]
+            C.sizeof(_lib.GraphDesc), _lib.IndexDesc.graph.offset, C.sizeof(_lib.TuneResult)]
     assert got == want
 
 
@@ -90,6 +91,8 @@ def test_argument_errors_without_a_device():
     assert L.spf_server_wait(None, 1, 0, None) == _lib.SPF_EINVAL
     assert L.spf_server_get_stats(None, None) == _lib.SPF_EINVAL
     L.spf_server_destroy(None)
+    r = _lib.TuneResult()
+    assert L.spf_autotune(None, None, 1, 4, 10, 8, None, None, C.byref(r)) == _lib.SPF_EINVAL
 
 
 def test_python_mirror_argument_errors():
