@@ -172,7 +172,6 @@ struct spf_index {
     // working-memory bound inputs: top_len[j] = entries in the j longest
     // postings; posting chunk counts at the VALU and matrix-core chunk sizes
     std::vector<uint64_t> top_len;
-    std::vector<uint32_t> h_post_len;   // entries per (device-order) posting
     uint64_t chunks_m = 0;
     uint64_t chunks_v[4] = {0, 0, 0, 0};   // posting chunks at 256, 512, 1024, 2048 entries
     // launch shapes (spf_autotune): VALU scan queries and entries per item,
@@ -442,7 +441,6 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
     }
     HIPCHK(upload(&ix->d_post_len, post_len.data(), post_len.size(), bytes));
     {
-        ix->h_post_len = post_len;
         std::vector<uint32_t> sorted(post_len);
         std::sort(sorted.begin(), sorted.end(), std::greater<uint32_t>());
         ix->top_len.assign(P + 1, 0);
@@ -891,7 +889,7 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     add(q + 1, 8);               // cand_len
     add(q, 4);                   // cand_cnt
     add(q + 1, 8);               // cand_off
-    add(P + 1, 4);               // post_cnt
+    add(P + 4, 4);               // post_cnt (+ the real-candidate counter)
     add(pairs, 4);               // qp_rel
     add(pairs, 4);               // prank
     add(P + 1, 4);               // pair_off
@@ -1213,7 +1211,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(w->cand_len.ensure(nq + 1));
     HIPCHK(w->cand_cnt.ensure(nq));
     HIPCHK(w->cand_off.ensure(nq + 1));
-    HIPCHK(w->post_cnt.ensure(P + 1));
+    HIPCHK(w->post_cnt.ensure(P + 4));
     HIPCHK(w->qp_rel.ensure((size_t)nq * npr));
     HIPCHK(w->prank.ensure((size_t)nq * npr));
     HIPCHK(w->pair_off.ensure(P + 1));
@@ -1248,7 +1246,10 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.G = G;
     pl.echunk = echunk;
     pl.align = z.mfma_scan ? 8u : 1u;   // the matrix-core scan stores aligned 16-byte rows
-    HIPCHK(hipMemsetAsync(w->post_cnt.p, 0, (P + 1) * sizeof(uint32_t), s));
+    // pairs per posting, then (8-byte aligned) the batch's candidates without
+    // alignment slots, zeroed together
+    pl.real_total = reinterpret_cast<unsigned long long *>(w->post_cnt.p + ((P + 2) & ~1u));
+    HIPCHK(hipMemsetAsync(w->post_cnt.p, 0, (P + 4) * sizeof(uint32_t), s));
     HIPCHK(launch_plan_q(pl, s));
     HIPCHK(launch_excl_scan_u64(w->cand_len.p, w->cand_off.p, nq, s));
     HIPCHK(launch_excl_scan_u32(w->post_cnt.p, w->pair_off.p, P, s));
@@ -1419,15 +1420,8 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         std::memset(&g_prof, 0, sizeof g_prof);
         for (int i = 0; i < SPF_N_STAGES; i++)
             HIPCHK(hipEventElapsedTime(&g_prof.stage_ms[i], w->ev[i], w->ev[i + 1]));
-        uint64_t real = total;   // candidates without the alignment slots
-        if (pl.align > 1) {
-            std::vector<uint32_t> npq(nq), pp((size_t)nq * npr);
-            HIPCHK(hipMemcpy(npq.data(), w->np_q.p, nq * 4, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(pp.data(), w->probe_post.p, pp.size() * 4, hipMemcpyDeviceToHost));
-            real = 0;
-            for (uint32_t q = 0; q < nq; q++)
-                for (uint32_t j = 0; j < npq[q]; j++) real += ix->h_post_len[pp[(size_t)q * npr + j]];
-        }
+        uint64_t real = 0;   // candidates without the alignment slots
+        HIPCHK(hipMemcpy(&real, pl.real_total, 8, hipMemcpyDeviceToHost));
         g_prof.candidates = real;
         g_prof.scan_code_bytes =
             real * (ix->quant == SPF_QUANT_RABITQ ? (uint64_t)ix->code_bytes

@@ -322,6 +322,7 @@ struct PlanParams {
     uint32_t G;                 // queries per work item
     uint32_t echunk;            // entries per work item
     uint32_t align;             // each probe's candidate slots start at a multiple of this
+    unsigned long long *real_total;   // += the batch's candidates without alignment slots
 };
 
 struct RerankParams {

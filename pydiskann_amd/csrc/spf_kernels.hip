@@ -628,12 +628,13 @@ __global__ __launch_bounds__(256) void k_plan_q(const PlanParams p) {
                 }
             }
     }
-    uint32_t run = 0;   // slots of the probes before this round
+    uint32_t run = 0, real = 0;   // slots of the probes before this round; entries
     for (uint32_t j0 = 0; j0 < kept; j0 += 64) {
         const uint32_t j = j0 + lane;
         uint32_t len = 0;
         if (j < kept) {
             const uint32_t post = (uint32_t)sel[j];
+            real += p.post_len[post];
             len = (p.post_len[post] + p.align - 1) / p.align * p.align;   // slots reserved
             const size_t i = (size_t)q * p.npr + j;
             p.probe_post[i] = post;
@@ -648,7 +649,9 @@ __global__ __launch_bounds__(256) void k_plan_q(const PlanParams p) {
         if (j < kept) p.qp_rel[(size_t)q * p.npr + j] = run + incl - len;
         run += __shfl(incl, 63, 64);
     }
+    for (int off = 32; off > 0; off >>= 1) real += __shfl_xor(real, off, 64);
     if (lane == 0) {
+        atomicAdd(p.real_total, (unsigned long long)real);
         p.np_q[q] = kept;
         p.cand_cnt[q] = run;
         p.cand_len[q] = (run + 31) & ~31ull;   // 32-aligned segments: a query owns whole bitmap words
