@@ -85,7 +85,7 @@ struct DBuf {
     X(uint64_t, rq_planes) X(float, rq_consts) X(uint32_t, pair_src) X(float, eq16)        \
     X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)        \
     X(uint32_t, gflag) X(uint32_t, gscr) X(uint32_t, sel_ccnt) X(uint32_t, sel_coff)            \
-    X(uint64_t, sel_tmp) X(uint64_t, probe_tmp)
+    X(uint64_t, sel_tmp) X(uint64_t, probe_tmp) X(uint32_t, cand_real)
 
 struct Workspace {
     hipStream_t stream = nullptr;
@@ -889,7 +889,8 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     add(q + 1, 8);               // cand_len
     add(q, 4);                   // cand_cnt
     add(q + 1, 8);               // cand_off
-    add(P + 4, 4);               // post_cnt (+ the real-candidate counter)
+    add(P + 1, 4);               // post_cnt
+    add(q, 4);                   // cand_real (profiled calls)
     add(pairs, 4);               // qp_rel
     add(pairs, 4);               // prank
     add(P + 1, 4);               // pair_off
@@ -1113,6 +1114,10 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         pp.qnbf = w->qnbf.p;
         pp.dpad32 = dpad32;
     }
+    // the planner's per-posting pair counts and candidate counter, zeroed by k_prep
+    HIPCHK(w->post_cnt.ensure(P + 1));
+    pp.zero = w->post_cnt.p;
+    pp.zero_n = P + 1;
     HIPCHK(launch_prep(pp, s));
     HIPCHK(mark(SPF_STAGE_PROBE));
     const float *qv_scan = pp.qv_scan;
@@ -1211,7 +1216,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(w->cand_len.ensure(nq + 1));
     HIPCHK(w->cand_cnt.ensure(nq));
     HIPCHK(w->cand_off.ensure(nq + 1));
-    HIPCHK(w->post_cnt.ensure(P + 4));
+    HIPCHK(w->post_cnt.ensure(P + 1));
     HIPCHK(w->qp_rel.ensure((size_t)nq * npr));
     HIPCHK(w->prank.ensure((size_t)nq * npr));
     HIPCHK(w->pair_off.ensure(P + 1));
@@ -1246,10 +1251,16 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.G = G;
     pl.echunk = echunk;
     pl.align = z.mfma_scan ? 8u : 1u;   // the matrix-core scan stores aligned 16-byte rows
-    // pairs per posting, then (8-byte aligned) the batch's candidates without
-    // alignment slots, zeroed together
-    pl.real_total = reinterpret_cast<unsigned long long *>(w->post_cnt.p + ((P + 2) & ~1u));
-    HIPCHK(hipMemsetAsync(w->post_cnt.p, 0, (P + 4) * sizeof(uint32_t), s));
+    if (mixed) {   // the rerank's query-order counters, zeroed by the planner
+        HIPCHK(w->perm_cnt.ensure(P + 2));
+        HIPCHK(w->perm_fill.ensure(P + 2));
+        pl.zero_a = w->perm_cnt.p;
+        pl.zero_b = w->perm_fill.p;
+    }
+    if (prof) {   // per-query candidates without alignment slots, for the profile
+        HIPCHK(w->cand_real.ensure(nq));
+        pl.cand_real = w->cand_real.p;
+    }
     HIPCHK(launch_plan_q(pl, s));
     HIPCHK(launch_excl_scan_u64(w->cand_len.p, w->cand_off.p, nq, s));
     HIPCHK(launch_excl_scan_u32(w->post_cnt.p, w->pair_off.p, P, s));
@@ -1421,7 +1432,11 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         for (int i = 0; i < SPF_N_STAGES; i++)
             HIPCHK(hipEventElapsedTime(&g_prof.stage_ms[i], w->ev[i], w->ev[i + 1]));
         uint64_t real = 0;   // candidates without the alignment slots
-        HIPCHK(hipMemcpy(&real, pl.real_total, 8, hipMemcpyDeviceToHost));
+        {
+            std::vector<uint32_t> cr(nq);
+            HIPCHK(hipMemcpy(cr.data(), pl.cand_real, nq * 4, hipMemcpyDeviceToHost));
+            for (uint32_t v : cr) real += v;
+        }
         g_prof.candidates = real;
         g_prof.scan_code_bytes =
             real * (ix->quant == SPF_QUANT_RABITQ ? (uint64_t)ix->code_bytes

@@ -99,6 +99,8 @@ struct PrepParams {
     const float *mu;            // [dim] centring vector of the bf16 probe (null = skip)
     uint16_t *qbf;              // [nq][dpad32] bf16(q - mu)
     float *qnbf;                // [nq] |bf16(q - mu)|^2
+    uint32_t *zero;             // zeroed alongside (the planner's counters; null = none)
+    uint32_t zero_n;
 };
 
 // Routing-graph probe (one wave per query): beam search over the centroid
@@ -322,7 +324,8 @@ struct PlanParams {
     uint32_t G;                 // queries per work item
     uint32_t echunk;            // entries per work item
     uint32_t align;             // each probe's candidate slots start at a multiple of this
-    unsigned long long *real_total;   // += the batch's candidates without alignment slots
+    uint32_t *cand_real;        // [nq] candidates without alignment slots (null = skip)
+    uint32_t *zero_a, *zero_b;  // [n_post + 2] each, zeroed by k_items (null = none)
 };
 
 struct RerankParams {

@@ -68,6 +68,8 @@ __device__ __forceinline__ uint32_t next_pow2(uint32_t v) {
 __global__ __launch_bounds__(64) void k_prep(const PrepParams p) {
     extern __shared__ float s_v[];   // dpad floats
     const uint32_t q = blockIdx.x, lane = threadIdx.x;
+    if (p.zero)
+        for (uint32_t i = q * 64 + lane; i < p.zero_n; i += gridDim.x * 64) p.zero[i] = 0u;
     const float *src = p.q + (size_t)q * p.dim;
     for (uint32_t i = lane; i < p.dpad; i += 64) {
         float v = i < p.dim ? src[i] : 0.0f;
@@ -651,7 +653,7 @@ __global__ __launch_bounds__(256) void k_plan_q(const PlanParams p) {
     }
     for (int off = 32; off > 0; off >>= 1) real += __shfl_xor(real, off, 64);
     if (lane == 0) {
-        atomicAdd(p.real_total, (unsigned long long)real);
+        if (p.cand_real) p.cand_real[q] = real;
         p.np_q[q] = kept;
         p.cand_cnt[q] = run;
         p.cand_len[q] = (run + 31) & ~31ull;   // 32-aligned segments: a query owns whole bitmap words
@@ -676,6 +678,11 @@ __global__ void k_scatter_pairs(const PlanParams p) {
 
 __global__ void k_items(const PlanParams p) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p.zero_a)   // the later query-order count arrays (n_post + 2 each), zeroed here
+        for (uint32_t i = b; i < p.n_post + 2; i += gridDim.x * blockDim.x) {
+            p.zero_a[i] = 0u;
+            p.zero_b[i] = 0u;
+        }
     if (b >= p.n_post) return;
     const uint32_t cnt = p.pair_off[b + 1] - p.pair_off[b];
     const uint32_t len = p.post_len[b];
@@ -979,7 +986,7 @@ hipError_t launch_scatter_pairs(const PlanParams &p, hipStream_t s) {
 }
 
 hipError_t launch_items(const PlanParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_items, dim3((p.n_post + 255) / 256), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_items, dim3((p.n_post + 2 + 255) / 256), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

@@ -1179,14 +1179,12 @@ hipError_t launch_probe_mfma(const ProbeMfmaParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
+// cnt and fill ([n_post + 2] each) arrive zeroed (the planner's k_items)
 hipError_t launch_query_perm(const uint32_t *np_q, const uint32_t *probe_post, uint32_t npr,
                              uint32_t nq, uint32_t n_post, uint32_t *cnt, uint32_t *off,
                              uint32_t *fill, uint32_t *qperm, hipStream_t s) {
     if (nq == 0) return hipSuccess;
-    hipError_t e = hipMemsetAsync(cnt, 0, (n_post + 2) * sizeof(uint32_t), s);
-    if (e != hipSuccess) return e;
-    e = hipMemsetAsync(fill, 0, (n_post + 2) * sizeof(uint32_t), s);
-    if (e != hipSuccess) return e;
+    hipError_t e;
     hipLaunchKernelGGL(k_perm_count, dim3((nq + 255) / 256), dim3(256), 0, s, np_q, probe_post,
                        npr, nq, n_post, cnt);
     e = launch_excl_scan_u32(cnt, off, n_post + 1, s);
