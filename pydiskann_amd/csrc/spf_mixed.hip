@@ -367,10 +367,7 @@ struct ProbeTab {
     }
 };
 
-// One wavefront per query (four queries per workgroup), so no step needs a
-// workgroup barrier: scans are wave shuffles and each wave owns its LDS
-// histogram.  Lane l holds bitmap-words l, l+64, ... of the segment (word w =
-// 32 candidate slots); REG keeps up to 4 words per lane in registers.
+// Wave-level scans and bin search for the pool select below.
 __device__ __forceinline__ uint32_t wave_excl(uint32_t v, uint32_t *tot) {
     const uint32_t incl = wave_incl_scan(v);
     *tot = __shfl(incl, 63, 64);
@@ -435,26 +432,22 @@ __device__ __forceinline__ void word_masks(const uint4 (&r)[4], uint32_t w, uint
     eq = e;
 }
 
-// the same for two thresholds T and TA
-__device__ __forceinline__ void word_masks2(const uint4 (&r)[4], uint32_t w, uint32_t n,
-                                            uint32_t T, uint32_t TA, uint32_t &live,
-                                            uint32_t &lt, uint32_t &eq, uint32_t &ltA,
-                                            uint32_t &eqA) {
+// bit masks of one 8-key unit for two thresholds T and TA: live keys, keys
+// below T, keys equal to T, keys below TA, keys equal to TA
+__device__ __forceinline__ void unit_masks2(const uint4 v, uint32_t u, uint32_t n, uint32_t T,
+                                            uint32_t TA, uint32_t &live, uint32_t &lt,
+                                            uint32_t &eq, uint32_t &ltA, uint32_t &eqA) {
     uint32_t lv = 0, l = 0, e = 0, la = 0, ea = 0;
+    const uint32_t v4[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const uint32_t v4[4] = {r[c].x, r[c].y, r[c].z, r[c].w};
-#pragma unroll
-        for (int t = 0; t < 8; t++) {
-            const uint32_t j = 8 * c + t;
-            const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
-            const uint32_t bit = (uint32_t)(kk != 0xffffu && 32 * w + j < n) << j;
-            lv |= bit;
-            l |= kk < T ? bit : 0u;
-            e |= kk == T ? bit : 0u;
-            la |= kk < TA ? bit : 0u;
-            ea |= kk == TA ? bit : 0u;
-        }
+    for (int t = 0; t < 8; t++) {
+        const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+        const uint32_t bit = (uint32_t)(kk != 0xffffu && 8 * u + t < n) << t;
+        lv |= bit;
+        l |= kk < T ? bit : 0u;
+        e |= kk == T ? bit : 0u;
+        la |= kk < TA ? bit : 0u;
+        ea |= kk == TA ? bit : 0u;
     }
     live = lv;
     lt = l;
@@ -463,14 +456,11 @@ __device__ __forceinline__ void word_masks2(const uint4 (&r)[4], uint32_t w, uin
     eqA = ea;
 }
 
-// key j (runtime) of a 32-key word
-__device__ __forceinline__ uint32_t word_key(const uint4 (&r)[4], uint32_t j) {
-    const uint32_t c = j >> 3, t = (j >> 1) & 3;
-    uint32_t d[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) d[i] = t == 0 ? r[i].x : t == 1 ? r[i].y : t == 2 ? r[i].z : r[i].w;
-    const uint32_t v = c == 0 ? d[0] : c == 1 ? d[1] : c == 2 ? d[2] : d[3];
-    return (v >> (16 * (j & 1))) & 0xffffu;
+// key j (runtime, 0..7) of an 8-key unit
+__device__ __forceinline__ uint32_t unit_key(const uint4 v, uint32_t j) {
+    const uint32_t t = j >> 1;
+    const uint32_t d = t == 0 ? v.x : t == 1 ? v.y : t == 2 ? v.z : v.w;
+    return (d >> (16 * (j & 1))) & 0xffffu;
 }
 
 // the lowest `c` set bits of m
@@ -485,161 +475,337 @@ __device__ __forceinline__ uint32_t lowest_bits(uint32_t m, uint32_t c) {
     return out;
 }
 
-// One wavefront per query: the `want` smallest 16-bit keys of its segment,
-// exactly-tied keys at the threshold taken in slot order, written as one
-// selection mask per 32-slot word (sel_bits, at the segment's word offset;
-// a_bits marks the `ca` smallest of them the same way), or else as entries
-// (map) or slots, with their keys (pool_key, optional).  hist: 768 words.
-// Lane l owns words
-// l, l+64, ... (word w = slots 32w..32w+31) and re-reads them from L2 in each
-// of the four passes; a word's four loads go out together.
-template <bool BITS>
-__device__ __forceinline__ void pool_wave(const uint16_t *kp, uint32_t q, uint32_t n,
-                                          uint32_t want, uint32_t *pool_entry,
-                                          uint16_t *pool_key, uint32_t *pool_want, bool map,
-                                          const ProbeTab &tab, uint32_t *hist,
-                                          uint32_t *sel_bits, uint32_t *a_bits, uint32_t ca) {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t nwords = (n + 31) / 32;
-    // the segment's 16-byte units (8 keys each), lane-interleaved, four units
-    // per lane in flight: body(unit, index)
+// The threads that select one query's pool: NW wavefronts (NW = 1: four
+// queries per workgroup and no workgroup barrier; NW = 4: one query per
+// workgroup).  Thread t of the group owns the segment's 16-byte units (8 keys)
+// t, t + 64*NW, t + 2*64*NW, ...; sx is the group's LDS for cross-wave sums.
+template <int NW>
+struct PoolGroup {
+    static constexpr uint32_t T = 64 * NW;
+    uint32_t *sx;        // 2 * NW scan totals (double-buffered) + 3 * NW reductions
+    uint32_t parity = 0;
+    __device__ static __forceinline__ uint32_t tid() { return NW == 1 ? (threadIdx.x & 63) : threadIdx.x; }
+    __device__ static __forceinline__ void sync() {
+        if (NW == 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            __syncthreads();
+        }
+    }
+    // exclusive scan over the group in thread order; *tot = group sum.  A
+    // buffer is rewritten two scans later, after a barrier that every reader
+    // of it has passed.
+    __device__ __forceinline__ uint32_t excl(uint32_t v, uint32_t *tot) {
+        uint32_t wt;
+        const uint32_t e = wave_excl(v, &wt);
+        if (NW == 1) {
+            *tot = wt;
+            return e;
+        }
+        const uint32_t wv = threadIdx.x >> 6;
+        uint32_t *buf = sx + parity * NW;
+        parity ^= 1u;
+        if ((threadIdx.x & 63) == 0) buf[wv] = wt;
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            const uint32_t x = buf[i];
+            before += (uint32_t)i < wv ? x : 0u;
+            all += x;
+        }
+        *tot = all;
+        return e + before;
+    }
+    // group-wide sum, min and max (one call per kernel)
+    __device__ __forceinline__ void reduce3(uint32_t &sum, uint32_t &mn, uint32_t &mx) {
+        for (int off = 32; off > 0; off >>= 1) {
+            sum += __shfl_xor(sum, off, 64);
+            mn = min(mn, (uint32_t)__shfl_xor(mn, off, 64));
+            mx = max(mx, (uint32_t)__shfl_xor(mx, off, 64));
+        }
+        if (NW == 1) return;
+        uint32_t *r = sx + 2 * NW;
+        const uint32_t wv = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            r[3 * wv] = sum;
+            r[3 * wv + 1] = mn;
+            r[3 * wv + 2] = mx;
+        }
+        __syncthreads();
+        sum = 0;
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            sum += r[3 * i];
+            mn = min(mn, r[3 * i + 1]);
+            mx = max(mx, r[3 * i + 2]);
+        }
+    }
+};
+
+// Two 16-bit keys per 32-bit register, handled with packed 16-bit integer ops.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// keys of unit u at slots >= n (the segment's last unit) read as "no key"
+__device__ __forceinline__ void unit_tail(uint4 &v, uint32_t u, uint32_t n) {
+    const uint32_t r = n - 8 * u;   // keys of this unit inside the segment
+    if (r >= 8) return;
+    uint32_t c[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++)
+        c[i] |= (2 * i >= r ? 0xffffu : 0u) | (2 * i + 1 >= r ? 0xffff0000u : 0u);
+    v = make_uint4(c[0], c[1], c[2], c[3]);
+}
+
+// smallest key of a unit
+__device__ __forceinline__ uint32_t unit_min(const uint4 v) {
+    const u16x2 m = __builtin_elementwise_min(
+        __builtin_elementwise_min(as_u16x2(v.x), as_u16x2(v.y)),
+        __builtin_elementwise_min(as_u16x2(v.z), as_u16x2(v.w)));
+    return min((uint32_t)m.x, (uint32_t)m.y);
+}
+
+// A wave's view of a histogram of 64 * NB bins in LDS: lane l covers bins
+// NB*l .. NB*l + NB-1 (their sum and the count below them); the histogram
+// stays in place while find() runs.
+template <int NB>
+struct BinTab {
+    const uint32_t *hist;
+    uint32_t sm, excl, tot;
+    __device__ __forceinline__ void load(const uint32_t *h) {
+        const uint32_t lane = threadIdx.x & 63;
+        hist = h;
+        sm = 0;
+#pragma unroll
+        for (int i = 0; i < NB; i++) sm += h[NB * lane + i];
+        excl = wave_excl(sm, &tot);
+    }
+    // bin where the running count reaches rem (1 <= rem <= tot): (bin, below)
+    __device__ __forceinline__ void find(uint32_t rem, uint32_t &bin, uint32_t &below) const {
+        const uint32_t lane = threadIdx.x & 63;
+        uint32_t b = 0, bl = 0;
+        const bool mine = excl < rem && rem <= excl + sm;
+        if (mine) {
+            uint32_t c = excl;
+            for (int i = 0; i < NB; i++) {
+                const uint32_t h = hist[NB * lane + i];
+                if (rem <= c + h) {
+                    b = NB * lane + i;
+                    bl = c;
+                    break;
+                }
+                c += h;
+            }
+        }
+        const uint64_t who = __ballot(mine);
+        const int src = who ? __ffsll((unsigned long long)who) - 1 : 0;
+        bin = __shfl(b, src, 64);
+        below = __shfl(bl, src, 64);
+    }
+};
+
+// The `want` smallest 16-bit keys of one query's segment, exactly-tied keys
+// at the threshold taken in slot order, written as one selection mask per
+// 32-slot word (sel_bits, at the segment's word offset; a_bits marks the `ca`
+// smallest of them the same way), or else as entries (map) or slots, with
+// their keys (pool_key, optional).  hist: 1,152 words of LDS.
+// The first pass loads the group's units -- the first RU per thread stay in
+// registers, so a segment of up to RU * 8 * 64 * NW keys is read from memory
+// once; the rest is re-read by each later pass.  Passes: min / max (packed
+// 16-bit); a 1,024-bin histogram of (key - min) >> sh, whose total is the
+// live count; only when the keys span more than 1,024 values, histograms of
+// the low sh bits inside the two threshold bins; collect in slot order
+// (rounds of 64 * NW units), where a unit whose smallest key is above the
+// pool threshold is known to hold no member.
+template <bool BITS, int NW, int RU>
+__device__ __forceinline__ void pool_group(const uint16_t *kp, uint32_t q, uint32_t n,
+                                           uint32_t want, uint32_t *pool_entry,
+                                           uint16_t *pool_key, uint32_t *pool_want, bool map,
+                                           const ProbeTab &tab, uint32_t *hist, PoolGroup<NW> &g,
+                                           uint32_t *sel_bits, uint32_t *a_bits, uint32_t ca) {
+    constexpr uint32_t T = PoolGroup<NW>::T;
+    const uint32_t t = PoolGroup<NW>::tid(), lane = threadIdx.x & 63;
     const uint4 *kp4 = reinterpret_cast<const uint4 *>(kp);
     const uint32_t nun = (n + 7) / 8;
+    uint4 reg[RU];
+#pragma unroll
+    for (int i = 0; i < RU; i++)
+        if (t + T * i < nun) reg[i] = kp4[t + T * i];
+#pragma unroll
+    for (int i = 0; i < RU; i++)
+        if (t + T * i < nun) unit_tail(reg[i], t + T * i, n);
+    // every owned unit once, registers first: body(unit, index)
     auto sweep = [&](auto &&body) {
-        for (uint32_t u0 = lane; u0 < nun; u0 += 256) {
+#pragma unroll
+        for (int i = 0; i < RU; i++)
+            if (t + T * i < nun) body(reg[i], t + T * i);
+        for (uint32_t u0 = t + T * RU; u0 < nun; u0 += 4 * T) {
             uint4 v[4];
 #pragma unroll
             for (int i = 0; i < 4; i++)
-                if (u0 + 64 * i < nun) v[i] = kp4[u0 + 64 * i];
+                if (u0 + T * i < nun) v[i] = kp4[u0 + T * i];
 #pragma unroll
             for (int i = 0; i < 4; i++)
-                if (u0 + 64 * i < nun) body(v[i], u0 + 64 * i);
+                if (u0 + T * i < nun) {
+                    unit_tail(v[i], u0 + T * i, n);
+                    body(v[i], u0 + T * i);
+                }
         }
     };
-    uint32_t inval = 0, kmn = 0xffffu, kmx = 0u;
-    sweep([&](const uint4 v, uint32_t u) {
-        const uint32_t v4[4] = {v.x, v.y, v.z, v.w};
+    for (uint32_t i = t; i < 1152; i += T) hist[i] = 0;
+    // min over raw keys ("no key" = 0xffff never wins); max over key + 1
+    // (0xffff wraps to 0 and never wins)
+    u16x2 mn2 = {0xffff, 0xffff}, mx2 = {0, 0};
+    const u16x2 one2 = {1, 1};
+    sweep([&](const uint4 v, uint32_t) {
+        const uint32_t c[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-            const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
-            if (8 * u + t >= n) continue;
-            if (kk == 0xffffu) {
-                inval++;
-            } else {
-                kmn = min(kmn, kk);
-                kmx = max(kmx, kk);
-            }
+        for (int i = 0; i < 4; i++) {
+            mn2 = __builtin_elementwise_min(mn2, as_u16x2(c[i]));
+            mx2 = __builtin_elementwise_max(mx2, as_u16x2(c[i]) + one2);
         }
     });
-    for (int off = 32; off > 0; off >>= 1) {
-        inval += __shfl_xor(inval, off, 64);
-        kmn = min(kmn, (uint32_t)__shfl_xor(kmn, off, 64));
-        kmx = max(kmx, (uint32_t)__shfl_xor(kmx, off, 64));
-    }
-    const uint32_t nvalid = n - inval;
-    const uint32_t W = min(want, nvalid);
-    if (lane == 0) pool_want[q] = W;
-    const bool all = W == nvalid;
-    // phase-A rank (a_bits): the `ca` smallest of the pool, same tie rule
-    const uint32_t nA = a_bits ? min(ca, W) : W;
-    const bool allA = nA == nvalid;
-    uint32_t T = 0xffffu, need = 0, TA = 0xffffu, needA = 0;
-    if (!allA) {   // (all implies allA)
-        uint32_t *h1 = hist, *h2 = hist + 256, *h3 = hist + 512;
-        const uint32_t span = kmx - kmn;
-        const uint32_t sh = span < 256 ? 0 : 32 - __clz(span) - 8;
+    uint32_t zero = 0, kmn = min((uint32_t)mn2.x, (uint32_t)mn2.y);
+    uint32_t kmx1 = max((uint32_t)mx2.x, (uint32_t)mx2.y);   // largest live key + 1; 0: none
+    g.reduce3(zero, kmn, kmx1);   // (its barrier also orders the hist clear)
+    if (NW == 1) g.sync();
+    uint32_t *h1 = hist, *h2 = hist + 1024, *h3 = hist + 1088;
+    uint32_t nvalid = 0, T1 = 0xffffu, need = 0, TA = 0xffffu, needA = 0, W = 0;
+    bool all = true, allA = true;
+    if (kmx1 > 0) {
+        const uint32_t span = kmx1 - 1 - kmn;
+        const uint32_t sh = span < 1024 ? 0 : 32 - __clz(span) - 10;
+        const u16x2 kmn2 = {(unsigned short)kmn, (unsigned short)kmn};
+        sweep([&](const uint4 v, uint32_t) {
+            const uint32_t c[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int i = 0; i < 12; i++) hist[12 * lane + i] = 0;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        sweep([&](const uint4 v, uint32_t u) {
-            const uint32_t v4[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int t = 0; t < 8; t++) {
-                const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
-                if (kk != 0xffffu && 8 * u + t < n) atomicAdd(&h1[(kk - kmn) >> sh], 1u);
+            for (int i = 0; i < 4; i++) {
+                const uint32_t d = as_u32(as_u16x2(c[i]) - kmn2);
+                if ((c[i] & 0xffffu) != 0xffffu) atomicAdd(&h1[(d & 0xffffu) >> sh], 1u);
+                if (c[i] < 0xffff0000u) atomicAdd(&h1[d >> (16 + sh)], 1u);
             }
         });
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        uint32_t bW = 0, belowW = 0, bA = 0, belowA = 0;
-        if (!all) wave_find_bin(h1, W, bW, belowW);
-        wave_find_bin(h1, nA, bA, belowA);
-        uint32_t b2W = 0, below2W = 0, b2A = 0, below2A = 0;
-        if (sh > 0) {
-            const uint32_t mask = (1u << sh) - 1u;
-            sweep([&](const uint4 v, uint32_t u) {
-                const uint32_t v4[4] = {v.x, v.y, v.z, v.w};
+        g.sync();
+        BinTab<16> b1;   // every wave finds the same bins
+        b1.load(h1);
+        nvalid = b1.tot;
+        W = min(want, nvalid);
+        all = W == nvalid;
+        // phase-A rank (a_bits): the `ca` smallest of the pool, same tie rule
+        const uint32_t nA = a_bits ? min(ca, W) : W;
+        allA = nA == nvalid;   // (all implies allA)
+        if (!allA) {
+            uint32_t bW = 0, belowW = 0, bA = 0, belowA = 0;
+            if (!all) b1.find(W, bW, belowW);
+            b1.find(nA, bA, belowA);
+            uint32_t b2W = 0, below2W = 0, b2A = 0, below2A = 0;
+            if (sh > 0) {   // at most 64 sub-bins each
+                const uint32_t mask = (1u << sh) - 1u;
+                sweep([&](const uint4 v, uint32_t) {
+                    const uint32_t c[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-                for (int t = 0; t < 8; t++) {
-                    const uint32_t kk = (v4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
-                    if (kk == 0xffffu || 8 * u + t >= n) continue;
-                    const uint32_t bin = (kk - kmn) >> sh;
-                    if (!all && bin == bW) atomicAdd(&h2[(kk - kmn) & mask], 1u);
-                    if (bin == bA) atomicAdd(&h3[(kk - kmn) & mask], 1u);
+                    for (int j = 0; j < 8; j++) {
+                        const uint32_t kk = (c[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+                        if (kk == 0xffffu) continue;
+                        const uint32_t bin = (kk - kmn) >> sh;
+                        if (!all && bin == bW) atomicAdd(&h2[(kk - kmn) & mask], 1u);
+                        if (bin == bA) atomicAdd(&h3[(kk - kmn) & mask], 1u);
+                    }
+                });
+                g.sync();
+                BinTab<1> b2, b3;
+                if (!all) {
+                    b2.load(h2);
+                    b2.find(W - belowW, b2W, below2W);
                 }
-            });
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            if (!all) wave_find_bin(h2, W - belowW, b2W, below2W);
-            wave_find_bin(h3, nA - belowA, b2A, below2A);
+                b3.load(h3);
+                b3.find(nA - belowA, b2A, below2A);
+            }
+            if (!all) {
+                T1 = kmn + (bW << sh) + b2W;
+                need = W - belowW - below2W;   // keys equal to T1 to take, in slot order
+            }
+            TA = kmn + (bA << sh) + b2A;
+            needA = nA - belowA - below2A;
         }
-        if (!all) {
-            T = kmn + (bW << sh) + b2W;
-            need = W - belowW - below2W;   // keys equal to T to take, in slot order
-        }
-        TA = kmn + (bA << sh) + b2A;
-        needA = nA - belowA - below2A;
     }
-    // collect: round m = words 64m .. 64m+63 in lane order (slot order)
+    if (t == 0) pool_want[q] = W;
+    // collect: round r = units r*T .. r*T + T-1 in thread order (slot order)
     uint32_t tie_run = 0, tieA_run = 0, mark_run = 0;
-    for (uint32_t m = 0; 64 * m < nwords; m++) {
-        const uint32_t w = lane + 64 * m;
-        uint4 r[4];
-        if (w < nwords) load_word_raw(kp, w, r);
+    auto collect = [&](const uint4 v, uint32_t u) {
         uint32_t live = 0, lt = 0, eq = 0, ltA = 0, eqA = 0;
-        if (w < nwords) word_masks2(r, w, n, T, TA, live, lt, eq, ltA, eqA);
-        // tie counts of this round (each at most 2048) packed in one scan
+        // TA <= T1, and T1 < 0xffff unless every live key is taken
+        if (u < nun && unit_min(v) <= T1) unit_masks2(v, u, n, T1, TA, live, lt, eq, ltA, eqA);
+        // tie counts of this round (each at most 8 * T) packed in one scan
         uint32_t tt;
-        const uint32_t pk = wave_excl((all ? 0u : __popc(eq)) | ((allA ? 0u : __popc(eqA)) << 16), &tt);
+        const uint32_t pk = g.excl((all ? 0u : __popc(eq)) | ((allA ? 0u : __popc(eqA)) << 16), &tt);
         const uint32_t tr = tie_run + (pk & 0xffffu), trA = tieA_run + (pk >> 16);
         tie_run += tt & 0xffffu;
         tieA_run += tt >> 16;
         const uint32_t sel = all ? live : lt | (tr < need ? lowest_bits(eq, need - tr) : 0u);
-        if (BITS) {   // one coalesced mask store per word
-            if (w < nwords) {
-                sel_bits[w] = sel;
-                if (a_bits)
-                    a_bits[w] = allA ? live : ltA | (trA < needA ? lowest_bits(eqA, needA - trA) : 0u);
+        if (BITS) {   // four neighbouring lanes hold one 32-slot word
+            const uint32_t sh8 = 8 * (lane & 3);
+            uint32_t ws = sel << sh8;
+            ws |= __shfl_xor(ws, 1, 64);
+            ws |= __shfl_xor(ws, 2, 64);
+            uint32_t wa = 0;
+            if (a_bits) {
+                wa = (allA ? live : ltA | (trA < needA ? lowest_bits(eqA, needA - trA) : 0u)) << sh8;
+                wa |= __shfl_xor(wa, 1, 64);
+                wa |= __shfl_xor(wa, 2, 64);
             }
-            continue;
+            if ((lane & 3) == 0 && u < nun) {
+                sel_bits[u / 4] = ws;
+                if (a_bits) a_bits[u / 4] = wa;
+            }
+            return;
         }
         uint32_t mt;
-        uint32_t pre = mark_run + wave_excl(__popc(sel), &mt);
+        uint32_t pre = mark_run + g.excl(__popc(sel), &mt);
         mark_run += mt;
         if (sel) {   // pool positions follow slot order
-            uint32_t word = sel;
-            uint32_t pj = map ? tab.find(32 * w + (__ffs(word) - 1)) : 0u;
-            while (word) {
-                const uint32_t j = __ffs(word) - 1;
-                word &= word - 1;
+            uint32_t bits = sel;
+            uint32_t pj = map ? tab.find(8 * u + (__ffs(bits) - 1)) : 0u;
+            while (bits) {
+                const uint32_t j = __ffs(bits) - 1;
+                bits &= bits - 1;
                 const size_t o = (size_t)q * want + pre++;
-                pool_entry[o] = map ? tab.entry_from(pj, 32 * w + j) : 32 * w + j;
-                if (pool_key) pool_key[o] = (uint16_t)word_key(r, j);
+                pool_entry[o] = map ? tab.entry_from(pj, 8 * u + j) : 8 * u + j;
+                if (pool_key) pool_key[o] = (uint16_t)unit_key(v, j);
             }
         }
+    };
+#pragma unroll
+    for (int i = 0; i < RU; i++)
+        if (T * i < nun) collect(reg[i], t + T * i);   // (uniform over the group)
+    for (uint32_t r0 = T * RU; r0 < nun; r0 += T) {
+        const uint32_t u = r0 + t;
+        uint4 v = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+        if (u < nun) {
+            v = kp4[u];
+            unit_tail(v, u, n);
+        }
+        collect(v, u);
     }
 }
 
 constexpr uint32_t kWaveTab = 256;   // probe tables held in LDS up to this n_probe
 
-template <bool BITS>
-__global__ __launch_bounds__(256, 8) void k_pool_select(const PoolParams p, uint32_t nq) {
-    __shared__ uint32_t hist[4][768];
-    __shared__ uint32_t s_pbase[BITS ? 1 : 4][BITS ? 1 : kWaveTab], s_ebase[BITS ? 1 : 4][BITS ? 1 : kWaveTab];
-    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t q = blockIdx.x * 4 + wv;
-    if (q >= nq) return;   // whole wave: nothing below synchronises the workgroup
+// NW wavefronts per query (4 / NW queries per 256-thread workgroup), RU
+// register-held units per thread.
+template <bool BITS, int NW, int RU>
+__global__ __launch_bounds__(256) void k_pool_select(const PoolParams p, uint32_t nq) {
+    constexpr int G = 4 / NW;
+    __shared__ uint32_t hist[G][1152];
+    __shared__ uint32_t sx[G][5 * NW];
+    __shared__ uint32_t s_pbase[BITS ? 1 : G][BITS ? 1 : kWaveTab], s_ebase[BITS ? 1 : G][BITS ? 1 : kWaveTab];
+    const uint32_t gi = threadIdx.x / (64 * NW);
+    const uint32_t t = PoolGroup<NW>::tid();
+    const uint32_t q = blockIdx.x * G + gi;
+    if (q >= nq) return;   // whole group: its barriers involve only its own threads
     const uint64_t b = p.seg_off ? p.seg_off[q] : (uint64_t)q * p.seg_stride;   // multiple of 32
     const uint32_t n = p.seg_cnt ? p.seg_cnt[q] : p.seg_len;
     ProbeTab tab;
@@ -649,19 +815,21 @@ __global__ __launch_bounds__(256, 8) void k_pool_select(const PoolParams p, uint
     tab.post_off = p.post_off;
     tab.seg = b;
     tab.pbase = tab.ebase = nullptr;
+    PoolGroup<NW> g;
+    g.sx = sx[gi];
     if (!BITS && tab.np && tab.np <= kWaveTab) {
-        for (uint32_t j = lane; j < tab.np; j += 64) {
-            s_pbase[wv][j] = (uint32_t)(tab.qp_row[j] - b);
-            s_ebase[wv][j] = p.post_off[tab.probe_row[j]];
+        for (uint32_t j = t; j < tab.np; j += 64 * NW) {
+            s_pbase[gi][j] = (uint32_t)(tab.qp_row[j] - b);
+            s_ebase[gi][j] = p.post_off[tab.probe_row[j]];
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        tab.pbase = s_pbase[wv];
-        tab.ebase = s_ebase[wv];
+        g.sync();
+        tab.pbase = s_pbase[gi];
+        tab.ebase = s_ebase[gi];
     }
-    pool_wave<BITS>(p.keys + b, q, n, p.want, p.pool_entry, p.pool_key, p.pool_want,
-                    !BITS && p.map, tab, hist[wv], BITS ? p.sel_bits + b / 32 : nullptr,
-                    BITS && p.a_bits ? p.a_bits + b / 32 : nullptr, p.ca);
+    pool_group<BITS, NW, RU>(p.keys + b, q, n, p.want, p.pool_entry, p.pool_key, p.pool_want,
+                             !BITS && p.map, tab, hist[gi], g,
+                             BITS ? p.sel_bits + b / 32 : nullptr,
+                             BITS && p.a_bits ? p.a_bits + b / 32 : nullptr, p.ca);
 }
 
 // ---------------------------------------------------------------- query order
@@ -1146,10 +1314,10 @@ hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) 
 
 hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s) {
     if (nq == 0) return hipSuccess;
-    if (p.sel_bits)
-        hipLaunchKernelGGL(k_pool_select<true>, dim3((nq + 3) / 4), dim3(256), 0, s, p, nq);
-    else
-        hipLaunchKernelGGL(k_pool_select<false>, dim3((nq + 3) / 4), dim3(256), 0, s, p, nq);
+    if (p.sel_bits)   // candidate pools: a workgroup per query, 24,576 keys in registers
+        hipLaunchKernelGGL((k_pool_select<true, 4, 12>), dim3(nq), dim3(256), 0, s, p, nq);
+    else              // probe pools: a wave per query, 4,096 keys in registers
+        hipLaunchKernelGGL((k_pool_select<false, 1, 8>), dim3((nq + 3) / 4), dim3(256), 0, s, p, nq);
     return hipGetLastError();
 }
 

@@ -161,3 +161,45 @@ def test_pruned_rerank_gathers_fewer_rows():
     print(f"rerank rows {pr['rerank_rows']} of {pr['pool_members']} pool members")
     assert pr["pool_members"] == 256 * 500
     assert 256 * 10 <= pr["rerank_rows"] < pr["pool_members"] // 2
+
+
+def integer_mixture(n, nq, d, comps, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0, 200, size=(comps, d))
+    x = np.clip(np.rint(c[rng.integers(0, comps, n)] + rng.normal(0, 20, (n, d))), 0, 255)
+    q = np.clip(np.rint(c[rng.integers(0, comps, nq)] + rng.normal(0, 20, (nq, d))), 0, 255)
+    return x.astype(np.float32), q.astype(np.float32)
+
+
+@pytest.mark.parametrize("pool", [1000, 2048])
+def test_mixed_long_candidate_lists(pool):
+    """Every query scans all 60,000 entries: above the 24,576 keys a pool
+    select workgroup holds in registers, so later passes re-read the rest.
+    Integer data: mixed = exact."""
+    x, q = integer_mixture(60000, 48, 32, 12, seed=21)
+    ix = build_index(x, "l2", None, n_postings=16, seed=0)
+    dev = DeviceIndex(ix)
+    print(f"[case] long lists pool={pool}", flush=True)
+    e = dev.search(q, 100, 16, precision=PREC_EXACT)
+    m = dev.search(q, 100, 16, precision=PREC_MIXED, rerank_size=pool)
+    f = dev.search(q, 100, 16, precision=PREC_MIXED, rerank_size=pool, flags=_lib.FLAG_FULL_RERANK)
+    for a in (m, f):
+        np.testing.assert_array_equal(a[2], e[2])
+        np.testing.assert_array_equal(a[0], e[0])
+        np.testing.assert_array_equal(a[1].view(np.uint32), e[1].view(np.uint32))
+
+
+def test_mixed_probe_above_4096_postings():
+    """The matrix-core probe over 5,000 postings (probe pools above the 4,096
+    keys a wave holds in registers) with its exact recheck: the probe list,
+    and so the result, equals the exact path's on integer data."""
+    x, q = integer_mixture(40000, 64, 32, 64, seed=22)
+    ix = build_index(x, "l2", None, n_postings=5000, seed=0, kmeans_iters=2)
+    dev = DeviceIndex(ix)
+    print("[case] mixed probe P=5000", flush=True)
+    for n_probe in (8, 64):
+        e = dev.search(q, 10, n_probe, precision=PREC_EXACT)
+        m = dev.search(q, 10, n_probe, precision=PREC_MIXED, rerank_size=2048)
+        np.testing.assert_array_equal(m[2], e[2])
+        np.testing.assert_array_equal(m[0], e[0])
+        np.testing.assert_array_equal(m[1].view(np.uint32), e[1].view(np.uint32))
