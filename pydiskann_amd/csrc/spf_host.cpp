@@ -911,8 +911,9 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
         add(pairs * 4 * (ix->rq_padded / 64), 8);
         add(pairs * 8, 4);
     }
-    // (+ up to 7 alignment slots per probe for the matrix-core scan)
-    const uint64_t total = q * ix->top_len[std::min<uint64_t>(npr, P)] + (z.mfma_scan ? 7 * pairs : 0);
+    // (+ up to kScanAlign - 1 alignment slots per probe for the matrix-core scan)
+    const uint64_t total = q * ix->top_len[std::min<uint64_t>(npr, P)] +
+                           (z.mfma_scan ? (uint64_t)(kScanAlign - 1) * pairs : 0);
     if (z.mixed) {
         if (candidates) {
             add(std::max<uint64_t>(total, 1), 2);       // cand16
@@ -1250,7 +1251,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.pair_src = ix->rq_l2 ? w->pair_src.p : nullptr;
     pl.G = G;
     pl.echunk = echunk;
-    pl.align = z.mfma_scan ? 8u : 1u;   // the matrix-core scan stores aligned 16-byte rows
+    pl.align = z.mfma_scan ? kScanAlign : 1u;   // the matrix-core scan stores aligned rows
     if (mixed) {   // the rerank's query-order counters, zeroed by the planner
         HIPCHK(w->perm_cnt.ensure(P + 2));
         HIPCHK(w->perm_fill.ensure(P + 2));

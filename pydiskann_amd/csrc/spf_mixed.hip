@@ -79,8 +79,10 @@ hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t
 // (f16 codes) are loaded once into registers, then reused by every 32-query
 // group of the item (two 16-row A tiles per group, 16x16x32 f16 MFMA, f32
 // accumulate).  Each (query, entry) writes the 16-bit key of its distance.
-template <int KS>
-__global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
+// Four workgroups per CU for the unfiltered scan up to 128 dims (the label
+// masks, or 256-dim query fragments, would spill there)
+template <int KS, bool FILT>
+__global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(const MixedParams p) {
     constexpr int T = KS <= 4 ? 4 : 2;   // tiles per wave, mfma_tiles()
     constexpr uint32_t EW = 64 * T;      // entries per item
     __shared__ __align__(16) uint16_t s_tile[32][EW];   // one query group's keys
@@ -111,82 +113,96 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const MixedParams p) {
             if (p.metric == M_DOT) qn = 1.0f;
             else if (p.metric == M_COS) qn = qn > 0.0f ? 1.0f / sqrtf(qn) : 0.0f;
             s_qn[tid] = qn;
-            if (p.qmask) s_mask[tid] = ok ? p.qmask[q] : 0ull;
+            if (FILT) s_mask[tid] = ok ? p.qmask[q] : 0ull;
         }
-        // the wave's B fragments, and per-entry norm / tombstone of its column
+        // the wave's B fragments, and per-entry norm / tombstone word / label
+        // of its column: every load of every tile is issued before any value
+        // is used (entries past the posting read entry e_begin and are masked)
         uint4 bf[T][KS];
         float xn[T], rx[T];
         bool live[T];
-        uint32_t lab[T];
+        uint32_t lab[T], dw[T];
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            const uint32_t e = e_begin + 16 * (wave + 4 * t) + col;
-            const bool valid = e < e_end;
+            const uint32_t e_raw = e_begin + 16 * (wave + 4 * t) + col;
+            const uint32_t e = e_raw < e_end ? e_raw : e_begin;
             const uint4 *src = p.codes16 + (size_t)(e >> 6) * p.units * 64 + (e & 63);
 #pragma unroll
             for (int s = 0; s < KS; s++) {
                 const uint32_t u = 4 * s + kg;
-                bf[t][s] = (valid && u < p.units) ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
+                bf[t][s] = u < p.units ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
             }
-            xn[t] = valid ? p.xn16[e] : 0.0f;
-            live[t] = valid && !(p.del && ((p.del[e >> 5] >> (e & 31)) & 1u));
-            lab[t] = (valid && p.qmask) ? p.label_e[e] : 0u;
+            xn[t] = p.xn16[e];
+            dw[t] = p.del ? p.del[e >> 5] : 0u;
+            lab[t] = FILT ? p.label_e[e] : 0u;
+        }
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const uint32_t e = e_begin + 16 * (wave + 4 * t) + col;
+            live[t] = e < e_end && !((dw[t] >> (e & 31)) & 1u);   // (a column's products
+            // depend on its own B values only, so the stand-in entry's are harmless)
             rx[t] = p.metric == M_COS ? (xn[t] > 0.0f ? 1.0f / sqrtf(xn[t]) : 0.0f) : 1.0f;
         }
         __syncthreads();
         for (uint32_t g0 = 0; g0 < gcnt; g0 += 32) {
-            // A fragments: 32 queries (two 16-row tiles) x dims
-            half8_t a[2][KS];
+            // two 16-query halves, one after the other (fewer live registers)
+            for (uint32_t h = 0; h < 2; h++) {
+                const uint32_t gb = g0 + 16 * h;
+                if (gb >= gcnt) break;   // (uniform; the half's tile rows are not stored)
+                // A fragments: 16 queries x dims
+                half8_t a[KS];
+                {
+                    const uint32_t g = gb + col;
+                    const bool ok = g < gcnt;
+                    const uint4 *qrow = p.q16 + (size_t)s_qi[ok ? g : 0] * p.q16_units;
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const uint32_t g = g0 + 16 * h + col;
-                const bool ok = g < gcnt;
-                const uint4 *qrow = p.q16 + (size_t)s_qi[ok ? g : 0] * p.q16_units;
-#pragma unroll
-                for (int s = 0; s < KS; s++) {
-                    uint4 v = make_uint4(0, 0, 0, 0);
-                    if (ok && s < (int)p.ks) v = qrow[4 * s + kg];   // rows hold 4*ks units
-                    a[h][s] = as_half8(v);
+                    for (int s = 0; s < KS; s++) {
+                        uint4 v = make_uint4(0, 0, 0, 0);
+                        if (ok && s < (int)p.ks) v = qrow[4 * s + kg];   // rows hold 4*ks units
+                        a[s] = as_half8(v);
+                    }
                 }
-            }
-#pragma unroll
-            for (int t = 0; t < T; t++) {
-                const uint32_t e0 = e_begin + 16 * (wave + 4 * t);
-                if (e0 >= e_end) break;   // later tiles are further out
-                floatx4_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-#pragma unroll
-                for (int s = 0; s < KS; s++) {
-                    const half8_t b = as_half8(bf[t][s]);
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b, acc1, 0, 0, 0);
-                }
-                // keys into the group tile (beyond e_end: "no key", the
-                // aligned range's tail)
-                const uint32_t e = e0 + col, el = e - e_begin;
-                const bool in = e < e_end;
+                // the lane's four rows 4*kg + j of the half: norm factor and
+                // label mask, read once (gb + 15 < kMixQ)
+                float qv[4];
+                uint64_t mv[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const uint32_t r0 = g0 + 4 * kg + j, r1 = r0 + 16;
-                    if (r0 < gcnt) {
-                        const float d = l2 ? xn[t] + s_qn[r0] - 2.0f * acc0[j]
-                                           : 1.0f - acc0[j] * s_qn[r0] * rx[t];
-                        const bool ok = in && live[t] && (!p.qmask || label_ok(s_mask[r0], lab[t]));
-                        s_tile[r0 - g0][el] = ok ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
-                    }
-                    if (r1 < gcnt) {
-                        const float d = l2 ? xn[t] + s_qn[r1] - 2.0f * acc1[j]
-                                           : 1.0f - acc1[j] * s_qn[r1] * rx[t];
-                        const bool ok = in && live[t] && (!p.qmask || label_ok(s_mask[r1], lab[t]));
-                        s_tile[r1 - g0][el] = ok ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
+                    qv[j] = s_qn[gb + 4 * kg + j];
+                    mv[j] = FILT ? s_mask[gb + 4 * kg + j] : 0ull;
+                }
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    const uint32_t e0 = e_begin + 16 * (wave + 4 * t);
+                    if (e0 >= e_end) break;   // later tiles are further out
+                    floatx4_t acc = {0, 0, 0, 0};
+#pragma unroll
+                    for (int s = 0; s < KS; s++)
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[s], as_half8(bf[t][s]), acc,
+                                                                     0, 0, 0);
+                    // keys into the group tile ("no key" for dead, filtered or
+                    // past-the-posting entries: the aligned range's tail).  Rows
+                    // past the group's queries are written too and never stored.
+                    const uint32_t el = e0 + col - e_begin;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const float d = l2 ? xn[t] + qv[j] - 2.0f * acc[j] : 1.0f - acc[j] * qv[j] * rx[t];
+                        const bool ok = live[t] && (!FILT || label_ok(mv[j], lab[t]));
+                        s_tile[16 * h + 4 * kg + j][el] = ok ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
                     }
                 }
             }
+            // the tiles written above end at the 16-entry multiple past the
+            // posting's end; "no key" from there to the reserved end
+            const uint32_t len = e_end - e_begin, l16 = (len + 15) & ~15u;
+            const uint32_t lp = e_end == pe ? (len + kScanAlign - 1) / kScanAlign * kScanAlign : len;
+            if (lp > l16)
+                for (uint32_t c = tid; c < 32 * (lp - l16); c += 256)
+                    s_tile[c / (lp - l16)][l16 + c % (lp - l16)] = 0xffff;
             __syncthreads();
             // the tile's rows as aligned 16-byte stores: each probe's slot
-            // range starts at a multiple of 8 and is reserved up to the next
-            // multiple of 8 past the posting's end
-            const uint32_t len = e_end - e_begin;
-            const uint32_t lp = e_end == pe ? (len + 7) & ~7u : len;
+            // range starts at a multiple of kScanAlign and is reserved up to
+            // the next multiple past the posting's end
             const uint32_t cpr = lp / 8, rows = min(32u, gcnt - g0);
             for (uint32_t c = tid; c < rows * cpr; c += 256) {
                 const uint32_t r = c / cpr, ch = c - r * cpr;
@@ -1300,13 +1316,18 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
 // ---------------------------------------------------------------- launchers
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) {
     if (grid == 0) return hipSuccess;
+    const bool f = p.qmask != nullptr;   // label filter
     switch (p.ks) {   // 32-dim K steps: dim <= 32, 64, 96, 128, 256
-    case 1: hipLaunchKernelGGL(k_scan_mfma<1>, dim3(grid), dim3(256), 0, s, p); break;
-    case 2: hipLaunchKernelGGL(k_scan_mfma<2>, dim3(grid), dim3(256), 0, s, p); break;
-    case 3: hipLaunchKernelGGL(k_scan_mfma<3>, dim3(grid), dim3(256), 0, s, p); break;
-    case 4: hipLaunchKernelGGL(k_scan_mfma<4>, dim3(grid), dim3(256), 0, s, p); break;
-    case 5: case 6: case 7: case 8:
-        hipLaunchKernelGGL(k_scan_mfma<8>, dim3(grid), dim3(256), 0, s, p); break;
+#define SPF_SCAN_MFMA(KS_)                                                                    \
+        if (f) hipLaunchKernelGGL((k_scan_mfma<KS_, true>), dim3(grid), dim3(256), 0, s, p);  \
+        else hipLaunchKernelGGL((k_scan_mfma<KS_, false>), dim3(grid), dim3(256), 0, s, p);   \
+        break;
+    case 1: SPF_SCAN_MFMA(1)
+    case 2: SPF_SCAN_MFMA(2)
+    case 3: SPF_SCAN_MFMA(3)
+    case 4: SPF_SCAN_MFMA(4)
+    case 5: case 6: case 7: case 8: SPF_SCAN_MFMA(8)
+#undef SPF_SCAN_MFMA
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
