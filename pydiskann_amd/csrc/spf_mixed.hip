@@ -226,17 +226,23 @@ __device__ __forceinline__ bf16x8v as_bf8(uint4 v) {
     return c.h;
 }
 
+// Each wave walks its 16-centroid tiles with the next tile's loads in flight;
+// keys collect in an LDS tile and leave as 16-byte row segments, so the walk
+// itself issues no global stores.  Rows are reserved up to a multiple of 32
+// keys: the last segment's keys past the centroids read "no key".
+template <int KS>
 __global__ __launch_bounds__(256) void k_probe_mfma(const ProbeMfmaParams p) {
+    __shared__ __align__(16) uint16_t s_out[16][kEChunk];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 15, kg = lane >> 4;
     const uint32_t nch = (p.n_post + kEChunk - 1) / kEChunk;
     const uint32_t qt = blockIdx.x / nch, ch = blockIdx.x - qt * nch;
     const uint32_t q0 = qt * 16;
     const uint32_t c_begin = ch * kEChunk, c_end = min(c_begin + kEChunk, p.n_post);
-    bf16x8v a[kMixKS];
+    bf16x8v a[KS];
     const uint32_t qa = q0 + col;
 #pragma unroll
-    for (int s = 0; s < kMixKS; s++) {
+    for (int s = 0; s < KS; s++) {
         uint4 v = make_uint4(0, 0, 0, 0);
         if (qa < p.nq && s < (int)p.ks) v = p.qbf[(size_t)qa * p.q_units + 4 * s + kg];
         a[s] = as_bf8(v);
@@ -247,31 +253,50 @@ __global__ __launch_bounds__(256) void k_probe_mfma(const ProbeMfmaParams p) {
         const uint32_t q = q0 + 4 * kg + j;
         qn[j] = q < p.nq ? p.qnbf[q] : 0.0f;
     }
-    for (uint32_t c0 = c_begin + 16 * wave; c0 < c_end; c0 += 64) {
-        const uint32_t c = c0 + col;
-        const bool valid = c < c_end;
+    // a tile's B fragment and the norm of the lane's centroid (columns past
+    // the end read the last centroid; their keys are never stored)
+    auto load_b = [&](uint32_t c0, uint4 (&b)[KS], float &cn) {
+        const uint32_t c = min(c0 + col, c_end - 1);
         const uint4 *src = p.cbf + (size_t)(c >> 6) * p.c_units * 64 + (c & 63);
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const uint32_t u = 4 * s + kg;
+            b[s] = (s < (int)p.ks && u < p.c_units) ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
+        }
+        cn = p.cnbf[c];
+    };
+    auto body = [&](uint32_t c0, const uint4 (&b)[KS], float cn) {
         floatx4_t acc = {0, 0, 0, 0};
 #pragma unroll
-        for (int s = 0; s < kMixKS; s++) {
-            if (s < (int)p.ks) {
-                const uint32_t u = 4 * s + kg;
-                uint4 v = make_uint4(0, 0, 0, 0);
-                if (valid && u < p.c_units) v = src[(size_t)u * 64];
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], as_bf8(v), acc, 0, 0, 0);
-            }
-        }
-        if (valid) {
-            const float cn = p.cnbf[c];
+        for (int s = 0; s < KS; s++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], as_bf8(b[s]), acc, 0, 0, 0);
+        const uint32_t cl = c0 + col - c_begin;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t q = q0 + 4 * kg + j;
-                if (q < p.nq) {
-                    const float d = cn + qn[j] - 2.0f * acc[j];
-                    p.out[(size_t)q * p.out_stride + c] = (uint16_t)(fkey_m(d) >> 16);
-                }
-            }
-        }
+        for (int j = 0; j < 4; j++)
+            s_out[4 * kg + j][cl] = (uint16_t)(fkey_m(cn + qn[j] - 2.0f * acc[j]) >> 16);
+    };
+    uint4 b0[KS], b1[KS];
+    float cn0 = 0.0f, cn1 = 0.0f;
+    const uint32_t start = c_begin + 16 * wave;
+    if (start < c_end) load_b(start, b0, cn0);
+    for (uint32_t c0 = start; c0 < c_end; c0 += 128) {
+        if (c0 + 64 < c_end) load_b(c0 + 64, b1, cn1);
+        body(c0, b0, cn0);
+        if (c0 + 64 >= c_end) break;
+        if (c0 + 128 < c_end) load_b(c0 + 128, b0, cn0);
+        body(c0 + 64, b1, cn1);
+    }
+    // keys past the last centroid up to the next multiple of 8
+    const uint32_t n = c_end - c_begin, n8 = (n + 7) & ~7u;
+    for (uint32_t i = tid; i < 16 * (n8 - n); i += 256)
+        s_out[i / (n8 - n)][n + i % (n8 - n)] = 0xffff;
+    __syncthreads();
+    const uint32_t cpr = n8 / 8;
+    for (uint32_t i = tid; i < 16 * cpr; i += 256) {
+        const uint32_t r = i / cpr, c8 = i - r * cpr;
+        if (q0 + r < p.nq)
+            *reinterpret_cast<uint4 *>(p.out + (size_t)(q0 + r) * p.out_stride + c_begin + 8 * c8) =
+                *reinterpret_cast<const uint4 *>(&s_out[r][8 * c8]);
     }
 }
 
@@ -1364,7 +1389,15 @@ hipError_t launch_probe_mfma(const ProbeMfmaParams &p, hipStream_t s) {
     const uint32_t nch = (p.n_post + kEChunk - 1) / kEChunk;
     const uint32_t grid = ((p.nq + 15) / 16) * nch;
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_probe_mfma, dim3(grid), dim3(256), 0, s, p);
+    switch (p.ks) {   // 32-dim K steps
+    case 1: hipLaunchKernelGGL(k_probe_mfma<1>, dim3(grid), dim3(256), 0, s, p); break;
+    case 2: hipLaunchKernelGGL(k_probe_mfma<2>, dim3(grid), dim3(256), 0, s, p); break;
+    case 3: hipLaunchKernelGGL(k_probe_mfma<3>, dim3(grid), dim3(256), 0, s, p); break;
+    case 4: hipLaunchKernelGGL(k_probe_mfma<4>, dim3(grid), dim3(256), 0, s, p); break;
+    case 5: case 6: case 7: case 8:
+        hipLaunchKernelGGL(k_probe_mfma<kMixKS>, dim3(grid), dim3(256), 0, s, p); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
