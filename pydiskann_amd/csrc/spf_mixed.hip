@@ -985,7 +985,7 @@ __device__ __forceinline__ void bitonic_lds(uint64_t *buf, uint32_t m) {
         }
 }
 
-constexpr int kRerankH = 4;   // pairs per quad in flight
+constexpr int kRerankH = 2, kRerankUC = 8;   // rows per quad, row units per lane in flight
 
 __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
     __shared__ uint64_t s_keys[kMaxK];
@@ -1012,7 +1012,8 @@ __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
             xr[h] = reinterpret_cast<const float4 *>(p.raw_e + (size_t)ee[h] * p.dpad);
         }
         float d[kRerankH];
-        quad_exact_h<kRerankH, 1>(p.metric, reinterpret_cast<const float4 *>(s_qv), xr, units, r, qn, d);
+        quad_exact_h<kRerankH, kRerankUC>(p.metric, reinterpret_cast<const float4 *>(s_qv), xr, units, r,
+                                          qn, d);
         if (r == 0) {
 #pragma unroll
             for (int h = 0; h < kRerankH; h++)
@@ -1023,14 +1024,24 @@ __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
                 }
         }
     }
+    const uint32_t kk = min(p.k, c);
+    if (tid == 0) p.out_cnt[q] = kk;
+    if (c <= 512) {   // short pools: rank by counting (keys are distinct), scatter
+        __syncthreads();
+        for (uint32_t i = tid; i < c; i += 256) {
+            const uint64_t v = s_keys[i];
+            uint32_t rk = 0;
+            for (uint32_t j = 0; j < c; j++) rk += s_keys[j] < v;
+            if (rk < kk) p.out[(size_t)q * p.k + rk] = v;
+        }
+        return;
+    }
     uint32_t m = 2;
     while (m < c) m <<= 1;
     for (uint32_t i = c + tid; i < m; i += 256) s_keys[i] = kNoKey;
     __syncthreads();
     bitonic_lds(s_keys, m);
-    const uint32_t kk = min(p.k, c);
     for (uint32_t i = tid; i < kk; i += 256) p.out[(size_t)q * p.k + i] = s_keys[i];
-    if (tid == 0) p.out_cnt[q] = kk;
 }
 
 // ---------------------------------------------------------------- pruned rerank
