@@ -85,7 +85,7 @@ struct DBuf {
     X(uint64_t, rq_planes) X(float, rq_consts) X(uint32_t, pair_src) X(float, eq16)        \
     X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)        \
     X(uint32_t, gflag) X(uint32_t, gscr) X(uint32_t, sel_ccnt) X(uint32_t, sel_coff)            \
-    X(uint64_t, sel_tmp) X(uint64_t, probe_tmp) X(uint32_t, cand_real)
+    X(uint64_t, sel_tmp) X(uint64_t, probe_tmp) X(uint32_t, cand_real) X(uint32_t, blk_cnt)
 
 struct Workspace {
     hipStream_t stream = nullptr;
@@ -890,6 +890,7 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     add(q, 4);                   // cand_cnt
     add(q + 1, 8);               // cand_off
     add(P + 1, 4);               // post_cnt
+    if (P <= kRankMaxPost) add((q + kRankQB - 1) / kRankQB * P, 4);   // blk_cnt
     add(q, 4);                   // cand_real (profiled calls)
     add(pairs, 4);               // qp_rel
     add(pairs, 4);               // prank
@@ -1262,7 +1263,13 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         HIPCHK(w->cand_real.ensure(nq));
         pl.cand_real = w->cand_real.p;
     }
+    if (P <= kRankMaxPost) {   // pair ranks by query block (else device-wide atomics)
+        HIPCHK(w->blk_cnt.ensure((size_t)(nq + kRankQB - 1) / kRankQB * P));
+        pl.qb = kRankQB;
+        pl.blk_cnt = w->blk_cnt.p;
+    }
     HIPCHK(launch_plan_q(pl, s));
+    HIPCHK(launch_pair_rank(pl, s));
     HIPCHK(launch_excl_scan_u64(w->cand_len.p, w->cand_off.p, nq, s));
     HIPCHK(launch_excl_scan_u32(w->post_cnt.p, w->pair_off.p, P, s));
     HIPCHK(launch_scatter_pairs(pl, s));

@@ -326,7 +326,14 @@ struct PlanParams {
     uint32_t align;             // each probe's candidate slots start at a multiple of this
     uint32_t *cand_real;        // [nq] candidates without alignment slots (null = skip)
     uint32_t *zero_a, *zero_b;  // [n_post + 2] each, zeroed by k_items (null = none)
+    // pair ranks without global atomics: queries in blocks of qb, per-block
+    // posting counts in LDS, then exclusive over blocks (0 = global atomics)
+    uint32_t qb;
+    uint32_t *blk_cnt;          // [ceil(nq / qb)][n_post]
 };
+
+constexpr uint32_t kRankQB = 256;              // queries per ranking block
+constexpr uint32_t kRankMaxPost = 16384;       // LDS counters per ranking block
 
 struct RerankParams {
     const uint64_t *sel;        // [nq][want] sorted pool keys
@@ -368,6 +375,7 @@ hipError_t launch_scatter_pairs(const PlanParams &p, hipStream_t s);
 hipError_t launch_items(const PlanParams &p, hipStream_t s);
 hipError_t launch_item_table(const PlanParams &p, hipStream_t s);
 hipError_t launch_excl_scan_u64(const uint64_t *in, uint64_t *out, uint32_t n, hipStream_t s);
+hipError_t launch_pair_rank(const PlanParams &p, hipStream_t s);
 hipError_t launch_excl_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t s);
 hipError_t launch_rerank(const RerankParams &p, hipStream_t s);
 hipError_t launch_graph_probe(const GraphParams &p, uint32_t n, hipStream_t s);

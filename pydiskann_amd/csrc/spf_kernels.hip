@@ -640,7 +640,7 @@ __global__ __launch_bounds__(256) void k_plan_q(const PlanParams p) {
             len = (p.post_len[post] + p.align - 1) / p.align * p.align;   // slots reserved
             const size_t i = (size_t)q * p.npr + j;
             p.probe_post[i] = post;
-            p.prank[i] = atomicAdd(&p.post_cnt[post], 1u);
+            if (!p.qb) p.prank[i] = atomicAdd(&p.post_cnt[post], 1u);
         }
         uint32_t incl = len;
 #pragma unroll
@@ -669,11 +669,57 @@ __global__ void k_scatter_pairs(const PlanParams p) {
     if (j >= p.np_q[q]) return;
     const uint32_t post = p.probe_post[i];
     const uint64_t slot = p.cand_off[q] + p.qp_rel[i];
-    const uint32_t pos = p.pair_off[post] + p.prank[i];
+    const uint32_t pos = p.pair_off[post] + p.prank[i] +
+                         (p.qb ? p.blk_cnt[(size_t)(q / p.qb) * p.n_post + post] : 0u);
     p.qp_slot[i] = slot;
     p.pair_q[pos] = q;
     p.pair_slot[pos] = slot;
     if (p.pair_src) p.pair_src[pos] = i;
+}
+
+// Pair ranks within their posting, by query block: the block's pairs take
+// ranks from LDS counters (one per posting), and the block's counts go to
+// blk_cnt[block][posting].  Hot postings cost LDS atomics, not serialised
+// device-wide atomics.
+__global__ __launch_bounds__(1024) void k_pair_rank(const PlanParams p) {
+    extern __shared__ uint32_t s_h[];   // n_post counters
+    const uint32_t tid = threadIdx.x, b = blockIdx.x;
+    for (uint32_t i = tid; i < p.n_post; i += 1024) s_h[i] = 0u;
+    __syncthreads();
+    const uint32_t q0 = b * p.qb, q1 = min(q0 + p.qb, p.nq);
+    for (uint32_t i = q0 * p.npr + tid; i < q1 * p.npr; i += 1024) {
+        const uint32_t q = i / p.npr, j = i - q * p.npr;
+        if (j < p.np_q[q]) p.prank[i] = atomicAdd(&s_h[p.probe_post[i]], 1u);
+    }
+    __syncthreads();
+    uint32_t *row = p.blk_cnt + (size_t)b * p.n_post;
+    for (uint32_t i = tid; i < p.n_post; i += 1024) row[i] = s_h[i];
+}
+
+// Per posting (one thread each): blk_cnt's column becomes exclusive over the
+// blocks and post_cnt its total.
+__global__ void k_blk_scan(const PlanParams p, uint32_t nblk) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= p.n_post) return;
+    uint32_t *col = p.blk_cnt + b;
+    const size_t st = p.n_post;
+    uint32_t run = 0, k = 0;
+    for (; k + 8 <= nblk; k += 8) {   // eight loads in flight
+        uint32_t v[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) v[i] = col[(k + i) * st];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            col[(k + i) * st] = run;
+            run += v[i];
+        }
+    }
+    for (; k < nblk; k++) {
+        const uint32_t v = col[k * st];
+        col[k * st] = run;
+        run += v;
+    }
+    p.post_cnt[b] = run;
 }
 
 __global__ void k_items(const PlanParams p) {
@@ -705,39 +751,45 @@ __global__ void k_item_table(const PlanParams p) {
     p.item_post[i] = lo;
 }
 
-template <typename T>
+// Exclusive scan by one 1024-thread workgroup: thread t takes E consecutive
+// values per pass (all loads in flight), so up to 1024 * E values take one
+// pass; out[n] = total.
+template <typename T, int E>
 __global__ __launch_bounds__(1024) void k_excl_scan(const T *in, T *out, uint32_t n) {
     __shared__ T s_w[16];
-    __shared__ T s_carry;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s_carry = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < n; base += 1024) {
-        const uint32_t i = base + tid;
-        const T v = i < n ? in[i] : (T)0;
-        T x = v;
+    T carry = 0;
+    for (uint32_t base = 0; base < n; base += 1024 * E) {
+        const uint32_t i0 = base + tid * E;
+        T v[E], sum = 0;
+#pragma unroll
+        for (int e = 0; e < E; e++) v[e] = i0 + e < n ? in[i0 + e] : (T)0;
+#pragma unroll
+        for (int e = 0; e < E; e++) sum += v[e];
+        T x = sum;
         for (uint32_t off = 1; off < 64; off <<= 1) {
             const T y = __shfl_up(x, off, 64);
             if (lane >= off) x += y;
         }
         if (lane == 63) s_w[wave] = x;
         __syncthreads();
-        if (tid == 0) {
-            T c = 0;
-            for (int w = 0; w < 16; w++) {
-                const T t = s_w[w];
-                s_w[w] = c;
-                c += t;
-            }
+        T before = 0, total = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < 16; w++) {
+            const T t = s_w[w];
+            before += w < wave ? t : (T)0;
+            total += t;
         }
-        __syncthreads();
-        const T carry = s_carry;
-        if (i < n) out[i] = carry + s_w[wave] + x - v;
-        __syncthreads();
-        if (tid == 1023) s_carry = carry + s_w[wave] + x;
-        __syncthreads();
+        T run = carry + before + x - sum;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            if (i0 + e < n) out[i0 + e] = run;
+            run += v[e];
+        }
+        carry += total;
+        __syncthreads();   // s_w is rewritten by the next pass
     }
-    if (tid == 0) out[n] = s_carry;
+    if (tid == 0) out[n] = carry;
 }
 
 // ---------------------------------------------------------------- rerank
@@ -997,13 +1049,23 @@ hipError_t launch_item_table(const PlanParams &p, hipStream_t s) {
 }
 
 hipError_t launch_excl_scan_u64(const uint64_t *in, uint64_t *out, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_excl_scan<unsigned long long>, dim3(1), dim3(1024), 0, s,
+    hipLaunchKernelGGL((k_excl_scan<unsigned long long, 16>), dim3(1), dim3(1024), 0, s,
                        (const unsigned long long *)in, (unsigned long long *)out, n);
     return hipGetLastError();
 }
 
 hipError_t launch_excl_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_excl_scan<uint32_t>, dim3(1), dim3(1024), 0, s, in, out, n);
+    hipLaunchKernelGGL((k_excl_scan<uint32_t, 16>), dim3(1), dim3(1024), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+// k_pair_rank + k_blk_scan (p.qb > 0; k_plan_q has written probe_post / np_q)
+hipError_t launch_pair_rank(const PlanParams &p, hipStream_t s) {
+    if (!p.qb || p.nq == 0) return hipSuccess;
+    if (p.n_post > kRankMaxPost) return hipErrorInvalidValue;
+    const uint32_t nblk = (p.nq + p.qb - 1) / p.qb;
+    hipLaunchKernelGGL(k_pair_rank, dim3(nblk), dim3(1024), p.n_post * sizeof(uint32_t), s, p);
+    hipLaunchKernelGGL(k_blk_scan, dim3((p.n_post + 255) / 256), dim3(256), 0, s, p, nblk);
     return hipGetLastError();
 }
 
