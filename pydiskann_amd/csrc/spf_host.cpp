@@ -85,7 +85,8 @@ struct DBuf {
     X(uint64_t, rq_planes) X(float, rq_consts) X(uint32_t, pair_src) X(float, eq16)        \
     X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)        \
     X(uint32_t, gflag) X(uint32_t, gscr) X(uint32_t, sel_ccnt) X(uint32_t, sel_coff)            \
-    X(uint64_t, sel_tmp) X(uint64_t, probe_tmp) X(uint32_t, cand_real) X(uint32_t, blk_cnt)
+    X(uint64_t, sel_tmp) X(uint64_t, probe_tmp) X(uint32_t, cand_real) X(uint32_t, blk_cnt) \
+    X(uint32_t, a_min)
 
 struct Workspace {
     hipStream_t stream = nullptr;
@@ -709,6 +710,8 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     pp.pool_want = w->pool_want.p;
     pp.sel_bits = w->sel_bits.p;
     pp.a_bits = w->a_bits.p;
+    HIPCHK(w->a_min.ensure(nq));
+    pp.a_min = w->a_min.p;
     // phase-A size of the rerank: k plus a margin for the 16-bit key resolution
     pp.ca = std::min(want, std::max(k, round_up(k + k / 8 + 8, 32)));
     HIPCHK(launch_pool_select(pp, nq, s));
@@ -722,6 +725,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     rp.seg_cnt = w->cand_cnt.p;
     rp.sel_bits = w->sel_bits.p;
     rp.a_bits = w->a_bits.p;
+    rp.a_min = w->a_min.p;
     rp.pool_want = w->pool_want.p;
     rp.qp_slot = w->qp_slot.p;
     rp.post_off = ix->d_post_off;
@@ -922,6 +926,7 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
             add(std::max<uint64_t>(total / 32, 1), 4);  // a_bits
         }
         add(q, 4);                                  // pool_want
+        add(q, 4);                                  // a_min
         add(q * k, 8);                              // fin
         add(q, 4);                                  // fin_cnt
         add(P + 2, 4 * 3);                          // perm_cnt, perm_off, perm_fill

@@ -197,6 +197,7 @@ struct RerankPrunedParams {
     const uint32_t *seg_cnt;    // [nq]
     const uint32_t *sel_bits;   // pool membership, one mask per 32-slot word
     const uint32_t *a_bits;     // members of phase A (the `ca` smallest keys)
+    const uint32_t *a_min;      // [nq] lower bound of the other members' keys (null = none)
     const uint32_t *pool_want;  // [nq] pool size
     const uint64_t *qp_slot;    // [nq][npr] absolute first slot of each probed posting
     const uint32_t *post_off;
@@ -281,6 +282,8 @@ struct PoolParams {
     uint32_t *sel_bits;         // non-null: write per-word selection masks instead
     uint32_t *a_bits;           // with sel_bits: masks of the `ca` smallest pool members
     uint32_t ca;
+    uint32_t *a_min;            // with a_bits, optional [nq]: a lower bound of the keys of
+                                // pool members outside phase A (0xffff: there are none)
 };
 
 // Exact f32 rerank of each query's pool and the final top-k, one workgroup per

@@ -666,7 +666,8 @@ __device__ __forceinline__ void pool_group(const uint16_t *kp, uint32_t q, uint3
                                            uint32_t want, uint32_t *pool_entry,
                                            uint16_t *pool_key, uint32_t *pool_want, bool map,
                                            const ProbeTab &tab, uint32_t *hist, PoolGroup<NW> &g,
-                                           uint32_t *sel_bits, uint32_t *a_bits, uint32_t ca) {
+                                           uint32_t *sel_bits, uint32_t *a_bits, uint32_t ca,
+                                           uint32_t *a_min) {
     constexpr uint32_t T = PoolGroup<NW>::T;
     const uint32_t t = PoolGroup<NW>::tid(), lane = threadIdx.x & 63;
     const uint4 *kp4 = reinterpret_cast<const uint4 *>(kp);
@@ -773,7 +774,11 @@ __device__ __forceinline__ void pool_group(const uint16_t *kp, uint32_t q, uint3
             needA = nA - belowA - below2A;
         }
     }
-    if (t == 0) pool_want[q] = W;
+    if (t == 0) {
+        pool_want[q] = W;
+        // members outside A have keys >= TA (A takes every key below it)
+        if (a_min) a_min[q] = (a_bits && min(ca, W) < W) ? TA : 0xffffu;
+    }
     // collect: round r = units r*T .. r*T + T-1 in thread order (slot order)
     uint32_t tie_run = 0, tieA_run = 0, mark_run = 0;
     auto collect = [&](const uint4 v, uint32_t u) {
@@ -870,7 +875,8 @@ __global__ __launch_bounds__(256) void k_pool_select(const PoolParams p, uint32_
     pool_group<BITS, NW, RU>(p.keys + b, q, n, p.want, p.pool_entry, p.pool_key, p.pool_want,
                              !BITS && p.map, tab, hist[gi], g,
                              BITS ? p.sel_bits + b / 32 : nullptr,
-                             BITS && p.a_bits ? p.a_bits + b / 32 : nullptr, p.ca);
+                             BITS && p.a_bits ? p.a_bits + b / 32 : nullptr, p.ca,
+                             BITS ? p.a_min : nullptr);
 }
 
 // ---------------------------------------------------------------- query order
@@ -1332,7 +1338,13 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
     }
     __syncthreads();
     const uint32_t thr = L.m[THR];
-    // 3. survivors: pool members outside A whose key is at most thr
+    // 3. survivors: pool members outside A whose key is at most thr (none
+    // when thr is below every such key)
+    if (p.a_min && thr < p.a_min[q]) {
+        smallest_k(L.ex, na, p.k, emit);
+        if (p.stats && tid == 0) p.stats[q] = na;
+        return;
+    }
     const uint16_t *kp = p.keys + b;
     const uint32_t nb = rebuild_members(L, w0, w1, [&](uint32_t w) {
         const uint32_t rest = sel_w(w) & ~a_w(w);
