@@ -212,3 +212,18 @@ def test_exact_probe_above_4096_postings(n_probe):
     dev = DeviceIndex(ix)
     q = rng.integers(-4, 5, size=(40, 8)).astype(np.float32)
     compare(dev, ix, q, 10, n_probe)
+
+
+@pytest.mark.parametrize("nq", [40, 300])
+def test_pair_ranks_above_16384_postings(nq):
+    """20,000 postings: above the 16,384 LDS counters of the block-wise pair
+    ranking, so ranks come from device-wide atomics; 300 queries span two
+    ranking blocks on the smaller index below it (8,000 postings)."""
+    rng = np.random.default_rng(13)
+    x = rng.normal(size=(60000, 8)).astype(np.float32)
+    for n_post in (20000, 8000):
+        cents = x[rng.choice(60000, n_post, replace=False)]
+        ix = build_index(x, "l2", None, centroids=cents, seed=0)
+        dev = DeviceIndex(ix)
+        q = rng.normal(size=(nq, 8)).astype(np.float32)
+        compare(dev, ix, q, 10, 64)
