@@ -497,12 +497,6 @@ __device__ __forceinline__ void unit_masks2(const uint4 v, uint32_t u, uint32_t 
     eqA = ea;
 }
 
-// key j (runtime, 0..7) of an 8-key unit
-__device__ __forceinline__ uint32_t unit_key(const uint4 v, uint32_t j) {
-    const uint32_t t = j >> 1;
-    const uint32_t d = t == 0 ? v.x : t == 1 ? v.y : t == 2 ? v.z : v.w;
-    return (d >> (16 * (j & 1))) & 0xffffu;
-}
 
 // the lowest `c` set bits of m
 __device__ __forceinline__ uint32_t lowest_bits(uint32_t m, uint32_t c) {
@@ -648,20 +642,31 @@ struct BinTab {
     }
 };
 
+// keys of word w (32 slots, four units) at slots >= n read as "no key"
+__device__ __forceinline__ void word_tail(uint4 (&r)[4], uint32_t w, uint32_t n) {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+        const uint32_t u = 4 * w + k;
+        if (8 * u >= n) r[k] = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+        else unit_tail(r[k], u, n);
+    }
+}
+
 // The `want` smallest 16-bit keys of one query's segment, exactly-tied keys
 // at the threshold taken in slot order, written as one selection mask per
 // 32-slot word (sel_bits, at the segment's word offset; a_bits marks the `ca`
 // smallest of them the same way), or else as entries (map) or slots, with
 // their keys (pool_key, optional).  hist: 1,152 words of LDS.
-// The first pass loads the group's units -- the first RU per thread stay in
-// registers, so a segment of up to RU * 8 * 64 * NW keys is read from memory
-// once; the rest is re-read by each later pass.  Passes: min / max (packed
-// 16-bit); a 1,024-bin histogram of (key - min) >> sh, whose total is the
-// live count; only when the keys span more than 1,024 values, histograms of
-// the low sh bits inside the two threshold bins; collect in slot order
-// (rounds of 64 * NW units), where a unit whose smallest key is above the
-// pool threshold is known to hold no member.
-template <bool BITS, int NW, int RU>
+// Thread t of the group owns the segment's 32-slot words t, t + 64*NW, ...;
+// the first RW stay in registers after the first pass, so a segment of up to
+// RW * 32 * 64 * NW keys is read from memory once; the rest is re-read by
+// each later pass.  Passes: min / max (packed 16-bit); a 1,024-bin histogram
+// of (key - min) >> sh, whose total is the live count; only when the keys
+// span more than 1,024 values, histograms of the low sh bits inside the two
+// threshold bins; collect in slot order (rounds of 64 * NW words, one group
+// scan each), where a unit whose smallest key is above the pool threshold is
+// known to hold no member.
+template <bool BITS, int NW, int RW>
 __device__ __forceinline__ void pool_group(const uint16_t *kp, uint32_t q, uint32_t n,
                                            uint32_t want, uint32_t *pool_entry,
                                            uint16_t *pool_key, uint32_t *pool_want, bool map,
@@ -669,31 +674,30 @@ __device__ __forceinline__ void pool_group(const uint16_t *kp, uint32_t q, uint3
                                            uint32_t *sel_bits, uint32_t *a_bits, uint32_t ca,
                                            uint32_t *a_min) {
     constexpr uint32_t T = PoolGroup<NW>::T;
-    const uint32_t t = PoolGroup<NW>::tid(), lane = threadIdx.x & 63;
-    const uint4 *kp4 = reinterpret_cast<const uint4 *>(kp);
-    const uint32_t nun = (n + 7) / 8;
-    uint4 reg[RU];
+    const uint32_t t = PoolGroup<NW>::tid();
+    const uint32_t nwords = (n + 31) / 32;
+    uint4 reg[RW][4];
 #pragma unroll
-    for (int i = 0; i < RU; i++)
-        if (t + T * i < nun) reg[i] = kp4[t + T * i];
+    for (int i = 0; i < RW; i++)
+        if (t + T * i < nwords) load_word_raw(kp, t + T * i, reg[i]);
 #pragma unroll
-    for (int i = 0; i < RU; i++)
-        if (t + T * i < nun) unit_tail(reg[i], t + T * i, n);
-    // every owned unit once, registers first: body(unit, index)
+    for (int i = 0; i < RW; i++)
+        if (t + T * i < nwords) word_tail(reg[i], t + T * i, n);
+    // every owned word once, registers first: body(word, index)
     auto sweep = [&](auto &&body) {
 #pragma unroll
-        for (int i = 0; i < RU; i++)
-            if (t + T * i < nun) body(reg[i], t + T * i);
-        for (uint32_t u0 = t + T * RU; u0 < nun; u0 += 4 * T) {
-            uint4 v[4];
+        for (int i = 0; i < RW; i++)
+            if (t + T * i < nwords) body(reg[i], t + T * i);
+        for (uint32_t w0 = t + T * RW; w0 < nwords; w0 += 2 * T) {
+            uint4 v[2][4];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (u0 + T * i < nun) v[i] = kp4[u0 + T * i];
+            for (int i = 0; i < 2; i++)
+                if (w0 + T * i < nwords) load_word_raw(kp, w0 + T * i, v[i]);
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (u0 + T * i < nun) {
-                    unit_tail(v[i], u0 + T * i, n);
-                    body(v[i], u0 + T * i);
+            for (int i = 0; i < 2; i++)
+                if (w0 + T * i < nwords) {
+                    word_tail(v[i], w0 + T * i, n);
+                    body(v[i], w0 + T * i);
                 }
         }
     };
@@ -702,12 +706,15 @@ __device__ __forceinline__ void pool_group(const uint16_t *kp, uint32_t q, uint3
     // (0xffff wraps to 0 and never wins)
     u16x2 mn2 = {0xffff, 0xffff}, mx2 = {0, 0};
     const u16x2 one2 = {1, 1};
-    sweep([&](const uint4 v, uint32_t) {
-        const uint32_t c[4] = {v.x, v.y, v.z, v.w};
+    sweep([&](const uint4 (&r)[4], uint32_t) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            mn2 = __builtin_elementwise_min(mn2, as_u16x2(c[i]));
-            mx2 = __builtin_elementwise_max(mx2, as_u16x2(c[i]) + one2);
+        for (int k = 0; k < 4; k++) {
+            const uint32_t c[4] = {r[k].x, r[k].y, r[k].z, r[k].w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                mn2 = __builtin_elementwise_min(mn2, as_u16x2(c[i]));
+                mx2 = __builtin_elementwise_max(mx2, as_u16x2(c[i]) + one2);
+            }
         }
     });
     uint32_t zero = 0, kmn = min((uint32_t)mn2.x, (uint32_t)mn2.y);
@@ -721,13 +728,16 @@ __device__ __forceinline__ void pool_group(const uint16_t *kp, uint32_t q, uint3
         const uint32_t span = kmx1 - 1 - kmn;
         const uint32_t sh = span < 1024 ? 0 : 32 - __clz(span) - 10;
         const u16x2 kmn2 = {(unsigned short)kmn, (unsigned short)kmn};
-        sweep([&](const uint4 v, uint32_t) {
-            const uint32_t c[4] = {v.x, v.y, v.z, v.w};
+        sweep([&](const uint4 (&r)[4], uint32_t) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t d = as_u32(as_u16x2(c[i]) - kmn2);
-                if ((c[i] & 0xffffu) != 0xffffu) atomicAdd(&h1[(d & 0xffffu) >> sh], 1u);
-                if (c[i] < 0xffff0000u) atomicAdd(&h1[d >> (16 + sh)], 1u);
+            for (int k = 0; k < 4; k++) {
+                const uint32_t c[4] = {r[k].x, r[k].y, r[k].z, r[k].w};
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t d = as_u32(as_u16x2(c[i]) - kmn2);
+                    if ((c[i] & 0xffffu) != 0xffffu) atomicAdd(&h1[(d & 0xffffu) >> sh], 1u);
+                    if (c[i] < 0xffff0000u) atomicAdd(&h1[d >> (16 + sh)], 1u);
+                }
             }
         });
         g.sync();
@@ -746,15 +756,18 @@ __device__ __forceinline__ void pool_group(const uint16_t *kp, uint32_t q, uint3
             uint32_t b2W = 0, below2W = 0, b2A = 0, below2A = 0;
             if (sh > 0) {   // at most 64 sub-bins each
                 const uint32_t mask = (1u << sh) - 1u;
-                sweep([&](const uint4 v, uint32_t) {
-                    const uint32_t c[4] = {v.x, v.y, v.z, v.w};
+                sweep([&](const uint4 (&r)[4], uint32_t) {
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                        const uint32_t kk = (c[j >> 1] >> (16 * (j & 1))) & 0xffffu;
-                        if (kk == 0xffffu) continue;
-                        const uint32_t bin = (kk - kmn) >> sh;
-                        if (!all && bin == bW) atomicAdd(&h2[(kk - kmn) & mask], 1u);
-                        if (bin == bA) atomicAdd(&h3[(kk - kmn) & mask], 1u);
+                    for (int k = 0; k < 4; k++) {
+                        const uint32_t c[4] = {r[k].x, r[k].y, r[k].z, r[k].w};
+#pragma unroll
+                        for (int j = 0; j < 8; j++) {
+                            const uint32_t kk = (c[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+                            if (kk == 0xffffu) continue;
+                            const uint32_t bin = (kk - kmn) >> sh;
+                            if (!all && bin == bW) atomicAdd(&h2[(kk - kmn) & mask], 1u);
+                            if (bin == bA) atomicAdd(&h3[(kk - kmn) & mask], 1u);
+                        }
                     }
                 });
                 g.sync();
@@ -779,33 +792,34 @@ __device__ __forceinline__ void pool_group(const uint16_t *kp, uint32_t q, uint3
         // members outside A have keys >= TA (A takes every key below it)
         if (a_min) a_min[q] = (a_bits && min(ca, W) < W) ? TA : 0xffffu;
     }
-    // collect: round r = units r*T .. r*T + T-1 in thread order (slot order)
+    // collect: round r = words r*T .. r*T + T-1 in thread order (slot order)
     uint32_t tie_run = 0, tieA_run = 0, mark_run = 0;
-    auto collect = [&](const uint4 v, uint32_t u) {
+    auto collect = [&](const uint4 (&r)[4], uint32_t w) {
         uint32_t live = 0, lt = 0, eq = 0, ltA = 0, eqA = 0;
-        // TA <= T1, and T1 < 0xffff unless every live key is taken
-        if (u < nun && unit_min(v) <= T1) unit_masks2(v, u, n, T1, TA, live, lt, eq, ltA, eqA);
-        // tie counts of this round (each at most 8 * T) packed in one scan
+        if (w < nwords)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                // TA <= T1, and T1 < 0xffff unless every live key is taken
+                if (unit_min(r[k]) > T1) continue;
+                uint32_t lv, l, e, la, ea;
+                unit_masks2(r[k], 4 * w + k, n, T1, TA, lv, l, e, la, ea);
+                live |= lv << (8 * k);
+                lt |= l << (8 * k);
+                eq |= e << (8 * k);
+                ltA |= la << (8 * k);
+                eqA |= ea << (8 * k);
+            }
+        // tie counts of this round (each at most 32 * T) packed in one scan
         uint32_t tt;
         const uint32_t pk = g.excl((all ? 0u : __popc(eq)) | ((allA ? 0u : __popc(eqA)) << 16), &tt);
         const uint32_t tr = tie_run + (pk & 0xffffu), trA = tieA_run + (pk >> 16);
         tie_run += tt & 0xffffu;
         tieA_run += tt >> 16;
         const uint32_t sel = all ? live : lt | (tr < need ? lowest_bits(eq, need - tr) : 0u);
-        if (BITS) {   // four neighbouring lanes hold one 32-slot word
-            const uint32_t sh8 = 8 * (lane & 3);
-            uint32_t ws = sel << sh8;
-            ws |= __shfl_xor(ws, 1, 64);
-            ws |= __shfl_xor(ws, 2, 64);
-            uint32_t wa = 0;
-            if (a_bits) {
-                wa = (allA ? live : ltA | (trA < needA ? lowest_bits(eqA, needA - trA) : 0u)) << sh8;
-                wa |= __shfl_xor(wa, 1, 64);
-                wa |= __shfl_xor(wa, 2, 64);
-            }
-            if ((lane & 3) == 0 && u < nun) {
-                sel_bits[u / 4] = ws;
-                if (a_bits) a_bits[u / 4] = wa;
+        if (BITS) {   // one mask store per word
+            if (w < nwords) {
+                sel_bits[w] = sel;
+                if (a_bits) a_bits[w] = allA ? live : ltA | (trA < needA ? lowest_bits(eqA, needA - trA) : 0u);
             }
             return;
         }
@@ -814,36 +828,37 @@ __device__ __forceinline__ void pool_group(const uint16_t *kp, uint32_t q, uint3
         mark_run += mt;
         if (sel) {   // pool positions follow slot order
             uint32_t bits = sel;
-            uint32_t pj = map ? tab.find(8 * u + (__ffs(bits) - 1)) : 0u;
+            uint32_t pj = map ? tab.find(32 * w + (__ffs(bits) - 1)) : 0u;
             while (bits) {
                 const uint32_t j = __ffs(bits) - 1;
                 bits &= bits - 1;
                 const size_t o = (size_t)q * want + pre++;
-                pool_entry[o] = map ? tab.entry_from(pj, 8 * u + j) : 8 * u + j;
-                if (pool_key) pool_key[o] = (uint16_t)unit_key(v, j);
+                pool_entry[o] = map ? tab.entry_from(pj, 32 * w + j) : 32 * w + j;
+                if (pool_key) pool_key[o] = kp[32 * w + j];   // (from memory: no dynamic
+                                                                 // index into the registers)
             }
         }
     };
 #pragma unroll
-    for (int i = 0; i < RU; i++)
-        if (T * i < nun) collect(reg[i], t + T * i);   // (uniform over the group)
-    for (uint32_t r0 = T * RU; r0 < nun; r0 += T) {
-        const uint32_t u = r0 + t;
-        uint4 v = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
-        if (u < nun) {
-            v = kp4[u];
-            unit_tail(v, u, n);
+    for (int i = 0; i < RW; i++)
+        if (T * i < nwords) collect(reg[i], t + T * i);   // (uniform over the group)
+    for (uint32_t r0 = T * RW; r0 < nwords; r0 += T) {
+        const uint32_t w = r0 + t;
+        uint4 v[4];
+        if (w < nwords) {
+            load_word_raw(kp, w, v);
+            word_tail(v, w, n);
         }
-        collect(v, u);
+        collect(v, w);
     }
 }
 
 constexpr uint32_t kWaveTab = 256;   // probe tables held in LDS up to this n_probe
 
-// NW wavefronts per query (4 / NW queries per 256-thread workgroup), RU
-// register-held units per thread.
-template <bool BITS, int NW, int RU>
-__global__ __launch_bounds__(256) void k_pool_select(const PoolParams p, uint32_t nq) {
+// NW wavefronts per query (4 / NW queries per 256-thread workgroup), RW
+// register-held 32-slot words per thread; four workgroups per CU.
+template <bool BITS, int NW, int RW>
+__global__ __launch_bounds__(256, 4) void k_pool_select(const PoolParams p, uint32_t nq) {
     constexpr int G = 4 / NW;
     __shared__ uint32_t hist[G][1152];
     __shared__ uint32_t sx[G][5 * NW];
@@ -872,7 +887,7 @@ __global__ __launch_bounds__(256) void k_pool_select(const PoolParams p, uint32_
         tab.pbase = s_pbase[gi];
         tab.ebase = s_ebase[gi];
     }
-    pool_group<BITS, NW, RU>(p.keys + b, q, n, p.want, p.pool_entry, p.pool_key, p.pool_want,
+    pool_group<BITS, NW, RW>(p.keys + b, q, n, p.want, p.pool_entry, p.pool_key, p.pool_want,
                              !BITS && p.map, tab, hist[gi], g,
                              BITS ? p.sel_bits + b / 32 : nullptr,
                              BITS && p.a_bits ? p.a_bits + b / 32 : nullptr, p.ca,
@@ -1384,9 +1399,9 @@ hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) 
 hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s) {
     if (nq == 0) return hipSuccess;
     if (p.sel_bits)   // candidate pools: a workgroup per query, 24,576 keys in registers
-        hipLaunchKernelGGL((k_pool_select<true, 4, 12>), dim3(nq), dim3(256), 0, s, p, nq);
+        hipLaunchKernelGGL((k_pool_select<true, 4, 3>), dim3(nq), dim3(256), 0, s, p, nq);
     else              // probe pools: a wave per query, 4,096 keys in registers
-        hipLaunchKernelGGL((k_pool_select<false, 1, 8>), dim3((nq + 3) / 4), dim3(256), 0, s, p, nq);
+        hipLaunchKernelGGL((k_pool_select<false, 1, 2>), dim3((nq + 3) / 4), dim3(256), 0, s, p, nq);
     return hipGetLastError();
 }
 
