@@ -515,10 +515,14 @@ __device__ __forceinline__ void select_body(const uint64_t *kp, uint64_t n, uint
     if (tid == 0 && cnt_out) *cnt_out = Kv;
 }
 
+// Segments with n_lo <= n <= n_hi keys; R > 0: held in registers (n <= R * 256),
+// one launch per size band so the register-heavy band does not lower the
+// occupancy of the others.
+template <int R>
 __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint64_t *seg_off,
                                                 const uint32_t *seg_cnt, uint64_t seg_stride,
                                                 uint32_t K, uint64_t *out, uint32_t *out_cnt,
-                                                uint64_t skip_above) {
+                                                uint64_t n_lo, uint64_t n_hi) {
     __shared__ uint64_t buf[kSelCap];
     __shared__ uint32_t hist[2048];
     __shared__ uint32_t part[kWG];
@@ -528,13 +532,33 @@ __global__ __launch_bounds__(256) void k_select(const uint64_t *keys, const uint
     const uint64_t b = seg_off ? seg_off[seg] : (uint64_t)seg * seg_stride;
     const uint64_t e = seg_off ? seg_off[seg + 1] : b + seg_stride;
     const uint64_t n = seg_cnt ? (uint64_t)seg_cnt[seg] : e - b;
-    if (skip_above && n > skip_above) return;   // a long segment: its chunks do it
-    if (n <= 16 * 256)
-        select_body<16>(keys + b, n, K, 0, out + (size_t)seg * K, out_cnt + seg, buf, hist, part,
-                        s_u, s_lh);
-    else
-        select_body<0>(keys + b, n, K, 0, out + (size_t)seg * K, out_cnt + seg, buf, hist, part,
-                       s_u, s_lh);
+    if (n < n_lo || n > n_hi) return;   // another band's (or, long, its chunks')
+    select_body<R>(keys + b, n, K, 0, out + (size_t)seg * K, out_cnt + seg, buf, hist, part, s_u,
+                   s_lh);
+}
+
+constexpr uint64_t kSelRegSmall = 16 * 256, kSelRegBig = 96 * 256;   // keys held in registers
+
+// every segment of at most n_max keys, by size band
+static hipError_t select_bands(const uint64_t *keys, const uint64_t *seg_off,
+                               const uint32_t *seg_cnt, uint64_t seg_stride, uint32_t nseg,
+                               uint32_t K, uint64_t *out, uint32_t *out_cnt, uint64_t n_max,
+                               hipStream_t s) {
+    // fixed-length rows (no seg_cnt) need only the band of their length;
+    // segments above n_max are done in chunks by the caller
+    const uint64_t lo = seg_cnt ? 0 : seg_stride;
+    if (lo > n_max) return hipSuccess;
+    const uint64_t hi = seg_cnt ? n_max : seg_stride;
+    if (lo <= kSelRegSmall)
+        hipLaunchKernelGGL(k_select<16>, dim3(nseg), dim3(kWG), 0, s, keys, seg_off, seg_cnt,
+                           seg_stride, K, out, out_cnt, (uint64_t)0, std::min(n_max, kSelRegSmall));
+    if (hi > kSelRegSmall && lo <= kSelRegBig)
+        hipLaunchKernelGGL(k_select<96>, dim3(nseg), dim3(kWG), 0, s, keys, seg_off, seg_cnt,
+                           seg_stride, K, out, out_cnt, kSelRegSmall + 1, std::min(n_max, kSelRegBig));
+    if (hi > kSelRegBig)
+        hipLaunchKernelGGL(k_select<0>, dim3(nseg), dim3(kWG), 0, s, keys, seg_off, seg_cnt,
+                           seg_stride, K, out, out_cnt, kSelRegBig + 1, n_max);
+    return hipGetLastError();
 }
 
 // Long segments (more than kSelChunk keys) in two rounds, so one query's
@@ -974,9 +998,7 @@ hipError_t launch_select(const uint64_t *keys, const uint64_t *seg_off, const ui
                          uint64_t seg_stride, uint32_t nseg, uint32_t K, uint64_t *out,
                          uint32_t *out_cnt, hipStream_t s) {
     if (nseg == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_select, dim3(nseg), dim3(kWG), 0, s, keys, seg_off, seg_cnt, seg_stride,
-                       K, out, out_cnt, (uint64_t)0);
-    return hipGetLastError();
+    return select_bands(keys, seg_off, seg_cnt, seg_stride, nseg, K, out, out_cnt, ~0ull, s);
 }
 
 hipError_t launch_select_long(const uint64_t *keys, const uint64_t *seg_off,
@@ -985,9 +1007,7 @@ hipError_t launch_select_long(const uint64_t *keys, const uint64_t *seg_off,
                               uint32_t *chunk_cnt, uint32_t *chunk_off, uint64_t *tmp,
                               uint64_t max_chunks, hipStream_t s) {
     if (nseg == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_select, dim3(nseg), dim3(kWG), 0, s, keys, seg_off, seg_cnt,
-                       seg_stride, K, out, out_cnt, (uint64_t)chunk);
-    hipError_t e = hipGetLastError();
+    hipError_t e = select_bands(keys, seg_off, seg_cnt, seg_stride, nseg, K, out, out_cnt, chunk, s);
     if (e != hipSuccess || max_chunks == 0) return e;
     hipLaunchKernelGGL(k_sel_chunk_count, dim3((nseg + 255) / 256), dim3(256), 0, s, seg_off,
                        seg_cnt, seg_stride, nseg, chunk, chunk_cnt);
