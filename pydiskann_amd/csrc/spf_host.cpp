@@ -831,8 +831,8 @@ bool graph_shape(const spf_index *ix, uint32_t n, GraphParams *gp);
 uint64_t graph_scratch_words(const spf_index *ix, const GraphParams &g);
 extern const uint64_t kGraphScratchBytes;
 
-// chunks of the long-segment select: a segment above 65,536 candidates takes
-// ceil(n / 65,536) <= 2n / 65,536 chunks
+// chunks of the long-segment select: a segment above kSelChunk candidates
+// takes ceil(n / kSelChunk) <= 2n / kSelChunk chunks
 inline uint64_t sel_chunks(uint64_t total) {
     return total > kSelChunk ? 2 * total / kSelChunk + 1 : 0;
 }

@@ -575,6 +575,8 @@ __global__ void k_sel_chunk_count(const uint64_t *seg_off, const uint32_t *seg_c
     chunk_cnt[seg] = n > chunk ? (uint32_t)((n + chunk - 1) / chunk) : 0u;
 }
 
+// R: keys per thread held in registers for chunks up to R * 256 keys (0: none)
+template <int R>
 __global__ __launch_bounds__(256) void k_select_chunk(const uint64_t *keys, const uint64_t *seg_off,
                                                       const uint32_t *seg_cnt, uint64_t seg_stride,
                                                       const uint32_t *chunk_off, uint32_t nseg,
@@ -600,8 +602,8 @@ __global__ __launch_bounds__(256) void k_select_chunk(const uint64_t *keys, cons
     const uint64_t n = seg_cnt ? (uint64_t)seg_cnt[seg] : seg_off ? seg_off[seg + 1] - s0 : seg_stride;
     const uint64_t b = s0 + j * chunk;
     const uint64_t m = min((uint64_t)chunk, n - j * chunk);
-    if (m <= 16 * 256)
-        select_body<16>(keys + b, m, K, K, tmp + (size_t)c * K, nullptr, buf, hist, part, s_u, s_lh);
+    if (R > 0 && m <= (uint64_t)R * 256)
+        select_body<R>(keys + b, m, K, K, tmp + (size_t)c * K, nullptr, buf, hist, part, s_u, s_lh);
     else
         select_body<0>(keys + b, m, K, K, tmp + (size_t)c * K, nullptr, buf, hist, part, s_u, s_lh);
 }
@@ -1013,8 +1015,15 @@ hipError_t launch_select_long(const uint64_t *keys, const uint64_t *seg_off,
                        seg_cnt, seg_stride, nseg, chunk, chunk_cnt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = launch_excl_scan_u32(chunk_cnt, chunk_off, nseg, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_select_chunk, dim3((uint32_t)max_chunks), dim3(kWG), 0, s, keys, seg_off,
-                       seg_cnt, seg_stride, chunk_off, nseg, chunk, K, tmp);
+    if (chunk <= 16 * 256)
+        hipLaunchKernelGGL(k_select_chunk<16>, dim3((uint32_t)max_chunks), dim3(kWG), 0, s, keys,
+                           seg_off, seg_cnt, seg_stride, chunk_off, nseg, chunk, K, tmp);
+    else if (chunk <= 96 * 256)
+        hipLaunchKernelGGL(k_select_chunk<96>, dim3((uint32_t)max_chunks), dim3(kWG), 0, s, keys,
+                           seg_off, seg_cnt, seg_stride, chunk_off, nseg, chunk, K, tmp);
+    else
+        hipLaunchKernelGGL(k_select_chunk<0>, dim3((uint32_t)max_chunks), dim3(kWG), 0, s, keys,
+                           seg_off, seg_cnt, seg_stride, chunk_off, nseg, chunk, K, tmp);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_select_merge, dim3(nseg), dim3(kWG), 0, s, tmp, chunk_off, K, out, out_cnt);
     return hipGetLastError();

@@ -188,7 +188,7 @@ def test_batch_matches_single_and_errors():
 
 @pytest.mark.parametrize("quant,k,rerank", [(None, 10, 0), (None, 1000, 0), ("int8", 50, 400)])
 def test_long_candidate_lists_select_in_chunks(quant, k, rerank):
-    """Candidate lists far above 65,536 keys per query (the chunked select)."""
+    """Candidate lists far above the 24,576-key chunks of the long-list select."""
     rng = np.random.default_rng(11)
     x = rng.normal(size=(300000, 16)).astype(np.float32)
     cents = np.concatenate([rng.normal(size=(3, 16)) * 0.01, rng.normal(size=(40, 16)) * 3])
