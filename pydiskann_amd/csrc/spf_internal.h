@@ -365,7 +365,7 @@ hipError_t launch_select_long(const uint64_t *keys, const uint64_t *seg_off,
                               uint32_t *chunk_cnt, uint32_t *chunk_off, uint64_t *tmp,
                               uint64_t max_chunks, hipStream_t s);
 constexpr uint32_t kSelChunk = 24576;      // candidate lists: keys per chunk (in registers)
-constexpr uint32_t kProbeChunk = 4096;     // dense probe rows: keys per chunk (in registers)
+constexpr uint32_t kProbeChunk = 24576;    // dense probe rows: keys per chunk (in registers)
 // The matrix-core scan's slot ranges start at multiples of this many 16-bit
 // keys (16 bytes: its store width) and are reserved up to the next one.
 // (Whole 128-byte lines, 64 keys, measured no faster and add 13% slots.)

@@ -202,8 +202,9 @@ def test_long_candidate_lists_select_in_chunks(quant, k, rerank):
 
 @pytest.mark.parametrize("n_probe", [1, 64, 700])
 def test_exact_probe_above_4096_postings(n_probe):
-    """Exact probe over 9,000 centroids (the register-chunked probe select),
-    including integer centroids with equal distances (block-order ties)."""
+    """Exact probe over 9,000 centroids (rows selected with 96 keys per thread
+    in registers), including integer centroids with equal distances
+    (block-order ties)."""
     rng = np.random.default_rng(12)
     x = rng.integers(-4, 5, size=(60000, 8)).astype(np.float32)
     cents = x[rng.choice(60000, 9000, replace=False)] + rng.integers(0, 2, size=(9000, 8))
@@ -216,12 +217,13 @@ def test_exact_probe_above_4096_postings(n_probe):
 
 @pytest.mark.parametrize("nq", [40, 300])
 def test_pair_ranks_above_16384_postings(nq):
-    """20,000 postings: above the 16,384 LDS counters of the block-wise pair
-    ranking, so ranks come from device-wide atomics; 300 queries span two
-    ranking blocks on the smaller index below it (8,000 postings)."""
+    """30,000 postings: above the 16,384 LDS counters of the block-wise pair
+    ranking, so ranks come from device-wide atomics, and above the 24,576-key
+    rows the probe select holds in registers (chunks, then a merge); 300
+    queries span two ranking blocks on the smaller index (8,000 postings)."""
     rng = np.random.default_rng(13)
     x = rng.normal(size=(60000, 8)).astype(np.float32)
-    for n_post in (20000, 8000):
+    for n_post in (30000, 8000):
         cents = x[rng.choice(60000, n_post, replace=False)]
         ix = build_index(x, "l2", None, centroids=cents, seed=0)
         dev = DeviceIndex(ix)
