@@ -4,13 +4,16 @@
 //                   v_mfma_f32_16x16x32_f16 with f32 accumulate:
 //                   dist = |x|^2 + |q|^2 - 2 q.x, stored as a 16-bit key
 //                   (top half of the order-preserving f32 key) per candidate
+//   k_probe_mfma    centroid probe as a bf16 GEMM (centroids centred on
+//                   their mean), 16-bit keys per (query, centroid)
 //   k_pool_select   per query: the `want` smallest 16-bit keys (ties by
-//                   candidate slot), marked in a bitmap over the candidate buffer
-//   k_rerank_items  posting-major exact f32 rerank of the marked candidates
-//                   (canonical sequential-fma order), one work item = one
-//                   posting x <= 32 queries, so a posting's raw rows are read once
-//                   for all queries that kept candidates in it
-//   k_sort_topk     per query: sort the reranked pool by (dist, id), keep k
+//                   candidate slot), marked in a bitmap over the candidate
+//                   buffer (candidates) or listed (probe band)
+//   k_rerank_pool   exact f32 recheck of a query's probe band, sorted
+//   k_rerank_pruned per query: exact f32 rerank of the pool's `ca` smallest,
+//                   a bound that excludes the other members it can, exact
+//                   rerank of the rest, top k written as ids and distances
+//   k_perm_*        query order by nearest probed posting (rerank locality)
 //
 // MFMA fragment layout (v_mfma_f32_16x16x32_{f16,bf16}, wave64), checked on the
 // device by spf_selftest: lane l holds A[l%16][8*(l/16) .. +7],
@@ -304,21 +307,6 @@ __global__ __launch_bounds__(256) void k_probe_mfma(const ProbeMfmaParams p) {
 // Per query: mark the `want` smallest 16-bit keys (0xffff = no candidate).
 // Exactly-tied keys at the boundary are taken in slot order.
 
-__device__ __forceinline__ uint32_t excl_scan256(uint32_t v, uint32_t *part) {
-    const uint32_t tid = threadIdx.x;
-    part[tid] = v;
-    __syncthreads();
-    for (uint32_t off = 1; off < 256; off <<= 1) {
-        const uint32_t t = tid >= off ? part[tid - off] : 0u;
-        __syncthreads();
-        part[tid] += t;
-        __syncthreads();
-    }
-    const uint32_t incl = part[tid];
-    __syncthreads();
-    return incl - v;
-}
-
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const uint32_t lane = threadIdx.x & 63;
 #pragma unroll
@@ -347,35 +335,11 @@ __device__ __forceinline__ uint32_t block_scan_fast(uint32_t v, uint32_t *s_w, u
     return base + incl - v;
 }
 
-// find the bin where the cumulative count reaches `rem`; returns (bin, below)
-__device__ __forceinline__ void find_bin256(const uint32_t *hist, uint32_t rem, uint32_t *s_w,
-                                            uint32_t *s_out) {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t h = hist[tid];
-    uint32_t tot;
-    const uint32_t excl = block_scan_fast(h, s_w, &tot);
-    if (excl < rem && rem <= excl + h) {
-        s_out[0] = tid;
-        s_out[1] = excl;
-    }
-    __syncthreads();
-}
-
 // 32 keys of one bitmap word (64 bytes; segments are 32-slot aligned)
 __device__ __forceinline__ void load_word_raw(const uint16_t *kp, uint32_t w, uint4 r[4]) {
     const uint4 *src = reinterpret_cast<const uint4 *>(kp + 32 * w);
 #pragma unroll
     for (int j = 0; j < 4; j++) r[j] = src[j];
-}
-
-__device__ __forceinline__ void unpack_word(const uint4 r[4], uint32_t w, uint32_t n, uint16_t k[32]) {
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        union { uint4 v; uint16_t h[8]; } c;
-        c.v = r[j];
-#pragma unroll
-        for (int t = 0; t < 8; t++) k[8 * j + t] = (32 * w + 8 * j + t < n) ? c.h[t] : (uint16_t)0xffff;
-    }
 }
 
 // The query's probe table: segment-relative first slot and first entry of each
@@ -413,42 +377,6 @@ __device__ __forceinline__ uint32_t wave_excl(uint32_t v, uint32_t *tot) {
     const uint32_t incl = wave_incl_scan(v);
     *tot = __shfl(incl, 63, 64);
     return incl - v;
-}
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// bin where the running count over hist[0..255] reaches rem: (bin, below)
-__device__ __forceinline__ void wave_find_bin(const uint32_t *hist, uint32_t rem, uint32_t &bin,
-                                              uint32_t &below) {
-    const uint32_t lane = threadIdx.x & 63;
-    uint32_t h[4], sm = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        h[i] = hist[4 * lane + i];
-        sm += h[i];
-    }
-    uint32_t tot;
-    const uint32_t excl = wave_excl(sm, &tot);
-    uint32_t b = 0, bl = 0;
-    const bool mine = excl < rem && rem <= excl + sm;
-    if (mine) {
-        uint32_t c = excl;
-        for (int i = 0; i < 4; i++) {
-            if (rem <= c + h[i]) {
-                b = 4 * lane + i;
-                bl = c;
-                break;
-            }
-            c += h[i];
-        }
-    }
-    const uint64_t who = __ballot(mine);
-    const int src = who ? __ffsll((unsigned long long)who) - 1 : 0;
-    bin = __shfl(b, src, 64);
-    below = __shfl(bl, src, 64);
 }
 
 // bit masks of one 32-key word: live keys, keys below T, keys equal to T
@@ -1081,14 +1009,6 @@ constexpr int kPrH = 2, kPrUC = 8;    // rows per quad, row units per lane in fl
 __device__ __forceinline__ uint32_t blk_excl(uint32_t v, uint32_t *s_w) {
     uint32_t tot;
     return block_scan_fast(v, s_w, &tot);
-}
-
-// bin b holding the R-th smallest (R >= 1) and the count below it
-__device__ __forceinline__ void find_rank_bin(uint32_t h, uint32_t excl, uint32_t R, uint32_t *out) {
-    if (R && excl < R && R <= excl + h) {
-        out[0] = threadIdx.x;
-        out[1] = excl;
-    }
 }
 
 // exact keys of `cnt` entries listed in LDS -> keys_out[0..cnt)
