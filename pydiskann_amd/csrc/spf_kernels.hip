@@ -288,26 +288,37 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
             } else {   // CK_U8 decode path (cosine / dot), src/quantized.rs:28-36
                 const float4 *s4 = reinterpret_cast<const float4 *>(s_q);
                 const float sc = p.scale, of = p.offset;
+                // unit u = dims 16u.. -> chains 0..3, in increasing u; four
+                // units' loads in flight at a time
 #pragma unroll 1
-                for (uint32_t u = 0; u < p.units; u++) {   // unit u = dims 16u.. -> chains 0..3
-                    const uint4 w = valid ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
-                    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-                    float x[16];
+                for (uint32_t u0 = 0; u0 < p.units; u0 += 4) {
+                    uint4 w4[4];
 #pragma unroll
-                    for (int j = 0; j < 16; j++) {
-                        const float c = (float)((ws[j >> 2] >> (8 * (j & 3))) & 0xffu);
-                        const float t = c * sc;
-                        x[j] = t + of;
-                    }
+                    for (int r = 0; r < 4; r++)
+                        w4[r] = (valid && u0 + r < p.units) ? src[(size_t)(u0 + r) * 64]
+                                                            : make_uint4(0, 0, 0, 0);
 #pragma unroll
-                    for (int g = 0; g < G; g++) {
+                    for (int r = 0; r < 4; r++) {
+                        const uint32_t u = u0 + r;
+                        if (u >= p.units) break;   // (uniform)
+                        const uint32_t ws[4] = {w4[r].x, w4[r].y, w4[r].z, w4[r].w};
+                        float x[16];
 #pragma unroll
-                        for (int j4 = 0; j4 < 4; j4++) {
-                            const float4 qq = s4[g * dq4 + 4 * u + j4];
-                            step<MET>(accf[g][j4], qq.x, x[4 * j4 + 0]);
-                            step<MET>(accf[g][j4], qq.y, x[4 * j4 + 1]);
-                            step<MET>(accf[g][j4], qq.z, x[4 * j4 + 2]);
-                            step<MET>(accf[g][j4], qq.w, x[4 * j4 + 3]);
+                        for (int j = 0; j < 16; j++) {
+                            const float c = (float)((ws[j >> 2] >> (8 * (j & 3))) & 0xffu);
+                            const float t = c * sc;
+                            x[j] = t + of;
+                        }
+#pragma unroll
+                        for (int g = 0; g < G; g++) {
+#pragma unroll
+                            for (int j4 = 0; j4 < 4; j4++) {
+                                const float4 qq = s4[g * dq4 + 4 * u + j4];
+                                step<MET>(accf[g][j4], qq.x, x[4 * j4 + 0]);
+                                step<MET>(accf[g][j4], qq.y, x[4 * j4 + 1]);
+                                step<MET>(accf[g][j4], qq.z, x[4 * j4 + 2]);
+                                step<MET>(accf[g][j4], qq.w, x[4 * j4 + 3]);
+                            }
                         }
                     }
                 }
