@@ -81,7 +81,6 @@ struct DBuf {
     X(float, out_d) X(uint32_t, out_c) X(uint16_t, q16) X(uint16_t, cand16) X(uint16_t, qbf) \
     X(uint16_t, probe16) X(float, qn16) X(float, qnbf) X(uint32_t, probe_pool)             \
     X(uint32_t, probe_pool_cnt) X(uint32_t, pool_want) X(uint32_t, cand_cnt)               \
-    X(uint32_t, perm_cnt) X(uint32_t, perm_off) X(uint32_t, perm_fill) X(uint32_t, qperm)  \
     X(uint64_t, rq_planes) X(float, rq_consts) X(uint32_t, pair_src) X(float, eq16)        \
     X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)        \
     X(uint32_t, gflag) X(uint32_t, gscr) X(uint32_t, sel_ccnt) X(uint32_t, sel_coff)            \
@@ -658,10 +657,6 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     HIPCHK(w->pool_want.ensure(nq));
     HIPCHK(w->fin.ensure((size_t)nq * k));
     HIPCHK(w->fin_cnt.ensure(nq));
-    HIPCHK(w->perm_cnt.ensure(ix->P + 2));
-    HIPCHK(w->perm_off.ensure(ix->P + 2));
-    HIPCHK(w->perm_fill.ensure(ix->P + 2));
-    HIPCHK(w->qperm.ensure(nq));
     MixedParams mp{};
     mp.item_off = w->item_off.p;
     mp.item_post = w->item_post.p;
@@ -715,10 +710,8 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     // phase-A size of the rerank: k plus a margin for the 16-bit key resolution
     pp.ca = std::min(want, std::max(k, round_up(k + k / 8 + 8, 32)));
     HIPCHK(launch_pool_select(pp, nq, s));
-    // nearest-posting query order, then the (pruned) exact rerank of each pool
+    // the (pruned) exact rerank of each pool
     HIPCHK(mark(SPF_STAGE_RERANK));
-    HIPCHK(launch_query_perm(w->np_q.p, w->probe_post.p, npr, nq, ix->P, w->perm_cnt.p,
-                             w->perm_off.p, w->perm_fill.p, w->qperm.p, s));
     RerankPrunedParams rp{};
     rp.keys = w->cand16.p;
     rp.seg_off = w->cand_off.p;
@@ -736,7 +729,6 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     rp.metric = ix->metric;
     rp.dpad = ix->dpad;
     rp.dim = ix->dim;
-    rp.qperm = w->qperm.p;
     rp.np_q = w->np_q.p;
     rp.probe_post = w->probe_post.p;
     rp.npr = npr;
@@ -929,8 +921,6 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
         add(q, 4);                                  // a_min
         add(q * k, 8);                              // fin
         add(q, 4);                                  // fin_cnt
-        add(P + 2, 4 * 3);                          // perm_cnt, perm_off, perm_fill
-        add(q, 4);                                  // qperm
         add(q, 4);                                  // rr_stats
     } else if (ix->rq_l2) {
         add(q * k, 8);
@@ -1258,12 +1248,6 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.G = G;
     pl.echunk = echunk;
     pl.align = z.mfma_scan ? kScanAlign : 1u;   // the matrix-core scan stores aligned rows
-    if (mixed) {   // the rerank's query-order counters, zeroed by the planner
-        HIPCHK(w->perm_cnt.ensure(P + 2));
-        HIPCHK(w->perm_fill.ensure(P + 2));
-        pl.zero_a = w->perm_cnt.p;
-        pl.zero_b = w->perm_fill.p;
-    }
     if (prof) {   // per-query candidates without alignment slots, for the profile
         HIPCHK(w->cand_real.ensure(nq));
         pl.cand_real = w->cand_real.p;

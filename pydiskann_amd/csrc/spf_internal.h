@@ -202,7 +202,6 @@ struct RerankPrunedParams {
     const uint64_t *qp_slot;    // [nq][npr] absolute first slot of each probed posting
     const uint32_t *post_off;
     uint32_t nq, want, k, ca, metric, dpad, dim;
-    const uint32_t *qperm;      // processing order (null = identity)
     const uint32_t *np_q;       // [nq]
     const uint32_t *probe_post; // [nq][npr]
     uint32_t npr;
@@ -287,11 +286,10 @@ struct PoolParams {
 };
 
 // Exact f32 rerank of each query's pool and the final top-k, one workgroup per
-// query, queries taken in `qperm` order.
+// query.
 struct RerankPoolParams {
     const uint32_t *pool_entry; // [nq][want]
     const uint32_t *pool_want;  // [nq]
-    const uint32_t *qperm;      // [nq] processing order (by nearest posting)
     uint32_t nq, want, k, metric, dpad;
     const float *qv;            // [nq][dpad]
     const float *qnorm;         // [nq]
@@ -328,7 +326,6 @@ struct PlanParams {
     uint32_t echunk;            // entries per work item
     uint32_t align;             // each probe's candidate slots start at a multiple of this
     uint32_t *cand_real;        // [nq] candidates without alignment slots (null = skip)
-    uint32_t *zero_a, *zero_b;  // [n_post + 2] each, zeroed by k_items (null = none)
     // pair ranks without global atomics: queries in blocks of qb, per-block
     // posting counts in LDS, then exclusive over blocks (0 = global atomics)
     uint32_t qb;
@@ -403,8 +400,5 @@ hipError_t launch_rq_prep(const RqParams &p, uint32_t n_pairs, hipStream_t s);
 hipError_t launch_rq_search_exact(const RqParams &p, hipStream_t s);
 hipError_t launch_rq_decode(const RqDecodeParams &p, uint32_t n, hipStream_t s);
 hipError_t launch_rq_scan(const RqParams &p, uint32_t grid, hipStream_t s);
-hipError_t launch_query_perm(const uint32_t *np_q, const uint32_t *probe_post, uint32_t npr,
-                             uint32_t nq, uint32_t n_post, uint32_t *cnt, uint32_t *off,
-                             uint32_t *fill, uint32_t *qperm, hipStream_t s);
 
 }  // namespace spf

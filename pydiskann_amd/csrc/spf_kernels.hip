@@ -761,11 +761,6 @@ __global__ void k_blk_scan(const PlanParams p, uint32_t nblk) {
 
 __global__ void k_items(const PlanParams p) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p.zero_a)   // the later query-order count arrays (n_post + 2 each), zeroed here
-        for (uint32_t i = b; i < p.n_post + 2; i += gridDim.x * blockDim.x) {
-            p.zero_a[i] = 0u;
-            p.zero_b[i] = 0u;
-        }
     if (b >= p.n_post) return;
     const uint32_t cnt = p.pair_off[b + 1] - p.pair_off[b];
     const uint32_t len = p.post_len[b];

@@ -13,7 +13,6 @@
 //   k_rerank_pruned per query: exact f32 rerank of the pool's `ca` smallest,
 //                   a bound that excludes the other members it can, exact
 //                   rerank of the rest, top k written as ids and distances
-//   k_perm_*        query order by nearest probed posting (rerank locality)
 //
 // MFMA fragment layout (v_mfma_f32_16x16x32_{f16,bf16}, wave64), checked on the
 // device by spf_selftest: lane l holds A[l%16][8*(l/16) .. +7],
@@ -822,26 +821,6 @@ __global__ __launch_bounds__(256, 4) void k_pool_select(const PoolParams p, uint
                              BITS ? p.a_min : nullptr);
 }
 
-// ---------------------------------------------------------------- query order
-// Counting sort of the batch by nearest probed posting: queries that share
-// postings then run side by side on one XCD and share raw rows in its L2.
-__global__ void k_perm_count(const uint32_t *np_q, const uint32_t *probe_post, uint32_t npr,
-                             uint32_t nq, uint32_t n_post, uint32_t *cnt) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
-    const uint32_t key = np_q[q] ? probe_post[(size_t)q * npr] : n_post;
-    atomicAdd(&cnt[key], 1u);
-}
-
-__global__ void k_perm_scatter(const uint32_t *np_q, const uint32_t *probe_post, uint32_t npr,
-                               uint32_t nq, uint32_t n_post, const uint32_t *off, uint32_t *fill,
-                               uint32_t *qperm) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
-    const uint32_t key = np_q[q] ? probe_post[(size_t)q * npr] : n_post;
-    qperm[off[key] + atomicAdd(&fill[key], 1u)] = q;
-}
-
 // ---------------------------------------------------------------- rerank
 // Exact f32 distance, four lanes per pair (lane r of the quad runs chain r over
 // units r, r+4, ...), H pairs per quad in flight; canonical 4-chain order.
@@ -939,12 +918,12 @@ constexpr int kRerankH = 2, kRerankUC = 8;   // rows per quad, row units per lan
 __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
     __shared__ uint64_t s_keys[kMaxK];
     extern __shared__ __align__(16) float s_qv[];   // dpad floats
-    // XCD x takes a contiguous range of the nearest-posting order
+    // XCD x takes a contiguous range of the queries
     const uint32_t x = blockIdx.x % kXcds, jx = blockIdx.x / kXcds;
     const uint32_t per = (p.nq + kXcds - 1) / kXcds;
     const uint32_t idx = x * per + jx;
     if (jx >= per || idx >= p.nq) return;
-    const uint32_t q = p.qperm ? p.qperm[idx] : idx;
+    const uint32_t q = idx;
     const uint32_t tid = threadIdx.x, r = tid & 3, quad = tid >> 2, units = p.dpad / 4;
     const uint32_t c = min(p.pool_want[q], p.want);
     for (uint32_t i = tid; i < p.dpad; i += 256) s_qv[i] = p.qv[(size_t)q * p.dpad + i];
@@ -994,7 +973,7 @@ __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
 }
 
 // ---------------------------------------------------------------- pruned rerank
-// One workgroup per query (nearest-posting order, XCD-contiguous) over its
+// One workgroup per query (XCD-contiguous ranges of queries) over its
 // pool of `c` (entry, 16-bit key) members, in pool order:
 //   1. A = the `ca` smallest keys (ties in pool order), by two radix passes
 //   2. A is reranked exactly; its k-th exact distance bounds the answer
@@ -1128,7 +1107,7 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
     const uint32_t per = (p.nq + kXcds - 1) / kXcds;
     const uint32_t idx = x * per + jx;
     if (jx >= per || idx >= p.nq) return;   // whole workgroup
-    const uint32_t q = p.qperm ? p.qperm[idx] : idx;
+    const uint32_t q = idx;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     const uint32_t c = min(p.pool_want[q], p.want);
     uint64_t *out = p.out + (size_t)q * p.k;
@@ -1356,21 +1335,6 @@ hipError_t launch_probe_mfma(const ProbeMfmaParams &p, hipStream_t s) {
         hipLaunchKernelGGL(k_probe_mfma<kMixKS>, dim3(grid), dim3(256), 0, s, p); break;
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
-}
-
-// cnt and fill ([n_post + 2] each) arrive zeroed (the planner's k_items)
-hipError_t launch_query_perm(const uint32_t *np_q, const uint32_t *probe_post, uint32_t npr,
-                             uint32_t nq, uint32_t n_post, uint32_t *cnt, uint32_t *off,
-                             uint32_t *fill, uint32_t *qperm, hipStream_t s) {
-    if (nq == 0) return hipSuccess;
-    hipError_t e;
-    hipLaunchKernelGGL(k_perm_count, dim3((nq + 255) / 256), dim3(256), 0, s, np_q, probe_post,
-                       npr, nq, n_post, cnt);
-    e = launch_excl_scan_u32(cnt, off, n_post + 1, s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_perm_scatter, dim3((nq + 255) / 256), dim3(256), 0, s, np_q, probe_post,
-                       npr, nq, n_post, off, fill, qperm);
     return hipGetLastError();
 }
 
