@@ -46,7 +46,7 @@ def recall(ids, gt, k=10):
 
 @pytest.mark.parametrize("quant", [None, "f16"])
 @pytest.mark.parametrize("metric", ["l2", "cosine", "dot", "l2_root"])
-@pytest.mark.parametrize("dim", [32, 100, 128])
+@pytest.mark.parametrize("dim", [32, 64, 96, 100, 128, 200])   # 1, 2, 3, 4, 4, 8 K steps of 32
 def test_mixed_exact_rerank_and_recall(quant, metric, dim):
     x = clustered(4000, dim, 25, seed=1)
     q = clustered(64, dim, 25, seed=2)
