@@ -131,8 +131,9 @@ def test_filtered_mixed_exact_rows_and_recall(quant, metric):
     assert r >= 0.9
 
 
-def test_filtered_mixed_equals_exact_when_pool_covers_all():
-    _, ix, q, masks = labelled(3000, 64, None, "l2", seed=5)
+@pytest.mark.parametrize("dim", [64, 128, 200])   # filtered matrix-core scan: 2, 4, 8 K steps
+def test_filtered_mixed_equals_exact_when_pool_covers_all(dim):
+    _, ix, q, masks = labelled(3000, dim, None, "l2", seed=5)
     dev = DeviceIndex(ix)
     e = dev.search(q, 10, 4, precision=PREC_EXACT, masks=masks)
     m = dev.search(q, 10, 4, precision=PREC_MIXED, rerank_size=2048, masks=masks)
