@@ -229,3 +229,15 @@ def test_pair_ranks_above_16384_postings(nq):
         dev = DeviceIndex(ix)
         q = rng.normal(size=(nq, 8)).astype(np.float32)
         compare(dev, ix, q, 10, 64)
+
+
+@pytest.mark.parametrize("k", [10, 100])
+def test_candidate_lists_in_register_band(k):
+    """About 10,000 candidates per query: the exact select's 4,097 - 24,576
+    band (96 keys per thread in registers)."""
+    rng = np.random.default_rng(14)
+    x = rng.normal(size=(20000, 24)).astype(np.float32)
+    ix = build_index(x, "l2", None, n_postings=16, seed=0)
+    dev = DeviceIndex(ix)
+    q = rng.normal(size=(24, 24)).astype(np.float32)
+    compare(dev, ix, q, k, 8)
