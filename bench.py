@@ -259,8 +259,8 @@ def main() -> int:
     ap.add_argument("--budget-gib", type=float, default=16.0, help="c5: working-memory budget")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not "
-                          "supported (the sharding design is described in DESIGN.md)"}))
+        print(json.dumps({"error": "runs on one GPU; --gpus above 1 is not supported "
+                          "(sharding queries or postings over GPUs is a design, not built)"}))
         return 2
 
     import torch
