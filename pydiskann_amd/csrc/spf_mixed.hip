@@ -1209,12 +1209,13 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
                                         L.ent, nA);
     rerank_list(p, L.ent, na, s_qv, qn, L.ex);
     __syncthreads();
-    if (na <= kPrCount) {
+    if (na <= kPrCount) {   // ranks kept (in nzp, free until the survivor pass)
         for (uint32_t i = tid; i < na; i += 256) {
             const uint64_t v = L.ex[i];
             uint32_t cnt = 0;
             for (uint32_t j = 0; j < na; j++) cnt += L.ex[j] < v;
             if (cnt == p.k - 1) L.kth = v;
+            L.nzp[i] = cnt;
         }
     } else {
         uint32_t mA = 2;
@@ -1254,8 +1255,11 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
     const uint32_t thr = L.m[THR];
     // 3. survivors: pool members outside A whose key is at most thr (none
     // when thr is below every such key)
-    if (p.a_min && thr < p.a_min[q]) {
-        smallest_k(L.ex, na, p.k, emit);
+    if (p.a_min && thr < p.a_min[q]) {   // A's k smallest, by the ranks found above
+        for (uint32_t i = tid; i < na; i += 256) {
+            const uint32_t rk = na <= kPrCount ? L.nzp[i] : i;   // (else A is sorted)
+            if (rk < p.k) emit(rk, L.ex[i]);
+        }
         if (p.stats && tid == 0) p.stats[q] = na;
         return;
     }
