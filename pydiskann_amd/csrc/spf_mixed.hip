@@ -1064,8 +1064,9 @@ __device__ __forceinline__ uint32_t rebuild_members(RerankLds<CAP> &L, uint32_t 
         nz += m != 0u;
         cnt += __popc(m);
     }
-    uint32_t zpos = blk_excl(nz, L.w);
-    uint32_t ppos = blk_excl(cnt, L.w);
+    // one scan of both counts (each total is at most the pool size, < 2^16)
+    const uint32_t pk = blk_excl(nz | (cnt << 16), L.w);
+    uint32_t zpos = pk & 0xffffu, ppos = pk >> 16;
     for (uint32_t w = w0; w < w1 && nz; w++) {
         const uint32_t m = mask_of(w);
         if (m && zpos < CAP) {
