@@ -913,7 +913,7 @@ __device__ __forceinline__ void bitonic_lds(uint64_t *buf, uint32_t m) {
         }
 }
 
-constexpr int kRerankH = 2, kRerankUC = 8;   // rows per quad, row units per lane in flight
+constexpr int kRerankH = 2, kRerankUC = 4;   // rows per quad, row units per lane in flight
 
 __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
     __shared__ uint64_t s_keys[kMaxK];
