@@ -983,7 +983,7 @@ __global__ __launch_bounds__(256) void k_rerank_pool(const RerankPoolParams p) {
 // Dropped members cannot enter the top k, so the result is that of reranking
 // the whole pool.  Without prune every member is reranked.
 constexpr uint32_t kPrCount = 512;    // lists up to this size are ranked by counting
-constexpr int kPrH = 2, kPrUC = 8;    // rows per quad, row units per lane in flight
+constexpr int kPrH = 2, kPrUC = 4;    // rows per quad, row units per lane in flight
 
 __device__ __forceinline__ uint32_t blk_excl(uint32_t v, uint32_t *s_w) {
     uint32_t tot;
