@@ -225,6 +225,118 @@ def serve(args, queries, ix, np) -> int:
     print(json.dumps(out), flush=True)
     return 0
 
+# MI355X peaks (dense): HBM3E bandwidth, f16/bf16 matrix cores
+MFMA_F16_TFLOPS = 2500.0
+
+
+def roofline_of(args, head, dim, nq, k, ix):
+    """Roofline of the headline run's posting scan (its dominant kernel).
+
+    `achieved` = the scan's algorithmic bytes under its posting-major work
+    items (the library's profile: every item reads its chunk of codes and its
+    queries' rows once and writes one key per candidate slot) over the scan's
+    HIP-event time on the call's stream; `bound` follows from the kernel's
+    arithmetic intensity against the ridge point; the matrix-core utilisation
+    of the scan and of the probe is reported beside it."""
+    stage, prof = head["stage"], head["prof"]
+    kern = STAGE_KERNELS[args.precision]
+    if ix.quant == 3:
+        kern = {"scan": "k_rq_scan" if args.precision == "mixed" else "k_rq_search_exact",
+                "select": "k_pool_select", "rerank": "k_rerank_pruned"}
+    ms = stage["scan"]
+    alg = prof["scan_bytes"]
+    ach = alg / (ms * 1e-3) / 1e9
+    frac = ach / HBM_PEAK_GBS
+    assert 0.0 < frac <= 1.0, f"roofline fraction {frac} outside (0, 1]"
+    flops = prof["scan_flops"]
+    ai = flops / alg if alg else 0.0
+    ridge = MFMA_F16_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+    wl = f"{args.workload}_{args.precision}_{ix.n_raw // 1000}kx{dim}"
+    traffic, traffic_src = read_traffic(kern["scan"], wl)
+    out = {"bound": "hbm" if ai < ridge else "mfma", "achieved": round(ach, 1),
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(frac, 4), "traffic": traffic,
+           "kernel": kern["scan"], "kernel_ms": round(ms, 4),
+           "algorithmic_bytes_per_launch": int(alg),
+           "arithmetic_intensity_flop_per_byte": round(ai, 2),
+           "ridge_flop_per_byte": round(ridge, 1),
+           "candidates_per_launch": prof["candidates"], "traffic_source": traffic_src}
+    if flops:
+        tf = flops / (ms * 1e-3) / 1e12
+        out["scan_mfma"] = {"tflops": round(tf, 1), "peak_tflops": MFMA_F16_TFLOPS,
+                            "frac": round(tf / MFMA_F16_TFLOPS, 4), "flops": int(flops)}
+    if prof["probe_flops"] and stage["probe"] > 0:
+        tf = prof["probe_flops"] / (stage["probe"] * 1e-3) / 1e12
+        out["probe_mfma" if args.precision == "mixed" else "probe_valu"] = {
+            "kernel": "k_probe_mfma" if args.precision == "mixed" else "k_scan (dense)",
+            "ms": round(stage["probe"], 4), "tflops": round(tf, 1),
+            "peak_tflops": MFMA_F16_TFLOPS if args.precision == "mixed" else 78.6,
+            "frac": round(tf / (MFMA_F16_TFLOPS if args.precision == "mixed" else 78.6), 4)}
+    if traffic:
+        # DRAM bytes the kernel actually moved (rocprofv3 counters) over its live time
+        tgb = traffic / (ms * 1e-3) / 1e9
+        out["traffic_GBs"] = round(tgb, 1)
+        out["traffic_frac"] = round(tgb / HBM_PEAK_GBS, 4)
+    return out
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, ix, queries, k, n_probe, np):
+    """The oracle (a C restatement of Inner::search) on this host's cores: one
+    task per query like the reference's rayon par_iter, all usable cores (at
+    most 16, the host share of one GPU) and 1 core, one warm-up then the median
+    of 5 runs over a bounded query sample.  Legs: the exact f32 search of the
+    same index, and (L2 f32 index) the reference's like-for-like form of the
+    mixed path -- F16 postings of the same partition with an exact rerank of
+    the `--rerank` best (src/spfresh.rs:695-723)."""
+    import oracle
+    from pydiskann_amd import codecs
+    from pydiskann_amd.index import IndexArrays
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = max(1, min(16, nproc or 1))
+    legs = {"exact": ix}
+    if ix.quant == 0 and args.workload != "c1":
+        legs[f"f16_rerank{args.rerank}"] = IndexArrays(
+            dim=ix.dim, metric=ix.metric, quant=1, block_ids=ix.block_ids,
+            centroids=ix.centroids, post_off=ix.post_off, ids=ix.ids,
+            codes=codecs.f16_encode(ix.raw[ix.ids.astype(np.int64)]), raw=ix.raw,
+            deleted=ix.deleted, rerank_size=args.rerank, probe_ratio=ix.probe_ratio)
+    nq = queries.shape[0]
+    out = {}
+    for name, lix in legs.items():
+        o = oracle.Oracle(lix)
+        for threads, ns in ((cores, min(args.cpu_sample, nq)), (1, min(args.cpu_sample // 10, nq))):
+            qs = np.ascontiguousarray(queries[:ns])
+            o.search_batch(qs, k, n_probe, threads=threads)   # warm-up
+            ts = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                o.search_batch(qs, k, n_probe, threads=threads)
+                ts.append(time.perf_counter() - t0)
+            t = float(np.median(ts))
+            out[f"{name}_{threads}t"] = {"qps": round(ns / t, 1), "threads": threads,
+                                        "queries": ns, "median_s": round(t, 4)}
+            log(f"[bench] cpu {name} {threads} threads: {ns / t:,.1f} QPS ({ns} queries)")
+    head = f"f16_rerank{args.rerank}_{cores}t" if f"f16_rerank{args.rerank}_{cores}t" in out \
+        else f"exact_{cores}t"
+    like = "F16 postings + exact rerank of " + str(args.rerank) if head.startswith("f16") \
+        else "the exact path"
+    return {"value": out[head]["qps"], "unit": "queries/s", "cores": cores, "kind": "port",
+            "sample": f"first {out[head]['queries']} of the {nq} queries ({like}, the same "
+                      f"partition), C restatement of Inner::search, one warm-up then the "
+                      f"median of 5 runs; 1-thread legs over {min(args.cpu_sample // 10, nq)} "
+                      f"queries",
+            "nproc": nproc, "cpu_model": cpu_model(), "legs": out}
+
 
 def main() -> int:
     ap = argparse.ArgumentParser()
@@ -344,56 +456,8 @@ def main() -> int:
     if head["recall"] < RECALL_FLOOR:
         log(f"[bench] WARNING: recall@10 {head['recall']:.4f} below {RECALL_FLOOR}")
 
-    # roofline of the dominant kernel of the headline run
-    stage, prof = head["stage"], head["prof"]
-    want = max(k, args.rerank) if args.precision == "mixed" else \
-        (max(k, ix.rerank_size) if ix.quant and ix.rerank_size else k)
-    cand = prof["candidates"]
-    if args.precision == "mixed":
-        # select: 16-bit keys read, pool (entry, key) written; rerank: pool
-        # read, exact rows actually gathered, results written
-        alg = {"scan": prof["scan_code_bytes"], "select": cand * 2 + nq * want * 6,
-               "rerank": nq * want * 6 + prof["rerank_rows"] * dim * 4 + nq * k * 8}
-    else:
-        alg = {"scan": prof["scan_code_bytes"], "select": cand * 8 + nq * want * 8,
-               "rerank": nq * want * dim * 4 if ix.quant else 0}
-    kernels = {}
-    for st in ("scan", "select", "rerank"):
-        ms = stage[st]
-        if ms <= 0 or alg[st] <= 0:
-            continue
-        ach = alg[st] / (ms * 1e-3) / 1e9
-        kernels[st] = {"kernel": STAGE_KERNELS[args.precision][st], "ms": round(ms, 4),
-                       "algorithmic_bytes": int(alg[st]), "achieved_GBs": round(ach, 1)}
-    dom = max(kernels, key=lambda s: kernels[s]["ms"])
-    dk = kernels[dom]
-    wl = f"{args.workload}_{args.precision}_{ix.n_raw // 1000}kx{dim}"
-    traffic, traffic_src = read_traffic(dk["kernel"], wl)
-    roofline = {"bound": "hbm", "achieved": dk["achieved_GBs"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(dk["achieved_GBs"] / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "kernel": dk["kernel"], "kernel_ms": dk["ms"],
-                "algorithmic_bytes_per_launch": dk["algorithmic_bytes"],
-                "traffic_source": traffic_src, "stages": kernels,
-                "candidates_per_launch": cand}
-    if traffic:
-        # DRAM bytes the kernel actually moved (rocprofv3 counters) over its live time
-        tgb = traffic / (dk["ms"] * 1e-3) / 1e9
-        roofline["traffic_GBs"] = round(tgb, 1)
-        roofline["traffic_frac"] = round(tgb / HBM_PEAK_GBS, 4)
-
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-        cores = min(16, os.cpu_count() or 1)
-        nsc = min(args.cpu_sample, nq)
-        o = oracle.Oracle(ix)
-        t0 = time.perf_counter()
-        o.search_batch(queries[:nsc], k, n_probe, threads=cores)
-        dt = time.perf_counter() - t0
-        cpu = {"value": round(nsc / dt, 1), "unit": "queries/s", "cores": cores, "kind": "port",
-               "sample": f"first {nsc} of the {nq} queries, same index, C restatement of "
-                         f"Inner::search (exact path), {cores} threads"}
-        log(f"[bench] cpu oracle {nsc / dt:,.1f} QPS on {cores} threads")
+    roofline = roofline_of(args, head, dim, nq, k, ix)
+    cpu = None if args.no_cpu else cpu_baseline(args, ix, queries, k, n_probe, np)
 
     qps, total = head["qps"], head["total"]
     out = {
@@ -413,7 +477,7 @@ def main() -> int:
                    "launch_shapes": head["tune"] or "defaults"},
         "recall_at_10": round(head["recall"], 4), "recall_floor_met": head["recall"] >= RECALL_FLOOR,
         "p50_ms": round(head["p50"], 3), "p99_ms": round(head["p99"], 3),
-        "stage_ms": {s: round(v, 4) for s, v in stage.items()},
+        "stage_ms": {s: round(v, 4) for s, v in head["stage"].items()},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     if other is not None:

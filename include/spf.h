@@ -111,12 +111,26 @@ typedef struct spf_index_desc {
 #define SPF_FLAG_FULL_RERANK 2u /* MIXED: rerank every pool member exactly instead of
                                    skipping members whose proven lower bound exceeds the
                                    k-th exact distance (same results, more work) */
+#define SPF_FLAG_SYNC_PLAN 4u   /* size the candidate buffers from the exact counts, read
+                                   back with one host synchronisation mid-batch (the
+                                   default when the index-derived bound exceeds
+                                   plan_bound_mib) */
+#define SPF_FLAG_NO_GRAPH 8u    /* do not capture / replay the call's launch chain */
 
+/* Planning.  A batch's candidate count is known only after its probe.  By
+ * default the library sizes the candidate buffers from a bound derived from
+ * the index (the n_probe longest postings per query) whenever those buffers
+ * stay within plan_bound_mib: the whole batch is then enqueued without a host
+ * round trip, and a repeated call (same arguments and device buffers) replays
+ * its captured launch chain as one hipGraph.  Above the limit (Zipf-sized
+ * postings, very large batches), and for profiled or routing-graph calls, the
+ * exact counts are read back once mid-batch.  Results are the same either way. */
 typedef struct spf_search_opts {
     uint32_t precision;             /* SPF_PREC_* (default EXACT) */
     uint32_t rerank_size;           /* pool for MIXED (0 = index rerank_size, min k) */
     uint32_t flags;                 /* SPF_FLAG_* */
-    uint32_t reserved[5];
+    uint32_t plan_bound_mib;        /* limit for planning on the device (0 = 2048 MiB) */
+    uint32_t reserved[4];
 } spf_search_opts;
 
 /* stages of one batch search, in order */
@@ -139,6 +153,13 @@ typedef struct spf_profile {
     uint32_t rerank_rows;           /* MIXED: raw rows gathered by the exact rerank */
     uint32_t pool_members;          /* MIXED: candidates in the rerank pools */
     uint32_t reserved[1];
+    /* the posting scan's algorithmic device-memory bytes under its work items:
+     * every item reads its entries' codes (plus per-entry norms, tombstone
+     * bits, labels) and its queries' rows once, and writes one key per
+     * candidate slot */
+    uint64_t scan_bytes;
+    uint64_t scan_flops;            /* 2 * (padded dim) * candidates of the posting scan */
+    uint64_t probe_flops;           /* 2 * (padded dim) * queries * centroids of a dense probe */
 } spf_profile;
 
 typedef struct spf_index_info {
@@ -146,7 +167,8 @@ typedef struct spf_index_info {
     uint64_t n_entries, n_raw, n_deleted;
     uint64_t device_bytes;          /* HBM held by the index */
     uint32_t max_posting;           /* longest posting (SPFreshStats::max_posting) */
-    uint32_t reserved[3];
+    uint32_t reserved;
+    uint64_t graph_replays;         /* search calls served by replaying a captured chain */
 } spf_index_info;
 
 /* Build the device mirror of a search-ready index (all host pointers are
@@ -166,10 +188,12 @@ int spf_search_batch(spf_index *ix, const float *queries, uint32_t nq, uint32_t 
                      uint64_t *out_ids, float *out_dists, uint32_t *out_counts);
 
 /* Same, with queries and outputs already in device memory, enqueued on
- * `stream` (a hipStream_t).  The call returns after the work is enqueued and
- * the host-side planning has synchronised once; outputs are final when
+ * `stream` (a hipStream_t).  The call returns once the work is enqueued (after
+ * one mid-batch host synchronisation only when the call plans with
+ * SPF_FLAG_SYNC_PLAN semantics, see spf_search_opts); outputs are final when
  * `stream` completes.  stream NULL = the library's own stream, and the call
- * returns with the outputs final. */
+ * returns with the outputs final.  Calls on different streams may overlap;
+ * each call's working buffers stay reserved until its work completes. */
 int spf_search_batch_device(spf_index *ix, const float *d_queries, uint32_t nq, uint32_t dim,
                             uint32_t k, uint32_t n_probe, const spf_search_opts *opts,
                             uint64_t *d_out_ids, float *d_out_dists, uint32_t *d_out_counts,
@@ -195,11 +219,14 @@ int spf_search_batch_filtered_device(spf_index *ix, const float *d_queries, uint
                                      uint32_t *d_out_counts, void *stream);
 
 /* Insert routing: for each of n vectors, the reference block index of its
- * nearest live posting and the metric distance to that posting's centroid
- * (exact over all live centroids; equal distances -> lower block index).
+ * nearest live posting and the metric distance to that posting's centroid,
+ * routed the way Inner::nearest(v, 1) routes it (src/spfresh.rs:358-378): up
+ * to 4,096 live postings, or with no routing graph, exact over all live
+ * centroids (equal distances -> lower block index); above 4,096 live postings
+ * with a routing graph, the graph's beam search (its first result).
  * Replaces the parallel part of Inner::insert, `nearest(v, 1)` per vector of a
- * chunk (src/spfresh.rs:437-444, nearest :358-372).  out_block gets UINT32_MAX
- * and out_dist +inf when the index has no live posting. */
+ * chunk (src/spfresh.rs:437-444).  out_block gets UINT32_MAX and out_dist +inf
+ * when the index has no live posting. */
 int spf_nearest_postings(spf_index *ix, const float *vectors, uint32_t n, uint32_t dim,
                          uint32_t *out_block, float *out_dist);
 

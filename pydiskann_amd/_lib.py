@@ -28,6 +28,8 @@ EXPORTS = ("spf_index_create", "spf_index_destroy", "spf_index_get_info", "spf_d
            "spf_last_profile", "spf_selftest", "spf_last_error", "spf_version")
 FLAG_PROFILE = 1
 FLAG_FULL_RERANK = 2
+FLAG_SYNC_PLAN = 4
+FLAG_NO_GRAPH = 8
 STAGES = ("prep", "probe", "probe_select", "plan", "scan", "select", "rerank", "finalize")
 
 
@@ -57,21 +59,24 @@ class IndexDesc(C.Structure):
 
 class SearchOpts(C.Structure):
     _fields_ = [("precision", C.c_uint32), ("rerank_size", C.c_uint32), ("flags", C.c_uint32),
-                ("reserved", C.c_uint32 * 5)]
+                ("plan_bound_mib", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 class Profile(C.Structure):
     _fields_ = [("stage_ms", C.c_float * 8), ("candidates", C.c_uint64),
                 ("scan_code_bytes", C.c_uint64), ("probe_pairs", C.c_uint64),
                 ("scan_items", C.c_uint32), ("rerank_rows", C.c_uint32),
-                ("pool_members", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+                ("pool_members", C.c_uint32), ("reserved", C.c_uint32 * 1),
+                ("scan_bytes", C.c_uint64), ("scan_flops", C.c_uint64),
+                ("probe_flops", C.c_uint64)]
 
 
 class IndexInfo(C.Structure):
     _fields_ = [("dim", C.c_uint32), ("metric", C.c_uint32), ("quantizer", C.c_uint32),
                 ("n_postings", C.c_uint32), ("n_entries", C.c_uint64), ("n_raw", C.c_uint64),
                 ("n_deleted", C.c_uint64), ("device_bytes", C.c_uint64),
-                ("max_posting", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("max_posting", C.c_uint32), ("reserved", C.c_uint32),
+                ("graph_replays", C.c_uint64)]
 
 
 class ServerOpts(C.Structure):
@@ -177,7 +182,9 @@ def last_profile() -> dict:
     return {"stage_ms": dict(zip(STAGES, [float(x) for x in pr.stage_ms])),
             "candidates": int(pr.candidates), "scan_code_bytes": int(pr.scan_code_bytes),
             "probe_pairs": int(pr.probe_pairs), "scan_items": int(pr.scan_items),
-            "rerank_rows": int(pr.rerank_rows), "pool_members": int(pr.pool_members)}
+            "rerank_rows": int(pr.rerank_rows), "pool_members": int(pr.pool_members),
+            "scan_bytes": int(pr.scan_bytes), "scan_flops": int(pr.scan_flops),
+            "probe_flops": int(pr.probe_flops)}
 
 
 def _p(a):
@@ -299,12 +306,12 @@ class DeviceIndex:
     def search_device(self, q_ptr: int, nq: int, dim: int, k: int, n_probe: int, ids_ptr: int,
                       dists_ptr: int, counts_ptr: int, stream: int = 0,
                       precision: int = PREC_EXACT, rerank_size: int = 0,
-                      profile: bool = False, masks_ptr: int = 0) -> None:
+                      profile: bool = False, masks_ptr: int = 0, flags: int = 0) -> None:
         """Device-resident form: raw device pointers (e.g. torch .data_ptr());
         ``masks_ptr`` (u64 [nq] on the device) makes it the filtered search."""
         o = SearchOpts()
         o.precision, o.rerank_size = precision, rerank_size
-        o.flags = FLAG_PROFILE if profile else 0
+        o.flags = flags | (FLAG_PROFILE if profile else 0)
         st = C.c_void_p(stream) if stream else None
         if masks_ptr:
             check(lib().spf_search_batch_filtered_device(

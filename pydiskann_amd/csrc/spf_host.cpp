@@ -12,6 +12,7 @@
 //   rank_id    u64 per rank; raw f32 rows per rank (dpad stride) for rerank
 //   tombstones 1 bit per entry
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -48,22 +49,60 @@ int fail(int code, const std::string &msg) {
                         std::string(#x) + ": " + hipGetErrorString(_e));            \
     } while (0)
 
+// Completion of the last batch enqueued with a workspace.  A buffer that batch
+// may still read or write is freed only after the event has completed.
+// `gen` counts buffer (re)allocations: a captured launch chain holds buffer
+// addresses and stays valid only while it is unchanged.
+struct BufOwner {
+    hipEvent_t ev = nullptr;   // recorded after a batch's last launch
+    bool pending = false;      // recorded and not yet seen complete
+    uint64_t gen = 0;
+    void wait() {
+        if (pending && ev) (void)hipEventSynchronize(ev);
+        pending = false;
+    }
+};
+
+// Fixed device memory that buffers are carved from instead of allocated
+// (the no-allocation entry point): a bump pointer reset at every call.
+struct Arena {
+    uint8_t *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
 template <class T>
 struct DBuf {
     T *p = nullptr;
     size_t cap = 0;
+    BufOwner *owner = nullptr;
+    Arena *arena = nullptr;
     hipError_t ensure(size_t n) {
         if (n <= cap && p) return hipSuccess;
+        if (arena) {   // carve n elements at 256-byte alignment; nothing is freed
+            const size_t b = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
+            if (arena->used + b > arena->cap) return hipErrorOutOfMemory;
+            p = reinterpret_cast<T *>(arena->base + arena->used);
+            arena->used += b;
+            cap = n;
+            if (owner) owner->gen++;
+            return hipSuccess;
+        }
+        if (p && owner) owner->wait();
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        if (owner) owner->gen++;
         size_t nn = n + n / 4 + 64;
         hipError_t e = hipMalloc((void **)&p, nn * sizeof(T));
         if (e == hipSuccess) cap = nn;
         return e;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (!arena) {
+            if (p && owner) owner->wait();
+            if (p) (void)hipFree(p);
+        }
+        if (p && owner) owner->gen++;
         p = nullptr;
         cap = 0;
     }
@@ -87,6 +126,23 @@ struct DBuf {
     X(uint64_t, sel_tmp) X(uint64_t, probe_tmp) X(uint32_t, cand_real) X(uint32_t, blk_cnt) \
     X(uint32_t, a_min)
 
+// A captured launch chain of one search call, replayed for later calls with
+// the same key (the same arguments, buffers and launch shapes).
+struct GraphKey {
+    const spf_index *ix;
+    uint64_t ix_gen, ws_gen;
+    const void *q, *ids, *dists, *counts, *masks;
+    uint32_t nq, k, n_probe, precision, rerank, flags, plan_mib;
+    uint32_t scan_g, scan_chunk, mix_g;
+};
+struct GraphEntry {
+    GraphKey key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t used = 0;
+};
+constexpr size_t kGraphCache = 8;   // captured chains kept per workspace
+
 struct Workspace {
     hipStream_t stream = nullptr;
 #define SPF_WS_DECL(T, n) DBuf<T> n;
@@ -94,6 +150,86 @@ struct Workspace {
 #undef SPF_WS_DECL
     uint64_t *h_summary = nullptr;   // pinned: [total candidates, total items, -, graph overflows]
     hipEvent_t ev[SPF_N_STAGES + 1] = {};
+    // the last batch enqueued with these buffers and the stream it went to
+    BufOwner busy;
+    hipStream_t last_stream = nullptr;
+    Arena arena;                     // carve mode (the no-allocation entry point)
+    std::vector<GraphEntry> graphs;  // captured launch chains (not in carve mode)
+    std::vector<GraphKey> seen;      // recent uncaptured calls: a repeat gets captured
+    uint64_t graph_clock = 0;
+
+    Workspace() {
+#define SPF_WS_GUARD(T, n) n.owner = &busy;
+        SPF_WS_BUFFERS(SPF_WS_GUARD)
+#undef SPF_WS_GUARD
+    }
+    // carve every buffer from [base, base + cap) instead of allocating
+    void use_arena(uint8_t *base, size_t cap) {
+        arena.base = base;
+        arena.cap = cap;
+        arena.used = 0;
+#define SPF_WS_ARENA(T, n) n.arena = &arena; n.p = nullptr; n.cap = 0;
+        SPF_WS_BUFFERS(SPF_WS_ARENA)
+#undef SPF_WS_ARENA
+        busy.gen++;
+    }
+    // true once the last enqueued batch has completed (or there is none)
+    bool idle() {
+        if (busy.pending && hipEventQuery(busy.ev) == hipSuccess) busy.pending = false;
+        return !busy.pending;
+    }
+    // Order work about to go to stream s after the last batch's: stream
+    // order covers the same stream; another stream waits on the event.
+    hipError_t order_after_last(hipStream_t s) {
+        if (!busy.pending || s == last_stream) return hipSuccess;
+        return hipStreamWaitEvent(s, busy.ev, 0);
+    }
+    // Mark the batch just enqueued on s; its buffers stay in use until it completes.
+    hipError_t mark_enqueued(hipStream_t s) {
+        hipError_t e = hipEventRecord(busy.ev, s);
+        busy.pending = e == hipSuccess;
+        last_stream = s;
+        return e;
+    }
+    // captured chain for `key`, dropping chains whose buffers have moved
+    GraphEntry *graph_find(const GraphKey &key) {
+        for (size_t i = graphs.size(); i-- > 0;)
+            if (graphs[i].key.ws_gen != busy.gen) drop_graph(i);
+        for (GraphEntry &g : graphs)
+            if (std::memcmp(&g.key, &key, sizeof key) == 0) {
+                g.used = ++graph_clock;
+                return &g;
+            }
+        return nullptr;
+    }
+    void graph_store(const GraphKey &key, hipGraph_t g, hipGraphExec_t x) {
+        if (graphs.size() >= kGraphCache) {
+            size_t old = 0;
+            for (size_t i = 1; i < graphs.size(); i++)
+                if (graphs[i].used < graphs[old].used) old = i;
+            drop_graph(old);
+        }
+        GraphEntry e;
+        e.key = key;
+        e.graph = g;
+        e.exec = x;
+        e.used = ++graph_clock;
+        graphs.push_back(e);
+    }
+    // true if `key` was run before (uncaptured); remembers it otherwise
+    bool seen_before(const GraphKey &key) {
+        for (const GraphKey &k : seen)
+            if (std::memcmp(&k, &key, sizeof key) == 0) return true;
+        if (seen.size() >= 4 * kGraphCache) seen.erase(seen.begin());
+        seen.push_back(key);
+        return false;
+    }
+    void drop_graph(size_t i) {
+        busy.wait();   // a replay may still be queued
+        if (graphs[i].exec) (void)hipGraphExecDestroy(graphs[i].exec);
+        if (graphs[i].graph) (void)hipGraphDestroy(graphs[i].graph);
+        graphs.erase(graphs.begin() + i);
+    }
 
     // device bytes the buffers hold now
     uint64_t bytes() const {
@@ -105,12 +241,16 @@ struct Workspace {
     }
     // free the device buffers (the stream, events and pinned words stay)
     void trim() {
+        while (!graphs.empty()) drop_graph(graphs.size() - 1);
 #define SPF_WS_FREE(T, n) n.release();
         SPF_WS_BUFFERS(SPF_WS_FREE)
 #undef SPF_WS_FREE
     }
     void release() {
+        busy.wait();
         trim();
+        if (busy.ev) (void)hipEventDestroy(busy.ev);
+        busy.ev = nullptr;
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         if (h_summary) (void)hipHostFree(h_summary);
@@ -169,6 +309,8 @@ struct spf_index {
     uint32_t rq_padded = 0;
     bool rq_l2 = false;
     uint64_t device_bytes = 0;
+    uint64_t gen = 0;                   // bumped when device buffers of the index move
+    std::atomic<uint64_t> replays{0};   // calls served by a captured launch chain
     // working-memory bound inputs: top_len[j] = entries in the j longest
     // postings; posting chunk counts at the VALU and matrix-core chunk sizes
     std::vector<uint64_t> top_len;
@@ -178,6 +320,7 @@ struct spf_index {
     // matrix-core scan queries per item
     uint32_t scan_g = 8, scan_chunk = kEChunk, mix_g = kMixQ;
 
+    std::vector<uint32_t> h_post_len;   // entries per (sorted) posting
     std::vector<uint64_t> rank_id;      // ids in ascending order
     std::vector<uint32_t> rank_entry;   // entry holding each rank
     std::vector<uint32_t> h_del;        // tombstone bit per entry
@@ -233,6 +376,7 @@ std::vector<uint4> tile_rows(const uint8_t *rows, uint64_t n, uint32_t row_bytes
 Workspace *new_workspace() {
     Workspace *w = new Workspace();
     if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&w->busy.ev, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&w->h_summary, 4 * sizeof(uint64_t)) != hipSuccess) {
         w->release();
         delete w;
@@ -241,13 +385,19 @@ Workspace *new_workspace() {
     return w;
 }
 
-Workspace *acquire(spf_index *ix) {
+// A workspace for a batch going to `stream` (nullptr: the workspace's own
+// stream): one whose last batch has completed, or one whose last batch went to
+// the same caller stream (stream order protects its buffers); otherwise a new
+// one, so a batch never overwrites buffers another stream's batch still uses.
+Workspace *acquire(spf_index *ix, hipStream_t stream = nullptr) {
     {
         std::lock_guard<std::mutex> g(ix->ws_mu);
-        if (!ix->pool.empty()) {
-            Workspace *w = ix->pool.back();
-            ix->pool.pop_back();
-            return w;
+        for (size_t i = ix->pool.size(); i-- > 0;) {
+            Workspace *w = ix->pool[i];
+            if (w->idle() || (stream && w->last_stream == stream)) {
+                ix->pool.erase(ix->pool.begin() + i);
+                return w;
+            }
         }
     }
     return new_workspace();
@@ -440,6 +590,7 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         HIPCHK(upload(&ix->d_block_sorted, bs.data(), bs.size(), bytes));
     }
     HIPCHK(upload(&ix->d_post_len, post_len.data(), post_len.size(), bytes));
+    ix->h_post_len = post_len;
     {
         std::vector<uint32_t> sorted(post_len);
         std::sort(sorted.begin(), sorted.end(), std::greater<uint32_t>());
@@ -649,7 +800,7 @@ template <class Mark>
 int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_t k,
                  uint32_t npr, uint32_t want, uint64_t total, uint32_t n_items, uint64_t *d_ids,
                  float *d_dists, uint32_t *d_counts, Mark &mark, RqParams *rq, bool full_rerank,
-                 bool stats, const uint64_t *d_masks) {
+                 bool stats, const uint64_t *d_masks, const uint32_t *n_items_dev) {
     const uint32_t dpad32 = round_up(ix->dim, 32);
     HIPCHK(w->cand16.ensure(std::max<uint64_t>(total, 1)));
     HIPCHK(w->sel_bits.ensure(std::max<uint64_t>(total / 32, 1)));
@@ -662,6 +813,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     mp.item_post = w->item_post.p;
     mp.n_post = ix->P;
     mp.n_items = n_items;
+    mp.n_items_dev = n_items_dev;
     mp.post_off = ix->d_post_off;
     mp.pair_off = w->pair_off.p;
     mp.pair_q = w->pair_q.p;
@@ -829,6 +981,55 @@ inline uint64_t sel_chunks(uint64_t total) {
     return total > kSelChunk ? 2 * total / kSelChunk + 1 : 0;
 }
 
+// Candidate slots of a batch, bounded from the index alone: a query's segment
+// holds its kept probes' postings, each reserved up to the scan's alignment,
+// rounded up to 32 slots (k_plan_q); the npr longest postings bound the sum.
+uint64_t cand_slots_bound(const spf_index *ix, const Shape &z, uint64_t nq) {
+    const uint64_t npr = std::min<uint64_t>(z.npr, ix->P);
+    const uint64_t align = z.mfma_scan ? kScanAlign : 1u;
+    return nq * (ix->top_len[npr] + (align - 1) * npr + 31);
+}
+
+// Work items of a batch: posting p takes ceil(pairs_p / G) * ceil(len_p / echunk)
+// items, pairs_p <= nq, so the sum is at most (pairs / G + 1) chunks of the
+// longest posting plus every probed posting's chunks once.
+uint64_t items_bound(const spf_index *ix, const Shape &z, uint64_t nq) {
+    const uint64_t pairs = nq * z.npr;
+    const uint64_t cmax = (ix->max_len + z.echunk - 1) / z.echunk;
+    uint64_t call = ix->chunks_m;
+    for (int c = 0; c < 4; c++)
+        if (z.echunk == (256u << c)) call = ix->chunks_v[c];
+    return (pairs + z.G - 1) / z.G * cmax + std::min(call, pairs * cmax);
+}
+
+// Device bytes of the candidate-sized buffers at the given counts (no growth slack).
+uint64_t cand_buffer_bytes(const spf_index *ix, const Shape &z, uint64_t slots, uint64_t items) {
+    uint64_t b = std::max<uint64_t>(items, 1) * 4;   // item_post
+    if (z.mixed)
+        b += std::max<uint64_t>(slots, 1) * 2 + 2 * std::max<uint64_t>(slots / 32, 1) * 4;
+    else if (!ix->rq_l2)
+        b += std::max<uint64_t>(slots, 1) * 8 + sel_chunks(slots) * z.want * 8;
+    return b;
+}
+
+// Default limit of the candidate buffers' bound for planning on the device.
+constexpr uint32_t kPlanBoundMiB = 2048;
+
+// Whether a call plans on the device alone: candidate buffers sized from the
+// index-derived bounds, the work-item count read by the kernels from device
+// memory, no host round trip in the middle of the batch (so the launch chain
+// can be captured and replayed).  Profiled calls, routing-graph probes and
+// bounds above the limit plan with one host synchronisation instead.
+bool plan_on_device(const spf_index *ix, const Shape &z, uint32_t nq, const spf_search_opts *opts) {
+    if (opts && (opts->flags & (SPF_FLAG_PROFILE | SPF_FLAG_SYNC_PLAN))) return false;
+    if (z.graph || ix->P == 0) return false;
+    const uint64_t slots = cand_slots_bound(ix, z, nq), items = items_bound(ix, z, nq);
+    if (items >= 0xffffffffull) return false;
+    const uint64_t lim = (uint64_t)(opts && opts->plan_bound_mib ? opts->plan_bound_mib
+                                                                 : kPlanBoundMiB) << 20;
+    return cand_buffer_bytes(ix, z, slots, items) <= lim;
+}
+
 // Upper bound on the device bytes one search call of nq queries adds to a
 // Workspace: every buffer it ensures, at DBuf's growth rule, with the
 // candidate count bounded by nq x the npr longest postings and the work-item
@@ -897,20 +1098,13 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     if (ix->rq_l2) add(pairs, 4);
     add(P + 1, 4);               // item_cnt
     add(P + 1, 4);               // item_off
-    // items of posting p = ceil(pairs_p / G) * ceil(len_p / echunk), pairs_p <= nq
-    const uint64_t cmax = (ix->max_len + z.echunk - 1) / z.echunk;
-    uint64_t call = ix->chunks_m;
-    for (int c = 0; c < 4; c++)
-        if (z.echunk == (256u << c)) call = ix->chunks_v[c];
-    const uint64_t items = (pairs + z.G - 1) / z.G * cmax + std::min(call, pairs * cmax);
+    const uint64_t items = items_bound(ix, z, q);
     if (candidates) add(std::max<uint64_t>(items, 1), 4);   // item_post
     if (ix->rq_l2) {
         add(pairs * 4 * (ix->rq_padded / 64), 8);
         add(pairs * 8, 4);
     }
-    // (+ up to kScanAlign - 1 alignment slots per probe for the matrix-core scan)
-    const uint64_t total = q * ix->top_len[std::min<uint64_t>(npr, P)] +
-                           (z.mfma_scan ? (uint64_t)(kScanAlign - 1) * pairs : 0);
+    const uint64_t total = cand_slots_bound(ix, z, q);
     if (z.mixed) {
         if (candidates) {
             add(std::max<uint64_t>(total, 1), 2);       // cand16
@@ -1042,10 +1236,10 @@ int graph_probe(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_
 // with that code before anything is allocated.
 using Reserve = std::function<int(uint64_t)>;
 
-int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, uint32_t nq,
+int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, uint32_t nq,
                 uint32_t k, uint32_t n_probe, const spf_search_opts *opts, uint64_t *d_ids,
                 float *d_dists, uint32_t *d_counts, const uint64_t *d_masks,
-                const Reserve *reserve = nullptr) {
+                const Reserve *reserve) {
     const uint32_t P = ix->P;
     if (d_masks && !ix->d_label_e) return fail(SPF_EINVAL, "filtered search on an index without labels");
     Shape z;
@@ -1264,14 +1458,24 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(launch_scatter_pairs(pl, s));
     HIPCHK(launch_items(pl, s));
     HIPCHK(launch_excl_scan_u32(w->item_cnt.p, w->item_off.p, P, s));
-    w->h_summary[1] = 0;
-    HIPCHK(hipMemcpyAsync(&w->h_summary[0], w->cand_off.p + nq, sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&w->h_summary[1], w->item_off.p + P, sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    const uint64_t total = w->h_summary[0];
-    const uint32_t n_items = (uint32_t)w->h_summary[1];
+    uint64_t total;
+    uint32_t n_items;
+    if (plan_on_device(ix, z, nq, opts)) {
+        // buffers and grids at the bounds; the kernels read the item count
+        total = cand_slots_bound(ix, z, nq);
+        n_items = (uint32_t)items_bound(ix, z, nq);
+    } else {
+        if (!w->h_summary) return fail(SPF_EINVAL, "this call needs a host round trip");
+        w->h_summary[1] = 0;
+        HIPCHK(hipMemcpyAsync(&w->h_summary[0], w->cand_off.p + nq, sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&w->h_summary[1], w->item_off.p + P, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        total = w->h_summary[0];
+        n_items = (uint32_t)w->h_summary[1];
+    }
+    const uint32_t *n_items_dev = w->item_off.p + P;
     if (reserve) {   // growth of the candidate-sized buffers, at DBuf's rule
         uint64_t extra = 0;
         auto grow = [&](const auto &b, uint64_t n, uint64_t size) {
@@ -1293,6 +1497,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.item_off = w->item_off.p;
     pl.item_post = w->item_post.p;
     pl.n_items = n_items;
+    pl.n_items_dev = n_items_dev;
     HIPCHK(launch_item_table(pl, s));
 
     RqParams rq{};
@@ -1322,6 +1527,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         rq.k = k;
         rq.metric = ix->metric;
         rq.n_items = n_items;
+        rq.n_items_dev = n_items_dev;
         rq.item_post = w->item_post.p;
         rq.item_off = w->item_off.p;
         rq.pair_off = w->pair_off.p;
@@ -1333,7 +1539,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     if (mixed) {
         int rc = search_mixed(ix, w, s, nq, k, npr, want, total, n_items, d_ids, d_dists,
                               d_counts, mark, ix->rq_l2 ? &rq : nullptr,
-                              opts->flags & SPF_FLAG_FULL_RERANK, prof, d_masks);
+                              opts->flags & SPF_FLAG_FULL_RERANK, prof, d_masks, n_items_dev);
         if (rc) return rc;
     } else if (ix->rq_l2) {
         // search_rabitq: per-pair query prep, then one wave per query
@@ -1372,6 +1578,7 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         sp.item_post = w->item_post.p;
         sp.n_post = P;
         sp.n_items = n_items;
+        sp.n_items_dev = n_items_dev;
         sp.post_off = ix->d_post_off;
         sp.pair_off = w->pair_off.p;
         sp.pair_q = w->pair_q.p;
@@ -1440,6 +1647,46 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                     : mixed ? 2ull * ix->dim : (uint64_t)ix->code_bytes);
         g_prof.probe_pairs = (uint64_t)nq * P;
         g_prof.scan_items = n_items;
+        {
+            // the scan's algorithmic bytes: per posting, every work item reads
+            // its chunk of entries (codes + per-entry words) and its queries'
+            // rows once; one key per candidate slot is written
+            std::vector<uint32_t> poff(P + 1);
+            HIPCHK(hipMemcpy(poff.data(), w->pair_off.p, (P + 1) * 4, hipMemcpyDeviceToHost));
+            const double tomb = 1.0 / 8, lab = d_masks ? 1.0 : 0.0;
+            double code = 0, qrow = 0, key = 0;
+            if (mixed && !ix->rq_l2) {
+                code = ix->units16 * 16.0 + 4 + tomb + lab;     // f16 code, |x16|^2
+                qrow = dpad32 * 2.0 + 4 + (d_masks ? 8 : 0);     // f16 row, |q16|^2
+                key = 2;
+            } else if (ix->rq_l2) {
+                code = ix->units * 16.0 + tomb + lab;            // bit words, |r|^2, factor
+                qrow = 4.0 * (ix->rq_padded / 64) * 8 + 32;      // 4-bit planes, constants
+                key = mixed ? 2 : 0;
+            } else {
+                code = ix->units * 16.0 + 4 + tomb + lab;        // code, rank
+                if (ix->d_xnorm) code += 4;
+                if (ix->d_xsq) code += 4;
+                qrow = dpad * 4.0 + (q8 ? dpad : 0) + 8;
+                key = 8;
+            }
+            double b = 0;
+            if (ix->rq_l2 && !mixed) {
+                // search_rabitq: each query's wave reads every probed entry's code
+                b = (double)real * code;
+            } else {
+                for (uint32_t p = 0; p < P; p++) {
+                    const uint64_t np = poff[p + 1] - poff[p], len = ix->h_post_len[p];
+                    if (!np || !len) continue;
+                    const uint64_t groups = (np + G - 1) / G, chunks = (len + echunk - 1) / echunk;
+                    b += (double)groups * len * code + (double)chunks * np * qrow;
+                }
+                b += (double)total * key;
+            }
+            g_prof.scan_bytes = (uint64_t)b;
+            g_prof.scan_flops = ix->rq_l2 ? 0 : 2ull * (mixed ? dpad32 : dpad) * real;
+            g_prof.probe_flops = z.graph ? 0 : 2ull * (mprobe ? dpad32 : dpad) * nq * P;
+        }
         if (mixed) {   // per-query counts, summed here
             std::vector<uint32_t> rows(nq), pool(nq);
             HIPCHK(hipMemcpy(rows.data(), w->rr_stats.p, nq * sizeof(uint32_t),
@@ -1456,6 +1703,76 @@ int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         }
         g_prof_valid = true;
     }
+    return SPF_OK;
+}
+
+// One search call: the launch chain of search_body, replayed from a captured
+// graph when the same call (arguments, buffers, launch shapes) was captured
+// before on this workspace.  A chain is captured after its first eager run,
+// when it plans on the device and the caller's stream is not itself being
+// captured (then the caller's graph records the launches).
+int search_impl(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, uint32_t nq,
+                uint32_t k, uint32_t n_probe, const spf_search_opts *opts, uint64_t *d_ids,
+                float *d_dists, uint32_t *d_counts, const uint64_t *d_masks,
+                const Reserve *reserve = nullptr) {
+    Shape z;
+    if (int rc = search_shape(ix, k, n_probe, opts, &z)) return rc;
+    bool capture = !w->arena.base && s && !(opts && (opts->flags & SPF_FLAG_NO_GRAPH)) &&
+                   plan_on_device(ix, z, nq, opts);
+    if (capture) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        capture = hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone;
+    }
+    GraphKey key;
+    std::memset(&key, 0, sizeof key);
+    if (capture) {
+        key.ix = ix;
+        key.ix_gen = ix->gen;
+        key.ws_gen = w->busy.gen;
+        key.q = d_q;
+        key.ids = d_ids;
+        key.dists = d_dists;
+        key.counts = d_counts;
+        key.masks = d_masks;
+        key.nq = nq;
+        key.k = k;
+        key.n_probe = n_probe;
+        key.precision = opts ? opts->precision : 0;
+        key.rerank = opts ? opts->rerank_size : 0;
+        key.flags = opts ? opts->flags : 0;
+        key.plan_mib = opts ? opts->plan_bound_mib : 0;
+        key.scan_g = ix->scan_g;
+        key.scan_chunk = ix->scan_chunk;
+        key.mix_g = ix->mix_g;
+        if (GraphEntry *g = w->graph_find(key)) {
+            HIPCHK(hipGraphLaunch(g->exec, s));
+            ix->replays++;
+            return SPF_OK;
+        }
+    }
+    int rc = search_body(ix, w, s, d_q, nq, k, n_probe, opts, d_ids, d_dists, d_counts, d_masks,
+                         reserve);
+    if (rc || !capture) return rc;
+    // capture on the second sight of a call, so one-off shapes cost nothing
+    key.ws_gen = w->busy.gen;
+    if (!w->seen_before(key)) return SPF_OK;
+    // capture the chain just run; its buffers are in place, so recording it
+    // allocates nothing.  Any failure only leaves the call uncaptured.
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return SPF_OK;
+    const std::string err = g_err;
+    const int rc2 = search_body(ix, w, s, d_q, nq, k, n_probe, opts, d_ids, d_dists, d_counts,
+                                d_masks, nullptr);
+    hipGraph_t graph = nullptr;
+    const hipError_t ee = hipStreamEndCapture(s, &graph);
+    g_err = err;
+    hipGraphExec_t exec = nullptr;
+    if (rc2 == SPF_OK && ee == hipSuccess && graph && w->busy.gen == key.ws_gen &&
+        hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+        w->graph_store(key, graph, exec);
+    } else if (graph) {
+        (void)hipGraphDestroy(graph);
+    }
+    (void)hipGetLastError();
     return SPF_OK;
 }
 
@@ -1696,6 +2013,7 @@ int spf_index_get_info(const spf_index *ix, spf_index_info *o) {
     o->n_deleted = ix->n_deleted;
     o->device_bytes = ix->device_bytes;
     o->max_posting = ix->max_len;
+    o->graph_replays = ix->replays.load();
     return SPF_OK;
 }
 
@@ -1730,11 +2048,19 @@ int spf_search_batch_filtered_device(spf_index *ix, const float *d_queries, uint
     if (rc) return rc;
     if (nq == 0) return SPF_OK;
     DeviceGuard dg(ix->device);
-    Workspace *w = acquire(ix);
+    Workspace *w = acquire(ix, (hipStream_t)stream);
     if (!w) return fail(SPF_EDEVICE, "workspace creation failed");
     hipStream_t s = stream ? (hipStream_t)stream : w->stream;
+    if (hipError_t e = w->order_after_last(s)) {
+        release_ws(ix, w);
+        return fail(SPF_EDEVICE, std::string("hipStreamWaitEvent: ") + hipGetErrorString(e));
+    }
     rc = search_impl(ix, w, s, d_queries, nq, k, n_probe, opts, d_out_ids, d_out_dists,
                      d_out_counts, d_masks);
+    // the workspace's buffers stay in use until the work just enqueued on s completes
+    const hipError_t me = w->mark_enqueued(s);
+    if (rc == SPF_OK && me != hipSuccess)
+        rc = fail(SPF_EDEVICE, std::string("hipEventRecord: ") + hipGetErrorString(me));
     if (rc == SPF_OK && !stream && hipStreamSynchronize(s) != hipSuccess)
         rc = fail(SPF_EDEVICE, "stream synchronize failed");
     release_ws(ix, w);

@@ -65,7 +65,8 @@ struct ScanParams {
     const uint32_t *item_off;   // [n_post + 1]
     const uint32_t *item_post;  // [n_items] posting of each work item
     uint32_t n_post;
-    uint32_t n_items;
+    uint32_t n_items;           // grid bound on the items (the count itself when n_items_dev is null)
+    const uint32_t *n_items_dev;   // device count of the items (null = n_items)
     const uint32_t *post_off;   // [n_post + 1] entry offsets
     const uint32_t *pair_off;   // [n_post + 1] (query, slot) pairs per posting
     const uint32_t *pair_q;
@@ -158,7 +159,8 @@ struct ProbeMfmaParams {
 struct MixedParams {
     const uint32_t *item_off;   // [n_post + 1]
     const uint32_t *item_post;  // [n_items]
-    uint32_t n_post, n_items;
+    uint32_t n_post, n_items;   // n_items: bound, or the count when n_items_dev is null
+    const uint32_t *n_items_dev;
     const uint32_t *post_off;   // [n_post + 1]
     const uint32_t *pair_off;   // [n_post + 1]
     const uint32_t *pair_q;
@@ -244,7 +246,8 @@ struct RqParams {
     uint64_t *out;              // [nq][k]
     uint32_t *out_cnt;
     // mixed scan over posting-major items
-    uint32_t n_items;
+    uint32_t n_items;           // bound, or the count when n_items_dev is null
+    const uint32_t *n_items_dev;
     const uint32_t *item_post, *item_off, *pair_off, *pair_src;
     const uint64_t *pair_slot;
     uint16_t *out16;
@@ -321,7 +324,8 @@ struct PlanParams {
     uint32_t *item_cnt;         // [n_post + 1]
     const uint32_t *item_off;   // [n_post + 1]
     uint32_t *item_post;        // [n_items]
-    uint32_t n_items;           // item table size (known after the item scan)
+    uint32_t n_items;           // item table size, or a bound on it when n_items_dev is set
+    const uint32_t *n_items_dev;   // device count (item_off[n_post]); null = n_items
     uint32_t G;                 // queries per work item
     uint32_t echunk;            // entries per work item
     uint32_t align;             // each probe's candidate slots start at a multiple of this

@@ -156,7 +156,8 @@ __global__ __launch_bounds__(256) void k_scan(const ScanParams p) {
     constexpr bool INT_L2 = (CK == CK_U8) && (MET == M_L2 || MET == M_L2SQ);
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t dq4 = p.dpad >> 2;   // float4 (or u32 words) per staged query row
-    const uint32_t n_items = DENSE ? ((p.nq + G - 1) / G) * p.n_echunks : p.n_items;
+    const uint32_t n_items = DENSE ? ((p.nq + G - 1) / G) * p.n_echunks
+                             : (p.n_items_dev ? *p.n_items_dev : p.n_items);
 
     for (XcdWalk it(n_items); it.more(); it.next()) {
         const uint32_t item = it.item;
@@ -771,7 +772,7 @@ __global__ void k_items(const PlanParams p) {
 // item_off[b] <= i < item_off[b + 1] (a long posting's items spread over threads)
 __global__ void k_item_table(const PlanParams p) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n_items) return;
+    if (i >= (p.n_items_dev ? *p.n_items_dev : p.n_items)) return;
     uint32_t lo = 0, hi = p.n_post;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) / 2;

@@ -95,7 +95,8 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 15, kg = lane >> 4;
     const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
-    for (XcdWalk it(p.n_items); it.more(); it.next()) {
+    const uint32_t n_items = p.n_items_dev ? *p.n_items_dev : p.n_items;
+    for (XcdWalk it(n_items); it.more(); it.next()) {
         const uint32_t item = it.item;
         const uint32_t pp = p.item_post[item];
         const uint32_t local = item - p.item_off[pp];

@@ -331,7 +331,8 @@ __global__ __launch_bounds__(256) void k_rq_scan(const RqParams p) {
     __shared__ uint64_t s_mask[kRqG];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t P = p.padded;
-    for (XcdWalk it(p.n_items); it.more(); it.next()) {
+    const uint32_t n_items = p.n_items_dev ? *p.n_items_dev : p.n_items;
+    for (XcdWalk it(n_items); it.more(); it.next()) {
         const uint32_t item = it.item;
         const uint32_t pp = p.item_post[item];
         const uint32_t local = item - p.item_off[pp];
