@@ -218,6 +218,26 @@ int spf_search_batch_filtered_device(spf_index *ix, const float *d_queries, uint
                                      uint64_t *d_out_ids, float *d_out_dists,
                                      uint32_t *d_out_counts, void *stream);
 
+/* No-allocation form of spf_search_batch_filtered_device (d_masks NULL =
+ * unfiltered).  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  It calls no hipMalloc,
+ * hipFree or other allocating routine: every working buffer is carved from a
+ * fixed 64 MiB device array of the library (one per GPU, held until the
+ * process ends), and the batch is planned on the device.  A batch whose
+ * working-memory bound exceeds 64 MiB is refused with SPF_ENOMEM (split it);
+ * profiling (SPF_EINVAL) and routing-graph probes (SPF_EUNSUPPORTED) are not
+ * available.  All calls on one GPU share that array and each call overwrites
+ * it: calls whose work overlaps on one GPU (different streams, or a call
+ * enqueued before the previous call's work on another stream completed) race
+ * and give wrong results; calls on one stream are ordered and safe.  Returns
+ * once the work is enqueued on `stream` (NULL = the null stream). */
+int spf_search_batch_device_noalloc(spf_index *ix, const float *d_queries, uint32_t nq,
+                                    uint32_t dim, uint32_t k, uint32_t n_probe,
+                                    const spf_search_opts *opts, const uint64_t *d_masks,
+                                    uint64_t *d_out_ids, float *d_out_dists,
+                                    uint32_t *d_out_counts, void *stream);
+
 /* Insert routing: for each of n vectors, the reference block index of its
  * nearest live posting and the metric distance to that posting's centroid,
  * routed the way Inner::nearest(v, 1) routes it (src/spfresh.rs:358-378): up

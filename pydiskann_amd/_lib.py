@@ -22,7 +22,8 @@ PREC_EXACT, PREC_MIXED = 0, 1
 # every entry point of include/spf.h
 EXPORTS = ("spf_index_create", "spf_index_destroy", "spf_index_get_info", "spf_delete",
            "spf_search_batch", "spf_search_batch_device", "spf_search_batch_filtered",
-           "spf_search_batch_filtered_device", "spf_nearest_postings", "spf_server_create",
+           "spf_search_batch_filtered_device", "spf_search_batch_device_noalloc",
+           "spf_nearest_postings", "spf_server_create",
            "spf_server_submit", "spf_server_wait", "spf_server_get_stats", "spf_server_destroy",
            "spf_autotune",
            "spf_last_profile", "spf_selftest", "spf_last_error", "spf_version")
@@ -142,6 +143,9 @@ def lib():
         L.spf_search_batch_filtered_device.argtypes = [P, P, u32, u32, u32, u32,
                                                        C.POINTER(SearchOpts), P, P, P, P, P]
         L.spf_search_batch_filtered_device.restype = C.c_int
+        L.spf_search_batch_device_noalloc.argtypes = [P, P, u32, u32, u32, u32,
+                                                      C.POINTER(SearchOpts), P, P, P, P, P]
+        L.spf_search_batch_device_noalloc.restype = C.c_int
         L.spf_nearest_postings.argtypes = [P, P, u32, u32, P, P]
         L.spf_nearest_postings.restype = C.c_int
         L.spf_server_create.argtypes = [P, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]
@@ -323,6 +327,21 @@ class DeviceIndex:
                                                 C.byref(o), C.c_void_p(ids_ptr),
                                                 C.c_void_p(dists_ptr), C.c_void_p(counts_ptr),
                                                 st))
+
+
+    def search_device_noalloc(self, q_ptr: int, nq: int, dim: int, k: int, n_probe: int,
+                              ids_ptr: int, dists_ptr: int, counts_ptr: int, stream: int = 0,
+                              precision: int = PREC_EXACT, rerank_size: int = 0,
+                              masks_ptr: int = 0, flags: int = 0) -> None:
+        """spf_search_batch_device_noalloc: the device-resident search with
+        its working buffers carved from the library's fixed device scratch."""
+        o = SearchOpts()
+        o.precision, o.rerank_size, o.flags = precision, rerank_size, flags
+        check(lib().spf_search_batch_device_noalloc(
+            self.h, C.c_void_p(q_ptr), nq, dim, k, n_probe, C.byref(o),
+            C.c_void_p(masks_ptr) if masks_ptr else None, C.c_void_p(ids_ptr),
+            C.c_void_p(dists_ptr), C.c_void_p(counts_ptr),
+            C.c_void_p(stream) if stream else None))
 
 
 class Ticket:

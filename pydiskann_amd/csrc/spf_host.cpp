@@ -1980,6 +1980,18 @@ struct ServerExec {
 
 }  // namespace
 
+// No-allocation search: the workspace of each device carves its buffers from
+// the device's fixed scratch array (spf_noalloc.hip); one host thread at a
+// time enqueues into it.
+namespace {
+constexpr int kNoAllocDevices = 64;
+struct NoAllocSlot {
+    std::mutex mu;
+    Workspace w;   // no stream, events or pinned memory of its own
+};
+NoAllocSlot g_noalloc[kNoAllocDevices];
+}  // namespace
+
 struct spf_server {
     ServerExec ex;
     std::unique_ptr<spf_sched::Scheduler<ServerExec>> sch;
@@ -2064,6 +2076,56 @@ int spf_search_batch_filtered_device(spf_index *ix, const float *d_queries, uint
     if (rc == SPF_OK && !stream && hipStreamSynchronize(s) != hipSuccess)
         rc = fail(SPF_EDEVICE, "stream synchronize failed");
     release_ws(ix, w);
+    return rc;
+}
+
+int spf_search_batch_device_noalloc(spf_index *ix, const float *d_queries, uint32_t nq,
+                                    uint32_t dim, uint32_t k, uint32_t n_probe,
+                                    const spf_search_opts *opts, const uint64_t *d_masks,
+                                    uint64_t *d_out_ids, float *d_out_dists,
+                                    uint32_t *d_out_counts, void *stream) {
+    int rc = check_search_args(ix, dim, k);
+    if (rc) return rc;
+    if (nq == 0) return SPF_OK;
+    if (!d_queries || !d_out_ids || !d_out_dists || !d_out_counts)
+        return fail(SPF_EINVAL, "null buffer");
+    if (ix->device < 0 || ix->device >= kNoAllocDevices)
+        return fail(SPF_EUNSUPPORTED, "device ordinal beyond the no-allocation slots");
+    if (opts && (opts->flags & SPF_FLAG_PROFILE))
+        return fail(SPF_EINVAL, "profiling needs events: not available without allocation");
+    if (d_masks && !ix->d_label_e) return fail(SPF_EINVAL, "filtered search on an index without labels");
+    // plan on the device whatever the bound (nothing is read back), no capture
+    spf_search_opts o = opts ? *opts : spf_search_opts{};
+    o.flags = (o.flags & ~SPF_FLAG_SYNC_PLAN) | SPF_FLAG_NO_GRAPH;
+    o.plan_bound_mib = 0xffffffffu;
+    Shape z;
+    if ((rc = search_shape(ix, k, n_probe, &o, &z))) return rc;
+    if (z.graph)
+        return fail(SPF_EUNSUPPORTED, "the routing-graph probe needs a host round trip");
+    if (ix->P && items_bound(ix, z, nq) >= 0xffffffffull)
+        return fail(SPF_ENOMEM, "batch too large for the no-allocation scratch");
+    // every buffer at DBuf's growth rule bounds its exact carve; 256-byte
+    // alignment adds under 64 KiB over all buffers
+    const uint64_t need = ws_bound(ix, z, nq, k, false, d_masks != nullptr) + (64u << 10);
+    if (need > kNoAllocScratchBytes) {
+        char m[192];
+        std::snprintf(m, sizeof m,
+                      "batch needs up to %llu bytes of working memory, above the %llu-byte "
+                      "no-allocation scratch; use fewer queries",
+                      (unsigned long long)need, (unsigned long long)kNoAllocScratchBytes);
+        return fail(SPF_ENOMEM, m);
+    }
+    DeviceGuard dg(ix->device);
+    void *base = nullptr;
+    if (hipError_t e = noalloc_scratch(ix->device, &base))
+        return fail(SPF_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    NoAllocSlot &slot = g_noalloc[ix->device];
+    std::lock_guard<std::mutex> g(slot.mu);
+    slot.w.use_arena(static_cast<uint8_t *>(base), kNoAllocScratchBytes);
+    rc = search_impl(ix, &slot.w, (hipStream_t)stream, d_queries, nq, k, n_probe, &o, d_out_ids,
+                     d_out_dists, d_out_counts, d_masks);
+    if (rc == SPF_ENOMEM)   // a carve beyond the scratch (not expected after the bound check)
+        return fail(SPF_ENOMEM, "batch exceeds the no-allocation scratch");
     return rc;
 }
 

@@ -405,4 +405,11 @@ hipError_t launch_rq_search_exact(const RqParams &p, hipStream_t s);
 hipError_t launch_rq_decode(const RqDecodeParams &p, uint32_t n, hipStream_t s);
 hipError_t launch_rq_scan(const RqParams &p, uint32_t grid, hipStream_t s);
 
+// Working memory of the no-allocation search: a file-scope device array of
+// this size (spf_noalloc.hip), one per device.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+// Device address of the scratch array on `device` (which must be current),
+// looked up once per device.
+hipError_t noalloc_scratch(int device, void **ptr);
+
 }  // namespace spf
