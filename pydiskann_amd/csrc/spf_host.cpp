@@ -124,7 +124,7 @@ struct DBuf {
     X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)        \
     X(uint32_t, gflag) X(uint32_t, gscr) X(uint32_t, sel_ccnt) X(uint32_t, sel_coff)            \
     X(uint64_t, sel_tmp) X(uint64_t, probe_tmp) X(uint32_t, cand_real) X(uint32_t, blk_cnt) \
-    X(uint32_t, a_min)
+    X(uint32_t, a_min) X(uint8_t, q8m) X(float, q8a) X(float, q8b)
 
 // A captured launch chain of one search call, replayed for later calls with
 // the same key (the same arguments, buffers and launch shapes).
@@ -546,13 +546,18 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         HIPCHK(upload(&ix->d_cent, ct.data(), ct.size(), bytes));
     }
     {
-        // bf16 probe operands: c - mu (mu = mean centroid) keeps the products small
+        // bf16 probe operands.  L2: c - mu (mu = mean centroid) keeps the
+        // products small and the key is |cb|^2 + |qb|^2 - 2 qb.cb; inner
+        // product and cosine take c itself (a shift would change q.c), the
+        // cosine key also 1 / |c|.
+        const bool l2m = d->metric == SPF_METRIC_L2 || d->metric == SPF_METRIC_L2SQ;
         const uint32_t dpad32 = round_up(d->dim, 32);
         std::vector<double> mu(d->dim, 0.0);
-        for (uint32_t i = 0; i < P; i++)
-            for (uint32_t j = 0; j < d->dim; j++) mu[j] += d->centroids[(size_t)perm[i] * d->dim + j];
+        if (l2m)
+            for (uint32_t i = 0; i < P; i++)
+                for (uint32_t j = 0; j < d->dim; j++) mu[j] += d->centroids[(size_t)perm[i] * d->dim + j];
         std::vector<float> muf(ix->dpad, 0.0f);
-        for (uint32_t j = 0; j < d->dim; j++) muf[j] = P ? (float)(mu[j] / P) : 0.0f;
+        for (uint32_t j = 0; j < d->dim; j++) muf[j] = P && l2m ? (float)(mu[j] / P) : 0.0f;
         auto bf16_rne = [](float f) -> uint16_t {
             uint32_t u;
             std::memcpy(&u, &f, 4);
@@ -564,6 +569,7 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         for (uint32_t i = 0; i < P; i++) {
             const float *c = d->centroids + (size_t)perm[i] * d->dim;
             float acc = 0.0f;
+            double c2 = 0.0;
             for (uint32_t j = 0; j < d->dim; j++) {
                 const uint16_t h = bf16_rne(c[j] - muf[j]);
                 cb[(size_t)i * dpad32 + j] = h;
@@ -571,9 +577,11 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
                 float v;
                 std::memcpy(&v, &u, 4);
                 acc += v * v;
+                c2 += (double)c[j] * c[j];
                 rows[(size_t)i * ix->dpad + j] = c[j];
             }
-            cn[i] = acc;
+            cn[i] = l2m ? acc : d->metric == SPF_METRIC_COSINE ? (c2 > 0 ? (float)(1.0 / std::sqrt(c2)) : 0.0f)
+                                                                : 0.0f;
         }
         ix->cbf_units = dpad32 / 8;
         std::vector<uint4> ct = tile_rows((const uint8_t *)cb.data(), P, 2 * dpad32, ix->cbf_units,
@@ -771,8 +779,9 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         }
     }
     HIPCHK(upload(&ix->d_del, ix->h_del.data(), ix->h_del.size(), bytes));
-    // norms for cosine, u8 squares for int8 L2
-    if (d->metric == SPF_METRIC_COSINE) {
+    // norms for cosine (and |decode(c)|^2 of int8 codes for the matrix-core
+    // scan), u8 squares for int8 L2
+    if (d->metric == SPF_METRIC_COSINE || ix->code_kind == CK_U8) {
         HIPCHK(hipMalloc((void **)&ix->d_xnorm, std::max<uint64_t>(N, 1) * 4));
         HIPCHK(hipMalloc((void **)&ix->d_cnorm, std::max<uint32_t>(P, 1) * 4));
         bytes += 4 * (N + P);
@@ -793,6 +802,17 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
     return SPF_OK;
 }
 
+// Launch shape of one search call: derived from the index and the options
+// alone (not from the queries), so the working-memory bound can use it too.
+struct Shape {
+    uint32_t npr, want, dpad, dpad32, Gv, G, echunk, band, Pp;
+    bool mixed, rerank, mfma_scan, mprobe, round16, q8;
+    bool int8;       // matrix-core scan of int8 codes (int8 MFMA, 64-dim K steps)
+    uint32_t dpad64, mks;   // dim rounded up to 64; K steps of the matrix-core scan
+    bool graph;   // probe through the routing graph (above kBruteCentroids live postings)
+    uint32_t nraw;   // the probe count as asked (the graph's result count and beam)
+};
+
 // Matrix-core (or RaBitQ estimator) scan -> 16-bit candidate keys -> per
 // query pool select -> bound-pruned exact rerank -> ids.  Runs after probe
 // and planning.
@@ -800,7 +820,8 @@ template <class Mark>
 int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_t k,
                  uint32_t npr, uint32_t want, uint64_t total, uint32_t n_items, uint64_t *d_ids,
                  float *d_dists, uint32_t *d_counts, Mark &mark, RqParams *rq, bool full_rerank,
-                 bool stats, const uint64_t *d_masks, const uint32_t *n_items_dev) {
+                 bool stats, const uint64_t *d_masks, const uint32_t *n_items_dev,
+                 const Shape &z) {
     const uint32_t dpad32 = round_up(ix->dim, 32);
     HIPCHK(w->cand16.ensure(std::max<uint64_t>(total, 1)));
     HIPCHK(w->sel_bits.ensure(std::max<uint64_t>(total / 32, 1)));
@@ -827,8 +848,20 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     mp.xn16 = ix->d_xn16;
     mp.del = ix->d_del;
     mp.metric = ix->metric;
-    mp.echunk = mfma_chunk(dpad32 / 32);
+    mp.echunk = z.echunk;
     mp.qg = ix->mix_g;
+    if (z.int8) {   // u8 codes against the int8 query rows
+        mp.int8 = 1;
+        mp.codes16 = ix->d_codes;
+        mp.units = ix->units;
+        mp.xn16 = ix->d_xnorm;
+        mp.q16 = (const uint4 *)w->q8m.p;
+        mp.q16_units = z.dpad64 / 16;
+        mp.ks = z.mks;
+        mp.q8a = w->q8a.p;
+        mp.q8b = w->q8b.p;
+        mp.qnorm = w->qnorm.p;
+    }
     mp.label_e = d_masks ? ix->d_label_e : nullptr;
     mp.qmask = d_masks;
     mp.out16 = w->cand16.p;
@@ -910,14 +943,6 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     return SPF_OK;
 }
 
-// Launch shape of one search call: derived from the index and the options
-// alone (not from the queries), so the working-memory bound can use it too.
-struct Shape {
-    uint32_t npr, want, dpad, dpad32, Gv, G, echunk, band, Pp;
-    bool mixed, rerank, mfma_scan, mprobe, round16, q8;
-    bool graph;   // probe through the routing graph (above kBruteCentroids live postings)
-    uint32_t nraw;   // the probe count as asked (the graph's result count and beam)
-};
 
 // Inner::nearest scans every centroid up to this many live postings and
 // searches the routing graph above it (src/spfresh.rs:20,358-378)
@@ -931,10 +956,13 @@ int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_se
     z.nraw = std::max(n_probe, 1u);
     z.mixed = opts && opts->precision == SPF_PREC_MIXED;
     if (opts && opts->precision > SPF_PREC_MIXED) return fail(SPF_EINVAL, "unknown precision");
-    if (z.mixed && !ix->rq_l2 && (!ix->d_codes16 || ix->dim > 256))
+    const bool i8 = ix->code_kind == CK_U8 && ix->quant == SPF_QUANT_INT8;
+    if (z.mixed && !ix->rq_l2 && !(ix->d_codes16 && ix->dim <= 256) && !(i8 && ix->dim <= 512))
         return fail(SPF_EUNSUPPORTED,
-                    "mixed precision serves f32 and f16 postings with dim <= 256 and RaBitQ "
-                    "postings with an L2 metric");
+                    "mixed precision serves f32 and f16 postings with dim <= 256, int8 postings "
+                    "with dim <= 512 and RaBitQ postings with an L2 metric");
+    z.int8 = z.mixed && i8;
+    z.dpad64 = round_up(ix->dim, 64);
     // RaBitQ + L2 takes search_rabitq (src/spfresh.rs:697-703): no rerank_size
     z.rerank = !z.mixed && !ix->rq_l2 && ix->quant != SPF_QUANT_NONE && ix->rerank_size > 0;
     if (ix->rq_l2 && !z.mixed && k > 1024)
@@ -957,15 +985,15 @@ int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_se
     // queries, the VALU scan Gv; both of the latter over kEChunk entries
     z.mfma_scan = z.mixed && !ix->rq_l2;
     z.G = z.mfma_scan ? ix->mix_g : z.mixed ? 8u : z.Gv;
-    z.echunk = z.mfma_scan ? mfma_chunk(z.dpad32 / 32) : (z.mixed ? kEChunk : ix->scan_chunk);
+    z.mks = z.int8 ? z.dpad64 / 64 : z.dpad32 / 32;
+    z.echunk = z.mfma_scan ? mfma_chunk(z.mks) : (z.mixed ? kEChunk : ix->scan_chunk);
     z.round16 = ix->quant == SPF_QUANT_F16 &&
                 (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
     z.q8 = ix->quant == SPF_QUANT_INT8 &&
            (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
     // matrix-core probe for L2 metrics when it has something to prune
     z.graph = ix->has_graph && P > kBruteCentroids;
-    z.mprobe = !z.graph && z.mixed && (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ) &&
-               P > z.npr && ix->dim <= 256 && z.npr + z.npr / 4 + 32 <= kMaxK;
+    z.mprobe = !z.graph && z.mixed && P > z.npr && ix->dim <= 1024 && z.npr + z.npr / 4 + 32 <= kMaxK;
     z.Pp = round_up(P, 32);
     z.band = std::min<uint32_t>(P, std::min<uint32_t>(kMaxK, std::max(z.npr + 32, z.npr + z.npr / 4)));
     return SPF_OK;
@@ -1056,6 +1084,11 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     add(q, 4);                   // sel_cnt
     if (z.mixed) {
         add(q * z.dpad32, 2);
+        add(q, 4);
+        add(q, 4);
+    }
+    if (z.int8) {
+        add(q * z.dpad64, 1);    // q8m
         add(q, 4);
         add(q, 4);
     }
@@ -1297,6 +1330,15 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         pp.qn16 = w->qn16.p;
         pp.eq16 = w->eq16.p;
     }
+    if (z.int8) {
+        HIPCHK(w->q8m.ensure((size_t)nq * z.dpad64));
+        HIPCHK(w->q8a.ensure(nq));
+        HIPCHK(w->q8b.ensure(nq));
+        pp.q8m = (int8_t *)w->q8m.p;
+        pp.dpad64 = z.dpad64;
+        pp.q8a = w->q8a.p;
+        pp.q8b = w->q8b.p;
+    }
     if (mprobe) {
         HIPCHK(w->qbf.ensure((size_t)nq * dpad32));
         HIPCHK(w->qnbf.ensure(nq));
@@ -1345,6 +1387,8 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         pm.c_units = ix->cbf_units;
         pm.n_post = P;
         pm.cnbf = ix->d_cnbf;
+        pm.metric = ix->metric;
+        pm.qnorm = w->qnorm.p;
         pm.out = w->probe16.p;
         pm.out_stride = Pp;
         HIPCHK(launch_probe_mfma(pm, s));
@@ -1539,7 +1583,7 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     if (mixed) {
         int rc = search_mixed(ix, w, s, nq, k, npr, want, total, n_items, d_ids, d_dists,
                               d_counts, mark, ix->rq_l2 ? &rq : nullptr,
-                              opts->flags & SPF_FLAG_FULL_RERANK, prof, d_masks, n_items_dev);
+                              opts->flags & SPF_FLAG_FULL_RERANK, prof, d_masks, n_items_dev, z);
         if (rc) return rc;
     } else if (ix->rq_l2) {
         // search_rabitq: per-pair query prep, then one wave per query
@@ -1655,7 +1699,11 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
             HIPCHK(hipMemcpy(poff.data(), w->pair_off.p, (P + 1) * 4, hipMemcpyDeviceToHost));
             const double tomb = 1.0 / 8, lab = d_masks ? 1.0 : 0.0;
             double code = 0, qrow = 0, key = 0;
-            if (mixed && !ix->rq_l2) {
+            if (mixed && z.int8) {
+                code = ix->units * 16.0 + 4 + tomb + lab;        // u8 code, |decode(c)|^2
+                qrow = z.dpad64 + 12.0 + (d_masks ? 8 : 0);      // int8 row, affine terms, |q|^2
+                key = 2;
+            } else if (mixed && !ix->rq_l2) {
                 code = ix->units16 * 16.0 + 4 + tomb + lab;     // f16 code, |x16|^2
                 qrow = dpad32 * 2.0 + 4 + (d_masks ? 8 : 0);     // f16 row, |q16|^2
                 key = 2;
@@ -1684,7 +1732,7 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                 b += (double)total * key;
             }
             g_prof.scan_bytes = (uint64_t)b;
-            g_prof.scan_flops = ix->rq_l2 ? 0 : 2ull * (mixed ? dpad32 : dpad) * real;
+            g_prof.scan_flops = ix->rq_l2 ? 0 : 2ull * (z.int8 ? z.dpad64 : mixed ? dpad32 : dpad) * real;
             g_prof.probe_flops = z.graph ? 0 : 2ull * (mprobe ? dpad32 : dpad) * nq * P;
         }
         if (mixed) {   // per-query counts, summed here
@@ -2193,10 +2241,10 @@ int spf_selftest(int device) {
     if (device < 0 || device >= ndev) return fail(SPF_EINVAL, "bad device ordinal");
     DeviceGuard dg(device);
     float *d = nullptr;
-    HIPCHK(hipMalloc((void **)&d, 2 * 256 * sizeof(float)));
-    std::vector<float> h(512);
-    hipError_t e = launch_selftest_mfma(d, d + 256, nullptr);
-    if (e == hipSuccess) e = hipMemcpy(h.data(), d, 512 * 4, hipMemcpyDeviceToHost);
+    HIPCHK(hipMalloc((void **)&d, 3 * 256 * sizeof(float)));
+    std::vector<float> h(768);
+    hipError_t e = launch_selftest_mfma(d, d + 256, nullptr);   // int8 product at d + 512
+    if (e == hipSuccess) e = hipMemcpy(h.data(), d, 768 * 4, hipMemcpyDeviceToHost);
     (void)hipFree(d);
     HIPCHK(e);
     for (int m = 0; m < 16; m++)
@@ -2208,6 +2256,15 @@ int spf_selftest(int device) {
                 char msg[128];
                 std::snprintf(msg, sizeof msg, "mfma 16x16x32 layout mismatch at (%d,%d): f16 %g bf16 %g want %g",
                               m, n, h[m * 16 + n], h[256 + m * 16 + n], ref);
+                return fail(SPF_EDEVICE, msg);
+            }
+            float r8 = 0.0f;
+            for (int k = 0; k < 64; k++)
+                r8 += (float)(((3 * m + 5 * k) % 7) - 3) * (float)(((2 * k + 7 * n) % 5) - 2);
+            if (h[512 + m * 16 + n] != r8) {
+                char msg[128];
+                std::snprintf(msg, sizeof msg, "mfma 16x16x64 int8 layout mismatch at (%d,%d): %g want %g",
+                              m, n, h[512 + m * 16 + n], r8);
                 return fail(SPF_EDEVICE, msg);
             }
         }

@@ -102,6 +102,11 @@ struct PrepParams {
     float *qnbf;                // [nq] |bf16(q - mu)|^2
     uint32_t *zero;             // zeroed alongside (the planner's counters; null = none)
     uint32_t zero_n;
+    // int8 matrix-core scan: the query as symmetric int8 (scale max|q| / 127)
+    // and the affine terms of q.decode(c) = a * sum q8 (c - 128) + b
+    int8_t *q8m;                // [nq][dpad64] (null = skip)
+    uint32_t dpad64;            // dim rounded up to 64
+    float *q8a, *q8b;           // [nq]
 };
 
 // Routing-graph probe (one wave per query): beam search over the centroid
@@ -142,15 +147,17 @@ inline size_t graph_lds_bytes(const GraphParams &p) {
     return (p.scratch ? 0 : graph_state_bytes(p)) + (size_t)p.dpad * 4 + 64 * 8;
 }
 
-// Matrix-core centroid probe: 16-bit keys of |c-mu|^2 + |q-mu|^2 - 2(q-mu).(c-mu)
-// in bf16 with f32 accumulate.
+// Matrix-core centroid probe in bf16 with f32 accumulate, 16-bit keys of
+// L2: |c-mu|^2 + |q-mu|^2 - 2(q-mu).(c-mu); dot: 1 - q.c; cosine: 1 - q.c / (|q||c|).
 struct ProbeMfmaParams {
     const uint4 *qbf;           // [nq][q_units] bf16 rows
     uint32_t q_units, ks, nq;
-    const float *qnbf;
-    const uint4 *cbf;           // tiled bf16 centred centroids
+    const float *qnbf;          // L2: |bf16(q - mu)|^2
+    const float *qnorm;         // cosine: exact |q|^2
+    const uint4 *cbf;           // tiled bf16 centroids (centred for L2)
     uint32_t c_units, n_post;
-    const float *cnbf;          // [n_post]
+    const float *cnbf;          // [n_post] L2: |bf16(c - mu)|^2; cosine: 1 / |c|
+    uint32_t metric;
     uint16_t *out;              // [nq][out_stride]
     uint32_t out_stride;
 };
@@ -181,6 +188,11 @@ struct MixedParams {
     const uint8_t *label_e;     // label filter: entry label, per-query mask (null = off)
     const uint64_t *qmask;
     uint16_t *out16;            // 16-bit candidate keys
+    // int8 postings (int8 MFMA, int32 sums): codes16 / units / xn16 / q16 /
+    // q16_units / ks then hold the u8 codes, |decode(c)|^2, the int8 query
+    // rows and their 64-dim K steps; per query the affine terms and |q|^2
+    uint32_t int8;
+    const float *q8a, *q8b, *qnorm;
 };
 
 // Matrix-core scan work items: (posting, `mfma_chunk(ks)` entries, up to

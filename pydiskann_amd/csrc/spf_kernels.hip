@@ -88,6 +88,35 @@ __global__ __launch_bounds__(64) void k_prep(const PrepParams p) {
         }
     }
     __syncthreads();
+    if (p.q8m) {
+        // symmetric int8 query: q ~ sq * q8 with sq = max|q| / 127; then
+        // q.decode(c) = scale * sq * (sum q8 (c - 128) + 128 sum q8) + offset * sum q
+        float m = 0.0f;
+        for (uint32_t i = lane; i < p.dim; i += 64) m = fmaxf(m, fabsf(s_v[i]));
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        const float inv = m > 0.0f ? 127.0f / m : 0.0f;
+        int q8sum = 0;
+        float qsum = 0.0f;
+        for (uint32_t i = lane; i < p.dpad64; i += 64) {
+            int v = 0;
+            if (i < p.dim) {
+                v = (int)rintf(s_v[i] * inv);
+                v = v > 127 ? 127 : (v < -127 ? -127 : v);
+                qsum += s_v[i];
+            }
+            q8sum += v;
+            p.q8m[(size_t)q * p.dpad64 + i] = (int8_t)v;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            q8sum += __shfl_xor(q8sum, off, 64);
+            qsum += __shfl_xor(qsum, off, 64);
+        }
+        if (lane == 0) {
+            const float a = p.scale * (m / 127.0f);
+            p.q8a[q] = a;
+            p.q8b[q] = a * 128.0f * (float)q8sum + p.offset * qsum;
+        }
+    }
     if (p.want_q8) {
         const float alpha = 1.0f / p.scale;
         const float beta = -p.offset / p.scale;

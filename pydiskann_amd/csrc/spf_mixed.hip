@@ -26,6 +26,13 @@ namespace spf {
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
+typedef int int32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int32x4_t as_i32x4(uint4 v) {
+    union { uint4 u; int32x4_t i; } c;
+    c.u = v;
+    return c.i;
+}
 
 __device__ __forceinline__ uint32_t fkey_m(float f) {
     uint32_t u = __float_as_uint(f);
@@ -46,7 +53,7 @@ __device__ __forceinline__ half8_t as_half8(uint4 v) {
 // A[m][k] = ((3m + 5k) % 7) - 3, B[k][n] = ((2k + 7n) % 5) - 2, D = A B (exact
 // in f16/bf16 and f32).  Each lane builds its fragments by the documented layout
 // and writes D back by the documented output layout.
-__global__ void k_selftest_mfma(float *out_f16, float *out_bf16) {
+__global__ void k_selftest_mfma(float *out_f16, float *out_bf16, float *out_i8) {
     const int l = threadIdx.x;
     half8_t a, b;
     bf16x8_t a2, b2;
@@ -68,10 +75,21 @@ __global__ void k_selftest_mfma(float *out_f16, float *out_bf16) {
         out_f16[m * 16 + n] = acc[j];
         out_bf16[m * 16 + n] = acc2[j];
     }
+    // 16x16x64 int8: lane l holds A[l%16][16*(l/16) .. +15] and B[16*(l/16) .. +15][l%16]
+    union { int8_t b[16]; int32x4_t v; } ai, bi;
+    for (int i = 0; i < 16; i++) {
+        const int m = l % 16, k = 16 * (l / 16) + i, n = l % 16;
+        ai.b[i] = (int8_t)(((3 * m + 5 * k) % 7) - 3);
+        bi.b[i] = (int8_t)(((2 * k + 7 * n) % 5) - 2);
+    }
+    int32x4_t acc3 = {0, 0, 0, 0};
+    acc3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(ai.v, bi.v, acc3, 0, 0, 0);
+    for (int j = 0; j < 4; j++) out_i8[(4 * (l / 16) + j) * 16 + l % 16] = (float)acc3[j];
 }
 
 hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t s) {
-    hipLaunchKernelGGL(k_selftest_mfma, dim3(1), dim3(64), 0, s, d_out_f16, d_out_bf16);
+    hipLaunchKernelGGL(k_selftest_mfma, dim3(1), dim3(64), 0, s, d_out_f16, d_out_bf16,
+                       d_out_f16 + 512);
     return hipGetLastError();
 }
 
@@ -82,8 +100,13 @@ hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t
 // group of the item (two 16-row A tiles per group, 16x16x32 f16 MFMA, f32
 // accumulate).  Each (query, entry) writes the 16-bit key of its distance.
 // Four workgroups per CU for the unfiltered scan up to 128 dims (the label
-// masks, or 256-dim query fragments, would spill there)
-template <int KS, bool FILT>
+// masks, or 256-dim query fragments, would spill there).
+//
+// I8: int8 postings.  A K step is 64 dims (16x16x64 int8 MFMA, int32 sums),
+// so a lane's fragment is again one 16-byte unit: u8 codes shifted to signed
+// (c - 128, by flipping the top bit of every byte) against the symmetric int8
+// query; q.decode(c) = a * sum + b with the query's affine terms (a, b).
+template <int KS, bool FILT, bool I8>
 __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(const MixedParams p) {
     constexpr int T = KS <= 4 ? 4 : 2;   // tiles per wave, mfma_tiles()
     constexpr uint32_t EW = 64 * T;      // entries per item
@@ -91,6 +114,7 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
     __shared__ uint64_t s_slot[kMixQ];
     __shared__ uint32_t s_qi[kMixQ];
     __shared__ float s_qn[kMixQ];
+    __shared__ float s_qa[I8 ? kMixQ : 1], s_qb[I8 ? kMixQ : 1];
     __shared__ uint64_t s_mask[kMixQ];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 15, kg = lane >> 4;
@@ -112,10 +136,14 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
             const uint32_t q = ok ? p.pair_q[pr0 + tid] : 0u;
             s_qi[tid] = q;
             s_slot[tid] = ok ? p.pair_slot[pr0 + tid] : 0ull;
-            float qn = ok ? p.qn16[q] : 0.0f;
+            float qn = ok ? (I8 ? p.qnorm[q] : p.qn16[q]) : 0.0f;
             if (p.metric == M_DOT) qn = 1.0f;
             else if (p.metric == M_COS) qn = qn > 0.0f ? 1.0f / sqrtf(qn) : 0.0f;
             s_qn[tid] = qn;
+            if (I8) {
+                s_qa[tid] = ok ? p.q8a[q] : 0.0f;
+                s_qb[tid] = ok ? p.q8b[q] : 0.0f;
+            }
             if (FILT) s_mask[tid] = ok ? p.qmask[q] : 0ull;
         }
         // the wave's B fragments, and per-entry norm / tombstone word / label
@@ -134,6 +162,12 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
             for (int s = 0; s < KS; s++) {
                 const uint32_t u = 4 * s + kg;
                 bf[t][s] = u < p.units ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
+                if (I8 && u < p.units) {   // u8 c -> signed c - 128
+                    bf[t][s].x ^= 0x80808080u;
+                    bf[t][s].y ^= 0x80808080u;
+                    bf[t][s].z ^= 0x80808080u;
+                    bf[t][s].w ^= 0x80808080u;
+                }
             }
             xn[t] = p.xn16[e];
             dw[t] = p.del ? p.del[e >> 5] : 0u;
@@ -153,7 +187,7 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
                 const uint32_t gb = g0 + 16 * h;
                 if (gb >= gcnt) break;   // (uniform; the half's tile rows are not stored)
                 // A fragments: 16 queries x dims
-                half8_t a[KS];
+                uint4 a[KS];
                 {
                     const uint32_t g = gb + col;
                     const bool ok = g < gcnt;
@@ -162,27 +196,40 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
                     for (int s = 0; s < KS; s++) {
                         uint4 v = make_uint4(0, 0, 0, 0);
                         if (ok && s < (int)p.ks) v = qrow[4 * s + kg];   // rows hold 4*ks units
-                        a[s] = as_half8(v);
+                        a[s] = v;
                     }
                 }
                 // the lane's four rows 4*kg + j of the half: norm factor and
                 // label mask, read once (gb + 15 < kMixQ)
-                float qv[4];
+                float qv[4], qa[4], qb[4];
                 uint64_t mv[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     qv[j] = s_qn[gb + 4 * kg + j];
                     mv[j] = FILT ? s_mask[gb + 4 * kg + j] : 0ull;
+                    qa[j] = I8 ? s_qa[gb + 4 * kg + j] : 0.0f;
+                    qb[j] = I8 ? s_qb[gb + 4 * kg + j] : 0.0f;
                 }
 #pragma unroll
                 for (int t = 0; t < T; t++) {
                     const uint32_t e0 = e_begin + 16 * (wave + 4 * t);
                     if (e0 >= e_end) break;   // later tiles are further out
                     floatx4_t acc = {0, 0, 0, 0};
+                    if (I8) {
+                        int32x4_t ai = {0, 0, 0, 0};
 #pragma unroll
-                    for (int s = 0; s < KS; s++)
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[s], as_half8(bf[t][s]), acc,
-                                                                     0, 0, 0);
+                        for (int s = 0; s < KS; s++)
+                            ai = __builtin_amdgcn_mfma_i32_16x16x64_i8(as_i32x4(a[s]),
+                                                                      as_i32x4(bf[t][s]), ai, 0, 0, 0);
+#pragma unroll
+                        for (int j = 0; j < 4; j++) acc[j] = qa[j] * (float)ai[j] + qb[j];   // q.x
+                    } else {
+#pragma unroll
+                        for (int s = 0; s < KS; s++)
+                            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(as_half8(a[s]),
+                                                                         as_half8(bf[t][s]), acc,
+                                                                         0, 0, 0);
+                    }
                     // keys into the group tile ("no key" for dead, filtered or
                     // past-the-posting entries: the aligned range's tail).  Rows
                     // past the group's queries are written too and never stored.
@@ -250,12 +297,27 @@ __global__ __launch_bounds__(256) void k_probe_mfma(const ProbeMfmaParams p) {
         if (qa < p.nq && s < (int)p.ks) v = p.qbf[(size_t)qa * p.q_units + 4 * s + kg];
         a[s] = as_bf8(v);
     }
+    // per row: L2 |bf16(q - mu)|^2, cosine 1 / |q|
     float qn[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const uint32_t q = q0 + 4 * kg + j;
-        qn[j] = q < p.nq ? p.qnbf[q] : 0.0f;
+        float v = 0.0f;
+        if (q < p.nq) {
+            if (p.metric == M_COS) {
+                const float n2 = p.qnorm[q];
+                v = n2 > 0.0f ? 1.0f / sqrtf(n2) : 0.0f;
+            } else {
+                v = p.qnbf[q];
+            }
+        }
+        qn[j] = v;
     }
+    auto key = [&](float acc, float cn, float qnj) -> float {
+        if (p.metric == M_DOT) return 1.0f - acc;
+        if (p.metric == M_COS) return 1.0f - acc * qnj * cn;
+        return cn + qnj - 2.0f * acc;
+    };
     // a tile's B fragment and the norm of the lane's centroid (columns past
     // the end read the last centroid; their keys are never stored)
     auto load_b = [&](uint32_t c0, uint4 (&b)[KS], float &cn) {
@@ -268,26 +330,56 @@ __global__ __launch_bounds__(256) void k_probe_mfma(const ProbeMfmaParams p) {
         }
         cn = p.cnbf[c];
     };
-    auto body = [&](uint32_t c0, const uint4 (&b)[KS], float cn) {
-        floatx4_t acc = {0, 0, 0, 0};
-#pragma unroll
-        for (int s = 0; s < KS; s++)
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], as_bf8(b[s]), acc, 0, 0, 0);
-        const uint32_t cl = c0 + col - c_begin;
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            s_out[4 * kg + j][cl] = (uint16_t)(fkey_m(cn + qn[j] - 2.0f * acc[j]) >> 16);
-    };
-    uint4 b0[KS], b1[KS];
-    float cn0 = 0.0f, cn1 = 0.0f;
     const uint32_t start = c_begin + 16 * wave;
-    if (start < c_end) load_b(start, b0, cn0);
-    for (uint32_t c0 = start; c0 < c_end; c0 += 128) {
-        if (c0 + 64 < c_end) load_b(c0 + 64, b1, cn1);
-        body(c0, b0, cn0);
-        if (c0 + 64 >= c_end) break;
-        if (c0 + 128 < c_end) load_b(c0 + 128, b0, cn0);
-        body(c0 + 64, b1, cn1);
+    if constexpr (KS <= kMixKS) {
+        auto body = [&](uint32_t c0, const uint4 (&b)[KS], float cn) {
+            floatx4_t acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < KS; s++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], as_bf8(b[s]), acc, 0, 0, 0);
+            const uint32_t cl = c0 + col - c_begin;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                s_out[4 * kg + j][cl] = (uint16_t)(fkey_m(key(acc[j], cn, qn[j])) >> 16);
+        };
+        uint4 b0[KS], b1[KS];
+        float cn0 = 0.0f, cn1 = 0.0f;
+        if (start < c_end) load_b(start, b0, cn0);
+        for (uint32_t c0 = start; c0 < c_end; c0 += 128) {
+            if (c0 + 64 < c_end) load_b(c0 + 64, b1, cn1);
+            body(c0, b0, cn0);
+            if (c0 + 64 >= c_end) break;
+            if (c0 + 128 < c_end) load_b(c0 + 128, b0, cn0);
+            body(c0 + 64, b1, cn1);
+        }
+    } else {
+        // long rows: the queries' fragments stay in registers, each tile's
+        // centroid fragments stream in chunks of kMixKS K steps
+        for (uint32_t c0 = start; c0 < c_end; c0 += 64) {
+            const uint32_t c = min(c0 + col, c_end - 1);
+            const uint4 *src = p.cbf + (size_t)(c >> 6) * p.c_units * 64 + (c & 63);
+            const float cn = p.cnbf[c];
+            floatx4_t acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int s0 = 0; s0 < KS; s0 += kMixKS) {
+                uint4 b[kMixKS];
+#pragma unroll
+                for (int i = 0; i < kMixKS; i++) {
+                    const uint32_t u = 4 * (s0 + i) + kg;
+                    b[i] = (s0 + i < (int)p.ks && u < p.c_units) ? src[(size_t)u * 64]
+                                                                 : make_uint4(0, 0, 0, 0);
+                }
+#pragma unroll
+                for (int i = 0; i < kMixKS; i++)
+                    if (s0 + i < (int)p.ks)
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s0 + i], as_bf8(b[i]), acc,
+                                                                      0, 0, 0);
+            }
+            const uint32_t cl = c0 + col - c_begin;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                s_out[4 * kg + j][cl] = (uint16_t)(fkey_m(key(acc[j], cn, qn[j])) >> 16);
+        }
     }
     // keys past the last centroid up to the next multiple of 8
     const uint32_t n = c_end - c_begin, n8 = (n + 7) & ~7u;
@@ -1285,10 +1377,16 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) {
     if (grid == 0) return hipSuccess;
     const bool f = p.qmask != nullptr;   // label filter
-    switch (p.ks) {   // 32-dim K steps: dim <= 32, 64, 96, 128, 256
-#define SPF_SCAN_MFMA(KS_)                                                                    \
-        if (f) hipLaunchKernelGGL((k_scan_mfma<KS_, true>), dim3(grid), dim3(256), 0, s, p);  \
-        else hipLaunchKernelGGL((k_scan_mfma<KS_, false>), dim3(grid), dim3(256), 0, s, p);   \
+    // K steps: 32 dims (f16: dim <= 32, 64, 96, 128, 256) or 64 dims (int8: dim <= 64 ... 512)
+    switch (p.ks) {
+#define SPF_SCAN_MFMA(KS_)                                                                          \
+        if (p.int8) {                                                                               \
+            if (f) hipLaunchKernelGGL((k_scan_mfma<KS_, true, true>), dim3(grid), dim3(256), 0, s, p); \
+            else hipLaunchKernelGGL((k_scan_mfma<KS_, false, true>), dim3(grid), dim3(256), 0, s, p);  \
+        } else {                                                                                    \
+            if (f) hipLaunchKernelGGL((k_scan_mfma<KS_, true, false>), dim3(grid), dim3(256), 0, s, p); \
+            else hipLaunchKernelGGL((k_scan_mfma<KS_, false, false>), dim3(grid), dim3(256), 0, s, p);  \
+        }                                                                                           \
         break;
     case 1: SPF_SCAN_MFMA(1)
     case 2: SPF_SCAN_MFMA(2)
@@ -1339,7 +1437,11 @@ hipError_t launch_probe_mfma(const ProbeMfmaParams &p, hipStream_t s) {
     case 4: hipLaunchKernelGGL(k_probe_mfma<4>, dim3(grid), dim3(256), 0, s, p); break;
     case 5: case 6: case 7: case 8:
         hipLaunchKernelGGL(k_probe_mfma<kMixKS>, dim3(grid), dim3(256), 0, s, p); break;
-    default: return hipErrorInvalidValue;
+    default:   // long rows (dim <= 1024): K steps rounded up to a multiple of kMixKS
+        if (p.ks <= 16) hipLaunchKernelGGL(k_probe_mfma<16>, dim3(grid), dim3(256), 0, s, p);
+        else if (p.ks <= 24) hipLaunchKernelGGL(k_probe_mfma<24>, dim3(grid), dim3(256), 0, s, p);
+        else if (p.ks <= 32) hipLaunchKernelGGL(k_probe_mfma<32>, dim3(grid), dim3(256), 0, s, p);
+        else return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

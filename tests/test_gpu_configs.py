@@ -120,6 +120,24 @@ def test_c3_int8_dot_rerank_matches_oracle(c3):
     compare_oracle(dev, ix, queries[:200], 10, 32, "c3 int8 dot rerank 400")
 
 
+def test_c3_int8_dot_mixed_recall(c3):
+    """Mixed precision on the Zipf int8 inner-product shape: bf16 matrix-core
+    probe, int8 matrix-core scan with int32 sums, exact f32 rerank of a
+    400-candidate pool."""
+    base, queries, ix, dev = c3
+    qs = queries[:1000]
+    ids, dd, cnt = dev.search(qs, 10, 32, precision=PREC_MIXED, rerank_size=400)
+    assert (cnt == 10).all()
+    check_exact_rows(ix, qs, ids, dd, cnt, range(0, 1000, 25))
+    gt = fp64_topk(base, qs, 10, ix.metric)
+    e_ids, _, _ = dev.search(qs, 10, 32, precision=PREC_EXACT)
+    r, re = recall(ids, gt), recall(e_ids, gt)
+    agree = recall(ids, e_ids)
+    print(f"c3 int8 dot recall@10 vs fp64: mixed {r:.4f}, exact {re:.4f}; "
+          f"mixed vs exact {agree:.4f}", flush=True)
+    assert agree >= 0.95 and r >= re - 0.03
+
+
 @pytest.fixture(scope="module")
 def c4():
     base, queries = config4(n=100_000, nq=200)

@@ -44,7 +44,7 @@ def recall(ids, gt, k=10):
                           for i in range(gt.shape[0])]))
 
 
-@pytest.mark.parametrize("quant", [None, "f16"])
+@pytest.mark.parametrize("quant", [None, "f16", "int8"])
 @pytest.mark.parametrize("metric", ["l2", "cosine", "dot", "l2_root"])
 @pytest.mark.parametrize("dim", [32, 64, 96, 100, 128, 200])   # 1, 2, 3, 4, 4, 8 K steps of 32
 def test_mixed_exact_rerank_and_recall(quant, metric, dim):
@@ -53,6 +53,23 @@ def test_mixed_exact_rerank_and_recall(quant, metric, dim):
     ix = build_index(x, metric, quant, n_postings=40, rerank_size=200, seed=0)
     dev = DeviceIndex(ix)
     print(f"[case] mixed quant={quant} metric={metric} dim={dim}", flush=True)
+    ids, dd, cnt = dev.search(q, 10, 16, precision=PREC_MIXED)
+    assert (cnt == 10).all()
+    check_rows(ix, q, ids, dd, cnt)
+    gt, _ = brute_force_f64(x, q, 10, ix.metric)
+    r = recall(ids, gt)
+    print(f"recall@10 vs fp64 = {r:.4f}")
+    assert r >= 0.95
+
+
+@pytest.mark.parametrize("metric", ["l2", "cosine", "dot"])
+@pytest.mark.parametrize("dim", [300, 512])   # int8 scan K steps of 64: 5, 8; long-row probe
+def test_mixed_int8_long_rows(metric, dim):
+    x = clustered(3000, dim, 25, seed=11)
+    q = clustered(40, dim, 25, seed=12)
+    ix = build_index(x, metric, "int8", n_postings=40, rerank_size=200, seed=0)
+    dev = DeviceIndex(ix)
+    print(f"[case] mixed int8 metric={metric} dim={dim}", flush=True)
     ids, dd, cnt = dev.search(q, 10, 16, precision=PREC_MIXED)
     assert (cnt == 10).all()
     check_rows(ix, q, ids, dd, cnt)
