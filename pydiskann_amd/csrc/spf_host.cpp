@@ -308,6 +308,13 @@ struct spf_index {
     float *d_rq_signs = nullptr;
     uint32_t rq_padded = 0;
     bool rq_l2 = false;
+    // RaBitQ bit codes (tiled, rq_bit_units per entry; the same buffer as
+    // d_codes with an L2 metric), f16 raw rows and |x|^2 per entry for the
+    // mixed estimator scan and its f16 rerank stage
+    uint4 *d_rq_bits = nullptr;
+    uint32_t rq_bit_units = 0;
+    uint16_t *d_raw16 = nullptr;
+    float *d_rq_xn = nullptr;
     uint64_t device_bytes = 0;
     uint64_t gen = 0;                   // bumped when device buffers of the index move
     std::atomic<uint64_t> replays{0};   // calls served by a captured launch chain
@@ -333,7 +340,8 @@ struct spf_index {
         void *ps[] = {d_cent, d_cnorm, d_post_off, d_post_len, d_block_sorted, d_codes, d_rank,
                       d_rank_id, d_raw, d_del, d_xnorm, d_xsq, d_rank_entry, d_raw_e, d_xn16,
                       d_mu, d_cnbf, d_cent_rows, d_cbf, d_rq_signs, d_post_ex, d_post_xn,
-                      d_label_e, d_gvec, d_gadj_off, d_gadj, d_glive, d_gpost};
+                      d_label_e, d_gvec, d_gadj_off, d_gadj, d_glive, d_gpost, d_raw16, d_rq_xn,
+                      d_rq_bits != d_codes ? d_rq_bits : nullptr};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         if (codes16_owned && d_codes16) (void)hipFree(d_codes16);
@@ -614,8 +622,10 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         HIPCHK(upload(&ix->d_rq_signs, d->rabitq_signs, 3 * (size_t)ix->rq_padded, bytes));
         const uint32_t bit_units = (d->code_bytes + 15) / 16;
         std::vector<uint4> ct = tile_rows(d->codes, N, d->code_bytes, bit_units, entry_src.data());
+        ix->rq_bit_units = bit_units;
         if (ix->rq_l2) {
             HIPCHK(upload(&ix->d_codes, ct.data(), ct.size(), bytes));
+            ix->d_rq_bits = ix->d_codes;
         } else {
             // decode_with once per entry into tiled f32 codes
             uint4 *d_bits = nullptr;
@@ -647,7 +657,8 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
                 he = launch_rq_decode(dp, (uint32_t)N, nullptr);
             }
             if (he == hipSuccess) he = hipDeviceSynchronize();
-            if (d_bits) (void)hipFree(d_bits);
+            ix->d_rq_bits = d_bits;   // kept for the mixed estimator scan
+            bytes += tmp_bytes - std::max<size_t>(N, 1) * 4;
             if (d_entry_post) (void)hipFree(d_entry_post);
             HIPCHK(he);
         }
@@ -715,6 +726,25 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
             std::memcpy(&rr[(size_t)e * ix->dpad], src, 4 * d->dim);
         }
         HIPCHK(upload(&ix->d_raw_e, rr.data(), rr.size(), bytes));
+    }
+    if (d->quantizer == SPF_QUANT_RABITQ) {
+        // f16 rows, |x|^2 and per-posting max |x - f16(x)| for the mixed path
+        HIPCHK(hipMalloc((void **)&ix->d_raw16, std::max<uint64_t>(N, 1) * ix->dpad * 2));
+        HIPCHK(hipMalloc((void **)&ix->d_rq_xn, std::max<uint64_t>(N, 1) * 4));
+        HIPCHK(hipMalloc((void **)&ix->d_post_ex, std::max<uint32_t>(P, 1) * 4));
+        bytes += std::max<uint64_t>(N, 1) * (ix->dpad * 2 + 4) + std::max<uint32_t>(P, 1) * 4;
+        HIPCHK(hipMemset(ix->d_post_ex, 0, std::max<uint32_t>(P, 1) * 4));
+        RqRowsParams rp{};
+        rp.raw_e = ix->d_raw_e;
+        rp.n = (uint32_t)N;
+        rp.dpad = ix->dpad;
+        rp.dim = d->dim;
+        rp.n_post = P;
+        rp.post_off = ix->d_post_off;
+        rp.raw16 = ix->d_raw16;
+        rp.xn = ix->d_rq_xn;
+        rp.post_ex = reinterpret_cast<uint32_t *>(ix->d_post_ex);
+        HIPCHK(launch_rq_rows(rp, nullptr));
     }
     // f16 codes for the matrix-core scan: the index's own f16 codes, or an
     // IEEE round-to-nearest-even copy of f32 codes
@@ -808,6 +838,7 @@ struct Shape {
     uint32_t npr, want, dpad, dpad32, Gv, G, echunk, band, Pp;
     bool mixed, rerank, mfma_scan, mprobe, round16, q8;
     bool int8;       // matrix-core scan of int8 codes (int8 MFMA, 64-dim K steps)
+    bool rq_est;     // RaBitQ estimator prep: search_rabitq (L2) or the mixed estimator scan
     uint32_t dpad64, mks;   // dim rounded up to 64; K steps of the matrix-core scan
     bool graph;   // probe through the routing graph (above kBruteCentroids live postings)
     uint32_t nraw;   // the probe count as asked (the graph's result count and beam)
@@ -938,7 +969,14 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     rp.ids = d_ids;
     rp.dists = d_dists;
     rp.counts = d_counts;
-    HIPCHK(launch_rerank_pruned(rp, s));
+    if (rq && !full_rerank && k <= 256 && ix->d_raw16) {
+        // RaBitQ: f16 rows order the pool by a proven bound, exact f32 in that order
+        rp.raw16 = ix->d_raw16;
+        rp.xn = ix->d_rq_xn;
+        HIPCHK(launch_rerank_f16(rp, s));
+    } else {
+        HIPCHK(launch_rerank_pruned(rp, s));
+    }
     HIPCHK(mark(SPF_STAGE_FINALIZE));
     return SPF_OK;
 }
@@ -957,10 +995,13 @@ int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_se
     z.mixed = opts && opts->precision == SPF_PREC_MIXED;
     if (opts && opts->precision > SPF_PREC_MIXED) return fail(SPF_EINVAL, "unknown precision");
     const bool i8 = ix->code_kind == CK_U8 && ix->quant == SPF_QUANT_INT8;
-    if (z.mixed && !ix->rq_l2 && !(ix->d_codes16 && ix->dim <= 256) && !(i8 && ix->dim <= 512))
+    const bool rq = ix->quant == SPF_QUANT_RABITQ;
+    if (z.mixed && !rq && !(ix->d_codes16 && ix->dim <= 256) && !(i8 && ix->dim <= 512))
         return fail(SPF_EUNSUPPORTED,
                     "mixed precision serves f32 and f16 postings with dim <= 256, int8 postings "
-                    "with dim <= 512 and RaBitQ postings with an L2 metric");
+                    "with dim <= 512 and RaBitQ postings");
+    // RaBitQ estimator prep: search_rabitq (L2, exact) or the mixed estimator scan
+    z.rq_est = rq && (ix->rq_l2 || z.mixed);
     z.int8 = z.mixed && i8;
     z.dpad64 = round_up(ix->dim, 64);
     // RaBitQ + L2 takes search_rabitq (src/spfresh.rs:697-703): no rerank_size
@@ -983,7 +1024,7 @@ int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_se
     // queries and entries per posting-scan item: the matrix-core scan takes
     // kMixQ queries over mfma_chunk entries, the RaBitQ estimator scan 8
     // queries, the VALU scan Gv; both of the latter over kEChunk entries
-    z.mfma_scan = z.mixed && !ix->rq_l2;
+    z.mfma_scan = z.mixed && !rq;
     z.G = z.mfma_scan ? ix->mix_g : z.mixed ? 8u : z.Gv;
     z.mks = z.int8 ? z.dpad64 / 64 : z.dpad32 / 32;
     z.echunk = z.mfma_scan ? mfma_chunk(z.mks) : (z.mixed ? kEChunk : ix->scan_chunk);
@@ -1128,12 +1169,12 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     add(pairs, 8);               // qp_slot
     add(pairs, 4);               // pair_q
     add(pairs, 8);               // pair_slot
-    if (ix->rq_l2) add(pairs, 4);
+    if (z.rq_est) add(pairs, 4);   // pair_src
     add(P + 1, 4);               // item_cnt
     add(P + 1, 4);               // item_off
     const uint64_t items = items_bound(ix, z, q);
     if (candidates) add(std::max<uint64_t>(items, 1), 4);   // item_post
-    if (ix->rq_l2) {
+    if (z.rq_est) {
         add(pairs * 4 * (ix->rq_padded / 64), 8);
         add(pairs * 8, 4);
     }
@@ -1458,7 +1499,7 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(w->qp_slot.ensure((size_t)nq * npr));
     HIPCHK(w->pair_q.ensure((size_t)nq * npr));
     HIPCHK(w->pair_slot.ensure((size_t)nq * npr));
-    if (ix->rq_l2) HIPCHK(w->pair_src.ensure((size_t)nq * npr));
+    if (z.rq_est) HIPCHK(w->pair_src.ensure((size_t)nq * npr));
     HIPCHK(w->item_cnt.ensure(P + 1));
     HIPCHK(w->item_off.ensure(P + 1));
     PlanParams pl{};
@@ -1482,7 +1523,7 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.pair_q = w->pair_q.p;
     pl.pair_slot = w->pair_slot.p;
     pl.item_cnt = w->item_cnt.p;
-    pl.pair_src = ix->rq_l2 ? w->pair_src.p : nullptr;
+    pl.pair_src = z.rq_est ? w->pair_src.p : nullptr;
     pl.G = G;
     pl.echunk = echunk;
     pl.align = z.mfma_scan ? kScanAlign : 1u;   // the matrix-core scan stores aligned rows
@@ -1545,7 +1586,7 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(launch_item_table(pl, s));
 
     RqParams rq{};
-    if (ix->rq_l2) {
+    if (z.rq_est) {
         const uint32_t words = ix->rq_padded / 64;
         HIPCHK(w->rq_planes.ensure((size_t)nq * npr * 4 * words));
         HIPCHK(w->rq_consts.ensure((size_t)nq * npr * 8));
@@ -1563,8 +1604,11 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         rq.planes = w->rq_planes.p;
         rq.consts = w->rq_consts.p;
         rq.post_off = ix->d_post_off;
-        rq.codes = ix->d_codes;
-        rq.units = ix->units;
+        rq.codes = ix->d_rq_bits;
+        rq.units = ix->rq_bit_units;
+        rq.mixed = mixed;
+        rq.xn = ix->d_rq_xn;
+        rq.qnorm = w->qnorm.p;
         rq.del = ix->d_del;
         rq.rank = ix->d_rank;
         rq.raw_e = ix->d_raw_e;
@@ -1582,7 +1626,7 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     }
     if (mixed) {
         int rc = search_mixed(ix, w, s, nq, k, npr, want, total, n_items, d_ids, d_dists,
-                              d_counts, mark, ix->rq_l2 ? &rq : nullptr,
+                              d_counts, mark, z.rq_est ? &rq : nullptr,
                               opts->flags & SPF_FLAG_FULL_RERANK, prof, d_masks, n_items_dev, z);
         if (rc) return rc;
     } else if (ix->rq_l2) {
@@ -1703,12 +1747,12 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                 code = ix->units * 16.0 + 4 + tomb + lab;        // u8 code, |decode(c)|^2
                 qrow = z.dpad64 + 12.0 + (d_masks ? 8 : 0);      // int8 row, affine terms, |q|^2
                 key = 2;
-            } else if (mixed && !ix->rq_l2) {
+            } else if (mixed && !z.rq_est) {
                 code = ix->units16 * 16.0 + 4 + tomb + lab;     // f16 code, |x16|^2
                 qrow = dpad32 * 2.0 + 4 + (d_masks ? 8 : 0);     // f16 row, |q16|^2
                 key = 2;
-            } else if (ix->rq_l2) {
-                code = ix->units * 16.0 + tomb + lab;            // bit words, |r|^2, factor
+            } else if (z.rq_est) {
+                code = ix->rq_bit_units * 16.0 + tomb + lab;     // bit words, |r|^2, factor
                 qrow = 4.0 * (ix->rq_padded / 64) * 8 + 32;      // 4-bit planes, constants
                 key = mixed ? 2 : 0;
             } else {
@@ -1732,7 +1776,8 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                 b += (double)total * key;
             }
             g_prof.scan_bytes = (uint64_t)b;
-            g_prof.scan_flops = ix->rq_l2 ? 0 : 2ull * (z.int8 ? z.dpad64 : mixed ? dpad32 : dpad) * real;
+            g_prof.scan_flops = ix->quant == SPF_QUANT_RABITQ ? 0
+                                : 2ull * (z.int8 ? z.dpad64 : mixed ? dpad32 : dpad) * real;
             g_prof.probe_flops = z.graph ? 0 : 2ull * (mprobe ? dpad32 : dpad) * nq * P;
         }
         if (mixed) {   // per-query counts, summed here

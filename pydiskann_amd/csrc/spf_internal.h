@@ -235,6 +235,11 @@ struct RerankPrunedParams {
     float *dists;
     uint32_t *counts;
     uint32_t *stats;            // optional [nq]: raw rows gathered per query
+    // f16 stage (RaBitQ pools): every member scored on its f16 row, ordered
+    // by a proven lower bound of its exact distance, reranked exactly in that
+    // order until the bound passes the k-th exact distance
+    const uint16_t *raw16;      // [n_entries][dpad] f16 rows (null = no f16 stage)
+    const float *xn;            // [n_entries] |x|^2
 };
 
 // RaBitQ: per-pair query prep, exact search, mixed estimator scan.
@@ -265,6 +270,23 @@ struct RqParams {
     uint16_t *out16;
     const uint8_t *label_e;     // label filter: entry label, per-query mask (null = off)
     const uint64_t *qmask;
+    // mixed precision: keys of the metric's estimate from the L2 estimate
+    // (cosine on the unit query: 1 - (1 + |x|^2 - est) / (2 |x|); dot:
+    // 1 - (|q|^2 + |x|^2 - est) / 2), the prep's norm as a wave sum
+    uint32_t mixed;
+    const float *xn;            // [n_entries] |x|^2 of the raw rows
+    const float *qnorm;         // [nq] |q|^2
+};
+
+// Per entry of a RaBitQ index: the f16 copy of its raw row, |x|^2, and the
+// posting's max |x - f16(x)| (the f16 rerank stage's bound), rounded up.
+struct RqRowsParams {
+    const float *raw_e;         // [n][dpad]
+    uint32_t n, dpad, dim, n_post;
+    const uint32_t *post_off;   // [n_post + 1]
+    uint16_t *raw16;            // [n][dpad] f16 bits
+    float *xn;                  // [n]
+    uint32_t *post_ex;          // [n_post] f32 bits, atomicMax (zeroed by the caller)
 };
 
 struct RqDecodeParams {
@@ -411,11 +433,13 @@ hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s);
 hipError_t launch_rerank_pool(const RerankPoolParams &p, hipStream_t s);
 hipError_t launch_rerank_pruned(const RerankPrunedParams &p, hipStream_t s);
+hipError_t launch_rerank_f16(const RerankPrunedParams &p, hipStream_t s);
 hipError_t launch_probe_mfma(const ProbeMfmaParams &p, hipStream_t s);
 hipError_t launch_rq_prep(const RqParams &p, uint32_t n_pairs, hipStream_t s);
 hipError_t launch_rq_search_exact(const RqParams &p, hipStream_t s);
 hipError_t launch_rq_decode(const RqDecodeParams &p, uint32_t n, hipStream_t s);
 hipError_t launch_rq_scan(const RqParams &p, uint32_t grid, hipStream_t s);
+hipError_t launch_rq_rows(const RqRowsParams &p, hipStream_t s);
 
 // Working memory of the no-allocation search: a file-scope device array of
 // this size (spf_noalloc.hip), one per device.

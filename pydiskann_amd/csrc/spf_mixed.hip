@@ -1373,6 +1373,180 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
     if (p.stats && tid == 0) p.stats[q] = na + nb;
 }
 
+// ---------------------------------------------------------------- f16 stage
+// RaBitQ pools: every member's distance on its f16 row (four lanes per
+// member, f32 accumulate) gives a proven lower bound of its exact f32
+// distance:
+//   L2     |q - x| >= |q - x16| - ex  (ex: the query's probed postings' max |x - f16(x)|)
+//   dot    q.x <= q.x16 + |q| ex
+//   cosine the same over |q| |x|
+// widened for the f32 evaluation of both sums ((dim + 16) 2^-23 relative).
+// Members are reranked exactly in bound order, a chunk at a time, until the
+// next bound exceeds the k-th exact distance: the result is the exact rerank
+// of the whole pool.
+constexpr uint32_t kTop16 = 256;   // largest k of the f16 stage
+
+template <uint32_t CAP>
+struct Rerank16Lds {
+    RerankLds<CAP> b;
+    uint64_t top[2][kTop16];       // exact top k so far (sorted), merge target
+    uint32_t list[64 * kPrH];      // entries of one exact chunk
+    uint64_t chunk[64 * kPrH];
+    uint32_t tc;
+};
+
+template <uint32_t CAP>
+__global__ __launch_bounds__(256) void k_rerank_f16(const RerankPrunedParams p) {
+    __shared__ Rerank16Lds<CAP> S;
+    RerankLds<CAP> &L = S.b;
+    extern __shared__ __align__(16) float s_qv[];   // dpad floats
+    const uint32_t x = blockIdx.x % kXcds, jx = blockIdx.x / kXcds;
+    const uint32_t per = (p.nq + kXcds - 1) / kXcds;
+    const uint32_t idx = x * per + jx;
+    if (jx >= per || idx >= p.nq) return;   // whole workgroup
+    const uint32_t q = idx;
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t c = min(p.pool_want[q], p.want);
+    const uint64_t b = p.seg_off[q];
+    const uint32_t n = p.seg_cnt[q];
+    const uint32_t nwords = (n + 31) / 32, Mw = (nwords + 255) / 256;
+    const uint32_t w0 = tid * Mw, w1 = min(w0 + Mw, nwords);
+    const uint32_t *sel = p.sel_bits + b / 32;
+    for (uint32_t i = tid; i < p.dpad; i += 256) s_qv[i] = p.qv[(size_t)q * p.dpad + i];
+    const float qn = p.qnorm[q];
+    ProbeTab tab;
+    tab.np = p.np_q[q];
+    tab.qp_row = p.qp_slot + (size_t)q * p.npr;
+    tab.probe_row = p.probe_post + (size_t)q * p.npr;
+    tab.post_off = p.post_off;
+    tab.seg = b;
+    tab.pbase = tab.ebase = nullptr;
+    float exq = 0.0f;
+    for (uint32_t j = tid; j < tab.np; j += 256) {
+        const uint32_t post = tab.probe_row[j];
+        if (tab.np <= kWaveTab) {
+            L.pbase[j] = (uint32_t)(tab.qp_row[j] - b);
+            L.ebase[j] = p.post_off[post];
+        }
+        exq = fmaxf(exq, p.post_ex[post]);
+    }
+    if (tab.np <= kWaveTab) {
+        tab.pbase = L.pbase;
+        tab.ebase = L.ebase;
+    }
+    for (int off = 32; off > 0; off >>= 1) exq = fmaxf(exq, __shfl_xor(exq, off, 64));
+    if (tid == 0) {
+        L.m[0] = 0u;
+        S.tc = 0u;
+    }
+    __syncthreads();
+    if (lane == 0) atomicMax(&L.m[0], __float_as_uint(exq));
+    // 1. the pool's members (rebuild_members' barriers publish L.m and the table)
+    const uint32_t cnt = rebuild_members(L, w0, w1, [&](uint32_t w) { return sel[w]; }, tab,
+                                         L.ent, c);
+    const float ex = __uint_as_float(L.m[0]);
+    const float gam = (float)(p.dim + 16) * (1.0f / 8388608.0f);
+    const float qa = sqrtf(qn);
+    // 2. f16-row distance -> lower bound; key = (bound, member index)
+    {
+        const uint32_t r = tid & 3, quad = tid >> 2, units = p.dpad / 8;
+        for (uint32_t i0 = 0; i0 < cnt; i0 += 64 * kPrH) {
+#pragma unroll
+            for (int h = 0; h < kPrH; h++) {
+                const uint32_t i = i0 + 64 * h + quad;
+                const uint32_t e = i < cnt ? L.ent[i] : L.ent[0];
+                const uint4 *xr = reinterpret_cast<const uint4 *>(p.raw16 + (size_t)e * p.dpad);
+                float acc = 0.0f;
+                for (uint32_t u = r; u < units; u += 4) {
+                    const half8_t hv = as_half8(xr[u]);
+                    const float4 qa4 = reinterpret_cast<const float4 *>(s_qv)[2 * u];
+                    const float4 qb4 = reinterpret_cast<const float4 *>(s_qv)[2 * u + 1];
+                    const float qq[8] = {qa4.x, qa4.y, qa4.z, qa4.w, qb4.x, qb4.y, qb4.z, qb4.w};
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const float xv = (float)hv[j];
+                        if (p.metric == M_L2 || p.metric == M_L2SQ) {
+                            const float t = qq[j] - xv;
+                            acc = __builtin_fmaf(t, t, acc);
+                        } else {
+                            acc = __builtin_fmaf(qq[j], xv, acc);
+                        }
+                    }
+                }
+                acc = quad_sum(acc);
+                if (r == 0 && i < cnt) {
+                    float lb;
+                    if (p.metric == M_L2 || p.metric == M_L2SQ) {
+                        const float sd = sqrtf(fmaxf(acc, 0.0f) / (1.0f + gam)) - ex;
+                        lb = sd > 0.0f ? (p.metric == M_L2 ? sd * sqrtf(1.0f - gam) * (1.0f - 2.4e-7f)
+                                                           : sd * sd * (1.0f - gam))
+                                       : 0.0f;
+                    } else {
+                        const float xa = sqrtf(p.xn[e]);
+                        const float slack = ex + gam * (2.0f * xa + ex);
+                        if (p.metric == M_DOT) {
+                            lb = (1.0f - acc) - qa * slack - 2.4e-7f * (1.0f + qa * (xa + ex));
+                        } else if (xa > 0.0f && qa > 0.0f) {
+                            lb = (1.0f - acc / (qa * xa)) - slack / xa - 4.0f * gam - 2.4e-7f;
+                        } else {
+                            lb = -__builtin_inff();
+                        }
+                    }
+                    L.ex[i] = ((uint64_t)fkey_m(lb) << 32) | i;
+                }
+            }
+        }
+    }
+    // 3. members in bound order
+    uint32_t m2 = 2;
+    while (m2 < cnt) m2 <<= 1;
+    __syncthreads();
+    for (uint32_t i = cnt + tid; i < m2; i += 256) L.ex[i] = kNoKey;
+    __syncthreads();
+    if (cnt > 1) bitonic_lds(L.ex, m2);
+    __syncthreads();
+    // 4. exact rerank in bound order until the next bound exceeds the k-th
+    uint32_t done = 0;
+    const uint32_t chunk = 64 * kPrH;
+    while (done < cnt) {
+        const uint32_t tc = S.tc;
+        if (tc >= p.k) {
+            const float nb = key_float_m((uint32_t)(L.ex[done] >> 32));
+            const float kth = key_float_m((uint32_t)(S.top[0][p.k - 1] >> 32));
+            if (nb > kth) break;   // (uniform: every thread reads the same words)
+        }
+        const uint32_t n1 = min(cnt - done, chunk);
+        for (uint32_t i = tid; i < n1; i += 256) S.list[i] = L.ent[(uint32_t)L.ex[done + i]];
+        __syncthreads();
+        rerank_list(p, S.list, n1, s_qv, qn, S.chunk);
+        __syncthreads();
+        const uint32_t tot = tc + n1;
+        for (uint32_t i = tid; i < tot; i += 256) {   // keys are distinct (rank in the low bits)
+            const uint64_t v = i < tc ? S.top[0][i] : S.chunk[i - tc];
+            uint32_t rk = 0;
+            for (uint32_t j = 0; j < tc; j++) rk += S.top[0][j] < v;
+            for (uint32_t j = 0; j < n1; j++) rk += S.chunk[j] < v;
+            if (rk < p.k) S.top[1][rk] = v;
+        }
+        __syncthreads();
+        const uint32_t ntc = min(p.k, tot);
+        for (uint32_t i = tid; i < ntc; i += 256) S.top[0][i] = S.top[1][i];
+        if (tid == 0) S.tc = ntc;
+        done += n1;
+        __syncthreads();
+    }
+    const uint32_t tc = S.tc;
+    for (uint32_t i = tid; i < p.k; i += 256) {   // the row, padded by UINT64_MAX / +inf
+        const uint64_t v = i < tc ? S.top[0][i] : kNoKey;
+        p.ids[(size_t)q * p.k + i] = i < tc ? p.rank_id[(uint32_t)v] : ~0ull;
+        p.dists[(size_t)q * p.k + i] = i < tc ? key_float_m((uint32_t)(v >> 32)) : __builtin_inff();
+    }
+    if (tid == 0) {
+        p.counts[q] = tc;
+        if (p.stats) p.stats[q] = done;
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) {
     if (grid == 0) return hipSuccess;
@@ -1423,6 +1597,18 @@ hipError_t launch_rerank_pruned(const RerankPrunedParams &p, hipStream_t s) {
         hipLaunchKernelGGL(k_rerank_pruned<1024>, dim3(grid), dim3(256), p.dpad * sizeof(float), s, p);
     else
         hipLaunchKernelGGL(k_rerank_pruned<kMaxK>, dim3(grid), dim3(256), p.dpad * sizeof(float), s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_rerank_f16(const RerankPrunedParams &p, hipStream_t s) {
+    if (p.nq == 0) return hipSuccess;
+    if (p.k > kTop16 || p.want > kMaxK) return hipErrorInvalidValue;
+    const uint32_t grid = (p.nq + kXcds - 1) / kXcds * kXcds;
+    const size_t lds = p.dpad * sizeof(float);
+    if (p.want <= 1024)
+        hipLaunchKernelGGL(k_rerank_f16<1024>, dim3(grid), dim3(256), lds, s, p);
+    else
+        hipLaunchKernelGGL(k_rerank_f16<kMaxK>, dim3(grid), dim3(256), lds, s, p);
     return hipGetLastError();
 }
 

@@ -58,9 +58,17 @@ __global__ __launch_bounds__(64) void k_rq_prep(const RqParams p) {
     const float *qv = p.qv + (size_t)q * p.dpad;
     const float *c = p.cent + (size_t)post * p.dpad;
     const uint32_t P = p.padded;
-    for (uint32_t i = lane; i < P; i += 64) s_r[i] = i < p.dim ? qv[i] - c[i] : 0.0f;
+    // mixed precision with cosine: the residual of the unit query
+    const float qs = p.mixed && p.metric == M_COS
+                         ? (p.qnorm[q] > 0.0f ? 1.0f / sqrtf(p.qnorm[q]) : 0.0f) : 1.0f;
+    for (uint32_t i = lane; i < P; i += 64) s_r[i] = i < p.dim ? qv[i] * qs - c[i] : 0.0f;
     wave_sync_lds();
-    if (lane == 0) {   // r.iter().map(|x| x * x).sum::<f32>(): sequential, unfused
+    if (p.mixed) {   // a wave sum (the estimate only orders candidates)
+        float acc = 0.0f;
+        for (uint32_t i = lane; i < P; i += 64) acc = __builtin_fmaf(s_r[i], s_r[i], acc);
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) s_norm = sqrtf(acc);
+    } else if (lane == 0) {   // r.iter().map(|x| x * x).sum::<f32>(): sequential, unfused
         float acc = 0.0f;
         for (uint32_t i = 0; i < P; i++) {
             const float sq = s_r[i] * s_r[i];
@@ -329,8 +337,10 @@ __global__ __launch_bounds__(256) void k_rq_scan(const RqParams p) {
     __shared__ float s_cs[kRqG][8];
     __shared__ uint64_t s_slot[kRqG];
     __shared__ uint64_t s_mask[kRqG];
+    __shared__ float s_qn[kRqG];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t P = p.padded;
+    const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
     const uint32_t n_items = p.n_items_dev ? *p.n_items_dev : p.n_items;
     for (XcdWalk it(n_items); it.more(); it.next()) {
         const uint32_t item = it.item;
@@ -353,7 +363,9 @@ __global__ __launch_bounds__(256) void k_rq_scan(const RqParams p) {
         }
         if (tid < gcnt) {
             s_slot[tid] = p.pair_slot[pr0 + tid];
-            if (p.qmask) s_mask[tid] = p.qmask[p.pair_src[pr0 + tid] / p.npr];
+            const uint32_t qi = p.pair_src[pr0 + tid] / p.npr;
+            if (p.qmask) s_mask[tid] = p.qmask[qi];
+            s_qn[tid] = p.metric == M_DOT ? p.qnorm[qi] : 0.0f;
         }
         __syncthreads();
         for (uint32_t e0 = e_begin + 64 * wave; e0 < e_end; e0 += 256) {
@@ -364,14 +376,50 @@ __global__ __launch_bounds__(256) void k_rq_scan(const RqParams p) {
             rq_load_code<W>(p.codes, p.units, e, words, n2, fac);
             const bool dead = p.del && ((p.del[e >> 5] >> (e & 31)) & 1u);
             const uint32_t lab = p.qmask ? p.label_e[e] : 0u;
+            const float xn = l2 ? 0.0f : p.xn[e];
+            const float rx = xn > 0.0f ? 0.5f / sqrtf(xn) : 0.0f;
             for (uint32_t g = 0; g < gcnt; g++) {
                 float est, err;
                 rq_est<W>(words, n2, fac, s_pl[g], s_cs[g], P, est, err);
+                // cosine (unit query): 1 - q.x / |x| with q.x = (1 + |x|^2 - |q - x|^2) / 2;
+                // dot: 1 - q.x with q.x = (|q|^2 + |x|^2 - |q - x|^2) / 2
+                float d = est;
+                if (p.metric == M_COS) d = 1.0f - (1.0f + xn - est) * rx;
+                else if (p.metric == M_DOT) d = 1.0f - 0.5f * (s_qn[g] + xn - est);
                 const bool skip = dead || (p.qmask && !label_ok(s_mask[g], lab));
-                p.out16[s_slot[g] + (e - pb)] = skip ? (uint16_t)0xffff : (uint16_t)(fkey_r(est) >> 16);
+                p.out16[s_slot[g] + (e - pb)] = skip ? (uint16_t)0xffff : (uint16_t)(fkey_r(d) >> 16);
             }
         }
     }
+}
+
+// ---------------------------------------------------------------- rows
+// One thread per entry: f16 copy of the raw row, |x|^2, and the posting's
+// max |x - f16(x)| in double, rounded up (non-negative floats order as their
+// bits, so atomicMax on the bits is a max).
+__global__ void k_rq_rows(const RqRowsParams p) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p.n) return;
+    uint32_t lo = 0, hi = p.n_post;   // posting: post_off[lo] <= e < post_off[lo + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (p.post_off[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    const float *x = p.raw_e + (size_t)e * p.dpad;
+    uint16_t *h = p.raw16 + (size_t)e * p.dpad;
+    double e2 = 0.0, n2 = 0.0;
+    for (uint32_t i = 0; i < p.dpad; i++) {
+        const float v = i < p.dim ? x[i] : 0.0f;
+        const _Float16 f = (_Float16)v;
+        h[i] = __builtin_bit_cast(uint16_t, f);
+        const double t = (double)v - (double)(float)f;
+        e2 += t * t;
+        n2 += (double)v * v;
+    }
+    p.xn[e] = (float)n2;
+    const float ex = (float)(sqrt(e2) * (1.0 + 1e-12));
+    atomicMax(p.post_ex + lo, __float_as_uint(ex) + (ex > 0.0f ? 1u : 0u));
 }
 
 // ---------------------------------------------------------------- launchers
@@ -406,6 +454,12 @@ hipError_t launch_rq_decode(const RqDecodeParams &p, uint32_t n, hipStream_t s) 
 #define L(W) hipLaunchKernelGGL(k_rq_decode<W>, dim3(n), dim3(64), p.padded * sizeof(float), s, p)
     SPF_RQ_DISPATCH(p.padded / 64, L)
 #undef L
+    return hipGetLastError();
+}
+
+hipError_t launch_rq_rows(const RqRowsParams &p, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rq_rows, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

@@ -163,8 +163,11 @@ def test_c4_rabitq_cosine_decode_matches_oracle(c4, rerank):
     compare_oracle(dev, ix, queries[:40], 10, 8, f"c4 rabitq cosine rerank {rerank}")
 
 
-def test_c4_rabitq_mixed_l2_exact_distances(c4):
-    base, queries, ix = c4
+@pytest.mark.parametrize("metric", [1, 2])   # L2 on the unit vectors; cosine (the workload's)
+def test_c4_rabitq_mixed_exact_distances(c4, metric):
+    import dataclasses
+    base, queries, ix0 = c4
+    ix = dataclasses.replace(ix0, metric=metric)
     dev = DeviceIndex(ix)
     ids, dd, cnt = dev.search(queries, 10, 16, precision=PREC_MIXED, rerank_size=200)
     assert (cnt == 10).all()
@@ -173,6 +176,6 @@ def test_c4_rabitq_mixed_l2_exact_distances(c4):
     r = recall(ids, gt)
     e_ids, _, _ = dev.search(queries, 10, 16, precision=PREC_EXACT)
     re = recall(e_ids, gt)
-    print(f"c4 rabitq l2 recall@10 vs fp64: mixed {r:.4f}, exact search_rabitq {re:.4f}",
+    print(f"c4 rabitq metric {metric} recall@10 vs fp64: mixed {r:.4f}, exact {re:.4f}",
           flush=True)
     assert r >= re - 0.02

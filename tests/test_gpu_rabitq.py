@@ -132,3 +132,41 @@ def test_rabitq_mixed_equals_exact_f32_when_pool_covers_all():
     np.testing.assert_array_equal(e_c, m_c)
     np.testing.assert_array_equal(e_ids, m_ids)
     np.testing.assert_array_equal(e_d.view(np.uint32), m_d.view(np.uint32))
+
+
+@pytest.mark.parametrize("metric", ["l2", "l2_root", "cosine", "dot"])
+@pytest.mark.parametrize("dim", [32, 128, 300, 768])
+def test_rabitq_mixed_every_metric_f16_stage_equals_full_pool(metric, dim):
+    """Mixed precision over 1-bit codes for every metric (the L2 estimate
+    turned into the metric's estimate with exact per-entry norms), then the
+    f16-row stage ordering the pool by a proven bound and the exact f32
+    rerank in that order: bit-identical to reranking the whole pool in f32,
+    every distance the exact distance of its id, and recall stated against
+    FP64."""
+    from pydiskann_amd._lib import FLAG_FULL_RERANK
+    x = clustered(6000, dim, 30, seed=33)
+    q = clustered(48, dim, 30, seed=34)
+    if metric in ("cosine", "dot"):
+        x /= np.linalg.norm(x, axis=1, keepdims=True)
+        q /= np.linalg.norm(q, axis=1, keepdims=True)
+    ix = build_index(x, metric, "rabitq", n_postings=48, seed=0)
+    dev = DeviceIndex(ix)
+    print(f"[case] rabitq mixed f16 stage metric={metric} dim={dim}", flush=True)
+    for k, pool in ((10, 200), (100, 1000)):
+        a = dev.search(q, k, 16, precision=PREC_MIXED, rerank_size=pool)
+        b = dev.search(q, k, 16, precision=PREC_MIXED, rerank_size=pool, flags=FLAG_FULL_RERANK)
+        np.testing.assert_array_equal(a[2], b[2])
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    ids, dd, cnt = dev.search(q, 10, 16, precision=PREC_MIXED, rerank_size=200)
+    assert (cnt == 10).all()
+    for i in range(0, q.shape[0], 4):
+        for j in range(10):
+            want = oracle.eval_metric(ix.metric, q[i], ix.raw[ids[i, j]])
+            assert np.float32(dd[i, j]).view(np.uint32) == np.float32(want).view(np.uint32)
+        keys = list(zip(dd[i].tolist(), ids[i].tolist()))
+        assert keys == sorted(keys)
+    gt, _ = brute_force_f64(x, q, 10, ix.metric)
+    r = recall(ids, gt)
+    print(f"recall@10 vs fp64 = {r:.4f}")
+    assert r >= 0.9
