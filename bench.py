@@ -68,11 +68,120 @@ def knn_routing_graph(ix, torch, r: int):
                          centroid_beam=64, base_n=n)
 
 
+def rabitq_encode_device(torch, x, order, cent_of, padded, signs, chunk=131072):
+    """RaBitQ codes of rows x[order] against their posting centroids, on the
+    GPU, with the operation order of codecs.rabitq_encode (src/rabitq.rs:
+    136-149, 218-256): residual, sequential norm fold, normalise, 3 x (sign
+    flip, unnormalised FHT with the reference's butterfly pairing), sign bits,
+    |r|^2 and the estimator factor.  Square roots and divisions run in f64 and
+    round once to f32, which is the correctly rounded f32 result."""
+    import numpy as np
+    n, d = order.shape[0], x.shape[1]
+    out = np.zeros((n, padded // 8 + 8), dtype=np.uint8)
+    sg = torch.from_numpy(np.ascontiguousarray(signs)).cuda()
+    sd = torch.sqrt(torch.tensor(float(padded), dtype=torch.float32, device="cuda"))
+    bitw = (2 ** torch.arange(8, device="cuda", dtype=torch.int32)).view(1, 1, 8)
+    for s0 in range(0, n, chunk):
+        e0 = min(n, s0 + chunk)
+        rows = torch.from_numpy(x[order[s0:e0]]).cuda()
+        r = torch.zeros((e0 - s0, padded), dtype=torch.float32, device="cuda")
+        r[:, :d] = rows - torch.from_numpy(cent_of[s0:e0]).cuda()
+        sq = r * r
+        acc = torch.zeros(e0 - s0, dtype=torch.float32, device="cuda")
+        for i in range(padded):   # Iterator::sum: a left fold
+            acc = acc + sq[:, i]
+        norm = torch.sqrt(acc.double()).float()
+        nz = norm > 0
+        r[nz] = (r[nz].double() / norm[nz, None].double()).float()
+        for rr in range(3):
+            r = r * sg[rr][None, :]
+            h = 1
+            while h < padded:
+                a = r.view(e0 - s0, padded // (2 * h), 2, h)
+                xa, ya = a[:, :, 0, :].clone(), a[:, :, 1, :].clone()
+                a[:, :, 0, :] = xa + ya
+                a[:, :, 1, :] = xa - ya
+                h *= 2
+        ab = torch.abs(r)
+        ip = torch.zeros(e0 - s0, dtype=torch.float32, device="cuda")
+        for i in range(padded):
+            ip = ip + ab[:, i]
+        ip = (ip.double() / sd.double()).float()
+        fac = torch.where(ip > 0, (norm.double() / ip.double()).float(), torch.zeros_like(ip))
+        bits = ((r > 0).to(torch.int32).view(e0 - s0, padded // 8, 8) * bitw).sum(2)
+        out[s0:e0, :padded // 8] = bits.to(torch.uint8).cpu().numpy()
+        out[s0:e0, padded // 8:padded // 8 + 4] = (norm * norm).cpu().numpy().view(np.uint8).reshape(-1, 4)
+        out[s0:e0, padded // 8 + 4:] = fac.cpu().numpy().view(np.uint8).reshape(-1, 4)
+    return out
+
+
+def build_c4(args, torch):
+    """5M x 768 unit vectors (anisotropic spectrum), 8,192 postings by seeded
+    k-means, RaBitQ codes padded to 1,024 dims (136-byte codes), cosine."""
+    import numpy as np
+    from pydiskann_amd import codecs
+    from pydiskann_amd.datasets import config4
+    from pydiskann_amd.index import IndexArrays, assign_nearest, kmeans
+    t0 = time.time()
+    base, queries = config4(n=args.n_base or 5_000_000, nq=args.nq)
+    log(f"[bench] data {base.shape} queries {queries.shape} in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    n, d = base.shape
+    cent = kmeans(base, 8192, iters=6, seed=0, device="cuda")
+    assign = assign_nearest(base, cent, device="cuda")
+    cnt = np.bincount(assign, minlength=cent.shape[0])
+    keep = np.nonzero(cnt > 0)[0]
+    remap = np.full(cent.shape[0], -1, dtype=np.int64)
+    remap[keep] = np.arange(keep.shape[0])
+    assign = remap[assign]
+    cent = np.ascontiguousarray(cent[keep])
+    order = np.argsort(assign, kind="stable")
+    post_off = np.zeros(cent.shape[0] + 1, dtype=np.uint64)
+    post_off[1:] = np.cumsum(np.bincount(assign, minlength=cent.shape[0]))
+    padded = codecs.rabitq_padded(d)
+    signs = codecs.rabitq_signs(padded)
+    log(f"[bench] partition {cent.shape[0]} postings in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    cent_of = cent[assign[order]]
+    codes = rabitq_encode_device(torch, base, order, cent_of, padded, signs)
+    ref = codecs.rabitq_encode(base[order[:4096]], cent_of[:4096], padded, signs)
+    if not np.array_equal(ref, codes[:4096]):
+        raise RuntimeError("GPU RaBitQ encoder differs from the host codec")
+    del cent_of
+    log(f"[bench] RaBitQ codes ({codes.shape[1]} B, checked against the host codec on 4,096 "
+        f"rows) in {time.time() - t0:.1f}s")
+    ix = IndexArrays(dim=d, metric=2, quant=3, block_ids=np.arange(cent.shape[0], dtype=np.uint32),
+                     centroids=cent, post_off=post_off, ids=order.astype(np.uint64), codes=codes,
+                     raw=base, rq_padded=padded, rq_signs=signs)
+    return base, queries, ix, 48, 10, np
+
+
+def fp64_topk(torch, base, queries, k, metric):
+    """FP64 brute-force top k on the GPU (recall reference)."""
+    import numpy as np
+    x = torch.from_numpy(base).cuda()
+    out = []
+    xn = (x.double() * x.double()).sum(1)
+    for s in range(0, queries.shape[0], 50):
+        q = torch.from_numpy(queries[s:s + 50]).cuda().double()
+        ip = torch.cat([q @ x[c:c + 1_000_000].double().T for c in range(0, x.shape[0], 1_000_000)], 1)
+        if metric in (0, 1):
+            dd = xn[None, :] - 2 * ip + (q * q).sum(1)[:, None]
+        elif metric == 3:
+            dd = 1.0 - ip
+        else:
+            dd = 1.0 - ip / ((q * q).sum(1).sqrt()[:, None] * xn.sqrt()[None, :])
+        out.append(torch.topk(dd, k, dim=1, largest=False).indices.cpu().numpy())
+    return np.concatenate(out)
+
+
 def build_workload(args, torch):
     import numpy as np
     from pydiskann_amd import build_index
     from pydiskann_amd.datasets import config1, config2, config3, labels
 
+    if args.workload == "c4":
+        return build_c4(args, torch)
     t0 = time.time()
     if args.workload in ("c3", "c5"):
         # 10M x 96 unit vectors, one posting per mixture component (Zipf
@@ -261,9 +370,11 @@ def roofline_of(args, head, dim, nq, k, ix):
            "ridge_flop_per_byte": round(ridge, 1),
            "candidates_per_launch": prof["candidates"], "traffic_source": traffic_src}
     if flops:
+        # int8 matrix cores (5.0 POP/s dense) for int8 postings, else f16 (2.5 PFLOP/s)
+        peak = 5000.0 if (ix.quant == 2 and args.precision == "mixed") else MFMA_F16_TFLOPS
         tf = flops / (ms * 1e-3) / 1e12
-        out["scan_mfma"] = {"tflops": round(tf, 1), "peak_tflops": MFMA_F16_TFLOPS,
-                            "frac": round(tf / MFMA_F16_TFLOPS, 4), "flops": int(flops)}
+        out["scan_mfma"] = {"tflops": round(tf, 1), "peak_tflops": peak,
+                            "frac": round(tf / peak, 4), "flops": int(flops)}
     if prof["probe_flops"] and stage["probe"] > 0:
         tf = prof["probe_flops"] / (stage["probe"] * 1e-3) / 1e12
         out["probe_mfma" if args.precision == "mixed" else "probe_valu"] = {
@@ -277,6 +388,17 @@ def roofline_of(args, head, dim, nq, k, ix):
         out["traffic_GBs"] = round(tgb, 1)
         out["traffic_frac"] = round(tgb / HBM_PEAK_GBS, 4)
     return out
+
+
+def dtype_of(args, ix) -> str:
+    if args.precision == "exact":
+        return {0: "f32", 1: "f16 codes, f32", 2: "int8 codes (int32 L2 sums / f32 decode), f32 rerank",
+                3: "1-bit codes (popcount estimator or f32 decode), f32"}[ix.quant]
+    if ix.quant == 2:
+        return "int8 scan (int32 sums), bf16 probe, f32 rerank"
+    if ix.quant == 3:
+        return "1-bit codes (popcount estimator), bf16 probe, f16 then f32 rerank"
+    return "f16/bf16 scan+probe, f32 rerank"
 
 
 def cpu_model() -> str:
@@ -301,20 +423,28 @@ def cpu_baseline(args, ix, queries, k, n_probe, np):
     import oracle
     from pydiskann_amd import codecs
     from pydiskann_amd.index import IndexArrays
+    import dataclasses
     nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     cores = max(1, min(16, nproc or 1))
-    legs = {"exact": ix}
+    nq = queries.shape[0]
+    # (leg, index, queries on all cores, queries on 1 core)
+    legs = [("exact", ix, args.cpu_sample, args.cpu_sample // 10)]
     if ix.quant == 0 and args.workload != "c1":
-        legs[f"f16_rerank{args.rerank}"] = IndexArrays(
+        legs.append((f"f16_rerank{args.rerank}", IndexArrays(
             dim=ix.dim, metric=ix.metric, quant=1, block_ids=ix.block_ids,
             centroids=ix.centroids, post_off=ix.post_off, ids=ix.ids,
             codes=codecs.f16_encode(ix.raw[ix.ids.astype(np.int64)]), raw=ix.raw,
-            deleted=ix.deleted, rerank_size=args.rerank, probe_ratio=ix.probe_ratio)
-    nq = queries.shape[0]
+            deleted=ix.deleted, rerank_size=args.rerank, probe_ratio=ix.probe_ratio),
+            args.cpu_sample, args.cpu_sample // 10))
+    if ix.quant == 3 and ix.metric == 2:
+        # cosine decodes every probed code (src/spfresh.rs:346-348): a small
+        # sample; the L2 bound path on the same unit vectors beside it
+        legs = [("exact_cosine_decode", ix, 32, 4),
+                ("l2_bound_path", dataclasses.replace(ix, metric=1), 200, 20)]
     out = {}
-    for name, lix in legs.items():
+    for name, lix, n_all, n_one in legs:
         o = oracle.Oracle(lix)
-        for threads, ns in ((cores, min(args.cpu_sample, nq)), (1, min(args.cpu_sample // 10, nq))):
+        for threads, ns in ((cores, min(n_all, nq)), (1, min(n_one, nq))):
             qs = np.ascontiguousarray(queries[:ns])
             o.search_batch(qs, k, n_probe, threads=threads)   # warm-up
             ts = []
@@ -326,15 +456,14 @@ def cpu_baseline(args, ix, queries, k, n_probe, np):
             out[f"{name}_{threads}t"] = {"qps": round(ns / t, 1), "threads": threads,
                                         "queries": ns, "median_s": round(t, 4)}
             log(f"[bench] cpu {name} {threads} threads: {ns / t:,.1f} QPS ({ns} queries)")
-    head = f"f16_rerank{args.rerank}_{cores}t" if f"f16_rerank{args.rerank}_{cores}t" in out \
-        else f"exact_{cores}t"
-    like = "F16 postings + exact rerank of " + str(args.rerank) if head.startswith("f16") \
-        else "the exact path"
+    head = next(k for k in (f"f16_rerank{args.rerank}_{cores}t", f"exact_{cores}t",
+                            f"exact_cosine_decode_{cores}t") if k in out)
+    like = ("F16 postings + exact rerank of " + str(args.rerank) if head.startswith("f16")
+            else "the cosine decode path" if "cosine" in head else "the exact path")
     return {"value": out[head]["qps"], "unit": "queries/s", "cores": cores, "kind": "port",
             "sample": f"first {out[head]['queries']} of the {nq} queries ({like}, the same "
                       f"partition), C restatement of Inner::search, one warm-up then the "
-                      f"median of 5 runs; 1-thread legs over {min(args.cpu_sample // 10, nq)} "
-                      f"queries",
+                      f"median of 5 runs; 1-thread legs over a tenth or less of that",
             "nproc": nproc, "cpu_model": cpu_model(), "legs": out}
 
 
@@ -343,13 +472,16 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c5"], default="c2",
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c2",
                     help="c2: the headline batch search; c3: int8 inner-product batch search "
-                         "over 10M x 96; c5: filtered serving over the c3 index")
+                         "over 10M x 96; c4: RaBitQ cosine over 5M x 768; c5: filtered serving "
+                         "over the c3 index")
     ap.add_argument("--precision", choices=["exact", "mixed"], default="mixed")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip timing the other precision")
-    ap.add_argument("--rerank", type=int, default=1000)
+    ap.add_argument("--rerank", type=int, default=0,
+                    help="mixed-precision rerank pool (0 = the workload's: c2 1,000, c3 400, "
+                         "c4 300)")
     ap.add_argument("--n-base", type=int, default=0, help="base size (0 = the workload's)")
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--cpu-sample", type=int, default=1000)
@@ -370,6 +502,8 @@ def main() -> int:
     ap.add_argument("--max-batch", type=int, default=1024, help="c5: rows per device batch")
     ap.add_argument("--budget-gib", type=float, default=16.0, help="c5: working-memory budget")
     args = ap.parse_args()
+    if not args.rerank:
+        args.rerank = {"c3": 400, "c5": 400, "c4": 300}.get(args.workload, 1000)
     if args.gpus != 1:
         print(json.dumps({"error": "runs on one GPU; --gpus above 1 is not supported "
                           "(sharding queries or postings over GPUs is a design, not built)"}))
@@ -383,9 +517,6 @@ def main() -> int:
     base, queries, ix, n_probe, k, np = build_workload(args, torch)
     if args.workload == "c5":
         return serve(args, queries, ix, np)
-    if args.workload == "c3" and args.precision == "mixed":
-        log("[bench] c3 postings are int8: the exact path (int8 scan + f32 rerank) is timed")
-        args.precision, args.no_compare = "exact", True
     nq, dim = queries.shape
     dev = DeviceIndex(ix, device=0)
     q_d = torch.from_numpy(queries).cuda()
@@ -394,7 +525,7 @@ def main() -> int:
     cnt_d = torch.empty((nq,), dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
     ns = min(args.recall_sample, nq)
-    gt, _ = brute_force_f64(base, queries[:ns], 10, ix.metric, chunk=16)
+    gt = fp64_topk(torch, base, queries[:ns], 10, ix.metric)
 
     def run(precision: str):
         prec = PREC_EXACT if precision == "exact" else PREC_MIXED
@@ -464,7 +595,7 @@ def main() -> int:
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(total * 1e3 / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if args.precision == "exact" else "f16/bf16 scan+probe, f32 rerank",
+        "dtype": dtype_of(args, ix),
         "data": "synthetic (seeded numpy mixture; index by seeded k-means or, c3, one "
                 "posting per mixture component)",
         "config": {"workload": f"{args.workload}: {ix.n_raw}x{dim} {METRIC_NAMES[ix.metric]} "

@@ -177,6 +177,43 @@ int spf_index_create(const spf_index_desc *desc, spf_index **out);
 void spf_index_destroy(spf_index *ix);
 int spf_index_get_info(const spf_index *ix, spf_index_info *out);
 
+/* Incremental sync of the mirror after mutations of the reference index
+ * (Inner::insert / split / reassign / gc / compact, src/spfresh.rs:416-669),
+ * which all write postings through mark_dirty (src/spfresh.rs:283-289):
+ * the blocks written since the last sync (new content, or freed), the raw
+ * rows and labels of the ids appended by insert (src/spfresh.rs:425-433), the
+ * ids tombstoned since (SPFresh::delete, :867-874) and, after compact
+ * renumbers the centroid graph (:645-669), the new routing graph.  Postings
+ * not named keep their device entries (gathered into the new layout on the
+ * device); the result equals spf_index_create on the mutated state.  Must not
+ * overlap a search on the same handle.  SPF_EUNSUPPORTED when the mirror's
+ * entry keys cannot follow (ids beyond 32 bits): create the mirror anew. */
+typedef struct spf_block_update {
+    uint32_t block;                 /* reference block index */
+    uint32_t live;                  /* 0: the block was freed (free_posting, :406-414) */
+    const float *centroid;          /* [dim] block_centroid[block] (live blocks) */
+    uint64_t n_entries;             /* entries in posting order (Inner::entry, :267-270) */
+    const uint64_t *entry_ids;      /* [n_entries] */
+    const uint8_t *codes;           /* [n_entries * code_bytes] */
+} spf_block_update;
+
+typedef struct spf_index_update_desc {
+    const spf_block_update *blocks; /* blocks written since the last sync */
+    uint32_t n_blocks;
+    uint32_t reserved0;
+    uint64_t n_raw;                 /* next_id after the mutations (never below the mirror's) */
+    const float *raw_rows;          /* [(n_raw - previous n_raw) * dim] rows of the appended ids */
+    const uint8_t *labels;          /* [n_raw - previous n_raw] their labels (labelled mirrors) */
+    const uint64_t *deleted_ids;    /* ids tombstoned since the last sync (may be NULL) */
+    uint64_t n_deleted;
+    const spf_graph_desc *graph;    /* routing graph after the mutations, node_post holding each
+                                       node's reference block index (-1 = none); NULL = the
+                                       nodes keep their blocks */
+    uint32_t reserved[4];
+} spf_index_update_desc;
+
+int spf_index_update(spf_index *ix, const spf_index_update_desc *u);
+
 /* Tombstone ids (additive, ids >= n_raw ignored), like SPFresh::delete. */
 int spf_delete(spf_index *ix, const uint64_t *ids, uint64_t n);
 

@@ -164,6 +164,15 @@ class SPFresh:
     def dim(self) -> int:
         return self.arrays.dim
 
+    def sync(self, arrays: IndexArrays) -> int:
+        """Adopt a mutated state of the index (the reference's insert / gc /
+        compact run on its CPU index): the device mirror is updated in place
+        with the postings that changed (spf_index_update).  Returns the
+        number of blocks sent."""
+        n = self._dev.update(arrays)
+        self.arrays = arrays
+        return n
+
     def insert(self, vectors):
         raise NotImplementedError("insert runs on the reference's CPU index")
 

@@ -20,7 +20,8 @@ SPF_EAGAIN = -6
 PREC_EXACT, PREC_MIXED = 0, 1
 
 # every entry point of include/spf.h
-EXPORTS = ("spf_index_create", "spf_index_destroy", "spf_index_get_info", "spf_delete",
+EXPORTS = ("spf_index_create", "spf_index_destroy", "spf_index_get_info", "spf_index_update",
+           "spf_delete",
            "spf_search_batch", "spf_search_batch_device", "spf_search_batch_filtered",
            "spf_search_batch_filtered_device", "spf_search_batch_device_noalloc",
            "spf_nearest_postings", "spf_server_create",
@@ -56,6 +57,18 @@ class IndexDesc(C.Structure):
         ("probe_ratio", C.c_float), ("rerank_size", C.c_uint32), ("device", C.c_int32),
         ("labels", C.c_void_p), ("graph", C.POINTER(GraphDesc)),
     ]
+
+
+class BlockUpdate(C.Structure):
+    _fields_ = [("block", C.c_uint32), ("live", C.c_uint32), ("centroid", C.c_void_p),
+                ("n_entries", C.c_uint64), ("entry_ids", C.c_void_p), ("codes", C.c_void_p)]
+
+
+class UpdateDesc(C.Structure):
+    _fields_ = [("blocks", C.POINTER(BlockUpdate)), ("n_blocks", C.c_uint32),
+                ("reserved0", C.c_uint32), ("n_raw", C.c_uint64), ("raw_rows", C.c_void_p),
+                ("labels", C.c_void_p), ("deleted_ids", C.c_void_p), ("n_deleted", C.c_uint64),
+                ("graph", C.POINTER(GraphDesc)), ("reserved", C.c_uint32 * 4)]
 
 
 class SearchOpts(C.Structure):
@@ -130,6 +143,8 @@ def lib():
         L.spf_index_destroy.restype = None
         L.spf_index_get_info.argtypes = [P, C.POINTER(IndexInfo)]
         L.spf_index_get_info.restype = C.c_int
+        L.spf_index_update.argtypes = [P, C.POINTER(UpdateDesc)]
+        L.spf_index_update.restype = C.c_int
         L.spf_delete.argtypes = [P, P, u64]
         L.spf_delete.restype = C.c_int
         L.spf_search_batch.argtypes = [P, P, u32, u32, u32, u32, C.POINTER(SearchOpts), P, P, P]
@@ -195,6 +210,21 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+def _graph_desc(g, node_post):
+    keep = [np.ascontiguousarray(g.vectors, dtype=np.float32),
+            np.ascontiguousarray(g.adj_off, dtype=np.uint32),
+            np.ascontiguousarray(g.adj, dtype=np.uint32),
+            np.ascontiguousarray(g.live, dtype=np.uint8),
+            np.ascontiguousarray(node_post, dtype=np.int64)]
+    gd = GraphDesc()
+    gd.n_nodes, gd.n_seeds = keep[0].shape[0], len(g.seeds)
+    for i, sd in enumerate(g.seeds):
+        gd.seeds[i] = int(sd)
+    gd.centroid_beam, gd.n_tombstones = g.centroid_beam, g.n_tombstones
+    gd.vectors, gd.adj_off, gd.adj, gd.live, gd.node_post = (_p(a) for a in keep)
+    return gd, keep
+
+
 class DeviceIndex:
     """Device mirror of an ``IndexArrays`` (one GPU)."""
 
@@ -220,19 +250,8 @@ class DeviceIndex:
         self._graph = None
         g = getattr(ix, "graph", None)
         if g is not None:
-            keep = [np.ascontiguousarray(g.vectors, dtype=np.float32),
-                    np.ascontiguousarray(g.adj_off, dtype=np.uint32),
-                    np.ascontiguousarray(g.adj, dtype=np.uint32),
-                    np.ascontiguousarray(g.live, dtype=np.uint8),
-                    np.ascontiguousarray(g.node_post, dtype=np.int64)]
-            gd = GraphDesc()
-            gd.n_nodes, gd.n_seeds = keep[0].shape[0], len(g.seeds)
-            for i, sd in enumerate(g.seeds):
-                gd.seeds[i] = int(sd)
-            gd.centroid_beam, gd.n_tombstones = g.centroid_beam, g.n_tombstones
-            gd.vectors, gd.adj_off, gd.adj, gd.live, gd.node_post = (_p(a) for a in keep)
-            self._graph = (gd, keep)
-            d.graph = C.pointer(gd)
+            self._graph = _graph_desc(g, np.asarray(g.node_post, dtype=np.int64))
+            d.graph = C.pointer(self._graph[0])
         h = C.c_void_p()
         check(lib().spf_index_create(C.byref(d), C.byref(h)))
         self.h = h
@@ -257,6 +276,62 @@ class DeviceIndex:
     def delete(self, ids) -> None:
         a = np.ascontiguousarray(np.asarray(ids, dtype=np.uint64).ravel())
         check(lib().spf_delete(self.h, _p(a), a.shape[0]))
+
+    def update(self, new, blocks=None) -> int:
+        """Sync the mirror to the mutated index `new` (spf_index_update): the
+        blocks written since the mirror's state (every block whose centroid
+        or entries differ when `blocks` is None; the reference tracks them
+        with mark_dirty), the appended raw rows and labels, the new
+        tombstones and the routing graph.  Returns the number of blocks sent."""
+        from .index import changed_blocks
+        old = self._src
+        if new.n_raw < old.n_raw:
+            raise ValueError("the raw store cannot shrink")
+        if blocks is None:
+            blocks = changed_blocks(old, new)
+        pos = {int(b): i for i, b in enumerate(new.block_ids.tolist())}
+        arr = (BlockUpdate * max(1, len(blocks)))()
+        keep = []
+        for j, b in enumerate(blocks):
+            u = arr[j]
+            u.block = int(b)
+            i = pos.get(int(b))
+            if i is None:
+                u.live = 0
+                continue
+            s, e = int(new.post_off[i]), int(new.post_off[i + 1])
+            cent = np.ascontiguousarray(new.centroids[i])
+            ids = np.ascontiguousarray(new.ids[s:e])
+            codes = np.ascontiguousarray(new.codes[s:e])
+            keep += [cent, ids, codes]
+            u.live, u.centroid, u.n_entries = 1, _p(cent), e - s
+            u.entry_ids, u.codes = _p(ids), _p(codes)
+        d = UpdateDesc()
+        d.blocks, d.n_blocks, d.n_raw = arr, len(blocks), new.n_raw
+        raw = np.ascontiguousarray(new.raw[old.n_raw:])
+        d.raw_rows = _p(raw) if raw.shape[0] else None
+        lab = None
+        if getattr(new, "labels", None) is not None and new.n_raw > old.n_raw:
+            lab = np.ascontiguousarray(new.labels[old.n_raw:], dtype=np.uint8)
+            d.labels = _p(lab)
+        was = np.unpackbits(old.deleted, bitorder="little")[: old.n_raw]
+        now = np.unpackbits(new.deleted, bitorder="little")[: new.n_raw]
+        dead = np.nonzero(now[: old.n_raw] > was)[0].tolist() + \
+            (np.nonzero(now[old.n_raw:])[0] + old.n_raw).tolist()
+        dead = np.ascontiguousarray(np.asarray(dead, dtype=np.uint64))
+        d.deleted_ids, d.n_deleted = (_p(dead) if dead.size else None), dead.size
+        g = getattr(new, "graph", None)
+        gkeep = None
+        if g is not None and g is not getattr(old, "graph", None):
+            npost = np.asarray(g.node_post, dtype=np.int64)
+            node_block = np.where(npost >= 0, new.block_ids[np.maximum(npost, 0)].astype(np.int64), -1)
+            gkeep = _graph_desc(g, node_block)
+            d.graph = C.pointer(gkeep[0])
+        check(lib().spf_index_update(self.h, C.byref(d)))
+        self._src = new
+        self._labels = None if getattr(new, "labels", None) is None else new.labels
+        self._graph = gkeep or self._graph
+        return len(blocks)
 
     def search(self, queries, k: int, n_probe: int, precision: int = PREC_EXACT,
                rerank_size: int = 0, flags: int = 0, masks=None):

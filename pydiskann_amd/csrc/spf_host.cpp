@@ -12,6 +12,7 @@
 //   rank_id    u64 per rank; raw f32 rows per rank (dpad stride) for rerank
 //   tombstones 1 bit per entry
 #include <algorithm>
+#include <unordered_map>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -315,7 +316,7 @@ struct spf_index {
     uint32_t rq_bit_units = 0;
     uint16_t *d_raw16 = nullptr;
     float *d_rq_xn = nullptr;
-    uint64_t device_bytes = 0;
+    std::unordered_map<void *, size_t> sizes;   // device allocations and their bytes
     uint64_t gen = 0;                   // bumped when device buffers of the index move
     std::atomic<uint64_t> replays{0};   // calls served by a captured launch chain
     // working-memory bound inputs: top_len[j] = entries in the j longest
@@ -328,6 +329,12 @@ struct spf_index {
     uint32_t scan_g = 8, scan_chunk = kEChunk, mix_g = kMixQ;
 
     std::vector<uint32_t> h_post_len;   // entries per (sorted) posting
+    std::vector<uint32_t> h_post_off;   // [P + 1]
+    std::vector<uint32_t> h_block_sorted;   // reference block index per (sorted) posting
+    bool id_keys = false;               // entry keys carry the id (rank_id is the identity)
+    std::vector<uint64_t> h_ids;        // id of every entry (id_keys)
+    std::vector<uint8_t> h_labels;      // label of every id (filtered search)
+    std::vector<int32_t> h_gpost;       // device posting of every routing-graph node
     std::vector<uint64_t> rank_id;      // ids in ascending order
     std::vector<uint32_t> rank_entry;   // entry holding each rank
     std::vector<uint32_t> h_del;        // tombstone bit per entry
@@ -337,14 +344,7 @@ struct spf_index {
     std::vector<Workspace *> pool;
 
     ~spf_index() {
-        void *ps[] = {d_cent, d_cnorm, d_post_off, d_post_len, d_block_sorted, d_codes, d_rank,
-                      d_rank_id, d_raw, d_del, d_xnorm, d_xsq, d_rank_entry, d_raw_e, d_xn16,
-                      d_mu, d_cnbf, d_cent_rows, d_cbf, d_rq_signs, d_post_ex, d_post_xn,
-                      d_label_e, d_gvec, d_gadj_off, d_gadj, d_glive, d_gpost, d_raw16, d_rq_xn,
-                      d_rq_bits != d_codes ? d_rq_bits : nullptr};
-        for (void *p : ps)
-            if (p) (void)hipFree(p);
-        if (codes16_owned && d_codes16) (void)hipFree(d_codes16);
+        for (auto &kv : sizes) (void)hipFree(kv.first);   // every device array (aliases once)
         for (Workspace *w : pool) {
             w->release();
             delete w;
@@ -354,14 +354,6 @@ struct spf_index {
 
 namespace {
 
-template <class T>
-hipError_t upload(T **dst, const T *src, size_t n, uint64_t &bytes) {
-    hipError_t e = hipMalloc((void **)dst, std::max<size_t>(n, 1) * sizeof(T));
-    if (e != hipSuccess) return e;
-    bytes += std::max<size_t>(n, 1) * sizeof(T);
-    if (n) e = hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice);
-    return e;
-}
 
 // row-major rows of `row_bytes` -> tiled 16-byte units
 std::vector<uint4> tile_rows(const uint8_t *rows, uint64_t n, uint32_t row_bytes, uint32_t units,
@@ -426,6 +418,297 @@ struct DeviceGuard {
         if (prev >= 0) (void)hipSetDevice(prev);
     }
 };
+
+// ---- device arrays of an index: allocations are tracked so that an update
+// can replace arrays and the index knows the HBM it holds
+hipError_t dev_alloc(spf_index *ix, void **p, size_t bytes) {
+    bytes = std::max<size_t>(bytes, 1);
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) ix->sizes[*p] = bytes;
+    return e;
+}
+void dev_free(spf_index *ix, void *p) {
+    if (!p) return;
+    auto it = ix->sizes.find(p);
+    if (it == ix->sizes.end()) return;   // an alias of a tracked array
+    ix->sizes.erase(it);
+    (void)hipFree(p);
+}
+template <class T>
+void dev_free_as(spf_index *ix, T *&p) {
+    dev_free(ix, (void *)p);
+    p = nullptr;
+}
+template <class T>
+hipError_t dev_upload(spf_index *ix, T **dst, const T *src, size_t n) {
+    hipError_t e = dev_alloc(ix, (void **)dst, n * sizeof(T));
+    if (e == hipSuccess && n) e = hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+uint64_t dev_bytes(const spf_index *ix) {
+    uint64_t b = 0;
+    for (const auto &kv : ix->sizes) b += kv.second;
+    return b;
+}
+
+// Centroid arrays from the sorted centroid rows (P x dim): tiled f32 (exact
+// probe), the matrix-core probe's bf16 operands (L2: c - mu with mu the mean
+// centroid, which keeps the products small, key |cb|^2 + |qb|^2 - 2 qb.cb;
+// inner product and cosine: c itself, since a shift would change q.c, and for
+// cosine 1 / |c|), f32 rows (recheck, RaBitQ prep) and |c|^2 (cosine, int8).
+int upload_centroids(spf_index *ix, const float *rows_in) {
+    const uint32_t P = ix->P, dim = ix->dim;
+    dev_free_as(ix, ix->d_cent);
+    dev_free_as(ix, ix->d_cbf);
+    dev_free_as(ix, ix->d_mu);
+    dev_free_as(ix, ix->d_cnbf);
+    dev_free_as(ix, ix->d_cent_rows);
+    dev_free_as(ix, ix->d_cnorm);
+    {
+        std::vector<uint4> ct = tile_rows((const uint8_t *)rows_in, P, 4 * dim, ix->cent_units, nullptr);
+        HIPCHK(dev_upload(ix, &ix->d_cent, ct.data(), ct.size()));
+    }
+    const bool l2m = ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ;
+    const uint32_t dpad32 = round_up(dim, 32);
+    std::vector<double> mu(dim, 0.0);
+    if (l2m)
+        for (uint32_t i = 0; i < P; i++)
+            for (uint32_t j = 0; j < dim; j++) mu[j] += rows_in[(size_t)i * dim + j];
+    std::vector<float> muf(ix->dpad, 0.0f);
+    for (uint32_t j = 0; j < dim; j++) muf[j] = P && l2m ? (float)(mu[j] / P) : 0.0f;
+    auto bf16_rne = [](float f) -> uint16_t {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+        return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    };
+    std::vector<uint16_t> cb((size_t)P * dpad32, 0);
+    std::vector<float> cn(std::max<uint32_t>(P, 1), 0.0f), rows((size_t)P * ix->dpad, 0.0f);
+    for (uint32_t i = 0; i < P; i++) {
+        const float *c = rows_in + (size_t)i * dim;
+        float acc = 0.0f;
+        double c2 = 0.0;
+        for (uint32_t j = 0; j < dim; j++) {
+            const uint16_t h = bf16_rne(c[j] - muf[j]);
+            cb[(size_t)i * dpad32 + j] = h;
+            const uint32_t u = (uint32_t)h << 16;
+            float v;
+            std::memcpy(&v, &u, 4);
+            acc += v * v;
+            c2 += (double)c[j] * c[j];
+            rows[(size_t)i * ix->dpad + j] = c[j];
+        }
+        cn[i] = l2m ? acc : ix->metric == SPF_METRIC_COSINE ? (c2 > 0 ? (float)(1.0 / std::sqrt(c2)) : 0.0f)
+                                                             : 0.0f;
+    }
+    ix->cbf_units = dpad32 / 8;
+    std::vector<uint4> ct = tile_rows((const uint8_t *)cb.data(), P, 2 * dpad32, ix->cbf_units, nullptr);
+    HIPCHK(dev_upload(ix, &ix->d_cbf, ct.data(), ct.size()));
+    HIPCHK(dev_upload(ix, &ix->d_mu, muf.data(), muf.size()));
+    HIPCHK(dev_upload(ix, &ix->d_cnbf, cn.data(), cn.size()));
+    HIPCHK(dev_upload(ix, &ix->d_cent_rows, rows.data(), rows.size()));
+    if (ix->metric == SPF_METRIC_COSINE || ix->code_kind == CK_U8) {
+        HIPCHK(dev_alloc(ix, (void **)&ix->d_cnorm, std::max<uint32_t>(P, 1) * 4));
+        HIPCHK(launch_entry_norms(CK_F32, ix->d_cent, ix->cent_units, P, dim, 1.0f, 0.0f,
+                                  ix->d_cnorm, nullptr, nullptr));
+    }
+    return SPF_OK;
+}
+
+// Posting arrays and the working-memory bound inputs from h_post_len /
+// h_block_sorted: offsets, lengths, block ids, top_len[j] = entries in the j
+// longest postings, chunk counts at the VALU and matrix-core chunk sizes.
+int upload_postings(spf_index *ix) {
+    const uint32_t P = ix->P;
+    dev_free_as(ix, ix->d_post_off);
+    dev_free_as(ix, ix->d_post_len);
+    dev_free_as(ix, ix->d_block_sorted);
+    ix->h_post_off.assign(P + 1, 0);
+    for (uint32_t i = 0; i < P; i++) ix->h_post_off[i + 1] = ix->h_post_off[i] + ix->h_post_len[i];
+    HIPCHK(dev_upload(ix, &ix->d_post_off, ix->h_post_off.data(), P + 1));
+    std::vector<uint32_t> bs(ix->h_block_sorted);
+    if (bs.empty()) bs.push_back(0);
+    HIPCHK(dev_upload(ix, &ix->d_block_sorted, bs.data(), bs.size()));
+    std::vector<uint32_t> pl(ix->h_post_len);
+    if (pl.empty()) pl.push_back(0);
+    HIPCHK(dev_upload(ix, &ix->d_post_len, pl.data(), pl.size()));
+    std::vector<uint32_t> sorted(ix->h_post_len);
+    std::sort(sorted.begin(), sorted.end(), std::greater<uint32_t>());
+    ix->top_len.assign(P + 1, 0);
+    for (uint32_t i = 0; i < P; i++) ix->top_len[i + 1] = ix->top_len[i] + sorted[i];
+    ix->max_len = P ? sorted[0] : 0;
+    const uint32_t cm = mfma_chunk(round_up(ix->dim, 32) / 32);
+    ix->chunks_m = 0;
+    for (int c = 0; c < 4; c++) ix->chunks_v[c] = 0;
+    for (uint32_t len : ix->h_post_len) {
+        for (int c = 0; c < 4; c++) ix->chunks_v[c] += (len + (256u << c) - 1) / (256u << c);
+        ix->chunks_m += (len + cm - 1) / cm;
+    }
+    return SPF_OK;
+}
+
+// Arrays derived on the device from the entries' codes (d_codes, or the
+// RaBitQ bits d_rq_bits), raw rows d_raw_e and the posting offsets:
+// RaBitQ decode_with output (cosine / dot postings search the decoded
+// vectors, src/spfresh.rs:346-348), f16 rows / |x|^2 / per-posting bound of
+// the RaBitQ mixed path, the f16 codes of the matrix-core scan and the pruned
+// rerank's per-posting bounds, per-entry norms (cosine, int8) and u8 squares
+// (int8 L2).
+int derive_entries(spf_index *ix) {
+    const uint64_t N = ix->N;
+    const uint32_t P = ix->P, dim = ix->dim;
+    if (ix->quant == SPF_QUANT_RABITQ && !ix->rq_l2) {
+        if (ix->d_codes != ix->d_rq_bits) dev_free_as(ix, ix->d_codes);
+        uint32_t *d_entry_post = nullptr;
+        HIPCHK(hipMalloc((void **)&d_entry_post, std::max<uint64_t>(N, 1) * 4));
+        const size_t n_units = ((N + 63) / 64) * ix->units * 64;
+        hipError_t he = dev_alloc(ix, (void **)&ix->d_codes, std::max<size_t>(n_units, 1) * 16);
+        if (he == hipSuccess) he = hipMemset(ix->d_codes, 0, std::max<size_t>(n_units, 1) * 16);
+        if (he == hipSuccess) he = launch_entry_post(ix->d_post_off, P, (uint32_t)N, d_entry_post, nullptr);
+        if (he == hipSuccess) {
+            RqDecodeParams dp{};
+            dp.codes = ix->d_rq_bits;
+            dp.units = ix->rq_bit_units;
+            dp.padded = ix->rq_padded;
+            dp.dpad = ix->dpad;
+            dp.dim = dim;
+            dp.out_units = ix->units;
+            dp.signs = ix->d_rq_signs;
+            dp.cent = ix->d_cent_rows;
+            dp.entry_post = d_entry_post;
+            dp.out = ix->d_codes;
+            he = launch_rq_decode(dp, (uint32_t)N, nullptr);
+        }
+        if (he == hipSuccess) he = hipDeviceSynchronize();
+        (void)hipFree(d_entry_post);
+        HIPCHK(he);
+    }
+    if (ix->quant == SPF_QUANT_RABITQ) {
+        dev_free_as(ix, ix->d_raw16);
+        dev_free_as(ix, ix->d_rq_xn);
+        dev_free_as(ix, ix->d_post_ex);
+        HIPCHK(dev_alloc(ix, (void **)&ix->d_raw16, std::max<uint64_t>(N, 1) * ix->dpad * 2));
+        HIPCHK(dev_alloc(ix, (void **)&ix->d_rq_xn, std::max<uint64_t>(N, 1) * 4));
+        HIPCHK(dev_alloc(ix, (void **)&ix->d_post_ex, std::max<uint32_t>(P, 1) * 4));
+        HIPCHK(hipMemset(ix->d_post_ex, 0, std::max<uint32_t>(P, 1) * 4));
+        RqRowsParams rp{};
+        rp.raw_e = ix->d_raw_e;
+        rp.n = (uint32_t)N;
+        rp.dpad = ix->dpad;
+        rp.dim = dim;
+        rp.n_post = P;
+        rp.post_off = ix->d_post_off;
+        rp.raw16 = ix->d_raw16;
+        rp.xn = ix->d_rq_xn;
+        rp.post_ex = reinterpret_cast<uint32_t *>(ix->d_post_ex);
+        HIPCHK(launch_rq_rows(rp, nullptr));
+    }
+    // f16 codes for the matrix-core scan: the index's own f16 codes, or a
+    // round-to-nearest-even copy of f32 codes; then the pruned rerank's bounds
+    if (ix->quant == SPF_QUANT_F16 || ix->quant == SPF_QUANT_NONE) {
+        if (ix->codes16_owned) dev_free_as(ix, ix->d_codes16);
+        dev_free_as(ix, ix->d_post_ex);
+        dev_free_as(ix, ix->d_post_xn);
+        dev_free_as(ix, ix->d_xn16);
+        if (ix->quant == SPF_QUANT_F16) {
+            ix->d_codes16 = ix->d_codes;
+            ix->units16 = ix->units;
+            ix->codes16_owned = false;
+        } else {
+            ix->units16 = ix->dpad / 8;
+            const size_t n_units = ((N + 63) / 64) * ix->units16 * 64;
+            HIPCHK(dev_alloc(ix, (void **)&ix->d_codes16, std::max<size_t>(n_units, 1) * 16));
+            HIPCHK(hipMemset(ix->d_codes16, 0, std::max<size_t>(n_units, 1) * 16));
+            HIPCHK(launch_tiled_f16(ix->d_codes, ix->units, (uint32_t)N, ix->d_codes16, ix->units16,
+                                    nullptr));
+            ix->codes16_owned = true;
+        }
+        HIPCHK(dev_alloc(ix, (void **)&ix->d_post_ex, std::max<uint32_t>(P, 1) * 4));
+        HIPCHK(dev_alloc(ix, (void **)&ix->d_post_xn, std::max<uint32_t>(P, 1) * 4));
+        HIPCHK(hipMemset(ix->d_post_ex, 0, std::max<uint32_t>(P, 1) * 4));
+        HIPCHK(hipMemset(ix->d_post_xn, 0, std::max<uint32_t>(P, 1) * 4));
+        HIPCHK(launch_post_bounds(ix->d_raw_e, ix->quant == SPF_QUANT_F16 ? ix->d_codes16 : nullptr,
+                                  ix->units16, (uint32_t)N, ix->dpad, dim, ix->d_post_off, P,
+                                  reinterpret_cast<uint32_t *>(ix->d_post_ex),
+                                  reinterpret_cast<uint32_t *>(ix->d_post_xn), nullptr));
+        HIPCHK(dev_alloc(ix, (void **)&ix->d_xn16, std::max<uint64_t>(N, 1) * 4));
+        HIPCHK(launch_entry_norms(CK_F16, ix->d_codes16, ix->units16, (uint32_t)N, dim, 1.0f, 0.0f,
+                                  ix->d_xn16, nullptr, nullptr));
+    }
+    // norms for cosine (and |decode(c)|^2 of int8 codes for the matrix-core
+    // scan), u8 squares for int8 L2
+    dev_free_as(ix, ix->d_xnorm);
+    dev_free_as(ix, ix->d_xsq);
+    if (ix->metric == SPF_METRIC_COSINE || ix->code_kind == CK_U8) {
+        HIPCHK(dev_alloc(ix, (void **)&ix->d_xnorm, std::max<uint64_t>(N, 1) * 4));
+        HIPCHK(launch_entry_norms(ix->code_kind, ix->d_codes, ix->units, (uint32_t)N, dim,
+                                  ix->scale, ix->offset, ix->d_xnorm, nullptr, nullptr));
+    }
+    if (ix->code_kind == CK_U8 && (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ)) {
+        HIPCHK(dev_alloc(ix, (void **)&ix->d_xsq, std::max<uint64_t>(N, 1) * 4));
+        HIPCHK(launch_entry_norms(CK_U8, ix->d_codes, ix->units, (uint32_t)N, dim, ix->scale,
+                                  ix->offset, nullptr, ix->d_xsq, nullptr));
+    }
+    return SPF_OK;
+}
+
+// Routing graph arrays (checked); post_of maps a descriptor node_post value
+// to a device posting index (-1 = none, -2 = invalid).
+template <class PostOf>
+int upload_graph(spf_index *ix, const spf_graph_desc *g, PostOf &&post_of) {
+    const uint32_t G = g->n_nodes;
+    if (!g->vectors || !g->adj_off || !g->adj || !g->live || !g->node_post)
+        return fail(SPF_EINVAL, "graph: null array");
+    if (g->n_seeds > 2 || (G == 0 && g->n_seeds)) return fail(SPF_EINVAL, "graph: bad seeds");
+    for (uint32_t i = 0; i < g->n_seeds; i++)
+        if (g->seeds[i] >= G) return fail(SPF_EINVAL, "graph: seed beyond the nodes");
+    if (g->adj_off[0] != 0) return fail(SPF_EINVAL, "graph: adj_off[0] != 0");
+    for (uint32_t u = 0; u < G; u++)
+        if (g->adj_off[u + 1] < g->adj_off[u]) return fail(SPF_EINVAL, "graph: adj_off decreasing");
+    const uint32_t E = G ? g->adj_off[G] : 0;
+    for (uint32_t e = 0; e < E; e++)
+        if (g->adj[e] >= G) return fail(SPF_EINVAL, "graph: neighbor beyond the nodes");
+    std::vector<int32_t> post(std::max<uint32_t>(G, 1), -1);
+    for (uint32_t u = 0; u < G; u++) {
+        const int64_t pi = post_of(g->node_post[u]);
+        if (pi == -2) return fail(SPF_EINVAL, "graph: node posting beyond the postings");
+        post[u] = (int32_t)pi;
+    }
+    std::vector<float> vec((size_t)std::max<uint32_t>(G, 1) * ix->dpad, 0.0f);
+    for (uint32_t u = 0; u < G; u++)
+        std::memcpy(&vec[(size_t)u * ix->dpad], g->vectors + (size_t)u * ix->dim, ix->dim * 4);
+    dev_free_as(ix, ix->d_gvec);
+    dev_free_as(ix, ix->d_gadj_off);
+    dev_free_as(ix, ix->d_gadj);
+    dev_free_as(ix, ix->d_glive);
+    dev_free_as(ix, ix->d_gpost);
+    HIPCHK(dev_upload(ix, &ix->d_gvec, vec.data(), vec.size()));
+    HIPCHK(dev_upload(ix, &ix->d_gadj_off, g->adj_off, (size_t)G + 1));
+    HIPCHK(dev_upload(ix, &ix->d_gadj, g->adj, E));
+    HIPCHK(dev_upload(ix, &ix->d_glive, g->live, G));
+    HIPCHK(dev_upload(ix, &ix->d_gpost, post.data(), post.size()));
+    ix->h_gpost.assign(post.begin(), post.begin() + G);
+    ix->g_nodes = G;
+    ix->g_n_seeds = g->n_seeds;
+    ix->g_seeds[0] = g->seeds[0];
+    ix->g_seeds[1] = g->seeds[1];
+    ix->g_beam = g->centroid_beam;
+    ix->g_tomb = g->n_tombstones;
+    ix->has_graph = true;
+    return SPF_OK;
+}
+
+// Tombstone bit of every entry from the per-id set, uploaded.
+int upload_tombstones(spf_index *ix, const std::vector<uint64_t> &entry_id) {
+    ix->h_del.assign((ix->N + 31) / 32 + 1, 0);
+    for (uint64_t e = 0; e < ix->N; e++) {
+        const uint64_t id = entry_id[e];
+        if (id < ix->n_raw && ((ix->id_del[id >> 3] >> (id & 7)) & 1)) ix->h_del[e >> 5] |= 1u << (e & 31);
+    }
+    dev_free_as(ix, ix->d_del);
+    HIPCHK(dev_upload(ix, &ix->d_del, ix->h_del.data(), ix->h_del.size()));
+    return SPF_OK;
+}
 
 int create_impl(const spf_index_desc *d, spf_index **out) {
     if (!d || !out) return fail(SPF_EINVAL, "null argument");
@@ -526,307 +809,112 @@ int create_impl(const spf_index_desc *d, spf_index **out) {
         }
         post_off[P] = (uint32_t)o;
     }
-    // ranks
-    std::vector<uint32_t> by_id(N);
-    std::iota(by_id.begin(), by_id.end(), 0u);
-    std::sort(by_id.begin(), by_id.end(), [&](uint32_t a, uint32_t b) {
-        return d->entry_ids[entry_src[a]] < d->entry_ids[entry_src[b]];
-    });
-    ix->rank_id.resize(N);
-    ix->rank_entry.resize(N);
+    // entry keys: the id itself when every id fits 32 bits and the id span is
+    // not much larger than the entry count (keys order by (distance, id)
+    // either way; id keys do not shift when ids come and go, which the
+    // incremental update needs), else the rank of the id among all entry ids
+    std::vector<uint64_t> entry_id(N);
+    uint64_t max_id = 0;
+    for (uint64_t e = 0; e < N; e++) {
+        entry_id[e] = d->entry_ids[entry_src[e]];
+        max_id = std::max(max_id, entry_id[e]);
+        if (need_raw && entry_id[e] >= d->n_raw) return fail(SPF_EINVAL, "entry id beyond the raw store");
+    }
+    const uint64_t span = std::max<uint64_t>(d->n_raw, N ? max_id + 1 : 0);
+    ix->id_keys = span < 0x7fffffffull && span <= std::max<uint64_t>(4 * N, N + (1u << 22));
     std::vector<uint32_t> entry_rank(N);
-    for (uint64_t r = 0; r < N; r++) {
-        const uint32_t e = by_id[r];
-        ix->rank_id[r] = d->entry_ids[entry_src[e]];
-        ix->rank_entry[r] = e;
-        entry_rank[e] = (uint32_t)r;
-        if (r && ix->rank_id[r] == ix->rank_id[r - 1])
-            return fail(SPF_EINVAL, "an id appears in more than one entry");
-        if (need_raw && ix->rank_id[r] >= d->n_raw)
-            return fail(SPF_EINVAL, "entry id beyond the raw store");
+    if (ix->id_keys) {
+        ix->rank_id.resize(span);
+        std::iota(ix->rank_id.begin(), ix->rank_id.end(), 0ull);
+        ix->rank_entry.assign(span, 0xffffffffu);
+        for (uint64_t e = 0; e < N; e++) {
+            const uint64_t id = entry_id[e];
+            if (ix->rank_entry[id] != 0xffffffffu) return fail(SPF_EINVAL, "an id appears in more than one entry");
+            ix->rank_entry[id] = (uint32_t)e;
+            entry_rank[e] = (uint32_t)id;
+        }
+        ix->h_ids = entry_id;
+    } else {
+        std::vector<uint32_t> by_id(N);
+        std::iota(by_id.begin(), by_id.end(), 0u);
+        std::sort(by_id.begin(), by_id.end(),
+                  [&](uint32_t a, uint32_t b) { return entry_id[a] < entry_id[b]; });
+        ix->rank_id.resize(N);
+        ix->rank_entry.resize(N);
+        for (uint64_t r = 0; r < N; r++) {
+            const uint32_t e = by_id[r];
+            ix->rank_id[r] = entry_id[e];
+            ix->rank_entry[r] = e;
+            entry_rank[e] = (uint32_t)r;
+            if (r && ix->rank_id[r] == ix->rank_id[r - 1])
+                return fail(SPF_EINVAL, "an id appears in more than one entry");
+        }
     }
 
-    uint64_t &bytes = ix->device_bytes;
-    // centroids
+    // centroids and postings, in block order
     {
-        std::vector<uint4> ct =
-            tile_rows((const uint8_t *)d->centroids, P, 4 * d->dim, ix->cent_units, perm.data());
-        HIPCHK(upload(&ix->d_cent, ct.data(), ct.size(), bytes));
+        std::vector<float> rows((size_t)P * d->dim);
+        for (uint32_t i = 0; i < P; i++)
+            std::memcpy(&rows[(size_t)i * d->dim], d->centroids + (size_t)perm[i] * d->dim, 4 * d->dim);
+        if (int rc = upload_centroids(ix, rows.data())) return rc;
     }
-    {
-        // bf16 probe operands.  L2: c - mu (mu = mean centroid) keeps the
-        // products small and the key is |cb|^2 + |qb|^2 - 2 qb.cb; inner
-        // product and cosine take c itself (a shift would change q.c), the
-        // cosine key also 1 / |c|.
-        const bool l2m = d->metric == SPF_METRIC_L2 || d->metric == SPF_METRIC_L2SQ;
-        const uint32_t dpad32 = round_up(d->dim, 32);
-        std::vector<double> mu(d->dim, 0.0);
-        if (l2m)
-            for (uint32_t i = 0; i < P; i++)
-                for (uint32_t j = 0; j < d->dim; j++) mu[j] += d->centroids[(size_t)perm[i] * d->dim + j];
-        std::vector<float> muf(ix->dpad, 0.0f);
-        for (uint32_t j = 0; j < d->dim; j++) muf[j] = P && l2m ? (float)(mu[j] / P) : 0.0f;
-        auto bf16_rne = [](float f) -> uint16_t {
-            uint32_t u;
-            std::memcpy(&u, &f, 4);
-            if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-            return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-        };
-        std::vector<uint16_t> cb((size_t)P * dpad32, 0);
-        std::vector<float> cn(std::max<uint32_t>(P, 1), 0.0f), rows((size_t)P * ix->dpad, 0.0f);
-        for (uint32_t i = 0; i < P; i++) {
-            const float *c = d->centroids + (size_t)perm[i] * d->dim;
-            float acc = 0.0f;
-            double c2 = 0.0;
-            for (uint32_t j = 0; j < d->dim; j++) {
-                const uint16_t h = bf16_rne(c[j] - muf[j]);
-                cb[(size_t)i * dpad32 + j] = h;
-                const uint32_t u = (uint32_t)h << 16;
-                float v;
-                std::memcpy(&v, &u, 4);
-                acc += v * v;
-                c2 += (double)c[j] * c[j];
-                rows[(size_t)i * ix->dpad + j] = c[j];
-            }
-            cn[i] = l2m ? acc : d->metric == SPF_METRIC_COSINE ? (c2 > 0 ? (float)(1.0 / std::sqrt(c2)) : 0.0f)
-                                                                : 0.0f;
-        }
-        ix->cbf_units = dpad32 / 8;
-        std::vector<uint4> ct = tile_rows((const uint8_t *)cb.data(), P, 2 * dpad32, ix->cbf_units,
-                                          nullptr);
-        HIPCHK(upload(&ix->d_cbf, ct.data(), ct.size(), bytes));
-        HIPCHK(upload(&ix->d_mu, muf.data(), muf.size(), bytes));
-        HIPCHK(upload(&ix->d_cnbf, cn.data(), cn.size(), bytes));
-        HIPCHK(upload(&ix->d_cent_rows, rows.data(), rows.size(), bytes));
-    }
-    HIPCHK(upload(&ix->d_post_off, post_off.data(), post_off.size(), bytes));
-    {
-        std::vector<uint32_t> bs(std::max<uint32_t>(P, 1), 0u);
-        for (uint32_t i = 0; i < P; i++) bs[i] = d->block_ids[perm[i]];
-        HIPCHK(upload(&ix->d_block_sorted, bs.data(), bs.size(), bytes));
-    }
-    HIPCHK(upload(&ix->d_post_len, post_len.data(), post_len.size(), bytes));
     ix->h_post_len = post_len;
-    {
-        std::vector<uint32_t> sorted(post_len);
-        std::sort(sorted.begin(), sorted.end(), std::greater<uint32_t>());
-        ix->top_len.assign(P + 1, 0);
-        for (uint32_t i = 0; i < P; i++) ix->top_len[i + 1] = ix->top_len[i] + sorted[i];
-        const uint32_t cm = mfma_chunk(round_up(d->dim, 32) / 32);
-        for (uint32_t len : post_len) {
-            for (int c = 0; c < 4; c++) ix->chunks_v[c] += (len + (256u << c) - 1) / (256u << c);
-            ix->chunks_m += (len + cm - 1) / cm;
-        }
-    }
+    ix->h_block_sorted.resize(P);
+    for (uint32_t i = 0; i < P; i++) ix->h_block_sorted[i] = d->block_ids[perm[i]];
+    if (int rc = upload_postings(ix)) return rc;
+    // codes (RaBitQ: the bit codes; an L2 metric searches them directly)
     if (d->quantizer == SPF_QUANT_RABITQ) {
-        HIPCHK(upload(&ix->d_rq_signs, d->rabitq_signs, 3 * (size_t)ix->rq_padded, bytes));
-        const uint32_t bit_units = (d->code_bytes + 15) / 16;
-        std::vector<uint4> ct = tile_rows(d->codes, N, d->code_bytes, bit_units, entry_src.data());
-        ix->rq_bit_units = bit_units;
-        if (ix->rq_l2) {
-            HIPCHK(upload(&ix->d_codes, ct.data(), ct.size(), bytes));
-            ix->d_rq_bits = ix->d_codes;
-        } else {
-            // decode_with once per entry into tiled f32 codes
-            uint4 *d_bits = nullptr;
-            uint32_t *d_entry_post = nullptr;
-            uint64_t tmp_bytes = 0;
-            std::vector<uint32_t> entry_post(N);
-            for (uint32_t i = 0; i < P; i++)
-                for (uint32_t e = post_off[i]; e < post_off[i + 1]; e++) entry_post[e] = i;
-            const size_t n_units = ((N + 63) / 64) * ix->units * 64;
-            hipError_t he = upload(&d_bits, ct.data(), ct.size(), tmp_bytes);
-            if (he == hipSuccess) he = upload(&d_entry_post, entry_post.data(), entry_post.size(), tmp_bytes);
-            if (he == hipSuccess) he = hipMalloc((void **)&ix->d_codes, std::max<size_t>(n_units, 1) * 16);
-            if (he == hipSuccess) {
-                bytes += std::max<size_t>(n_units, 1) * 16;
-                he = hipMemset(ix->d_codes, 0, std::max<size_t>(n_units, 1) * 16);
-            }
-            if (he == hipSuccess) {
-                RqDecodeParams dp{};
-                dp.codes = d_bits;
-                dp.units = bit_units;
-                dp.padded = ix->rq_padded;
-                dp.dpad = ix->dpad;
-                dp.dim = d->dim;
-                dp.out_units = ix->units;
-                dp.signs = ix->d_rq_signs;
-                dp.cent = ix->d_cent_rows;
-                dp.entry_post = d_entry_post;
-                dp.out = ix->d_codes;
-                he = launch_rq_decode(dp, (uint32_t)N, nullptr);
-            }
-            if (he == hipSuccess) he = hipDeviceSynchronize();
-            ix->d_rq_bits = d_bits;   // kept for the mixed estimator scan
-            bytes += tmp_bytes - std::max<size_t>(N, 1) * 4;
-            if (d_entry_post) (void)hipFree(d_entry_post);
-            HIPCHK(he);
-        }
+        HIPCHK(dev_upload(ix, &ix->d_rq_signs, d->rabitq_signs, 3 * (size_t)ix->rq_padded));
+        ix->rq_bit_units = (d->code_bytes + 15) / 16;
+        std::vector<uint4> ct = tile_rows(d->codes, N, d->code_bytes, ix->rq_bit_units, entry_src.data());
+        HIPCHK(dev_upload(ix, &ix->d_rq_bits, ct.data(), ct.size()));
+        if (ix->rq_l2) ix->d_codes = ix->d_rq_bits;
     } else {
         std::vector<uint4> ct = tile_rows(d->codes, N, d->code_bytes, ix->units, entry_src.data());
-        HIPCHK(upload(&ix->d_codes, ct.data(), ct.size(), bytes));
+        HIPCHK(dev_upload(ix, &ix->d_codes, ct.data(), ct.size()));
     }
-    HIPCHK(upload(&ix->d_rank, entry_rank.data(), entry_rank.size(), bytes));
+    HIPCHK(dev_upload(ix, &ix->d_rank, entry_rank.data(), entry_rank.size()));
     if (d->labels) {   // label of each entry's id (the raw store covers every id)
+        ix->h_labels.assign(d->labels, d->labels + d->n_raw);
         std::vector<uint8_t> lab(std::max<uint64_t>(N, 1), 0);
         for (uint64_t e = 0; e < N; e++) {
-            const uint64_t id = d->entry_ids[entry_src[e]];
+            const uint64_t id = entry_id[e];
             if (id >= d->n_raw) return fail(SPF_EINVAL, "entry id beyond the label array");
             lab[e] = d->labels[id];
         }
-        HIPCHK(upload(&ix->d_label_e, lab.data(), lab.size(), bytes));
+        HIPCHK(dev_upload(ix, &ix->d_label_e, lab.data(), lab.size()));
     }
-    if (d->graph) {   // routing graph, checked and mapped to device posting indices
-        const spf_graph_desc *g = d->graph;
-        const uint32_t G = g->n_nodes;
-        if (!g->vectors || !g->adj_off || !g->adj || !g->live || !g->node_post)
-            return fail(SPF_EINVAL, "graph: null array");
-        if (g->n_seeds > 2 || (G == 0 && g->n_seeds)) return fail(SPF_EINVAL, "graph: bad seeds");
-        for (uint32_t i = 0; i < g->n_seeds; i++)
-            if (g->seeds[i] >= G) return fail(SPF_EINVAL, "graph: seed beyond the nodes");
-        if (g->adj_off[0] != 0) return fail(SPF_EINVAL, "graph: adj_off[0] != 0");
-        for (uint32_t u = 0; u < G; u++)
-            if (g->adj_off[u + 1] < g->adj_off[u]) return fail(SPF_EINVAL, "graph: adj_off decreasing");
-        const uint32_t E = G ? g->adj_off[G] : 0;
-        for (uint32_t e = 0; e < E; e++)
-            if (g->adj[e] >= G) return fail(SPF_EINVAL, "graph: neighbor beyond the nodes");
+    if (d->graph) {   // routing graph, node postings as sorted posting indices
         std::vector<uint32_t> inv(std::max<uint32_t>(P, 1), 0);
         for (uint32_t i = 0; i < P; i++) inv[perm[i]] = i;
-        std::vector<int32_t> post(std::max<uint32_t>(G, 1), -1);
-        for (uint32_t u = 0; u < G; u++) {
-            const int64_t pi = g->node_post[u];
-            if (pi >= (int64_t)P) return fail(SPF_EINVAL, "graph: node posting beyond the postings");
-            post[u] = pi < 0 ? -1 : (int32_t)inv[(uint32_t)pi];
-        }
-        std::vector<float> vec((size_t)std::max<uint32_t>(G, 1) * ix->dpad, 0.0f);
-        for (uint32_t u = 0; u < G; u++)
-            std::memcpy(&vec[(size_t)u * ix->dpad], g->vectors + (size_t)u * d->dim, d->dim * 4);
-        HIPCHK(upload(&ix->d_gvec, vec.data(), vec.size(), bytes));
-        HIPCHK(upload(&ix->d_gadj_off, g->adj_off, (size_t)G + 1, bytes));
-        HIPCHK(upload(&ix->d_gadj, g->adj, E, bytes));
-        HIPCHK(upload(&ix->d_glive, g->live, G, bytes));
-        HIPCHK(upload(&ix->d_gpost, post.data(), post.size(), bytes));
-        ix->g_nodes = G;
-        ix->g_n_seeds = g->n_seeds;
-        ix->g_seeds[0] = g->seeds[0];
-        ix->g_seeds[1] = g->seeds[1];
-        ix->g_beam = g->centroid_beam;
-        ix->g_tomb = g->n_tombstones;
-        ix->has_graph = true;
+        if (int rc = upload_graph(ix, d->graph, [&](int64_t pi) -> int64_t {
+                return pi < 0 ? -1 : pi >= (int64_t)P ? -2 : (int64_t)inv[(uint32_t)pi];
+            }))
+            return rc;
     }
-    HIPCHK(upload(&ix->d_rank_id, ix->rank_id.data(), ix->rank_id.size(), bytes));
-    HIPCHK(upload(&ix->d_rank_entry, ix->rank_entry.data(), ix->rank_entry.size(), bytes));
+    HIPCHK(dev_upload(ix, &ix->d_rank_id, ix->rank_id.data(), ix->rank_id.size()));
+    HIPCHK(dev_upload(ix, &ix->d_rank_entry, ix->rank_entry.data(), ix->rank_entry.size()));
     {
         // raw f32 rows in entry order: the rerank source of both precisions
         std::vector<float> rr((size_t)N * ix->dpad, 0.0f);
         for (uint64_t e = 0; e < N; e++) {
             const float *src = need_raw
-                                   ? d->raw + (size_t)d->entry_ids[entry_src[e]] * d->dim
+                                   ? d->raw + (size_t)entry_id[e] * d->dim
                                    : (const float *)(d->codes + (size_t)entry_src[e] * d->code_bytes);
             std::memcpy(&rr[(size_t)e * ix->dpad], src, 4 * d->dim);
         }
-        HIPCHK(upload(&ix->d_raw_e, rr.data(), rr.size(), bytes));
-    }
-    if (d->quantizer == SPF_QUANT_RABITQ) {
-        // f16 rows, |x|^2 and per-posting max |x - f16(x)| for the mixed path
-        HIPCHK(hipMalloc((void **)&ix->d_raw16, std::max<uint64_t>(N, 1) * ix->dpad * 2));
-        HIPCHK(hipMalloc((void **)&ix->d_rq_xn, std::max<uint64_t>(N, 1) * 4));
-        HIPCHK(hipMalloc((void **)&ix->d_post_ex, std::max<uint32_t>(P, 1) * 4));
-        bytes += std::max<uint64_t>(N, 1) * (ix->dpad * 2 + 4) + std::max<uint32_t>(P, 1) * 4;
-        HIPCHK(hipMemset(ix->d_post_ex, 0, std::max<uint32_t>(P, 1) * 4));
-        RqRowsParams rp{};
-        rp.raw_e = ix->d_raw_e;
-        rp.n = (uint32_t)N;
-        rp.dpad = ix->dpad;
-        rp.dim = d->dim;
-        rp.n_post = P;
-        rp.post_off = ix->d_post_off;
-        rp.raw16 = ix->d_raw16;
-        rp.xn = ix->d_rq_xn;
-        rp.post_ex = reinterpret_cast<uint32_t *>(ix->d_post_ex);
-        HIPCHK(launch_rq_rows(rp, nullptr));
-    }
-    // f16 codes for the matrix-core scan: the index's own f16 codes, or an
-    // IEEE round-to-nearest-even copy of f32 codes
-    if (d->quantizer == SPF_QUANT_F16) {
-        ix->d_codes16 = ix->d_codes;
-        ix->units16 = ix->units;
-    } else if (d->quantizer == SPF_QUANT_NONE) {
-        ix->units16 = ix->dpad / 8;
-        std::vector<_Float16> h((size_t)N * d->dim);
-        for (uint64_t e = 0; e < N; e++) {
-            const float *src = (const float *)(d->codes + (size_t)entry_src[e] * d->code_bytes);
-            for (uint32_t i = 0; i < d->dim; i++) h[(size_t)e * d->dim + i] = (_Float16)src[i];
-        }
-        std::vector<uint4> ct = tile_rows((const uint8_t *)h.data(), N, 2 * d->dim, ix->units16,
-                                          nullptr);
-        HIPCHK(upload(&ix->d_codes16, ct.data(), ct.size(), bytes));
-        ix->codes16_owned = true;
-    }
-    if (ix->d_codes16) {
-        // bound inputs of the pruned rerank, in double and rounded up
-        std::vector<float> pex(std::max<uint32_t>(P, 1), 0.0f), pxn(std::max<uint32_t>(P, 1), 0.0f);
-        for (uint32_t i = 0; i < P; i++) {
-            double ex = 0.0, xn = 0.0;
-            for (uint32_t e = post_off[i]; e < post_off[i + 1]; e++) {
-                const float *raw = d->quantizer == SPF_QUANT_NONE
-                    ? (const float *)(d->codes + (size_t)entry_src[e] * d->code_bytes)
-                    : d->raw + (size_t)d->entry_ids[entry_src[e]] * d->dim;
-                const _Float16 *c16 = d->quantizer == SPF_QUANT_F16
-                    ? (const _Float16 *)(d->codes + (size_t)entry_src[e] * d->code_bytes) : nullptr;
-                double s2 = 0.0, n2 = 0.0;
-                for (uint32_t j = 0; j < d->dim; j++) {
-                    const double h = c16 ? (double)(float)c16[j] : (double)(float)(_Float16)raw[j];
-                    const double t = (double)raw[j] - h;
-                    s2 += t * t;
-                    n2 += h * h;
-                }
-                ex = std::max(ex, std::sqrt(s2));
-                xn = std::max(xn, n2);
-            }
-            pex[i] = std::nextafter((float)(ex * (1.0 + 1e-12)), INFINITY);
-            pxn[i] = std::nextafter((float)(xn * (1.0 + 1e-12)), INFINITY);
-        }
-        HIPCHK(upload(&ix->d_post_ex, pex.data(), pex.size(), bytes));
-        HIPCHK(upload(&ix->d_post_xn, pxn.data(), pxn.size(), bytes));
-        HIPCHK(hipMalloc((void **)&ix->d_xn16, std::max<uint64_t>(N, 1) * 4));
-        bytes += 4 * N;
-        HIPCHK(launch_entry_norms(CK_F16, ix->d_codes16, ix->units16, (uint32_t)N, d->dim, 1.0f,
-                                  0.0f, ix->d_xn16, nullptr, nullptr));
+        HIPCHK(dev_upload(ix, &ix->d_raw_e, rr.data(), rr.size()));
     }
     // tombstones
     ix->id_del.assign((d->n_raw + 7) / 8, 0);
-    ix->h_del.assign((N + 31) / 32 + 1, 0);
     for (uint64_t i = 0; i < d->n_deleted; i++) {
         const uint64_t id = d->deleted_ids[i];
         if (id >= d->n_raw || ((ix->id_del[id >> 3] >> (id & 7)) & 1)) continue;
         ix->id_del[id >> 3] |= (uint8_t)(1u << (id & 7));
         ix->n_deleted++;
-        auto it = std::lower_bound(ix->rank_id.begin(), ix->rank_id.end(), id);
-        if (it != ix->rank_id.end() && *it == id) {
-            const uint32_t e = ix->rank_entry[it - ix->rank_id.begin()];
-            ix->h_del[e >> 5] |= 1u << (e & 31);
-        }
     }
-    HIPCHK(upload(&ix->d_del, ix->h_del.data(), ix->h_del.size(), bytes));
-    // norms for cosine (and |decode(c)|^2 of int8 codes for the matrix-core
-    // scan), u8 squares for int8 L2
-    if (d->metric == SPF_METRIC_COSINE || ix->code_kind == CK_U8) {
-        HIPCHK(hipMalloc((void **)&ix->d_xnorm, std::max<uint64_t>(N, 1) * 4));
-        HIPCHK(hipMalloc((void **)&ix->d_cnorm, std::max<uint32_t>(P, 1) * 4));
-        bytes += 4 * (N + P);
-        HIPCHK(launch_entry_norms(ix->code_kind, ix->d_codes, ix->units, (uint32_t)N, d->dim,
-                                  ix->scale, ix->offset, ix->d_xnorm, nullptr, nullptr));
-        HIPCHK(launch_entry_norms(CK_F32, ix->d_cent, ix->cent_units, P, d->dim, 1.0f, 0.0f,
-                                  ix->d_cnorm, nullptr, nullptr));
-    }
-    if (ix->code_kind == CK_U8 &&
-        (d->metric == SPF_METRIC_L2 || d->metric == SPF_METRIC_L2SQ)) {
-        HIPCHK(hipMalloc((void **)&ix->d_xsq, std::max<uint64_t>(N, 1) * 4));
-        bytes += 4 * N;
-        HIPCHK(launch_entry_norms(CK_U8, ix->d_codes, ix->units, (uint32_t)N, d->dim, ix->scale,
-                                  ix->offset, nullptr, ix->d_xsq, nullptr));
-    }
+    if (int rc = upload_tombstones(ix, entry_id)) return rc;
+    if (int rc = derive_entries(ix)) return rc;
     HIPCHK(hipDeviceSynchronize());
     *out = guard.release();
     return SPF_OK;
@@ -843,6 +931,240 @@ struct Shape {
     bool graph;   // probe through the routing graph (above kBruteCentroids live postings)
     uint32_t nraw;   // the probe count as asked (the graph's result count and beam)
 };
+
+// Incremental update of the mirror after mutations of the reference index
+// (spf_index_update).  Postings not named by the update keep their device
+// entries, which are gathered into the new layout on the device; only the
+// named blocks' entries, the appended raw rows, labels and tombstones cross
+// the bus.  The result equals a mirror created from the mutated state.
+int update_impl(spf_index *ix, const spf_index_update_desc *u) {
+    if (!ix || !u) return fail(SPF_EINVAL, "null argument");
+    if (!ix->id_keys)
+        return fail(SPF_EUNSUPPORTED, "this mirror keys entries by rank (ids beyond 32 bits or "
+                                      "sparse ids): create it anew");
+    if (u->n_blocks && !u->blocks) return fail(SPF_EINVAL, "blocks missing");
+    if (u->n_raw < ix->n_raw) return fail(SPF_EINVAL, "n_raw below the mirror's");
+    const uint64_t n_add = u->n_raw - ix->n_raw, old_raw = ix->n_raw;
+    const bool need_raw = ix->quant != SPF_QUANT_NONE;
+    if (n_add && need_raw && !u->raw_rows) return fail(SPF_EINVAL, "appended ids need their raw rows");
+    if (n_add && !ix->h_labels.empty() && !u->labels)
+        return fail(SPF_EINVAL, "appended ids need their labels");
+    if (u->n_deleted && !u->deleted_ids) return fail(SPF_EINVAL, "deleted ids missing");
+    const uint32_t dim = ix->dim, cb = ix->code_bytes;
+    // updated blocks in block order
+    std::vector<uint32_t> ub(u->n_blocks);
+    std::iota(ub.begin(), ub.end(), 0u);
+    std::sort(ub.begin(), ub.end(), [&](uint32_t a, uint32_t b) { return u->blocks[a].block < u->blocks[b].block; });
+    for (uint32_t i = 0; i < u->n_blocks; i++) {
+        const spf_block_update &bu = u->blocks[ub[i]];
+        if (i && bu.block == u->blocks[ub[i - 1]].block) return fail(SPF_EINVAL, "a block is named twice");
+        if (bu.live && !bu.centroid) return fail(SPF_EINVAL, "a live block needs its centroid");
+        if (bu.live && bu.n_entries && (!bu.entry_ids || !bu.codes))
+            return fail(SPF_EINVAL, "entry arrays missing");
+    }
+    // the new postings: old ones not named, then named live blocks, in block order
+    struct NewPost { uint32_t block; int64_t old, up; };
+    std::vector<NewPost> np;
+    {
+        size_t i = 0, j = 0;
+        const size_t P0 = ix->P, U = ub.size();
+        while (i < P0 || j < U) {
+            const uint32_t bo = i < P0 ? ix->h_block_sorted[i] : 0xffffffffu;
+            const uint32_t bu = j < U ? u->blocks[ub[j]].block : 0xffffffffu;
+            if (j < U && (i >= P0 || bu <= bo)) {
+                if (u->blocks[ub[j]].live) np.push_back({bu, -1, (int64_t)ub[j]});
+                if (i < P0 && bo == bu) i++;   // replaced or freed
+                j++;
+            } else {
+                np.push_back({bo, (int64_t)i, -1});
+                i++;
+            }
+        }
+    }
+    const uint32_t P = (uint32_t)np.size();
+    std::vector<uint32_t> post_len(P);
+    uint64_t N = 0, n_up = 0;
+    for (uint32_t i = 0; i < P; i++) {
+        post_len[i] = np[i].old >= 0 ? ix->h_post_len[np[i].old]
+                                     : (uint32_t)u->blocks[np[i].up].n_entries;
+        N += post_len[i];
+        if (np[i].up >= 0) n_up += post_len[i];
+    }
+    if (N >= 0x7fffffffull) return fail(SPF_EINVAL, "more than 2^31-1 entries");
+    // entry ids and sources: bit 31 set = the upload, else the old entry
+    std::vector<uint64_t> ids(N);
+    std::vector<uint32_t> src_code(N), src_raw(N);
+    std::vector<uint8_t> up_codes((size_t)n_up * cb);
+    uint64_t max_id = 0;
+    {
+        uint64_t e = 0, j = 0;
+        for (uint32_t i = 0; i < P; i++) {
+            if (np[i].old >= 0) {
+                const uint32_t o0 = ix->h_post_off[np[i].old];
+                for (uint32_t t = 0; t < post_len[i]; t++, e++) {
+                    ids[e] = ix->h_ids[o0 + t];
+                    src_code[e] = src_raw[e] = o0 + t;
+                    max_id = std::max(max_id, ids[e]);
+                }
+            } else {
+                const spf_block_update &bu = u->blocks[np[i].up];
+                for (uint64_t t = 0; t < bu.n_entries; t++, e++, j++) {
+                    const uint64_t id = bu.entry_ids[t];
+                    ids[e] = id;
+                    max_id = std::max(max_id, id);
+                    std::memcpy(&up_codes[(size_t)j * cb], bu.codes + (size_t)t * cb, cb);
+                    src_code[e] = 0x80000000u | (uint32_t)j;
+                    if (need_raw && id >= u->n_raw) return fail(SPF_EINVAL, "entry id beyond the raw store");
+                    if (!need_raw) {   // f32 codes are the raw rows
+                        src_raw[e] = 0x80000000u | (uint32_t)j;
+                    } else if (id < ix->rank_entry.size() && ix->rank_entry[id] != 0xffffffffu) {
+                        src_raw[e] = ix->rank_entry[id];   // the id's row moves with it
+                    } else if (id >= old_raw) {
+                        src_raw[e] = 0x80000000u | (uint32_t)(id - old_raw);
+                    } else {
+                        return fail(SPF_EINVAL, "entry id has no raw row on the device (neither "
+                                                "an entry before nor appended)");
+                    }
+                }
+            }
+        }
+    }
+    const uint64_t span = std::max<uint64_t>(u->n_raw, N ? max_id + 1 : 0);
+    if (!(span < 0x7fffffffull && span <= std::max<uint64_t>(4 * N, N + (1u << 22))))
+        return fail(SPF_EUNSUPPORTED, "ids too sparse for entry keys: create the mirror anew");
+    std::vector<uint32_t> rank_entry(span, 0xffffffffu);
+    for (uint64_t e = 0; e < N; e++) {
+        if (rank_entry[ids[e]] != 0xffffffffu) return fail(SPF_EINVAL, "an id appears in more than one entry");
+        rank_entry[ids[e]] = (uint32_t)e;
+    }
+    DeviceGuard dg(ix->device);
+    // gather the entries' codes and raw rows into the new layout
+    const bool rq = ix->quant == SPF_QUANT_RABITQ;
+    const uint32_t cu = rq ? ix->rq_bit_units : ix->units;
+    const uint4 *old_codes = rq ? ix->d_rq_bits : ix->d_codes;
+    uint4 *codes = nullptr, *up_t = nullptr;
+    float *raw = nullptr, *up_raw = nullptr;
+    uint32_t *d_src = nullptr;
+    auto cleanup = [&]() {
+        if (up_t) (void)hipFree(up_t);
+        if (up_raw) (void)hipFree(up_raw);
+        if (d_src) (void)hipFree(d_src);
+    };
+    hipError_t he = hipSuccess;
+    {
+        std::vector<uint4> t = tile_rows(up_codes.data(), n_up, cb, cu, nullptr);
+        const size_t n_units = ((N + 63) / 64) * cu * 64;
+        he = dev_alloc(ix, (void **)&codes, std::max<size_t>(n_units, 1) * 16);
+        if (he == hipSuccess) he = hipMemset(codes, 0, std::max<size_t>(n_units, 1) * 16);
+        if (he == hipSuccess) he = hipMalloc((void **)&up_t, std::max<size_t>(t.size(), 1) * 16);
+        if (he == hipSuccess && !t.empty()) he = hipMemcpy(up_t, t.data(), t.size() * 16, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMalloc((void **)&d_src, std::max<uint64_t>(N, 1) * 4);
+        if (he == hipSuccess && N) he = hipMemcpy(d_src, src_code.data(), N * 4, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = launch_gather_tiled(old_codes, up_t, d_src, (uint32_t)N, cu, codes, nullptr);
+        if (he == hipSuccess) he = dev_alloc(ix, (void **)&raw, std::max<uint64_t>(N, 1) * ix->dpad * 4);
+        const float *rows = need_raw ? u->raw_rows : (const float *)up_codes.data();
+        const uint64_t n_rows = need_raw ? n_add : n_up;
+        if (he == hipSuccess) he = hipMalloc((void **)&up_raw, std::max<uint64_t>(n_rows * dim, 1) * 4);
+        if (he == hipSuccess && n_rows) he = hipMemcpy(up_raw, rows, n_rows * dim * 4, hipMemcpyHostToDevice);
+        if (he == hipSuccess && N) he = hipMemcpy(d_src, src_raw.data(), N * 4, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = launch_gather_rows(ix->d_raw_e, up_raw, d_src, (uint32_t)N, ix->dpad, dim,
+                                                      dim, raw, nullptr);
+        if (he == hipSuccess) he = hipDeviceSynchronize();
+    }
+    cleanup();
+    if (he != hipSuccess) {
+        dev_free_as(ix, codes);
+        dev_free_as(ix, raw);
+        return fail(he == hipErrorOutOfMemory ? SPF_ENOMEM : SPF_EDEVICE,
+                    std::string("update gather: ") + hipGetErrorString(he));
+    }
+    // centroid rows of the new postings (old rows read back)
+    std::vector<float> old_rows((size_t)ix->P * ix->dpad), rows((size_t)P * dim);
+    if (ix->P)
+        HIPCHK(hipMemcpy(old_rows.data(), ix->d_cent_rows, old_rows.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < P; i++) {
+        const float *c = np[i].old >= 0 ? &old_rows[(size_t)np[i].old * ix->dpad]
+                                        : u->blocks[np[i].up].centroid;
+        std::memcpy(&rows[(size_t)i * dim], c, 4 * dim);
+    }
+    // graph node postings: old posting -> block -> new posting
+    std::vector<uint32_t> new_blocks(P);
+    for (uint32_t i = 0; i < P; i++) new_blocks[i] = np[i].block;
+    auto post_of_block = [&](int64_t b) -> int64_t {
+        if (b < 0) return -1;
+        auto it = std::lower_bound(new_blocks.begin(), new_blocks.end(), (uint64_t)b);
+        return it != new_blocks.end() && *it == (uint64_t)b ? (int64_t)(it - new_blocks.begin()) : -1;
+    };
+    std::vector<int32_t> gpost;
+    if (!u->graph && ix->has_graph)
+        for (int32_t op : ix->h_gpost)
+            gpost.push_back(op < 0 ? -1 : (int32_t)post_of_block(ix->h_block_sorted[op]));
+    // swap in the new entries and postings
+    if (ix->d_codes != ix->d_rq_bits) dev_free_as(ix, ix->d_codes);
+    dev_free_as(ix, ix->d_rq_bits);
+    ix->d_codes = nullptr;
+    if (!ix->codes16_owned) ix->d_codes16 = nullptr;
+    dev_free_as(ix, ix->d_raw_e);
+    if (rq) {
+        ix->d_rq_bits = codes;
+        if (ix->rq_l2) ix->d_codes = codes;
+    } else {
+        ix->d_codes = codes;
+    }
+    ix->d_raw_e = raw;
+    ix->P = P;
+    ix->N = N;
+    ix->n_raw = u->n_raw;
+    ix->h_ids = std::move(ids);
+    ix->h_post_len = post_len;
+    ix->h_block_sorted = new_blocks;
+    if (int rc = upload_postings(ix)) return rc;
+    if (int rc = upload_centroids(ix, rows.data())) return rc;
+    {
+        std::vector<uint32_t> er(std::max<uint64_t>(N, 1), 0);
+        for (uint64_t e = 0; e < N; e++) er[e] = (uint32_t)ix->h_ids[e];
+        dev_free_as(ix, ix->d_rank);
+        HIPCHK(dev_upload(ix, &ix->d_rank, er.data(), er.size()));
+    }
+    if (!ix->h_labels.empty()) {
+        ix->h_labels.insert(ix->h_labels.end(), u->labels, u->labels + n_add);
+        std::vector<uint8_t> lab(std::max<uint64_t>(N, 1), 0);
+        for (uint64_t e = 0; e < N; e++) lab[e] = ix->h_labels[ix->h_ids[e]];
+        dev_free_as(ix, ix->d_label_e);
+        HIPCHK(dev_upload(ix, &ix->d_label_e, lab.data(), lab.size()));
+    }
+    ix->rank_id.resize(span);
+    std::iota(ix->rank_id.begin(), ix->rank_id.end(), 0ull);
+    ix->rank_entry = std::move(rank_entry);
+    dev_free_as(ix, ix->d_rank_id);
+    dev_free_as(ix, ix->d_rank_entry);
+    HIPCHK(dev_upload(ix, &ix->d_rank_id, ix->rank_id.data(), ix->rank_id.size()));
+    HIPCHK(dev_upload(ix, &ix->d_rank_entry, ix->rank_entry.data(), ix->rank_entry.size()));
+    // tombstones: the set grows with the raw store
+    ix->id_del.resize((u->n_raw + 7) / 8, 0);
+    for (uint64_t i = 0; i < u->n_deleted; i++) {
+        const uint64_t id = u->deleted_ids[i];
+        if (id >= u->n_raw || ((ix->id_del[id >> 3] >> (id & 7)) & 1)) continue;
+        ix->id_del[id >> 3] |= (uint8_t)(1u << (id & 7));
+        ix->n_deleted++;
+    }
+    if (int rc = upload_tombstones(ix, ix->h_ids)) return rc;
+    if (u->graph) {
+        if (int rc = upload_graph(ix, u->graph, [&](int64_t b) -> int64_t {
+                return b < -1 ? -2 : post_of_block(b);
+            }))
+            return rc;
+    } else if (ix->has_graph) {
+        ix->h_gpost = gpost;
+        dev_free_as(ix, ix->d_gpost);
+        if (gpost.empty()) gpost.push_back(-1);
+        HIPCHK(dev_upload(ix, &ix->d_gpost, gpost.data(), gpost.size()));
+    }
+    if (int rc = derive_entries(ix)) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    ix->gen++;
+    return SPF_OK;
+}
 
 // Matrix-core (or RaBitQ estimator) scan -> 16-bit candidate keys -> per
 // query pool select -> bound-pruned exact rerank -> ids.  Runs after probe
@@ -2116,10 +2438,18 @@ int spf_index_get_info(const spf_index *ix, spf_index_info *o) {
     o->n_entries = ix->N;
     o->n_raw = ix->n_raw;
     o->n_deleted = ix->n_deleted;
-    o->device_bytes = ix->device_bytes;
+    o->device_bytes = dev_bytes(ix);
     o->max_posting = ix->max_len;
     o->graph_replays = ix->replays.load();
     return SPF_OK;
+}
+
+int spf_index_update(spf_index *ix, const spf_index_update_desc *u) {
+    try {
+        return update_impl(ix, u);
+    } catch (const std::bad_alloc &) {
+        return fail(SPF_ENOMEM, "host allocation failed");
+    }
 }
 
 int spf_delete(spf_index *ix, const uint64_t *ids, uint64_t n) {
@@ -2134,6 +2464,7 @@ int spf_delete(spf_index *ix, const uint64_t *ids, uint64_t n) {
         auto it = std::lower_bound(ix->rank_id.begin(), ix->rank_id.end(), id);
         if (it != ix->rank_id.end() && *it == id) {
             const uint32_t e = ix->rank_entry[it - ix->rank_id.begin()];
+            if (e == 0xffffffffu) continue;   // an id without an entry
             ix->h_del[e >> 5] |= 1u << (e & 31);
             changed = true;
         }

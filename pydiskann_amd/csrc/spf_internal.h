@@ -441,6 +441,25 @@ hipError_t launch_rq_decode(const RqDecodeParams &p, uint32_t n, hipStream_t s);
 hipError_t launch_rq_scan(const RqParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_rq_rows(const RqRowsParams &p, hipStream_t s);
 
+// creation / incremental update (spf_update.hip); src maps: bit 31 set = the
+// upload, else the old array, low bits = source entry
+hipError_t launch_gather_tiled(const uint4 *old_t, const uint4 *up_t, const uint32_t *src,
+                               uint32_t n, uint32_t units, uint4 *dst, hipStream_t s);
+hipError_t launch_gather_rows(const float *old_r, const float *up_r, const uint32_t *src,
+                              uint32_t n, uint32_t dpad, uint32_t up_stride, uint32_t dim,
+                              float *dst, hipStream_t s);
+hipError_t launch_tiled_f16(const uint4 *f32t, uint32_t units32, uint32_t n, uint4 *f16t,
+                            uint32_t units16, hipStream_t s);
+hipError_t launch_post_bounds(const float *raw_e, const uint4 *codes16, uint32_t units16,
+                              uint32_t n, uint32_t dpad, uint32_t dim, const uint32_t *post_off,
+                              uint32_t n_post, uint32_t *post_ex, uint32_t *post_xn,
+                              hipStream_t s);
+hipError_t launch_entry_post(const uint32_t *post_off, uint32_t n_post, uint32_t n, uint32_t *out,
+                             hipStream_t s);
+hipError_t launch_id_entry(const uint32_t *entry_id, uint32_t n, uint32_t *id_entry, uint64_t span,
+                           hipStream_t s);
+hipError_t launch_iota_u64(uint64_t *p, uint64_t n, hipStream_t s);
+
 // Working memory of the no-allocation search: a file-scope device array of
 // this size (spf_noalloc.hip), one per device.
 constexpr size_t kNoAllocScratchBytes = 64ull << 20;

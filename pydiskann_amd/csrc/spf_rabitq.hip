@@ -419,7 +419,7 @@ __global__ void k_rq_rows(const RqRowsParams p) {
     }
     p.xn[e] = (float)n2;
     const float ex = (float)(sqrt(e2) * (1.0 + 1e-12));
-    atomicMax(p.post_ex + lo, __float_as_uint(ex) + (ex > 0.0f ? 1u : 0u));
+    atomicMax(p.post_ex + lo, __float_as_uint(ex) + 1u);   // the next float up
 }
 
 // ---------------------------------------------------------------- launchers

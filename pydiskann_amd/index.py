@@ -266,6 +266,30 @@ def build_index(vectors, metric: str = "l2", quantizer=None, n_postings: int | N
                        probe_ratio=probe_ratio, rerank_size=rerank_size)
 
 
+def changed_blocks(old: IndexArrays, new: IndexArrays) -> list:
+    """Reference block indices whose posting differs between two states of an
+    index: blocks present in only one of them (created or freed) and blocks
+    whose centroid or (id, code) entries changed -- the set Inner::mark_dirty
+    records for a device mirror (src/spfresh.rs:283-289)."""
+    def table(ix):
+        t = {}
+        for i, b in enumerate(ix.block_ids.tolist()):
+            s, e = int(ix.post_off[i]), int(ix.post_off[i + 1])
+            t[b] = (ix.centroids[i], ix.ids[s:e], ix.codes[s:e])
+        return t
+    a, b = table(old), table(new)
+    out = []
+    for blk in sorted(set(a) | set(b)):
+        if blk not in a or blk not in b:
+            out.append(blk)
+            continue
+        (ca, ia, xa), (cb, ib, xb) = a[blk], b[blk]
+        if not (np.array_equal(ca.view(np.uint32), cb.view(np.uint32)) and np.array_equal(ia, ib)
+                and np.array_equal(xa, xb)):
+            out.append(blk)
+    return out
+
+
 def brute_force_f64(raw: np.ndarray, queries: np.ndarray, k: int, metric: int,
                     live: np.ndarray | None = None, chunk: int = 256):
     """Exact k-NN in float64 (ids, dists); ties by id.  Used for recall."""
