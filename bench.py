@@ -350,8 +350,8 @@ def roofline_of(args, head, dim, nq, k, ix):
     stage, prof = head["stage"], head["prof"]
     kern = STAGE_KERNELS[args.precision]
     if ix.quant == 3:
-        kern = {"scan": "k_rq_scan" if args.precision == "mixed" else "k_rq_search_exact",
-                "select": "k_pool_select", "rerank": "k_rerank_pruned"}
+        kern = {"scan": "k_rq_scan_mfma" if args.precision == "mixed" else "k_rq_search_exact",
+                "select": "k_pool_select", "rerank": "k_rerank_f16"}
     ms = stage["scan"]
     alg = prof["scan_bytes"]
     ach = alg / (ms * 1e-3) / 1e9
@@ -359,7 +359,9 @@ def roofline_of(args, head, dim, nq, k, ix):
     assert 0.0 < frac <= 1.0, f"roofline fraction {frac} outside (0, 1]"
     flops = prof["scan_flops"]
     ai = flops / alg if alg else 0.0
-    ridge = MFMA_F16_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+    # int8 matrix cores (5.0 POP/s dense) for int8 and 1-bit postings, else f16 (2.5 PFLOP/s)
+    peak = 5000.0 if (ix.quant in (2, 3) and args.precision == "mixed") else MFMA_F16_TFLOPS
+    ridge = peak * 1e12 / (HBM_PEAK_GBS * 1e9)
     wl = f"{args.workload}_{args.precision}_{ix.n_raw // 1000}kx{dim}"
     traffic, traffic_src = read_traffic(kern["scan"], wl)
     out = {"bound": "hbm" if ai < ridge else "mfma", "achieved": round(ach, 1),
@@ -370,11 +372,13 @@ def roofline_of(args, head, dim, nq, k, ix):
            "ridge_flop_per_byte": round(ridge, 1),
            "candidates_per_launch": prof["candidates"], "traffic_source": traffic_src}
     if flops:
-        # int8 matrix cores (5.0 POP/s dense) for int8 postings, else f16 (2.5 PFLOP/s)
-        peak = 5000.0 if (ix.quant == 2 and args.precision == "mixed") else MFMA_F16_TFLOPS
         tf = flops / (ms * 1e-3) / 1e12
         out["scan_mfma"] = {"tflops": round(tf, 1), "peak_tflops": peak,
                             "frac": round(tf / peak, 4), "flops": int(flops)}
+        if out["bound"] == "mfma":   # compute-bound: the roofline is the matrix-core one
+            assert 0.0 < tf / peak <= 1.0, f"matrix-core fraction {tf / peak} outside (0, 1]"
+            out.update(achieved=round(tf, 1), peak=peak, unit="TFLOP/s", frac=round(tf / peak, 4),
+                       hbm_GBs=round(ach, 1), hbm_frac=round(frac, 4))
     if prof["probe_flops"] and stage["probe"] > 0:
         tf = prof["probe_flops"] / (stage["probe"] * 1e-3) / 1e12
         out["probe_mfma" if args.precision == "mixed" else "probe_valu"] = {
@@ -397,7 +401,7 @@ def dtype_of(args, ix) -> str:
     if ix.quant == 2:
         return "int8 scan (int32 sums), bf16 probe, f32 rerank"
     if ix.quant == 3:
-        return "1-bit codes (popcount estimator), bf16 probe, f16 then f32 rerank"
+        return "1-bit codes (int8 matrix-core estimator), bf16 probe, f16 then f32 rerank"
     return "f16/bf16 scan+probe, f32 rerank"
 
 

@@ -116,6 +116,9 @@ typedef struct spf_index_desc {
                                    default when the index-derived bound exceeds
                                    plan_bound_mib) */
 #define SPF_FLAG_NO_GRAPH 8u    /* do not capture / replay the call's launch chain */
+#define SPF_FLAG_RQ_POPCOUNT 16u /* MIXED over RaBitQ codes: the estimator's integer sum by
+                                   popcounts over bit planes on the vector units instead of
+                                   int8 matrix-core products (the same integers, same result) */
 
 /* Planning.  A batch's candidate count is known only after its probe.  By
  * default the library sizes the candidate buffers from a bound derived from

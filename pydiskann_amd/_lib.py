@@ -32,6 +32,7 @@ FLAG_PROFILE = 1
 FLAG_FULL_RERANK = 2
 FLAG_SYNC_PLAN = 4
 FLAG_NO_GRAPH = 8
+FLAG_RQ_POPCOUNT = 16
 STAGES = ("prep", "probe", "probe_select", "plan", "scan", "select", "rerank", "finalize")
 
 

@@ -125,7 +125,7 @@ struct DBuf {
     X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)        \
     X(uint32_t, gflag) X(uint32_t, gscr) X(uint32_t, sel_ccnt) X(uint32_t, sel_coff)            \
     X(uint64_t, sel_tmp) X(uint64_t, probe_tmp) X(uint32_t, cand_real) X(uint32_t, blk_cnt) \
-    X(uint32_t, a_min) X(uint8_t, q8m) X(float, q8a) X(float, q8b)
+    X(uint32_t, a_min) X(uint8_t, q8m) X(float, q8a) X(float, q8b) X(uint8_t, rq_levels)
 
 // A captured launch chain of one search call, replayed for later calls with
 // the same key (the same arguments, buffers and launch shapes).
@@ -927,6 +927,7 @@ struct Shape {
     bool mixed, rerank, mfma_scan, mprobe, round16, q8;
     bool int8;       // matrix-core scan of int8 codes (int8 MFMA, 64-dim K steps)
     bool rq_est;     // RaBitQ estimator prep: search_rabitq (L2) or the mixed estimator scan
+    bool rq_mfma;    // mixed RaBitQ: the estimator's integer sums on int8 matrix cores
     uint32_t dpad64, mks;   // dim rounded up to 64; K steps of the matrix-core scan
     bool graph;   // probe through the routing graph (above kBruteCentroids live postings)
     uint32_t nraw;   // the probe count as asked (the graph's result count and beam)
@@ -1223,7 +1224,8 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     if (rq) {
         rq->out16 = w->cand16.p;
         HIPCHK(launch_rq_prep(*rq, nq * npr, s));
-        HIPCHK(launch_rq_scan(*rq, grid, s));
+        if (rq->levels) HIPCHK(launch_rq_scan_mfma(*rq, grid, s));
+        else HIPCHK(launch_rq_scan(*rq, grid, s));
     } else {
         HIPCHK(launch_scan_mfma(mp, grid, s));
     }
@@ -1324,6 +1326,7 @@ int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_se
                     "with dim <= 512 and RaBitQ postings");
     // RaBitQ estimator prep: search_rabitq (L2, exact) or the mixed estimator scan
     z.rq_est = rq && (ix->rq_l2 || z.mixed);
+    z.rq_mfma = rq && z.mixed && !(opts && (opts->flags & SPF_FLAG_RQ_POPCOUNT));
     z.int8 = z.mixed && i8;
     z.dpad64 = round_up(ix->dim, 64);
     // RaBitQ + L2 takes search_rabitq (src/spfresh.rs:697-703): no rerank_size
@@ -1347,9 +1350,11 @@ int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_se
     // kMixQ queries over mfma_chunk entries, the RaBitQ estimator scan 8
     // queries, the VALU scan Gv; both of the latter over kEChunk entries
     z.mfma_scan = z.mixed && !rq;
-    z.G = z.mfma_scan ? ix->mix_g : z.mixed ? 8u : z.Gv;
+    z.G = z.mfma_scan ? ix->mix_g : z.rq_mfma ? kRqMfmaQ : z.mixed ? 8u : z.Gv;
     z.mks = z.int8 ? z.dpad64 / 64 : z.dpad32 / 32;
-    z.echunk = z.mfma_scan ? mfma_chunk(z.mks) : (z.mixed ? kEChunk : ix->scan_chunk);
+    z.echunk = z.mfma_scan ? mfma_chunk(z.mks)
+               : z.rq_mfma ? 64 * rq_mfma_tiles(ix->rq_padded / 64)
+               : (z.mixed ? kEChunk : ix->scan_chunk);
     z.round16 = ix->quant == SPF_QUANT_F16 &&
                 (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
     z.q8 = ix->quant == SPF_QUANT_INT8 &&
@@ -1500,6 +1505,7 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
         add(pairs * 4 * (ix->rq_padded / 64), 8);
         add(pairs * 8, 4);
     }
+    if (z.rq_mfma) add(pairs * ix->rq_padded, 1);   // rq_levels
     const uint64_t total = cand_slots_bound(ix, z, q);
     if (z.mixed) {
         if (candidates) {
@@ -1912,6 +1918,10 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         const uint32_t words = ix->rq_padded / 64;
         HIPCHK(w->rq_planes.ensure((size_t)nq * npr * 4 * words));
         HIPCHK(w->rq_consts.ensure((size_t)nq * npr * 8));
+        if (z.rq_mfma) {
+            HIPCHK(w->rq_levels.ensure((size_t)nq * npr * ix->rq_padded));
+            rq.levels = w->rq_levels.p;
+        }
         rq.qv = w->qv_probe.p;
         rq.dpad = dpad;
         rq.dim = ix->dim;
@@ -2075,7 +2085,8 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                 key = 2;
             } else if (z.rq_est) {
                 code = ix->rq_bit_units * 16.0 + tomb + lab;     // bit words, |r|^2, factor
-                qrow = 4.0 * (ix->rq_padded / 64) * 8 + 32;      // 4-bit planes, constants
+                qrow = z.rq_mfma ? ix->rq_padded + 32.0          // 4-bit levels, constants
+                                 : 4.0 * (ix->rq_padded / 64) * 8 + 32;   // 4-bit planes, constants
                 key = mixed ? 2 : 0;
             } else {
                 code = ix->units * 16.0 + 4 + tomb + lab;        // code, rank
@@ -2098,8 +2109,9 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                 b += (double)total * key;
             }
             g_prof.scan_bytes = (uint64_t)b;
-            g_prof.scan_flops = ix->quant == SPF_QUANT_RABITQ ? 0
-                                : 2ull * (z.int8 ? z.dpad64 : mixed ? dpad32 : dpad) * real;
+            g_prof.scan_flops = ix->quant == SPF_QUANT_RABITQ
+                                    ? (z.rq_mfma ? 2ull * ix->rq_padded * real : 0)
+                                    : 2ull * (z.int8 ? z.dpad64 : mixed ? dpad32 : dpad) * real;
             g_prof.probe_flops = z.graph ? 0 : 2ull * (mprobe ? dpad32 : dpad) * nq * P;
         }
         if (mixed) {   // per-query counts, summed here

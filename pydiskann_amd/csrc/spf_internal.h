@@ -276,7 +276,14 @@ struct RqParams {
     uint32_t mixed;
     const float *xn;            // [n_entries] |x|^2 of the raw rows
     const float *qnorm;         // [nq] |q|^2
+    // matrix-core estimator: the prep also writes each pair's 4-bit levels
+    // as int8 rows ([nq*npr][padded]); the scan takes sum(bit * level) from
+    // int8 MFMA (items of up to kRqMfmaQ pairs over 64 * rq_mfma_tiles entries)
+    uint8_t *levels;            // null = bit planes only (popcount scan)
 };
+
+constexpr uint32_t kRqMfmaQ = 64;   // pairs per matrix-core estimator item
+inline uint32_t rq_mfma_tiles(uint32_t words) { return words <= 16 ? 4u : 2u; }
 
 // Per entry of a RaBitQ index: the f16 copy of its raw row, |x|^2, and the
 // posting's max |x - f16(x)| (the f16 rerank stage's bound), rounded up.
@@ -440,6 +447,7 @@ hipError_t launch_rq_search_exact(const RqParams &p, hipStream_t s);
 hipError_t launch_rq_decode(const RqDecodeParams &p, uint32_t n, hipStream_t s);
 hipError_t launch_rq_scan(const RqParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_rq_rows(const RqRowsParams &p, hipStream_t s);
+hipError_t launch_rq_scan_mfma(const RqParams &p, uint32_t grid, hipStream_t s);
 
 // creation / incremental update (spf_update.hip); src maps: bit 31 set = the
 // upload, else the old array, low bits = source entry

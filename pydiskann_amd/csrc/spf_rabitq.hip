@@ -106,6 +106,7 @@ __global__ __launch_bounds__(64) void k_rq_prep(const RqParams p) {
         uint32_t u = t > 0.0f ? (t >= 4294967295.0f ? 0xffffffffu : (uint32_t)t) : 0u;
         u = u > 15u ? 15u : u;
         sum_u += u;
+        if (p.levels) p.levels[(size_t)pair * P + 64 * m + lane] = (uint8_t)u;
 #pragma unroll
         for (uint32_t b = 0; b < 4; b++) {
             const uint64_t bits = __ballot((u >> b) & 1u);
@@ -393,6 +394,137 @@ __global__ __launch_bounds__(256) void k_rq_scan(const RqParams p) {
     }
 }
 
+// ---------------------------------------------------------------- matrix-core estimator
+// Items of up to kRqMfmaQ (query, posting) pairs over 64 T entries of one
+// posting (T = rq_mfma_tiles(W) tiles of 16 entries per wave).  The
+// estimator's integer part S = sum_i bit_i * level_i, which the popcount scan
+// sums over four bit planes (src/rabitq.rs:46-58), is an int8 product here:
+// code bits unpacked to {0, 1} (B: each lane's 16-bit slice of a 64-dim K
+// step) against the pair's 4-bit levels (A), v_mfma_i32_16x16x64_i8 with
+// int32 sums.  S is the same integer, so estimates and keys equal the
+// popcount scan's bit for bit.
+typedef int i32x4r_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t nib_bytes(uint32_t n) {   // bit i of n (i < 4) -> byte i
+    return (n * 0x00204081u) & 0x01010101u;
+}
+
+template <uint32_t W>
+__global__ __launch_bounds__(256) void k_rq_scan_mfma(const RqParams p) {
+    constexpr uint32_t T = W <= 16 ? 4 : 2;   // rq_mfma_tiles
+    constexpr uint32_t EW = 64 * T;
+    constexpr uint32_t NS = (W + 1) / 2;      // dwords of a lane's 16-bit slices
+    __shared__ float s_cs[kRqMfmaQ][5];
+    __shared__ uint64_t s_slot[kRqMfmaQ];
+    __shared__ uint64_t s_mask[kRqMfmaQ];
+    __shared__ float s_qn[kRqMfmaQ];
+    __shared__ uint32_t s_src[kRqMfmaQ];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t col = lane & 15, kg = lane >> 4;
+    const uint32_t P = p.padded;
+    const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
+    const uint32_t n_items = p.n_items_dev ? *p.n_items_dev : p.n_items;
+    for (XcdWalk it(n_items); it.more(); it.next()) {
+        const uint32_t item = it.item;
+        const uint32_t pp = p.item_post[item];
+        const uint32_t local = item - p.item_off[pp];
+        const uint32_t pb = p.post_off[pp], pe = p.post_off[pp + 1];
+        const uint32_t nec = (pe - pb + EW - 1) / EW;
+        const uint32_t qc = local / nec, ec = local - qc * nec;
+        const uint32_t pr0 = p.pair_off[pp] + qc * kRqMfmaQ;
+        const uint32_t gcnt = min(kRqMfmaQ, p.pair_off[pp + 1] - pr0);
+        const uint32_t e_begin = pb + ec * EW, e_end = min(e_begin + EW, pe);
+        __syncthreads();   // the previous item is done with the pair table
+        if (tid < kRqMfmaQ) {
+            const bool ok = tid < gcnt;
+            const uint32_t src = ok ? p.pair_src[pr0 + tid] : 0u;
+            const uint32_t qi = src / p.npr;
+            s_src[tid] = src;
+            s_slot[tid] = ok ? p.pair_slot[pr0 + tid] : 0ull;
+            s_mask[tid] = (ok && p.qmask) ? p.qmask[qi] : 0ull;
+            s_qn[tid] = (ok && p.metric == M_DOT) ? p.qnorm[qi] : 0.0f;
+        }
+        for (uint32_t i = tid; i < kRqMfmaQ * 5; i += 256) {
+            const uint32_t g = i / 5, c = i - g * 5;
+            s_cs[g][c] = g < gcnt ? p.consts[(size_t)p.pair_src[pr0 + g] * 8 + c] : 0.0f;
+        }
+        // the wave's tiles: each lane keeps its 16-bit slice of every word of
+        // its column's entry, plus the entry's popcount, |r|^2, factor,
+        // liveness, label and |x|^2 (entries past the posting read e_begin)
+        uint32_t sl[T][NS], cnt[T], lab[T];
+        float n2[T], fac[T], xn[T];
+        bool live[T];
+#pragma unroll
+        for (uint32_t t = 0; t < T; t++) {
+            const uint32_t e_raw = e_begin + 16 * (wave + 4 * t) + col;
+            const uint32_t e = e_raw < e_end ? e_raw : e_begin;
+            uint64_t words[W];
+            rq_load_code<W>(p.codes, p.units, e, words, n2[t], fac[t]);
+            uint32_t c = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < NS; i++) sl[t][i] = 0u;
+#pragma unroll
+            for (uint32_t s = 0; s < W; s++) {
+                c += __popcll(words[s]);
+                sl[t][s / 2] |= ((uint32_t)(words[s] >> (16 * kg)) & 0xffffu) << (16 * (s & 1));
+            }
+            cnt[t] = c;
+            const bool dead = p.del && ((p.del[e >> 5] >> (e & 31)) & 1u);
+            live[t] = e_raw < e_end && !dead;
+            lab[t] = p.qmask ? p.label_e[e] : 0u;
+            xn[t] = l2 ? 0.0f : p.xn[e];
+        }
+        __syncthreads();
+        for (uint32_t gb = 0; gb < gcnt; gb += 16) {   // 16 pairs (A rows) at a time
+            // unpack the bits again for every group rather than keeping all
+            // tiles' unpacked fragments live (four times the registers)
+#pragma unroll
+            for (uint32_t t = 0; t < T; t++)
+#pragma unroll
+                for (uint32_t i = 0; i < NS; i++) asm volatile("" : "+v"(sl[t][i]));
+            const uint32_t g = gb + col;
+            const uint4 *arow = reinterpret_cast<const uint4 *>(p.levels + (size_t)s_src[g < gcnt ? g : gb] * P);
+            i32x4r_t acc[T];
+#pragma unroll
+            for (uint32_t t = 0; t < T; t++) acc[t] = (i32x4r_t){0, 0, 0, 0};
+#pragma unroll
+            for (uint32_t s = 0; s < W; s++) {
+                union { uint4 u; i32x4r_t i; } a;
+                a.u = arow[4 * s + kg];   // levels 64 s + 16 kg .. + 15
+#pragma unroll
+                for (uint32_t t = 0; t < T; t++) {
+                    const uint32_t v = (sl[t][s / 2] >> (16 * (s & 1))) & 0xffffu;
+                    const i32x4r_t b = {(int)nib_bytes(v & 0xfu), (int)nib_bytes((v >> 4) & 0xfu),
+                                        (int)nib_bytes((v >> 8) & 0xfu), (int)nib_bytes(v >> 12)};
+                    acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a.i, b, acc[t], 0, 0, 0);
+                }
+            }
+            // D[4 kg + j][col]: pair gb + 4 kg + j against entry col of the tile
+#pragma unroll
+            for (uint32_t t = 0; t < T; t++) {
+                const uint32_t e = e_begin + 16 * (wave + 4 * t) + col;
+                const float rx = xn[t] > 0.0f ? 0.5f / sqrtf(xn[t]) : 0.0f;
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++) {
+                    const uint32_t g2 = gb + 4 * kg + j;
+                    if (e >= e_end || g2 >= gcnt) continue;
+                    const float *cs = s_cs[g2];
+                    // distance_with_bound's estimate (src/rabitq.rs:202-215), as in rq_est
+                    float inner = cs[2] * (float)acc[t][j] + cs[3] * (float)cnt[t];
+                    inner = inner + cs[4];
+                    const float sf = cs[1] * fac[t];
+                    const float est = fmaxf((n2[t] + cs[0]) - sf * inner, 0.0f);
+                    float d = est;
+                    if (p.metric == M_COS) d = 1.0f - (1.0f + xn[t] - est) * rx;
+                    else if (p.metric == M_DOT) d = 1.0f - 0.5f * (s_qn[g2] + xn[t] - est);
+                    const bool ok = live[t] && (!p.qmask || label_ok(s_mask[g2], lab[t]));
+                    p.out16[s_slot[g2] + (e - pb)] = ok ? (uint16_t)(fkey_r(d) >> 16) : (uint16_t)0xffff;
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- rows
 // One thread per entry: f16 copy of the raw row, |x|^2, and the posting's
 // max |x - f16(x)| in double, rounded up (non-negative floats order as their
@@ -453,6 +585,14 @@ hipError_t launch_rq_decode(const RqDecodeParams &p, uint32_t n, hipStream_t s) 
     if (n == 0) return hipSuccess;
 #define L(W) hipLaunchKernelGGL(k_rq_decode<W>, dim3(n), dim3(64), p.padded * sizeof(float), s, p)
     SPF_RQ_DISPATCH(p.padded / 64, L)
+#undef L
+    return hipGetLastError();
+}
+
+hipError_t launch_rq_scan_mfma(const RqParams &p, uint32_t grid, hipStream_t s) {
+    if (grid == 0) return hipSuccess;
+#define L(W) hipLaunchKernelGGL(k_rq_scan_mfma<W>, dim3(grid), dim3(256), 0, s, p)
+    SPF_RQ_DISPATCH(p.words, L)
 #undef L
     return hipGetLastError();
 }
