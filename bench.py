@@ -494,6 +494,8 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=0,
                     help="c3/c5: probe through a routing graph of this many neighbours per "
                          "centroid (the reference's path above 4,096 postings)")
+    ap.add_argument("--admit", action="store_true",
+                    help="mixed: two-phase scan with admission (SPF_FLAG_ADMIT)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes (by default each precision is first "
                          "autotuned on the batch itself, untimed)")
@@ -515,7 +517,7 @@ def main() -> int:
 
     import torch
     from pydiskann_amd import DeviceIndex, PREC_EXACT, PREC_MIXED
-    from pydiskann_amd._lib import last_profile
+    from pydiskann_amd._lib import last_profile, FLAG_ADMIT
     from pydiskann_amd.index import brute_force_f64
 
     base, queries, ix, n_probe, k, np = build_workload(args, torch)
@@ -544,7 +546,7 @@ def main() -> int:
             dev.search_device(q_d.data_ptr(), nq, dim, k, n_probe, ids_d.data_ptr(),
                               dd_d.data_ptr(), cnt_d.data_ptr(), stream=stream.cuda_stream,
                               precision=prec, rerank_size=args.rerank if prec else 0,
-                              profile=profile)
+                              profile=profile, flags=FLAG_ADMIT if args.admit else 0)
 
         for _ in range(args.warmup):
             step()
@@ -578,7 +580,8 @@ def main() -> int:
         lat_sorted = sorted(lat)
         if precision == "mixed":
             log(f"[bench] mixed: rerank gathered {profs[-1]['rerank_rows']:,} rows of "
-                f"{profs[-1]['pool_members']:,} pool members")
+                f"{profs[-1]['pool_members']:,} pool members; second scan phase listed "
+                f"{profs[-1]['admitted']:,} of {profs[-1]['candidates']:,} candidates")
         return {"qps": qps, "total": total, "recall": rec, "stage": stage, "prof": profs[-1],
                 "tune": tune,
                 "p50": lat_sorted[len(lat) // 2],

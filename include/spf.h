@@ -119,6 +119,12 @@ typedef struct spf_index_desc {
 #define SPF_FLAG_RQ_POPCOUNT 16u /* MIXED over RaBitQ codes: the estimator's integer sum by
                                    popcounts over bit planes on the vector units instead of
                                    int8 matrix-core products (the same integers, same result) */
+#define SPF_FLAG_ADMIT 32u      /* MIXED over f32 / f16 / int8 codes: scan each query's
+                                   nearest probes first (until they hold rerank_size entries),
+                                   bound its pool by their keys, then store only the keys
+                                   within that bound instead of every candidate's key (the
+                                   same pools, same result; pays off when selecting pools
+                                   from long candidate lists dominates the scan) */
 
 /* Planning.  A batch's candidate count is known only after its probe.  By
  * default the library sizes the candidate buffers from a bound derived from
@@ -155,11 +161,12 @@ typedef struct spf_profile {
     uint32_t scan_items;            /* posting-scan work items */
     uint32_t rerank_rows;           /* MIXED: raw rows gathered by the exact rerank */
     uint32_t pool_members;          /* MIXED: candidates in the rerank pools */
-    uint32_t reserved[1];
+    uint32_t admitted;              /* MIXED with admission: keys the second phase listed */
     /* the posting scan's algorithmic device-memory bytes under its work items:
      * every item reads its entries' codes (plus per-entry norms, tombstone
      * bits, labels) and its queries' rows once, and writes one key per
-     * candidate slot */
+     * candidate slot (with admission: one per first-phase slot and one
+     * list entry per admitted key) */
     uint64_t scan_bytes;
     uint64_t scan_flops;            /* 2 * (padded dim) * candidates of the posting scan */
     uint64_t probe_flops;           /* 2 * (padded dim) * queries * centroids of a dense probe */

@@ -33,6 +33,7 @@ FLAG_FULL_RERANK = 2
 FLAG_SYNC_PLAN = 4
 FLAG_NO_GRAPH = 8
 FLAG_RQ_POPCOUNT = 16
+FLAG_ADMIT = 32
 STAGES = ("prep", "probe", "probe_select", "plan", "scan", "select", "rerank", "finalize")
 
 
@@ -81,7 +82,7 @@ class Profile(C.Structure):
     _fields_ = [("stage_ms", C.c_float * 8), ("candidates", C.c_uint64),
                 ("scan_code_bytes", C.c_uint64), ("probe_pairs", C.c_uint64),
                 ("scan_items", C.c_uint32), ("rerank_rows", C.c_uint32),
-                ("pool_members", C.c_uint32), ("reserved", C.c_uint32 * 1),
+                ("pool_members", C.c_uint32), ("admitted", C.c_uint32),
                 ("scan_bytes", C.c_uint64), ("scan_flops", C.c_uint64),
                 ("probe_flops", C.c_uint64)]
 
@@ -203,6 +204,7 @@ def last_profile() -> dict:
             "candidates": int(pr.candidates), "scan_code_bytes": int(pr.scan_code_bytes),
             "probe_pairs": int(pr.probe_pairs), "scan_items": int(pr.scan_items),
             "rerank_rows": int(pr.rerank_rows), "pool_members": int(pr.pool_members),
+            "admitted": int(pr.admitted),
             "scan_bytes": int(pr.scan_bytes), "scan_flops": int(pr.scan_flops),
             "probe_flops": int(pr.probe_flops)}
 

@@ -125,7 +125,9 @@ struct DBuf {
     X(uint32_t, rr_stats) X(uint32_t, sel_bits) X(uint32_t, a_bits) X(uint64_t, qmask)        \
     X(uint32_t, gflag) X(uint32_t, gscr) X(uint32_t, sel_ccnt) X(uint32_t, sel_coff)            \
     X(uint64_t, sel_tmp) X(uint64_t, probe_tmp) X(uint32_t, cand_real) X(uint32_t, blk_cnt) \
-    X(uint32_t, a_min) X(uint8_t, q8m) X(float, q8a) X(float, q8b) X(uint8_t, rq_levels)
+    X(uint32_t, a_min) X(uint8_t, q8m) X(float, q8a) X(float, q8b) X(uint8_t, rq_levels)   \
+    X(uint32_t, m1) X(uint32_t, s1) X(uint32_t, tau) X(uint32_t, adm_cnt) X(uint32_t, adm_list) \
+    X(uint32_t, pool_list) X(uint32_t, pool_na)
 
 // A captured launch chain of one search call, replayed for later calls with
 // the same key (the same arguments, buffers and launch shapes).
@@ -931,6 +933,10 @@ struct Shape {
     uint32_t dpad64, mks;   // dim rounded up to 64; K steps of the matrix-core scan
     bool graph;   // probe through the routing graph (above kBruteCentroids live postings)
     uint32_t nraw;   // the probe count as asked (the graph's result count and beam)
+    // matrix-core scan in two phases with admission: pair tables over 2 P
+    // virtual postings, compact lists of key << 16 | slot (segments < 2^16 slots)
+    bool admit;
+    uint32_t Pv;     // postings of the pair and item tables (P, or 2 P with admit)
 };
 
 // Incremental update of the mirror after mutations of the reference index
@@ -1177,9 +1183,18 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
                  bool stats, const uint64_t *d_masks, const uint32_t *n_items_dev,
                  const Shape &z) {
     const uint32_t dpad32 = round_up(ix->dim, 32);
+    const bool admit = z.admit;
     HIPCHK(w->cand16.ensure(std::max<uint64_t>(total, 1)));
-    HIPCHK(w->sel_bits.ensure(std::max<uint64_t>(total / 32, 1)));
-    HIPCHK(w->a_bits.ensure(std::max<uint64_t>(total / 32, 1)));
+    if (admit) {
+        HIPCHK(w->adm_list.ensure(std::max<uint64_t>(total, 1)));
+        HIPCHK(w->tau.ensure(nq));
+        HIPCHK(w->adm_cnt.ensure(nq));
+        HIPCHK(w->pool_list.ensure((size_t)nq * want));
+        HIPCHK(w->pool_na.ensure(nq));
+    } else {
+        HIPCHK(w->sel_bits.ensure(std::max<uint64_t>(total / 32, 1)));
+        HIPCHK(w->a_bits.ensure(std::max<uint64_t>(total / 32, 1)));
+    }
     HIPCHK(w->pool_want.ensure(nq));
     HIPCHK(w->fin.ensure((size_t)nq * k));
     HIPCHK(w->fin_cnt.ensure(nq));
@@ -1226,11 +1241,57 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
         HIPCHK(launch_rq_prep(*rq, nq * npr, s));
         if (rq->levels) HIPCHK(launch_rq_scan_mfma(*rq, grid, s));
         else HIPCHK(launch_rq_scan(*rq, grid, s));
+    } else if (admit) {
+        // first phase (each query's nearest probes): dense keys; the bound of
+        // every query from them; second phase: only keys within the bound
+        mp.n_items_dev = w->item_off.p + ix->P;
+        HIPCHK(launch_scan_mfma(mp, grid, s));
+        AdmitParams ap{};
+        ap.keys = w->cand16.p;
+        ap.seg_off = w->cand_off.p;
+        ap.s1 = w->s1.p;
+        ap.nq = nq;
+        ap.want = want;
+        ap.tau = w->tau.p;
+        HIPCHK(launch_admit(ap, s));
+        mp.item_lo_dev = w->item_off.p + ix->P;
+        mp.n_items_dev = n_items_dev;
+        mp.tau = w->tau.p;
+        mp.seg_off = w->cand_off.p;
+        mp.adm_list = w->adm_list.p;
+        HIPCHK(launch_scan_mfma(mp, grid, s));
     } else {
         HIPCHK(launch_scan_mfma(mp, grid, s));
     }
     // per query: the `want` smallest keys (pool) with their keys
     HIPCHK(mark(SPF_STAGE_SELECT));
+    // phase-A size of the rerank: k plus a margin for the 16-bit key resolution
+    const uint32_t ca = std::min(want, std::max(k, round_up(k + k / 8 + 8, 32)));
+    HIPCHK(w->a_min.ensure(nq));
+    if (admit) {
+        PoolListParams pq{};
+        pq.keys = w->cand16.p;
+        pq.seg_off = w->cand_off.p;
+        pq.s1 = w->s1.p;
+        pq.m1 = w->m1.p;
+        pq.np_q = w->np_q.p;
+        pq.qp_slot = w->qp_slot.p;
+        pq.cand_cnt = w->cand_cnt.p;
+        pq.tau = w->tau.p;
+        pq.adm_list = w->adm_list.p;
+        pq.nq = nq;
+        pq.npr = npr;
+        pq.echunk = z.echunk;
+        pq.want = want;
+        pq.ca = ca;
+        pq.k = k;
+        pq.pool = w->pool_list.p;
+        pq.pool_want = w->pool_want.p;
+        pq.pool_na = w->pool_na.p;
+        pq.a_min = w->a_min.p;
+        pq.adm_cnt = w->adm_cnt.p;
+        HIPCHK(launch_pool_list(pq, s));
+    } else {
     PoolParams pp{};
     pp.keys = w->cand16.p;
     pp.seg_off = w->cand_off.p;
@@ -1245,19 +1306,20 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     pp.pool_want = w->pool_want.p;
     pp.sel_bits = w->sel_bits.p;
     pp.a_bits = w->a_bits.p;
-    HIPCHK(w->a_min.ensure(nq));
     pp.a_min = w->a_min.p;
-    // phase-A size of the rerank: k plus a margin for the 16-bit key resolution
-    pp.ca = std::min(want, std::max(k, round_up(k + k / 8 + 8, 32)));
+    pp.ca = ca;
     HIPCHK(launch_pool_select(pp, nq, s));
+    }
     // the (pruned) exact rerank of each pool
     HIPCHK(mark(SPF_STAGE_RERANK));
     RerankPrunedParams rp{};
     rp.keys = w->cand16.p;
     rp.seg_off = w->cand_off.p;
     rp.seg_cnt = w->cand_cnt.p;
-    rp.sel_bits = w->sel_bits.p;
-    rp.a_bits = w->a_bits.p;
+    rp.sel_bits = admit ? nullptr : w->sel_bits.p;
+    rp.a_bits = admit ? nullptr : w->a_bits.p;
+    rp.pool = admit ? w->pool_list.p : nullptr;
+    rp.pool_na = admit ? w->pool_na.p : nullptr;
     rp.a_min = w->a_min.p;
     rp.pool_want = w->pool_want.p;
     rp.qp_slot = w->qp_slot.p;
@@ -1265,7 +1327,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     rp.nq = nq;
     rp.want = want;
     rp.k = k;
-    rp.ca = pp.ca;
+    rp.ca = ca;
     rp.metric = ix->metric;
     rp.dpad = ix->dpad;
     rp.dim = ix->dim;
@@ -1364,6 +1426,11 @@ int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_se
     z.mprobe = !z.graph && z.mixed && P > z.npr && ix->dim <= 1024 && z.npr + z.npr / 4 + 32 <= kMaxK;
     z.Pp = round_up(P, 32);
     z.band = std::min<uint32_t>(P, std::min<uint32_t>(kMaxK, std::max(z.npr + 32, z.npr + z.npr / 4)));
+    // admission: every query's segment bound (its npr longest postings, each
+    // aligned, rounded to 32) must fit the lists' 16-bit slots
+    z.admit = z.mfma_scan && opts && (opts->flags & SPF_FLAG_ADMIT) && P > 0 &&
+              (uint64_t)ix->top_len[std::min(z.npr, P)] + (uint64_t)(kScanAlign - 1) * z.npr + 31 <= 0xffffu;
+    z.Pv = z.admit ? 2 * P : P;
     return SPF_OK;
 }
 
@@ -1395,13 +1462,17 @@ uint64_t items_bound(const spf_index *ix, const Shape &z, uint64_t nq) {
     uint64_t call = ix->chunks_m;
     for (int c = 0; c < 4; c++)
         if (z.echunk == (256u << c)) call = ix->chunks_v[c];
-    return (pairs + z.G - 1) / z.G * cmax + std::min(call, pairs * cmax);
+    // (two phases: a posting's pairs split over two virtual postings, each
+    // rounding up once)
+    return (pairs + z.G - 1) / z.G * cmax + std::min((z.admit ? 2 : 1) * call, pairs * cmax);
 }
 
 // Device bytes of the candidate-sized buffers at the given counts (no growth slack).
 uint64_t cand_buffer_bytes(const spf_index *ix, const Shape &z, uint64_t slots, uint64_t items) {
     uint64_t b = std::max<uint64_t>(items, 1) * 4;   // item_post
-    if (z.mixed)
+    if (z.admit)   // dense keys, admission lists
+        b += std::max<uint64_t>(slots, 1) * (2 + 4);
+    else if (z.mixed)
         b += std::max<uint64_t>(slots, 1) * 2 + 2 * std::max<uint64_t>(slots / 32, 1) * 4;
     else if (!ix->rq_l2)
         b += std::max<uint64_t>(slots, 1) * 8 + sel_chunks(slots) * z.want * 8;
@@ -1487,18 +1558,23 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     add(q + 1, 8);               // cand_len
     add(q, 4);                   // cand_cnt
     add(q + 1, 8);               // cand_off
-    add(P + 1, 4);               // post_cnt
-    if (P <= kRankMaxPost) add((q + kRankQB - 1) / kRankQB * P, 4);   // blk_cnt
+    const uint64_t Pv = z.Pv;   // pair / item tables over (virtual) postings
+    add(Pv + 1, 4);              // post_cnt
+    if (Pv <= kRankMaxPost) add((q + kRankQB - 1) / kRankQB * Pv, 4);   // blk_cnt
+    if (z.admit) {
+        add(q, 4 * 5);           // m1, s1, tau, adm_cnt, pool_na
+        add(q * z.want, 4);      // pool_list
+    }
     add(q, 4);                   // cand_real (profiled calls)
     add(pairs, 4);               // qp_rel
     add(pairs, 4);               // prank
-    add(P + 1, 4);               // pair_off
+    add(Pv + 1, 4);              // pair_off
     add(pairs, 8);               // qp_slot
     add(pairs, 4);               // pair_q
     add(pairs, 8);               // pair_slot
     if (z.rq_est) add(pairs, 4);   // pair_src
-    add(P + 1, 4);               // item_cnt
-    add(P + 1, 4);               // item_off
+    add(Pv + 1, 4);              // item_cnt
+    add(Pv + 1, 4);              // item_off
     const uint64_t items = items_bound(ix, z, q);
     if (candidates) add(std::max<uint64_t>(items, 1), 4);   // item_post
     if (z.rq_est) {
@@ -1510,8 +1586,12 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     if (z.mixed) {
         if (candidates) {
             add(std::max<uint64_t>(total, 1), 2);       // cand16
-            add(std::max<uint64_t>(total / 32, 1), 4);  // sel_bits
-            add(std::max<uint64_t>(total / 32, 1), 4);  // a_bits
+            if (z.admit) {
+                add(std::max<uint64_t>(total, 1), 4);   // adm_list
+            } else {
+                add(std::max<uint64_t>(total / 32, 1), 4);   // sel_bits
+                add(std::max<uint64_t>(total / 32, 1), 4);   // a_bits
+            }
         }
         add(q, 4);                                  // pool_want
         add(q, 4);                                  // a_min
@@ -1717,9 +1797,9 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         pp.dpad32 = dpad32;
     }
     // the planner's per-posting pair counts and candidate counter, zeroed by k_prep
-    HIPCHK(w->post_cnt.ensure(P + 1));
+    HIPCHK(w->post_cnt.ensure(z.Pv + 1));
     pp.zero = w->post_cnt.p;
-    pp.zero_n = P + 1;
+    pp.zero_n = z.Pv + 1;
     HIPCHK(launch_prep(pp, s));
     HIPCHK(mark(SPF_STAGE_PROBE));
     const float *qv_scan = pp.qv_scan;
@@ -1820,19 +1900,28 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     HIPCHK(w->cand_len.ensure(nq + 1));
     HIPCHK(w->cand_cnt.ensure(nq));
     HIPCHK(w->cand_off.ensure(nq + 1));
-    HIPCHK(w->post_cnt.ensure(P + 1));
+    const uint32_t Pv = z.Pv;   // postings of the pair and item tables
+    HIPCHK(w->post_cnt.ensure(Pv + 1));
     HIPCHK(w->qp_rel.ensure((size_t)nq * npr));
     HIPCHK(w->prank.ensure((size_t)nq * npr));
-    HIPCHK(w->pair_off.ensure(P + 1));
+    HIPCHK(w->pair_off.ensure(Pv + 1));
     HIPCHK(w->qp_slot.ensure((size_t)nq * npr));
     HIPCHK(w->pair_q.ensure((size_t)nq * npr));
     HIPCHK(w->pair_slot.ensure((size_t)nq * npr));
     if (z.rq_est) HIPCHK(w->pair_src.ensure((size_t)nq * npr));
-    HIPCHK(w->item_cnt.ensure(P + 1));
-    HIPCHK(w->item_off.ensure(P + 1));
+    HIPCHK(w->item_cnt.ensure(Pv + 1));
+    HIPCHK(w->item_off.ensure(Pv + 1));
     PlanParams pl{};
     pl.nq = nq;
-    pl.n_post = P;
+    pl.n_post = Pv;
+    if (z.admit) {   // two phases: each query's nearest probes up to `want` entries first
+        HIPCHK(w->m1.ensure(nq));
+        HIPCHK(w->s1.ensure(nq));
+        pl.split = P;
+        pl.want = want;
+        pl.m1 = w->m1.p;
+        pl.s1 = w->s1.p;
+    }
     pl.npr = npr;
     pl.probe_sel = w->probe_sel.p;
     pl.probe_cnt = w->probe_cnt.p;
@@ -1859,18 +1948,18 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         HIPCHK(w->cand_real.ensure(nq));
         pl.cand_real = w->cand_real.p;
     }
-    if (P <= kRankMaxPost) {   // pair ranks by query block (else device-wide atomics)
-        HIPCHK(w->blk_cnt.ensure((size_t)(nq + kRankQB - 1) / kRankQB * P));
+    if (Pv <= kRankMaxPost) {   // pair ranks by query block (else device-wide atomics)
+        HIPCHK(w->blk_cnt.ensure((size_t)(nq + kRankQB - 1) / kRankQB * Pv));
         pl.qb = kRankQB;
         pl.blk_cnt = w->blk_cnt.p;
     }
     HIPCHK(launch_plan_q(pl, s));
     HIPCHK(launch_pair_rank(pl, s));
     HIPCHK(launch_excl_scan_u64(w->cand_len.p, w->cand_off.p, nq, s));
-    HIPCHK(launch_excl_scan_u32(w->post_cnt.p, w->pair_off.p, P, s));
+    HIPCHK(launch_excl_scan_u32(w->post_cnt.p, w->pair_off.p, Pv, s));
     HIPCHK(launch_scatter_pairs(pl, s));
     HIPCHK(launch_items(pl, s));
-    HIPCHK(launch_excl_scan_u32(w->item_cnt.p, w->item_off.p, P, s));
+    HIPCHK(launch_excl_scan_u32(w->item_cnt.p, w->item_off.p, Pv, s));
     uint64_t total;
     uint32_t n_items;
     if (plan_on_device(ix, z, nq, opts)) {
@@ -1882,13 +1971,13 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         w->h_summary[1] = 0;
         HIPCHK(hipMemcpyAsync(&w->h_summary[0], w->cand_off.p + nq, sizeof(uint64_t),
                               hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(&w->h_summary[1], w->item_off.p + P, sizeof(uint32_t),
+        HIPCHK(hipMemcpyAsync(&w->h_summary[1], w->item_off.p + Pv, sizeof(uint32_t),
                               hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         total = w->h_summary[0];
         n_items = (uint32_t)w->h_summary[1];
     }
-    const uint32_t *n_items_dev = w->item_off.p + P;
+    const uint32_t *n_items_dev = w->item_off.p + Pv;
     if (reserve) {   // growth of the candidate-sized buffers, at DBuf's rule
         uint64_t extra = 0;
         auto grow = [&](const auto &b, uint64_t n, uint64_t size) {
@@ -1897,8 +1986,12 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
         grow(w->item_post, std::max<uint32_t>(n_items, 1), 4);
         if (mixed) {
             grow(w->cand16, std::max<uint64_t>(total, 1), 2);
-            grow(w->sel_bits, std::max<uint64_t>(total / 32, 1), 4);
-            grow(w->a_bits, std::max<uint64_t>(total / 32, 1), 4);
+            if (z.admit) {
+                grow(w->adm_list, std::max<uint64_t>(total, 1), 4);
+            } else {
+                grow(w->sel_bits, std::max<uint64_t>(total / 32, 1), 4);
+                grow(w->a_bits, std::max<uint64_t>(total / 32, 1), 4);
+            }
         } else if (!ix->rq_l2) {
             grow(w->cand, std::max<uint64_t>(total, 1), 8);
             if (const uint64_t mc = sel_chunks(total)) grow(w->sel_tmp, mc * want, 8);
@@ -2067,12 +2160,14 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                     : mixed ? 2ull * ix->dim : (uint64_t)ix->code_bytes);
         g_prof.probe_pairs = (uint64_t)nq * P;
         g_prof.scan_items = n_items;
+        uint64_t admitted = 0;
         {
             // the scan's algorithmic bytes: per posting, every work item reads
             // its chunk of entries (codes + per-entry words) and its queries'
             // rows once; one key per candidate slot is written
-            std::vector<uint32_t> poff(P + 1);
-            HIPCHK(hipMemcpy(poff.data(), w->pair_off.p, (P + 1) * 4, hipMemcpyDeviceToHost));
+            const uint32_t Pv = z.Pv;
+            std::vector<uint32_t> poff(Pv + 1);
+            HIPCHK(hipMemcpy(poff.data(), w->pair_off.p, (Pv + 1) * 4, hipMemcpyDeviceToHost));
             const double tomb = 1.0 / 8, lab = d_masks ? 1.0 : 0.0;
             double code = 0, qrow = 0, key = 0;
             if (mixed && z.int8) {
@@ -2100,15 +2195,29 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                 // search_rabitq: each query's wave reads every probed entry's code
                 b = (double)real * code;
             } else {
-                for (uint32_t p = 0; p < P; p++) {
-                    const uint64_t np = poff[p + 1] - poff[p], len = ix->h_post_len[p];
+                for (uint32_t p = 0; p < Pv; p++) {
+                    const uint64_t np = poff[p + 1] - poff[p], len = ix->h_post_len[p % P];
                     if (!np || !len) continue;
                     const uint64_t groups = (np + G - 1) / G, chunks = (len + echunk - 1) / echunk;
                     b += (double)groups * len * code + (double)chunks * np * qrow;
                 }
-                b += (double)total * key;
+                if (z.admit) {   // first-phase keys, then one list entry per admitted key
+                    std::vector<uint32_t> s1(nq), ac(nq);
+                    HIPCHK(hipMemcpy(s1.data(), w->s1.p, nq * 4, hipMemcpyDeviceToHost));
+                    HIPCHK(hipMemcpy(ac.data(), w->adm_cnt.p, nq * 4, hipMemcpyDeviceToHost));
+                    uint64_t d1 = 0, adm = 0;
+                    for (uint32_t i = 0; i < nq; i++) {
+                        d1 += s1[i];
+                        adm += ac[i];
+                    }
+                    b += (double)d1 * key + (double)adm * 4 + 4.0 * nq;   // (+ the bounds)
+                    admitted = adm;
+                } else {
+                    b += (double)total * key;
+                }
             }
             g_prof.scan_bytes = (uint64_t)b;
+            g_prof.admitted = (uint32_t)std::min<uint64_t>(admitted, 0xffffffffu);
             g_prof.scan_flops = ix->quant == SPF_QUANT_RABITQ
                                     ? (z.rq_mfma ? 2ull * ix->rq_padded * real : 0)
                                     : 2ull * (z.int8 ? z.dpad64 : mixed ? dpad32 : dpad) * real;

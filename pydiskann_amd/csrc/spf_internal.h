@@ -193,6 +193,48 @@ struct MixedParams {
     // rows and their 64-dim K steps; per query the affine terms and |q|^2
     uint32_t int8;
     const float *q8a, *q8b, *qnorm;
+    // two-phase plan: item_post holds virtual postings (v >= n_post is posting
+    // v - n_post); the launch walks items [*item_lo_dev, *n_items_dev)
+    const uint32_t *item_lo_dev;   // null = 0
+    // admission (second phase): a candidate whose key is at most its query's
+    // bound tau[q] is listed as key << 16 | slot (within the segment at
+    // seg_off[q]) at the front of its pair's slot range for the work item's
+    // chunk, and the number listed goes to out16 at the chunk's first slot;
+    // nothing else is stored
+    const uint32_t *tau;
+    const uint64_t *seg_off;
+    uint32_t *adm_list;         // [slots] (null = dense keys into out16)
+};
+
+// Admission bound of each query: the `want`-th smallest live key among its
+// first-phase slots [seg_off[q], seg_off[q] + s1[q]), 0xfffe when they hold
+// fewer.
+struct AdmitParams {
+    const uint16_t *keys;
+    const uint64_t *seg_off;
+    const uint32_t *s1;
+    uint32_t nq, want;
+    uint32_t *tau;
+};
+
+// Pool of each query from the first phase's dense keys (at most tau) and the
+// second phase's per-chunk lists: the `want` smallest (key, slot) values as
+// key << 16 | slot, phase A (at least min(k, pool) of the members with the
+// smallest keys, at most ca when the keys allow) first.
+struct PoolListParams {
+    const uint16_t *keys;       // dense keys; second-phase chunk counts at chunk starts
+    const uint64_t *seg_off;
+    const uint32_t *s1, *m1, *np_q;
+    const uint32_t *cand_cnt;   // [nq] slots of each segment
+    const uint64_t *qp_slot;    // [nq][npr] absolute first slot of each probe
+    const uint32_t *tau;
+    const uint32_t *adm_list;
+    uint32_t nq, npr, echunk, want, ca, k;
+    uint32_t *pool;             // [nq][want]
+    uint32_t *pool_want;        // [nq]
+    uint32_t *pool_na;          // [nq] phase-A members
+    uint32_t *a_min;            // [nq] smallest key outside phase A (0xffff: none)
+    uint32_t *adm_cnt;          // [nq] second-phase keys listed (for the profile)
 };
 
 // Matrix-core scan work items: (posting, `mfma_chunk(ks)` entries, up to
@@ -240,6 +282,10 @@ struct RerankPrunedParams {
     // order until the bound passes the k-th exact distance
     const uint16_t *raw16;      // [n_entries][dpad] f16 rows (null = no f16 stage)
     const float *xn;            // [n_entries] |x|^2
+    // pool as lists (admission): [nq][want] key << 16 | slot, phase A's
+    // pool_na[q] members first; sel_bits / a_bits / keys unused
+    const uint32_t *pool;
+    const uint32_t *pool_na;
 };
 
 // RaBitQ: per-pair query prep, exact search, mixed estimator scan.
@@ -375,6 +421,12 @@ struct PlanParams {
     // posting counts in LDS, then exclusive over blocks (0 = global atomics)
     uint32_t qb;
     uint32_t *blk_cnt;          // [ceil(nq / qb)][n_post]
+    // two phases (split = real posting count, n_post = 2 split): a query's
+    // probes whose preceding probes hold fewer than `want` entries form the
+    // first phase, the rest are ranked under virtual posting post + split
+    uint32_t split, want;
+    uint32_t *m1;               // [nq] first-phase probes (probes j < m1)
+    uint32_t *s1;               // [nq] their slots (a prefix of the segment)
 };
 
 constexpr uint32_t kRankQB = 256;              // queries per ranking block
@@ -438,6 +490,8 @@ hipError_t launch_entry_norms(uint32_t code_kind, const uint4 *data, uint32_t un
 hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t s);
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s);
+hipError_t launch_admit(const AdmitParams &p, hipStream_t s);
+hipError_t launch_pool_list(const PoolListParams &p, hipStream_t s);
 hipError_t launch_rerank_pool(const RerankPoolParams &p, hipStream_t s);
 hipError_t launch_rerank_pruned(const RerankPrunedParams &p, hipStream_t s);
 hipError_t launch_rerank_f16(const RerankPrunedParams &p, hipStream_t s);

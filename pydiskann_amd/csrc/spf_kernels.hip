@@ -697,30 +697,50 @@ __global__ __launch_bounds__(256) void k_plan_q(const PlanParams p) {
                 }
             }
     }
-    uint32_t run = 0, real = 0;   // slots of the probes before this round; entries
+    // slots and entries of the probes before this round; the first probe of
+    // the second phase (two-phase plans) and the slots before it
+    uint32_t run = 0, real = 0, m1 = kept, s1 = 0;
     for (uint32_t j0 = 0; j0 < kept; j0 += 64) {
         const uint32_t j = j0 + lane;
-        uint32_t len = 0;
+        uint32_t len = 0, rl = 0, post = 0;
         if (j < kept) {
-            const uint32_t post = (uint32_t)sel[j];
-            real += p.post_len[post];
-            len = (p.post_len[post] + p.align - 1) / p.align * p.align;   // slots reserved
-            const size_t i = (size_t)q * p.npr + j;
-            p.probe_post[i] = post;
-            if (!p.qb) p.prank[i] = atomicAdd(&p.post_cnt[post], 1u);
+            post = (uint32_t)sel[j];
+            rl = p.post_len[post];
+            len = (rl + p.align - 1) / p.align * p.align;   // slots reserved
+            p.probe_post[(size_t)q * p.npr + j] = post;
         }
-        uint32_t incl = len;
+        uint32_t incl = len, rin = rl;
 #pragma unroll
         for (uint32_t off = 1; off < 64; off <<= 1) {
-            const uint32_t t = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += t;
+            const uint32_t t = __shfl_up(incl, off, 64), r = __shfl_up(rin, off, 64);
+            if (lane >= off) {
+                incl += t;
+                rin += r;
+            }
         }
-        if (j < kept) p.qp_rel[(size_t)q * p.npr + j] = run + incl - len;
+        // second phase: the probes before j already hold `want` entries
+        const bool second = p.split && j < kept && real + rin - rl >= p.want;
+        if (j < kept) {
+            const size_t i = (size_t)q * p.npr + j;
+            p.qp_rel[i] = run + incl - len;
+            if (!p.qb) p.prank[i] = atomicAdd(&p.post_cnt[post + (second ? p.split : 0u)], 1u);
+        }
+        const uint64_t sm = __ballot(second);
+        if (m1 == kept && sm) {
+            const uint32_t f = (uint32_t)(__ffsll((unsigned long long)sm) - 1);
+            m1 = j0 + f;
+            s1 = run + __shfl(incl - len, f, 64);
+        }
         run += __shfl(incl, 63, 64);
+        real += __shfl(rin, 63, 64);
     }
-    for (int off = 32; off > 0; off >>= 1) real += __shfl_xor(real, off, 64);
+    if (m1 == kept) s1 = run;
     if (lane == 0) {
         if (p.cand_real) p.cand_real[q] = real;
+        if (p.split) {
+            p.m1[q] = m1;
+            p.s1[q] = s1;
+        }
         p.np_q[q] = kept;
         p.cand_cnt[q] = run;
         p.cand_len[q] = (run + 31) & ~31ull;   // 32-aligned segments: a query owns whole bitmap words
@@ -734,8 +754,9 @@ __global__ void k_scatter_pairs(const PlanParams p) {
     if (i >= p.nq * p.npr) return;
     const uint32_t q = i / p.npr, j = i - q * p.npr;
     if (j >= p.np_q[q]) return;
-    const uint32_t post = p.probe_post[i];
     const uint64_t slot = p.cand_off[q] + p.qp_rel[i];
+    // (two phases: second-phase probes rank under post + split)
+    const uint32_t post = p.probe_post[i] + (p.split && j >= p.m1[q] ? p.split : 0u);
     const uint32_t pos = p.pair_off[post] + p.prank[i] +
                          (p.qb ? p.blk_cnt[(size_t)(q / p.qb) * p.n_post + post] : 0u);
     p.qp_slot[i] = slot;
@@ -756,7 +777,8 @@ __global__ __launch_bounds__(1024) void k_pair_rank(const PlanParams p) {
     const uint32_t q0 = b * p.qb, q1 = min(q0 + p.qb, p.nq);
     for (uint32_t i = q0 * p.npr + tid; i < q1 * p.npr; i += 1024) {
         const uint32_t q = i / p.npr, j = i - q * p.npr;
-        if (j < p.np_q[q]) p.prank[i] = atomicAdd(&s_h[p.probe_post[i]], 1u);
+        if (j < p.np_q[q])
+            p.prank[i] = atomicAdd(&s_h[p.probe_post[i] + (p.split && j >= p.m1[q] ? p.split : 0u)], 1u);
     }
     __syncthreads();
     uint32_t *row = p.blk_cnt + (size_t)b * p.n_post;
@@ -793,7 +815,7 @@ __global__ void k_items(const PlanParams p) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= p.n_post) return;
     const uint32_t cnt = p.pair_off[b + 1] - p.pair_off[b];
-    const uint32_t len = p.post_len[b];
+    const uint32_t len = p.post_len[p.split && b >= p.split ? b - p.split : b];
     p.item_cnt[b] = (cnt && len) ? ((cnt + p.G - 1) / p.G) * ((len + p.echunk - 1) / p.echunk) : 0u;
 }
 

@@ -106,29 +106,37 @@ hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t
 // so a lane's fragment is again one 16-byte unit: u8 codes shifted to signed
 // (c - 128, by flipping the top bit of every byte) against the symmetric int8
 // query; q.decode(c) = a * sum + b with the query's affine terms (a, b).
-template <int KS, bool FILT, bool I8>
+//
+// ADM: second phase of a two-phase scan (admission; no tile, no dense keys).
+template <int KS, bool FILT, bool I8, bool ADM>
 __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(const MixedParams p) {
     constexpr int T = KS <= 4 ? 4 : 2;   // tiles per wave, mfma_tiles()
     constexpr uint32_t EW = 64 * T;      // entries per item
-    __shared__ __align__(16) uint16_t s_tile[32][EW];   // one query group's keys
+    __shared__ __align__(16) uint16_t s_tile[ADM ? 1 : 32][ADM ? 8 : EW];   // one query group's keys
     __shared__ uint64_t s_slot[kMixQ];
     __shared__ uint32_t s_qi[kMixQ];
     __shared__ float s_qn[kMixQ];
     __shared__ float s_qa[I8 ? kMixQ : 1], s_qb[I8 ? kMixQ : 1];
     __shared__ uint64_t s_mask[kMixQ];
+    __shared__ uint32_t s_tau[ADM ? kMixQ : 1];
+    __shared__ uint64_t s_seg[ADM ? kMixQ : 1];
+    // admission: admitted keys of each row (pair) in the item's chunk, counted
+    __shared__ uint32_t s_rc[ADM ? kMixQ : 1];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 15, kg = lane >> 4;
     const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
-    const uint32_t n_items = p.n_items_dev ? *p.n_items_dev : p.n_items;
+    const uint32_t lo = p.item_lo_dev ? *p.item_lo_dev : 0u;
+    const uint32_t n_items = (p.n_items_dev ? *p.n_items_dev : p.n_items) - lo;
     for (XcdWalk it(n_items); it.more(); it.next()) {
-        const uint32_t item = it.item;
-        const uint32_t pp = p.item_post[item];
-        const uint32_t local = item - p.item_off[pp];
+        const uint32_t item = lo + it.item;
+        const uint32_t vp = p.item_post[item];   // (virtual) posting of the pair table
+        const uint32_t pp = vp >= p.n_post ? vp - p.n_post : vp;
+        const uint32_t local = item - p.item_off[vp];
         const uint32_t pb = p.post_off[pp], pe = p.post_off[pp + 1];
         const uint32_t nec = (pe - pb + p.echunk - 1) / p.echunk;
         const uint32_t qc = local / nec, ec = local - qc * nec;
-        const uint32_t pr0 = p.pair_off[pp] + qc * p.qg;
-        const uint32_t gcnt = min(p.qg, p.pair_off[pp + 1] - pr0);
+        const uint32_t pr0 = p.pair_off[vp] + qc * p.qg;
+        const uint32_t gcnt = min(p.qg, p.pair_off[vp + 1] - pr0);
         const uint32_t e_begin = pb + ec * p.echunk, e_end = min(e_begin + p.echunk, pe);
         __syncthreads();   // the previous item is done with the query table
         if (tid < kMixQ) {
@@ -136,6 +144,11 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
             const uint32_t q = ok ? p.pair_q[pr0 + tid] : 0u;
             s_qi[tid] = q;
             s_slot[tid] = ok ? p.pair_slot[pr0 + tid] : 0ull;
+            if constexpr (ADM) {   // bound + 1: rows past the group's queries admit nothing
+                s_tau[tid] = ok ? p.tau[q] + 1u : 0u;
+                s_seg[tid] = ok ? p.seg_off[q] : 0ull;
+                s_rc[tid] = 0u;
+            }
             float qn = ok ? (I8 ? p.qnorm[q] : p.qn16[q]) : 0.0f;
             if (p.metric == M_DOT) qn = 1.0f;
             else if (p.metric == M_COS) qn = qn > 0.0f ? 1.0f / sqrtf(qn) : 0.0f;
@@ -234,32 +247,58 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
                     // past-the-posting entries: the aligned range's tail).  Rows
                     // past the group's queries are written too and never stored.
                     const uint32_t el = e0 + col - e_begin;
+                    uint32_t kv[4];
+                    bool in[4];
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
                         const float d = l2 ? xn[t] + qv[j] - 2.0f * acc[j] : 1.0f - acc[j] * qv[j] * rx[t];
                         const bool ok = live[t] && (!FILT || label_ok(mv[j], lab[t]));
-                        s_tile[16 * h + 4 * kg + j][el] = ok ? (uint16_t)(fkey_m(d) >> 16) : (uint16_t)0xffff;
+                        kv[j] = ok ? fkey_m(d) >> 16 : 0xffffu;
+                        if constexpr (ADM) in[j] = kv[j] < s_tau[gb + 4 * kg + j];
+                        else s_tile[16 * h + 4 * kg + j][el] = (uint16_t)kv[j];
+                    }
+                    // admission: the few keys within their query's bound go to
+                    // the front of the pair's own slot range for this chunk
+                    // (its rank among them from an LDS counter), nothing else
+                    if constexpr (ADM) {
+                        if (__ballot(in[0] || in[1] || in[2] || in[3])) {
+#pragma unroll
+                            for (int j = 0; j < 4; j++)
+                                if (in[j]) {
+                                    const uint32_t g = gb + 4 * kg + j;
+                                    const uint64_t base = s_slot[g] + (e_begin - pb);
+                                    const uint32_t v = (kv[j] << 16) |
+                                                       ((uint32_t)(base - s_seg[g]) + (e0 + col - e_begin));
+                                    p.adm_list[base + atomicAdd(&s_rc[g], 1u)] = v;
+                                }
+                        }
                     }
                 }
             }
-            // the tiles written above end at the 16-entry multiple past the
-            // posting's end; "no key" from there to the reserved end
-            const uint32_t len = e_end - e_begin, l16 = (len + 15) & ~15u;
-            const uint32_t lp = e_end == pe ? (len + kScanAlign - 1) / kScanAlign * kScanAlign : len;
-            if (lp > l16)
-                for (uint32_t c = tid; c < 32 * (lp - l16); c += 256)
-                    s_tile[c / (lp - l16)][l16 + c % (lp - l16)] = 0xffff;
-            __syncthreads();
-            // the tile's rows as aligned 16-byte stores: each probe's slot
-            // range starts at a multiple of kScanAlign and is reserved up to
-            // the next multiple past the posting's end
-            const uint32_t cpr = lp / 8, rows = min(32u, gcnt - g0);
-            for (uint32_t c = tid; c < rows * cpr; c += 256) {
-                const uint32_t r = c / cpr, ch = c - r * cpr;
-                const uint4 v = *reinterpret_cast<const uint4 *>(&s_tile[r][8 * ch]);
-                *reinterpret_cast<uint4 *>(p.out16 + s_slot[g0 + r] + (e_begin - pb) + 8 * ch) = v;
+            if constexpr (!ADM) {   // (with admission the keys are listed above; no tile)
+                const uint32_t len = e_end - e_begin, l16 = (len + 15) & ~15u;
+                // the tiles written above end at the 16-entry multiple past the
+                // posting's end; "no key" from there to the reserved end
+                const uint32_t lp = e_end == pe ? (len + kScanAlign - 1) / kScanAlign * kScanAlign : len;
+                if (lp > l16)
+                    for (uint32_t c = tid; c < 32 * (lp - l16); c += 256)
+                        s_tile[c / (lp - l16)][l16 + c % (lp - l16)] = 0xffff;
+                __syncthreads();
+                // the tile's rows as aligned 16-byte stores: each probe's slot
+                // range starts at a multiple of kScanAlign and is reserved up to
+                // the next multiple past the posting's end
+                const uint32_t cpr = lp / 8, rows = min(32u, gcnt - g0);
+                for (uint32_t c = tid; c < rows * cpr; c += 256) {
+                    const uint32_t r = c / cpr, ch = c - r * cpr;
+                    const uint4 v = *reinterpret_cast<const uint4 *>(&s_tile[r][8 * ch]);
+                    *reinterpret_cast<uint4 *>(p.out16 + s_slot[g0 + r] + (e_begin - pb) + 8 * ch) = v;
+                }
+                __syncthreads();   // the next group reuses the tile
             }
-            __syncthreads();   // the next group reuses the tile
+        }
+        if constexpr (ADM) {   // each pair's count for the chunk, at the chunk's first slot
+            __syncthreads();
+            if (tid < gcnt) p.out16[s_slot[tid] + (e_begin - pb)] = (uint16_t)s_rc[tid];
         }
     }
 }
@@ -914,6 +953,448 @@ __global__ __launch_bounds__(256, 4) void k_pool_select(const PoolParams p, uint
                              BITS ? p.a_min : nullptr);
 }
 
+// ---------------------------------------------------------------- admission
+// Two-phase scans: the first phase (each query's nearest probes, up to `want`
+// entries) stores dense keys; k_admit takes the `want`-th smallest live key
+// among them as the query's bound, an upper bound of its pool's largest key
+// (the pool is the `want` smallest keys of a superset); the second phase lists
+// only keys at most the bound; k_pool_list selects the pool from both.
+
+// bin (of the wave's 256-bin histogram h) holding the need-th smallest value
+// (1 <= need <= total); *below = values in lower bins, *cnt = values in it
+__device__ __forceinline__ uint32_t wave_bin(const uint32_t *h, uint32_t need, uint32_t *below,
+                                             uint32_t *cnt) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t c[4], sum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        c[i] = h[4 * lane + i];
+        sum += c[i];
+    }
+    const uint32_t incl = wave_incl_scan(sum);
+    const uint64_t m = __ballot(incl >= need);
+    const uint32_t f = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+    uint32_t b = 0, run = incl - sum, cb = 0;
+    if (lane == f) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (run + c[i] >= need) {
+                b = 4 * lane + i;
+                cb = c[i];
+                break;
+            }
+            run += c[i];
+        }
+    }
+    *below = __shfl(run, f, 64);
+    *cnt = __shfl(cb, f, 64);
+    return __shfl(b, f, 64);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Wave-level radix select over the values in [vmin, vmax] that visit(f)
+// passes (f(v) once per counted value; lanes may make different numbers of
+// calls): the need-th smallest, or, when a digit's bin holds exactly the
+// values still needed, the largest value that bin could hold (clamped to
+// vmax).  Either way at least `need` counted values (exactly `need` when they
+// are distinct) lie at or below the result, and fewer below it.  Digits are
+// 8 bits of v - vmin from the top of the range down, so the first digit
+// already spreads the values over the bins (few same-bin LDS atomics).
+template <class Visit>
+__device__ __forceinline__ uint32_t wave_select(uint32_t *h, uint32_t need, uint32_t vmin,
+                                                uint32_t vmax, Visit &&visit) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t span = vmax - vmin;
+    const int bits = span ? 32 - __clz(span) : 0;
+    int sh = bits > 8 ? bits - 8 : 0, fixed = 32;   // offset bits >= fixed are decided
+    uint32_t prefix = 0;
+    for (;;) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) h[4 * lane + i] = 0u;
+        wave_lds_sync();
+        visit([&](uint32_t v) {
+            if (v < vmin || v > vmax) return;
+            const uint32_t o = v - vmin;
+            if (fixed < 32 && (o >> fixed) != (prefix >> fixed)) return;
+            atomicAdd(&h[(o >> sh) & 255u], 1u);
+        });
+        wave_lds_sync();
+        uint32_t below, cb;
+        const uint32_t b = wave_bin(h, need, &below, &cb);
+        prefix |= b << sh;
+        need -= below;
+        wave_lds_sync();   // every lane has read the histogram
+        if (cb == need) return vmin + min(prefix | ((1u << sh) - 1u), span);
+        if (sh == 0) return vmin + prefix;
+        fixed = sh;
+        sh = sh > 8 ? sh - 8 : 0;
+    }
+}
+
+__device__ __forceinline__ void wave_minmax(uint32_t &mn, uint32_t &mx) {
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor(mn, off, 64));
+        mx = max(mx, (uint32_t)__shfl_xor(mx, off, 64));
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// the eight 16-bit keys of unit u (u < nu) of a segment, every lane alike
+template <class F>
+__device__ __forceinline__ void visit_units(const uint4 *kp, uint32_t nu, F &&f) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t u0 = 0; u0 < nu; u0 += 128) {   // two loads in flight per lane
+        uint4 v[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const uint32_t u = u0 + 64 * c + lane;
+            v[c] = u < nu ? kp[u] : make_uint4(~0u, ~0u, ~0u, ~0u);
+        }
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const uint32_t u = u0 + 64 * c + lane;
+            const uint32_t w4[4] = {v[c].x, v[c].y, v[c].z, v[c].w};
+#pragma unroll
+            for (int t = 0; t < 8; t++) f((w4[t >> 1] >> (16 * (t & 1))) & 0xffffu, 8 * u + t);
+        }
+    }
+}
+
+// one wave per query, four per workgroup (no workgroup barriers)
+__global__ __launch_bounds__(256) void k_admit(const AdmitParams p) {
+    __shared__ uint32_t hist[4][256];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t q = blockIdx.x * 4 + wv;
+    if (q >= p.nq) return;   // whole wave
+    const uint4 *kp = reinterpret_cast<const uint4 *>(p.keys + p.seg_off[q]);
+    const uint32_t nu = p.s1[q] / 8;   // first-phase slots: a multiple of the scan alignment (8)
+    uint32_t n = 0, mn = ~0u, mx = 0u;
+    visit_units(kp, nu, [&](uint32_t k, uint32_t) {
+        if (k == 0xffffu) return;
+        n++;
+        mn = min(mn, k);
+        mx = max(mx, k);
+    });
+    n = wave_sum(n);
+    wave_minmax(mn, mx);
+    uint32_t tau = 0xfffeu;   // fewer live keys than `want`: admit every key
+    if (n >= p.want)
+        tau = wave_select(hist[wv], p.want, mn, mx, [&](auto &&f) {
+            visit_units(kp, nu, [&](uint32_t k, uint32_t) {
+                if (k != 0xffffu) f(k);
+            });
+        });
+    if (lane == 0) p.tau[q] = tau;
+}
+
+hipError_t launch_admit(const AdmitParams &p, hipStream_t s) {
+    if (p.nq == 0) return hipSuccess;
+    if (p.want == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_admit, dim3((p.nq + 3) / 4), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+constexpr uint32_t kPoolLds = 8192;     // a query's pool candidates held in LDS (else re-read)
+constexpr uint32_t kPoolRanges = 512;   // second-phase chunk lists indexed in LDS
+
+struct PoolListLds {
+    uint32_t v[kPoolLds];                       // candidates, key << 16 | slot
+    uint32_t h[256];                            // histogram
+    uint32_t rs[kPoolRanges], rp[kPoolRanges];  // chunk list: start slot, prefix
+    uint32_t w[8], r[8];
+};
+
+// bin of L.h holding the need-th smallest value (1 <= need <= total), over
+// the workgroup: *below = values in lower bins, *cnt = values in it
+__device__ __forceinline__ uint32_t block_bin(PoolListLds &L, uint32_t need, uint32_t *below,
+                                              uint32_t *cnt) {
+    const uint32_t t = threadIdx.x, c = L.h[t];
+    uint32_t tot;
+    const uint32_t ex = block_scan_fast(c, L.w, &tot);
+    if (ex < need && ex + c >= need) {
+        L.r[0] = t;
+        L.r[1] = ex;
+        L.r[2] = c;
+    }
+    __syncthreads();
+    const uint32_t b = L.r[0];
+    *below = L.r[1];
+    *cnt = L.r[2];
+    __syncthreads();   // (L.r and the histogram are rewritten next)
+    return b;
+}
+
+// One workgroup per query.  Candidates: first-phase keys at most tau, then
+// every second-phase chunk's listed keys (the chunks, from the probes' slot
+// ranges, indexed by a prefix table and copied by flat index: independent
+// loads, no atomics).  A key histogram (bins of the keys' range, a finer
+// pass when it exceeds 256 keys) gives the pool's boundary key K (its
+// `want`-th smallest) and phase A's boundary key KA (its min(ca, pool)-th
+// smallest); pool members: keys below K, and of the keys equal to K those
+// with the smallest slots (a radix select over the slots, only when not all
+// of them are taken); phase A: members with keys below KA when there are k of
+// them, else up to KA.
+__global__ __launch_bounds__(256) void k_pool_list(const PoolListParams p, uint32_t k) {
+    __shared__ PoolListLds L;
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t q = blockIdx.x;
+    const uint64_t seg = p.seg_off[q];
+    const uint4 *kp = reinterpret_cast<const uint4 *>(p.keys + seg);
+    const uint32_t nu = p.s1[q] / 8, tau = p.tau[q], m1 = p.m1[q], np = p.np_q[q];
+    const uint32_t send = p.cand_cnt[q];   // the segment's slots (the last probe's end)
+    const uint64_t *qp = p.qp_slot + (size_t)q * p.npr;
+    uint32_t kmin = 0xffffu, kmax = 0u;
+    // 1. first-phase candidates: thread per 8-key unit, positions by a scan
+    uint32_t n1 = 0;
+    for (uint32_t u0 = 0; u0 < nu; u0 += 256) {
+        const uint32_t u = u0 + tid;
+        uint32_t w4[4] = {~0u, ~0u, ~0u, ~0u};
+        if (u < nu) {
+            const uint4 v = kp[u];
+            w4[0] = v.x;
+            w4[1] = v.y;
+            w4[2] = v.z;
+            w4[3] = v.w;
+        }
+        uint32_t m = 0;
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+            m |= (uint32_t)(((w4[t >> 1] >> (16 * (t & 1))) & 0xffffu) <= tau) << t;
+        uint32_t tot;
+        uint32_t pos = n1 + block_scan_fast((uint32_t)__popc(m), L.w, &tot);
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+            if ((m >> t) & 1u) {
+                const uint32_t kk = (w4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+                if (pos < kPoolLds) L.v[pos] = (kk << 16) | (8 * u + t);
+                pos++;
+                kmin = min(kmin, kk);
+                kmax = max(kmax, kk);
+            }
+        n1 += tot;
+    }
+    // 2. second-phase chunk lists from the probes' slot ranges (each probe's
+    // range is its posting's length rounded up to the scan alignment, which
+    // divides the chunk size): thread per probe, (start slot, prefix)
+    uint32_t nr = 0, na2 = 0;
+    bool table = true;
+    for (uint32_t j0 = m1; j0 < np; j0 += 256) {
+        const uint32_t j = j0 + tid;
+        const uint32_t rel = j < np ? (uint32_t)(qp[j] - seg) : send;
+        const uint32_t end = j + 1 < np ? (uint32_t)(qp[j + 1] - seg) : send;
+        const uint32_t nch = j < np ? (end - rel + p.echunk - 1) / p.echunk : 0u;
+        uint32_t cnt[4], run = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++) {   // (most probes span few chunks: loads in flight)
+            cnt[c] = c < nch ? p.keys[seg + rel + c * p.echunk] : 0u;
+            run += cnt[c];
+        }
+        for (uint32_t c = 4; c < nch; c++) run += p.keys[seg + rel + c * p.echunk];
+        uint32_t tot_ch, tot_k;
+        const uint32_t ch0 = nr + block_scan_fast(nch, L.w, &tot_ch);
+        const uint32_t k0 = na2 + block_scan_fast(run, L.w, &tot_k);
+        if (nr + tot_ch <= kPoolRanges) {
+            uint32_t acc = k0;
+#pragma unroll
+            for (uint32_t c = 0; c < 4; c++)
+                if (c < nch) {
+                    L.rs[ch0 + c] = rel + c * p.echunk;
+                    L.rp[ch0 + c] = acc;
+                    acc += cnt[c];
+                }
+            for (uint32_t c = 4; c < nch; c++) {
+                L.rs[ch0 + c] = rel + c * p.echunk;
+                L.rp[ch0 + c] = acc;
+                acc += p.keys[seg + rel + c * p.echunk];
+            }
+        } else {
+            table = false;
+        }
+        nr += tot_ch;
+        na2 += tot_k;
+    }
+    __syncthreads();   // the table
+    // list entry i of the second phase (table present)
+    auto entry = [&](uint32_t i) {
+        uint32_t lo = 0, hi = nr;   // rp[lo] <= i < rp[lo + 1] (empty lists share a prefix)
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (L.rp[mid] <= i) lo = mid; else hi = mid;
+        }
+        return p.adm_list[seg + L.rs[lo] + (i - L.rp[lo])];
+    };
+    // every second-phase candidate once, f(v)
+    auto visit_list = [&](auto &&f) {
+        if (table) {
+            for (uint32_t i0 = 0; i0 < na2; i0 += 1024) {   // four loads in flight per thread
+                uint32_t v[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const uint32_t i = i0 + 256 * c + tid;
+                    v[c] = i < na2 ? entry(i) : 0u;
+                }
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    if (i0 + 256 * c + tid < na2) f(v[c]);
+            }
+        } else {   // more chunks than the table holds: probe by probe
+            for (uint32_t j = m1 + tid; j < np; j += 256) {
+                const uint32_t rel = (uint32_t)(qp[j] - seg);
+                const uint32_t end = j + 1 < np ? (uint32_t)(qp[j + 1] - seg) : send;
+                for (uint32_t c0 = rel; c0 < end; c0 += p.echunk) {
+                    const uint32_t cnt = p.keys[seg + c0];
+                    for (uint32_t i = 0; i < cnt; i++) f(p.adm_list[seg + c0 + i]);
+                }
+            }
+        }
+    };
+    const uint32_t n = n1 + na2;
+    const bool lds = n <= kPoolLds;
+    if (tid == 0) L.r[4] = n1;
+    __syncthreads();
+    visit_list([&](uint32_t v) {   // the list after the first-phase candidates; the key range
+        if (lds) L.v[atomicAdd(&L.r[4], 1u)] = v;
+        kmin = min(kmin, v >> 16);
+        kmax = max(kmax, v >> 16);
+    });
+    // key range over the workgroup
+    wave_minmax(kmin, kmax);
+    if (lane == 0) {
+        L.w[4 + (tid >> 6)] = kmin;
+        L.h[tid >> 6] = kmax;
+    }
+    __syncthreads();
+    kmin = min(min(L.w[4], L.w[5]), min(L.w[6], L.w[7]));
+    kmax = max(max(L.h[0], L.h[1]), max(L.h[2], L.h[3]));
+    __syncthreads();
+    // every candidate once, f(v)
+    auto visit = [&](auto &&f) {
+        if (lds) {
+            for (uint32_t i = tid; i < n; i += 256) f(L.v[i]);
+        } else {
+            for (uint32_t u = tid; u < nu; u += 256) {
+                const uint4 v = kp[u];
+                const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    const uint32_t kk = (w4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+                    if (kk <= tau) f((kk << 16) | (8 * u + t));
+                }
+            }
+            visit_list(f);
+        }
+    };
+    // histogram of sel(v) (values whose bin is below 256)
+    auto hist = [&](auto &&sel) {
+        L.h[tid] = 0u;
+        __syncthreads();
+        visit([&](uint32_t v) {
+            const uint32_t b = sel(v);
+            if (b < 256u) atomicAdd(&L.h[b], 1u);
+        });
+        __syncthreads();
+    };
+    const uint32_t c = min(n, p.want), need_a = min(c, p.ca);
+    uint32_t kst = 0x10000u, vst = ~0u, ka = 0x10000u, nA = c;   // (n <= want: every candidate)
+    bool a_lt = false;   // phase A: keys below ka (else up to ka)
+    if (n > 0) {
+        const uint32_t span = kmax - kmin;
+        const int sh = span > 255 ? 32 - __clz(span) - 8 : 0;   // (span < 2^16: sh <= 8)
+        const uint32_t lowm = (1u << sh) - 1u;
+        hist([&](uint32_t v) { return ((v >> 16) - kmin) >> sh; });
+        const bool bound = n > p.want;
+        uint32_t bl_s = 0, cb_s = 0, bl_a, cb_a, bs = 0;
+        if (bound) bs = block_bin(L, p.want, &bl_s, &cb_s);
+        uint32_t ba = block_bin(L, need_a, &bl_a, &cb_a);
+        // the finer digit of a coarse bin: the key holding the need-th value
+        auto fine = [&](uint32_t coarse, uint32_t need, uint32_t &bl, uint32_t &cb) -> uint32_t {
+            hist([&](uint32_t v) {
+                const uint32_t o = (v >> 16) - kmin;
+                return (o >> sh) == coarse ? (o & lowm) : 256u;
+            });
+            uint32_t l2, c2;
+            const uint32_t b2 = block_bin(L, need - bl, &l2, &c2);
+            bl += l2;
+            cb = c2;
+            return (coarse << sh) + b2;
+        };
+        if (sh > 0) {
+            if (bound) bs = fine(bs, p.want, bl_s, cb_s);
+            ba = fine(ba, need_a, bl_a, cb_a);
+        }
+        ka = kmin + ba;
+        if (bound) {
+            kst = kmin + bs;
+            uint32_t m = p.want - bl_s;   // keys equal to kst the pool takes
+            if (m < cb_s) {   // the m smallest slots among them: two 8-bit digits
+                const uint32_t base = kst << 16;
+                hist([&](uint32_t v) { return (v >> 16) == kst ? (v >> 8) & 255u : 256u; });
+                uint32_t l1, c1;
+                const uint32_t s_hi = block_bin(L, m, &l1, &c1);
+                m -= l1;
+                if (m < c1) {
+                    const uint32_t hi8 = (base >> 8) | s_hi;
+                    hist([&](uint32_t v) { return (v >> 8) == hi8 ? v & 255u : 256u; });
+                    uint32_t l2, c2;
+                    vst = base | (s_hi << 8) | block_bin(L, m, &l2, &c2);
+                } else {
+                    vst = base | (s_hi << 8) | 255u;
+                }
+            } else {
+                vst = (kst << 16) | 0xffffu;
+            }
+        }
+        // phase A: keys below ka when k of them exist (its size then at most
+        // ca), else keys up to ka (all members when ka is the pool's boundary)
+        a_lt = bl_a >= k;
+        nA = a_lt ? bl_a : ka < kst ? bl_a + cb_a : c;
+    }
+    // members: phase A first, then the rest (each part in any order)
+    if (tid == 0) {
+        L.r[4] = 0u;
+        L.r[5] = nA;
+        L.r[6] = 0xffffu;
+    }
+    __syncthreads();
+    uint32_t *out = p.pool + (size_t)q * p.want;
+    uint32_t amin = 0xffffu;
+    visit([&](uint32_t v) {
+        const uint32_t kk = v >> 16;
+        if (!(kk < kst || (kk == kst && v <= vst))) return;
+        if (a_lt ? kk < ka : kk <= ka) {
+            out[atomicAdd(&L.r[4], 1u)] = v;
+        } else {
+            out[atomicAdd(&L.r[5], 1u)] = v;
+            amin = min(amin, kk);
+        }
+    });
+    for (int off = 32; off > 0; off >>= 1) amin = min(amin, (uint32_t)__shfl_xor(amin, off, 64));
+    if (lane == 0 && amin != 0xffffu) atomicMin(&L.r[6], amin);
+    __syncthreads();
+    if (tid == 0) {
+        p.pool_want[q] = c;
+        p.pool_na[q] = nA;
+        p.a_min[q] = L.r[6];
+        p.adm_cnt[q] = na2;
+    }
+}
+
+hipError_t launch_pool_list(const PoolListParams &p, hipStream_t s) {
+    if (p.nq == 0) return hipSuccess;
+    if (p.ca == 0 || p.ca > p.want || p.echunk == 0 || p.k == 0 || p.k > p.ca)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pool_list, dim3(p.nq), dim3(256), 0, s, p, p.k);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- rerank
 // Exact f32 distance, four lanes per pair (lane r of the quad runs chain r over
 // units r, r+4, ...), H pairs per quad in flight; canonical 4-chain order.
@@ -1205,17 +1686,20 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     const uint32_t c = min(p.pool_want[q], p.want);
     uint64_t *out = p.out + (size_t)q * p.k;
-    const uint32_t nA = min(p.ca, c);
+    // phase A: the `ca` smallest keys, or with list pools the members
+    // listed first (at least min(ca, c) of them)
+    const uint32_t nA = p.pool ? min(p.pool_na[q], c) : min(p.ca, c);
     const bool prune = p.prune && nA < c && nA >= p.k;
     const uint64_t b = p.seg_off[q];
     const uint32_t n = p.seg_cnt[q];
     const uint32_t nwords = (n + 31) / 32, Mw = (nwords + 255) / 256;
     const uint32_t w0 = tid * Mw, w1 = min(w0 + Mw, nwords);
-    const uint32_t *sel = p.sel_bits + b / 32, *abits = p.a_bits + b / 32;
+    const uint32_t *sel = p.sel_bits ? p.sel_bits + b / 32 : nullptr;
+    const uint32_t *abits = p.a_bits ? p.a_bits + b / 32 : nullptr;
     // the thread's mask words, loaded with the table below (up to 4 in registers)
     const bool cached = Mw <= 4;
     uint32_t sm[4] = {0u, 0u, 0u, 0u}, am[4] = {0u, 0u, 0u, 0u};
-    if (cached)
+    if (cached && !p.pool)
 #pragma unroll
         for (uint32_t i = 0; i < 4; i++)
             if (w0 + i < w1) {
@@ -1279,9 +1763,26 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
         tab.pbase = L.pbase;
         tab.ebase = L.ebase;
     }
+    // list pools: members i0 .. i1 - 1 with key at most thr, as entries (any order)
+    const uint32_t *pool = p.pool ? p.pool + (size_t)q * p.want : nullptr;
+    auto list_members = [&](uint32_t i0, uint32_t i1, uint32_t thr, uint32_t *dst) -> uint32_t {
+        if (tid == 0) L.m[13] = 0u;
+        __syncthreads();   // (also publishes the probe table)
+        for (uint32_t i = i0 + tid; i < i1; i += 256) {
+            const uint32_t v = pool[i];
+            if ((v >> 16) > thr) continue;
+            const uint32_t slot = v & 0xffffu, j = tab.find(slot);
+            dst[atomicAdd(&L.m[13], 1u)] = tab.first_entry(j) + (slot - tab.rel(j));
+        }
+        __syncthreads();
+        const uint32_t cnt = L.m[13];
+        __syncthreads();
+        return cnt;
+    };
     if (!prune) {   // rerank the whole pool (rebuild_members' barriers publish the table)
-        const uint32_t cnt = rebuild_members(L, w0, w1, [&](uint32_t w) { return sel_w(w); }, tab,
-                                             L.ent, c);
+        const uint32_t cnt = pool ? list_members(0, c, 0xffffu, L.ent)
+                                  : rebuild_members(L, w0, w1, [&](uint32_t w) { return sel_w(w); },
+                                                    tab, L.ent, c);
         rerank_list(p, L.ent, cnt, s_qv, qn, L.ex);
         __syncthreads();
         smallest_k(L.ex, cnt, p.k, emit);
@@ -1299,8 +1800,9 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
     }
     // 1. phase A (the `ca` smallest keys, marked by pool selection): rerank,
     // then its k-th exact key (keys are distinct: the rank is in the low bits)
-    const uint32_t na = rebuild_members(L, w0, w1, [&](uint32_t w) { return a_w(w); }, tab,
-                                        L.ent, nA);
+    const uint32_t na = pool ? list_members(0, nA, 0xffffu, L.ent)
+                             : rebuild_members(L, w0, w1, [&](uint32_t w) { return a_w(w); }, tab,
+                                               L.ent, nA);
     rerank_list(p, L.ent, na, s_qv, qn, L.ex);
     __syncthreads();
     if (na <= kPrCount) {   // ranks kept (in nzp, free until the survivor pass)
@@ -1358,7 +1860,7 @@ __global__ __launch_bounds__(256) void k_rerank_pruned(const RerankPrunedParams 
         return;
     }
     const uint16_t *kp = p.keys + b;
-    const uint32_t nb = rebuild_members(L, w0, w1, [&](uint32_t w) {
+    const uint32_t nb = pool ? list_members(nA, c, thr, L.ent + na) : rebuild_members(L, w0, w1, [&](uint32_t w) {
         const uint32_t rest = sel_w(w) & ~a_w(w);
         if (!rest) return 0u;
         uint4 r[4];
@@ -1553,14 +2055,16 @@ hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) 
     const bool f = p.qmask != nullptr;   // label filter
     // K steps: 32 dims (f16: dim <= 32, 64, 96, 128, 256) or 64 dims (int8: dim <= 64 ... 512)
     switch (p.ks) {
-#define SPF_SCAN_MFMA(KS_)                                                                          \
+#define SPF_SCAN_MFMA_A(KS_, A_)                                                                    \
         if (p.int8) {                                                                               \
-            if (f) hipLaunchKernelGGL((k_scan_mfma<KS_, true, true>), dim3(grid), dim3(256), 0, s, p); \
-            else hipLaunchKernelGGL((k_scan_mfma<KS_, false, true>), dim3(grid), dim3(256), 0, s, p);  \
+            if (f) hipLaunchKernelGGL((k_scan_mfma<KS_, true, true, A_>), dim3(grid), dim3(256), 0, s, p); \
+            else hipLaunchKernelGGL((k_scan_mfma<KS_, false, true, A_>), dim3(grid), dim3(256), 0, s, p);  \
         } else {                                                                                    \
-            if (f) hipLaunchKernelGGL((k_scan_mfma<KS_, true, false>), dim3(grid), dim3(256), 0, s, p); \
-            else hipLaunchKernelGGL((k_scan_mfma<KS_, false, false>), dim3(grid), dim3(256), 0, s, p);  \
-        }                                                                                           \
+            if (f) hipLaunchKernelGGL((k_scan_mfma<KS_, true, false, A_>), dim3(grid), dim3(256), 0, s, p); \
+            else hipLaunchKernelGGL((k_scan_mfma<KS_, false, false, A_>), dim3(grid), dim3(256), 0, s, p);  \
+        }
+#define SPF_SCAN_MFMA(KS_)                                                                          \
+        if (p.adm_list) { SPF_SCAN_MFMA_A(KS_, true) } else { SPF_SCAN_MFMA_A(KS_, false) }        \
         break;
     case 1: SPF_SCAN_MFMA(1)
     case 2: SPF_SCAN_MFMA(2)
@@ -1568,6 +2072,7 @@ hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) 
     case 4: SPF_SCAN_MFMA(4)
     case 5: case 6: case 7: case 8: SPF_SCAN_MFMA(8)
 #undef SPF_SCAN_MFMA
+#undef SPF_SCAN_MFMA_A
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
