@@ -121,10 +121,11 @@ typedef struct spf_index_desc {
                                    int8 matrix-core products (the same integers, same result) */
 #define SPF_FLAG_ADMIT 32u      /* MIXED over f32 / f16 / int8 codes: scan each query's
                                    nearest probes first (until they hold rerank_size entries),
-                                   bound its pool by their keys, then store only the keys
-                                   within that bound instead of every candidate's key (the
-                                   same pools, same result; pays off when selecting pools
-                                   from long candidate lists dominates the scan) */
+                                   bound its pool by their keys, then list only the keys
+                                   within that bound, per query, instead of storing every
+                                   candidate's key (the same pools, same result; pays off
+                                   when few queries share a posting and selecting pools from
+                                   long candidate lists dominates the scan) */
 
 /* Planning.  A batch's candidate count is known only after its probe.  By
  * default the library sizes the candidate buffers from a bound derived from

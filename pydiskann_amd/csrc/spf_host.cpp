@@ -937,6 +937,10 @@ struct Shape {
     // virtual postings, compact lists of key << 16 | slot (segments < 2^16 slots)
     bool admit;
     uint32_t Pv;     // postings of the pair and item tables (P, or 2 P with admit)
+    // matrix-core scan by waves: one wave per item of G = 32 or 64 queries
+    // (the tuned mix_g below kMixQ), kWaveChunk entries; else the workgroup
+    // scan (kMixQ queries, mfma_chunk entries)
+    bool wave;
 };
 
 // Incremental update of the mirror after mutations of the reference index
@@ -1218,7 +1222,7 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     mp.del = ix->d_del;
     mp.metric = ix->metric;
     mp.echunk = z.echunk;
-    mp.qg = ix->mix_g;
+    mp.qg = z.G;
     if (z.int8) {   // u8 codes against the int8 query rows
         mp.int8 = 1;
         mp.codes16 = ix->d_codes;
@@ -1243,9 +1247,11 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
         else HIPCHK(launch_rq_scan(*rq, grid, s));
     } else if (admit) {
         // first phase (each query's nearest probes): dense keys; the bound of
-        // every query from them; second phase: only keys within the bound
+        // every query from them; second phase: only keys within the bound,
+        // listed per query
+        const uint32_t wgrid = xcd_grid(n_items, 65536);
         mp.n_items_dev = w->item_off.p + ix->P;
-        HIPCHK(launch_scan_mfma(mp, grid, s));
+        HIPCHK(launch_scan_wave(mp, wgrid, s));
         AdmitParams ap{};
         ap.keys = w->cand16.p;
         ap.seg_off = w->cand_off.p;
@@ -1253,13 +1259,17 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
         ap.nq = nq;
         ap.want = want;
         ap.tau = w->tau.p;
+        ap.cnt = w->adm_cnt.p;
         HIPCHK(launch_admit(ap, s));
         mp.item_lo_dev = w->item_off.p + ix->P;
         mp.n_items_dev = n_items_dev;
         mp.tau = w->tau.p;
         mp.seg_off = w->cand_off.p;
         mp.adm_list = w->adm_list.p;
-        HIPCHK(launch_scan_mfma(mp, grid, s));
+        mp.adm_cnt = w->adm_cnt.p;
+        HIPCHK(launch_scan_wave(mp, wgrid, s));
+    } else if (z.wave) {
+        HIPCHK(launch_scan_wave(mp, xcd_grid(n_items, 65536), s));
     } else {
         HIPCHK(launch_scan_mfma(mp, grid, s));
     }
@@ -1273,15 +1283,10 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
         pq.keys = w->cand16.p;
         pq.seg_off = w->cand_off.p;
         pq.s1 = w->s1.p;
-        pq.m1 = w->m1.p;
-        pq.np_q = w->np_q.p;
-        pq.qp_slot = w->qp_slot.p;
-        pq.cand_cnt = w->cand_cnt.p;
         pq.tau = w->tau.p;
         pq.adm_list = w->adm_list.p;
+        pq.adm_cnt = w->adm_cnt.p;
         pq.nq = nq;
-        pq.npr = npr;
-        pq.echunk = z.echunk;
         pq.want = want;
         pq.ca = ca;
         pq.k = k;
@@ -1289,7 +1294,6 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
         pq.pool_want = w->pool_want.p;
         pq.pool_na = w->pool_na.p;
         pq.a_min = w->a_min.p;
-        pq.adm_cnt = w->adm_cnt.p;
         HIPCHK(launch_pool_list(pq, s));
     } else {
     PoolParams pp{};
@@ -1414,7 +1418,13 @@ int search_shape(const spf_index *ix, uint32_t k, uint32_t n_probe, const spf_se
     z.mfma_scan = z.mixed && !rq;
     z.G = z.mfma_scan ? ix->mix_g : z.rq_mfma ? kRqMfmaQ : z.mixed ? 8u : z.Gv;
     z.mks = z.int8 ? z.dpad64 / 64 : z.dpad32 / 32;
-    z.echunk = z.mfma_scan ? mfma_chunk(z.mks)
+    // (admission lists its second phase from the wave scan: 64-query items)
+    const bool adm_asked = z.mfma_scan && opts && (opts->flags & SPF_FLAG_ADMIT);
+    z.wave = z.mfma_scan && (adm_asked || ix->mix_g < kMixQ);
+    if (z.wave) z.G = std::min(ix->mix_g, z.mks > 4 ? 32u : 64u);   // (64 queries of 8 K steps
+                                                                      // would not fit the registers)
+    z.echunk = z.wave ? kWaveChunk
+               : z.mfma_scan ? mfma_chunk(z.mks)
                : z.rq_mfma ? 64 * rq_mfma_tiles(ix->rq_padded / 64)
                : (z.mixed ? kEChunk : ix->scan_chunk);
     z.round16 = ix->quant == SPF_QUANT_F16 &&

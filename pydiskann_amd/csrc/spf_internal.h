@@ -196,14 +196,15 @@ struct MixedParams {
     // two-phase plan: item_post holds virtual postings (v >= n_post is posting
     // v - n_post); the launch walks items [*item_lo_dev, *n_items_dev)
     const uint32_t *item_lo_dev;   // null = 0
-    // admission (second phase): a candidate whose key is at most its query's
-    // bound tau[q] is listed as key << 16 | slot (within the segment at
-    // seg_off[q]) at the front of its pair's slot range for the work item's
-    // chunk, and the number listed goes to out16 at the chunk's first slot;
+    // admission (second phase, wave scan): a candidate whose key is at most
+    // its query's bound tau[q] is appended to the query's list, which starts
+    // at adm_list + seg_off[q] (room for every slot of the segment), as
+    // key << 16 | slot within the segment, its length counted in adm_cnt[q];
     // nothing else is stored
     const uint32_t *tau;
     const uint64_t *seg_off;
     uint32_t *adm_list;         // [slots] (null = dense keys into out16)
+    uint32_t *adm_cnt;          // [nq]
 };
 
 // Admission bound of each query: the `want`-th smallest live key among its
@@ -215,26 +216,25 @@ struct AdmitParams {
     const uint32_t *s1;
     uint32_t nq, want;
     uint32_t *tau;
+    uint32_t *cnt;              // [nq] second-phase list lengths, zeroed here
 };
 
 // Pool of each query from the first phase's dense keys (at most tau) and the
-// second phase's per-chunk lists: the `want` smallest (key, slot) values as
+// second phase's list: the `want` smallest (key, slot) values as
 // key << 16 | slot, phase A (at least min(k, pool) of the members with the
 // smallest keys, at most ca when the keys allow) first.
 struct PoolListParams {
-    const uint16_t *keys;       // dense keys; second-phase chunk counts at chunk starts
+    const uint16_t *keys;       // first-phase dense keys
     const uint64_t *seg_off;
-    const uint32_t *s1, *m1, *np_q;
-    const uint32_t *cand_cnt;   // [nq] slots of each segment
-    const uint64_t *qp_slot;    // [nq][npr] absolute first slot of each probe
+    const uint32_t *s1;
     const uint32_t *tau;
-    const uint32_t *adm_list;
-    uint32_t nq, npr, echunk, want, ca, k;
+    const uint32_t *adm_list;   // the second phase's lists (from seg_off[q])
+    const uint32_t *adm_cnt;    // [nq] their lengths
+    uint32_t nq, want, ca, k;
     uint32_t *pool;             // [nq][want]
     uint32_t *pool_want;        // [nq]
     uint32_t *pool_na;          // [nq] phase-A members
     uint32_t *a_min;            // [nq] smallest key outside phase A (0xffff: none)
-    uint32_t *adm_cnt;          // [nq] second-phase keys listed (for the profile)
 };
 
 // Matrix-core scan work items: (posting, `mfma_chunk(ks)` entries, up to
@@ -242,6 +242,8 @@ struct PoolListParams {
 constexpr uint32_t kMixQ = 128;
 inline uint32_t mfma_tiles(uint32_t ks) { return ks <= 4 ? 4u : 2u; }   // 16-entry tiles per wave
 inline uint32_t mfma_chunk(uint32_t ks) { return 64u * mfma_tiles(ks); }
+// Entries per work item of the wave scan (k_scan_wave: one wave, 32 or 64 queries).
+constexpr uint32_t kWaveChunk = 256;
 
 // Exact rerank of each query's pool (selection masks over the candidate
 // keys from pool selection).  With prune, only the `ca` smallest keys and the members whose
@@ -489,6 +491,7 @@ hipError_t launch_entry_norms(uint32_t code_kind, const uint4 *data, uint32_t un
 
 hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t s);
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s);
+hipError_t launch_scan_wave(const MixedParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s);
 hipError_t launch_admit(const AdmitParams &p, hipStream_t s);
 hipError_t launch_pool_list(const PoolListParams &p, hipStream_t s);

@@ -106,25 +106,18 @@ hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t
 // so a lane's fragment is again one 16-byte unit: u8 codes shifted to signed
 // (c - 128, by flipping the top bit of every byte) against the symmetric int8
 // query; q.decode(c) = a * sum + b with the query's affine terms (a, b).
-//
-// ADM: second phase of a two-phase scan (admission; no tile, no dense keys).
-template <int KS, bool FILT, bool I8, bool ADM>
+template <int KS, bool FILT, bool I8, bool L2M>
 __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(const MixedParams p) {
     constexpr int T = KS <= 4 ? 4 : 2;   // tiles per wave, mfma_tiles()
     constexpr uint32_t EW = 64 * T;      // entries per item
-    __shared__ __align__(16) uint16_t s_tile[ADM ? 1 : 32][ADM ? 8 : EW];   // one query group's keys
+    __shared__ __align__(16) uint16_t s_tile[32][EW];   // one query group's keys
     __shared__ uint64_t s_slot[kMixQ];
     __shared__ uint32_t s_qi[kMixQ];
     __shared__ float s_qn[kMixQ];
     __shared__ float s_qa[I8 ? kMixQ : 1], s_qb[I8 ? kMixQ : 1];
     __shared__ uint64_t s_mask[kMixQ];
-    __shared__ uint32_t s_tau[ADM ? kMixQ : 1];
-    __shared__ uint64_t s_seg[ADM ? kMixQ : 1];
-    // admission: admitted keys of each row (pair) in the item's chunk, counted
-    __shared__ uint32_t s_rc[ADM ? kMixQ : 1];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 15, kg = lane >> 4;
-    const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
     const uint32_t lo = p.item_lo_dev ? *p.item_lo_dev : 0u;
     const uint32_t n_items = (p.n_items_dev ? *p.n_items_dev : p.n_items) - lo;
     for (XcdWalk it(n_items); it.more(); it.next()) {
@@ -144,14 +137,8 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
             const uint32_t q = ok ? p.pair_q[pr0 + tid] : 0u;
             s_qi[tid] = q;
             s_slot[tid] = ok ? p.pair_slot[pr0 + tid] : 0ull;
-            if constexpr (ADM) {   // bound + 1: rows past the group's queries admit nothing
-                s_tau[tid] = ok ? p.tau[q] + 1u : 0u;
-                s_seg[tid] = ok ? p.seg_off[q] : 0ull;
-                s_rc[tid] = 0u;
-            }
             float qn = ok ? (I8 ? p.qnorm[q] : p.qn16[q]) : 0.0f;
-            if (p.metric == M_DOT) qn = 1.0f;
-            else if (p.metric == M_COS) qn = qn > 0.0f ? 1.0f / sqrtf(qn) : 0.0f;
+            if (!L2M) qn = p.metric == M_DOT ? 1.0f : (qn > 0.0f ? 1.0f / sqrtf(qn) : 0.0f);
             s_qn[tid] = qn;
             if (I8) {
                 s_qa[tid] = ok ? p.q8a[q] : 0.0f;
@@ -191,7 +178,7 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
             const uint32_t e = e_begin + 16 * (wave + 4 * t) + col;
             live[t] = e < e_end && !((dw[t] >> (e & 31)) & 1u);   // (a column's products
             // depend on its own B values only, so the stand-in entry's are harmless)
-            rx[t] = p.metric == M_COS ? (xn[t] > 0.0f ? 1.0f / sqrtf(xn[t]) : 0.0f) : 1.0f;
+            rx[t] = !L2M && p.metric == M_COS ? (xn[t] > 0.0f ? 1.0f / sqrtf(xn[t]) : 0.0f) : 1.0f;
         }
         __syncthreads();
         for (uint32_t g0 = 0; g0 < gcnt; g0 += 32) {
@@ -248,34 +235,18 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
                     // past the group's queries are written too and never stored.
                     const uint32_t el = e0 + col - e_begin;
                     uint32_t kv[4];
-                    bool in[4];
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
-                        const float d = l2 ? xn[t] + qv[j] - 2.0f * acc[j] : 1.0f - acc[j] * qv[j] * rx[t];
+                        // (2 acc is exact: the fma rounds as (xn + qn) - (acc + acc) does)
+                        const float d = L2M ? __builtin_fmaf(-2.0f, acc[j], xn[t] + qv[j])
+                                            : 1.0f - acc[j] * qv[j] * rx[t];
                         const bool ok = live[t] && (!FILT || label_ok(mv[j], lab[t]));
                         kv[j] = ok ? fkey_m(d) >> 16 : 0xffffu;
-                        if constexpr (ADM) in[j] = kv[j] < s_tau[gb + 4 * kg + j];
-                        else s_tile[16 * h + 4 * kg + j][el] = (uint16_t)kv[j];
-                    }
-                    // admission: the few keys within their query's bound go to
-                    // the front of the pair's own slot range for this chunk
-                    // (its rank among them from an LDS counter), nothing else
-                    if constexpr (ADM) {
-                        if (__ballot(in[0] || in[1] || in[2] || in[3])) {
-#pragma unroll
-                            for (int j = 0; j < 4; j++)
-                                if (in[j]) {
-                                    const uint32_t g = gb + 4 * kg + j;
-                                    const uint64_t base = s_slot[g] + (e_begin - pb);
-                                    const uint32_t v = (kv[j] << 16) |
-                                                       ((uint32_t)(base - s_seg[g]) + (e0 + col - e_begin));
-                                    p.adm_list[base + atomicAdd(&s_rc[g], 1u)] = v;
-                                }
-                        }
+                        s_tile[16 * h + 4 * kg + j][el] = (uint16_t)kv[j];
                     }
                 }
             }
-            if constexpr (!ADM) {   // (with admission the keys are listed above; no tile)
+            {
                 const uint32_t len = e_end - e_begin, l16 = (len + 15) & ~15u;
                 // the tiles written above end at the 16-entry multiple past the
                 // posting's end; "no key" from there to the reserved end
@@ -296,9 +267,192 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
                 __syncthreads();   // the next group reuses the tile
             }
         }
-        if constexpr (ADM) {   // each pair's count for the chunk, at the chunk's first slot
-            __syncthreads();
-            if (tid < gcnt) p.out16[s_slot[tid] + (e_begin - pb)] = (uint16_t)s_rc[tid];
+    }
+}
+
+// ---------------------------------------------------------------- wave scan
+// Work item = (posting, echunk entries, up to 16 * NH queries), one wave per
+// item.  The wave keeps its queries' fragments (the B operand, 16 queries per
+// half) in registers for the whole item and streams the chunk's 16-entry code
+// tiles (the A operand) with the next tile's loads in flight, so the only
+// dependent load chain of an item is its set-up.  With the codes as A,
+// D[4 kg + j][col] is entry 4 kg + j of the tile against query col of the
+// half: a lane holds four consecutive keys of one query row, packs their top
+// halves into two words (v_perm) and stores them as one 8-byte word at the
+// row's slots, so keys never pass through LDS and the wave never waits for
+// another.  Keys are bit-identical to k_scan_mfma's: the same f32 expression
+// per (query, entry) (2 acc is exact, so fma(-2, acc, xn + qn) rounds as
+// (xn + qn) - (acc + acc) does).
+__device__ __forceinline__ uint32_t key_pair(float d0, float d1) {
+    // top halves of fkey(d0), fkey(d1) as d1 << 16 | d0: the raw top halves,
+    // xor'd with 0xffff (negative: every bit flips) or 0x8000 (sign set)
+    const uint32_t u0 = __float_as_uint(d0), u1 = __float_as_uint(d1);
+    const uint32_t hi = __builtin_amdgcn_perm(u1, u0, 0x07060302u);
+    const uint32_t sg = __builtin_amdgcn_perm(u1, u0, 0x0b0b0909u);   // sign bytes of u0, u1
+    return hi ^ (sg | 0x80008000u);
+}
+
+//
+// ADM: second phase of a two-phase scan.  A key at most its query's bound
+// tau[q] is appended to the query's list (adm_list from seg_off[q], as
+// key << 16 | slot within the segment; one returned atomic on adm_cnt[q] per
+// lane with admitted keys); nothing else is stored.  The list order depends
+// on timing; the pool taken from it does not.
+template <int KS, int NH, bool FILT, bool I8, bool L2M, bool ADM>
+__global__ __launch_bounds__(64) void k_scan_wave(const MixedParams p) {
+    const uint32_t lane = threadIdx.x, col = lane & 15, kg = lane >> 4;
+    const uint32_t lo = p.item_lo_dev ? *p.item_lo_dev : 0u;
+    const uint32_t n_items = (p.n_items_dev ? *p.n_items_dev : p.n_items) - lo;
+    for (XcdWalk it(n_items); it.more(); it.next()) {
+        const uint32_t item = lo + it.item;
+        const uint32_t vp = p.item_post[item];   // (virtual) posting of the pair table
+        const uint32_t pp = vp >= p.n_post ? vp - p.n_post : vp;
+        const uint32_t local = item - p.item_off[vp];
+        const uint32_t pb = p.post_off[pp], pe = p.post_off[pp + 1];
+        const uint32_t nec = (pe - pb + p.echunk - 1) / p.echunk;
+        const uint32_t qc = local / nec, ec = local - qc * nec;
+        const uint32_t pr0 = p.pair_off[vp] + qc * p.qg;
+        const uint32_t gcnt = min(p.qg, p.pair_off[vp + 1] - pr0);
+        const uint32_t e_begin = pb + ec * p.echunk, e_end = min(e_begin + p.echunk, pe);
+        // a probe's slot range is reserved up to the next multiple of
+        // kScanAlign past the posting's end (slots relative to pb)
+        const uint32_t lim = (pe - pb + kScanAlign - 1) / kScanAlign * kScanAlign;
+        // this lane's query (column col) of every half, its fragments and terms
+        uint4 bq[NH][KS];
+        uint64_t slot[NH], qm[NH];
+        float qv[NH], qa[NH], qb[NH];
+        uint32_t aq[NH], atau[NH], asl[NH];   // admission: query, bound, slot of the range in its segment
+#pragma unroll
+        for (int h = 0; h < NH; h++) {
+            const uint32_t g = 16 * h + col;
+            const bool ok = g < gcnt;
+            const uint32_t q = ok ? p.pair_q[pr0 + g] : 0u;
+            slot[h] = ok ? p.pair_slot[pr0 + g] : ~0ull;
+            if (ADM) {   // (rows past the group's queries admit nothing: bound 0 < every key's 0xffff)
+                aq[h] = q;
+                atau[h] = ok ? p.tau[q] : 0u;
+                asl[h] = ok ? (uint32_t)(slot[h] - p.seg_off[q]) : 0u;
+            }
+            const uint4 *qrow = p.q16 + (size_t)q * p.q16_units;
+#pragma unroll
+            for (int s = 0; s < KS; s++)
+                bq[h][s] = (ok && s < (int)p.ks) ? qrow[4 * s + kg] : make_uint4(0, 0, 0, 0);
+            float v = ok ? (I8 ? p.qnorm[q] : p.qn16[q]) : 0.0f;
+            if (!L2M) v = p.metric == M_DOT ? 1.0f : (v > 0.0f ? 1.0f / sqrtf(v) : 0.0f);
+            qv[h] = v;
+            qa[h] = I8 && ok ? p.q8a[q] : 0.0f;
+            qb[h] = I8 && ok ? p.q8b[q] : 0.0f;
+            qm[h] = FILT && ok ? p.qmask[q] : 0ull;
+        }
+        const uint32_t ntile = (e_end - e_begin + 15) / 16;
+        // tile t: entry e_begin + 16 t + col (past the chunk: entry e_begin,
+        // masked), its code units 4 s + kg, |x|^2, tombstone word and label
+        uint4 a[KS];
+        float xv;
+        uint32_t dw, lb;
+        auto load = [&](uint32_t t, uint4 (&fa)[KS], float &fx, uint32_t &fd, uint32_t &fl) {
+            const uint32_t e_raw = e_begin + 16 * t + col;
+            const uint32_t e = e_raw < e_end ? e_raw : e_begin;
+            const uint4 *src = p.codes16 + (size_t)(e >> 6) * p.units * 64 + (e & 63);
+#pragma unroll
+            for (int s = 0; s < KS; s++) {
+                const uint32_t u = 4 * s + kg;
+                uint4 v = u < p.units ? src[(size_t)u * 64] : make_uint4(0, 0, 0, 0);
+                if (I8 && u < p.units) {   // u8 c -> signed c - 128
+                    v.x ^= 0x80808080u;
+                    v.y ^= 0x80808080u;
+                    v.z ^= 0x80808080u;
+                    v.w ^= 0x80808080u;
+                }
+                fa[s] = v;
+            }
+            fx = p.xn16[e];
+            fd = p.del ? p.del[e >> 5] : 0u;
+            fl = FILT ? p.label_e[e] : 0u;
+        };
+        load(0, a, xv, dw, lb);
+        for (uint32_t t = 0; t < ntile; t++) {
+            // the next tile's loads in flight (the last tile reloads itself)
+            uint4 an[KS];
+            float xvn;
+            uint32_t dwn, lbn;
+            load(min(t + 1, ntile - 1), an, xvn, dwn, lbn);
+            const uint32_t e0 = e_begin + 16 * t;
+            const uint32_t e_raw = e0 + col;
+            const bool lv = e_raw < e_end && !((dw >> (e_raw & 31)) & 1u);
+            // rows 4 kg + j of the tile: liveness bits, |x|^2 (or 1/|x|), label
+            const uint32_t m4 = (uint32_t)(__ballot(lv) >> (4 * kg)) & 0xfu;
+            const float xr = L2M ? xv
+                             : p.metric == M_COS ? (xv > 0.0f ? 1.0f / sqrtf(xv) : 0.0f) : 1.0f;
+            float xj[4];
+            uint32_t lj[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                xj[j] = __shfl(xr, 4 * kg + j);
+                lj[j] = FILT ? (uint32_t)__shfl((int)lb, 4 * kg + j) : 0u;
+            }
+            const uint32_t rel = e0 - pb + 4 * kg;   // slot offset of the lane's four keys
+            const bool in = rel < lim;
+            uint32_t dm01 = ((m4 & 1u) ? 0u : 0xffffu) | ((m4 & 2u) ? 0u : 0xffff0000u);
+            uint32_t dm23 = ((m4 & 4u) ? 0u : 0xffffu) | ((m4 & 8u) ? 0u : 0xffff0000u);
+#pragma unroll
+            for (int h = 0; h < NH; h++) {
+                if (16 * h >= (int)gcnt) break;   // (uniform)
+                floatx4_t acc = {0, 0, 0, 0};
+                if (I8) {
+                    int32x4_t ai = {0, 0, 0, 0};
+#pragma unroll
+                    for (int s = 0; s < KS; s++)
+                        ai = __builtin_amdgcn_mfma_i32_16x16x64_i8(as_i32x4(a[s]), as_i32x4(bq[h][s]),
+                                                                  ai, 0, 0, 0);
+#pragma unroll
+                    for (int j = 0; j < 4; j++) acc[j] = qa[h] * (float)ai[j] + qb[h];   // q.x
+                } else {
+#pragma unroll
+                    for (int s = 0; s < KS; s++)
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(as_half8(a[s]), as_half8(bq[h][s]),
+                                                                     acc, 0, 0, 0);
+                }
+                float d[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    d[j] = L2M ? __builtin_fmaf(-2.0f, acc[j], xj[j] + qv[h])
+                               : 1.0f - acc[j] * qv[h] * xj[j];
+                uint32_t k01 = key_pair(d[0], d[1]), k23 = key_pair(d[2], d[3]);
+                if (FILT) {
+                    const uint32_t f0 = label_ok(qm[h], lj[0]) ? 0u : 0xffffu;
+                    const uint32_t f1 = label_ok(qm[h], lj[1]) ? 0u : 0xffff0000u;
+                    const uint32_t f2 = label_ok(qm[h], lj[2]) ? 0u : 0xffffu;
+                    const uint32_t f3 = label_ok(qm[h], lj[3]) ? 0u : 0xffff0000u;
+                    k01 |= dm01 | f0 | f1;
+                    k23 |= dm23 | f2 | f3;
+                } else {
+                    k01 |= dm01;
+                    k23 |= dm23;
+                }
+                if constexpr (ADM) {
+                    // (dead and past-the-range keys read 0xffff, above every bound)
+                    const uint32_t kk[4] = {k01 & 0xffffu, k01 >> 16, k23 & 0xffffu, k23 >> 16};
+                    const bool row = slot[h] != ~0ull;
+                    uint32_t n = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) n += row && kk[j] <= atau[h] ? 1u : 0u;
+                    if (n) {
+                        uint32_t pos = atomicAdd(p.adm_cnt + aq[h], n);
+                        uint32_t *dst = p.adm_list + (slot[h] - asl[h]);   // the segment's list
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            if (kk[j] <= atau[h]) dst[pos++] = (kk[j] << 16) | (asl[h] + rel + j);
+                    }
+                } else if (in && slot[h] != ~0ull) {
+                    *reinterpret_cast<uint2 *>(p.out16 + slot[h] + rel) = make_uint2(k01, k23);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < KS; s++) a[s] = an[s];
+            xv = xvn;
+            dw = dwn;
+            lb = lbn;
         }
     }
 }
@@ -1092,7 +1246,10 @@ __global__ __launch_bounds__(256) void k_admit(const AdmitParams p) {
                 if (k != 0xffffu) f(k);
             });
         });
-    if (lane == 0) p.tau[q] = tau;
+    if (lane == 0) {
+        p.tau[q] = tau;
+        p.cnt[q] = 0u;   // the second phase's list of the query starts empty
+    }
 }
 
 hipError_t launch_admit(const AdmitParams &p, hipStream_t s) {
@@ -1103,12 +1260,10 @@ hipError_t launch_admit(const AdmitParams &p, hipStream_t s) {
 }
 
 constexpr uint32_t kPoolLds = 8192;     // a query's pool candidates held in LDS (else re-read)
-constexpr uint32_t kPoolRanges = 512;   // second-phase chunk lists indexed in LDS
 
 struct PoolListLds {
     uint32_t v[kPoolLds];                       // candidates, key << 16 | slot
     uint32_t h[256];                            // histogram
-    uint32_t rs[kPoolRanges], rp[kPoolRanges];  // chunk list: start slot, prefix
     uint32_t w[8], r[8];
 };
 
@@ -1133,24 +1288,22 @@ __device__ __forceinline__ uint32_t block_bin(PoolListLds &L, uint32_t need, uin
 }
 
 // One workgroup per query.  Candidates: first-phase keys at most tau, then
-// every second-phase chunk's listed keys (the chunks, from the probes' slot
-// ranges, indexed by a prefix table and copied by flat index: independent
-// loads, no atomics).  A key histogram (bins of the keys' range, a finer
-// pass when it exceeds 256 keys) gives the pool's boundary key K (its
-// `want`-th smallest) and phase A's boundary key KA (its min(ca, pool)-th
-// smallest); pool members: keys below K, and of the keys equal to K those
-// with the smallest slots (a radix select over the slots, only when not all
-// of them are taken); phase A: members with keys below KA when there are k of
-// them, else up to KA.
+// the query's second-phase list (contiguous).  A key histogram (bins of the
+// keys' range, a finer pass when it exceeds 256 keys) gives the pool's
+// boundary key K (its `want`-th smallest) and phase A's boundary key KA (its
+// min(ca, pool)-th smallest); pool members: keys below K, and of the keys
+// equal to K those with the smallest slots (a radix select over the slots,
+// only when not all of them are taken); phase A: members with keys below KA
+// when there are k of them, else up to KA.
 __global__ __launch_bounds__(256) void k_pool_list(const PoolListParams p, uint32_t k) {
     __shared__ PoolListLds L;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     const uint32_t q = blockIdx.x;
     const uint64_t seg = p.seg_off[q];
     const uint4 *kp = reinterpret_cast<const uint4 *>(p.keys + seg);
-    const uint32_t nu = p.s1[q] / 8, tau = p.tau[q], m1 = p.m1[q], np = p.np_q[q];
-    const uint32_t send = p.cand_cnt[q];   // the segment's slots (the last probe's end)
-    const uint64_t *qp = p.qp_slot + (size_t)q * p.npr;
+    const uint32_t nu = p.s1[q] / 8, tau = p.tau[q];
+    const uint32_t *lst = p.adm_list + seg;
+    const uint32_t na2 = p.adm_cnt[q];
     uint32_t kmin = 0xffffu, kmax = 0u;
     // 1. first-phase candidates: thread per 8-key unit, positions by a scan
     uint32_t n1 = 0;
@@ -1181,87 +1334,24 @@ __global__ __launch_bounds__(256) void k_pool_list(const PoolListParams p, uint3
             }
         n1 += tot;
     }
-    // 2. second-phase chunk lists from the probes' slot ranges (each probe's
-    // range is its posting's length rounded up to the scan alignment, which
-    // divides the chunk size): thread per probe, (start slot, prefix)
-    uint32_t nr = 0, na2 = 0;
-    bool table = true;
-    for (uint32_t j0 = m1; j0 < np; j0 += 256) {
-        const uint32_t j = j0 + tid;
-        const uint32_t rel = j < np ? (uint32_t)(qp[j] - seg) : send;
-        const uint32_t end = j + 1 < np ? (uint32_t)(qp[j + 1] - seg) : send;
-        const uint32_t nch = j < np ? (end - rel + p.echunk - 1) / p.echunk : 0u;
-        uint32_t cnt[4], run = 0;
-#pragma unroll
-        for (uint32_t c = 0; c < 4; c++) {   // (most probes span few chunks: loads in flight)
-            cnt[c] = c < nch ? p.keys[seg + rel + c * p.echunk] : 0u;
-            run += cnt[c];
-        }
-        for (uint32_t c = 4; c < nch; c++) run += p.keys[seg + rel + c * p.echunk];
-        uint32_t tot_ch, tot_k;
-        const uint32_t ch0 = nr + block_scan_fast(nch, L.w, &tot_ch);
-        const uint32_t k0 = na2 + block_scan_fast(run, L.w, &tot_k);
-        if (nr + tot_ch <= kPoolRanges) {
-            uint32_t acc = k0;
-#pragma unroll
-            for (uint32_t c = 0; c < 4; c++)
-                if (c < nch) {
-                    L.rs[ch0 + c] = rel + c * p.echunk;
-                    L.rp[ch0 + c] = acc;
-                    acc += cnt[c];
-                }
-            for (uint32_t c = 4; c < nch; c++) {
-                L.rs[ch0 + c] = rel + c * p.echunk;
-                L.rp[ch0 + c] = acc;
-                acc += p.keys[seg + rel + c * p.echunk];
-            }
-        } else {
-            table = false;
-        }
-        nr += tot_ch;
-        na2 += tot_k;
-    }
-    __syncthreads();   // the table
-    // list entry i of the second phase (table present)
-    auto entry = [&](uint32_t i) {
-        uint32_t lo = 0, hi = nr;   // rp[lo] <= i < rp[lo + 1] (empty lists share a prefix)
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (L.rp[mid] <= i) lo = mid; else hi = mid;
-        }
-        return p.adm_list[seg + L.rs[lo] + (i - L.rp[lo])];
-    };
-    // every second-phase candidate once, f(v)
-    auto visit_list = [&](auto &&f) {
-        if (table) {
-            for (uint32_t i0 = 0; i0 < na2; i0 += 1024) {   // four loads in flight per thread
-                uint32_t v[4];
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const uint32_t i = i0 + 256 * c + tid;
-                    v[c] = i < na2 ? entry(i) : 0u;
-                }
-#pragma unroll
-                for (int c = 0; c < 4; c++)
-                    if (i0 + 256 * c + tid < na2) f(v[c]);
-            }
-        } else {   // more chunks than the table holds: probe by probe
-            for (uint32_t j = m1 + tid; j < np; j += 256) {
-                const uint32_t rel = (uint32_t)(qp[j] - seg);
-                const uint32_t end = j + 1 < np ? (uint32_t)(qp[j + 1] - seg) : send;
-                for (uint32_t c0 = rel; c0 < end; c0 += p.echunk) {
-                    const uint32_t cnt = p.keys[seg + c0];
-                    for (uint32_t i = 0; i < cnt; i++) f(p.adm_list[seg + c0 + i]);
-                }
-            }
-        }
-    };
+    // 2. the second-phase list after them (four loads in flight per thread)
     const uint32_t n = n1 + na2;
     const bool lds = n <= kPoolLds;
-    if (tid == 0) L.r[4] = n1;
-    __syncthreads();
-    visit_list([&](uint32_t v) {   // the list after the first-phase candidates; the key range
-        if (lds) L.v[atomicAdd(&L.r[4], 1u)] = v;
+    auto visit_list = [&](auto &&f) {
+        for (uint32_t i0 = 0; i0 < na2; i0 += 1024) {
+            uint32_t v[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const uint32_t i = i0 + 256 * c + tid;
+                v[c] = i < na2 ? lst[i] : 0u;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (i0 + 256 * c + tid < na2) f(i0 + 256 * c + tid, v[c]);
+        }
+    };
+    visit_list([&](uint32_t i, uint32_t v) {
+        if (lds) L.v[n1 + i] = v;
         kmin = min(kmin, v >> 16);
         kmax = max(kmax, v >> 16);
     });
@@ -1289,7 +1379,7 @@ __global__ __launch_bounds__(256) void k_pool_list(const PoolListParams p, uint3
                     if (kk <= tau) f((kk << 16) | (8 * u + t));
                 }
             }
-            visit_list(f);
+            visit_list([&](uint32_t, uint32_t v) { f(v); });
         }
     };
     // histogram of sel(v) (values whose bin is below 256)
@@ -1383,13 +1473,12 @@ __global__ __launch_bounds__(256) void k_pool_list(const PoolListParams p, uint3
         p.pool_want[q] = c;
         p.pool_na[q] = nA;
         p.a_min[q] = L.r[6];
-        p.adm_cnt[q] = na2;
     }
 }
 
 hipError_t launch_pool_list(const PoolListParams &p, hipStream_t s) {
     if (p.nq == 0) return hipSuccess;
-    if (p.ca == 0 || p.ca > p.want || p.echunk == 0 || p.k == 0 || p.k > p.ca)
+    if (p.ca == 0 || p.ca > p.want || p.k == 0 || p.k > p.ca)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_pool_list, dim3(p.nq), dim3(256), 0, s, p, p.k);
     return hipGetLastError();
@@ -2053,18 +2142,18 @@ __global__ __launch_bounds__(256) void k_rerank_f16(const RerankPrunedParams p) 
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) {
     if (grid == 0) return hipSuccess;
     const bool f = p.qmask != nullptr;   // label filter
+    const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
     // K steps: 32 dims (f16: dim <= 32, 64, 96, 128, 256) or 64 dims (int8: dim <= 64 ... 512)
     switch (p.ks) {
-#define SPF_SCAN_MFMA_A(KS_, A_)                                                                    \
-        if (p.int8) {                                                                               \
-            if (f) hipLaunchKernelGGL((k_scan_mfma<KS_, true, true, A_>), dim3(grid), dim3(256), 0, s, p); \
-            else hipLaunchKernelGGL((k_scan_mfma<KS_, false, true, A_>), dim3(grid), dim3(256), 0, s, p);  \
-        } else {                                                                                    \
-            if (f) hipLaunchKernelGGL((k_scan_mfma<KS_, true, false, A_>), dim3(grid), dim3(256), 0, s, p); \
-            else hipLaunchKernelGGL((k_scan_mfma<KS_, false, false, A_>), dim3(grid), dim3(256), 0, s, p);  \
-        }
+#define SPF_SCAN_MFMA_L(KS_, F_, I_)                                                                \
+        if (l2) hipLaunchKernelGGL((k_scan_mfma<KS_, F_, I_, true>), dim3(grid), dim3(256), 0, s, p); \
+        else hipLaunchKernelGGL((k_scan_mfma<KS_, F_, I_, false>), dim3(grid), dim3(256), 0, s, p);
 #define SPF_SCAN_MFMA(KS_)                                                                          \
-        if (p.adm_list) { SPF_SCAN_MFMA_A(KS_, true) } else { SPF_SCAN_MFMA_A(KS_, false) }        \
+        if (p.int8) {                                                                               \
+            if (f) { SPF_SCAN_MFMA_L(KS_, true, true) } else { SPF_SCAN_MFMA_L(KS_, false, true) }  \
+        } else {                                                                                    \
+            if (f) { SPF_SCAN_MFMA_L(KS_, true, false) } else { SPF_SCAN_MFMA_L(KS_, false, false) } \
+        }                                                                                           \
         break;
     case 1: SPF_SCAN_MFMA(1)
     case 2: SPF_SCAN_MFMA(2)
@@ -2072,9 +2161,46 @@ hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s) 
     case 4: SPF_SCAN_MFMA(4)
     case 5: case 6: case 7: case 8: SPF_SCAN_MFMA(8)
 #undef SPF_SCAN_MFMA
-#undef SPF_SCAN_MFMA_A
+#undef SPF_SCAN_MFMA_L
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// The wave scan: NH = qg / 16 query halves per wave (qg 32 or 64), one
+// 64-thread workgroup per wave.
+hipError_t launch_scan_wave(const MixedParams &p, uint32_t grid, hipStream_t s) {
+    if (grid == 0) return hipSuccess;
+    const bool f = p.qmask != nullptr;
+    const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
+    const bool adm = p.adm_list != nullptr;
+    if (p.qg != 32 && p.qg != 64) return hipErrorInvalidValue;
+#define SPF_WAVE_A(KS_, NH_, F_, I_, L_)                                                            \
+    if (adm) hipLaunchKernelGGL((k_scan_wave<KS_, NH_, F_, I_, L_, true>), dim3(grid), dim3(64), 0, s, p); \
+    else hipLaunchKernelGGL((k_scan_wave<KS_, NH_, F_, I_, L_, false>), dim3(grid), dim3(64), 0, s, p);
+#define SPF_WAVE_L(KS_, NH_, F_, I_)                                                                \
+    if (l2) { SPF_WAVE_A(KS_, NH_, F_, I_, true) } else { SPF_WAVE_A(KS_, NH_, F_, I_, false) }
+#define SPF_WAVE_F(KS_, NH_)                                                                        \
+    if (p.int8) {                                                                                   \
+        if (f) { SPF_WAVE_L(KS_, NH_, true, true) } else { SPF_WAVE_L(KS_, NH_, false, true) }     \
+    } else {                                                                                        \
+        if (f) { SPF_WAVE_L(KS_, NH_, true, false) } else { SPF_WAVE_L(KS_, NH_, false, false) }   \
+    }
+#define SPF_WAVE(KS_)                                                                               \
+    if (p.qg == 32) { SPF_WAVE_F(KS_, 2) } else { SPF_WAVE_F(KS_, 4) }                              \
+    break;
+    switch (p.ks) {
+    case 1: SPF_WAVE(1)
+    case 2: SPF_WAVE(2)
+    case 3: SPF_WAVE(3)
+    case 4: SPF_WAVE(4)
+    case 5: case 6: case 7: case 8: SPF_WAVE(8)
+    default: return hipErrorInvalidValue;
+    }
+#undef SPF_WAVE
+#undef SPF_WAVE_F
+#undef SPF_WAVE_L
+#undef SPF_WAVE_A
     return hipGetLastError();
 }
 
