@@ -240,6 +240,7 @@ def serve(args, queries, ix, np) -> int:
     import threading
 
     from pydiskann_amd import DeviceIndex, Server, PREC_EXACT
+    from pydiskann_amd._lib import FLAG_NO_GRAPH
     from pydiskann_amd.datasets import label_masks
 
     rng = np.random.default_rng(5)
@@ -251,7 +252,7 @@ def serve(args, queries, ix, np) -> int:
         r = np.random.default_rng(seed)
         out = []
         for _ in range(n):
-            m = int(r.integers(1, 257))
+            m = int(r.integers(1, args.req_rows + 1))
             rows = r.integers(0, nq_all, size=m)
             out.append((np.ascontiguousarray(queries[rows]), np.ascontiguousarray(masks_all[rows]),
                         int(r.choice([10, 100])), int(r.choice([16, 64]))))
@@ -275,7 +276,7 @@ def serve(args, queries, ix, np) -> int:
 
         wt = threading.Thread(target=waiter)
         wt.start()
-        mean_rows = 128.5
+        mean_rows = (args.req_rows + 1) / 2
         gaps = rng.exponential(mean_rows / rate_qps, size=len(reqs)) if rate_qps else None
         t0 = time.perf_counter()
         due = t0
@@ -285,7 +286,8 @@ def serve(args, queries, ix, np) -> int:
                 d = due - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-            tickets.put(srv.submit(q, k, npr, precision=PREC_EXACT, masks=m))
+            tickets.put(srv.submit(q, k, npr, precision=PREC_EXACT, masks=m,
+                                   flags=FLAG_NO_GRAPH if args.no_graph else 0))
         tickets.put(None)
         wt.join()
         wall = time.perf_counter() - t0
@@ -326,13 +328,47 @@ def serve(args, queries, ix, np) -> int:
         "dtype": "int8 scan (int32 sums), f32 rerank",
         "data": "synthetic (seeded vMF-like mixture, labels U{0..63}, masks of 1-8 labels)",
         "config": {"workload": f"c5: {ix.n_raw}x{ix.dim} dot int8, {ix.n_post} postings, "
-                               f"rerank 400, requests of U{{1..256}} queries, k in {{10,100}}, "
-                               f"n_probe in {{16,64}}, {args.requests} requests per point",
+                               f"rerank 400, requests of U{{1..{args.req_rows}}} queries, "
+                               f"k in {{10,100}}, n_probe in {{16,64}}, {args.requests} requests "
+                               f"per point",
                    "slots": args.slots, "max_batch": args.max_batch,
-                   "budget_gib": args.budget_gib},
+                   "budget_gib": args.budget_gib,
+                   "graph_replay": not args.no_graph},
+        "graph_replays": int(dev.info().graph_replays),
     }
     print(json.dumps(out), flush=True)
     return 0
+
+def sweep_point(args, dev, queries, q_d, ids_d, dd_d, cnt_d, stream, gt, ns, k, n_probe, pool, np):
+    """QPS (10 timed batches after 2 untimed) and recall@10 on the recall
+    sample at one (n_probe, pool) point of the headline precision."""
+    import torch
+    from pydiskann_amd import PREC_EXACT, PREC_MIXED
+    from pydiskann_amd._lib import FLAG_NO_GRAPH
+    prec = PREC_EXACT if args.precision == "exact" else PREC_MIXED
+    nq, dim = queries.shape
+
+    def step():
+        dev.search_device(q_d.data_ptr(), nq, dim, k, n_probe, ids_d.data_ptr(), dd_d.data_ptr(),
+                          cnt_d.data_ptr(), stream=stream.cuda_stream, precision=prec,
+                          rerank_size=pool if prec else 0,
+                          flags=FLAG_NO_GRAPH if args.no_graph else 0)
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 10
+    ids = ids_d.cpu().numpy().view(np.uint64)
+    rec = float(np.mean([len(set(ids[i, :10].tolist()) & set(gt[i].tolist())) / 10.0
+                         for i in range(ns)]))
+    log(f"[bench] sweep n_probe {n_probe} pool {pool}: {nq / dt:,.0f} QPS, recall@10 {rec:.4f}")
+    return {"n_probe": n_probe, "pool": pool if prec else None, "qps": round(nq / dt, 1),
+            "ms_per_batch": round(dt * 1e3, 3), "recall_at_10": round(rec, 4)}
+
 
 # MI355X peaks (dense): HBM3E bandwidth, f16/bf16 matrix cores
 MFMA_F16_TFLOPS = 2500.0
@@ -341,42 +377,51 @@ MFMA_F16_TFLOPS = 2500.0
 def roofline_of(args, head, dim, nq, k, ix):
     """Roofline of the headline run's posting scan (its dominant kernel).
 
-    `achieved` = the scan's algorithmic bytes under its posting-major work
-    items (the library's profile: every item reads its chunk of codes and its
-    queries' rows once and writes one key per candidate slot) over the scan's
-    HIP-event time on the call's stream; `bound` follows from the kernel's
-    arithmetic intensity against the ridge point; the matrix-core utilisation
-    of the scan and of the probe is reported beside it."""
+    `achieved` = the scan's minimal HBM bytes per launch (the library's
+    profile: each probed posting's entries -- codes, norms, tombstone bits,
+    labels -- read once, each query's row read once, one key per candidate
+    slot written) over the scan's HIP-event time on the call's stream; `bound`
+    follows from the kernel's arithmetic intensity against the ridge point.
+    Beside it: the bytes under the work items (every item rereads its
+    posting's entries and its queries' rows: the L2-side traffic), the
+    matrix-core utilisation of the scan and of the probe, and, from the
+    committed rocprofv3 counter summary, the DRAM bytes actually moved."""
     stage, prof = head["stage"], head["prof"]
     kern = STAGE_KERNELS[args.precision]
     if ix.quant == 3:
         kern = {"scan": "k_rq_scan_mfma" if args.precision == "mixed" else "k_rq_search_exact",
                 "select": "k_pool_select", "rerank": "k_rerank_f16"}
     ms = stage["scan"]
-    alg = prof["scan_bytes"]
+    alg = prof["scan_hbm_bytes"]
     ach = alg / (ms * 1e-3) / 1e9
     frac = ach / HBM_PEAK_GBS
     assert 0.0 < frac <= 1.0, f"roofline fraction {frac} outside (0, 1]"
+    item = prof["scan_bytes"]
     flops = prof["scan_flops"]
     ai = flops / alg if alg else 0.0
     # int8 matrix cores (5.0 POP/s dense) for int8 and 1-bit postings, else f16 (2.5 PFLOP/s)
     peak = 5000.0 if (ix.quant in (2, 3) and args.precision == "mixed") else MFMA_F16_TFLOPS
+    if args.precision == "exact":
+        peak = 78.6 if ix.quant != 3 else peak   # the f32 VALU scan
     ridge = peak * 1e12 / (HBM_PEAK_GBS * 1e9)
     wl = f"{args.workload}_{args.precision}_{ix.n_raw // 1000}kx{dim}"
     traffic, traffic_src = read_traffic(kern["scan"], wl)
-    out = {"bound": "hbm" if ai < ridge else "mfma", "achieved": round(ach, 1),
+    out = {"bound": "hbm" if ai < ridge else ("mfma" if args.precision == "mixed" else "valu"),
+           "achieved": round(ach, 1),
            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(frac, 4), "traffic": traffic,
            "kernel": kern["scan"], "kernel_ms": round(ms, 4),
            "algorithmic_bytes_per_launch": int(alg),
+           "item_bytes_per_launch": int(item),
+           "item_GBs": round(item / (ms * 1e-3) / 1e9, 1),
            "arithmetic_intensity_flop_per_byte": round(ai, 2),
            "ridge_flop_per_byte": round(ridge, 1),
            "candidates_per_launch": prof["candidates"], "traffic_source": traffic_src}
     if flops:
         tf = flops / (ms * 1e-3) / 1e12
-        out["scan_mfma"] = {"tflops": round(tf, 1), "peak_tflops": peak,
-                            "frac": round(tf / peak, 4), "flops": int(flops)}
-        if out["bound"] == "mfma":   # compute-bound: the roofline is the matrix-core one
-            assert 0.0 < tf / peak <= 1.0, f"matrix-core fraction {tf / peak} outside (0, 1]"
+        out["scan_compute"] = {"tflops": round(tf, 1), "peak_tflops": peak,
+                               "frac": round(tf / peak, 4), "flops": int(flops)}
+        if out["bound"] != "hbm":   # compute-bound: the roofline is the compute one
+            assert 0.0 < tf / peak <= 1.0, f"compute fraction {tf / peak} outside (0, 1]"
             out.update(achieved=round(tf, 1), peak=peak, unit="TFLOP/s", frac=round(tf / peak, 4),
                        hbm_GBs=round(ach, 1), hbm_frac=round(frac, 4))
     if prof["probe_flops"] and stage["probe"] > 0:
@@ -507,6 +552,13 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=3, help="c5: concurrent device batches")
     ap.add_argument("--max-batch", type=int, default=1024, help="c5: rows per device batch")
     ap.add_argument("--budget-gib", type=float, default=16.0, help="c5: working-memory budget")
+    ap.add_argument("--req-rows", type=int, default=256,
+                    help="c5: request sizes uniform in 1..this many query rows")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="never replay captured launch chains (SPF_FLAG_NO_GRAPH)")
+    ap.add_argument("--sweep", default="",
+                    help="batch workloads: comma-separated n_probe[:pool] points, each timed "
+                         "and scored for recall@10 after the headline run")
     args = ap.parse_args()
     if not args.rerank:
         args.rerank = {"c3": 400, "c5": 400, "c4": 300}.get(args.workload, 1000)
@@ -517,7 +569,7 @@ def main() -> int:
 
     import torch
     from pydiskann_amd import DeviceIndex, PREC_EXACT, PREC_MIXED
-    from pydiskann_amd._lib import last_profile, FLAG_ADMIT
+    from pydiskann_amd._lib import last_profile, FLAG_ADMIT, FLAG_NO_GRAPH
     from pydiskann_amd.index import brute_force_f64
 
     base, queries, ix, n_probe, k, np = build_workload(args, torch)
@@ -546,7 +598,8 @@ def main() -> int:
             dev.search_device(q_d.data_ptr(), nq, dim, k, n_probe, ids_d.data_ptr(),
                               dd_d.data_ptr(), cnt_d.data_ptr(), stream=stream.cuda_stream,
                               precision=prec, rerank_size=args.rerank if prec else 0,
-                              profile=profile, flags=FLAG_ADMIT if args.admit else 0)
+                              profile=profile, flags=(FLAG_ADMIT if args.admit else 0) |
+                              (FLAG_NO_GRAPH if args.no_graph else 0))
 
         for _ in range(args.warmup):
             step()
@@ -594,6 +647,11 @@ def main() -> int:
     if head["recall"] < RECALL_FLOOR:
         log(f"[bench] WARNING: recall@10 {head['recall']:.4f} below {RECALL_FLOOR}")
 
+    sweep = []
+    for pt in [v for v in args.sweep.split(",") if v]:
+        npv, _, pool = pt.partition(":")
+        sweep.append(sweep_point(args, dev, queries, q_d, ids_d, dd_d, cnt_d, stream, gt, ns, k,
+                                 int(npv), int(pool) if pool else args.rerank, np))
     roofline = roofline_of(args, head, dim, nq, k, ix)
     cpu = None if args.no_cpu else cpu_baseline(args, ix, queries, k, n_probe, np)
 
@@ -618,6 +676,8 @@ def main() -> int:
         "stage_ms": {s: round(v, 4) for s, v in head["stage"].items()},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if sweep:
+        out["sweep"] = sweep
     if other is not None:
         oname = "exact" if args.precision == "mixed" else "mixed"
         out[f"{oname}_gpu"] = {"value": round(other["qps"], 1), "unit": "queries/s",

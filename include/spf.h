@@ -171,6 +171,11 @@ typedef struct spf_profile {
     uint64_t scan_bytes;
     uint64_t scan_flops;            /* 2 * (padded dim) * candidates of the posting scan */
     uint64_t probe_flops;           /* 2 * (padded dim) * queries * centroids of a dense probe */
+    /* the posting scan's minimal HBM bytes: each probed posting's entries read
+     * once (a posting's work items all run on one XCD, whose L2 serves the
+     * rereads of its other items), each query's row read once, and the same
+     * key (or list) bytes written as scan_bytes */
+    uint64_t scan_hbm_bytes;
 } spf_profile;
 
 typedef struct spf_index_info {

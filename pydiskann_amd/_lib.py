@@ -84,7 +84,7 @@ class Profile(C.Structure):
                 ("scan_items", C.c_uint32), ("rerank_rows", C.c_uint32),
                 ("pool_members", C.c_uint32), ("admitted", C.c_uint32),
                 ("scan_bytes", C.c_uint64), ("scan_flops", C.c_uint64),
-                ("probe_flops", C.c_uint64)]
+                ("probe_flops", C.c_uint64), ("scan_hbm_bytes", C.c_uint64)]
 
 
 class IndexInfo(C.Structure):
@@ -206,7 +206,7 @@ def last_profile() -> dict:
             "rerank_rows": int(pr.rerank_rows), "pool_members": int(pr.pool_members),
             "admitted": int(pr.admitted),
             "scan_bytes": int(pr.scan_bytes), "scan_flops": int(pr.scan_flops),
-            "probe_flops": int(pr.probe_flops)}
+            "probe_flops": int(pr.probe_flops), "scan_hbm_bytes": int(pr.scan_hbm_bytes)}
 
 
 def _p(a):

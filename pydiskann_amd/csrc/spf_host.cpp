@@ -1239,10 +1239,12 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     mp.qmask = d_masks;
     mp.out16 = w->cand16.p;
     const uint32_t grid = xcd_grid(n_items, 16384);
-    HIPCHK(mark(SPF_STAGE_SCAN));
-    if (rq) {
+    if (rq) {   // (the per-pair query prep counts to the plan stage)
         rq->out16 = w->cand16.p;
         HIPCHK(launch_rq_prep(*rq, nq * npr, s));
+    }
+    HIPCHK(mark(SPF_STAGE_SCAN));
+    if (rq) {
         if (rq->levels) HIPCHK(launch_rq_scan_mfma(*rq, grid, s));
         else HIPCHK(launch_rq_scan(*rq, grid, s));
     } else if (admit) {
@@ -2200,7 +2202,12 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                 qrow = dpad * 4.0 + (q8 ? dpad : 0) + 8;
                 key = 8;
             }
-            double b = 0;
+            double b = 0, hb = 0;   // under the work items; minimal HBM
+            for (uint32_t p = 0; p < Pv; p++) {   // each probed posting's entries once
+                const uint64_t np = poff[p + 1] - poff[p], len = ix->h_post_len[p % P];
+                if (np && len) hb += (double)len * code;
+            }
+            hb += (double)nq * qrow;
             if (ix->rq_l2 && !mixed) {
                 // search_rabitq: each query's wave reads every probed entry's code
                 b = (double)real * code;
@@ -2221,12 +2228,15 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
                         adm += ac[i];
                     }
                     b += (double)d1 * key + (double)adm * 4 + 4.0 * nq;   // (+ the bounds)
+                    hb += (double)d1 * key + (double)adm * 4 + 4.0 * nq;
                     admitted = adm;
                 } else {
                     b += (double)total * key;
+                    hb += (double)total * key;
                 }
             }
             g_prof.scan_bytes = (uint64_t)b;
+            g_prof.scan_hbm_bytes = (uint64_t)hb;
             g_prof.admitted = (uint32_t)std::min<uint64_t>(admitted, 0xffffffffu);
             g_prof.scan_flops = ix->quant == SPF_QUANT_RABITQ
                                     ? (z.rq_mfma ? 2ull * ix->rq_padded * real : 0)

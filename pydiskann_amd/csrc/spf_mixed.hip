@@ -2096,9 +2096,12 @@ __global__ __launch_bounds__(256) void k_rerank_f16(const RerankPrunedParams p) 
     __syncthreads();
     if (cnt > 1) bitonic_lds(L.ex, m2);
     __syncthreads();
-    // 4. exact rerank in bound order until the next bound exceeds the k-th
+    // 4. exact rerank in bound order until the next bound exceeds the k-th; the
+    // first chunk holds k plus a margin (the bound is usually past the k-th
+    // exact distance by then), later chunks a full pass of the workgroup
     uint32_t done = 0;
     const uint32_t chunk = 64 * kPrH;
+    const uint32_t first = min(chunk, max(32u, (p.k + p.k / 2 + 31) / 32 * 32));
     while (done < cnt) {
         const uint32_t tc = S.tc;
         if (tc >= p.k) {
@@ -2106,7 +2109,7 @@ __global__ __launch_bounds__(256) void k_rerank_f16(const RerankPrunedParams p) 
             const float kth = key_float_m((uint32_t)(S.top[0][p.k - 1] >> 32));
             if (nb > kth) break;   // (uniform: every thread reads the same words)
         }
-        const uint32_t n1 = min(cnt - done, chunk);
+        const uint32_t n1 = min(cnt - done, done ? chunk : first);
         for (uint32_t i = tid; i < n1; i += 256) S.list[i] = L.ent[(uint32_t)L.ex[done + i]];
         __syncthreads();
         rerank_list(p, S.list, n1, s_qv, qn, S.chunk);

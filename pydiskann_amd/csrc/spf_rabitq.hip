@@ -1,10 +1,11 @@
 // RaBitQ postings on gfx950 (src/rabitq.rs, src/spfresh.rs:731-770).
 //
 //   k_rq_prep         query_with(q, centroid_b) for every (query, probed posting)
-//                     pair: residual, sequential norm, 3 x (sign flip + FHT) in
-//                     LDS with the reference's butterfly pairing (so every element
-//                     is the same IEEE sum), 4-bit quantisation, bit planes by
-//                     wave ballot, estimator constants
+//                     pair: residual, norm (sequential for the exact path),
+//                     3 x (sign flip + FHT) in registers and lane shuffles with
+//                     the reference's butterfly pairing (so every element is the
+//                     same IEEE sum), 4-bit quantisation, bit planes by wave
+//                     ballot, estimator constants
 //   k_rq_search_exact search_rabitq semantics: one wave per query walks its
 //                     probes and entries in scan order; estimates, bounds and
 //                     exact distances run in parallel per 64-entry chunk, heap
@@ -48,8 +49,19 @@ __device__ __forceinline__ void wave_fht(float *v, uint32_t n) {
 }
 
 // ---------------------------------------------------------------- query prep
+// One wave per (query, probed posting).  Lane l holds elements i = 64 m + l
+// (m < W) in registers.  The FHT's butterfly pairs and their IEEE operations
+// are the reference's (src/rabitq.rs:31-43): stage h pairs j and j + h with
+// v[j] = x + y, v[j + h] = x - y, stages in increasing h.  Stages h < 64 pair
+// lane l with lane l ^ h (one shuffle per register: the lane without bit h
+// adds, the other subtracts its own value from its partner's); stages
+// h >= 64 pair registers m and m + h / 64 of the same lane.  Which lane
+// computes a butterfly does not change its result, so the rotated residual
+// is the same as a sequential transform's, bit for bit.
+template <uint32_t W>
 __global__ __launch_bounds__(64) void k_rq_prep(const RqParams p) {
-    extern __shared__ float s_r[];   // padded floats
+    constexpr uint32_t P = 64 * W;
+    extern __shared__ float s_seq[];   // [P]: the exact path's sequential norm only
     __shared__ float s_norm;
     const uint32_t pair = blockIdx.x, lane = threadIdx.x;
     const uint32_t q = pair / p.npr, j = pair - q * p.npr;
@@ -57,39 +69,68 @@ __global__ __launch_bounds__(64) void k_rq_prep(const RqParams p) {
     const uint32_t post = p.probe_post[pair];
     const float *qv = p.qv + (size_t)q * p.dpad;
     const float *c = p.cent + (size_t)post * p.dpad;
-    const uint32_t P = p.padded;
     // mixed precision with cosine: the residual of the unit query
     const float qs = p.mixed && p.metric == M_COS
                          ? (p.qnorm[q] > 0.0f ? 1.0f / sqrtf(p.qnorm[q]) : 0.0f) : 1.0f;
-    for (uint32_t i = lane; i < P; i += 64) s_r[i] = i < p.dim ? qv[i] * qs - c[i] : 0.0f;
-    wave_sync_lds();
+    float r[W];
+#pragma unroll
+    for (uint32_t m = 0; m < W; m++) {
+        const uint32_t i = 64 * m + lane;
+        r[m] = i < p.dim ? qv[i] * qs - c[i] : 0.0f;
+    }
+    float norm;
     if (p.mixed) {   // a wave sum (the estimate only orders candidates)
         float acc = 0.0f;
-        for (uint32_t i = lane; i < P; i += 64) acc = __builtin_fmaf(s_r[i], s_r[i], acc);
+#pragma unroll
+        for (uint32_t m = 0; m < W; m++) acc = __builtin_fmaf(r[m], r[m], acc);
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-        if (lane == 0) s_norm = sqrtf(acc);
-    } else if (lane == 0) {   // r.iter().map(|x| x * x).sum::<f32>(): sequential, unfused
-        float acc = 0.0f;
-        for (uint32_t i = 0; i < P; i++) {
-            const float sq = s_r[i] * s_r[i];
-            acc = acc + sq;
-        }
-        s_norm = sqrtf(acc);
-    }
-    wave_sync_lds();
-    const float norm = s_norm;
-    if (norm > 0.0f)
-        for (uint32_t i = lane; i < P; i += 64) s_r[i] = s_r[i] / norm;
-    wave_sync_lds();
-    for (uint32_t rd = 0; rd < 3; rd++) {
-        for (uint32_t i = lane; i < P; i += 64) s_r[i] = s_r[i] * p.signs[rd * P + i];
+        norm = sqrtf(acc);   // (every lane holds the same sum)
+    } else {   // r.iter().map(|x| x * x).sum::<f32>(): sequential, unfused
+#pragma unroll
+        for (uint32_t m = 0; m < W; m++) s_seq[64 * m + lane] = r[m];
         wave_sync_lds();
-        wave_fht(s_r, P);
+        if (lane == 0) {
+            float acc = 0.0f;
+            for (uint32_t i = 0; i < P; i++) {
+                const float sq = s_seq[i] * s_seq[i];
+                acc = acc + sq;
+            }
+            s_norm = sqrtf(acc);
+        }
+        wave_sync_lds();
+        norm = s_norm;
+        wave_sync_lds();   // (the next pair rewrites s_seq / s_norm)
+    }
+    if (norm > 0.0f)
+#pragma unroll
+        for (uint32_t m = 0; m < W; m++) r[m] = r[m] / norm;
+    for (uint32_t rd = 0; rd < 3; rd++) {
+#pragma unroll
+        for (uint32_t m = 0; m < W; m++) r[m] = r[m] * p.signs[rd * P + 64 * m + lane];
+#pragma unroll
+        for (uint32_t h = 1; h < 64; h *= 2) {
+            const bool hi_half = (lane & h) != 0;
+#pragma unroll
+            for (uint32_t m = 0; m < W; m++) {
+                const float o = __shfl_xor(r[m], (int)h, 64);
+                r[m] = hi_half ? o - r[m] : r[m] + o;
+            }
+        }
+#pragma unroll
+        for (uint32_t g = 1; g < W; g *= 2)
+#pragma unroll
+            for (uint32_t m = 0; m < W; m++)
+                if ((m & g) == 0) {
+                    const float x = r[m], y = r[m + g];
+                    r[m] = x + y;
+                    r[m + g] = x - y;
+                }
     }
     float lo = 3.402823466e+38f, hi = -3.402823466e+38f;
-    for (uint32_t i = lane; i < P; i += 64) {
-        lo = fminf(lo, s_r[i]);
-        hi = fmaxf(hi, s_r[i]);
+#pragma unroll
+    for (uint32_t m = 0; m < W; m++) {
+        lo = fminf(lo, r[m]);
+        hi = fmaxf(hi, r[m]);
     }
     for (int off = 32; off > 0; off >>= 1) {
         lo = fminf(lo, __shfl_xor(lo, off, 64));
@@ -98,11 +139,11 @@ __global__ __launch_bounds__(64) void k_rq_prep(const RqParams p) {
     const float levels = 15.0f;
     const float delta = (hi - lo) / levels;
     const float inv = delta > 0.0f ? 1.0f / delta : 0.0f;
-    const uint32_t W = P / 64;
     uint64_t *planes = p.planes + (size_t)pair * 4 * p.words;
     uint32_t sum_u = 0;
+#pragma unroll
     for (uint32_t m = 0; m < W; m++) {   // element i = 64 m + lane -> bit `lane` of word m
-        const float t = roundf((s_r[64 * m + lane] - lo) * inv);
+        const float t = roundf((r[m] - lo) * inv);
         uint32_t u = t > 0.0f ? (t >= 4294967295.0f ? 0xffffffffu : (uint32_t)t) : 0u;
         u = u > 15u ? 15u : u;
         sum_u += u;
@@ -569,7 +610,10 @@ __global__ void k_rq_rows(const RqRowsParams p) {
 
 hipError_t launch_rq_prep(const RqParams &p, uint32_t n_pairs, hipStream_t s) {
     if (n_pairs == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rq_prep, dim3(n_pairs), dim3(64), p.padded * sizeof(float), s, p);
+    if (p.words * 64 != p.padded) return hipErrorInvalidValue;
+#define L(W) hipLaunchKernelGGL(k_rq_prep<W>, dim3(n_pairs), dim3(64), p.mixed ? 0 : 64 * W * sizeof(float), s, p)
+    SPF_RQ_DISPATCH(p.words, L)
+#undef L
     return hipGetLastError();
 }
 
