@@ -1461,7 +1461,7 @@ inline uint64_t sel_chunks(uint64_t total) {
 // rounded up to 32 slots (k_plan_q); the npr longest postings bound the sum.
 uint64_t cand_slots_bound(const spf_index *ix, const Shape &z, uint64_t nq) {
     const uint64_t npr = std::min<uint64_t>(z.npr, ix->P);
-    const uint64_t align = z.mfma_scan ? kScanAlign : 1u;
+    const uint64_t align = (z.mfma_scan || z.rq_mfma) ? kScanAlign : 1u;
     return nq * (ix->top_len[npr] + (align - 1) * npr + 31);
 }
 
@@ -1955,7 +1955,8 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
     pl.pair_src = z.rq_est ? w->pair_src.p : nullptr;
     pl.G = G;
     pl.echunk = echunk;
-    pl.align = z.mfma_scan ? kScanAlign : 1u;   // the matrix-core scan stores aligned rows
+    // the matrix-core scans store aligned rows
+    pl.align = (z.mfma_scan || z.rq_mfma) ? kScanAlign : 1u;
     if (prof) {   // per-query candidates without alignment slots, for the profile
         HIPCHK(w->cand_real.ensure(nq));
         pl.cand_real = w->cand_real.p;

@@ -450,11 +450,22 @@ __device__ __forceinline__ uint32_t nib_bytes(uint32_t n) {   // bit i of n (i <
     return (n * 0x00204081u) & 0x01010101u;
 }
 
-template <uint32_t W>
-__global__ __launch_bounds__(256) void k_rq_scan_mfma(const RqParams p) {
+// The item's code bits are unpacked once per 64-dim K step and used by every
+// 16-pair group of the item: the groups' accumulators (G4 x T) stay live
+// across the K loop, which streams one 16-byte code unit (two words) per tile
+// and the pairs' level rows per step, the next unit's loads in flight.  The
+// keys go through an LDS tile and leave as aligned 16-byte row segments (each
+// pair's slot range starts at a multiple of kScanAlign; the posting's last
+// item fills the range's tail with "no key").  MET: the metric family of the
+// key (0: L2, 1: cosine, 2: dot).
+template <uint32_t W, int MET>
+__global__ __launch_bounds__(256, 2) void k_rq_scan_mfma(const RqParams p) {
     constexpr uint32_t T = W <= 16 ? 4 : 2;   // rq_mfma_tiles
     constexpr uint32_t EW = 64 * T;
-    constexpr uint32_t NS = (W + 1) / 2;      // dwords of a lane's 16-bit slices
+    constexpr uint32_t NU = (W + 1) / 2;      // code units holding bit words
+    constexpr uint32_t G4 = kRqMfmaQ / 16;    // 16-pair groups per item
+    constexpr uint32_t RS = EW + 8;           // tile row stride (16-byte rows, 4-bank shift per row)
+    __shared__ __align__(16) uint16_t s_tile[kRqMfmaQ][RS];
     __shared__ float s_cs[kRqMfmaQ][5];
     __shared__ uint64_t s_slot[kRqMfmaQ];
     __shared__ uint64_t s_mask[kRqMfmaQ];
@@ -463,7 +474,6 @@ __global__ __launch_bounds__(256) void k_rq_scan_mfma(const RqParams p) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 15, kg = lane >> 4;
     const uint32_t P = p.padded;
-    const bool l2 = p.metric == M_L2 || p.metric == M_L2SQ;
     const uint32_t n_items = p.n_items_dev ? *p.n_items_dev : p.n_items;
     for (XcdWalk it(n_items); it.more(); it.next()) {
         const uint32_t item = it.item;
@@ -483,85 +493,146 @@ __global__ __launch_bounds__(256) void k_rq_scan_mfma(const RqParams p) {
             s_src[tid] = src;
             s_slot[tid] = ok ? p.pair_slot[pr0 + tid] : 0ull;
             s_mask[tid] = (ok && p.qmask) ? p.qmask[qi] : 0ull;
-            s_qn[tid] = (ok && p.metric == M_DOT) ? p.qnorm[qi] : 0.0f;
+            s_qn[tid] = (ok && MET == 2) ? p.qnorm[qi] : 0.0f;
         }
         for (uint32_t i = tid; i < kRqMfmaQ * 5; i += 256) {
             const uint32_t g = i / 5, c = i - g * 5;
             s_cs[g][c] = g < gcnt ? p.consts[(size_t)p.pair_src[pr0 + g] * 8 + c] : 0.0f;
         }
-        // the wave's tiles: each lane keeps its 16-bit slice of every word of
-        // its column's entry, plus the entry's popcount, |r|^2, factor,
-        // liveness, label and |x|^2 (entries past the posting read e_begin)
-        uint32_t sl[T][NS], cnt[T], lab[T];
+        // the wave's tiles: code unit pointers, |r|^2 and factor, liveness,
+        // label and |x|^2 of its column's entry (entries past the posting
+        // read e_begin)
+        const uint4 *csrc[T];
         float n2[T], fac[T], xn[T];
+        uint32_t lab[T];
         bool live[T];
 #pragma unroll
         for (uint32_t t = 0; t < T; t++) {
             const uint32_t e_raw = e_begin + 16 * (wave + 4 * t) + col;
             const uint32_t e = e_raw < e_end ? e_raw : e_begin;
-            uint64_t words[W];
-            rq_load_code<W>(p.codes, p.units, e, words, n2[t], fac[t]);
-            uint32_t c = 0;
-#pragma unroll
-            for (uint32_t i = 0; i < NS; i++) sl[t][i] = 0u;
-#pragma unroll
-            for (uint32_t s = 0; s < W; s++) {
-                c += __popcll(words[s]);
-                sl[t][s / 2] |= ((uint32_t)(words[s] >> (16 * kg)) & 0xffffu) << (16 * (s & 1));
+            csrc[t] = p.codes + (size_t)(e >> 6) * p.units * 64 + (e & 63);
+            if (W & 1) {   // padded = 64: one word, then n2 / fac in the same unit
+                const uint4 v = csrc[t][0];
+                n2[t] = __uint_as_float(v.z);
+                fac[t] = __uint_as_float(v.w);
+            } else {
+                const uint4 v = csrc[t][(size_t)(W / 2) * 64];
+                n2[t] = __uint_as_float(v.x);
+                fac[t] = __uint_as_float(v.y);
             }
-            cnt[t] = c;
             const bool dead = p.del && ((p.del[e >> 5] >> (e & 31)) & 1u);
             live[t] = e_raw < e_end && !dead;
             lab[t] = p.qmask ? p.label_e[e] : 0u;
-            xn[t] = l2 ? 0.0f : p.xn[e];
+            xn[t] = MET == 0 ? 0.0f : p.xn[e];
         }
         __syncthreads();
-        for (uint32_t gb = 0; gb < gcnt; gb += 16) {   // 16 pairs (A rows) at a time
-            // unpack the bits again for every group rather than keeping all
-            // tiles' unpacked fragments live (four times the registers)
+        const uint32_t ng = (gcnt + 15) / 16;   // groups in use (uniform)
+        // the level row of this lane's pair in every group (A: pair 16 g + col)
+        const uint4 *arow[G4];
 #pragma unroll
-            for (uint32_t t = 0; t < T; t++)
+        for (uint32_t g = 0; g < G4; g++) {
+            const uint32_t pr = 16 * g + col;
+            arow[g] = reinterpret_cast<const uint4 *>(p.levels + (size_t)s_src[pr < gcnt ? pr : 0] * P);
+        }
+        i32x4r_t acc[G4][T];
 #pragma unroll
-                for (uint32_t i = 0; i < NS; i++) asm volatile("" : "+v"(sl[t][i]));
-            const uint32_t g = gb + col;
-            const uint4 *arow = reinterpret_cast<const uint4 *>(p.levels + (size_t)s_src[g < gcnt ? g : gb] * P);
-            i32x4r_t acc[T];
+        for (uint32_t g = 0; g < G4; g++)
 #pragma unroll
-            for (uint32_t t = 0; t < T; t++) acc[t] = (i32x4r_t){0, 0, 0, 0};
+            for (uint32_t t = 0; t < T; t++) acc[g][t] = (i32x4r_t){0, 0, 0, 0};
+        uint32_t cnt[T] = {};
+        // unit u: words 2u, 2u + 1 of every tile's entry; levels of steps 2u, 2u + 1
+        uint4 cu[T], a[G4][2];
+        auto load_unit = [&](uint32_t u, uint4 (&c)[T], uint4 (&l)[G4][2]) {
 #pragma unroll
-            for (uint32_t s = 0; s < W; s++) {
-                union { uint4 u; i32x4r_t i; } a;
-                a.u = arow[4 * s + kg];   // levels 64 s + 16 kg .. + 15
+            for (uint32_t t = 0; t < T; t++) c[t] = csrc[t][(size_t)u * 64];
+#pragma unroll
+            for (uint32_t g = 0; g < G4; g++)
+#pragma unroll
+                for (uint32_t h = 0; h < 2; h++)
+                    if (g < ng && 2 * u + h < W) l[g][h] = arow[g][4 * (2 * u + h) + kg];
+        };
+        load_unit(0, cu, a);
+#pragma unroll 1
+        for (uint32_t u = 0; u < NU; u++) {
+            uint4 cn[T], an[G4][2];
+            if (u + 1 < NU) load_unit(u + 1, cn, an);
+#pragma unroll
+            for (uint32_t h = 0; h < 2; h++) {
+                if (2 * u + h >= W) break;
+                i32x4r_t bt[T];
 #pragma unroll
                 for (uint32_t t = 0; t < T; t++) {
-                    const uint32_t v = (sl[t][s / 2] >> (16 * (s & 1))) & 0xffffu;
-                    const i32x4r_t b = {(int)nib_bytes(v & 0xfu), (int)nib_bytes((v >> 4) & 0xfu),
-                                        (int)nib_bytes((v >> 8) & 0xfu), (int)nib_bytes(v >> 12)};
-                    acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a.i, b, acc[t], 0, 0, 0);
+                    const uint32_t lo = h ? cu[t].z : cu[t].x, hi = h ? cu[t].w : cu[t].y;
+                    cnt[t] += __popc(lo) + __popc(hi);
+                    // the lane's 16-bit slice of the word: bits 16 kg .. 16 kg + 15
+                    const uint32_t v = ((kg < 2 ? lo : hi) >> (16 * (kg & 1))) & 0xffffu;
+                    bt[t] = (i32x4r_t){(int)nib_bytes(v & 0xfu), (int)nib_bytes((v >> 4) & 0xfu),
+                                       (int)nib_bytes((v >> 8) & 0xfu), (int)nib_bytes(v >> 12)};
+                }
+#pragma unroll
+                for (uint32_t g = 0; g < G4; g++) {
+                    if (g >= ng) break;
+                    union { uint4 u; i32x4r_t i; } av;
+                    av.u = a[g][h];   // levels 64 s + 16 kg .. + 15
+#pragma unroll
+                    for (uint32_t t = 0; t < T; t++)
+                        acc[g][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av.i, bt[t], acc[g][t], 0, 0, 0);
                 }
             }
-            // D[4 kg + j][col]: pair gb + 4 kg + j against entry col of the tile
+            if (u + 1 < NU) {
 #pragma unroll
-            for (uint32_t t = 0; t < T; t++) {
-                const uint32_t e = e_begin + 16 * (wave + 4 * t) + col;
-                const float rx = xn[t] > 0.0f ? 0.5f / sqrtf(xn[t]) : 0.0f;
+                for (uint32_t t = 0; t < T; t++) cu[t] = cn[t];
 #pragma unroll
-                for (uint32_t j = 0; j < 4; j++) {
-                    const uint32_t g2 = gb + 4 * kg + j;
-                    if (e >= e_end || g2 >= gcnt) continue;
-                    const float *cs = s_cs[g2];
+                for (uint32_t g = 0; g < G4; g++) {
+                    a[g][0] = an[g][0];
+                    a[g][1] = an[g][1];
+                }
+            }
+        }
+        // D[4 kg + j][col]: pair 16 g + 4 kg + j against entry col of tile t;
+        // each pair's constants are read once for the lane's tiles
+        float rx[T];
+#pragma unroll
+        for (uint32_t t = 0; t < T; t++)
+            rx[t] = MET == 1 ? (xn[t] > 0.0f ? 0.5f / sqrtf(xn[t]) : 0.0f) : 0.0f;
+#pragma unroll
+        for (uint32_t g = 0; g < G4; g++) {
+            if (g >= ng) break;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) {
+                const uint32_t g2 = 16 * g + 4 * kg + j;
+                const float c0 = s_cs[g2][0], c1 = s_cs[g2][1], c2 = s_cs[g2][2], c3 = s_cs[g2][3],
+                            c4 = s_cs[g2][4];
+                const float qn2 = MET == 2 ? s_qn[g2] : 0.0f;
+                const uint64_t msk = p.qmask ? s_mask[g2] : 0ull;
+#pragma unroll
+                for (uint32_t t = 0; t < T; t++) {
+                    const uint32_t el = 16 * (wave + 4 * t) + col;   // entry within the item
                     // distance_with_bound's estimate (src/rabitq.rs:202-215), as in rq_est
-                    float inner = cs[2] * (float)acc[t][j] + cs[3] * (float)cnt[t];
-                    inner = inner + cs[4];
-                    const float sf = cs[1] * fac[t];
-                    const float est = fmaxf((n2[t] + cs[0]) - sf * inner, 0.0f);
+                    float inner = c2 * (float)acc[g][t][j] + c3 * (float)cnt[t];
+                    inner = inner + c4;
+                    const float sf = c1 * fac[t];
+                    const float est = fmaxf((n2[t] + c0) - sf * inner, 0.0f);
                     float d = est;
-                    if (p.metric == M_COS) d = 1.0f - (1.0f + xn[t] - est) * rx;
-                    else if (p.metric == M_DOT) d = 1.0f - 0.5f * (s_qn[g2] + xn[t] - est);
-                    const bool ok = live[t] && (!p.qmask || label_ok(s_mask[g2], lab[t]));
-                    p.out16[s_slot[g2] + (e - pb)] = ok ? (uint16_t)(fkey_r(d) >> 16) : (uint16_t)0xffff;
+                    if (MET == 1) d = 1.0f - (1.0f + xn[t] - est) * rx[t];
+                    else if (MET == 2) d = 1.0f - 0.5f * (qn2 + xn[t] - est);
+                    // ("no key" past the posting: the range's tail; rows past
+                    // the item's pairs are written and never stored)
+                    const bool ok = e_begin + el < e_end && live[t] && (!p.qmask || label_ok(msk, lab[t]));
+                    s_tile[g2][el] = ok ? (uint16_t)(fkey_r(d) >> 16) : (uint16_t)0xffff;
                 }
             }
+        }
+        __syncthreads();
+        // the pairs' rows as aligned 16-byte stores, up to the reserved end of
+        // the range on the posting's last item
+        const uint32_t len = e_end - e_begin;
+        const uint32_t lp = e_end == pe ? (len + kScanAlign - 1) / kScanAlign * kScanAlign : len;
+        const uint32_t cpr = lp / 8;
+        for (uint32_t c = tid; c < gcnt * cpr; c += 256) {
+            const uint32_t r = c / cpr, ch = c - r * cpr;
+            const uint4 v = *reinterpret_cast<const uint4 *>(&s_tile[r][8 * ch]);
+            *reinterpret_cast<uint4 *>(p.out16 + s_slot[r] + (e_begin - pb) + 8 * ch) = v;
         }
     }
 }
@@ -635,7 +706,11 @@ hipError_t launch_rq_decode(const RqDecodeParams &p, uint32_t n, hipStream_t s) 
 
 hipError_t launch_rq_scan_mfma(const RqParams &p, uint32_t grid, hipStream_t s) {
     if (grid == 0) return hipSuccess;
-#define L(W) hipLaunchKernelGGL(k_rq_scan_mfma<W>, dim3(grid), dim3(256), 0, s, p)
+    const int met = (p.metric == M_L2 || p.metric == M_L2SQ) ? 0 : p.metric == M_COS ? 1 : 2;
+#define L(W)                                                                                       \
+    if (met == 0) hipLaunchKernelGGL((k_rq_scan_mfma<W, 0>), dim3(grid), dim3(256), 0, s, p);      \
+    else if (met == 1) hipLaunchKernelGGL((k_rq_scan_mfma<W, 1>), dim3(grid), dim3(256), 0, s, p); \
+    else hipLaunchKernelGGL((k_rq_scan_mfma<W, 2>), dim3(grid), dim3(256), 0, s, p)
     SPF_RQ_DISPATCH(p.words, L)
 #undef L
     return hipGetLastError();
