@@ -127,7 +127,8 @@ struct DBuf {
     X(uint64_t, sel_tmp) X(uint64_t, probe_tmp) X(uint32_t, cand_real) X(uint32_t, blk_cnt) \
     X(uint32_t, a_min) X(uint8_t, q8m) X(float, q8a) X(float, q8b) X(uint8_t, rq_levels)   \
     X(uint32_t, m1) X(uint32_t, s1) X(uint32_t, tau) X(uint32_t, adm_cnt) X(uint32_t, adm_list) \
-    X(uint32_t, pool_list) X(uint32_t, pool_na)
+    X(uint32_t, pool_list) X(uint32_t, pool_na) X(uint16_t, lkey) X(uint32_t, lslot)         \
+    X(uint32_t, lcnt) X(uint32_t, lsel) X(uint32_t, la)
 
 // A captured launch chain of one search call, replayed for later calls with
 // the same key (the same arguments, buffers and launch shapes).
@@ -1180,6 +1181,15 @@ int update_impl(spf_index *ix, const spf_index_update_desc *u) {
 // Matrix-core (or RaBitQ estimator) scan -> 16-bit candidate keys -> per
 // query pool select -> bound-pruned exact rerank -> ids.  Runs after probe
 // and planning.
+// chunks of the long-segment select: a segment above kSelChunk candidates
+// takes ceil(n / kSelChunk) <= 2n / kSelChunk chunks
+inline uint64_t sel_chunks(uint64_t total) {
+    return total > kSelChunk ? 2 * total / kSelChunk + 1 : 0;
+}
+
+// Chunked pool select of segments above kSelChunk slots: list slots per chunk.
+inline uint32_t pool_stride(uint32_t want) { return (want + 31) / 32 * 32; }
+
 template <class Mark>
 int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_t k,
                  uint32_t npr, uint32_t want, uint64_t total, uint32_t n_items, uint64_t *d_ids,
@@ -1314,7 +1324,46 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     pp.a_bits = w->a_bits.p;
     pp.a_min = w->a_min.p;
     pp.ca = ca;
+    // segments above both kSelChunk slots and half the batch's slots per
+    // concurrent pool-select workgroup (four per CU) are selected by many
+    // workgroups: they would otherwise finish long after the rest of the batch
+    // (a lower bar adds more merge work than it removes from the tail)
+    const uint64_t seg_bound = ix->top_len[std::min(z.npr, ix->P)] + (kScanAlign - 1) * (uint64_t)z.npr + 31;
+    const uint64_t long_from = std::max<uint64_t>(kSelChunk, total / 2048);
+    const uint64_t mc = seg_bound > long_from && long_from < 0xffffffffull ? sel_chunks(total) : 0;
+    pp.long_from = mc ? (uint32_t)long_from : 0u;
     HIPCHK(launch_pool_select(pp, nq, s));
+    if (mc) {
+        const uint32_t st = pool_stride(want);
+        HIPCHK(w->sel_ccnt.ensure(nq + 1));
+        HIPCHK(w->sel_coff.ensure(nq + 1));
+        HIPCHK(w->lkey.ensure(mc * st));
+        HIPCHK(w->lslot.ensure(mc * st));
+        HIPCHK(w->lcnt.ensure(mc));
+        HIPCHK(w->lsel.ensure(mc * st / 32));
+        HIPCHK(w->la.ensure(mc * st / 32));
+        PoolLongParams pg{};
+        pg.keys = w->cand16.p;
+        pg.seg_off = w->cand_off.p;
+        pg.seg_cnt = w->cand_cnt.p;
+        pg.nq = nq;
+        pg.stride = st;
+        pg.want = want;
+        pg.ca = ca;
+        pg.long_from = (uint32_t)long_from;
+        pg.chunk_cnt = w->sel_ccnt.p;
+        pg.chunk_off = w->sel_coff.p;
+        pg.lkey = w->lkey.p;
+        pg.lslot = w->lslot.p;
+        pg.lcnt = w->lcnt.p;
+        pg.lsel = w->lsel.p;
+        pg.la = w->la.p;
+        pg.sel_bits = w->sel_bits.p;
+        pg.a_bits = w->a_bits.p;
+        pg.pool_want = w->pool_want.p;
+        pg.a_min = w->a_min.p;
+        HIPCHK(launch_pool_long(pg, mc, s));
+    }
     }
     // the (pruned) exact rerank of each pool
     HIPCHK(mark(SPF_STAGE_RERANK));
@@ -1450,11 +1499,6 @@ bool graph_shape(const spf_index *ix, uint32_t n, GraphParams *gp);
 uint64_t graph_scratch_words(const spf_index *ix, const GraphParams &g);
 extern const uint64_t kGraphScratchBytes;
 
-// chunks of the long-segment select: a segment above kSelChunk candidates
-// takes ceil(n / kSelChunk) <= 2n / kSelChunk chunks
-inline uint64_t sel_chunks(uint64_t total) {
-    return total > kSelChunk ? 2 * total / kSelChunk + 1 : 0;
-}
 
 // Candidate slots of a batch, bounded from the index alone: a query's segment
 // holds its kept probes' postings, each reserved up to the scan's alignment,
@@ -1485,7 +1529,8 @@ uint64_t cand_buffer_bytes(const spf_index *ix, const Shape &z, uint64_t slots, 
     if (z.admit)   // dense keys, admission lists
         b += std::max<uint64_t>(slots, 1) * (2 + 4);
     else if (z.mixed)
-        b += std::max<uint64_t>(slots, 1) * 2 + 2 * std::max<uint64_t>(slots / 32, 1) * 4;
+        b += std::max<uint64_t>(slots, 1) * 2 + 2 * std::max<uint64_t>(slots / 32, 1) * 4 +
+             sel_chunks(slots) * (pool_stride(z.want) * (2 + 4 + 1) + 4);   // chunked pools
     else if (!ix->rq_l2)
         b += std::max<uint64_t>(slots, 1) * 8 + sel_chunks(slots) * z.want * 8;
     return b;
@@ -1603,8 +1648,16 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
             } else {
                 add(std::max<uint64_t>(total / 32, 1), 4);   // sel_bits
                 add(std::max<uint64_t>(total / 32, 1), 4);   // a_bits
+                if (const uint64_t mc = sel_chunks(total)) {   // chunked pools
+                    const uint64_t st = pool_stride(z.want);
+                    add(mc * st, 2);                         // lkey
+                    add(mc * st, 4);                         // lslot
+                    add(mc, 4);                              // lcnt
+                    add(mc * st / 32, 4 * 2);                // lsel, la
+                }
             }
         }
+        add(q + 1, 4 * 2);                          // sel_ccnt, sel_coff (chunk table)
         add(q, 4);                                  // pool_want
         add(q, 4);                                  // a_min
         add(q * k, 8);                              // fin
@@ -2004,6 +2057,14 @@ int search_body(spf_index *ix, Workspace *w, hipStream_t s, const float *d_q, ui
             } else {
                 grow(w->sel_bits, std::max<uint64_t>(total / 32, 1), 4);
                 grow(w->a_bits, std::max<uint64_t>(total / 32, 1), 4);
+                if (const uint64_t mc = sel_chunks(total)) {
+                    const uint64_t st = pool_stride(want);
+                    grow(w->lkey, mc * st, 2);
+                    grow(w->lslot, mc * st, 4);
+                    grow(w->lcnt, mc, 4);
+                    grow(w->lsel, mc * st / 32, 4);
+                    grow(w->la, mc * st / 32, 4);
+                }
             }
         } else if (!ix->rq_l2) {
             grow(w->cand, std::max<uint64_t>(total, 1), 8);

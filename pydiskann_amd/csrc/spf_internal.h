@@ -375,6 +375,29 @@ struct PoolParams {
     uint32_t ca;
     uint32_t *a_min;            // with a_bits, optional [nq]: a lower bound of the keys of
                                 // pool members outside phase A (0xffff: there are none)
+    uint32_t long_from;         // with sel_bits: segments above this many slots are left to
+                                // the chunked select (launch_pool_long); 0 = none
+};
+
+// Candidate pools of segments above kSelChunk slots (launch_pool_long): each
+// chunk of kSelChunk slots contributes its `stride` smallest (key, slot)
+// values in slot order to a list (keys padded with "no key" to the stride);
+// the pool and phase A are selected from a query's lists, concatenated in
+// chunk order (= slot order), and their bits set in the segment's masks.
+struct PoolLongParams {
+    const uint16_t *keys;
+    const uint64_t *seg_off;    // [nq] 32-aligned segments
+    const uint32_t *seg_cnt;    // [nq]
+    uint32_t nq, stride;        // stride: want rounded up to 32
+    uint32_t want, ca;
+    uint32_t long_from;         // segments above this many slots (>= kSelChunk) are chunked
+    uint32_t *chunk_cnt;        // [nq] chunks of each query (0: selected whole)
+    uint32_t *chunk_off;        // [nq + 1]
+    uint16_t *lkey;             // [chunks][stride]
+    uint32_t *lslot;            // [chunks][stride] slot within the segment
+    uint32_t *lcnt;             // [chunks]
+    uint32_t *lsel, *la;        // [chunks * stride / 32] masks over the lists
+    uint32_t *sel_bits, *a_bits, *pool_want, *a_min;
 };
 
 // Exact f32 rerank of each query's pool and the final top-k, one workgroup per
@@ -493,6 +516,7 @@ hipError_t launch_selftest_mfma(float *d_out_f16, float *d_out_bf16, hipStream_t
 hipError_t launch_scan_mfma(const MixedParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_scan_wave(const MixedParams &p, uint32_t grid, hipStream_t s);
 hipError_t launch_pool_select(const PoolParams &p, uint32_t nq, hipStream_t s);
+hipError_t launch_pool_long(const PoolLongParams &p, uint64_t max_chunks, hipStream_t s);
 hipError_t launch_admit(const AdmitParams &p, hipStream_t s);
 hipError_t launch_pool_list(const PoolListParams &p, hipStream_t s);
 hipError_t launch_rerank_pool(const RerankPoolParams &p, hipStream_t s);

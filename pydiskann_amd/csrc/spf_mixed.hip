@@ -1082,6 +1082,7 @@ __global__ __launch_bounds__(256, 4) void k_pool_select(const PoolParams p, uint
     if (q >= nq) return;   // whole group: its barriers involve only its own threads
     const uint64_t b = p.seg_off ? p.seg_off[q] : (uint64_t)q * p.seg_stride;   // multiple of 32
     const uint32_t n = p.seg_cnt ? p.seg_cnt[q] : p.seg_len;
+    if (BITS && p.long_from && n > p.long_from) return;   // (the chunked select's)
     ProbeTab tab;
     tab.np = (!BITS && p.map) ? p.np_q[q] : 0u;
     tab.qp_row = p.qp_slot + (size_t)q * p.npr;
@@ -1105,6 +1106,87 @@ __global__ __launch_bounds__(256, 4) void k_pool_select(const PoolParams p, uint
                              BITS ? p.sel_bits + b / 32 : nullptr,
                              BITS && p.a_bits ? p.a_bits + b / 32 : nullptr, p.ca,
                              BITS ? p.a_min : nullptr);
+}
+
+// ---------------------------------------------------------------- long candidate lists
+// Segments far above the batch's average work per pool-select workgroup (the
+// Zipf-sized postings of a skewed index give some queries over a million
+// candidates, which one workgroup would select long after the others are
+// done) are selected by many workgroups, one per kSelChunk slots.  A chunk's `stride` smallest (key, slot) values
+// contain every pool member in the chunk (the pool is the `want` <= stride
+// smallest of the segment), and the chunk lists of a query, concatenated in
+// chunk order, are in slot order, so the pool and phase A taken from them
+// with the same rule (keys, ties in slot order) are the segment's.
+__global__ void k_pool_chunk_count(const PoolLongParams p) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= p.nq) return;
+    const uint32_t n = p.seg_cnt[q];
+    p.chunk_cnt[q] = n > p.long_from ? (n + kSelChunk - 1) / kSelChunk : 0u;
+}
+
+__global__ __launch_bounds__(256, 4) void k_pool_chunk(const PoolLongParams p) {
+    __shared__ uint32_t hist[1152];
+    __shared__ uint32_t sx[20];
+    const uint32_t c = blockIdx.x, tid = threadIdx.x;
+    if (c >= p.chunk_off[p.nq]) return;   // (the grid is a bound)
+    uint32_t lo = 0, hi = p.nq;            // the query whose chunk range holds c
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (p.chunk_off[mid] <= c) lo = mid; else hi = mid;
+    }
+    const uint32_t q = lo, j = c - p.chunk_off[q];
+    const uint64_t b = p.seg_off[q] + (uint64_t)j * kSelChunk;   // a multiple of 32
+    const uint32_t m = min(kSelChunk, p.seg_cnt[q] - j * kSelChunk);
+    // the chunk's words of the segment's masks start empty (the merge sets bits)
+    for (uint32_t w = tid; w < (m + 31) / 32; w += 256) {
+        p.sel_bits[b / 32 + w] = 0u;
+        p.a_bits[b / 32 + w] = 0u;
+    }
+    ProbeTab tab;
+    tab.np = 0;
+    tab.pbase = tab.ebase = nullptr;
+    PoolGroup<4> g;
+    g.sx = sx;
+    pool_group<false, 4, 3>(p.keys + b, c, m, p.stride, p.lslot, p.lkey, p.lcnt, false, tab, hist, g,
+                            nullptr, nullptr, 0, nullptr);
+    __syncthreads();
+    const uint32_t cnt = p.lcnt[c];
+    for (uint32_t i = tid; i < p.stride; i += 256) {
+        const size_t o = (size_t)c * p.stride + i;
+        if (i < cnt) p.lslot[o] += j * kSelChunk;   // chunk slot -> segment slot
+        else p.lkey[o] = 0xffffu;                   // "no key" to the stride
+    }
+}
+
+__global__ __launch_bounds__(256, 4) void k_pool_merge(const PoolLongParams p) {
+    __shared__ uint32_t hist[1152];
+    __shared__ uint32_t sx[20];
+    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    if (q >= p.nq || p.chunk_cnt[q] == 0) return;   // whole workgroup
+    const uint32_t c0 = p.chunk_off[q];
+    const size_t base = (size_t)c0 * p.stride;   // a multiple of 32
+    const uint32_t n = p.chunk_cnt[q] * p.stride;
+    ProbeTab tab;
+    tab.np = 0;
+    tab.pbase = tab.ebase = nullptr;
+    PoolGroup<4> g;
+    g.sx = sx;
+    pool_group<true, 4, 3>(p.lkey + base, q, n, p.want, nullptr, nullptr, p.pool_want, false, tab,
+                           hist, g, p.lsel + base / 32, p.la + base / 32, p.ca, p.a_min);
+    __syncthreads();
+    // the members' bits at their segment slots
+    uint32_t *sel = p.sel_bits + p.seg_off[q] / 32, *a = p.a_bits + p.seg_off[q] / 32;
+    for (uint32_t w = tid; w < n / 32; w += 256) {
+        uint32_t ms = p.lsel[base / 32 + w];
+        const uint32_t ma = p.la[base / 32 + w];
+        while (ms) {
+            const uint32_t i = __ffs(ms) - 1;
+            ms &= ms - 1;
+            const uint32_t slot = p.lslot[base + 32 * w + i];
+            atomicOr(&sel[slot >> 5], 1u << (slot & 31));
+            if ((ma >> i) & 1u) atomicOr(&a[slot >> 5], 1u << (slot & 31));
+        }
+    }
 }
 
 // ---------------------------------------------------------------- admission
@@ -2204,6 +2286,21 @@ hipError_t launch_scan_wave(const MixedParams &p, uint32_t grid, hipStream_t s) 
 #undef SPF_WAVE_F
 #undef SPF_WAVE_L
 #undef SPF_WAVE_A
+    return hipGetLastError();
+}
+
+hipError_t launch_pool_long(const PoolLongParams &p, uint64_t max_chunks, hipStream_t s) {
+    if (p.nq == 0 || max_chunks == 0) return hipSuccess;
+    if (p.stride % 32 || p.stride < p.want || p.stride > kSelChunk || p.ca > p.want ||
+        p.long_from < kSelChunk)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pool_chunk_count, dim3((p.nq + 255) / 256), dim3(256), 0, s, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = launch_excl_scan_u32(p.chunk_cnt, p.chunk_off, p.nq, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pool_chunk, dim3((uint32_t)max_chunks), dim3(256), 0, s, p);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pool_merge, dim3(p.nq), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
