@@ -121,6 +121,11 @@ __global__ __launch_bounds__(64) void k_graph_probe(const GraphParams p) {
     extern __shared__ __align__(16) uint32_t s_mem[];
     __shared__ uint32_t s_go, s_cur, s_wn, s_fn, s_rn, s_abort;
     const uint32_t q = p.q0 + blockIdx.x, lane = threadIdx.x;
+    if (q >= p.nq) return;
+    if (p.only && !p.only[q]) {   // walked by an earlier launch
+        if (lane == 0 && p.aborted) p.aborted[q] = 0u;
+        return;
+    }
     uint32_t *vis = GL ? p.scratch + (size_t)blockIdx.x * p.scratch_words : s_mem;
     GC *w = reinterpret_cast<GC *>(vis + p.bitmap_words);
     GC *fr = w + p.wcap;
@@ -262,6 +267,7 @@ __global__ __launch_bounds__(64) void k_graph_probe(const GraphParams p) {
 
     // output: first k of the result order, nodes without a posting dropped
     uint64_t *out = p.out + (size_t)q * p.npr;
+    if (lane == 0 && p.aborted) p.aborted[q] = s_abort;
     if (s_abort) {
         if (lane == 0) {
             p.out_cnt[q] = 0;

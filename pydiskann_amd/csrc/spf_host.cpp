@@ -128,7 +128,7 @@ struct DBuf {
     X(uint32_t, a_min) X(uint8_t, q8m) X(float, q8a) X(float, q8b) X(uint8_t, rq_levels)   \
     X(uint32_t, m1) X(uint32_t, s1) X(uint32_t, tau) X(uint32_t, adm_cnt) X(uint32_t, adm_list) \
     X(uint32_t, pool_list) X(uint32_t, pool_na) X(uint16_t, lkey) X(uint32_t, lslot)         \
-    X(uint32_t, lcnt) X(uint32_t, lsel) X(uint32_t, la)
+    X(uint32_t, lcnt) X(uint32_t, lsel) X(uint32_t, la) X(uint32_t, gabort)
 
 // A captured launch chain of one search call, replayed for later calls with
 // the same key (the same arguments, buffers and launch shapes).
@@ -1596,6 +1596,7 @@ uint64_t ws_bound(const spf_index *ix, const Shape &z, uint32_t nq, uint32_t k, 
     add(q, 4);                   // probe_cnt
     if (z.graph) {
         add(1, 4);               // gflag
+        add(2 * q, 4);           // gabort
         GraphParams g{};
         if (graph_shape(ix, z.nraw, &g)) {   // global-mode scratch, at most one launch's worth
             const uint64_t words = graph_scratch_words(ix, g);
@@ -1714,6 +1715,12 @@ bool graph_shape(const spf_index *ix, uint32_t n, GraphParams *gp) {
     return true;
 }
 
+// A first pass gives each query kGraphLdsFirst bytes of LDS (the frontier
+// what that leaves, usually far more than a walk needs), so that many more
+// queries walk at once than with the full 64 KiB; the few whose frontier
+// outgrows it walk again with the full share, then in global scratch.
+constexpr size_t kGraphLdsFirst = 16384;
+
 // global-mode scratch words per query: the frontier holds every node at most once
 uint64_t graph_scratch_words(const spf_index *ix, const GraphParams &g) {
     GraphParams t = g;
@@ -1749,21 +1756,49 @@ int graph_probe(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_
     g.out = out;
     g.out_cnt = out_cnt;
     HIPCHK(w->gflag.ensure(1));
+    HIPCHK(w->gabort.ensure(2 * (size_t)nq));
     g.overflow = w->gflag.p;
-    bool global = g.fcap < g.wcap;
-    if (!global) {
+    uint32_t *ab[2] = {w->gabort.p, w->gabort.p + nq};
+    int lvl = 0;   // aborted flags of the last launch: ab[lvl]
+    // a launch over the queries still to walk; returns whether any overflowed
+    auto pass = [&](const uint32_t *only, uint32_t *aborted) -> int {
         HIPCHK(hipMemsetAsync(w->gflag.p, 0, sizeof(uint32_t), s));
         g.q0 = 0;
+        g.only = only;
+        g.aborted = aborted;
         HIPCHK(launch_graph_probe(g, nq, s));
-        if (g.fcap < ix->g_nodes + 2) {   // the frontier could have outgrown LDS
-            w->h_summary[3] = 0;
-            HIPCHK(hipMemcpyAsync(&w->h_summary[3], w->gflag.p, sizeof(uint32_t),
-                                  hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            global = w->h_summary[3] != 0;
+        if (g.fcap >= ix->g_nodes + 2) return 0;   // the frontier cannot outgrow it
+        w->h_summary[3] = 0;
+        HIPCHK(hipMemcpyAsync(&w->h_summary[3], w->gflag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return w->h_summary[3] != 0 ? 1 : 0;
+    };
+    bool global = g.fcap < g.wcap, rest = true;   // rest: queries left to walk
+    const uint32_t *pending = nullptr;            // null = every query
+    if (!global) {
+        const uint32_t full = g.fcap;
+        const size_t used = graph_lds_bytes(g) - (size_t)full * 8;
+        const uint32_t small = used < kGraphLdsFirst ? (uint32_t)((kGraphLdsFirst - used) / 8) & ~1u : 0u;
+        if (small >= g.wcap && small < full) {   // a first pass with the smaller share
+            g.fcap = small;
+            const int o = pass(nullptr, ab[0]);
+            if (o < 0) return o;
+            rest = o > 0;
+            pending = ab[0];
+            lvl = 1;
+            g.fcap = full;
         }
+        if (rest) {
+            const int o = pass(pending, ab[lvl]);
+            if (o < 0) return o;
+            rest = o > 0;
+            pending = ab[lvl];
+        }
+        global = rest;
     }
     if (global) {
+        g.only = pending;
+        g.aborted = nullptr;
         g.fcap = std::max<uint32_t>((ix->g_nodes + 3) & ~1u, g.wcap);
         g.scratch_words = graph_scratch_words(ix, g);
         const uint64_t per = g.scratch_words * 4;

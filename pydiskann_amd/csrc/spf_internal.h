@@ -132,6 +132,12 @@ struct GraphParams {
     uint64_t *out;              // [nq][npr] fkey(dist) << 32 | posting
     uint32_t *out_cnt;          // [nq]
     uint32_t *overflow;         // queries whose frontier outgrew fcap
+    // per-query escalation: a launch walks only the queries whose `only` flag
+    // is set (null = every query) and sets `aborted` for those whose frontier
+    // outgrew fcap (cleared for the others), which the next launch, with a
+    // larger frontier, walks again
+    const uint32_t *only;
+    uint32_t *aborted;
     // global mode (state too large for LDS): bitmap and heaps of query
     // q0 + blockIdx.x at scratch + blockIdx.x * scratch_words
     uint32_t *scratch;          // null = LDS mode
