@@ -1715,11 +1715,17 @@ bool graph_shape(const spf_index *ix, uint32_t n, GraphParams *gp) {
     return true;
 }
 
-// A first pass gives each query kGraphLdsFirst bytes of LDS (the frontier
-// what that leaves, usually far more than a walk needs), so that many more
-// queries walk at once than with the full 64 KiB; the few whose frontier
-// outgrows it walk again with the full share, then in global scratch.
-constexpr size_t kGraphLdsFirst = 16384;
+// A first pass gives each query's frontier room for max(640, 20 x the w
+// heap) entries, which most walks never outgrow, instead of all that 64 KiB
+// leaves: the LDS share shrinks (about 8 KiB for a 16k-node graph, beam 32),
+// so several times more queries walk at once on a CU.  The queries whose
+// frontier outgrows it walk again with the full share, then in global scratch.
+#ifndef SPF_GRAPH_FIRST_MIN
+#define SPF_GRAPH_FIRST_MIN 640u
+#endif
+uint32_t graph_first_fcap(const GraphParams &g) {
+    return std::max<uint32_t>(SPF_GRAPH_FIRST_MIN, 20u * g.wcap) & ~1u;
+}
 
 // global-mode scratch words per query: the frontier holds every node at most once
 uint64_t graph_scratch_words(const spf_index *ix, const GraphParams &g) {
@@ -1766,7 +1772,10 @@ int graph_probe(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_
         g.q0 = 0;
         g.only = only;
         g.aborted = aborted;
-        HIPCHK(launch_graph_probe(g, nq, s));
+        // the wave-uniform walk where it applies (no result list, beam <= 64)
+        const bool walk = !g.filtered && g.wcap <= 64 && g.fcap >= 64 &&
+                          (size_t)g.bitmap_words * 4 + (size_t)g.fcap * 8 + (size_t)g.dpad * 4 <= 65536;
+        HIPCHK(walk ? launch_graph_walk(g, nq, s) : launch_graph_probe(g, nq, s));
         if (g.fcap >= ix->g_nodes + 2) return 0;   // the frontier cannot outgrow it
         w->h_summary[3] = 0;
         HIPCHK(hipMemcpyAsync(&w->h_summary[3], w->gflag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1777,9 +1786,8 @@ int graph_probe(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32_
     const uint32_t *pending = nullptr;            // null = every query
     if (!global) {
         const uint32_t full = g.fcap;
-        const size_t used = graph_lds_bytes(g) - (size_t)full * 8;
-        const uint32_t small = used < kGraphLdsFirst ? (uint32_t)((kGraphLdsFirst - used) / 8) & ~1u : 0u;
-        if (small >= g.wcap && small < full) {   // a first pass with the smaller share
+        const uint32_t small = graph_first_fcap(g);
+        if (small < full) {   // a first pass with the smaller share
             g.fcap = small;
             const int o = pass(nullptr, ab[0]);
             if (o < 0) return o;

@@ -507,6 +507,7 @@ hipError_t launch_pair_rank(const PlanParams &p, hipStream_t s);
 hipError_t launch_excl_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t s);
 hipError_t launch_rerank(const RerankParams &p, hipStream_t s);
 hipError_t launch_graph_probe(const GraphParams &p, uint32_t n, hipStream_t s);
+hipError_t launch_graph_walk(const GraphParams &p, uint32_t n, hipStream_t s);
 hipError_t launch_nearest_out(const uint64_t *probe_sel, const uint32_t *probe_cnt, uint32_t n,
                               const uint32_t *block_sorted, uint32_t *out_block, float *out_dist,
                               hipStream_t s);

@@ -66,3 +66,32 @@ def graph_index(n=30000, dim=16, n_post=4200, quant=None, metric="l2", r=12, del
     if metric in ("dot", "cosine"):
         q /= np.linalg.norm(q, axis=1, keepdims=True)
     return x, ix, q
+
+
+def line_index(n=20000, r=64, beam=32, seed=0):
+    """Postings whose centroids lie on a line (1.0 apart along axis 0), each
+    linked to the r nodes on either side, entered at one end.  A query beyond
+    the far end makes every expansion reveal r nodes nearer than all seen so
+    far, each of which enters the w heap and the frontier: the frontier grows
+    by about r - 1 per expansion, so the walk's frontier size is set by how
+    far along the line the query lies."""
+    rng = np.random.default_rng(seed)
+    cents = np.zeros((n, 8), np.float32)
+    cents[:, 0] = np.arange(n, dtype=np.float32)
+    x = (np.repeat(cents, 2, axis=0) + rng.normal(0, 0.05, size=(2 * n, 8))).astype(np.float32)
+    ix = build_index(x, "l2", None, centroids=cents, seed=seed)
+    c = ix.centroids.astype(np.float32)
+    order = np.argsort(c[:, 0], kind="stable")
+    p = c.shape[0]
+    lists = [None] * p
+    for j in range(p):
+        nb = [order[t] for t in range(j + 1, min(p, j + r + 1))] + \
+             [order[t] for t in range(max(0, j - r), j)]
+        lists[order[j]] = np.array(nb, dtype=np.uint32)
+    adj_off = np.zeros(p + 1, dtype=np.uint32)
+    adj_off[1:] = np.cumsum([len(v) for v in lists])
+    ix.graph = CentroidGraph(vectors=c, adj_off=adj_off, adj=np.concatenate(lists),
+                             live=np.ones(p, np.uint8), seeds=[int(order[0])],
+                             node_post=np.arange(p, dtype=np.int64), n_tombstones=0,
+                             centroid_beam=beam, base_n=p)
+    return x, ix

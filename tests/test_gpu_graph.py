@@ -44,12 +44,15 @@ def test_graph_probe_parity_layers_and_tombstones(delta, dead):
     compare(dev, ix, q, 100, 64)
 
 
+# dead 0: no tombstones, beam 64 (the wave-uniform walk); dead 25: tombstones
+# (the walk with a result list)
+@pytest.mark.parametrize("dead", [0, 25])
 @pytest.mark.parametrize("quant,metric,rerank", [(None, "cosine", 0), (None, "dot", 0),
                                                  ("int8", "l2", 64), ("f16", "l2", 0),
                                                  ("rabitq", "l2", 0)])
-def test_graph_probe_parity_metrics_and_quantizers(quant, metric, rerank):
+def test_graph_probe_parity_metrics_and_quantizers(quant, metric, rerank, dead):
     x, ix, q = graph_index(n=24000, n_post=4150, quant=quant, metric=metric, rerank=rerank,
-                           delta=200, dead=25, seed=2)
+                           delta=200, dead=dead, seed=2)
     dev = DeviceIndex(ix)
     compare(dev, ix, q, 10, 16)
 
@@ -91,3 +94,33 @@ def test_graph_probe_mixed_precision():
     np.testing.assert_array_equal(e[2], m[2])
     np.testing.assert_array_equal(e[0], m[0])
     np.testing.assert_array_equal(e[1].view(np.uint32), m[1].view(np.uint32))
+
+
+def test_graph_probe_frontier_outgrows_first_share():
+    # Along a 20,000-node line with 128 links per node (two 64-lane batches
+    # per expansion) and beam 32, queries near the entry end walk with a small
+    # frontier, queries 1,000-3,000 along it outgrow the first LDS share
+    # (640 entries) and walk again with the full one, and queries beyond
+    # 12,000 outgrow that too and walk in global scratch; one batch holds all
+    # three kinds, interleaved.
+    from graph_util import line_index
+    x, ix = line_index()
+    rng = np.random.default_rng(8)
+    at = np.concatenate([rng.uniform(-5, 40, 8), rng.uniform(1000, 3000, 8),
+                         rng.uniform(12000, 20050, 8)])
+    q = np.zeros((24, 8), np.float32)
+    q[:, 0] = at[np.argsort(rng.random(24))]
+    q[:, 1:] = rng.normal(0, 0.3, size=(24, 7))
+    dev = DeviceIndex(ix)
+    compare(dev, ix, q, 10, 8)
+    compare(dev, ix, q, 10, 40)
+
+
+@pytest.mark.parametrize("metric", ["l2", "cosine"])
+def test_graph_walk_wide_rows(metric):
+    # 200 dims: 50 four-dim units, each lane of a pair loads them in 3 batches
+    x, ix, q = graph_index(n=20000, dim=200, n_post=4300, metric=metric, delta=0, dead=0, r=24,
+                           seed=9)
+    dev = DeviceIndex(ix)
+    compare(dev, ix, q, 10, 16)
+    compare(dev, ix, q, 50, 64)
