@@ -410,29 +410,47 @@ __device__ __forceinline__ GC lds_get(const GC *h, uint32_t i) {
 __device__ __forceinline__ void lds_set(GC *h, uint32_t i, GC c, uint32_t lane) {
     if (lane == 0) h[i] = c;
 }
+// BinaryHeap sift_up of e from slot pos, all ancestors read at once: lane j
+// (1 <= j <= depth) holds the j-th ancestor; the walk up stops below the
+// first one at or before e in the frontier's order (Reverse: a.d <= e.d), the
+// ancestors below it move down one level, e takes the freed slot.
 __device__ __forceinline__ void fr_sift_up(GC *h, uint32_t pos, GC e, uint32_t lane) {
-    while (pos > 0) {
-        const uint32_t parent = (pos - 1) / 2;
-        const GC pe = lds_get(h, parent);
-        if (pe.d <= e.d) break;
-        lds_set(h, pos, pe, lane);
-        pos = parent;
-    }
-    lds_set(h, pos, e, lane);
+    const uint32_t I = pos + 1, depth = 31u - (uint32_t)__builtin_clz(I);   // 1-based slot
+    const bool mine = lane >= 1 && lane <= depth;
+    GC v{0.0f, 0u};
+    if (mine) v = h[(I >> lane) - 1];
+    const uint64_t stop = __ballot(mine && v.d <= e.d);
+    const uint32_t J = stop ? (uint32_t)__builtin_ctzll(stop) : depth + 1;
+    if (mine && lane < J) h[(I >> (lane - 1)) - 1] = v;
+    if (lane == 0) h[(I >> (J - 1)) - 1] = e;
 }
+
+// BinaryHeap pop (sift_down_to_bottom, then sift_up).  The descent reads the
+// five levels below its position at once (lane t < 62: the node at depth
+// k = log2(t + 2) below, path t + 2 - 2^k) and takes five steps from them.
+constexpr uint32_t kPopLevels = 5;
 __device__ __forceinline__ GC fr_pop(GC *h, uint32_t &n, uint32_t lane) {
     const GC item = lds_get(h, n - 1);
     n--;
     if (n == 0) return item;
     const GC top = lds_get(h, 0);
     const uint32_t end = n;
+    const uint32_t k = 31u - (uint32_t)__builtin_clz(lane + 2), b = lane + 2 - (1u << k);
     uint32_t pos = 0, child = 1;
     while (end >= 2 && child <= end - 2) {
-        const GC c0 = lds_get(h, child), c1 = lds_get(h, child + 1);
-        const bool right = c1.d <= c0.d;   // Reverse order: c0 <= c1 iff c1.d <= c0.d
-        lds_set(h, pos, right ? c1 : c0, lane);
-        pos = child + (right ? 1u : 0u);
-        child = 2 * pos + 1;
+        const uint32_t idx = ((pos + 1) << k) + b - 1;
+        GC v{0.0f, 0u};
+        if (lane < 62 && idx < end) v = h[idx];
+        uint32_t rel = 1;   // the position in the loaded subtree, 1-based (its root: pos)
+        for (uint32_t l = 0; l < kPopLevels && end >= 2 && child <= end - 2; l++) {
+            const GC c0{rl_f(v.d, 2 * rel - 2), rl_u(v.id, 2 * rel - 2)};
+            const GC c1{rl_f(v.d, 2 * rel - 1), rl_u(v.id, 2 * rel - 1)};
+            const bool right = c1.d <= c0.d;   // Reverse order: c0 <= c1 iff c1.d <= c0.d
+            lds_set(h, pos, right ? c1 : c0, lane);
+            pos = child + (right ? 1u : 0u);
+            rel = 2 * rel + (right ? 1u : 0u);
+            child = 2 * pos + 1;
+        }
     }
     if (child == end - 1) {
         lds_set(h, pos, lds_get(h, child), lane);
@@ -531,7 +549,6 @@ __global__ __launch_bounds__(64) void k_graph_walk(const GraphParams p) {
             if (!rl_u(ok ? 1u : 0u, 2 * i)) continue;
             const GC c{rl_f(d, 2 * i), rl_u(sid, 2 * i)};
             if (lane == 0) vis[c.id >> 5] |= 1u << (c.id & 31);
-            lds_set(fr, fn, c, lane);
             fn++;
             fr_sift_up(fr, fn - 1, c, lane);
             w.push(wn, c);
@@ -583,7 +600,6 @@ __global__ __launch_bounds__(64) void k_graph_walk(const GraphParams p) {
                     abort = true;
                     break;
                 }
-                lds_set(fr, fn, c, lane);
                 fn++;
                 fr_sift_up(fr, fn - 1, c, lane);
             }
