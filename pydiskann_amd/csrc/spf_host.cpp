@@ -586,7 +586,9 @@ int derive_entries(spf_index *ix) {
         (void)hipFree(d_entry_post);
         HIPCHK(he);
     }
-    if (ix->quant == SPF_QUANT_RABITQ) {
+    // f16 rows for the rerank's bound stage: RaBitQ and int8 postings (their
+    // scan keys carry no usable error bound of their own)
+    if (ix->quant == SPF_QUANT_RABITQ || ix->quant == SPF_QUANT_INT8) {
         dev_free_as(ix, ix->d_raw16);
         dev_free_as(ix, ix->d_rq_xn);
         dev_free_as(ix, ix->d_post_ex);
@@ -1393,7 +1395,9 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     rp.qnorm = w->qnorm.p;
     rp.raw_e = ix->d_raw_e;
     rp.rank = ix->d_rank;
-    rp.prune = !rq && !full_rerank && ix->d_post_ex &&
+    // the key-based bound holds for f16-scan keys (f16 or f32 postings) only
+    rp.prune = !full_rerank && ix->d_post_ex &&
+               (ix->quant == SPF_QUANT_F16 || ix->quant == SPF_QUANT_NONE) &&
                (ix->metric == SPF_METRIC_L2 || ix->metric == SPF_METRIC_L2SQ);
     rp.kdim = dpad32;
     rp.qn16 = w->qn16.p;
@@ -1410,8 +1414,9 @@ int search_mixed(spf_index *ix, Workspace *w, hipStream_t s, uint32_t nq, uint32
     rp.ids = d_ids;
     rp.dists = d_dists;
     rp.counts = d_counts;
-    if (rq && !full_rerank && k <= 256 && ix->d_raw16) {
-        // RaBitQ: f16 rows order the pool by a proven bound, exact f32 in that order
+    if ((rq || ix->quant == SPF_QUANT_INT8) && !full_rerank && k <= 256 && ix->d_raw16 &&
+        rp.sel_bits) {
+        // RaBitQ and int8: f16 rows order the pool by a proven bound, exact f32 in that order
         rp.raw16 = ix->d_raw16;
         rp.xn = ix->d_rq_xn;
         HIPCHK(launch_rerank_f16(rp, s));

@@ -2324,7 +2324,9 @@ hipError_t launch_rerank_pruned(const RerankPrunedParams &p, hipStream_t s) {
     if (p.nq == 0) return hipSuccess;
     if (p.k > p.ca || p.ca > p.want || p.want > kMaxK) return hipErrorInvalidValue;
     const uint32_t grid = (p.nq + kXcds - 1) / kXcds * kXcds;
-    if (p.want <= 1024)
+    if (p.want <= 512)   // smaller pools: half the LDS, twice the workgroups per CU
+        hipLaunchKernelGGL(k_rerank_pruned<512>, dim3(grid), dim3(256), p.dpad * sizeof(float), s, p);
+    else if (p.want <= 1024)
         hipLaunchKernelGGL(k_rerank_pruned<1024>, dim3(grid), dim3(256), p.dpad * sizeof(float), s, p);
     else
         hipLaunchKernelGGL(k_rerank_pruned<kMaxK>, dim3(grid), dim3(256), p.dpad * sizeof(float), s, p);
@@ -2336,7 +2338,9 @@ hipError_t launch_rerank_f16(const RerankPrunedParams &p, hipStream_t s) {
     if (p.k > kTop16 || p.want > kMaxK) return hipErrorInvalidValue;
     const uint32_t grid = (p.nq + kXcds - 1) / kXcds * kXcds;
     const size_t lds = p.dpad * sizeof(float);
-    if (p.want <= 1024)
+    if (p.want <= 512)   // smaller pools: half the LDS, twice the workgroups per CU
+        hipLaunchKernelGGL(k_rerank_f16<512>, dim3(grid), dim3(256), lds, s, p);
+    else if (p.want <= 1024)
         hipLaunchKernelGGL(k_rerank_f16<1024>, dim3(grid), dim3(256), lds, s, p);
     else
         hipLaunchKernelGGL(k_rerank_f16<kMaxK>, dim3(grid), dim3(256), lds, s, p);
