@@ -1844,9 +1844,10 @@ __device__ __forceinline__ uint32_t rebuild_members(RerankLds<CAP> &L, uint32_t 
     return total;
 }
 
-// (at most 96 VGPRs: five workgroups a CU instead of four, no spills)
+// (up to 1,024 members: at most 96 VGPRs, five workgroups a CU instead of
+// four, no spills; the largest pools are bound by their LDS)
 template <uint32_t CAP>
-__global__ __launch_bounds__(256, 5) void k_rerank_pruned(const RerankPrunedParams p) {
+__global__ __launch_bounds__(256, CAP <= 1024 ? 5 : 1) void k_rerank_pruned(const RerankPrunedParams p) {
     __shared__ RerankLds<CAP> L;
     extern __shared__ __align__(16) float s_qv[];   // dpad floats
     enum { EXQ, XNQ, THR };
