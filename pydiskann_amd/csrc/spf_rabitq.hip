@@ -109,11 +109,13 @@ __global__ __launch_bounds__(64) void k_rq_prep(const RqParams p) {
         for (uint32_t m = 0; m < W; m++) r[m] = r[m] * p.signs[rd * P + 64 * m + lane];
 #pragma unroll
         for (uint32_t h = 1; h < 64; h *= 2) {
-            const bool hi_half = (lane & h) != 0;
+            // o - r (the lane with bit h) or r + o, as one fma: the product
+            // by -1 or 1 is exact, so the single rounding is the sum's
+            const float sg = (lane & h) ? -1.0f : 1.0f;
 #pragma unroll
             for (uint32_t m = 0; m < W; m++) {
                 const float o = __shfl_xor(r[m], (int)h, 64);
-                r[m] = hi_half ? o - r[m] : r[m] + o;
+                r[m] = __builtin_fmaf(sg, r[m], o);
             }
         }
 #pragma unroll
