@@ -640,32 +640,53 @@ __global__ __launch_bounds__(256, 2) void k_rq_scan_mfma(const RqParams p) {
 }
 
 // ---------------------------------------------------------------- rows
-// One thread per entry: f16 copy of the raw row, |x|^2, and the posting's
-// max |x - f16(x)| in double, rounded up (non-negative floats order as their
-// bits, so atomicMax on the bits is a max).
-__global__ void k_rq_rows(const RqRowsParams p) {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p.n) return;
-    uint32_t lo = 0, hi = p.n_post;   // posting: post_off[lo] <= e < post_off[lo + 1]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (p.post_off[mid] <= e) lo = mid;
-        else hi = mid;
+// 64 entries per workgroup: their postings found by 64 threads at once, then
+// each wave takes 16 of them one after another, lane l over dims l, l + 64, ...
+// (the row read coalesced, once): f16 copy of the raw row, |x|^2, and the
+// posting's max |x - f16(x)| in double, rounded up (non-negative floats order
+// as their bits, so atomicMax on the bits is a max).
+__global__ __launch_bounds__(256) void k_rq_rows(const RqRowsParams p) {
+    __shared__ uint32_t s_post[64];
+    const uint32_t e0 = blockIdx.x * 64, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < 64 && e0 + threadIdx.x < p.n) {
+        const uint32_t e = e0 + threadIdx.x;
+        uint32_t lo = 0, hi = p.n_post;   // posting: post_off[lo] <= e < post_off[lo + 1]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (p.post_off[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        s_post[threadIdx.x] = lo;
     }
-    const float *x = p.raw_e + (size_t)e * p.dpad;
-    uint16_t *h = p.raw16 + (size_t)e * p.dpad;
-    double e2 = 0.0, n2 = 0.0;
-    for (uint32_t i = 0; i < p.dpad; i++) {
-        const float v = i < p.dim ? x[i] : 0.0f;
-        const _Float16 f = (_Float16)v;
-        h[i] = __builtin_bit_cast(uint16_t, f);
-        const double t = (double)v - (double)(float)f;
-        e2 += t * t;
-        n2 += (double)v * v;
+    __syncthreads();
+    for (uint32_t k = 16 * wave; k < 16 * wave + 16; k++) {
+        const uint32_t e = e0 + k;
+        if (e >= p.n) break;   // whole wave
+        const float *x = p.raw_e + (size_t)e * p.dpad;
+        uint16_t *h = p.raw16 + (size_t)e * p.dpad;
+        double e2 = 0.0, n2 = 0.0;
+        for (uint32_t i = lane; i < p.dpad; i += 64) {
+            const float v = i < p.dim ? x[i] : 0.0f;
+            const _Float16 f = (_Float16)v;
+            h[i] = __builtin_bit_cast(uint16_t, f);
+            const double t = (double)v - (double)(float)f;
+            e2 += t * t;
+            n2 += (double)v * v;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            e2 += __shfl_xor(e2, off, 64);
+            n2 += __shfl_xor(n2, off, 64);
+        }
+        if (lane == 0) {
+            p.xn[e] = (float)n2;
+            const float ex = (float)(sqrt(e2) * (1.0 + 1e-12));
+            const uint32_t up = __float_as_uint(ex) + 1u;   // the next float up
+            // a long posting's entries would queue on one address: the atomic
+            // only when the value seen is below (a stale read just costs the atomic)
+            uint32_t *a = p.post_ex + s_post[k];
+            if (__atomic_load_n(a, __ATOMIC_RELAXED) < up) atomicMax(a, up);
+        }
     }
-    p.xn[e] = (float)n2;
-    const float ex = (float)(sqrt(e2) * (1.0 + 1e-12));
-    atomicMax(p.post_ex + lo, __float_as_uint(ex) + 1u);   // the next float up
 }
 
 // ---------------------------------------------------------------- launchers
@@ -720,7 +741,7 @@ hipError_t launch_rq_scan_mfma(const RqParams &p, uint32_t grid, hipStream_t s) 
 
 hipError_t launch_rq_rows(const RqRowsParams &p, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rq_rows, dim3((p.n + 255) / 256), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_rq_rows, dim3((p.n + 63) / 64), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

@@ -71,36 +71,57 @@ __device__ __forceinline__ uint32_t posting_of(const uint32_t *post_off, uint32_
     return lo;
 }
 
-// One thread per entry: |x - x16| and |x16|^2 in double (x16 = the tiled f16
-// code when given, else the raw row rounded to f16), rounded up, max per
-// posting by atomicMax on the bits (non-negative floats order as their bits).
-__global__ void k_post_bounds(const float *raw_e, const uint4 *codes16, uint32_t units16,
-                              uint32_t n, uint32_t dpad, uint32_t dim, const uint32_t *post_off,
-                              uint32_t n_post, uint32_t *post_ex, uint32_t *post_xn) {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const uint32_t p = posting_of(post_off, n_post, e);
-    const float *x = raw_e + (size_t)e * dpad;
-    const uint4 *c = codes16 ? codes16 + (size_t)(e >> 6) * units16 * 64 + (e & 63) : nullptr;
-    double s2 = 0.0, n2 = 0.0;
-    for (uint32_t j = 0; j < dim; j++) {
-        double h;
-        if (c) {
-            const uint4 v = c[(size_t)(j >> 3) * 64];
-            const uint32_t w = (j & 7) < 4 ? ((j & 7) < 2 ? v.x : v.y) : ((j & 7) < 6 ? v.z : v.w);
-            const uint16_t b = (uint16_t)((j & 1) ? (w >> 16) : (w & 0xffffu));
-            h = (double)(float)__builtin_bit_cast(_Float16, b);
-        } else {
-            h = (double)(float)(_Float16)x[j];
+// 64 entries per workgroup: their postings found by 64 threads at once, then
+// each wave takes 16 of them one after another, lane l over dims l, l + 64, ...
+// (the row read coalesced, once): |x - x16| and |x16|^2 in double (x16 = the
+// tiled f16 code when given, else the raw row rounded to f16), rounded up, max
+// per posting by atomicMax on the bits (non-negative floats order as their bits).
+__device__ __forceinline__ void max_bits(uint32_t *a, uint32_t v) {
+    // a long posting's entries would queue on one address: the atomic only
+    // when the value seen is below (a stale read just costs the atomic)
+    if (__atomic_load_n(a, __ATOMIC_RELAXED) < v) atomicMax(a, v);
+}
+__global__ __launch_bounds__(256) void k_post_bounds(const float *raw_e, const uint4 *codes16,
+                                                     uint32_t units16, uint32_t n, uint32_t dpad,
+                                                     uint32_t dim, const uint32_t *post_off,
+                                                     uint32_t n_post, uint32_t *post_ex,
+                                                     uint32_t *post_xn) {
+    __shared__ uint32_t s_post[64];
+    const uint32_t e0 = blockIdx.x * 64, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < 64 && e0 + threadIdx.x < n) s_post[threadIdx.x] = posting_of(post_off, n_post, e0 + threadIdx.x);
+    __syncthreads();
+    for (uint32_t k = 16 * wave; k < 16 * wave + 16; k++) {
+        const uint32_t e = e0 + k;
+        if (e >= n) break;   // whole wave
+        const float *x = raw_e + (size_t)e * dpad;
+        const uint4 *c = codes16 ? codes16 + (size_t)(e >> 6) * units16 * 64 + (e & 63) : nullptr;
+        double s2 = 0.0, n2 = 0.0;
+        for (uint32_t j = lane; j < dim; j += 64) {
+            double h;
+            if (c) {
+                const uint4 v = c[(size_t)(j >> 3) * 64];
+                const uint32_t w = (j & 7) < 4 ? ((j & 7) < 2 ? v.x : v.y) : ((j & 7) < 6 ? v.z : v.w);
+                const uint16_t b = (uint16_t)((j & 1) ? (w >> 16) : (w & 0xffffu));
+                h = (double)(float)__builtin_bit_cast(_Float16, b);
+            } else {
+                h = (double)(float)(_Float16)x[j];
+            }
+            const double t = (double)x[j] - h;
+            s2 += t * t;
+            n2 += h * h;
         }
-        const double t = (double)x[j] - h;
-        s2 += t * t;
-        n2 += h * h;
+        for (int off = 32; off > 0; off >>= 1) {
+            s2 += __shfl_xor(s2, off, 64);
+            n2 += __shfl_xor(n2, off, 64);
+        }
+        if (lane == 0) {
+            const uint32_t p = s_post[k];
+            const float ex = (float)(sqrt(s2) * (1.0 + 1e-12));
+            const float xn = (float)(n2 * (1.0 + 1e-12));
+            max_bits(post_ex + p, __float_as_uint(ex) + 1u);   // the next float up
+            max_bits(post_xn + p, __float_as_uint(xn) + 1u);
+        }
     }
-    const float ex = (float)(sqrt(s2) * (1.0 + 1e-12));
-    const float xn = (float)(n2 * (1.0 + 1e-12));
-    atomicMax(post_ex + p, __float_as_uint(ex) + 1u);   // the next float up
-    atomicMax(post_xn + p, __float_as_uint(xn) + 1u);
 }
 
 __global__ void k_entry_post(const uint32_t *post_off, uint32_t n_post, uint32_t n, uint32_t *out) {
@@ -156,7 +177,7 @@ hipError_t launch_post_bounds(const float *raw_e, const uint4 *codes16, uint32_t
                               uint32_t n_post, uint32_t *post_ex, uint32_t *post_xn,
                               hipStream_t s) {
     if (!n || !n_post) return hipSuccess;
-    hipLaunchKernelGGL(k_post_bounds, dim3(blocks_for(n)), dim3(256), 0, s, raw_e, codes16,
+    hipLaunchKernelGGL(k_post_bounds, dim3((n + 63) / 64), dim3(256), 0, s, raw_e, codes16,
                        units16, n, dpad, dim, post_off, n_post, post_ex, post_xn);
     return hipGetLastError();
 }
