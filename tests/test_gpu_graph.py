@@ -124,3 +124,25 @@ def test_graph_walk_wide_rows(metric):
     dev = DeviceIndex(ix)
     compare(dev, ix, q, 10, 16)
     compare(dev, ix, q, 50, 64)
+
+
+@pytest.mark.parametrize("r,repeat", [(12, 3), (24, 24)])
+def test_graph_walk_neighbours_listed_twice(r, repeat):
+    # Each node's list followed by a repeat of its first `repeat` neighbours:
+    # 12 + 3 repeats fall in one 32-neighbour pass (the repeat found by the
+    # visited bits' atomics, then its first copy by a shuffle scan); 24 + 24
+    # spill into a second pass, where the repeats are already visited.
+    from pydiskann_amd.index import CentroidGraph
+    x, ix, q = graph_index(n=30000, n_post=4200, r=r, delta=0, dead=0, seed=12)
+    g = ix.graph
+    lists = [g.adj[g.adj_off[i]:g.adj_off[i + 1]] for i in range(g.n_nodes)]
+    lists = [np.concatenate([v, v[:repeat]]).astype(np.uint32) for v in lists]
+    adj_off = np.zeros(g.n_nodes + 1, dtype=np.uint32)
+    adj_off[1:] = np.cumsum([len(v) for v in lists])
+    ix.graph = CentroidGraph(vectors=g.vectors, adj_off=adj_off, adj=np.concatenate(lists),
+                             live=g.live, seeds=g.seeds, node_post=g.node_post,
+                             n_tombstones=g.n_tombstones, centroid_beam=g.centroid_beam,
+                             base_n=g.base_n)
+    dev = DeviceIndex(ix)
+    compare(dev, ix, q, 10, 16)
+    compare(dev, ix, q, 10, 64)
