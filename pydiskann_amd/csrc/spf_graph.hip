@@ -338,9 +338,10 @@ __global__ __launch_bounds__(64) void k_graph_probe(const GraphParams p) {
 // Wave-uniform walk: the same beam search with every lane of the wave running
 // the heap updates in lockstep instead of lane 0 alone behind barriers.  The
 // walk state (heap sizes, current node) sits in scalar registers, the w heap
-// (at most 64 entries) in one VGPR pair, entry i in lane i (read with
-// readlane, written by a lane select), the frontier heap in LDS.  The heap algorithms
-// and the visit order are those of k_graph_probe, so results are identical.
+// (at most 64 entries) in one VGPR pair, entry i in lane i (a sift moves
+// every entry on its path at once, by shuffles), the frontier heap in LDS.
+// The heap algorithms and the visit order are those of k_graph_probe, so
+// results are identical.
 // Used when the graph has no tombstones (no result list) and the beam is at
 // most 64, with the visited bitmap and frontier in LDS.
 //
@@ -360,41 +361,60 @@ struct LaneHeap {
     uint32_t wi;
     uint32_t lane;
     __device__ __forceinline__ GC get(uint32_t i) const { return GC{rl_f(wd, i), rl_u(wi, i)}; }
-    __device__ __forceinline__ void set(uint32_t i, GC c) {
-        const bool mine = lane == i;
-        wd = mine ? c.d : wd;
-        wi = mine ? c.id : wi;
-    }
+    // BinaryHeap sift_up of e from slot pos, every lane at once: lane j
+    // (1 <= j <= depth) reads the j-th ancestor's distance, a ballot finds the
+    // first at or above e (the walk up stops below it), the ancestors below
+    // that move down one level (each takes its parent's entry), e takes the
+    // freed slot
     __device__ __forceinline__ void sift_up(uint32_t pos, GC e) {
-        while (pos > 0) {
-            const uint32_t parent = (pos - 1) / 2;
-            const GC pe = get(parent);
-            if (e.d <= pe.d) break;
-            set(pos, pe);
-            pos = parent;
+        const uint32_t I = pos + 1, dp = 31u - (uint32_t)__builtin_clz(I);   // 1-based slot
+        const bool mine = lane >= 1 && lane <= dp;
+        const float ad = __shfl(wd, mine ? (int)((I >> lane) - 1) : 0, 64);
+        const int par = lane > 0 ? (int)((lane - 1) / 2) : 0;
+        const float pd = __shfl(wd, par, 64);
+        const uint32_t pi = __shfl(wi, par, 64);
+        const uint64_t stop = __ballot(mine && e.d <= ad);
+        const uint32_t J = stop ? (uint32_t)__builtin_ctzll(stop) : dp + 1;
+        // this lane's slot: k levels above pos on its path (k = 0: pos)
+        const int k = (int)dp - (int)(31u - (uint32_t)__builtin_clz(lane + 1));
+        const bool on = k >= 0 && (I >> k) == lane + 1;
+        if (on && (uint32_t)k + 1 < J) {
+            wd = pd;
+            wi = pi;
+        } else if (on && (uint32_t)k + 1 == J) {
+            wd = e.d;
+            wi = e.id;
         }
-        set(pos, e);
     }
     __device__ __forceinline__ void push(uint32_t &n, GC c) {
         n++;
         sift_up(n - 1, c);
     }
-    __device__ __forceinline__ void pop(uint32_t &n) {   // BinaryHeap::pop, result unused
+    // BinaryHeap::pop (result unused): the descent's path is found by
+    // readlane compares, then every entry on it moves up one level at once
+    // (each takes its child toward the path's end), then the last entry is
+    // sifted up from there
+    __device__ __forceinline__ void pop(uint32_t &n) {
         const GC item = get(n - 1);
         n--;
         if (n == 0) return;
         const uint32_t end = n;
         uint32_t pos = 0, child = 1;
         while (end >= 2 && child <= end - 2) {
-            const GC c0 = get(child), c1 = get(child + 1);
-            const bool right = c0.d <= c1.d;
-            set(pos, right ? c1 : c0);
+            const bool right = rl_f(wd, child) <= rl_f(wd, child + 1);
             pos = child + (right ? 1u : 0u);
             child = 2 * pos + 1;
         }
-        if (child == end - 1) {
-            set(pos, get(child));
-            pos = child;
+        if (child == end - 1) pos = child;
+        const uint32_t I = pos + 1, dp = 31u - (uint32_t)__builtin_clz(I);
+        const int k = (int)dp - (int)(31u - (uint32_t)__builtin_clz(lane + 1));
+        const bool on = k >= 1 && (I >> k) == lane + 1;
+        const int src = on ? (int)((I >> (k - 1)) - 1) : (int)lane;
+        const float sd = __shfl(wd, src, 64);
+        const uint32_t si = __shfl(wi, src, 64);
+        if (on) {
+            wd = sd;
+            wi = si;
         }
         sift_up(pos, item);
     }
