@@ -1797,23 +1797,31 @@ struct RerankLds {
 // entries out[0..): non-empty words are listed with the position of their
 // first member, then each thread takes members i = tid, tid + 256, ... (word
 // by binary search, bit by rank), so the work is even however the members
-// cluster.  Returns the member count.  Thread t owns words [w0, w1).
-template <uint32_t CAP, class MaskOf>
+// cluster.  Returns the member count.  Thread t owns words [w0, w1), read B
+// at a time in the counting pass (cheap mask_of: plain loads in flight
+// together), then again up to its last non-empty word.
+template <uint32_t CAP, int B = 1, class MaskOf>
 __device__ __forceinline__ uint32_t rebuild_members(RerankLds<CAP> &L, uint32_t w0, uint32_t w1,
                                                     MaskOf &&mask_of, const ProbeTab &tab,
                                                     uint32_t *out, uint32_t cap) {
     const uint32_t tid = threadIdx.x;
     uint32_t nz = 0, cnt = 0;
-    for (uint32_t w = w0; w < w1; w++) {
-        const uint32_t m = mask_of(w);
-        nz += m != 0u;
-        cnt += __popc(m);
+    for (uint32_t w = w0; w < w1; w += B) {
+        uint32_t m[B];
+#pragma unroll
+        for (int i = 0; i < B; i++) m[i] = w + i < w1 ? mask_of(w + i) : 0u;
+#pragma unroll
+        for (int i = 0; i < B; i++) {
+            nz += m[i] != 0u;
+            cnt += __popc(m[i]);
+        }
     }
     // one scan of both counts (each total is at most the pool size, < 2^16)
     const uint32_t pk = blk_excl(nz | (cnt << 16), L.w);
     uint32_t zpos = pk & 0xffffu, ppos = pk >> 16;
-    for (uint32_t w = w0; w < w1 && nz; w++) {
+    for (uint32_t w = w0, left = nz; w < w1 && left; w++) {
         const uint32_t m = mask_of(w);
+        left -= m != 0u;
         if (m && zpos < CAP) {
             L.nzw[zpos] = w;
             L.nzm[zpos] = m;
@@ -2117,7 +2125,7 @@ __global__ __launch_bounds__(256) void k_rerank_f16(const RerankPrunedParams p) 
     __syncthreads();
     if (lane == 0) atomicMax(&L.m[0], __float_as_uint(exq));
     // 1. the pool's members (rebuild_members' barriers publish L.m and the table)
-    const uint32_t cnt = rebuild_members(L, w0, w1, [&](uint32_t w) { return sel[w]; }, tab,
+    const uint32_t cnt = rebuild_members<CAP, 8>(L, w0, w1, [&](uint32_t w) { return sel[w]; }, tab,
                                          L.ent, c);
     const float ex = __uint_as_float(L.m[0]);
     const float gam = (float)(p.dim + 16) * (1.0f / 8388608.0f);
