@@ -2078,7 +2078,9 @@ struct Rerank16Lds {
     uint32_t tc;
 };
 
-template <uint32_t CAP>
+// LONG: rows of at least 256 dims (8+ units a lane), whose loads are unrolled
+// eight deep (more registers, fewer waves: a loss for short rows)
+template <uint32_t CAP, bool LONG>
 __global__ __launch_bounds__(256) void k_rerank_f16(const RerankPrunedParams p) {
     __shared__ Rerank16Lds<CAP> S;
     RerankLds<CAP> &L = S.b;
@@ -2140,7 +2142,7 @@ __global__ __launch_bounds__(256) void k_rerank_f16(const RerankPrunedParams p) 
                 const uint32_t e = i < cnt ? L.ent[i] : L.ent[0];
                 const uint4 *xr = reinterpret_cast<const uint4 *>(p.raw16 + (size_t)e * p.dpad);
                 float acc = 0.0f;
-                for (uint32_t u = r; u < units; u += 4) {
+                auto unit = [&](uint32_t u) {
                     const half8_t hv = as_half8(xr[u]);
                     const float4 qa4 = reinterpret_cast<const float4 *>(s_qv)[2 * u];
                     const float4 qb4 = reinterpret_cast<const float4 *>(s_qv)[2 * u + 1];
@@ -2155,6 +2157,12 @@ __global__ __launch_bounds__(256) void k_rerank_f16(const RerankPrunedParams p) 
                             acc = __builtin_fmaf(qq[j], xv, acc);
                         }
                     }
+                };
+                if constexpr (LONG) {
+#pragma unroll 8
+                    for (uint32_t u = r; u < units; u += 4) unit(u);
+                } else {
+                    for (uint32_t u = r; u < units; u += 4) unit(u);
                 }
                 acc = quad_sum(acc);
                 if (r == 0 && i < cnt) {
@@ -2348,12 +2356,19 @@ hipError_t launch_rerank_f16(const RerankPrunedParams &p, hipStream_t s) {
     if (p.k > kTop16 || p.want > kMaxK) return hipErrorInvalidValue;
     const uint32_t grid = (p.nq + kXcds - 1) / kXcds * kXcds;
     const size_t lds = p.dpad * sizeof(float);
-    if (p.want <= 512)   // smaller pools: half the LDS, twice the workgroups per CU
-        hipLaunchKernelGGL(k_rerank_f16<512>, dim3(grid), dim3(256), lds, s, p);
-    else if (p.want <= 1024)
-        hipLaunchKernelGGL(k_rerank_f16<1024>, dim3(grid), dim3(256), lds, s, p);
-    else
-        hipLaunchKernelGGL(k_rerank_f16<kMaxK>, dim3(grid), dim3(256), lds, s, p);
+#define SPF_RR16(L_)                                                                        \
+    if (p.want <= 512)   /* smaller pools: half the LDS, more workgroups per CU */            \
+        hipLaunchKernelGGL((k_rerank_f16<512, L_>), dim3(grid), dim3(256), lds, s, p);        \
+    else if (p.want <= 1024)                                                                  \
+        hipLaunchKernelGGL((k_rerank_f16<1024, L_>), dim3(grid), dim3(256), lds, s, p);       \
+    else                                                                                      \
+        hipLaunchKernelGGL((k_rerank_f16<kMaxK, L_>), dim3(grid), dim3(256), lds, s, p);
+    if (p.dpad >= 256) {
+        SPF_RR16(true)
+    } else {
+        SPF_RR16(false)
+    }
+#undef SPF_RR16
     return hipGetLastError();
 }
 
