@@ -48,13 +48,38 @@ __device__ __forceinline__ void wave_fht(float *v, uint32_t n) {
     }
 }
 
+// Lane l's partner l ^ H: DPP within a quad (H = 1, 2) or a 16-lane row
+// rotated by 8 (H = 8); the other stages through the LDS crossbar (measured:
+// two rotations and a select for H = 4, or the row-swap permutes for 16 and
+// 32, were slower than the shuffle)
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <uint32_t H>
+__device__ __forceinline__ float xor_lane(float v) {
+    if constexpr (H == 1) return dpp_f<0xB1>(v);    // quad_perm [1,0,3,2]
+    else if constexpr (H == 2) return dpp_f<0x4E>(v);    // quad_perm [2,3,0,1]
+    else if constexpr (H == 8) return dpp_f<0x128>(v);   // row_ror:8
+    else return __shfl_xor(v, (int)H, 64);
+}
+// FHT stage h = H < 64 over lanes: o - r (the lane with bit H) or r + o, as
+// one fma: the product by -1 or 1 is exact, so the single rounding is the sum's
+template <uint32_t H, uint32_t W>
+__device__ __forceinline__ void fht_lanes(float (&r)[W], uint32_t lane) {
+    const float sg = (lane & H) ? -1.0f : 1.0f;
+#pragma unroll
+    for (uint32_t m = 0; m < W; m++) r[m] = __builtin_fmaf(sg, r[m], xor_lane<H>(r[m]));
+}
+
 // ---------------------------------------------------------------- query prep
 // One wave per (query, probed posting).  Lane l holds elements i = 64 m + l
 // (m < W) in registers.  The FHT's butterfly pairs and their IEEE operations
 // are the reference's (src/rabitq.rs:31-43): stage h pairs j and j + h with
 // v[j] = x + y, v[j + h] = x - y, stages in increasing h.  Stages h < 64 pair
-// lane l with lane l ^ h (one shuffle per register: the lane without bit h
-// adds, the other subtracts its own value from its partner's); stages
+// lane l with lane l ^ h (one cross-lane move per register, DPP or shuffle:
+// the lane without bit h adds, the other subtracts its own value from its
+// partner's); stages
 // h >= 64 pair registers m and m + h / 64 of the same lane.  Which lane
 // computes a butterfly does not change its result, so the rotated residual
 // is the same as a sequential transform's, bit for bit.
@@ -107,17 +132,12 @@ __global__ __launch_bounds__(64) void k_rq_prep(const RqParams p) {
     for (uint32_t rd = 0; rd < 3; rd++) {
 #pragma unroll
         for (uint32_t m = 0; m < W; m++) r[m] = r[m] * p.signs[rd * P + 64 * m + lane];
-#pragma unroll
-        for (uint32_t h = 1; h < 64; h *= 2) {
-            // o - r (the lane with bit h) or r + o, as one fma: the product
-            // by -1 or 1 is exact, so the single rounding is the sum's
-            const float sg = (lane & h) ? -1.0f : 1.0f;
-#pragma unroll
-            for (uint32_t m = 0; m < W; m++) {
-                const float o = __shfl_xor(r[m], (int)h, 64);
-                r[m] = __builtin_fmaf(sg, r[m], o);
-            }
-        }
+        fht_lanes<1, W>(r, lane);
+        fht_lanes<2, W>(r, lane);
+        fht_lanes<4, W>(r, lane);
+        fht_lanes<8, W>(r, lane);
+        fht_lanes<16, W>(r, lane);
+        fht_lanes<32, W>(r, lane);
 #pragma unroll
         for (uint32_t g = 1; g < W; g *= 2)
 #pragma unroll
