@@ -368,8 +368,9 @@ void spf_server_destroy(spf_server *sv);
  * device launch shapes).  Times the search of the given sample queries under
  * each candidate shape -- VALU scan (EXACT): 4 / 8 / 16 queries x 256 / 512 /
  * 1,024 / 2,048 entries per work item; matrix-core scan (MIXED): 32 / 64 / 128
- * queries per work item -- and keeps the fastest on the index for later
- * searches.  With cache_path (may be NULL), a line keyed by the device, the
+ * queries per work item -- in interleaved rounds (median of five each), and
+ * keeps the fastest on the index for later searches; another shape replaces
+ * the default one only when at least 2% faster.  With cache_path (may be NULL), a line keyed by the device, the
  * index shape and the search settings is looked up first (nothing is timed
  * then) and appended after measuring.  Results never depend on the shape.
  * Must not run concurrently with searches on the same index. */

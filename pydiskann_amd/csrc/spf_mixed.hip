@@ -300,6 +300,9 @@ __device__ __forceinline__ uint32_t key_pair(float d0, float d1) {
 // on timing; the pool taken from it does not.
 template <int KS, int NH, bool FILT, bool I8, bool L2M, bool ADM>
 __global__ __launch_bounds__(64) void k_scan_wave(const MixedParams p) {
+    // keys of up to four tiles per query row, stored as 16-byte units of a row
+    // (up to 128 contiguous bytes a row) once the four tiles are done
+    __shared__ __align__(16) uint16_t s_k[ADM ? 1 : NH][16][64];
     const uint32_t lane = threadIdx.x, col = lane & 15, kg = lane >> 4;
     const uint32_t lo = p.item_lo_dev ? *p.item_lo_dev : 0u;
     const uint32_t n_items = (p.n_items_dev ? *p.n_items_dev : p.n_items) - lo;
@@ -444,8 +447,32 @@ __global__ __launch_bounds__(64) void k_scan_wave(const MixedParams p) {
                         for (int j = 0; j < 4; j++)
                             if (kk[j] <= atau[h]) dst[pos++] = (kk[j] << 16) | (asl[h] + rel + j);
                     }
-                } else if (in && slot[h] != ~0ull) {
-                    *reinterpret_cast<uint2 *>(p.out16 + slot[h] + rel) = make_uint2(k01, k23);
+                } else {
+                    (void)in;
+                    *reinterpret_cast<uint2 *>(&s_k[h][col][(t & 3) * 16 + 4 * kg]) = make_uint2(k01, k23);
+                }
+            }
+            if constexpr (!ADM) {
+                if ((t & 3) == 3 || t + 1 == ntile) {   // the group's rows leave as 16-byte units
+                    __syncthreads();
+                    const uint32_t g0t = t & ~3u, nu = 2 * (t - g0t + 1);   // units a row
+                    const uint32_t relg = e_begin + 16 * g0t - pb;        // (a multiple of 16)
+                    const uint32_t r8 = lane >> 3, c = lane & 7;
+#pragma unroll
+                    for (int h = 0; h < NH; h++) {
+                        if (16 * h >= (int)gcnt) break;   // (uniform)
+#pragma unroll
+                        for (uint32_t rr = 0; rr < 16; rr += 8) {
+                            const uint32_t r = rr + r8;   // lane r holds row r's slot
+                            const uint32_t slo = __shfl((uint32_t)slot[h], (int)r, 64);
+                            const uint32_t shi = __shfl((uint32_t)(slot[h] >> 32), (int)r, 64);
+                            const uint64_t sl = ((uint64_t)shi << 32) | slo;
+                            if (c < nu && sl != ~0ull && relg + 8 * c < lim)
+                                *reinterpret_cast<uint4 *>(p.out16 + sl + relg + 8 * c) =
+                                    *reinterpret_cast<const uint4 *>(&s_k[h][r][8 * c]);
+                        }
+                    }
+                    __syncthreads();   // (the next group rewrites the tile rows)
                 }
             }
 #pragma unroll
