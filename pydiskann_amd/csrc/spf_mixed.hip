@@ -395,7 +395,6 @@ __global__ __launch_bounds__(64) void k_scan_wave(const MixedParams p) {
                 lj[j] = FILT ? (uint32_t)__shfl((int)lb, 4 * kg + j) : 0u;
             }
             const uint32_t rel = e0 - pb + 4 * kg;   // slot offset of the lane's four keys
-            const bool in = rel < lim;
             uint32_t dm01 = ((m4 & 1u) ? 0u : 0xffffu) | ((m4 & 2u) ? 0u : 0xffff0000u);
             uint32_t dm23 = ((m4 & 4u) ? 0u : 0xffffu) | ((m4 & 8u) ? 0u : 0xffff0000u);
 #pragma unroll
@@ -448,7 +447,6 @@ __global__ __launch_bounds__(64) void k_scan_wave(const MixedParams p) {
                             if (kk[j] <= atau[h]) dst[pos++] = (kk[j] << 16) | (asl[h] + rel + j);
                     }
                 } else {
-                    (void)in;
                     *reinterpret_cast<uint2 *>(&s_k[h][col][(t & 3) * 16 + 4 * kg]) = make_uint2(k01, k23);
                 }
             }
