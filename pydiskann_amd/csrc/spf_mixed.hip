@@ -300,9 +300,10 @@ __device__ __forceinline__ uint32_t key_pair(float d0, float d1) {
 // on timing; the pool taken from it does not.
 template <int KS, int NH, bool FILT, bool I8, bool L2M, bool ADM>
 __global__ __launch_bounds__(64) void k_scan_wave(const MixedParams p) {
-    // keys of up to four tiles per query row, stored as 16-byte units of a row
-    // (up to 128 contiguous bytes a row) once the four tiles are done
-    __shared__ __align__(16) uint16_t s_k[ADM ? 1 : NH][16][64];
+    // keys of GT tiles per query row (8 KiB in all), stored as 16-byte units of
+    // a row (up to 32 GT contiguous bytes a row) once the GT tiles are done
+    constexpr uint32_t GT = NH == 2 ? 8 : 4;
+    __shared__ __align__(16) uint16_t s_k[ADM ? 1 : NH][16][16 * GT];
     const uint32_t lane = threadIdx.x, col = lane & 15, kg = lane >> 4;
     const uint32_t lo = p.item_lo_dev ? *p.item_lo_dev : 0u;
     const uint32_t n_items = (p.n_items_dev ? *p.n_items_dev : p.n_items) - lo;
@@ -447,13 +448,13 @@ __global__ __launch_bounds__(64) void k_scan_wave(const MixedParams p) {
                             if (kk[j] <= atau[h]) dst[pos++] = (kk[j] << 16) | (asl[h] + rel + j);
                     }
                 } else {
-                    *reinterpret_cast<uint2 *>(&s_k[h][col][(t & 3) * 16 + 4 * kg]) = make_uint2(k01, k23);
+                    *reinterpret_cast<uint2 *>(&s_k[h][col][(t % GT) * 16 + 4 * kg]) = make_uint2(k01, k23);
                 }
             }
             if constexpr (!ADM) {
-                if ((t & 3) == 3 || t + 1 == ntile) {   // the group's rows leave as 16-byte units
+                if (t % GT == GT - 1 || t + 1 == ntile) {   // the group's rows leave as 16-byte units
                     __syncthreads();
-                    const uint32_t g0t = t & ~3u, nu = 2 * (t - g0t + 1);   // units a row
+                    const uint32_t g0t = t - t % GT, nu = 2 * (t - g0t + 1);   // units a row
                     const uint32_t relg = e_begin + 16 * g0t - pb;        // (a multiple of 16)
                     const uint32_t r8 = lane >> 3, c = lane & 7;
 #pragma unroll
@@ -465,9 +466,11 @@ __global__ __launch_bounds__(64) void k_scan_wave(const MixedParams p) {
                             const uint32_t slo = __shfl((uint32_t)slot[h], (int)r, 64);
                             const uint32_t shi = __shfl((uint32_t)(slot[h] >> 32), (int)r, 64);
                             const uint64_t sl = ((uint64_t)shi << 32) | slo;
-                            if (c < nu && sl != ~0ull && relg + 8 * c < lim)
-                                *reinterpret_cast<uint4 *>(p.out16 + sl + relg + 8 * c) =
-                                    *reinterpret_cast<const uint4 *>(&s_k[h][r][8 * c]);
+#pragma unroll
+                            for (uint32_t cu = c; cu < 2 * GT; cu += 8)
+                                if (cu < nu && sl != ~0ull && relg + 8 * cu < lim)
+                                    *reinterpret_cast<uint4 *>(p.out16 + sl + relg + 8 * cu) =
+                                        *reinterpret_cast<const uint4 *>(&s_k[h][r][8 * cu]);
                         }
                     }
                     __syncthreads();   // (the next group rewrites the tile rows)
