@@ -111,6 +111,9 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
     constexpr int T = KS <= 4 ? 4 : 2;   // tiles per wave, mfma_tiles()
     constexpr uint32_t EW = 64 * T;      // entries per item
     __shared__ __align__(16) uint16_t s_tile[32][EW];   // one query group's keys
+    // one query group's rows (the A fragments), loaded by the whole workgroup
+    // (a row padded by one unit: rows start in different banks)
+    __shared__ __align__(16) uint4 s_a[32][4 * KS + 1];
     __shared__ uint64_t s_slot[kMixQ];
     __shared__ uint32_t s_qi[kMixQ];
     __shared__ float s_qn[kMixQ];
@@ -182,23 +185,24 @@ __global__ __launch_bounds__(256, (FILT || KS > 4) ? 3 : 4) void k_scan_mfma(con
         }
         __syncthreads();
         for (uint32_t g0 = 0; g0 < gcnt; g0 += 32) {
+            // the group's query rows into LDS, coalesced (rows hold 4*ks units;
+            // rows past the group's queries read zero)
+            for (uint32_t i = tid; i < 32 * 4 * KS; i += 256) {
+                const uint32_t r = i / (4 * KS), u = i - r * (4 * KS), g = g0 + r;
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (g < gcnt && u < 4 * p.ks) v = p.q16[(size_t)s_qi[g] * p.q16_units + u];
+                s_a[r][u] = v;
+            }
+            __syncthreads();
             // two 16-query halves, one after the other (fewer live registers)
+#pragma unroll 1
             for (uint32_t h = 0; h < 2; h++) {
                 const uint32_t gb = g0 + 16 * h;
                 if (gb >= gcnt) break;   // (uniform; the half's tile rows are not stored)
                 // A fragments: 16 queries x dims
                 uint4 a[KS];
-                {
-                    const uint32_t g = gb + col;
-                    const bool ok = g < gcnt;
-                    const uint4 *qrow = p.q16 + (size_t)s_qi[ok ? g : 0] * p.q16_units;
 #pragma unroll
-                    for (int s = 0; s < KS; s++) {
-                        uint4 v = make_uint4(0, 0, 0, 0);
-                        if (ok && s < (int)p.ks) v = qrow[4 * s + kg];   // rows hold 4*ks units
-                        a[s] = v;
-                    }
-                }
+                for (int s = 0; s < KS; s++) a[s] = s_a[16 * h + col][4 * s + kg];
                 // the lane's four rows 4*kg + j of the half: norm factor and
                 // label mask, read once (gb + 15 < kMixQ)
                 float qv[4], qa[4], qb[4];
